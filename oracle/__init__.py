@@ -1,0 +1,226 @@
+"""ctypes bindings of the CPU oracle (oracle/build/libks_oracle.so).
+
+TEST INFRASTRUCTURE. The oracle is the FP64 CPU restatement of KerrSplat's geodesic march and
+polarized transfer that the HIP product is checked against. Only tests/, __graft_entry__.smoke()
+and bench.py's CPU-baseline leg may import this module; the product path never does.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "build", "libks_oracle.so")
+_lib = None
+
+_d = ctypes.c_double
+_i = ctypes.c_int
+_P = ctypes.c_void_p
+
+
+def build():
+    subprocess.run(["make", "-C", _HERE, "-s"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _lib.kso_planck.restype = _d
+        _lib.kso_thermal_leung.restype = _d
+        _lib.kso_kappa_hyp.restype = _d
+        for name in ("kso_planck", "kso_thermal_leung", "kso_kappa_hyp"):
+            getattr(_lib, name).argtypes = None
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_P) if a is not None else None
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def set_threads(n):
+    lib().kso_set_threads(_i(n))
+
+
+def num_threads():
+    return lib().kso_num_threads()
+
+
+def pixel_constants(a, thetao, alpha, beta):
+    out = np.zeros(17)
+    lib().kso_pixel_constants(_d(a), _d(thetao), _d(alpha), _d(beta), _p(out))
+    return {"nreal": int(out[0]), "case": int(out[1]), "tau_total": out[2], "fo": out[3], "k": out[4],
+            "lam": out[5], "eta": out[6], "up": out[7], "um": out[8],
+            "roots": out[9:17].reshape(4, 2)}
+
+
+def root_cases(a, thetao, alphas, betas):
+    al, be = _f64(alphas), _f64(betas)
+    out = np.zeros(len(al), dtype=np.int8)
+    lib().kso_root_cases(_d(a), _d(thetao), _i(len(al)), _p(al), _p(be), _p(out))
+    return out
+
+
+def case_permutation(numreals):
+    nr = np.ascontiguousarray(numreals, dtype=np.int8)
+    perm = np.zeros(len(nr), dtype=np.int64)
+    counts = np.zeros(3, dtype=np.int64)
+    lib().kso_case_permutation(_i(len(nr)), _p(nr), _p(perm), _p(counts))
+    return perm, counts
+
+
+def samples(a, thetao, alphas, betas, N, M=64, marcher=1):
+    al, be = _f64(alphas), _f64(betas)
+    n = len(al)
+    t, r, th, ph = (np.zeros((n, N)) for _ in range(4))
+    flags = np.zeros((n, N), dtype=np.uint8)
+    rt, rp, tt = np.zeros(n), np.zeros(n), np.zeros(n)
+    lib().kso_samples(_d(a), _d(thetao), _i(n), _p(al), _p(be), _i(N), _i(M), _i(marcher),
+                      _p(t), _p(r), _p(th), _p(ph), _p(flags), _p(rt), _p(rp), _p(tt))
+    return {"t": t, "r": r, "theta": th, "phi": ph, "flags": flags, "resid_t": rt, "resid_phi": rp, "tau_total": tt}
+
+
+def thin_image(a, thetao, alphas, betas, N, params, t_obs, M=64, marcher=1):
+    al, be = _f64(alphas), _f64(betas)
+    P = np.asfortranarray(params, dtype=np.float64)
+    out = np.zeros(len(al))
+    lib().kso_thin_image(_d(a), _d(thetao), _i(len(al)), _p(al), _p(be), _i(N), _i(M), _i(marcher),
+                         _i(P.shape[1]), _p(P), _d(t_obs), _p(out))
+    return out
+
+
+KIND = {"thin": 0, "thermal": 1, "powerlaw": 2, "kappa": 3, "knot": 4}
+
+
+def polarized_image(a, thetao, alphas, betas, N, params, t_obs, nu, L, kind="thermal", hyp=None, gamma_max=1e5,
+                    knots=None, knot_times=None, nmax=-1, slab=0.0, M=64, marcher=1):
+    al, be = _f64(alphas), _f64(betas)
+    P = np.asfortranarray(params, dtype=np.float64)
+    hy = _f64(hyp) if hyp is not None else None
+    kn = np.asfortranarray(knots, dtype=np.float64) if knots is not None else None
+    kt = _f64(knot_times) if knot_times is not None else None
+    nk = 0 if kt is None else len(kt)
+    n = len(al)
+    out = np.zeros((n, 4))
+    rt, rp = np.zeros(n), np.zeros(n)
+    lib().kso_polarized_image(_d(a), _d(thetao), _i(n), _p(al), _p(be), _i(N), _i(M), _i(marcher),
+                              _i(KIND[kind]), _i(P.shape[0]), _i(P.shape[1]), _p(P), _p(hy), _d(gamma_max),
+                              _p(kn), _p(kt), _i(nk), _d(t_obs), _d(nu), _d(L), _i(nmax), _d(slab),
+                              _p(out), _p(rt), _p(rp))
+    return out, rt, rp
+
+
+def count_polarized(a, thetao, alphas, betas, N, params, t_obs, nu, L, kind="thermal", hyp=None, gamma_max=1e5, M=64):
+    al, be = _f64(alphas), _f64(betas)
+    P = np.asfortranarray(params, dtype=np.float64)
+    hy = _f64(hyp) if hyp is not None else None
+    f, s = _d(0), _d(0)
+    lib().kso_count_polarized(_d(a), _d(thetao), _i(len(al)), _p(al), _p(be), _i(N), _i(M), _i(KIND[kind]),
+                              _i(P.shape[0]), _i(P.shape[1]), _p(P), _p(hy), _d(gamma_max), _d(t_obs), _d(nu), _d(L),
+                              ctypes.byref(f), ctypes.byref(s))
+    return f.value, s.value
+
+
+def thermal(ne, Th, B, nu, th, dexter_rhoV=False, pandya=False):
+    out = np.zeros(8)
+    lib().kso_thermal(_d(ne), _d(Th), _d(B), _d(nu), _d(th), _i(int(dexter_rhoV)), _i(int(pandya)), _p(out))
+    return out
+
+
+def thermal_pandya(ne, Th, B, nu, th):
+    out = np.zeros(3)
+    lib().kso_thermal_pandya(_d(ne), _d(Th), _d(B), _d(nu), _d(th), _p(out))
+    return out
+
+
+def thermal_leung(ne, Th, B, nu, th):
+    f = lib().kso_thermal_leung
+    f.argtypes = [_d] * 5
+    return f(ne, Th, B, nu, th)
+
+
+def powerlaw(ne, p, gmin, gmax, B, nu, th, rotativities=True):
+    out = np.zeros(8)
+    lib().kso_powerlaw(_d(ne), _d(p), _d(gmin), _d(gmax), _d(B), _d(nu), _d(th), _i(int(rotativities)), _p(out))
+    return out
+
+
+def kappa(ne, kap, w, B, nu, th, hyp):
+    out = np.zeros(8)
+    lib().kso_kappa(_d(ne), _d(kap), _d(w), _d(B), _d(nu), _d(th), _d(hyp), _p(out))
+    return out
+
+
+def kappa_hyp(kap, w):
+    f = lib().kso_kappa_hyp
+    f.argtypes = [_d, _d]
+    return f(kap, w)
+
+
+def besselk01(x):
+    k0, k1 = _d(0), _d(0)
+    lib().kso_besselk01(_d(x), ctypes.byref(k0), ctypes.byref(k1))
+    return k0.value, k1.value
+
+
+def planck(nu, Th):
+    f = lib().kso_planck
+    f.argtypes = [_d, _d]
+    return f(nu, Th)
+
+
+def step_operator(alpha4, rho3, D):
+    al, rh = _f64(alpha4), _f64(rho3)
+    O, E = np.zeros(16), np.zeros(16)
+    lib().kso_step_operator(_p(al), _p(rh), _d(D), _p(O), _p(E))
+    return O.reshape(4, 4), E.reshape(4, 4)
+
+
+def cap(c8):
+    c = _f64(c8)
+    out = np.zeros(8)
+    lib().kso_cap(_p(c), _p(out))
+    return out
+
+
+def local_frame(a, r, th, eta, lam, nur, nuth, alpha, beta, thetao, u, B):
+    uu, BB = _f64(u), _f64(B)
+    out = np.zeros(3)
+    lib().kso_local_frame(_d(a), _d(r), _d(th), _d(eta), _d(lam), _i(int(nur)), _i(int(nuth)), _d(alpha), _d(beta),
+                          _d(thetao), _p(uu), _p(BB), _p(out))
+    return out
+
+
+def elliptic(phi, m, n):
+    out = np.zeros(7)
+    lib().kso_elliptic(_d(phi), _d(m), _d(n), _p(out))
+    return {"F": out[0], "E": out[1], "Pi": out[2], "sn": out[3], "cn": out[4], "dn": out[5], "am": out[6]}
+
+
+def visibilities(images, nalpha, nbeta, psize, u, v, taper=None):
+    """images: (nframes, npix, 4) screen order; returns complex (nframes, nuv, 4)."""
+    im = _f64(images)
+    nf = im.shape[0]
+    uu, vv = _f64(u), _f64(v)
+    tp = _f64(taper) if taper is not None else None
+    Vr, Vi = np.zeros((nf, len(uu), 4)), np.zeros((nf, len(uu), 4))
+    lib().kso_visibilities(_i(nalpha), _i(nbeta), _i(nf), _p(im), _d(psize), _i(len(uu)), _p(uu), _p(vv), _p(tp), _p(Vr), _p(Vi))
+    return Vr + 1j * Vi
+
+
+def vis_seed(W, nalpha, nbeta, psize, u, v, taper=None):
+    Wc = np.asarray(W)
+    nf = Wc.shape[0]
+    Wr, Wi = _f64(Wc.real), _f64(Wc.imag)
+    uu, vv = _f64(u), _f64(v)
+    tp = _f64(taper) if taper is not None else None
+    seed = np.zeros((nf, nalpha * nbeta, 4))
+    lib().kso_vis_seed(_i(nalpha), _i(nbeta), _i(nf), _d(psize), _i(len(uu)), _p(uu), _p(vv), _p(tp), _p(Wr), _p(Wi), _p(seed))
+    return seed

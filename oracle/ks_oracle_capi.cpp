@@ -1,0 +1,227 @@
+// C entry points of the CPU oracle, loaded by tests/ (through oracle/__init__.py), by
+// __graft_entry__.smoke() and by bench.py's CPU-baseline leg. TEST INFRASTRUCTURE: the product
+// (kerrsplat_amd/libkerrsplat_hip.so) never links or calls this library.
+//
+// Every entry point works in screen order on host arrays and returns 0 on success. Parallel over
+// pixels with OpenMP (one ray per iteration, the reference's KA CPU() backend layout).
+#include "ks_splats.hpp"
+#include "ks_vis.hpp"
+#include <cstring>
+#include <vector>
+#include <numeric>
+#include <omp.h>
+
+using namespace kso;
+
+namespace {
+template <class T> Model make_model(int kind, int nrows, int ns, const double* params, const double* hyp, double gamma_max,
+                                    const double* knots, const double* knot_times, int nknots) {
+    Model m;
+    m.kind = kind; m.nrows = nrows; m.ns = ns; m.params = params; m.hyp = hyp; m.gamma_max = gamma_max;
+    m.knots = knots; m.knot_times = knot_times; m.nknots = nknots;
+    return m;
+}
+
+// march one ray with the chosen marcher and fold a consumer over its samples
+template <class T, class F> void march(const Pixel<T>& p, int N, int M, int marcher, F&& f, T& rt, T& rp) {
+    if (marcher == 0) { march_direct(p, N, f); rt = 0.0; rp = 0.0; }
+    else march_ray(p, N, M, f, rt, rp);
+}
+
+// the polarized transport of one ray (transport.jl:263-290, with the half-orbit counter when nmax >= 0)
+template <class T> void polarized_ray(const Model& m, double a, double thetao, double alpha, double beta, int N, int M, int marcher,
+                                      double t_obs, double nu, double L, int nmax, double slab, T out[4], T* rt, T* rp) {
+    Pixel<T> p = pixel_constants<T>(T(a), T(alpha), T(beta), T(thetao));
+    RayCtx rc{a, alpha, beta, thetao, val(p.eta), val(p.lam)};
+    RadState<T> st = radstate_zero<T>();
+    Winding<T> w = winding_zero<T>();
+    double hor = (1.0 + std::sqrt(1.0 - a * a)) * (1.0 + 1e-3);
+    auto f = [&](int k, const Sample<T>& s, T dtau) {
+        if (nmax >= 0) {
+            if (!(s.ok && s.r > hor)) return;
+            wind(w, s.r * kcos(s.th), T(slab));
+            if (w.n > nmax) return;
+        }
+        M4<T> O; T E[4];
+        if (sample_step(m, rc, s, dtau, T(t_obs), T(nu), T(L), O, E)) advance(st, O, E);
+    };
+    T r1, r2;
+    march(p, N, M, marcher, f, r1, r2);
+    T nh = nu / NU0;
+    for (int q = 0; q < 4; ++q) out[q] = st.S[q] * nh * nh * nh;
+    if (rt) *rt = r1;
+    if (rp) *rp = r2;
+}
+}  // namespace
+
+extern "C" {
+
+int kso_num_threads(void) { return omp_get_max_threads(); }
+void kso_set_threads(int n) { if (n > 0) omp_set_num_threads(n); }
+
+// per-pixel constants: out = [nreal, case, tau_total, fo, k, lam, eta, up, um, roots re/im x4 (8)] (17 doubles)
+int kso_pixel_constants(double a, double thetao, double alpha, double beta, double* out) {
+    Pixel<double> p = pixel_constants<double>(a, alpha, beta, thetao);
+    out[0] = p.nreal; out[1] = p.rcase; out[2] = p.tau_total; out[3] = p.fo; out[4] = p.k;
+    out[5] = p.lam; out[6] = p.eta; out[7] = p.up; out[8] = p.um;
+    for (int i = 0; i < 4; ++i) { out[9 + 2 * i] = p.roots[i].re; out[10 + 2 * i] = p.roots[i].im; }
+    return 0;
+}
+
+// root case per pixel (screen order): number of real radial roots
+int kso_root_cases(double a, double thetao, int npix, const double* alphas, const double* betas, int8_t* numreals) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < npix; ++i) numreals[i] = (int8_t)num_real_roots<double>(a, alphas[i], betas[i], thetao);
+    return 0;
+}
+
+// stable permutation 4 / 2 / 0 real roots (1-based, as the reference's case_permutation) and the case counts
+int kso_case_permutation(int npix, const int8_t* numreals, int64_t* perm, int64_t* counts) {
+    std::vector<int64_t> idx(npix);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int64_t x, int64_t y) { return numreals[x] > numreals[y]; });
+    counts[0] = counts[1] = counts[2] = 0;
+    for (int j = 0; j < npix; ++j) {
+        perm[j] = idx[j] + 1;
+        int n = numreals[idx[j]];
+        counts[n == 4 ? 0 : (n == 2 ? 1 : 2)]++;
+    }
+    return 0;
+}
+
+// stored samples (screen order), npix x N row-major per pixel: marcher 0 Direct, 1 Recurrence(M)
+int kso_samples(double a, double thetao, int npix, const double* alphas, const double* betas, int N, int M, int marcher,
+                double* t, double* r, double* th, double* ph, uint8_t* flags, double* resid_t, double* resid_ph, double* tau_total) {
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int i = 0; i < npix; ++i) {
+        Pixel<double> p = pixel_constants<double>(a, alphas[i], betas[i], thetao);
+        double rt, rp;
+        auto f = [&](int k, const Sample<double>& s, double) {
+            size_t o = (size_t)i * N + (k - 1);
+            t[o] = s.t; r[o] = s.r; th[o] = s.th; ph[o] = s.ph;
+            flags[o] = (s.ok ? SAMPLE_OK : 0) | (s.nur ? SAMPLE_NUR : 0) | (s.nuth ? SAMPLE_NUTH : 0);
+        };
+        march(p, N, M, marcher, f, rt, rp);
+        if (resid_t) resid_t[i] = rt;
+        if (resid_ph) resid_ph[i] = rp;
+        if (tau_total) tau_total[i] = p.tau_total;
+    }
+    return 0;
+}
+
+// optically thin Stokes I image (screen order)
+int kso_thin_image(double a, double thetao, int npix, const double* alphas, const double* betas, int N, int M, int marcher,
+                   int ns, const double* params, double t_obs, double* out) {
+    Model m = make_model<double>(MODEL_THIN, 14, ns, params, nullptr, 0, nullptr, nullptr, 0);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int i = 0; i < npix; ++i) {
+        Pixel<double> p = pixel_constants<double>(a, alphas[i], betas[i], thetao);
+        RayCtx rc{a, alphas[i], betas[i], thetao, p.eta, p.lam};
+        double acc = 0, rt, rp;
+        auto f = [&](int, const Sample<double>& s, double dtau) { acc += thin_sample(m, rc, s, dtau, t_obs); };
+        march(p, N, M, marcher, f, rt, rp);
+        out[i] = acc;
+    }
+    return 0;
+}
+
+// polarized Stokes image (screen order, npix x 4 observed I, Q, U, V in cgs)
+int kso_polarized_image(double a, double thetao, int npix, const double* alphas, const double* betas, int N, int M, int marcher,
+                        int kind, int nrows, int ns, const double* params, const double* hyp, double gamma_max,
+                        const double* knots, const double* knot_times, int nknots,
+                        double t_obs, double nu, double L, int nmax, double slab, double* out, double* resid_t, double* resid_ph) {
+    Model m = make_model<double>(kind, nrows, ns, params, hyp, gamma_max, knots, knot_times, nknots);
+#pragma omp parallel for schedule(dynamic, 8)
+    for (int i = 0; i < npix; ++i) {
+        double rt, rp;
+        polarized_ray<double>(m, a, thetao, alphas[i], betas[i], N, M, marcher, t_obs, nu, L, nmax, slab, out + 4 * (size_t)i, &rt, &rp);
+        if (resid_t) resid_t[i] = rt;
+        if (resid_ph) resid_ph[i] = rp;
+    }
+    return 0;
+}
+
+// algorithmic operation count of the polarized forward transport over a set of rays (Counted<double>)
+int kso_count_polarized(double a, double thetao, int npix, const double* alphas, const double* betas, int N, int M,
+                        int kind, int nrows, int ns, const double* params, const double* hyp, double gamma_max,
+                        double t_obs, double nu, double L, double* flops, double* special) {
+    Model m = make_model<Counted>(kind, nrows, ns, params, hyp, gamma_max, nullptr, nullptr, 0);
+    double f = 0, s = 0;
+#pragma omp parallel for reduction(+ : f, s) schedule(dynamic, 4)
+    for (int i = 0; i < npix; ++i) {
+        op_counts().reset();
+        Counted out[4];
+        polarized_ray<Counted>(m, a, thetao, alphas[i], betas[i], N, M, 1, t_obs, nu, L, -1, 0.0, out, nullptr, nullptr);
+        f += op_counts().flops; s += op_counts().special;
+    }
+    *flops = f; *special = s;
+    return 0;
+}
+
+// ---- unit-level entry points (coefficients, step, frames, special functions) ----
+int kso_thermal(double ne, double Th, double B, double nu, double th, int dexter_rhoV, int pandya, double* out) {
+    Coef<double> c = thermal_synchrotron(ne, Th, B, nu, th, dexter_rhoV != 0, pandya != 0);
+    double v[8] = {c.jI, c.jQ, c.jV, c.aI, c.aQ, c.aV, c.rQ, c.rV};
+    std::memcpy(out, v, sizeof v);
+    return 0;
+}
+int kso_thermal_pandya(double ne, double Th, double B, double nu, double th, double* out) {
+    thermal_pandya(ne, Th, B, nu, th, out[0], out[1], out[2]);
+    return 0;
+}
+double kso_thermal_leung(double ne, double Th, double B, double nu, double th) { return thermal_leung(ne, Th, B, nu, th); }
+int kso_powerlaw(double ne, double p, double gmin, double gmax, double B, double nu, double th, int rot, double* out) {
+    Coef<double> c = powerlaw_synchrotron(ne, p, gmin, gmax, B, nu, th, rot != 0);
+    double v[8] = {c.jI, c.jQ, c.jV, c.aI, c.aQ, c.aV, c.rQ, c.rV};
+    std::memcpy(out, v, sizeof v);
+    return 0;
+}
+int kso_kappa(double ne, double kappa, double w, double B, double nu, double th, double hyp, double* out) {
+    Coef<double> c = kappa_synchrotron(ne, kappa, w, B, nu, th, hyp);
+    double v[8] = {c.jI, c.jQ, c.jV, c.aI, c.aQ, c.aV, c.rQ, c.rV};
+    std::memcpy(out, v, sizeof v);
+    return 0;
+}
+double kso_kappa_hyp(double kappa, double w) { return kappa_hypergeometric(kappa, w); }
+int kso_besselk01(double x, double* k0, double* k1) { besselk01(x, *k0, *k1); return 0; }
+double kso_planck(double nu, double Th) { return planck(nu, Th); }
+int kso_step_operator(const double* al, const double* rh, double D, double* O, double* Ej) {
+    M4<double> Om, Em;
+    step_operator(al, rh, D, Om, Em);
+    for (int i = 0; i < 4; ++i) for (int j = 0; j < 4; ++j) { O[4 * i + j] = Om.m[i][j]; Ej[4 * i + j] = Em.m[i][j]; }
+    return 0;
+}
+int kso_cap(const double* c8, double* out) {
+    Coef<double> c{c8[0], c8[1], c8[2], c8[3], c8[4], c8[5], c8[6], c8[7]};
+    Coef<double> r = cap_polarization(c);
+    double v[8] = {r.jI, r.jQ, r.jV, r.aI, r.aQ, r.aV, r.rQ, r.rV};
+    std::memcpy(out, v, sizeof v);
+    return 0;
+}
+int kso_local_frame(double a, double r, double th, double eta, double lam, int nur, int nuth, double alpha, double beta, double thetao,
+                    const double* u, const double* B, double* out) {
+    Frame<double> f = local_frame(a, r, th, eta, lam, nur != 0, nuth != 0, alpha, beta, thetao, u, B);
+    out[0] = f.g; out[1] = f.cosB; out[2] = f.chi;
+    return 0;
+}
+int kso_elliptic(double phi, double m, double n, double* out) {
+    out[0] = ellF(phi, m); out[1] = ellE(phi, m); out[2] = ellPi(n, phi, m);
+    Jac<double> j = jacobi(phi, m);
+    out[3] = j.sn; out[4] = j.cn; out[5] = j.dn;
+    out[6] = jacobi_am_any(phi, m);
+    return 0;
+}
+
+// visibilities of Stokes images (ks_vis.hpp); images npix x 4 per frame in screen order
+int kso_visibilities(int nalpha, int nbeta, int nframes, const double* images, double psize, int nuv, const double* u, const double* v,
+                     const double* taper, double* Vre, double* Vim) {
+    visibilities(nalpha, nbeta, nframes, images, psize, nuv, u, v, taper, Vre, Vim);
+    return 0;
+}
+int kso_vis_seed(int nalpha, int nbeta, int nframes, double psize, int nuv, const double* u, const double* v, const double* taper,
+                 const double* Wre, const double* Wim, double* seed) {
+    visibility_seed(nalpha, nbeta, nframes, psize, nuv, u, v, taper, Wre, Wim, seed);
+    return 0;
+}
+
+}  // extern "C"
