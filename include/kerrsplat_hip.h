@@ -1,0 +1,110 @@
+/* kerrsplat_hip.h - C-ABI of libkerrsplat_hip.so, the MI355X (gfx950) backend of KerrSplat's fused
+ * geodesic march + polarized transfer.
+ *
+ * Plain C types only (pointers, sizes, doubles); every function returns an int status
+ * (KS_OK = 0) and ks_last_error() describes the last failure of the calling thread. Calls that
+ * produce host results synchronize the context's stream before returning, as every reference
+ * API call synchronizes its backend (src/Geodesics/cache.jl:148-151,173,208). Host arrays are in
+ * Julia's layouts: parameter matrices column-major nrows x ns, images in screen (column-major) order
+ * or in the cache's sorted-slot order. Not re-entrant per cache.
+ *
+ * Each declaration names the reference interface it replaces (path:line in danielpalumbo/kerrsplat).
+ */
+#ifndef KERRSPLAT_HIP_H
+#define KERRSPLAT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes: the reference's ArgumentError / DimensionMismatch / state errors (cache.jl:155-156,
+ * 203, 249-250; dual_gradient.jl:193-195) and HIP runtime failures */
+enum { KS_OK = 0, KS_ERR_ARG = 1, KS_ERR_DIM = 2, KS_ERR_STATE = 3, KS_ERR_HIP = 4 };
+
+/* marchers (cache.jl:57-61): Direct(), Recurrence(M), Fused(M) */
+enum { KS_MARCHER_DIRECT = 0, KS_MARCHER_RECURRENCE = 1, KS_MARCHER_FUSED = 2 };
+
+/* splat models: ThinRenderer (Splats.jl:105), PolarizedSplats (polarized.jl:41), PowerLawSplats,
+ * KappaSplats (populations.jl:34-60), KnotSplats (trajectories.jl:184) */
+enum { KS_MODEL_THIN = 0, KS_MODEL_THERMAL = 1, KS_MODEL_POWERLAW = 2, KS_MODEL_KAPPA = 3, KS_MODEL_KNOT = 4 };
+
+typedef struct ks_ctx ks_ctx;
+typedef struct ks_cache ks_cache;
+typedef struct ks_model ks_model;
+
+/* Description of the last error on this thread (empty string if none). */
+const char* ks_last_error(void);
+
+/* Device context: one GPU and one HIP stream (the reference's KA backend object,
+ * cache.jl:87 `backend`; prepare_backend! cache.jl:10-42 has no HIP counterpart). */
+int ks_ctx_create(int device, ks_ctx** ctx);
+int ks_ctx_destroy(ks_ctx* ctx);
+int ks_ctx_sync(ks_ctx* ctx);
+void* ks_ctx_stream(ks_ctx* ctx);                 /* the hipStream_t every launch goes to */
+/* HIP events on the context's stream around a timed region (no reference counterpart; bench.py). */
+int ks_timing_begin(ks_ctx* ctx);
+int ks_timing_end(ks_ctx* ctx, float* ms);
+
+/* GeodesicCache(backend, camera, Val(N); store_samples) - cache.jl:106-125. */
+int ks_cache_create(ks_ctx* ctx, int64_t npix, int N, int store_samples, ks_cache** cache);
+int ks_cache_destroy(ks_cache* cache);
+/* npixels / nsamples / generation / length.(ranges) - cache.jl:127-141. */
+int ks_cache_info(const ks_cache* cache, int64_t* npix, int* N, int* generation, int64_t counts[3]);
+
+/* regenerate!(cache, a, θo, camera; marcher, workgroup) - cache.jl:153-180, 248-255.
+ * alpha, beta: the camera's pixels in screen order (nalpha x nbeta, column-major, camera.jl:110-118).
+ * K0 root cases -> stable 4/2/0 case sort -> K1 per-pixel constants -> (Direct / Recurrence(M))
+ * stored samples; Fused(M) stops after K1. */
+int ks_regenerate(ks_cache* cache, const double* alpha, const double* beta, int nalpha, int nbeta,
+                  double a, double theta_o, int marcher, int M);
+
+/* cache.perm_host (1-based, perm[j] = screen index of sorted slot j) and the case range lengths
+ * (case2, case3, case4) - cache.jl:94-95, pixel_constants.jl:167-176. */
+int ks_get_perm(const ks_cache* cache, int64_t* perm, int64_t counts[3]);
+/* cache.residual_t, cache.residual_ϕ (sorted order) - cache.jl:98-99, quadrature.jl:283-291. */
+int ks_get_residuals(const ks_cache* cache, double* resid_t, double* resid_phi);
+/* host(cache.consts) - Geodesics.jl:66-73 over PixelConstants (pixel_constants.jl:271-296):
+ * ks_num_const_fields() doubles per pixel, field-major, sorted order (layout in DESIGN.md). */
+int ks_num_const_fields(void);
+int ks_copy_consts(const ks_cache* cache, double* soa);
+/* host(cache.samples) - GeodesicSamples (direct_march.jl:71-115): N x npix per field
+ * (sample-major, sorted order), flags bit 0 ok, bit 1 nu_r, bit 2 nu_theta. */
+int ks_copy_samples(const ks_cache* cache, double* t, double* r, double* theta, double* phi, uint8_t* flags);
+
+/* A splat parameter set on the device: the model argument of ThinRenderer / RadiativeTransport
+ * (Splats.jl:105-111, polarized.jl:41-51, populations.jl:34-60, trajectories.jl:184-192).
+ * params: nrows x ns column-major (14 thin, 21 thermal/knot, 22 power law/kappa);
+ * gamma_max: power law; knots: 3 x nknots x ns, knot_times: nknots (knot splats). */
+int ks_model_create(ks_ctx* ctx, int kind, int nrows, int ns, const double* params, double gamma_max,
+                    const double* knots, const double* knot_times, int nknots, ks_model** model);
+int ks_model_destroy(ks_model* model);
+
+/* thin_image!(out, cache, params, t_obs) - Splats.jl:137-147: Stokes I, screen or sorted order. */
+int ks_thin_image(ks_cache* cache, const ks_model* model, double t_obs, double* out, int screen_order);
+
+/* polarized_image! / polarized_cube - polarized.jl:160-198: one fused launch over all
+ * (pixel, time, frequency) items; out_stokes = observed (I, Q, U, V) in cgs,
+ * 4 x npix x nt x nnu (frame index it + nt*inu), screen or sorted order. nmax >= 0 truncates
+ * the rays after their nmax-th passage through |z| < slab (step.jl:469-517). */
+int ks_render(ks_cache* cache, const ks_model* model, const double* times, int nt, const double* nus, int nnu,
+              double L, int nmax, double slab, double* out_stokes, int screen_order);
+/* The same launch with the result left on the device (no host copy, no synchronization). */
+int ks_render_device(ks_cache* cache, const ks_model* model, const double* times, int nt, const double* nus, int nnu,
+                     double L, int nmax, double slab);
+/* Copy the first `frames` rendered frames to the host (to_screen, cache.jl:293-296, when screen_order). */
+int ks_copy_stokes(ks_cache* cache, int64_t frames, double* out_stokes, int screen_order);
+const double* ks_stokes_device_ptr(const ks_cache* cache);
+
+/* visibilities / vis_kernel! - visibilities.jl:21-40, device_visibilities.jl:61-73: V of the
+ * rendered frames at nuv (u, v) points [wavelengths], psize = Δα L / D [rad], taper per point
+ * (NULL = 1); V = nframes x nuv x 4 complex as (re, im) pairs, Jy. */
+int ks_visibilities(ks_cache* cache, int64_t frames, double psize, const double* u, const double* v,
+                    const double* taper, int nuv, double* V);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KERRSPLAT_HIP_H */

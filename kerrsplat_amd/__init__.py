@@ -1,0 +1,24 @@
+"""kerrsplat_amd: MI355X (gfx950) backend of KerrSplat's fused geodesic march + polarized transfer.
+
+The host side mirrors the reference's Julia API (src/Geodesics/cache.jl, src/Splats/*.jl) over the
+C-ABI of libkerrsplat_hip.so (include/kerrsplat_hip.h):
+
+    cam   = Camera((-12, 12), (-12, 12), 64)
+    cache = GeodesicCache(cam, N=512, store_samples=False)
+    regenerate(cache, a, theta_o, marcher=Fused(64))
+    img   = thin_image(cache, params, t_obs)                       # screen-shaped Stokes I
+    st    = polarized_image(cache, params, t_obs, nu, L)          # (n_alpha, n_beta, 4)
+    cube  = polarized_cube(cache, params, times, nus, L)          # (n_alpha, n_beta, nt, nnu, 4)
+
+Every call goes to the GPU library; there is no CPU fallback. Loading fails loudly when the library
+is missing or no GPU is present.
+"""
+from .api import (Camera, Direct, Recurrence, Fused, GeodesicCache, Model, regenerate, thin_image, polarized_image,
+                  polarized_cube, render_device, visibilities, unsort, to_screen, samples, residuals, KernelError,
+                  NSPLATPARAMS, NPOLARIZEDPARAMS, NPOWERLAWPARAMS, NKAPPAPARAMS, gravitational_radius, grid_axis,
+                  library_path)
+
+__all__ = ["Camera", "Direct", "Recurrence", "Fused", "GeodesicCache", "Model", "regenerate", "thin_image",
+           "polarized_image", "polarized_cube", "render_device", "visibilities", "unsort", "to_screen", "samples",
+           "residuals", "KernelError", "NSPLATPARAMS", "NPOLARIZEDPARAMS", "NPOWERLAWPARAMS", "NKAPPAPARAMS",
+           "gravitational_radius", "grid_axis", "library_path"]

@@ -1,0 +1,344 @@
+"""Python mirror of the reference API over the C-ABI (include/kerrsplat_hip.h).
+
+Names, argument meaning and errors follow the reference's Julia functions; each wrapper cites the
+function it mirrors. Exceptions: ValueError for the reference's ArgumentError, IndexError-like
+DimensionMismatch as `DimensionMismatch` (a ValueError subclass), RuntimeError (KernelError) for
+HIP runtime failures.
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_NAME = "libkerrsplat_hip.so"
+
+KS_OK, KS_ERR_ARG, KS_ERR_DIM, KS_ERR_STATE, KS_ERR_HIP = 0, 1, 2, 3, 4
+KS_MARCHER_DIRECT, KS_MARCHER_RECURRENCE, KS_MARCHER_FUSED = 0, 1, 2
+MODEL_KIND = {"thin": 0, "thermal": 1, "powerlaw": 2, "kappa": 3, "knot": 4}
+NSPLATPARAMS, NPOLARIZEDPARAMS, NPOWERLAWPARAMS, NKAPPAPARAMS = 14, 21, 22, 22
+_NROWS = {"thin": 14, "thermal": 21, "knot": 21, "powerlaw": 22, "kappa": 22}
+
+# cgs constants (src/Transfer/constants.jl:3-12)
+GNEWT, MSUN, CL = 6.6742e-8, 1.989e33, 2.99792458e10
+
+
+def gravitational_radius(m_solar):
+    """L = GM/c^2 [cm] (transport.jl:16)."""
+    return GNEWT * m_solar * MSUN / (CL * CL)
+
+
+class KernelError(RuntimeError):
+    pass
+
+
+class DimensionMismatch(ValueError):
+    pass
+
+
+class StateError(ValueError):
+    pass
+
+
+def library_path():
+    return os.path.join(_HERE, _LIB_NAME)
+
+
+_lib = None
+_ctx = None
+_P = ctypes.c_void_p
+_d = ctypes.c_double
+_i = ctypes.c_int
+_i64 = ctypes.c_int64
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = library_path()
+    if not os.path.exists(path):
+        raise KernelError(f"{path} is missing: build it with __graft_entry__.build() (hipcc --offload-arch=gfx950)")
+    lib = ctypes.CDLL(path)
+    lib.ks_last_error.restype = ctypes.c_char_p
+    lib.ks_ctx_stream.restype = _P
+    lib.ks_stokes_device_ptr.restype = _P
+    lib.ks_cache_create.argtypes = [_P, _i64, _i, _i, ctypes.POINTER(_P)]
+    lib.ks_regenerate.argtypes = [_P, _P, _P, _i, _i, _d, _d, _i, _i]
+    lib.ks_model_create.argtypes = [_P, _i, _i, _i, _P, _d, _P, _P, _i, ctypes.POINTER(_P)]
+    lib.ks_render.argtypes = [_P, _P, _P, _i, _P, _i, _d, _i, _d, _P, _i]
+    lib.ks_render_device.argtypes = [_P, _P, _P, _i, _P, _i, _d, _i, _d]
+    lib.ks_copy_stokes.argtypes = [_P, _i64, _P, _i]
+    lib.ks_thin_image.argtypes = [_P, _P, _d, _P, _i]
+    lib.ks_visibilities.argtypes = [_P, _i64, _d, _P, _P, _P, _i, _P]
+    lib.ks_timing_end.argtypes = [_P, ctypes.POINTER(ctypes.c_float)]
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc == KS_OK:
+        return
+    msg = _load().ks_last_error().decode()
+    if rc == KS_ERR_ARG:
+        raise ValueError(msg)
+    if rc == KS_ERR_DIM:
+        raise DimensionMismatch(msg)
+    if rc == KS_ERR_STATE:
+        raise StateError(msg)
+    raise KernelError(msg)
+
+
+def context(device=0):
+    """The process-wide device context (one GPU, one stream)."""
+    global _ctx
+    if _ctx is None:
+        lib = _load()
+        p = _P()
+        _check(lib.ks_ctx_create(_i(device), ctypes.byref(p)))
+        _ctx = p
+    return _ctx
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_P) if a is not None else None
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+# ---- camera (src/Geodesics/camera.jl) ---------------------------------------------------------------------
+def grid_axis(lo, hi, n):
+    """Krang's screen formula lo + (hi - lo)(i - 1)/(n - 1) (camera.jl:122-124)."""
+    if n == 1:
+        return np.array([float(lo)])
+    i = np.arange(1, n + 1, dtype=np.float64)
+    return float(lo) + (float(hi) - float(lo)) * (i - 1) / (n - 1)
+
+
+class Camera:
+    """Camera((alo, ahi), (blo, bhi), res | (na, nb)) or Camera(alphas, betas) (camera.jl:97-120)."""
+
+    def __init__(self, arange, brange=None, size=None):
+        if brange is None or (size is None and np.ndim(arange) == 1 and len(arange) != 2):
+            raise ValueError("Camera((alo, ahi), (blo, bhi), res) or Camera.points(alphas, betas)")
+        if np.ndim(size) == 0:
+            size = (int(size), int(size))
+        na, nb = size
+        av, bv = grid_axis(arange[0], arange[1], na), grid_axis(brange[0], brange[1], nb)
+        self.alphas = np.repeat(av[:, None], nb, axis=1).ravel(order="F")
+        self.betas = np.repeat(bv[None, :], na, axis=0).ravel(order="F")
+        self.size = (na, nb)
+
+    @classmethod
+    def points(cls, alphas, betas, size=None):
+        c = cls.__new__(cls)
+        c.alphas = _f64(alphas).ravel()
+        c.betas = _f64(betas).ravel()
+        if len(c.alphas) != len(c.betas):
+            raise DimensionMismatch("camera pixel vectors do not match the screen size")
+        c.size = size if size is not None else (len(c.alphas), 1)
+        if c.size[0] * c.size[1] != len(c.alphas):
+            raise DimensionMismatch("camera pixel vectors do not match the screen size")
+        return c
+
+    @property
+    def npix(self):
+        return len(self.alphas)
+
+
+class Direct:
+    code, M = KS_MARCHER_DIRECT, 64
+
+
+class Recurrence:
+    code = KS_MARCHER_RECURRENCE
+
+    def __init__(self, M=64):
+        self.M = int(M)
+
+
+class Fused:
+    code = KS_MARCHER_FUSED
+
+    def __init__(self, M=64):
+        self.M = int(M)
+
+
+class GeodesicCache:
+    """GeodesicCache(camera, N; store_samples) on the GPU (cache.jl:85-125)."""
+
+    def __init__(self, camera, N, store_samples=True, device=0):
+        if N < 1:
+            raise ValueError("need at least one sample per ray")
+        self._lib = _load()
+        self._ctx = context(device)
+        self.camera = camera
+        self.N = int(N)
+        self.store_samples = bool(store_samples)
+        p = _P()
+        _check(self._lib.ks_cache_create(self._ctx, _i64(camera.npix), _i(self.N), _i(int(self.store_samples)), ctypes.byref(p)))
+        self._h = p
+        self.spin = math.nan
+        self.theta_o = math.nan
+        self.generation = 0
+        self.marcher = Direct()
+        self.perm_host = np.arange(1, camera.npix + 1)
+        self.ranges = (0, 0, 0)
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None) is not None:
+                self._lib.ks_cache_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    @property
+    def npix(self):
+        return self.camera.npix
+
+    @property
+    def screen_size(self):
+        return self.camera.size
+
+    def info(self):
+        npix, N, gen = _i64(), _i(), _i()
+        counts = (_i64 * 3)()
+        _check(self._lib.ks_cache_info(self._h, ctypes.byref(npix), ctypes.byref(N), ctypes.byref(gen), counts))
+        return npix.value, N.value, gen.value, tuple(counts)
+
+
+def regenerate(cache, a, theta_o, camera=None, marcher=None):
+    """regenerate!(cache, a, θo[, camera]; marcher) (cache.jl:153-180, 248-255)."""
+    marcher = marcher if marcher is not None else Direct()
+    if camera is not None:
+        if camera.npix != cache.npix:
+            raise DimensionMismatch(f"camera has {camera.npix} pixels, cache was built for {cache.npix}")
+        cache.camera = camera
+    cam = cache.camera
+    al, be = _f64(cam.alphas), _f64(cam.betas)
+    lib = cache._lib
+    _check(lib.ks_regenerate(cache._h, _ptr(al), _ptr(be), _i(cam.size[0]), _i(cam.size[1]), _d(a), _d(theta_o),
+                             _i(marcher.code), _i(getattr(marcher, "M", 64))))
+    perm = np.zeros(cache.npix, dtype=np.int64)
+    counts = np.zeros(3, dtype=np.int64)
+    _check(lib.ks_get_perm(cache._h, _ptr(perm), _ptr(counts)))
+    cache.perm_host = perm
+    cache.ranges = tuple(int(c) for c in counts)
+    cache.spin, cache.theta_o = float(a), float(theta_o)
+    cache.generation += 1
+    cache.marcher = marcher
+    return cache
+
+
+def unsort(cache, x):
+    """unsort(cache, x): sorted-order vector/matrix (first axis npix) back to screen order (cache.jl:271-286)."""
+    x = np.asarray(x)
+    if x.shape[0] != cache.npix:
+        raise DimensionMismatch("first dimension must be the pixel count")
+    out = np.empty_like(x)
+    out[cache.perm_host - 1] = x
+    return out
+
+
+def to_screen(cache, x):
+    """to_screen(cache, x) (cache.jl:293-296): unsort and reshape to (n_alpha, n_beta, ...)."""
+    y = unsort(cache, x)
+    return y.reshape(cache.screen_size + y.shape[1:], order="F")
+
+
+def samples(cache):
+    """host(cache.samples) in sorted order: dict of (npix, N) arrays (direct_march.jl:71-115)."""
+    n, N = cache.npix, cache.N
+    t, r, th, ph = (np.zeros((N, n)) for _ in range(4))
+    fl = np.zeros((N, n), dtype=np.uint8)
+    _check(cache._lib.ks_copy_samples(cache._h, _ptr(t), _ptr(r), _ptr(th), _ptr(ph), _ptr(fl)))
+    return {"t": t.T.copy(), "r": r.T.copy(), "theta": th.T.copy(), "phi": ph.T.copy(), "flags": fl.T.copy()}
+
+
+def residuals(cache):
+    rt, rp = np.zeros(cache.npix), np.zeros(cache.npix)
+    _check(cache._lib.ks_get_residuals(cache._h, _ptr(rt), _ptr(rp)))
+    return rt, rp
+
+
+class Model:
+    """A splat parameter matrix on the device: ThinRenderer / PolarizedSplats / PowerLawSplats /
+    KappaSplats / KnotSplats (Splats.jl:105, polarized.jl:41, populations.jl:34-60, trajectories.jl:184)."""
+
+    def __init__(self, params, kind="thermal", gamma_max=1e5, knots=None, knot_times=None, device=0):
+        if kind not in MODEL_KIND:
+            raise ValueError(f"unknown model kind {kind}")
+        P = np.asfortranarray(params, dtype=np.float64)
+        if P.ndim != 2 or P.shape[0] != _NROWS[kind]:
+            raise DimensionMismatch(f"{kind} splats take {_NROWS[kind]} parameter rows, got {P.shape}")
+        self.kind, self.params = kind, P
+        self._lib = _load()
+        kn = np.asfortranarray(knots, dtype=np.float64) if knots is not None else None
+        kt = _f64(knot_times) if knot_times is not None else None
+        h = _P()
+        _check(self._lib.ks_model_create(context(device), _i(MODEL_KIND[kind]), _i(P.shape[0]), _i(P.shape[1]), _ptr(P),
+                                         _d(gamma_max), _ptr(kn), _ptr(kt), _i(0 if kt is None else len(kt)), ctypes.byref(h)))
+        self._h = h
+        self.ns = P.shape[1]
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None) is not None:
+                self._lib.ks_model_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+def _as_model(params, kind):
+    return params if isinstance(params, Model) else Model(params, kind=kind)
+
+
+def thin_image(cache, params, t_obs, screen=True):
+    """thin_image(cache, params, t_obs) (Splats.jl:137-147): screen-shaped (or sorted) Stokes I."""
+    m = _as_model(params, "thin")
+    out = np.zeros(cache.npix)
+    _check(cache._lib.ks_thin_image(cache._h, m._h, _d(t_obs), _ptr(out), _i(int(screen))))
+    return out.reshape(cache.screen_size, order="F") if screen else out
+
+
+def render_device(cache, model, times, nus, L, nmax=-1, slab=0.0):
+    """The fused polarized launch with the result left on the device (ks_render_device)."""
+    t, n = _f64(np.atleast_1d(times)), _f64(np.atleast_1d(nus))
+    _check(cache._lib.ks_render_device(cache._h, model._h, _ptr(t), _i(len(t)), _ptr(n), _i(len(n)), _d(L), _i(nmax), _d(slab)))
+
+
+def polarized_cube(cache, params, times, nus, L, nmax=-1, slab=0.0, kind="thermal", screen=True):
+    """polarized_cube(cache, params, times, νs, L; nmax, slab) (polarized.jl:187-198), all frames
+    and frequencies in one launch: (n_alpha, n_beta, nt, nnu, 4) observed Stokes [cgs]."""
+    m = _as_model(params, kind)
+    t, n = _f64(np.atleast_1d(times)), _f64(np.atleast_1d(nus))
+    out = np.zeros((len(n), len(t), cache.npix, 4))
+    _check(cache._lib.ks_render(cache._h, m._h, _ptr(t), _i(len(t)), _ptr(n), _i(len(n)), _d(L), _i(nmax), _d(slab),
+                                _ptr(out), _i(int(screen))))
+    if not screen:
+        return out
+    na, nb = cache.screen_size
+    img = out.reshape(len(n), len(t), nb, na, 4)            # pixel index = i + na j
+    return np.transpose(img, (3, 2, 1, 0, 4))
+
+
+def polarized_image(cache, params, t_obs, nu, L, nmax=-1, slab=0.0, kind="thermal", screen=True):
+    """polarized_image(cache, params, t_obs, ν, L; nmax, slab) (polarized.jl:174-179):
+    (n_alpha, n_beta, 4) observed Stokes (or (npix, 4) in sorted order with screen=False)."""
+    c = polarized_cube(cache, params, [t_obs], [nu], L, nmax=nmax, slab=slab, kind=kind, screen=screen)
+    return c[:, :, 0, 0, :] if screen else c[0, 0]
+
+
+def visibilities(cache, frames, psize, u, v, taper=None):
+    """Visibilities of the rendered frames (visibilities.jl:21-40, device_visibilities.jl:61-73):
+    complex (frames, nuv, 4) in Jy for pixel size psize = Δα L / D [rad]."""
+    uu, vv = _f64(u), _f64(v)
+    tp = _f64(taper) if taper is not None else None
+    V = np.zeros((frames, len(uu), 4, 2))
+    _check(cache._lib.ks_visibilities(cache._h, _i64(frames), _d(psize), _ptr(uu), _ptr(vv), _ptr(tp), _i(len(uu)), _ptr(V)))
+    return V[..., 0] + 1j * V[..., 1]

@@ -1,0 +1,458 @@
+// Device polarized synchrotron transfer (FP64): per-splat constants staged in LDS, the splat weight
+// and bounding-sphere cull, the fluid frame and Walker-Penrose screen angle, thermal / power-law /
+// kappa coefficients, the polarization cap and nu-hat invariants, and the exact constant-coefficient
+// 4x4 Stokes step composed front to back in registers.
+//
+// Reference: src/Transfer/{coefficients,kappa,step,frames,transport}.jl and
+// src/Splats/{Splats,polarized,populations,trajectories}.jl. The synchrotron fits are the published
+// ones (Dexter 2016, Shcherbakov 2008, Pandya+ 2016, Marszewski+ 2021, Jones & O'Dell 1977) and are
+// written out as such; everything that depends only on a splat (Bessel ratios at 1/Theta_e, Gamma
+// functions, the kappa 2F1 factor, the quaternion rotation, field direction, boost) is computed
+// once per splat on the host instead of at every sample.
+#pragma once
+#include "ks_dgeo.hpp"
+
+namespace ksd {
+
+constexpr double ME = 9.1093826e-28, CL = 2.99792458e10, EE = 4.80320680e-10, HPL = 6.6260693e-27;
+constexpr double NU0 = 1e11;
+constexpr double PLANCK0 = 2.0 * HPL * NU0 * NU0 * NU0 / (CL * CL);
+constexpr double WEIGHT_CUTOFF = 1e-6;
+
+enum ModelKind : int { MODEL_THIN = 0, MODEL_THERMAL = 1, MODEL_POWERLAW = 2, MODEL_KAPPA = 3, MODEL_KNOT = 4 };
+
+// Per-splat constants (built on the host by ks_model_create; 64 doubles, LDS-resident in kernels)
+struct SplatC {
+    double x, y, z;            // centre at t0 (rows 1-3)
+    double R[9];               // quaternion rotation, row-major R[a][b]
+    double is[3];              // exp(-s_a)
+    double t0, iw;             // t0 and exp(-logw)
+    double omega;              // pattern angular velocity
+    double rsup2;              // support sphere radius^2 = -2 ln(cutoff) exp(2 max s)
+    double lnA;                // thin: logA; others: logne
+    double Bmag, B[3];         // field magnitude and vector (ZAMO axes r, phi, -theta)
+    double u[3];               // fluid spatial 4-velocity (ZAMO axes)
+    double p1, p2, p3;         // thermal: Theta_e, K0/K2, K1/K2 at 1/Theta_e; power law: p, gamma_min, gamma_max;
+                               // kappa: kappa, w, hyp
+    double g[6];               // power law: Gamma((3p-1)/12) Gamma((3p+19)/12), Gamma((3p+2)/12) Gamma((3p+22)/12);
+                               // kappa: Gamma(k-4/3), Gamma(k-2), Gamma(k/4-1/3), Gamma(k/4+4/3), Gamma(2+k/2), K0/(K2+1e-40) at 1/w
+    int knot0;                 // knot splats: index of the splat's first knot
+    int pad;
+};
+
+// ---- geometry ---------------------------------------------------------------------------------------------
+KS_DEV void quasi_cartesian_ks(double a, double rp, double rm, double r, double th, double ph, double& x, double& y, double& z) {
+    double temp = sqrt(1.0 - a * a);
+    double phks = ph + a / (2.0 * temp) * log(fabs((r - rp + EPS) / (r - rm + EPS))) - atan2(a, r);
+    double st, ct, sp, cp;
+    sincos(th, &st, &ct);
+    sincos(phks, &sp, &cp);
+    x = r * st * cp; y = r * st * sp; z = r * ct;
+}
+
+// centre of a splat at time t under the pattern rotation and the squared distance (for the cull)
+KS_DEV bool outside_support(const SplatC& s, double t, double x, double y, double z, double& sp, double& cp) {
+    double ph = s.omega * (t - s.t0);
+    sincos(ph, &sp, &cp);
+    double cx = s.x * cp - s.y * sp, cy = s.x * sp + s.y * cp;
+    double dx = x - cx, dy = y - cy, dz = z - s.z;
+    return dx * dx + dy * dy + dz * dz > s.rsup2;
+}
+// the Gaussian exponent q^2 + tau^2 given the rotation phase (Splats.jl:58-93)
+KS_DEV double gauss_exponent(const SplatC& s, double t, double x, double y, double z, double sp, double cp) {
+    double cx = s.x * cp - s.y * sp, cy = s.x * sp + s.y * cp;
+    double d0 = x - cx, d1 = y - cy, d2 = z - s.z;
+    double e0 = d0 * cp + d1 * sp, e1 = -d0 * sp + d1 * cp, e2 = d2;
+    double u0 = s.R[0] * e0 + s.R[3] * e1 + s.R[6] * e2;
+    double u1 = s.R[1] * e0 + s.R[4] * e1 + s.R[7] * e2;
+    double u2 = s.R[2] * e0 + s.R[5] * e1 + s.R[8] * e2;
+    double q0 = u0 * s.is[0], q1 = u1 * s.is[1], q2 = u2 * s.is[2];
+    double tt = (t - s.t0) * s.iw;
+    return q0 * q0 + q1 * q1 + q2 * q2 + tt * tt;
+}
+
+// ---- metric, ZAMO tetrad, photon momentum ----------------------------------------------------------------------
+struct Geo {   // everything of a sample the splats share
+    double r, th, st, ct, S, Dl, Xi, lapse, omega, sgrr, sgthth, sgpp;
+    double pu[4];        // contravariant BL photon momentum
+    double pz[4];        // ZAMO components (t, r, phi, -theta)
+};
+KS_DEV void make_geo(double a, double r, double th, double eta, double lam, bool nur, bool nuth, Geo& g) {
+    g.r = r; g.th = th;
+    sincos(th, &g.st, &g.ct);
+    g.S = r * r + a * a * g.ct * g.ct;
+    g.Dl = r * r - 2.0 * r + a * a;
+    g.Xi = (r * r + a * a) * (r * r + a * a) - a * a * g.Dl * g.st * g.st;
+    g.lapse = sqrt(g.Dl * g.S / g.Xi);
+    g.omega = 2.0 * a * r / g.Xi;
+    g.sgrr = sqrt(g.S / g.Dl);
+    g.sgthth = sqrt(g.S);
+    g.sgpp = g.st * sqrt(g.Xi / g.S);
+    double w = r * r + a * a - a * lam;
+    double R = w * w - g.Dl * (eta + (lam - a) * (lam - a));
+    double Th = eta + a * a * g.ct * g.ct - lam * lam * g.ct * g.ct / (g.st * g.st);
+    double pr = (nur ? 1.0 : -1.0) * (R > 0.0 ? sqrt(R) : 0.0) / g.Dl;
+    double pth = (nuth ? 1.0 : -1.0) * (Th > 0.0 ? sqrt(Th) : 0.0);
+    // p^mu = g^{mu nu} p_nu with p_t = -1, p_phi = lam
+    double DS = g.Dl * g.S;
+    double gtt = -g.Xi / DS, gtp = -2.0 * a * r / DS, gpp = (g.Dl - a * a * g.st * g.st) / (DS * g.st * g.st);
+    g.pu[0] = -gtt + gtp * lam;
+    g.pu[1] = g.Dl / g.S * pr;
+    g.pu[2] = pth / g.S;
+    g.pu[3] = -gtp + gpp * lam;
+    g.pz[0] = g.lapse * g.pu[0];
+    g.pz[1] = g.sgrr * g.pu[1];
+    g.pz[2] = g.sgpp * (g.pu[3] - g.omega * g.pu[0]);
+    g.pz[3] = -g.sgthth * g.pu[2];
+}
+
+struct Frame { double g, cosB, chi; };
+
+// frames.jl:108-128 for one splat (its boost built from u)
+KS_DEV Frame local_frame(const Geo& G, double a, double alpha, double beta, double thetao_sin, const double u[3], const double B[3], double Bmag) {
+    double u2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    double gam = sqrt(1.0 + u2);
+    double q = 1.0 / (gam + 1.0);
+    // p_f = Lambda p_z
+    double ud = u[0] * G.pz[1] + u[1] * G.pz[2] + u[2] * G.pz[3];
+    double pf0 = gam * G.pz[0] - ud;
+    double c1 = -G.pz[0] + ud * q;
+    double pf1 = G.pz[1] + u[0] * c1, pf2 = G.pz[2] + u[1] * c1, pf3 = G.pz[3] + u[2] * c1;
+    Frame fr;
+    fr.g = 1.0 / pf0;
+    double k0 = pf1 * fr.g, k1 = pf2 * fr.g, k2 = pf3 * fr.g;
+    double b0, b1, b2;
+    if (Bmag > 0.0) { double ib = 1.0 / Bmag; b0 = B[0] * ib; b1 = B[1] * ib; b2 = B[2] * ib; }
+    else { b0 = 0.0; b1 = 0.0; b2 = 1.0; }
+    fr.cosB = k0 * b0 + k1 * b1 + k2 * b2;
+    double f0 = k1 * b2 - k2 * b1, f1 = k2 * b0 - k0 * b2, f2 = k0 * b1 - k1 * b0;
+    double fn = sqrt(f0 * f0 + f1 * f1 + f2 * f2);
+    if (!(fn > 0.0)) { fr.chi = 0.0; return fr; }
+    double ifn = 1.0 / fn;
+    f0 *= ifn; f1 *= ifn; f2 *= ifn;
+    // inverse boost of (0, f): time part = u.f, spatial f + u (u.f) q
+    double uf = u[0] * f0 + u[1] * f1 + u[2] * f2;
+    double fz0 = uf;
+    double fz1 = f0 + u[0] * uf * q, fz2 = f1 + u[1] * uf * q, fz3 = f2 + u[2] * uf * q;
+    // ZAMO -> BL contravariant
+    double ft = fz0 / G.lapse;
+    double fr_ = fz1 / G.sgrr;
+    double fth = -fz3 / G.sgthth;
+    double fph = fz2 / G.sgpp + G.omega * ft;
+    const double* p = G.pu;
+    double A = p[0] * fr_ - p[1] * ft + a * G.st * G.st * (p[1] * fph - p[3] * fr_);
+    double Bw = ((G.r * G.r + a * a) * (p[3] * fth - p[2] * fph) - a * (p[0] * fth - p[2] * ft)) * G.st;
+    double kk1 = A * G.r - Bw * a * G.ct, kk2 = -(A * a * G.ct + Bw * G.r);
+    double mu = -(alpha + a * thetao_sin);
+    double ea = beta * kk2 - mu * kk1, eb = beta * kk1 + mu * kk2;
+    fr.chi = atan2(-ea, eb);
+    return fr;
+}
+
+// ---- coefficients ------------------------------------------------------------------------------------------------
+struct Coef { double jI, jQ, jV, aI, aQ, aV, rQ, rV; };
+
+KS_DEV double planck(double nu, double Th) {
+    double x = HPL * nu / (ME * CL * CL * Th);
+    double nh = nu * (1.0 / NU0);
+    double pref = PLANCK0 * nh * nh * nh;
+    if (x < 2e-3) return pref / (x / 24.0 * (24.0 + x * (12.0 + x * (4.0 + x))));
+    return pref / expm1(x);
+}
+
+KS_DEV double cbrt_pos(double x) { return cbrt(x); }
+
+// thermal (coefficients.jl:185-229) with the splat's Bessel ratios k0/k2, k1/k2 (k2 > 0 in practice)
+KS_DEV Coef thermal(double ne, double Th, double k0k2, double k1k2, double B, double nu, double sth, double cth) {
+    Coef c;
+    double w = 2.0 * PI * nu;
+    double w0w = EE * B / (ME * CL) / w;
+    double wp2w2 = 4.0 * PI * ne * EE * EE / ME / (w * w);
+    if (!(sth > 0.0)) {
+        c.jI = c.jQ = c.jV = c.aI = c.aQ = c.aV = c.rQ = 0.0;
+        c.rV = w / CL * (wp2w2 * w0w * k0k2 * cth);
+        return c;
+    }
+    double x = Th * sqrt(1.4142135623730951 * sth * (1e3 * w0w));
+    double lx = log(x);
+    double extraterm = (0.011 * exp(-x / 47.2) - 0.7937005259840998 / 67.44733739010397 * PI * 1e4 * exp(-8.0 / 3.0 * log(x + 1e-16))) *
+                       (0.5 + 0.5 * tanh((lx - 4.787491742782046) / 0.1));
+    double jffunc = 2.011 * exp(-exp(1.035 * lx) / 4.7) - cos(x * 0.5) * exp(-exp(1.2 * lx) / 2.73) - 0.011 * exp(-x / 47.2) + extraterm;
+    double eps11m22 = jffunc * wp2w2 * w0w * w0w * (k1k2 + 6.0 * Th) * sth * sth;
+    c.rQ = w / (2.0 * CL) * eps11m22;
+    double ff = k0k2 * (1.0 - 0.11 * log1p(0.035 * x));
+    c.rV = w / CL * (wp2w2 * w0w * ff * cth);
+    double nus = 3.0 * EE * B * sth / (4.0 * PI * ME * CL) * Th * Th + 1.0;
+    double xs = nu / nus;
+    double x13 = cbrt(xs), xm13 = 1.0 / x13, xm23 = xm13 * xm13;
+    double e = exp(-1.8899 * x13);
+    double pref = ne * EE * EE * nu / (2.0 * 1.7320508075688772 * CL * Th * Th);
+    c.jI = pref * 2.5651 * (1.0 + 1.92 * xm13 + 0.9977 * xm23) * e;
+    c.jQ = pref * 2.5651 * (1.0 + 0.93193 * xm13 + 0.499873 * xm23) * e;
+    c.jV = 2.0 * ne * EE * EE * nu * cth / sth / (3.0 * 1.7320508075688772 * CL * Th * Th * Th) *
+           (1.81384 / xs + 3.42319 * xm23 + 0.0292545 / sqrt(xs) + 2.03773 * xm13) * e;
+    double Bn = planck(nu, Th);
+    double ib = Bn > 0.0 ? 1.0 / Bn : 0.0;
+    c.aI = c.jI * ib; c.aQ = c.jQ * ib; c.aV = c.jV * ib;
+    return c;
+}
+
+// power law (coefficients.jl:292-340); gPL = {Gamma((3p-1)/12) Gamma((3p+19)/12), Gamma((3p+2)/12) Gamma((3p+22)/12)}
+KS_DEV Coef powerlaw(double ne, double p, double gmin, double gmax, const double* gPL, double B, double nu, double sth, double cth) {
+    Coef c;
+    double nuc = EE * B / (2.0 * PI * ME * CL);
+    double gspan = pow(gmin, 1.0 - p) - pow(gmax, 1.0 - p);
+    double pref = ne * EE * EE * nuc / CL;
+    double X = nu / (nuc * sth);
+    c.jI = pref * (pow(3.0, p * 0.5) * (p - 1.0) * sth) / (2.0 * (p + 1.0) * gspan) * gPL[0] * pow(X, -(p - 1.0) * 0.5);
+    c.jQ = (p + 1.0) / (p + 7.0 / 3.0) * c.jI;
+    c.jV = (171.0 / 250.0) * pow(p, 0.49) * cth / sth * pow(nu / (3.0 * nuc * sth), -0.5) * c.jI;
+    double apref = ne * EE * EE / (nu * ME * CL);
+    c.aI = apref * pow(3.0, (p + 1.0) * 0.5) * (p - 1.0) / (4.0 * gspan) * gPL[1] * pow(X, -(p + 2.0) * 0.5);
+    c.aQ = c.aI * pow((17.0 / 500.0) * p - 43.0 / 1250.0, 43.0 / 500.0);
+    double term6 = pow(3.1 * pow(sth, -48.0 / 25.0) - 3.1, 64.0 / 125.0);
+    double sgn = cth > 0.0 ? 1.0 : (cth < 0.0 ? -1.0 : 0.0);
+    c.aV = c.aI * pow(0.71 * p + 22.0 / 625.0, 197.0 / 500.0) * term6 * pow(X, -0.5) * sgn;
+    double nuB = nuc * sth;
+    double rperp = ne * EE * EE * (p - 1.0) / (ME * CL * nuB * gspan);
+    double numin = gmin * gmin * nuB;
+    double rb = nuB / nu;
+    c.rQ = -rperp * rb * rb * rb * pow(gmin, 2.0 - p) * (1.0 - pow(numin / nu, p * 0.5 - 1.0)) / (p * 0.5 - 1.0);
+    c.rV = 2.0 * (p + 2.0) / (p + 1.0) * rperp * rb * rb * pow(gmin, -(p + 1.0)) * log(gmin) * cth / sth;
+    return c;
+}
+
+KS_DEV double kappa_rhoQ_fit(int i, double w, double X) {
+    double sw = sqrt(w), e5 = exp(-5.0 * w);
+    if (i == 1) return (17.0 * w - 3.0 * sw + 7.0 * sw * e5) * (1.0 - exp(-pow(X, 0.84) / 30.0) - sin(X / 10.0) * exp(-1.5 * pow(X, 0.471)));
+    if (i == 2) return (46.0 * w / 3.0 - 5.0 * sw / 3.0 + 17.0 * sw / 3.0 * e5) * (1.0 - exp(-pow(X, 0.84) / 18.0) - sin(X / 6.0) * exp(-1.75 * sqrt(X)));
+    if (i == 3) return (14.0 * w - 13.0 * sw / 8.0 + 4.5 * sw * e5) * (1.0 - exp(-pow(X, 0.84) / 12.0) - sin(X / 4.0) * exp(-2.0 * pow(X, 0.525)));
+    return (12.5 * w - sw + 5.0 * sw * e5) * (1.0 - exp(-pow(X, 0.84) / 8.0) - sin(0.375 * X) * exp(-2.25 * pow(X, 0.541)));
+}
+KS_DEV double kappa_rhoV_fit(int i, double w, double X) {
+    double xm = 1.0 / sqrt(X);
+    if (i == 1) return (w * w + 2.0 * w + 1.0) / (3.125 * w * w + 4.0 * w + 1.0) * (1.0 - 0.17 * log1p(0.447 * xm));
+    if (i == 2) return (w * w + 54.0 * w + 50.0) / (30.0 * w * w / 11.0 + 134.0 * w + 50.0) * (1.0 - 0.17 * log1p(0.391 * xm));
+    if (i == 3) return (w * w + 43.0 * w + 38.0) / (7.0 * w * w / 3.0 + 92.5 * w + 38.0) * (1.0 - 0.17 * log1p(0.348 * xm));
+    return (w + 13.0 / 14.0) / (2.0 * w + 13.0 / 14.0) * (1.0 - 0.17 * log1p(0.313 * xm));
+}
+// kappa (kappa.jl:25-119); gK = {G(k-4/3), G(k-2), G(k/4-1/3), G(k/4+4/3), G(2+k/2), K0/(K2+1e-40) at 1/w}
+KS_DEV Coef kappa(double ne, double kap, double w, double hyp, const double* gK, double B, double nu, double sth, double cth) {
+    Coef c;
+    const double small = 1e-40;
+    double nuc = EE * B / (2.0 * PI * ME * CL);
+    double nuw = (w * kap) * (w * kap) * nuc * sth;
+    double X = nu / nuw;
+    double pref = ne * EE * EE * nuc * sth / CL;
+    double g43 = gK[0], g2 = gK[1], gq = gK[2], gp = gK[3], gk2 = gK[4];
+    const double c37 = 12.980246132766677;      // 3^(7/3)
+    double NlowI = 4.0 * PI * g43 / (c37 * g2);
+    double hi = 0.25 * pow(3.0, (kap - 1.0) * 0.5) * (kap - 2.0) * (kap - 1.0) * gq * gp;
+    double NhighI = hi + small;
+    double X13 = cbrt(X);
+    double xI = 3.0 * pow(kap, -1.5);
+    c.jI = pref * NlowI * X13 * pow(1.0 + pow(X, xI * (3.0 * kap - 4.0) / 6.0) * pow(NlowI / NhighI, xI), -1.0 / xI);
+    double NlowQ = -0.5 * 4.0 * PI * g43 / (c37 * g2);
+    double NhighQ = -(0.64 + kap / 50.0) * hi + small;
+    double xQ = 3.7 * pow(kap, -1.6);
+    c.jQ = -(pref * NlowQ * X13 * pow(1.0 + pow(X, xQ * (3.0 * kap - 4.0) / 6.0) * pow(NlowQ / NhighQ, xQ), -1.0 / xQ));
+    double NlowV = -0.5625 * pow(pow(sth, -2.4) - 1.0, 0.48) * (pow(kap, -0.528) / w) * pow(X, -0.35) * 4.0 * PI * g43 / (c37 * g2);
+    double NhighV = -0.765625 * pow(pow(sth, -2.5) - 1.0, 0.44) * (pow(kap, -0.44) / w) / sqrt(X) * hi - small;
+    double xV = xI;
+    double sgn = cth > 0.0 ? 1.0 : (cth < 0.0 ? -1.0 : 0.0);
+    c.jV = fabs(NhighV) < small * small ? 0.0
+         : -(pref * NlowV * X13 * pow(1.0 + pow(X, xV * (3.0 * kap - 4.0) / 6.0) * pow(NlowV / NhighV, xV), -1.0 / xV)) * sgn;
+    double apref = ne * EE / (B * sth);
+    const double g53 = 0.9027452929509338;     // Gamma(5/3)
+    double ck = (kap - 2.0) * (kap - 1.0) * kap;
+    const double c16 = 1.2009369551760027;     // 3^(1/6)
+    const double tpi2 = 39.47841760435743;     // (2 pi)^2
+    double wk = w * kap;
+    double lowA = c16 * (10.0 / 41.0) * tpi2 / pow(wk, 16.0 / 3.0 - kap) * ck / (3.0 * kap - 1.0) * g53 * hyp;
+    double highA = 2.0 * 17.493418327624862 / 3.0 * ck / pow(wk, 5.0) * (2.0 * gk2 / (2.0 + kap) - 1.0);   // 2 pi^(5/2)/3
+    double NlowA = lowA;
+    double NhighA = highA * (pow(3.0 / kap, 4.75) + 0.6) + small;
+    double xA = pow(-1.75 + 1.6 * kap, -0.86);
+    double Xm53 = pow(X, -5.0 / 3.0);
+    c.aI = apref * NlowA * Xm53 * pow(1.0 + pow(X, xA * (3.0 * kap - 1.0) / 6.0) * pow(NlowA / NhighA, xA), -1.0 / xA);
+    double NlowQA = -(25.0 / 48.0) * lowA;
+    double NhighQA = -(441.0 * pow(kap, -5.76) + 0.55) * highA + small;
+    double xQA = 1.4 * pow(kap, -1.15);
+    c.aQ = -(apref * NlowQA * Xm53 * pow(1.0 + pow(X, xQA * (3.0 * kap - 1.0) / 6.0) * pow(NlowQA / NhighQA, xQA), -1.0 / xQA));
+    double NlowVA = -(0.77 / w) * pow(pow(sth, -2.28) - 1.0, 0.446) * pow(X, -0.35) * pow(kap, -0.7) * lowA;
+    double NhighVA = -(14.3 * pow(w, -0.928)) * sqrt(pow(sth, -2.05) - 1.0) *
+                     (169.0 * pow(kap, -8.0) + 13.0 / 2500.0 * kap - 263.0 / 5000.0 + 47.0 / (200.0 * kap)) / sqrt(X) * highA + small;
+    double xVA = 1.22 * pow(kap, -1.136) + 0.007;
+    c.aV = -(apref * NlowVA * Xm53 * pow(1.0 + pow(X, xVA * (3.0 * kap - 1.0) / 6.0) * pow(NlowVA / NhighVA, xVA), -1.0 / xVA)) * sgn;
+    // rotativities (Marszewski+ 2021), linear in kappa between the fits
+    double kc = dmin(dmax(kap, 3.5), 5.0);
+    int i = kc < 4.0 ? 1 : (kc < 4.5 ? 2 : 3);
+    double klo = 3.5 + (i - 1) * 0.5;
+    double f = (kc - klo) * 2.0;
+    double Xr = nu / ((w * kap) * (w * kap) * nuc * sth);
+    double fQ = (1.0 - f) * kappa_rhoQ_fit(i, w, Xr) + f * kappa_rhoQ_fit(i + 1, w, Xr);
+    double fV = (1.0 - f) * kappa_rhoV_fit(i, w, Xr) + f * kappa_rhoV_fit(i + 1, w, Xr);
+    c.rQ = -(ne * EE * EE * nuc * nuc * sth * sth) / (ME * CL * nu * nu * nu) * fQ;
+    c.rV = 2.0 * (ne * EE * EE * nuc * cth) / (ME * CL * nu * nu) * gK[5] * fV;
+    return c;
+}
+
+// ---- cap, invariants, rotation -----------------------------------------------------------------------------------
+KS_DEV double pow2_below(double x) {
+    if (!(isfinite(x) && x > 0.0)) return 1.0;
+    int e; frexp(x, &e);
+    return ldexp(1.0, e - 1);
+}
+KS_DEV double cap_factor(double I, double Q, double V) {
+    const double fmax = 0.99;
+    if (!(I > 0.0)) return (Q == 0.0 && V == 0.0) ? 1.0 : 0.0;
+    double s = pow2_below(dmax(dmax(fabs(I), fabs(Q)), fabs(V)));
+    double Ih = I / s, q = Q / s, v = V / s;
+    double pp = q * q + v * v;
+    double p = pp > 0.0 ? sqrt(pp) : 0.0;
+    return p > fmax * Ih ? fmax * Ih / p : 1.0;
+}
+
+// ---- the exact 4x4 step -------------------------------------------------------------------------------------------
+KS_DEV double phi1(double a) {
+    if (fabs(a) < 1e-3) return 1.0 - a * (0.5 - a * (1.0 / 6.0 - a * (1.0 / 24.0 - a / 120.0)));
+    return -expm1(-a) / a;
+}
+// M_n(a) = int_0^1 t^n e^{-at} dt, n = 0..9
+KS_DEV void moments(double a, double M[10]) {
+    if (fabs(a) < 1.0) {
+        for (int n = 0; n < 10; ++n) {
+            double term = 1.0, sum = 1.0 / (n + 1.0);
+            for (int m = 1; m < 24; ++m) { term *= -a / m; sum += term / (n + m + 1.0); }
+            M[n] = sum;
+        }
+    } else if (a < 40.0) {
+        double e = exp(-a);
+        for (int n = 0; n < 10; ++n) {
+            double term = 1.0 / (n + 1.0), sum = term;
+            for (int m = 1; m < 200; ++m) {
+                term *= a / (n + m + 1.0);
+                sum += term;
+                if (term < 1e-18 * sum) break;
+            }
+            M[n] = e * sum;
+        }
+    } else {
+        double e = exp(-a);
+        M[0] = (1.0 - e) / a;
+        for (int n = 1; n < 10; ++n) M[n] = (n * M[n - 1] - e) / a;
+    }
+}
+KS_DEV double sinhc(double b) { return fabs(b) < 0.1 ? 1.0 + b * b / 6.0 * (1.0 + b * b / 20.0 * (1.0 + b * b / 42.0 * (1.0 + b * b / 72.0))) : sinh(b) / b; }
+KS_DEV double sinc_(double b) { return fabs(b) < 0.1 ? 1.0 - b * b / 6.0 * (1.0 - b * b / 20.0 * (1.0 - b * b / 42.0 * (1.0 - b * b / 72.0))) : sin(b) / b; }
+
+// O and E = Ej j for K = alpha_I + K' (step.jl:164-243), D > 0. Writes O row-major.
+KS_DEV void transfer_step(const double j4[4], const double al[4], const double rh[3], double D, double O[16], double E[4]) {
+    double Dp = pow2_below(D);
+    double f = D / Dp;
+    double aI = al[0] * Dp, aQ = al[1] * Dp, aU = al[2] * Dp, aV = al[3] * Dp;
+    double rQ = rh[0] * Dp, rU = rh[1] * Dp, rV = rh[2] * Dp;
+    double K[16] = {0.0, aQ, aU, aV, aQ, 0.0, rV, -rU, aU, -rV, 0.0, rQ, aV, rU, -rQ, 0.0};
+    double a2 = aQ * aQ + aU * aU + aV * aV, r2 = rQ * rQ + rU * rU + rV * rV;
+    double a = aI * f;
+    double e = exp(-a);
+    // coefficients of O = o0 1 + o1 K + o2 K^2 + o3 K^3 and Ej = e0 1 + e1 K + e2 K^2 + e3 K^3 (times Dp)
+    double o0, o1, o2, o3, e0, e1, e2, e3;
+    double K2[16], K3[16];
+    bool cubic = false;
+    if ((a2 + r2) * f * f < 1e-200) {
+        double M[10]; moments(a, M);
+        o0 = e; o1 = -f * e; o2 = o3 = 0.0;
+        e0 = f * phi1(a); e1 = -f * f * M[1]; e2 = e3 = 0.0;
+    } else {
+        cubic = true;
+        double ar = aQ * rQ + aU * rU + aV * rV;
+        double h = (a2 - r2) * 0.5;
+        double root = sqrt(h * h + ar * ar);
+        double L1sq, L2sq;
+        if (h >= 0.0) { L1sq = h + root; L2sq = L1sq > 0.0 ? ar * ar / L1sq : 0.0; }
+        else { L2sq = root - h; L1sq = L2sq > 0.0 ? ar * ar / L2sq : 0.0; }
+        double L1 = L1sq > 0.0 ? sqrt(L1sq) : 0.0, L2 = L2sq > 0.0 ? sqrt(L2sq) : 0.0;
+        double b1 = L1 * f, b2 = L2 * f;
+        double Th = L1sq + L2sq;
+        double M[10]; moments(a, M);
+        if (Th * f * f * (1.0 + (a2 + r2) * f * f) < 1e-17) {
+            o0 = e; o1 = -f * e; o2 = f * f * 0.5 * e; o3 = -f * f * f / 6.0 * e;
+            e0 = f * M[0]; e1 = -f * f * M[1]; e2 = f * f * f * M[2] * 0.5; e3 = -f * f * f * f * M[3] / 6.0;
+        } else {
+            double ech, esh, echm1;
+            if (b1 < 1.0) {
+                ech = e * cosh(b1); esh = e * sinhc(b1); double sh = sinh(0.5 * b1); echm1 = e * 2.0 * sh * sh;
+            } else {
+                double em = exp(-(a - b1)), ep = exp(-(a + b1));
+                ech = 0.5 * (em + ep); esh = (em - ep) / (2.0 * b1); echm1 = 0.5 * (em - 2.0 * e + ep);
+            }
+            double sb2h = sin(0.5 * b2);
+            double versin2 = 2.0 * sb2h * sb2h;
+            double cb2 = cos(b2);
+            double iTh = 1.0 / Th;
+            o0 = (L2sq * ech + L1sq * e * cb2) * iTh;
+            o2 = (echm1 + e * versin2) * iTh;
+            double s3 = f * esh, s2 = f * e * sinc_(b2);
+            o1 = -(s3 * L2sq + s2 * L1sq) * iTh;
+            o3 = -(s3 - s2) * iTh;
+            double pa = phi1(a);
+            // int_0^1 e^{-at}(cosh bt - 1), (1 - cos bt), sin(bt)/b, (sinh(bt)/b - t)
+            double icm1, ivs, isc, ish;
+            if (b1 < 0.1) {
+                double bb = b1 * b1;
+                icm1 = bb * (M[2] / 2.0 + bb * (M[4] / 24.0 + bb * (M[6] / 720.0 + bb * M[8] / 40320.0)));
+                ish = bb * (M[3] / 6.0 + bb * (M[5] / 120.0 + bb * (M[7] / 5040.0 + bb * M[9] / 362880.0)));
+            } else {
+                double pm = phi1(a - b1), pp = phi1(a + b1);
+                icm1 = 0.5 * (pm + pp) - pa;
+                ish = (pm - pp) / (2.0 * b1) - M[1];
+            }
+            if (b2 < 0.1) {
+                double bb = b2 * b2;
+                ivs = bb * (M[2] / 2.0 - bb * (M[4] / 24.0 - bb * (M[6] / 720.0 - bb * M[8] / 40320.0)));
+                isc = M[1] - bb * (M[3] / 6.0 - bb * (M[5] / 120.0 - bb * (M[7] / 5040.0 - bb * M[9] / 362880.0)));
+            } else {
+                double sb = sin(b2);
+                double den = 1.0 / (a * a + b2 * b2);
+                ivs = pa - (a - e * (a * cb2 - b2 * sb)) * den;
+                isc = (1.0 - e * (a * sb / b2 + cb2)) * den;
+            }
+            double Ich = f * (pa + icm1);
+            double Ic = f * (pa - ivs);
+            double Ish = f * f * (M[1] + ish);
+            double Is = f * f * isc;
+            e0 = (L2sq * Ich + L1sq * Ic) * iTh;
+            e2 = f * (icm1 + ivs) * iTh;
+            e1 = -(Ish * L2sq + Is * L1sq) * iTh;
+            e3 = -(Ish - Is) * iTh;
+        }
+    }
+    if (cubic) {
+        for (int i = 0; i < 4; ++i) for (int k = 0; k < 4; ++k) {
+            double s = 0.0;
+            for (int q = 0; q < 4; ++q) s += K[4 * i + q] * K[4 * q + k];
+            K2[4 * i + k] = s;
+        }
+        for (int i = 0; i < 4; ++i) for (int k = 0; k < 4; ++k) {
+            double s = 0.0;
+            for (int q = 0; q < 4; ++q) s += K2[4 * i + q] * K[4 * q + k];
+            K3[4 * i + k] = s;
+        }
+    } else {
+        for (int i = 0; i < 16; ++i) { K2[i] = 0.0; K3[i] = 0.0; }
+    }
+    for (int i = 0; i < 4; ++i) {
+        double ei = 0.0;
+        for (int k = 0; k < 4; ++k) {
+            double id = i == k ? 1.0 : 0.0;
+            int ix = 4 * i + k;
+            O[ix] = o0 * id + o1 * K[ix] + o2 * K2[ix] + o3 * K3[ix];
+            double ej = e0 * id + e1 * K[ix] + e2 * K2[ix] + e3 * K3[ix];
+            ei += ej * j4[k];
+        }
+        E[i] = Dp * ei;
+    }
+}
+
+}  // namespace ksd

@@ -14,12 +14,24 @@
 using namespace kso;
 
 namespace {
-template <class T> Model make_model(int kind, int nrows, int ns, const double* params, const double* hyp, double gamma_max,
-                                    const double* knots, const double* knot_times, int nknots) {
-    Model m;
+// the model and its per-splat Bessel table (kept alive by the caller's ModelHolder)
+struct ModelHolder { Model m; std::vector<double> bessel; };
+template <class T> ModelHolder make_model(int kind, int nrows, int ns, const double* params, const double* hyp, double gamma_max,
+                                          const double* knots, const double* knot_times, int nknots) {
+    ModelHolder h;
+    Model& m = h.m;
     m.kind = kind; m.nrows = nrows; m.ns = ns; m.params = params; m.hyp = hyp; m.gamma_max = gamma_max;
     m.knots = knots; m.knot_times = knot_times; m.nknots = nknots;
-    return m;
+    h.bessel.assign(3 * (size_t)std::max(ns, 1), 0.0);
+    for (int i = 0; i < ns; ++i) {
+        double x = 0.0;
+        if (kind == MODEL_THERMAL || kind == MODEL_KNOT) x = 1.0 / std::exp(params[13 + (size_t)nrows * i]);
+        else if (kind == MODEL_KAPPA) x = 1.0 / std::exp(params[14 + (size_t)nrows * i]);
+        else continue;
+        besselk012(x, h.bessel[3 * i], h.bessel[3 * i + 1], h.bessel[3 * i + 2]);
+    }
+    m.bessel = h.bessel.data();
+    return h;
 }
 
 // march one ray with the chosen marcher and fold a consumer over its samples
@@ -112,7 +124,8 @@ int kso_samples(double a, double thetao, int npix, const double* alphas, const d
 // optically thin Stokes I image (screen order)
 int kso_thin_image(double a, double thetao, int npix, const double* alphas, const double* betas, int N, int M, int marcher,
                    int ns, const double* params, double t_obs, double* out) {
-    Model m = make_model<double>(MODEL_THIN, 14, ns, params, nullptr, 0, nullptr, nullptr, 0);
+    ModelHolder mh = make_model<double>(MODEL_THIN, 14, ns, params, nullptr, 0, nullptr, nullptr, 0);
+    const Model& m = mh.m;
 #pragma omp parallel for schedule(dynamic, 16)
     for (int i = 0; i < npix; ++i) {
         Pixel<double> p = pixel_constants<double>(a, alphas[i], betas[i], thetao);
@@ -130,7 +143,8 @@ int kso_polarized_image(double a, double thetao, int npix, const double* alphas,
                         int kind, int nrows, int ns, const double* params, const double* hyp, double gamma_max,
                         const double* knots, const double* knot_times, int nknots,
                         double t_obs, double nu, double L, int nmax, double slab, double* out, double* resid_t, double* resid_ph) {
-    Model m = make_model<double>(kind, nrows, ns, params, hyp, gamma_max, knots, knot_times, nknots);
+    ModelHolder mh = make_model<double>(kind, nrows, ns, params, hyp, gamma_max, knots, knot_times, nknots);
+    const Model& m = mh.m;
 #pragma omp parallel for schedule(dynamic, 8)
     for (int i = 0; i < npix; ++i) {
         double rt, rp;
@@ -145,7 +159,8 @@ int kso_polarized_image(double a, double thetao, int npix, const double* alphas,
 int kso_count_polarized(double a, double thetao, int npix, const double* alphas, const double* betas, int N, int M,
                         int kind, int nrows, int ns, const double* params, const double* hyp, double gamma_max,
                         double t_obs, double nu, double L, double* flops, double* special) {
-    Model m = make_model<Counted>(kind, nrows, ns, params, hyp, gamma_max, nullptr, nullptr, 0);
+    ModelHolder mh = make_model<Counted>(kind, nrows, ns, params, hyp, gamma_max, nullptr, nullptr, 0);
+    const Model& m = mh.m;
     double f = 0, s = 0;
 #pragma omp parallel for reduction(+ : f, s) schedule(dynamic, 4)
     for (int i = 0; i < npix; ++i) {

@@ -27,6 +27,9 @@ struct Model {
     const double* knots = nullptr;    // 3 x nknots x ns
     const double* knot_times = nullptr;
     int nknots = 0;
+    // per-splat Bessel values that depend only on the splat (K0, K1, K2 at 1/Theta_e for thermal
+    // and knot splats, at 1/w for kappa splats), filled once per model
+    const double* bessel = nullptr;
     double P(int row, int i) const { return params[row + nrows * i]; }
 };
 
@@ -115,11 +118,13 @@ template <class T> bool element(const Model& m, int i, const RayCtx& rc, const S
     T thB = kacos(kmin(kmax(fr.cosB, T(-1.0)), T(1.0)));
     T ne = kexp(T(m.P(12, i))) * G;
     if (m.kind == MODEL_THERMAL || m.kind == MODEL_KNOT) {
-        cf = thermal_synchrotron(ne, kexp(T(m.P(13, i))), Bmag, nuf, thB);
+        const double* kb = m.bessel + 3 * i;
+        cf = thermal_synchrotron_k(ne, kexp(T(m.P(13, i))), Bmag, nuf, thB, T(kb[0]), T(kb[1]), T(kb[2]));
     } else if (m.kind == MODEL_POWERLAW) {
         cf = powerlaw_synchrotron(ne, T(m.P(13, i)), kexp(T(m.P(14, i))), T(m.gamma_max), Bmag, nuf, thB);
     } else {
-        cf = kappa_synchrotron(ne, T(m.P(13, i)), kexp(T(m.P(14, i))), Bmag, nuf, thB, T(m.hyp[i]));
+        const double* kb = m.bessel + 3 * i;
+        cf = kappa_synchrotron(ne, T(m.P(13, i)), kexp(T(m.P(14, i))), Bmag, nuf, thB, T(m.hyp[i]), T(kb[0] / (kb[2] + 1e-40)));
     }
     return true;
 }

@@ -205,7 +205,7 @@ template <class T> T kappa_rhoV_fit(int i, T w, T X) {
     if (i == 3) return (w * w + 43.0 * w + 38.0) / (7.0 * w * w / 3.0 + 185.0 * w / 2.0 + 38.0) * (1.0 - 0.17 * klog(1.0 + 0.348 * kpow(X, T(-0.5))));
     return (w + 13.0 / 14.0) / (2.0 * w + 13.0 / 14.0) * (1.0 - 0.17 * klog(1.0 + 0.313 * kpow(X, T(-0.5))));
 }
-template <class T> void kappa_rotativities(T ne, T kappa, T w, T B, T nu, T th, T& rQ, T& rV) {
+template <class T> void kappa_rotativities(T ne, T kappa, T w, T B, T nu, T th, T& rQ, T& rV, T k0k2 = T(-1.0)) {
     T sth = ksin(th), cth = kcos(th);
     T nuc = EE * B / (2.0 * KS_PI * ME * CL);
     T X = nu / ((w * kappa) * (w * kappa) * nuc * sth);
@@ -215,12 +215,15 @@ template <class T> void kappa_rotativities(T ne, T kappa, T w, T B, T nu, T th, 
     T f = (kc - klo) * 2.0;
     T fQ = (1.0 - f) * kappa_rhoQ_fit(i, w, X) + f * kappa_rhoQ_fit(i + 1, w, X);
     T fV = (1.0 - f) * kappa_rhoV_fit(i, w, X) + f * kappa_rhoV_fit(i + 1, w, X);
-    T k0, k1, k2;
-    besselk012(1.0 / w, k0, k1, k2);
+    if (k0k2 < 0.0) {
+        T k0, k1, k2;
+        besselk012(1.0 / w, k0, k1, k2);
+        k0k2 = k0 / (k2 + 1e-40);
+    }
     rQ = -(ne * EE * EE * nuc * nuc * sth * sth) / (ME * CL * nu * nu * nu) * fQ;
-    rV = 2.0 * (ne * EE * EE * nuc * cth) / (ME * CL * nu * nu) * (k0 / (k2 + 1e-40)) * fV;
+    rV = 2.0 * (ne * EE * EE * nuc * cth) / (ME * CL * nu * nu) * k0k2 * fV;
 }
-template <class T> Coef<T> kappa_synchrotron(T ne, T kappa, T w, T B, T nu, T th, T hyp) {
+template <class T> Coef<T> kappa_synchrotron(T ne, T kappa, T w, T B, T nu, T th, T hyp, T k0k2 = T(-1.0)) {
     T sth = ksin(th), cth = kcos(th);
     T nuc = EE * B / (2.0 * KS_PI * ME * CL);
     T nuw = (w * kappa) * (w * kappa) * nuc * sth;
@@ -268,7 +271,7 @@ template <class T> Coef<T> kappa_synchrotron(T ne, T kappa, T w, T B, T nu, T th
     T xVA = (61.0 / 50.0) * kpow(kappa, T(-142.0 / 125.0)) + 7.0 / 1000.0;
     T aV = -(apref * NlowVA * kpow(X, T(-5.0 / 3.0)) * kpow(1.0 + kpow(X, xVA * (3.0 * kappa - 1.0) / 6.0) * kpow(NlowVA / NhighVA, xVA), -1.0 / xVA)) * sgn;
     T rQ, rV;
-    kappa_rotativities(ne, kappa, w, B, nu, th, rQ, rV);
+    kappa_rotativities(ne, kappa, w, B, nu, th, rQ, rV, k0k2);
     return {jI, jQ, jV, aI, aQ, aV, rQ, rV};
 }
 
