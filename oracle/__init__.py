@@ -224,3 +224,15 @@ def vis_seed(W, nalpha, nbeta, psize, u, v, taper=None):
     seed = np.zeros((nf, nalpha * nbeta, 4))
     lib().kso_vis_seed(_i(nalpha), _i(nbeta), _i(nf), _d(psize), _i(len(uu)), _p(uu), _p(vv), _p(tp), _p(Wr), _p(Wi), _p(seed))
     return seed
+
+
+def polarized_gradient(a, thetao, alphas, betas, N, params, t_obs, nu, L, dstokes, nmax=-1, slab=0.0, M=64):
+    """The dual-sweep gradient (thermal splats): returns (dparams 21 x ns, image npix x 4)."""
+    al, be = _f64(alphas), _f64(betas)
+    P = np.asfortranarray(params, dtype=np.float64)
+    ds = _f64(dstokes)
+    g = np.zeros(P.shape, order="F")
+    img = np.zeros((len(al), 4))
+    lib().kso_polarized_gradient(_d(a), _d(thetao), _i(len(al)), _p(al), _p(be), _i(N), _i(M), _i(P.shape[1]), _p(P),
+                                 _d(t_obs), _d(nu), _d(L), _i(nmax), _d(slab), _p(ds), g.ctypes.data_as(_P), _p(img))
+    return g, img

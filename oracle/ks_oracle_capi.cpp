@@ -6,6 +6,7 @@
 // pixels with OpenMP (one ray per iteration, the reference's KA CPU() backend layout).
 #include "ks_splats.hpp"
 #include "ks_vis.hpp"
+#include "ks_adjoint.hpp"
 #include <cstring>
 #include <vector>
 #include <numeric>
@@ -170,6 +171,26 @@ int kso_count_polarized(double a, double thetao, int npix, const double* alphas,
         f += op_counts().flops; s += op_counts().special;
     }
     *flops = f; *special = s;
+    return 0;
+}
+
+// gradient of sum_pixels dstokes . image w.r.t. thermal splat parameters (the dual sweep,
+// dual_gradient.jl:225-245): dparams (21 x ns, column-major) accumulates; image (npix x 4) is written
+int kso_polarized_gradient(double a, double thetao, int npix, const double* alphas, const double* betas, int N, int M,
+                           int ns, const double* params, double t_obs, double nu, double L, int nmax, double slab,
+                           const double* dstokes, double* dparams, double* image) {
+    ModelHolder mh = make_model<double>(MODEL_THERMAL, 21, ns, params, nullptr, 0, nullptr, nullptr, 0);
+    const Model& m = mh.m;
+    int nth = omp_get_max_threads();
+    std::vector<std::vector<double>> part(nth, std::vector<double>(21 * (size_t)ns, 0.0));
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int i = 0; i < npix; ++i) {
+        std::vector<double>& g = part[omp_get_thread_num()];
+        polarized_gradient_ray(m, a, thetao, alphas[i], betas[i], N, M, t_obs, nu, L, nmax, slab, dstokes + 4 * (size_t)i, g.data(),
+                               image + 4 * (size_t)i);
+    }
+    for (int t = 0; t < nth; ++t)
+        for (size_t k = 0; k < 21 * (size_t)ns; ++k) dparams[k] += part[t][k];
     return 0;
 }
 
