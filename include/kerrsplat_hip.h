@@ -46,6 +46,10 @@ void* ks_ctx_stream(ks_ctx* ctx);                 /* the hipStream_t every launc
 /* HIP events on the context's stream around a timed region (no reference counterpart; bench.py). */
 int ks_timing_begin(ks_ctx* ctx);
 int ks_timing_end(ks_ctx* ctx, float* ms);
+/* Duration of the last launch of the dominant kernel (the segment march + transfer, or the adjoint's
+ * per-splat sweep), bracketed by HIP events on the context's stream; synchronizes. No reference
+ * counterpart (measurement hook for bench.py). */
+int ks_last_kernel_ms(ks_ctx* ctx, float* ms);
 
 /* GeodesicCache(backend, camera, Val(N); store_samples) - cache.jl:106-125. */
 int ks_cache_create(ks_ctx* ctx, int64_t npix, int N, int store_samples, ks_cache** cache);
@@ -96,6 +100,18 @@ int ks_render_device(ks_cache* cache, const ks_model* model, const double* times
 /* Copy the first `frames` rendered frames to the host (to_screen, cache.jl:293-296, when screen_order). */
 int ks_copy_stokes(ks_cache* cache, int64_t frames, double* out_stokes, int screen_order);
 const double* ks_stokes_device_ptr(const ks_cache* cache);
+
+/* polarized_gradient!(dparams, dstokes, cache, params, t_obs, ν_obs, L; method = :dual, nmax, slab)
+ * - polarized.jl:473-480 with polarized_tails! and polarized_dual_sweep! (dual_gradient.jl:180-249):
+ * the gradient of l = Σ dstokes · I_obs over every (pixel, time, frequency) with respect to the
+ * thermal splat parameters, ADDED into dparams (21 x ns, column-major, as the reference's
+ * dparams .+= ...). dstokes and the optional image (the observed Stokes vectors of the same
+ * forward pass) are 4 x npix x nt x nν, in screen order when screen_order, else in the cache's
+ * sorted order. Geodesic samples are not stored: the forward segment march records each sample's
+ * coefficients, and the adjoint re-uses them. Thermal splats only (KS_ERR_ARG otherwise). */
+int ks_polarized_gradient(ks_cache* cache, const ks_model* model, const double* times, int nt, const double* nus, int nnu,
+                          double L, int nmax, double slab, const double* dstokes, int screen_order,
+                          double* dparams, double* image);
 
 /* visibilities / vis_kernel! - visibilities.jl:21-40, device_visibilities.jl:61-73: V of the
  * rendered frames at nuv (u, v) points [wavelengths], psize = Δα L / D [rad], taper per point

@@ -73,6 +73,7 @@ def _load():
     lib.ks_thin_image.argtypes = [_P, _P, _d, _P, _i]
     lib.ks_visibilities.argtypes = [_P, _i64, _d, _P, _P, _P, _i, _P]
     lib.ks_timing_end.argtypes = [_P, ctypes.POINTER(ctypes.c_float)]
+    lib.ks_last_kernel_ms.argtypes = [_P, ctypes.POINTER(ctypes.c_float)]
     _lib = lib
     return lib
 

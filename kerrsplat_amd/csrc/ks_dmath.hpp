@@ -11,8 +11,7 @@
 #include <math.h>
 #include <stdint.h>
 
-#define KS_DEV __device__ __forceinline__
-#define KS_HD __host__ __device__ __forceinline__
+#define KS_DEV __host__ __device__ __forceinline__
 
 namespace ksd {
 

@@ -11,6 +11,7 @@
 // once per splat on the host instead of at every sample.
 #pragma once
 #include "ks_dgeo.hpp"
+#include "ks_ddual.hpp"
 
 namespace ksd {
 
@@ -30,7 +31,7 @@ struct SplatC {
     double omega;              // pattern angular velocity
     double rsup2;              // support sphere radius^2 = -2 ln(cutoff) exp(2 max s)
     double lnA;                // thin: logA; others: logne
-    double Bmag, B[3];         // field magnitude and vector (ZAMO axes r, phi, -theta)
+    double Bmag, bh[3];        // field magnitude and unit direction (ZAMO axes r, phi, -theta)
     double u[3];               // fluid spatial 4-velocity (ZAMO axes)
     double p1, p2, p3;         // thermal: Theta_e, K0/K2, K1/K2 at 1/Theta_e; power law: p, gamma_min, gamma_max;
                                // kappa: kappa, w, hyp
@@ -106,93 +107,94 @@ KS_DEV void make_geo(double a, double r, double th, double eta, double lam, bool
     g.pz[3] = -g.sgthth * g.pu[2];
 }
 
-struct Frame { double g, cosB, chi; };
+template <class T> struct FrameT { T g, cosB, chi; };
+using Frame = FrameT<double>;
 
-// frames.jl:108-128 for one splat (its boost built from u)
-KS_DEV Frame local_frame(const Geo& G, double a, double alpha, double beta, double thetao_sin, const double u[3], const double B[3], double Bmag) {
-    double u2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-    double gam = sqrt(1.0 + u2);
-    double q = 1.0 / (gam + 1.0);
+// frames.jl:108-128 for one splat: its boost built from the fluid velocity u, the field direction bh
+// (unit); T = double, or duals in u and bh for the adjoint
+template <class T>
+KS_DEV FrameT<T> local_frame(const Geo& G, double a, double alpha, double beta, double thetao_sin, const T u[3], const T bh[3]) {
+    T u2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    T gam = sqrt(1.0 + u2);
+    T q = 1.0 / (gam + 1.0);
     // p_f = Lambda p_z
-    double ud = u[0] * G.pz[1] + u[1] * G.pz[2] + u[2] * G.pz[3];
-    double pf0 = gam * G.pz[0] - ud;
-    double c1 = -G.pz[0] + ud * q;
-    double pf1 = G.pz[1] + u[0] * c1, pf2 = G.pz[2] + u[1] * c1, pf3 = G.pz[3] + u[2] * c1;
-    Frame fr;
+    T ud = u[0] * G.pz[1] + u[1] * G.pz[2] + u[2] * G.pz[3];
+    T pf0 = gam * G.pz[0] - ud;
+    T c1 = ud * q - G.pz[0];
+    T pf1 = G.pz[1] + u[0] * c1, pf2 = G.pz[2] + u[1] * c1, pf3 = G.pz[3] + u[2] * c1;
+    FrameT<T> fr;
     fr.g = 1.0 / pf0;
-    double k0 = pf1 * fr.g, k1 = pf2 * fr.g, k2 = pf3 * fr.g;
-    double b0, b1, b2;
-    if (Bmag > 0.0) { double ib = 1.0 / Bmag; b0 = B[0] * ib; b1 = B[1] * ib; b2 = B[2] * ib; }
-    else { b0 = 0.0; b1 = 0.0; b2 = 1.0; }
-    fr.cosB = k0 * b0 + k1 * b1 + k2 * b2;
-    double f0 = k1 * b2 - k2 * b1, f1 = k2 * b0 - k0 * b2, f2 = k0 * b1 - k1 * b0;
-    double fn = sqrt(f0 * f0 + f1 * f1 + f2 * f2);
-    if (!(fn > 0.0)) { fr.chi = 0.0; return fr; }
-    double ifn = 1.0 / fn;
-    f0 *= ifn; f1 *= ifn; f2 *= ifn;
+    T k0 = pf1 * fr.g, k1 = pf2 * fr.g, k2 = pf3 * fr.g;
+    fr.cosB = k0 * bh[0] + k1 * bh[1] + k2 * bh[2];
+    T f0 = k1 * bh[2] - k2 * bh[1], f1 = k2 * bh[0] - k0 * bh[2], f2 = k0 * bh[1] - k1 * bh[0];
+    T fn = sqrt(f0 * f0 + f1 * f1 + f2 * f2);
+    if (!(fn > 0.0)) { fr.chi = T(0.0); return fr; }
+    T ifn = 1.0 / fn;
+    f0 = f0 * ifn; f1 = f1 * ifn; f2 = f2 * ifn;
     // inverse boost of (0, f): time part = u.f, spatial f + u (u.f) q
-    double uf = u[0] * f0 + u[1] * f1 + u[2] * f2;
-    double fz0 = uf;
-    double fz1 = f0 + u[0] * uf * q, fz2 = f1 + u[1] * uf * q, fz3 = f2 + u[2] * uf * q;
+    T uf = u[0] * f0 + u[1] * f1 + u[2] * f2;
+    T ufq = uf * q;
+    T fz1 = f0 + u[0] * ufq, fz2 = f1 + u[1] * ufq, fz3 = f2 + u[2] * ufq;
     // ZAMO -> BL contravariant
-    double ft = fz0 / G.lapse;
-    double fr_ = fz1 / G.sgrr;
-    double fth = -fz3 / G.sgthth;
-    double fph = fz2 / G.sgpp + G.omega * ft;
+    T ft = uf * (1.0 / G.lapse);
+    T fr_ = fz1 * (1.0 / G.sgrr);
+    T fth = fz3 * (-1.0 / G.sgthth);
+    T fph = fz2 * (1.0 / G.sgpp) + G.omega * ft;
     const double* p = G.pu;
-    double A = p[0] * fr_ - p[1] * ft + a * G.st * G.st * (p[1] * fph - p[3] * fr_);
-    double Bw = ((G.r * G.r + a * a) * (p[3] * fth - p[2] * fph) - a * (p[0] * fth - p[2] * ft)) * G.st;
-    double kk1 = A * G.r - Bw * a * G.ct, kk2 = -(A * a * G.ct + Bw * G.r);
+    T A = p[0] * fr_ - p[1] * ft + (a * G.st * G.st) * (p[1] * fph - p[3] * fr_);
+    T Bw = ((G.r * G.r + a * a) * (p[3] * fth - p[2] * fph) - a * (p[0] * fth - p[2] * ft)) * G.st;
+    T kk1 = A * G.r - Bw * (a * G.ct), kk2 = -(A * (a * G.ct) + Bw * G.r);
     double mu = -(alpha + a * thetao_sin);
-    double ea = beta * kk2 - mu * kk1, eb = beta * kk1 + mu * kk2;
+    T ea = beta * kk2 - mu * kk1, eb = beta * kk1 + mu * kk2;
     fr.chi = atan2(-ea, eb);
     return fr;
 }
 
 // ---- coefficients ------------------------------------------------------------------------------------------------
-struct Coef { double jI, jQ, jV, aI, aQ, aV, rQ, rV; };
+template <class T> struct CoefT { T jI, jQ, jV, aI, aQ, aV, rQ, rV; };
+using Coef = CoefT<double>;
 
-KS_DEV double planck(double nu, double Th) {
-    double x = HPL * nu / (ME * CL * CL * Th);
-    double nh = nu * (1.0 / NU0);
-    double pref = PLANCK0 * nh * nh * nh;
+template <class T> KS_DEV T planck(T nu, T Th) {
+    T x = HPL * nu / (ME * CL * CL * Th);
+    T nh = nu * (1.0 / NU0);
+    T pref = PLANCK0 * nh * nh * nh;
     if (x < 2e-3) return pref / (x / 24.0 * (24.0 + x * (12.0 + x * (4.0 + x))));
     return pref / expm1(x);
 }
 
-KS_DEV double cbrt_pos(double x) { return cbrt(x); }
-
-// thermal (coefficients.jl:185-229) with the splat's Bessel ratios k0/k2, k1/k2 (k2 > 0 in practice)
-KS_DEV Coef thermal(double ne, double Th, double k0k2, double k1k2, double B, double nu, double sth, double cth) {
-    Coef c;
-    double w = 2.0 * PI * nu;
-    double w0w = EE * B / (ME * CL) / w;
-    double wp2w2 = 4.0 * PI * ne * EE * EE / ME / (w * w);
+// thermal (coefficients.jl:185-229) with the splat's Bessel ratios k0/k2, k1/k2 (k2 > 0 in practice);
+// ne enters linearly and stays a plain number in the adjoint
+template <class T>
+KS_DEV CoefT<T> thermal(double ne, T Th, T k0k2, T k1k2, T B, T nu, T sth, T cth) {
+    CoefT<T> c;
+    T w = 2.0 * PI * nu;
+    T w0w = EE * B / (ME * CL) / w;
+    T wp2w2 = (4.0 * PI * ne * EE * EE / ME) / (w * w);
     if (!(sth > 0.0)) {
-        c.jI = c.jQ = c.jV = c.aI = c.aQ = c.aV = c.rQ = 0.0;
+        c.jI = c.jQ = c.jV = c.aI = c.aQ = c.aV = c.rQ = T(0.0);
         c.rV = w / CL * (wp2w2 * w0w * k0k2 * cth);
         return c;
     }
-    double x = Th * sqrt(1.4142135623730951 * sth * (1e3 * w0w));
-    double lx = log(x);
-    double extraterm = (0.011 * exp(-x / 47.2) - 0.7937005259840998 / 67.44733739010397 * PI * 1e4 * exp(-8.0 / 3.0 * log(x + 1e-16))) *
-                       (0.5 + 0.5 * tanh((lx - 4.787491742782046) / 0.1));
-    double jffunc = 2.011 * exp(-exp(1.035 * lx) / 4.7) - cos(x * 0.5) * exp(-exp(1.2 * lx) / 2.73) - 0.011 * exp(-x / 47.2) + extraterm;
-    double eps11m22 = jffunc * wp2w2 * w0w * w0w * (k1k2 + 6.0 * Th) * sth * sth;
+    T x = Th * sqrt(1.4142135623730951 * sth * (1e3 * w0w));
+    T lx = log(x);
+    T extraterm = (0.011 * exp(-x / 47.2) - 0.7937005259840998 / 67.44733739010397 * PI * 1e4 * exp(-8.0 / 3.0 * log(x + 1e-16))) *
+                  (0.5 + 0.5 * tanh((lx - 4.787491742782046) / 0.1));
+    T jffunc = 2.011 * exp(-exp(1.035 * lx) / 4.7) - cos(x * 0.5) * exp(-exp(1.2 * lx) / 2.73) - 0.011 * exp(-x / 47.2) + extraterm;
+    T eps11m22 = jffunc * wp2w2 * w0w * w0w * (k1k2 + 6.0 * Th) * sth * sth;
     c.rQ = w / (2.0 * CL) * eps11m22;
-    double ff = k0k2 * (1.0 - 0.11 * log1p(0.035 * x));
+    T ff = k0k2 * (1.0 - 0.11 * log1p(0.035 * x));
     c.rV = w / CL * (wp2w2 * w0w * ff * cth);
-    double nus = 3.0 * EE * B * sth / (4.0 * PI * ME * CL) * Th * Th + 1.0;
-    double xs = nu / nus;
-    double x13 = cbrt(xs), xm13 = 1.0 / x13, xm23 = xm13 * xm13;
-    double e = exp(-1.8899 * x13);
-    double pref = ne * EE * EE * nu / (2.0 * 1.7320508075688772 * CL * Th * Th);
+    T nus = 3.0 * EE * B * sth / (4.0 * PI * ME * CL) * Th * Th + 1.0;
+    T xs = nu / nus;
+    T x13 = cbrt(xs), xm13 = 1.0 / x13, xm23 = xm13 * xm13;
+    T e = exp(-1.8899 * x13);
+    T pref = ne * EE * EE * nu / (2.0 * 1.7320508075688772 * CL * Th * Th);
     c.jI = pref * 2.5651 * (1.0 + 1.92 * xm13 + 0.9977 * xm23) * e;
     c.jQ = pref * 2.5651 * (1.0 + 0.93193 * xm13 + 0.499873 * xm23) * e;
     c.jV = 2.0 * ne * EE * EE * nu * cth / sth / (3.0 * 1.7320508075688772 * CL * Th * Th * Th) *
            (1.81384 / xs + 3.42319 * xm23 + 0.0292545 / sqrt(xs) + 2.03773 * xm13) * e;
-    double Bn = planck(nu, Th);
-    double ib = Bn > 0.0 ? 1.0 / Bn : 0.0;
+    T Bn = planck(nu, Th);
+    T ib = Bn > 0.0 ? 1.0 / Bn : T(0.0);
     c.aI = c.jI * ib; c.aQ = c.jQ * ib; c.aV = c.jV * ib;
     return c;
 }
@@ -303,155 +305,199 @@ KS_DEV double pow2_below(double x) {
     int e; frexp(x, &e);
     return ldexp(1.0, e - 1);
 }
-KS_DEV double cap_factor(double I, double Q, double V) {
+template <class T> KS_DEV T cap_factor(T I, T Q, T V) {
     const double fmax = 0.99;
-    if (!(I > 0.0)) return (Q == 0.0 && V == 0.0) ? 1.0 : 0.0;
-    double s = pow2_below(dmax(dmax(fabs(I), fabs(Q)), fabs(V)));
-    double Ih = I / s, q = Q / s, v = V / s;
-    double pp = q * q + v * v;
-    double p = pp > 0.0 ? sqrt(pp) : 0.0;
-    return p > fmax * Ih ? fmax * Ih / p : 1.0;
+    if (!(I > 0.0)) return (val(Q) == 0.0 && val(V) == 0.0) ? T(1.0) : T(0.0);
+    const double s = pow2_below(dmax(dmax(fabs(val(I)), fabs(val(Q))), fabs(val(V))));
+    const double is = 1.0 / s;
+    T Ih = I * is, q = Q * is, v = V * is;
+    T pp = q * q + v * v;
+    T p = pp > 0.0 ? sqrt(pp) : T(0.0);
+    return p > fmax * Ih ? fmax * Ih / p : T(1.0);
 }
 
 // ---- the exact 4x4 step -------------------------------------------------------------------------------------------
-KS_DEV double phi1(double a) {
-    if (fabs(a) < 1e-3) return 1.0 - a * (0.5 - a * (1.0 / 6.0 - a * (1.0 / 24.0 - a / 120.0)));
+template <class T> KS_DEV T phi1(T a) {
+    if (fabs(val(a)) < 1e-3) return 1.0 - a * (0.5 - a * (1.0 / 6.0 - a * (1.0 / 24.0 - a / 120.0)));
     return -expm1(-a) / a;
 }
 // M_n(a) = int_0^1 t^n e^{-at} dt, n = 0..9
-KS_DEV void moments(double a, double M[10]) {
-    if (fabs(a) < 1.0) {
+template <class T> KS_DEV void moments(T a, T M[10]) {
+    const double av = val(a);
+    if (fabs(av) < 1.0) {
         for (int n = 0; n < 10; ++n) {
-            double term = 1.0, sum = 1.0 / (n + 1.0);
-            for (int m = 1; m < 24; ++m) { term *= -a / m; sum += term / (n + m + 1.0); }
+            T term = T(1.0), sum = T(1.0 / (n + 1.0));
+            for (int m = 1; m < 24; ++m) { term = term * (-a / (double)m); sum = sum + term / (n + m + 1.0); }
             M[n] = sum;
         }
-    } else if (a < 40.0) {
-        double e = exp(-a);
+    } else if (av < 40.0) {
+        T e = exp(-a);
         for (int n = 0; n < 10; ++n) {
-            double term = 1.0 / (n + 1.0), sum = term;
+            T term = T(1.0 / (n + 1.0)), sum = term;
             for (int m = 1; m < 200; ++m) {
-                term *= a / (n + m + 1.0);
-                sum += term;
-                if (term < 1e-18 * sum) break;
+                term = term * (a / (n + m + 1.0));
+                sum = sum + term;
+                if (val(term) < 1e-18 * val(sum)) break;
             }
             M[n] = e * sum;
         }
     } else {
-        double e = exp(-a);
+        T e = exp(-a);
         M[0] = (1.0 - e) / a;
-        for (int n = 1; n < 10; ++n) M[n] = (n * M[n - 1] - e) / a;
+        for (int n = 1; n < 10; ++n) M[n] = ((double)n * M[n - 1] - e) / a;
     }
 }
-KS_DEV double sinhc(double b) { return fabs(b) < 0.1 ? 1.0 + b * b / 6.0 * (1.0 + b * b / 20.0 * (1.0 + b * b / 42.0 * (1.0 + b * b / 72.0))) : sinh(b) / b; }
-KS_DEV double sinc_(double b) { return fabs(b) < 0.1 ? 1.0 - b * b / 6.0 * (1.0 - b * b / 20.0 * (1.0 - b * b / 42.0 * (1.0 - b * b / 72.0))) : sin(b) / b; }
+template <class T> KS_DEV T sinhc(T b) {
+    if (fabs(val(b)) < 0.1) { T bb = b * b; return 1.0 + bb / 6.0 * (1.0 + bb / 20.0 * (1.0 + bb / 42.0 * (1.0 + bb / 72.0))); }
+    return sinh(b) / b;
+}
+template <class T> KS_DEV T sinc_(T b) {
+    if (fabs(val(b)) < 0.1) { T bb = b * b; return 1.0 - bb / 6.0 * (1.0 - bb / 20.0 * (1.0 - bb / 42.0 * (1.0 - bb / 72.0))); }
+    return sin(b) / b;
+}
+
+// The step of one sample (step.jl:164-243) as polynomials in the scaled K' (K = alpha_I + K'):
+// O = o0 + o1 K' + o2 K'^2 + o3 K'^3 and Ej = Dp (e0 + e1 K' + e2 K'^2 + e3 K'^3), with
+// K'6 = (aQ, aU, aV, rQ, rU, rV) times Dp = pow2_below(D); D > 0.
+template <class T>
+KS_DEV void step_coefs(const T al[4], const T rh[3], double D, double& Dp, T K6[6], T o[4], T e4[4]) {
+    Dp = pow2_below(D);
+    const double f = D / Dp;
+    T aI = al[0] * Dp;
+    T aQ = al[1] * Dp, aU = al[2] * Dp, aV = al[3] * Dp;
+    T rQ = rh[0] * Dp, rU = rh[1] * Dp, rV = rh[2] * Dp;
+    K6[0] = aQ; K6[1] = aU; K6[2] = aV; K6[3] = rQ; K6[4] = rU; K6[5] = rV;
+    T a2 = aQ * aQ + aU * aU + aV * aV, r2 = rQ * rQ + rU * rU + rV * rV;
+    T a = aI * f;
+    T e = exp(-a);
+    if (val(a2 + r2) * f * f < 1e-200) {
+        T M[10]; moments(a, M);
+        o[0] = e; o[1] = -f * e; o[2] = o[3] = T(0.0);
+        e4[0] = f * phi1(a); e4[1] = -f * f * M[1]; e4[2] = e4[3] = T(0.0);
+        return;
+    }
+    T ar = aQ * rQ + aU * rU + aV * rV;
+    T h = (a2 - r2) * 0.5;
+    T root = sqrt(h * h + ar * ar);
+    T L1sq, L2sq;
+    if (h >= 0.0) { L1sq = h + root; L2sq = L1sq > 0.0 ? ar * ar / L1sq : T(0.0); }
+    else { L2sq = root - h; L1sq = L2sq > 0.0 ? ar * ar / L2sq : T(0.0); }
+    T L1 = L1sq > 0.0 ? sqrt(L1sq) : T(0.0), L2 = L2sq > 0.0 ? sqrt(L2sq) : T(0.0);
+    T b1 = L1 * f, b2 = L2 * f;
+    T Th = L1sq + L2sq;
+    T M[10]; moments(a, M);
+    if (val(Th) * f * f * (1.0 + val(a2 + r2) * f * f) < 1e-17) {
+        o[0] = e; o[1] = -f * e; o[2] = f * f * 0.5 * e; o[3] = -f * f * f / 6.0 * e;
+        e4[0] = f * M[0]; e4[1] = -f * f * M[1]; e4[2] = f * f * f * M[2] * 0.5; e4[3] = -f * f * f * f * M[3] / 6.0;
+        return;
+    }
+    T ech, esh, echm1;
+    if (b1 < 1.0) {
+        T sh = sinh(0.5 * b1);
+        ech = e * cosh(b1); esh = e * sinhc(b1); echm1 = e * 2.0 * sh * sh;
+    } else {
+        T em = exp(-(a - b1)), ep = exp(-(a + b1));
+        ech = 0.5 * (em + ep); esh = (em - ep) / (2.0 * b1); echm1 = 0.5 * (em - 2.0 * e + ep);
+    }
+    T sb2h = sin(0.5 * b2);
+    T versin2 = 2.0 * sb2h * sb2h;
+    T cb2 = cos(b2);
+    T iTh = 1.0 / Th;
+    o[0] = (L2sq * ech + L1sq * e * cb2) * iTh;
+    o[2] = (echm1 + e * versin2) * iTh;
+    T s3 = f * esh, s2 = f * e * sinc_(b2);
+    o[1] = -(s3 * L2sq + s2 * L1sq) * iTh;
+    o[3] = -(s3 - s2) * iTh;
+    T pa = phi1(a);
+    // int_0^1 e^{-at}(cosh bt - 1), (1 - cos bt), sin(bt)/b, (sinh(bt)/b - t)
+    T icm1, ivs, isc, ish;
+    if (b1 < 0.1) {
+        T bb = b1 * b1;
+        icm1 = bb * (M[2] / 2.0 + bb * (M[4] / 24.0 + bb * (M[6] / 720.0 + bb * M[8] / 40320.0)));
+        ish = bb * (M[3] / 6.0 + bb * (M[5] / 120.0 + bb * (M[7] / 5040.0 + bb * M[9] / 362880.0)));
+    } else {
+        T pm = phi1(a - b1), pp = phi1(a + b1);
+        icm1 = 0.5 * (pm + pp) - pa;
+        ish = (pm - pp) / (2.0 * b1) - M[1];
+    }
+    if (b2 < 0.1) {
+        T bb = b2 * b2;
+        ivs = bb * (M[2] / 2.0 - bb * (M[4] / 24.0 - bb * (M[6] / 720.0 - bb * M[8] / 40320.0)));
+        isc = M[1] - bb * (M[3] / 6.0 - bb * (M[5] / 120.0 - bb * (M[7] / 5040.0 - bb * M[9] / 362880.0)));
+    } else {
+        T sb = sin(b2);
+        T den = 1.0 / (a * a + b2 * b2);
+        ivs = pa - (a - e * (a * cb2 - b2 * sb)) * den;
+        isc = (1.0 - e * (a * sb / b2 + cb2)) * den;
+    }
+    T Ich = f * (pa + icm1);
+    T Ic = f * (pa - ivs);
+    T Ish = f * f * (M[1] + ish);
+    T Is = f * f * isc;
+    e4[0] = (L2sq * Ich + L1sq * Ic) * iTh;
+    e4[2] = f * (icm1 + ivs) * iTh;
+    e4[1] = -(Ish * L2sq + Is * L1sq) * iTh;
+    e4[3] = -(Ish - Is) * iTh;
+}
+
+// y = K' v and y = K'^T v with K' = [0 aQ aU aV; aQ 0 rV -rU; aU -rV 0 rQ; aV rU -rQ 0]
+template <class T, class U, class R>
+KS_DEV void kprime_mul(const T K6[6], const U v[4], R y[4]) {
+    const T &aQ = K6[0], &aU = K6[1], &aV = K6[2], &rQ = K6[3], &rU = K6[4], &rV = K6[5];
+    y[0] = aQ * v[1] + aU * v[2] + aV * v[3];
+    y[1] = aQ * v[0] + rV * v[2] - rU * v[3];
+    y[2] = aU * v[0] - rV * v[1] + rQ * v[3];
+    y[3] = aV * v[0] + rU * v[1] - rQ * v[2];
+}
+template <class T, class U, class R>
+KS_DEV void kprime_tmul(const T K6[6], const U v[4], R y[4]) {
+    const T &aQ = K6[0], &aU = K6[1], &aV = K6[2], &rQ = K6[3], &rU = K6[4], &rV = K6[5];
+    y[0] = aQ * v[1] + aU * v[2] + aV * v[3];
+    y[1] = aQ * v[0] - rV * v[2] + rU * v[3];
+    y[2] = aU * v[0] + rV * v[1] - rQ * v[3];
+    y[3] = aV * v[0] - rU * v[1] + rQ * v[2];
+}
 
 // O and E = Ej j for K = alpha_I + K' (step.jl:164-243), D > 0. Writes O row-major.
 KS_DEV void transfer_step(const double j4[4], const double al[4], const double rh[3], double D, double O[16], double E[4]) {
-    double Dp = pow2_below(D);
-    double f = D / Dp;
-    double aI = al[0] * Dp, aQ = al[1] * Dp, aU = al[2] * Dp, aV = al[3] * Dp;
-    double rQ = rh[0] * Dp, rU = rh[1] * Dp, rV = rh[2] * Dp;
-    double K[16] = {0.0, aQ, aU, aV, aQ, 0.0, rV, -rU, aU, -rV, 0.0, rQ, aV, rU, -rQ, 0.0};
-    double a2 = aQ * aQ + aU * aU + aV * aV, r2 = rQ * rQ + rU * rU + rV * rV;
-    double a = aI * f;
-    double e = exp(-a);
-    // coefficients of O = o0 1 + o1 K + o2 K^2 + o3 K^3 and Ej = e0 1 + e1 K + e2 K^2 + e3 K^3 (times Dp)
-    double o0, o1, o2, o3, e0, e1, e2, e3;
+    double Dp, K6[6], o[4], e4[4];
+    step_coefs(al, rh, D, Dp, K6, o, e4);
+    const double aQ = K6[0], aU = K6[1], aV = K6[2], rQ = K6[3], rU = K6[4], rV = K6[5];
+    const double K[16] = {0.0, aQ, aU, aV, aQ, 0.0, rV, -rU, aU, -rV, 0.0, rQ, aV, rU, -rQ, 0.0};
     double K2[16], K3[16];
-    bool cubic = false;
-    if ((a2 + r2) * f * f < 1e-200) {
-        double M[10]; moments(a, M);
-        o0 = e; o1 = -f * e; o2 = o3 = 0.0;
-        e0 = f * phi1(a); e1 = -f * f * M[1]; e2 = e3 = 0.0;
-    } else {
-        cubic = true;
-        double ar = aQ * rQ + aU * rU + aV * rV;
-        double h = (a2 - r2) * 0.5;
-        double root = sqrt(h * h + ar * ar);
-        double L1sq, L2sq;
-        if (h >= 0.0) { L1sq = h + root; L2sq = L1sq > 0.0 ? ar * ar / L1sq : 0.0; }
-        else { L2sq = root - h; L1sq = L2sq > 0.0 ? ar * ar / L2sq : 0.0; }
-        double L1 = L1sq > 0.0 ? sqrt(L1sq) : 0.0, L2 = L2sq > 0.0 ? sqrt(L2sq) : 0.0;
-        double b1 = L1 * f, b2 = L2 * f;
-        double Th = L1sq + L2sq;
-        double M[10]; moments(a, M);
-        if (Th * f * f * (1.0 + (a2 + r2) * f * f) < 1e-17) {
-            o0 = e; o1 = -f * e; o2 = f * f * 0.5 * e; o3 = -f * f * f / 6.0 * e;
-            e0 = f * M[0]; e1 = -f * f * M[1]; e2 = f * f * f * M[2] * 0.5; e3 = -f * f * f * f * M[3] / 6.0;
-        } else {
-            double ech, esh, echm1;
-            if (b1 < 1.0) {
-                ech = e * cosh(b1); esh = e * sinhc(b1); double sh = sinh(0.5 * b1); echm1 = e * 2.0 * sh * sh;
-            } else {
-                double em = exp(-(a - b1)), ep = exp(-(a + b1));
-                ech = 0.5 * (em + ep); esh = (em - ep) / (2.0 * b1); echm1 = 0.5 * (em - 2.0 * e + ep);
-            }
-            double sb2h = sin(0.5 * b2);
-            double versin2 = 2.0 * sb2h * sb2h;
-            double cb2 = cos(b2);
-            double iTh = 1.0 / Th;
-            o0 = (L2sq * ech + L1sq * e * cb2) * iTh;
-            o2 = (echm1 + e * versin2) * iTh;
-            double s3 = f * esh, s2 = f * e * sinc_(b2);
-            o1 = -(s3 * L2sq + s2 * L1sq) * iTh;
-            o3 = -(s3 - s2) * iTh;
-            double pa = phi1(a);
-            // int_0^1 e^{-at}(cosh bt - 1), (1 - cos bt), sin(bt)/b, (sinh(bt)/b - t)
-            double icm1, ivs, isc, ish;
-            if (b1 < 0.1) {
-                double bb = b1 * b1;
-                icm1 = bb * (M[2] / 2.0 + bb * (M[4] / 24.0 + bb * (M[6] / 720.0 + bb * M[8] / 40320.0)));
-                ish = bb * (M[3] / 6.0 + bb * (M[5] / 120.0 + bb * (M[7] / 5040.0 + bb * M[9] / 362880.0)));
-            } else {
-                double pm = phi1(a - b1), pp = phi1(a + b1);
-                icm1 = 0.5 * (pm + pp) - pa;
-                ish = (pm - pp) / (2.0 * b1) - M[1];
-            }
-            if (b2 < 0.1) {
-                double bb = b2 * b2;
-                ivs = bb * (M[2] / 2.0 - bb * (M[4] / 24.0 - bb * (M[6] / 720.0 - bb * M[8] / 40320.0)));
-                isc = M[1] - bb * (M[3] / 6.0 - bb * (M[5] / 120.0 - bb * (M[7] / 5040.0 - bb * M[9] / 362880.0)));
-            } else {
-                double sb = sin(b2);
-                double den = 1.0 / (a * a + b2 * b2);
-                ivs = pa - (a - e * (a * cb2 - b2 * sb)) * den;
-                isc = (1.0 - e * (a * sb / b2 + cb2)) * den;
-            }
-            double Ich = f * (pa + icm1);
-            double Ic = f * (pa - ivs);
-            double Ish = f * f * (M[1] + ish);
-            double Is = f * f * isc;
-            e0 = (L2sq * Ich + L1sq * Ic) * iTh;
-            e2 = f * (icm1 + ivs) * iTh;
-            e1 = -(Ish * L2sq + Is * L1sq) * iTh;
-            e3 = -(Ish - Is) * iTh;
-        }
+    for (int i = 0; i < 4; ++i) for (int k = 0; k < 4; ++k) {
+        double s = 0.0;
+        for (int q = 0; q < 4; ++q) s += K[4 * i + q] * K[4 * q + k];
+        K2[4 * i + k] = s;
     }
-    if (cubic) {
-        for (int i = 0; i < 4; ++i) for (int k = 0; k < 4; ++k) {
-            double s = 0.0;
-            for (int q = 0; q < 4; ++q) s += K[4 * i + q] * K[4 * q + k];
-            K2[4 * i + k] = s;
-        }
-        for (int i = 0; i < 4; ++i) for (int k = 0; k < 4; ++k) {
-            double s = 0.0;
-            for (int q = 0; q < 4; ++q) s += K2[4 * i + q] * K[4 * q + k];
-            K3[4 * i + k] = s;
-        }
-    } else {
-        for (int i = 0; i < 16; ++i) { K2[i] = 0.0; K3[i] = 0.0; }
+    for (int i = 0; i < 4; ++i) for (int k = 0; k < 4; ++k) {
+        double s = 0.0;
+        for (int q = 0; q < 4; ++q) s += K2[4 * i + q] * K[4 * q + k];
+        K3[4 * i + k] = s;
     }
     for (int i = 0; i < 4; ++i) {
         double ei = 0.0;
         for (int k = 0; k < 4; ++k) {
             double id = i == k ? 1.0 : 0.0;
             int ix = 4 * i + k;
-            O[ix] = o0 * id + o1 * K[ix] + o2 * K2[ix] + o3 * K3[ix];
-            double ej = e0 * id + e1 * K[ix] + e2 * K2[ix] + e3 * K3[ix];
+            O[ix] = o[0] * id + o[1] * K[ix] + o[2] * K2[ix] + o[3] * K3[ix];
+            double ej = e4[0] * id + e4[1] * K[ix] + e4[2] * K2[ix] + e4[3] * K3[ix];
             ei += ej * j4[k];
         }
         E[i] = Dp * ei;
+    }
+}
+
+// T_out = E + O T_in for one sample from its polynomial coefficients (the tails recursion,
+// dual_gradient.jl:129-147), by repeated products with K' instead of the 4x4 matrices
+KS_DEV void step_apply(const double j4[4], const double al[4], const double rh[3], double D, const double Tin[4], double Tout[4]) {
+    double Dp, K6[6], o[4], e4[4];
+    step_coefs(al, rh, D, Dp, K6, o, e4);
+    double v[4], u[4], y[4], z[4];
+    for (int i = 0; i < 4; ++i) { v[i] = Tin[i]; u[i] = j4[i]; Tout[i] = o[0] * Tin[i] + Dp * e4[0] * j4[i]; }
+    for (int n = 1; n < 4; ++n) {
+        kprime_mul(K6, v, y); kprime_mul(K6, u, z);
+        for (int i = 0; i < 4; ++i) { v[i] = y[i]; u[i] = z[i]; Tout[i] += o[n] * v[i] + Dp * e4[n] * u[i]; }
     }
 }
 

@@ -123,9 +123,19 @@ KS_DEV int anchor_interval(int M, double one_minus_k) {
     return r > 4 ? r : 4;
 }
 
-// March one ray; f(k, Samp, dtau) is called for k = 1..N. Returns the largest anchor residuals.
+// Samples kbeg..kend of one ray; f(k, Samp, dtau) is called for those k only. The largest anchor
+// residuals are taken over the anchors k with kbeg < k <= kend + 1 (so that the segments of a ray
+// [1, L], [L+1, 2L], ... together cover every anchor exactly once).
+//
+// A ray can be cut at any anchor: there the Jacobi states are reset to the closed forms, and t~, phi
+// too whenever the anchor sample is a valid Krang sample (quadrature.jl:283-291). The march state at
+// such an anchor is a function of the pixel alone, so a march started there reproduces the march of
+// the whole ray. kbeg must be 1 or an anchor (kbeg - 1 a multiple of the anchor interval); when the
+// anchor kbeg is not a valid sample, t~ and phi carry the quadrature from the last valid anchor
+// before it, and the march starts from that anchor (or from sample 1, always a closed-form start)
+// and runs without calling f up to kbeg.
 template <class F>
-KS_DEV void march_ray(const PixC& p, const Spin& S, double thetao, int N, int M, F& f, double& resid_t, double& resid_ph) {
+KS_DEV void march_range(const PixC& p, const Spin& S, double thetao, int N, int M, int kbeg, int kend, F& f, double& resid_t, double& resid_ph) {
     const QC q = make_qc(p, S);
     const VisC vis = vis_constants(p, S.a, thetao);
     const double dtau = p.tau_total / (double)(N + 1);
@@ -133,22 +143,34 @@ KS_DEV void march_ray(const PixC& p, const Spin& S, double thetao, int N, int M,
     const int Man = anchor_interval(M, 1.0 - p.k);
     const JS Dr = js_step_constants(-p.c * h, p.k);
     const JS Dt = js_step_constants(p.scale * h / p.f, p.mu);
-    JS xr = radial_state(p, dtau);
-    JS xt = polar_state(p, dtau);
-    RP pa = ray_point(p, S, q, xr, xt, dtau);
-    double t, ph;
-    t_phi_direct(p, S, dtau, t, ph);
     resid_t = 0.0; resid_ph = 0.0;
-    {
-        Samp s{t, pa.r, pa.th, ph, pa.nur, pa.nuth, (dtau <= p.tau_h) && krang_visible(vis, pa.cth)};
-        f(1, s, dtau);
+    // start: the last anchor A <= kbeg whose sample is valid, or sample 1
+    int A = kbeg;
+    JS xr, xt;
+    RP pa;
+    for (;;) {
+        const double ta = (double)A * dtau;
+        xr = radial_state(p, ta);
+        xt = polar_state(p, ta);
+        pa = ray_point(p, S, q, xr, xt, ta);
+        if (A <= 1 || ((ta <= p.tau_h) && krang_visible(vis, pa.cth))) break;
+        A = A - Man > 1 ? A - Man : 1;
     }
-    for (int k = 2; k <= N; ++k) {
+    double t, ph;
+    t_phi_direct(p, S, (double)A * dtau, t, ph);
+    if (A == kbeg) {
+        const double ta = (double)A * dtau;
+        Samp s{t, pa.r, pa.th, ph, pa.nur, pa.nuth, (ta <= p.tau_h) && krang_visible(vis, pa.cth)};
+        f(A, s, dtau);
+    }
+    const int klast = kend < N ? kend + 1 : N;
+    for (int k = A + 1; k <= klast; ++k) {
         const double ta = (double)(k - 1) * dtau, tb = (double)k * dtau;
+        const bool anchor = ((k - 1) % Man) == 0;
+        if (k > kend && !anchor) break;            // the step past kend only serves the next anchor's residual
         xr = js_step(xr, Dr, p.k);
         xt = js_step(xt, Dt, p.mu);
         RP pmid = ray_point(p, S, q, xr, xt, ta + h);
-        const bool anchor = ((k - 1) % Man) == 0;
         if (anchor) { xr = radial_state(p, tb); xt = polar_state(p, tb); }
         else { xr = js_step(xr, Dr, p.k); xt = js_step(xt, Dt, p.mu); }
         RP pb = ray_point(p, S, q, xr, xt, tb);
@@ -168,21 +190,38 @@ KS_DEV void march_ray(const PixC& p, const Spin& S, double thetao, int N, int M,
             t += It;
             ph += Ip;
         }
-        if (anchor) {
+        const bool okb = (tb <= p.tau_h) && krang_visible(vis, pb.cth);
+        if (anchor && okb) {
             // closed-form anchor; Krang's ok for the sample decides whether it resets (quadrature.jl:283-291)
             double td, pd;
             t_phi_direct(p, S, tb, td, pd);
-            bool okd = (tb <= p.tau_h) && krang_visible(vis, pb.cth);
-            if (okd) {
+            if (k > kbeg) {
                 resid_t = dmax(resid_t, fabs(t - td));
                 resid_ph = dmax(resid_ph, fabs(ph - pd));
-                t = td; ph = pd;
             }
+            t = td; ph = pd;
         }
-        Samp s{t, pb.r, pb.th, ph, pb.nur, pb.nuth, (tb <= p.tau_h) && krang_visible(vis, pb.cth)};
-        f(k, s, dtau);
+        if (k >= kbeg && k <= kend) {
+            Samp s{t, pb.r, pb.th, ph, pb.nur, pb.nuth, okb};
+            f(k, s, dtau);
+        }
         pa = pb;
     }
 }
+
+// March one whole ray (samples 1..N). Returns the largest anchor residuals.
+template <class F>
+KS_DEV void march_ray(const PixC& p, const Spin& S, double thetao, int N, int M, F& f, double& resid_t, double& resid_ph) {
+    march_range(p, S, thetao, N, M, 1, N, f, resid_t, resid_ph);
+}
+
+// Segments of a ray for parallel marching: whole multiples of the anchor interval, at least SEG_MIN
+// samples (at most 2 SEG_MIN - 1, or the anchor interval itself when that is longer).
+constexpr int SEG_MIN = 32;
+KS_DEV int segment_length(int M, double one_minus_k) {
+    const int Man = anchor_interval(M, one_minus_k);
+    return Man * ((SEG_MIN + Man - 1) / Man);
+}
+__host__ __device__ inline int max_segments(int N) { return (N + SEG_MIN - 1) / SEG_MIN; }
 
 }  // namespace ksd
