@@ -1,13 +1,16 @@
-"""Benchmark: ray-samples/s of the fused geodesic march + polarized transfer on one MI355X.
+"""Benchmark: ray-samples/s of the geodesic march + polarized transfer + dual-sweep adjoint on one MI355X.
 
 Workload (the configuration the headline metric is quoted on that fits one GPU): a = 0.5,
 theta_o = 60 deg, 128 x 128 screen over +-15 M, 1024 Mino-time samples per ray, 32 thermal splats,
-full IQUV at 230 GHz, visibilities on 512 uv points (seeded synthetic draws). One step = one
-polarized render of the screen through the fused Recurrence(64) march (K2 + transfer) plus the
-visibilities of the image, with the per-pixel constants (K0/K1) regenerated once beforehand and
-the splat parameters resident on the device.
+full IQUV at 230 GHz, visibilities on 512 uv points (seeded synthetic draws). The per-pixel
+constants (K0/K1) are built once beforehand and the splat parameters are resident on the device.
+One fwd+adjoint step = the Recurrence(64) march of every ray into sample storage, the polarized
+transfer with the adjoint records, the dual sweep (the gradient of sum(dstokes * I) for a fixed
+seeded dstokes with respect to all 21 x 32 splat parameters) and the visibilities of the image.
+The forward-only step (march + transfer + visibilities) is timed beside it.
 
-Prints one JSON line; value = ray-samples per second (pixels x samples x frames x frequencies / s).
+Prints one JSON line; value = ray-samples per second (pixels x samples x frames x frequencies / s)
+of the fwd+adjoint step.
 """
 import argparse
 import json
@@ -23,6 +26,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "ray-samples/s fwd+adjoint, mixed FP32 vs plain FP64 on 1 GPU @ ≤1e-4 Stokes err"
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 
 
 def parse():
@@ -36,26 +40,28 @@ def parse():
     return p.parse_args()
 
 
-def cpu_baseline(cfg, P, L, target_s):
-    """The oracle (FP64 C++ restatement of the reference's per-ray CPU path, OpenMP over all host
-    cores) on a bounded, strided sample of the same rays; ray-samples/s."""
+def cpu_baseline(cfg, P, L, target_s, ds):
+    """The oracle (FP64 C++ restatement of the reference's per-ray CPU path: Recurrence(64) march,
+    transfer and dual sweep, OpenMP over all host cores) on a bounded random subset of the same rays
+    with the same seeds; fwd+adjoint ray-samples/s."""
     import oracle
     a, th, res, N = cfg["a"], math.radians(cfg["theta_o_deg"]), cfg["res"], cfg["N"]
     from kerrsplat_amd import Camera
     cam = Camera((-cfg["fov"], cfg["fov"]), (-cfg["fov"], cfg["fov"]), res)
+    dso = ds[:, :, 0, 0, :].reshape(cam.npix, 4, order="F")
     threads = oracle.num_threads()
     rng = np.random.default_rng(7)
     order = rng.permutation(cam.npix)
     n = 64
     t0 = time.perf_counter()
-    oracle.polarized_image(a, th, cam.alphas[order[:n]], cam.betas[order[:n]], N, P, 0.0, cfg["nus"][0], L)
+    oracle.polarized_gradient(a, th, cam.alphas[order[:n]], cam.betas[order[:n]], N, P, 0.0, cfg["nus"][0], L, dso[order[:n]])
     dt = time.perf_counter() - t0
     n2 = int(min(cam.npix, max(n, n * target_s / max(dt, 1e-3))))
     t0 = time.perf_counter()
-    oracle.polarized_image(a, th, cam.alphas[order[:n2]], cam.betas[order[:n2]], N, P, 0.0, cfg["nus"][0], L)
+    oracle.polarized_gradient(a, th, cam.alphas[order[:n2]], cam.betas[order[:n2]], N, P, 0.0, cfg["nus"][0], L, dso[order[:n2]])
     dt2 = time.perf_counter() - t0
     return {"value": n2 * N / dt2, "unit": "ray-samples/s", "cores": threads, "kind": "port",
-            "sample": f"{n2} of {cam.npix} rays (random subset) x {N} samples, FP64 oracle, {dt2:.1f} s"}
+            "sample": f"{n2} of {cam.npix} rays (random subset) x {N} samples, fwd+adjoint (dual sweep), FP64 oracle, {dt2:.1f} s"}
 
 
 def main():
@@ -72,59 +78,114 @@ def main():
     cam = K.Camera((-cfg["fov"], cfg["fov"]), (-cfg["fov"], cfg["fov"]), res)
     L = syn.length_unit()
     P = syn.thermal_splats(args.config, a, cfg["ns"])
-    cache = K.GeodesicCache(cam, N, store_samples=False)
-    K.regenerate(cache, a, th, marcher=K.Fused(64))
+    cache = K.GeodesicCache(cam, N, store_samples=True)
+    K.regenerate(cache, a, th, marcher=K.Recurrence(64))
     model = K.Model(P, kind="thermal")
+    ds = np.random.default_rng(20261202).normal(size=(res, res, len(cfg["times"]), len(cfg["nus"]), 4))
     u, v = syn.uv_disc(args.config, cfg["nuv"])
     psize = syn.pixel_size_rad(-cfg["fov"], cfg["fov"], res)
     lib, ctx = api._load(), api.context()
     times, nus = cfg["times"], cfg["nus"]
 
-    def step():
+    def forward_step():
+        K.march(cache)
         K.render_device(cache, model, times, nus, L)
         if cfg["nuv"]:
             K.visibilities(cache, 1, psize, u, v)
 
-    for _ in range(args.warmup):
-        step()
-    lib.ks_ctx_sync(ctx)
-    # the timed region: K steps, synchronized on both sides; the dominant kernel's duration per step
-    # (the segment march + transfer, ks_seg_polarized) from the HIP events the library records around
-    # its launch on the context's stream, read after the timed region
-    wall0 = time.perf_counter()
-    kms = []
-    for _ in range(args.steps):
-        step()
-        ms = api.ctypes.c_float()
-        if lib.ks_last_kernel_ms(ctx, api.ctypes.byref(ms)) == 0:
-            kms.append(ms.value)
-    lib.ks_ctx_sync(ctx)
-    wall = time.perf_counter() - wall0
-    ms_per_step = 1e3 * wall / args.steps
-    work = cam.npix * N * len(times) * len(nus)
-    value = work / (wall / args.steps)
-    kernel_ms = float(np.mean(kms))
+    def adjoint_step():
+        K.march(cache)
+        K.polarized_gradient(cache, model, times, nus, L, ds)
+        if cfg["nuv"]:
+            K.visibilities(cache, 1, psize, u, v)
 
-    roofline = None
+    def timed(step, stages):
+        """W untimed warmups, then K steps bracketed by stream synchronizations; per step, each listed
+        stage's kernel duration from the HIP events the library records around its launch."""
+        for _ in range(args.warmup):
+            step()
+        lib.ks_ctx_sync(ctx)
+        ms = {k: [] for k in stages}
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+            for k in stages:
+                ms[k].append(api.stage_ms(k))
+        lib.ks_ctx_sync(ctx)
+        return (time.perf_counter() - t0) / args.steps, {k: float(np.mean(v)) for k, v in ms.items()}
+
+    STG_FWD, STG_ADJ = ["march", "transfer"], ["march", "transfer", "adjoint_steps", "adjoint_elements"]
+    work = cam.npix * N * len(times) * len(nus)
+    runs = {}
+    for prec in ("fp64", "mixed"):
+        K.precision(cache, prec)
+        fwd_s, fwd_ms = timed(forward_step, STG_FWD)
+        adj_s, adj_ms = timed(adjoint_step, STG_ADJ)
+        g, cube = K.polarized_gradient(cache, model, times, nus, L, ds)
+        runs[prec] = {"fwd_s": fwd_s, "fwd_ms": fwd_ms, "adj_s": adj_s, "adj_ms": adj_ms, "grad": g.copy(), "cube": cube}
+    img_err = float(np.abs(runs["mixed"]["cube"] - runs["fp64"]["cube"]).max() / np.abs(runs["fp64"]["cube"][..., 0]).max())
+    grad_err = float(np.abs(runs["mixed"]["grad"] - runs["fp64"]["grad"]).max() / np.abs(runs["fp64"]["grad"]).max())
+    head = "mixed" if img_err <= 1e-4 else "fp64"        # the mixed path counts only within its error bar
+    adj_s, adj_ms = runs[head]["adj_s"], runs[head]["adj_ms"]
+    ms_per_step = 1e3 * adj_s
+    value = work / adj_s
+
+    # per-stage rooflines from the algorithmic work model (oracle on the Counted scalar): VALU bound,
+    # against the FP64 vector peak on the FP64 path and the FP32 vector peak on the mixed path
     wm_path = os.path.join(ROOT, "profiles", "work_model.json")
-    if os.path.exists(wm_path):
-        wm = json.load(open(wm_path)).get(cfg["name"])
-        if wm:
-            flops = wm["flops_per_ray_sample"] * work
-            achieved = flops / (kernel_ms * 1e-3) / 1e12
-            roofline = {"bound": "valu-fp64", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                        "frac": achieved / FP64_PEAK_TFLOPS, "traffic": wm.get("hbm_bytes_per_launch"),
-                        "kernel": "ks_seg_polarized", "kernel_ms": kernel_ms,
-                        "flops_per_ray_sample": wm["flops_per_ray_sample"]}
+    wm = json.load(open(wm_path)).get(cfg["name"]) if os.path.exists(wm_path) else None
+    kernel_of = {"march": "ks_march_store", "transfer": "ks_chunk_transfer", "adjoint_steps": "ks_adj_segment",
+                 "adjoint_elements": "ks_adj_splats"}
+
+    def stage_table(prec):
+        peak = FP64_PEAK_TFLOPS if prec == "fp64" else FP32_PEAK_TFLOPS
+        out = {}
+        for k, t_ms in runs[prec]["adj_ms"].items():
+            st = {"kernel": kernel_of[k], "ms": t_ms}
+            if wm and "phases" in wm:
+                ph = wm["phases"]
+                per = {"march": ph["march"]["flops_per_ray_sample"], "transfer": ph["transfer"]["flops_per_ray_sample"],
+                       "adjoint_steps": ph["adjoint_steps"]["flops_per_ray_sample"] + ph["tails"]["flops_per_ray_sample"],
+                       "adjoint_elements": ph["adjoint_elements"]["flops_per_ray_sample"]}[k]
+                units = cam.npix * N if k == "march" else work
+                achieved = per * units / (t_ms * 1e-3) / 1e12
+                pk = FP64_PEAK_TFLOPS if k == "march" else peak     # the geodesics are FP64 on both paths
+                st.update({"flops_per_ray_sample": per, "achieved_tflops": achieved, "peak_tflops": pk, "frac": achieved / pk})
+            out[k] = st
+        return out
+
+    stages = {p_: stage_table(p_) for p_ in runs}
+    roofline = None
+    dom = max(adj_ms, key=adj_ms.get)
+    d = stages[head][dom]
+    if "achieved_tflops" in d:
+        roofline = {"bound": "valu-fp64" if (head == "fp64" or dom == "march") else "valu-fp32", "achieved": d["achieved_tflops"],
+                    "peak": d["peak_tflops"], "unit": "TFLOP/s", "frac": d["frac"], "traffic": None, "kernel": d["kernel"],
+                    "kernel_ms": d["ms"], "flops_per_ray_sample": d["flops_per_ray_sample"]}
+    paths = {p_: {"fwd_adjoint": {"value": work / r["adj_s"], "unit": "ray-samples/s", "ms_per_step": 1e3 * r["adj_s"],
+                                  "gradient_evals_per_s": 1.0 / r["adj_s"], "stage_ms": r["adj_ms"]},
+                  "forward": {"value": work / r["fwd_s"], "unit": "ray-samples/s", "ms_per_step": 1e3 * r["fwd_s"], "stage_ms": r["fwd_ms"]}}
+             for p_, r in runs.items()}
+
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(cfg, P, L, args.cpu_seconds)
+        cpu = cpu_baseline(cfg, P, L, args.cpu_seconds, ds)
     out = {"metric": METRIC, "value": value, "unit": "ray-samples/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded splat draws and uv disc)",
+           "vs_baseline": None, "dtype": "mixed f32/f64" if head == "mixed" else "f64",
+           "data": "synthetic (seeded splat draws and uv disc)",
            "config": {"workload": cfg["name"], "a": a, "theta_o_deg": cfg["theta_o_deg"], "screen": [res, res],
                       "samples_per_ray": N, "splats": cfg["ns"], "kind": cfg["kind"], "frames": len(times),
-                      "frequencies": len(nus), "uv_points": cfg["nuv"], "path": "plain FP64 forward (Recurrence(64) fused, anchor-segment parallel)"},
+                      "frequencies": len(nus), "uv_points": cfg["nuv"],
+                      "path": ("mixed: FP64 geodesics (Recurrence(64) into stored samples, anchor segments in parallel), "
+                               "FP32 transfer and dual-sweep adjoint in 32-sample chunks (FP64 step above Faraday depth 10), visibilities")
+                              if head == "mixed" else
+                              "plain FP64: Recurrence(64) march into stored samples (anchor segments in parallel), "
+                              "transfer in 32-sample chunks, dual-sweep adjoint, visibilities"},
+           "mixed_over_fp64": paths["fp64"]["fwd_adjoint"]["ms_per_step"] / paths["mixed"]["fwd_adjoint"]["ms_per_step"],
+           "mixed_error_vs_fp64": {"stokes_over_peak_I": img_err, "gradient_over_max": grad_err},
+           "paths": paths, "gradient_evals_per_s": 1.0 / adj_s,
+           "stages": stages,
            "roofline": roofline, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 

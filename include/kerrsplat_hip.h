@@ -46,10 +46,16 @@ void* ks_ctx_stream(ks_ctx* ctx);                 /* the hipStream_t every launc
 /* HIP events on the context's stream around a timed region (no reference counterpart; bench.py). */
 int ks_timing_begin(ks_ctx* ctx);
 int ks_timing_end(ks_ctx* ctx, float* ms);
-/* Duration of the last launch of the dominant kernel (the segment march + transfer, or the adjoint's
- * per-splat sweep), bracketed by HIP events on the context's stream; synchronizes. No reference
+/* Duration of the last launch of a pipeline stage on this context, bracketed by HIP events on its
+ * stream; synchronizes. Stages: KS_STAGE_TRANSFER (the polarized transfer kernel of a render or of
+ * the gradient's forward pass), KS_STAGE_ADJ_STEPS (the per-sample step adjoints), KS_STAGE_ADJ_ELEMENTS
+ * (the per-splat element adjoints), KS_STAGE_MARCH (the stored-sample march). No reference
  * counterpart (measurement hook for bench.py). */
-int ks_last_kernel_ms(ks_ctx* ctx, float* ms);
+#define KS_STAGE_TRANSFER 0
+#define KS_STAGE_ADJ_STEPS 1
+#define KS_STAGE_ADJ_ELEMENTS 2
+#define KS_STAGE_MARCH 3
+int ks_stage_ms(ks_ctx* ctx, int stage, float* ms);
 
 /* GeodesicCache(backend, camera, Val(N); store_samples) - cache.jl:106-125. */
 int ks_cache_create(ks_ctx* ctx, int64_t npix, int N, int store_samples, ks_cache** cache);
@@ -66,6 +72,19 @@ int ks_regenerate(ks_cache* cache, const double* alpha, const double* beta, int 
 
 /* cache.perm_host (1-based, perm[j] = screen index of sorted slot j) and the case range lengths
  * (case2, case3, case4) - cache.jl:94-95, pixel_constants.jl:167-176. */
+/* precision(cache, T2) - cache.jl:212-232: the scalar of the transfer and the dual sweep over the
+ * cache's stored samples. KS_PRECISION_FP64 (default): everything FP64. KS_PRECISION_MIXED: the
+ * geodesics stay FP64 (marched and stored in FP64, the per-sample geometry built in FP64); the splat
+ * sums, frames, coefficients, the 4x4 step (FP64 where the Faraday depth exceeds 10) and the adjoint
+ * run in FP32 with compensated Stokes sums. Changes the cache in place (the reference returns a
+ * converted copy). Caches without stored samples always run FP64. */
+#define KS_PRECISION_FP64 0
+#define KS_PRECISION_MIXED 1
+int ks_cache_precision(ks_cache* cache, int precision);
+
+/* march!(marcher, cache, ...) - cache.jl:182-190: re-run the cache's Direct or Recurrence(M) marcher
+ * into its sample storage on the current pixel constants (KS_ERR_STATE without stored samples). */
+int ks_march(ks_cache* cache);
 int ks_get_perm(const ks_cache* cache, int64_t* perm, int64_t counts[3]);
 /* cache.residual_t, cache.residual_ϕ (sorted order) - cache.jl:98-99, quadrature.jl:283-291. */
 int ks_get_residuals(const ks_cache* cache, double* resid_t, double* resid_phi);

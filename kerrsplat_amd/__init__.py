@@ -9,16 +9,17 @@ C-ABI of libkerrsplat_hip.so (include/kerrsplat_hip.h):
     img   = thin_image(cache, params, t_obs)                       # screen-shaped Stokes I
     st    = polarized_image(cache, params, t_obs, nu, L)          # (n_alpha, n_beta, 4)
     cube  = polarized_cube(cache, params, times, nus, L)          # (n_alpha, n_beta, nt, nnu, 4)
+    dp, cube = polarized_gradient(cache, params, times, nus, L, dstokes)   # dual sweep (stored samples)
 
 Every call goes to the GPU library; there is no CPU fallback. Loading fails loudly when the library
 is missing or no GPU is present.
 """
 from .api import (Camera, Direct, Recurrence, Fused, GeodesicCache, Model, regenerate, thin_image, polarized_image,
-                  polarized_cube, render_device, visibilities, unsort, to_screen, samples, residuals, KernelError,
+                  polarized_cube, polarized_gradient, march, precision, stage_ms, render_device, visibilities, unsort, to_screen, samples, residuals, KernelError,
                   NSPLATPARAMS, NPOLARIZEDPARAMS, NPOWERLAWPARAMS, NKAPPAPARAMS, gravitational_radius, grid_axis,
                   library_path)
 
 __all__ = ["Camera", "Direct", "Recurrence", "Fused", "GeodesicCache", "Model", "regenerate", "thin_image",
-           "polarized_image", "polarized_cube", "render_device", "visibilities", "unsort", "to_screen", "samples",
+           "polarized_image", "polarized_cube", "polarized_gradient", "march", "precision", "stage_ms", "render_device", "visibilities", "unsort", "to_screen", "samples",
            "residuals", "KernelError", "NSPLATPARAMS", "NPOLARIZEDPARAMS", "NPOWERLAWPARAMS", "NKAPPAPARAMS",
            "gravitational_radius", "grid_axis", "library_path"]

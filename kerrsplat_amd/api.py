@@ -42,7 +42,8 @@ class StateError(ValueError):
 
 
 def library_path():
-    return os.path.join(_HERE, _LIB_NAME)
+    """The in-tree HIP library; KERRSPLAT_HIP_LIB names another build of it (to compare builds)."""
+    return os.environ.get("KERRSPLAT_HIP_LIB") or os.path.join(_HERE, _LIB_NAME)
 
 
 _lib = None
@@ -73,7 +74,10 @@ def _load():
     lib.ks_thin_image.argtypes = [_P, _P, _d, _P, _i]
     lib.ks_visibilities.argtypes = [_P, _i64, _d, _P, _P, _P, _i, _P]
     lib.ks_timing_end.argtypes = [_P, ctypes.POINTER(ctypes.c_float)]
-    lib.ks_last_kernel_ms.argtypes = [_P, ctypes.POINTER(ctypes.c_float)]
+    lib.ks_stage_ms.argtypes = [_P, _i, ctypes.POINTER(ctypes.c_float)]
+    lib.ks_polarized_gradient.argtypes = [_P, _P, _P, _i, _P, _i, _d, _i, _d, _P, _i, _P, _P]
+    lib.ks_march.argtypes = [_P]
+    lib.ks_cache_precision.argtypes = [_P, _i]
     _lib = lib
     return lib
 
@@ -188,6 +192,7 @@ class GeodesicCache:
         self.marcher = Direct()
         self.perm_host = np.arange(1, camera.npix + 1)
         self.ranges = (0, 0, 0)
+        self.precision = "fp64"
 
     def __del__(self):
         try:
@@ -333,6 +338,72 @@ def polarized_image(cache, params, t_obs, nu, L, nmax=-1, slab=0.0, kind="therma
     (n_alpha, n_beta, 4) observed Stokes (or (npix, 4) in sorted order with screen=False)."""
     c = polarized_cube(cache, params, [t_obs], [nu], L, nmax=nmax, slab=slab, kind=kind, screen=screen)
     return c[:, :, 0, 0, :] if screen else c[0, 0]
+
+
+STAGES = {"transfer": 0, "adjoint_steps": 1, "adjoint_elements": 2, "march": 3}
+
+
+def stage_ms(stage, device=0):
+    """Duration [ms] of the last launch of a pipeline stage ("transfer", "adjoint_steps",
+    "adjoint_elements", "march") on the device's context, from HIP events on its stream."""
+    ms = ctypes.c_float()
+    _check(_load().ks_stage_ms(context(device), _i(STAGES[stage]), ctypes.byref(ms)))
+    return ms.value
+
+
+PRECISIONS = {"fp64": 0, "mixed": 1}
+
+
+def precision(cache, policy):
+    """precision(cache, T2) (cache.jl:212-232): "fp64" (default) or "mixed" -- the transfer and the
+    dual sweep over the stored samples in FP32 (FP64 geodesics and per-sample geometry; the step in
+    FP64 where the Faraday depth exceeds 10; compensated Stokes sums). Set in place; returns the cache."""
+    if policy not in PRECISIONS:
+        raise ValueError(f"precision policy must be one of {sorted(PRECISIONS)}, got {policy!r}")
+    _check(cache._lib.ks_cache_precision(cache._h, _i(PRECISIONS[policy])))
+    cache.precision = policy
+    return cache
+
+
+def march(cache):
+    """march!(marcher, cache) (cache.jl:182-190): re-run the cache's storing marcher into its samples."""
+    _check(cache._lib.ks_march(cache._h))
+    return cache
+
+
+def polarized_gradient(cache, params, times, nus, L, dstokes, dparams=None, nmax=-1, slab=0.0, screen=True):
+    """polarized_gradient!(dparams, dstokes, cache, params, t_obs, ν, L; method = :dual, nmax, slab)
+    (polarized.jl:473-480, dual_gradient.jl:180-249): the gradient of sum(dstokes * I_obs) over every
+    pixel, time and frequency with respect to the thermal splat parameters, added into dparams
+    (21 x ns, allocated as zeros when None). dstokes has the shape polarized_cube returns
+    ((n_alpha, n_beta, nt, nnu, 4) with screen=True, (nnu, nt, npix, 4) in sorted order otherwise);
+    returns (dparams, cube) with the cube of the same forward pass. Needs stored samples."""
+    m = _as_model(params, "thermal")
+    if m.kind != "thermal":
+        raise ValueError("the dual sweep takes thermal splats (PolarizedSplats)")
+    t, n = _f64(np.atleast_1d(times)), _f64(np.atleast_1d(nus))
+    na, nb = cache.screen_size
+    ds = np.asarray(dstokes, dtype=np.float64)
+    if screen:
+        if ds.shape != (na, nb, len(t), len(n), 4):
+            raise DimensionMismatch(f"dstokes must be {(na, nb, len(t), len(n), 4)}, got {ds.shape}")
+        # (n_alpha, n_beta, nt, nnu, 4) -> (nnu, nt, npix, 4) with pixel = i + na j
+        ds = np.ascontiguousarray(np.transpose(ds, (3, 2, 1, 0, 4)).reshape(len(n), len(t), cache.npix, 4))
+    else:
+        if ds.shape != (len(n), len(t), cache.npix, 4):
+            raise DimensionMismatch(f"dstokes must be {(len(n), len(t), cache.npix, 4)}, got {ds.shape}")
+        ds = np.ascontiguousarray(ds)
+    if dparams is None:
+        dparams = np.zeros(m.params.shape, order="F")
+    elif dparams.shape != m.params.shape or not dparams.flags.f_contiguous or dparams.dtype != np.float64:
+        raise DimensionMismatch("dparams must be a Fortran-ordered float64 array shaped like the parameters")
+    out = np.zeros((len(n), len(t), cache.npix, 4))
+    _check(cache._lib.ks_polarized_gradient(cache._h, m._h, _ptr(t), _i(len(t)), _ptr(n), _i(len(n)), _d(L), _i(nmax), _d(slab),
+                                            _ptr(ds), _i(int(screen)), _ptr(dparams), _ptr(out)))
+    if not screen:
+        return dparams, out
+    img = out.reshape(len(n), len(t), nb, na, 4)
+    return dparams, np.transpose(img, (3, 2, 1, 0, 4))
 
 
 def visibilities(cache, frames, psize, u, v, taper=None):

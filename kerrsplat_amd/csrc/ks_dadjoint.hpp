@@ -16,29 +16,31 @@
 
 namespace ksd {
 
-using D7 = DN<7>;
-
-// per-splat constants of the adjoint, built on the host with the SplatC
-struct SplatAdj {
-    double qn[4];          // normalised quaternion (w, x, y, z) of rows 7-10
-    double iqn;            // 1 / |q|
-    double dk0k2, dk1k2;   // d(K0/K2)/d ln Theta_e, d(K1/K2)/d ln Theta_e at 1/Theta_e
-    double sthB, cthB, sphB, cphB;   // field direction angles (rows 16, 17)
-    double pad[5];
+// per-splat constants of the adjoint, built on the host with the splat constants (both precisions)
+template <class S> struct SplatAdjT {
+    S qn[4];          // normalised quaternion (w, x, y, z) of rows 7-10
+    S iqn;            // 1 / |q|
+    S dk0k2, dk1k2;   // d(K0/K2)/d ln Theta_e, d(K1/K2)/d ln Theta_e at 1/Theta_e
+    S sthB, cthB, sphB, cphB;   // field direction angles (rows 16, 17)
+    S pad[5];
 };
+using SplatAdj = SplatAdjT<double>;
+using SplatAdjf = SplatAdjT<f32>;
 
 // One sample: given its coefficients, the tail behind it (R = T_{k+1}) and the adjoint w in front of
 // it, returns O^T w, j_bar, alpha_bar, rho_bar.
-KS_DEV void sample_adjoint(const double j4[4], const double a4[4], const double r3[3], double D, const double w[4], const double R[4],
-                           double Otw[4], double jbar[4], double abar[4], double rbar[3]) {
+template <class S>
+KS_DEV void sample_adjoint(const S j4[4], const S a4[4], const S r3[3], S D, const S w[4], const S R[4],
+                           S Otw[4], S jbar[4], S abar[4], S rbar[3]) {
+    using D7 = DN<7, S>;
     D7 al[4], rh[3];
     for (int i = 0; i < 4; ++i) al[i] = D7::seed(a4[i], i);
     for (int i = 0; i < 3; ++i) rh[i] = D7::seed(r3[i], 4 + i);
-    double Dp;
+    S Dp;
     D7 K6[6], o[4], e4[4];
     step_coefs(al, rh, D, Dp, K6, o, e4);
     // q = w . (O R + Ej j)
-    double wR = 0.0, wj = 0.0;
+    S wR(0.0), wj(0.0);
     for (int i = 0; i < 4; ++i) { wR += w[i] * R[i]; wj += w[i] * j4[i]; }
     D7 q = o[0] * wR + (Dp * wj) * e4[0];
     D7 vR[4], vj[4];
@@ -57,9 +59,9 @@ KS_DEV void sample_adjoint(const double j4[4], const double a4[4], const double 
     for (int i = 0; i < 4; ++i) abar[i] = q.d[i];
     for (int i = 0; i < 3; ++i) rbar[i] = q.d[4 + i];
     // values: O^T w = sum o_n K'^T^n w, Ej^T w = Dp sum e_n K'^T^n w
-    double K6v[6];
+    S K6v[6];
     for (int i = 0; i < 6; ++i) K6v[i] = K6[i].v;
-    double u[4], y[4];
+    S u[4], y[4];
     for (int i = 0; i < 4; ++i) { u[i] = w[i]; Otw[i] = o[0].v * w[i]; jbar[i] = Dp * e4[0].v * w[i]; }
     for (int n = 1; n < 4; ++n) {
         kprime_tmul(K6v, u, y);
@@ -69,41 +71,42 @@ KS_DEV void sample_adjoint(const double j4[4], const double a4[4], const double 
 
 // d ln G of the splat weight (Splats.jl:58-93) with respect to rows 1-12 and the pattern rate; the
 // Gaussian itself is returned (0 when culled). dl[0..11] = rows 1..12, dl[12] = omega.
-KS_DEV double weight_dlog(const SplatC& c, const SplatAdj& sa, double t, double x, double y, double z, double dl[13]) {
-    double sp, cp;
-    if (outside_support(c, t, x, y, z, sp, cp)) return 0.0;
-    const double cx = c.x * cp - c.y * sp, cy = c.x * sp + c.y * cp;
-    const double d0 = x - cx, d1 = y - cy, d2 = z - c.z;
-    const double e[3] = {d0 * cp + d1 * sp, -d0 * sp + d1 * cp, d2};
-    double u[3], q[3];
+template <class S>
+KS_DEV S weight_dlog(const SplatCT<S>& c, const SplatAdjT<S>& sa, S t, S x, S y, S z, S dl[13]) {
+    S sp, cp;
+    if (outside_support(c, t, x, y, z, sp, cp)) return S(0.0);
+    const S cx = c.x * cp - c.y * sp, cy = c.x * sp + c.y * cp;
+    const S d0 = x - cx, d1 = y - cy, d2 = z - c.z;
+    const S e[3] = {d0 * cp + d1 * sp, -d0 * sp + d1 * cp, d2};
+    S u[3], q[3];
     for (int b = 0; b < 3; ++b) { u[b] = c.R[b] * e[0] + c.R[3 + b] * e[1] + c.R[6 + b] * e[2]; q[b] = u[b] * c.is[b]; }
-    const double dt = t - c.t0;
-    const double tt = dt * c.iw;
-    const double gexp = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + tt * tt;
-    const double G = exp(-0.5 * gexp);
-    if (!(G > WEIGHT_CUTOFF)) return 0.0;
+    const S dt = t - c.t0;
+    const S tt = dt * c.iw;
+    const S gexp = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + tt * tt;
+    const S G = exp(-0.5 * gexp);
+    if (!(G > WEIGHT_CUTOFF)) return S(0.0);
     // ln G = -(|q|^2 + tt^2) / 2, q_b = u_b exp(-s_b), u = R^T e, e = Rz(-ph) (X - c(ph))
-    double gu[3], ge[3];
+    S gu[3], ge[3];
     for (int b = 0; b < 3; ++b) { gu[b] = -q[b] * c.is[b]; dl[3 + b] = q[b] * q[b]; }
     for (int a_ = 0; a_ < 3; ++a_) ge[a_] = c.R[3 * a_] * gu[0] + c.R[3 * a_ + 1] * gu[1] + c.R[3 * a_ + 2] * gu[2];
     // rotation: dlnG/dR[a][b] = e_a gu_b, through the normalised quaternion (w, x, y, z)
-    double GR[9];
+    S GR[9];
     for (int a_ = 0; a_ < 3; ++a_) for (int b = 0; b < 3; ++b) GR[3 * a_ + b] = e[a_] * gu[b];
-    const double qw = sa.qn[0], qx = sa.qn[1], qy = sa.qn[2], qz = sa.qn[3];
-    double gn[4];
+    const S qw = sa.qn[0], qx = sa.qn[1], qy = sa.qn[2], qz = sa.qn[3];
+    S gn[4];
     gn[0] = 2.0 * (-qz * GR[1] + qy * GR[2] + qz * GR[3] - qx * GR[5] - qy * GR[6] + qx * GR[7]);
     gn[1] = 2.0 * (qy * GR[1] + qz * GR[2] + qy * GR[3] - 2.0 * qx * GR[4] - qw * GR[5] + qz * GR[6] + qw * GR[7] - 2.0 * qx * GR[8]);
     gn[2] = 2.0 * (-2.0 * qy * GR[0] + qx * GR[1] + qw * GR[2] + qx * GR[3] + qz * GR[5] - qw * GR[6] + qz * GR[7] - 2.0 * qy * GR[8]);
     gn[3] = 2.0 * (-2.0 * qz * GR[0] - qw * GR[1] + qx * GR[2] + qw * GR[3] - 2.0 * qz * GR[4] + qy * GR[5] + qx * GR[6] + qy * GR[7]);
-    const double ng = qw * gn[0] + qx * gn[1] + qy * gn[2] + qz * gn[3];
+    const S ng = qw * gn[0] + qx * gn[1] + qy * gn[2] + qz * gn[3];
     for (int i = 0; i < 4; ++i) dl[6 + i] = (gn[i] - sa.qn[i] * ng) * sa.iqn;
     // centre: dlnG/dc = -Rz(ph) ge, c = Rz(ph) (c0, c1) + c2 e_z
-    const double gd0 = ge[0] * cp - ge[1] * sp, gd1 = ge[0] * sp + ge[1] * cp;
+    const S gd0 = ge[0] * cp - ge[1] * sp, gd1 = ge[0] * sp + ge[1] * cp;
     dl[0] = -(gd0 * cp + gd1 * sp);
     dl[1] = -(gd1 * cp - gd0 * sp);
     dl[2] = -ge[2];
     // pattern phase ph = omega (t - t0): de/dph = (e1 + c1, -(e0 + c0), 0)
-    const double dph = ge[0] * (e[1] + c.y) - ge[1] * (e[0] + c.x);
+    const S dph = ge[0] * (e[1] + c.y) - ge[1] * (e[0] + c.x);
     dl[10] = -c.omega * dph + tt * c.iw;        // t0 (row 11)
     dl[11] = tt * tt;                           // log temporal width (row 12)
     dl[12] = dph * dt;                          // pattern rate
@@ -113,8 +116,10 @@ KS_DEV double weight_dlog(const SplatC& c, const SplatAdj& sa, double t, double 
 // One thermal splat's share of one sample (dual_gradient.jl:46-76), given the splat's weight Gw > 0 at
 // the sample and d ln G (weight_dlog): adds into grad (the splat's 21-row column).
 // bars = (j_bar[4], alpha_bar[4], rho_bar[3]).
-KS_DEV void element_adjoint_thermal(const SplatC& c, const SplatAdj& sa, const Geo& G, double a, double alpha, double beta,
-                                    double thetao_sin, double nu, const double bars[11], double Gw, const double dl[13], double grad[21]) {
+template <class S>
+KS_DEV void element_adjoint_thermal(const SplatCT<S>& c, const SplatAdjT<S>& sa, const GeoT<S>& G, S a, S alpha, S beta,
+                                    S thetao_sin, S nu, const S bars[11], S Gw, const S dl[13], S grad[21]) {
+    using D7 = DN<7, S>;
     // fluid rows as 7 partials: ln Theta_e, ln B, theta_B, phi_B, u1, u2, u3
     D7 u[3] = {D7::seed(c.u[0], 4), D7::seed(c.u[1], 5), D7::seed(c.u[2], 6)};
     D7 bh[3];
@@ -131,7 +136,7 @@ KS_DEV void element_adjoint_thermal(const SplatC& c, const SplatAdj& sa, const G
     D7 thB = acos(cb);
     D7 sth, cth;
     sincos(thB, &sth, &cth);
-    const double ne = exp(c.lnA) * Gw;
+    const S ne = exp(c.lnA) * Gw;
     CoefT<D7> cf = thermal(ne, Th, k0k2, k1k2, Bm, nuf, sth, cth);
     if (!(cf.jI > 0.0 || cf.aI > 0.0 || cf.rQ.v != 0.0 || cf.rV.v != 0.0)) return;
     D7 fe = cap_factor(cf.jI, cf.jQ, cf.jV), fa = cap_factor(cf.aI, cf.aQ, cf.aV);
@@ -143,7 +148,7 @@ KS_DEV void element_adjoint_thermal(const SplatC& c, const SplatAdj& sa, const G
     D7 q = bars[0] * (cf.jI * i2) + bars[1] * (jQ * c2c) + bars[2] * (jQ * s2c) + bars[3] * (fe * cf.jV * i2);
     q = q + bars[4] * (cf.aI * nh) + bars[5] * (aQ * c2c) + bars[6] * (aQ * s2c) + bars[7] * (fa * cf.aV * nh);
     q = q + bars[8] * (rQ * c2c) + bars[9] * (rQ * s2c) + bars[10] * (cf.rV * nh);
-    const double sv = q.v;
+    const S sv = q.v;
     for (int r = 0; r < 12; ++r) grad[r] += sv * dl[r];
     grad[12] += sv;
     for (int k = 0; k < 7; ++k) grad[13 + k] += q.d[k];

@@ -1,7 +1,8 @@
-// Device polarized synchrotron transfer (FP64): per-splat constants staged in LDS, the splat weight
-// and bounding-sphere cull, the fluid frame and Walker-Penrose screen angle, thermal / power-law /
-// kappa coefficients, the polarization cap and nu-hat invariants, and the exact constant-coefficient
-// 4x4 Stokes step composed front to back in registers.
+// Device polarized synchrotron transfer: per-splat constants, the splat weight and bounding-sphere
+// cull, the fluid frame and Walker-Penrose screen angle, thermal / power-law / kappa coefficients,
+// the polarization cap and nu-hat invariants, and the exact constant-coefficient 4x4 Stokes step.
+// Everything is a template on its scalar: double (the FP64 path), f32 (the FP32 path) or duals of
+// either (the adjoint); see ks_ddual.hpp.
 //
 // Reference: src/Transfer/{coefficients,kappa,step,frames,transport}.jl and
 // src/Splats/{Splats,polarized,populations,trajectories}.jl. The synchrotron fits are the published
@@ -22,24 +23,27 @@ constexpr double WEIGHT_CUTOFF = 1e-6;
 
 enum ModelKind : int { MODEL_THIN = 0, MODEL_THERMAL = 1, MODEL_POWERLAW = 2, MODEL_KAPPA = 3, MODEL_KNOT = 4 };
 
-// Per-splat constants (built on the host by ks_model_create; 64 doubles, LDS-resident in kernels)
-struct SplatC {
-    double x, y, z;            // centre at t0 (rows 1-3)
-    double R[9];               // quaternion rotation, row-major R[a][b]
-    double is[3];              // exp(-s_a)
-    double t0, iw;             // t0 and exp(-logw)
-    double omega;              // pattern angular velocity
-    double rsup2;              // support sphere radius^2 = -2 ln(cutoff) exp(2 max s)
-    double lnA;                // thin: logA; others: logne
-    double Bmag, bh[3];        // field magnitude and unit direction (ZAMO axes r, phi, -theta)
-    double u[3];               // fluid spatial 4-velocity (ZAMO axes)
-    double p1, p2, p3;         // thermal: Theta_e, K0/K2, K1/K2 at 1/Theta_e; power law: p, gamma_min, gamma_max;
-                               // kappa: kappa, w, hyp
-    double g[6];               // power law: Gamma((3p-1)/12) Gamma((3p+19)/12), Gamma((3p+2)/12) Gamma((3p+22)/12);
-                               // kappa: Gamma(k-4/3), Gamma(k-2), Gamma(k/4-1/3), Gamma(k/4+4/3), Gamma(2+k/2), K0/(K2+1e-40) at 1/w
-    int knot0;                 // knot splats: index of the splat's first knot
+// Per-splat constants, built on the host by ks_model_create in both precisions (SplatC, SplatCf);
+// a kernel reads them with wave-uniform addresses.
+template <class S> struct SplatCT {
+    S x, y, z;            // centre at t0 (rows 1-3)
+    S R[9];               // quaternion rotation, row-major R[a][b]
+    S is[3];              // exp(-s_a)
+    S t0, iw;             // t0 and exp(-logw)
+    S omega;              // pattern angular velocity
+    S rsup2;              // support sphere radius^2 = -2 ln(cutoff) exp(2 max s)
+    S lnA;                // thin: logA; others: logne
+    S Bmag, bh[3];        // field magnitude and unit direction (ZAMO axes r, phi, -theta)
+    S u[3];               // fluid spatial 4-velocity (ZAMO axes)
+    S p1, p2, p3;         // thermal: Theta_e, K0/K2, K1/K2 at 1/Theta_e; power law: p, gamma_min, gamma_max;
+                          // kappa: kappa, w, hyp
+    S g[6];               // power law: Gamma((3p-1)/12) Gamma((3p+19)/12), Gamma((3p+2)/12) Gamma((3p+22)/12);
+                          // kappa: Gamma(k-4/3), Gamma(k-2), Gamma(k/4-1/3), Gamma(k/4+4/3), Gamma(2+k/2), K0/(K2+1e-40) at 1/w
+    int knot0;            // knot splats: index of the splat's first knot
     int pad;
 };
+using SplatC = SplatCT<double>;
+using SplatCf = SplatCT<f32>;
 
 // ---- geometry ---------------------------------------------------------------------------------------------
 KS_DEV void quasi_cartesian_ks(double a, double rp, double rm, double r, double th, double ph, double& x, double& y, double& z) {
@@ -52,32 +56,35 @@ KS_DEV void quasi_cartesian_ks(double a, double rp, double rm, double r, double 
 }
 
 // centre of a splat at time t under the pattern rotation and the squared distance (for the cull)
-KS_DEV bool outside_support(const SplatC& s, double t, double x, double y, double z, double& sp, double& cp) {
-    double ph = s.omega * (t - s.t0);
+template <class S>
+KS_DEV bool outside_support(const SplatCT<S>& s, S t, S x, S y, S z, S& sp, S& cp) {
+    S ph = s.omega * (t - s.t0);
     sincos(ph, &sp, &cp);
-    double cx = s.x * cp - s.y * sp, cy = s.x * sp + s.y * cp;
-    double dx = x - cx, dy = y - cy, dz = z - s.z;
+    S cx = s.x * cp - s.y * sp, cy = s.x * sp + s.y * cp;
+    S dx = x - cx, dy = y - cy, dz = z - s.z;
     return dx * dx + dy * dy + dz * dz > s.rsup2;
 }
 // the Gaussian exponent q^2 + tau^2 given the rotation phase (Splats.jl:58-93)
-KS_DEV double gauss_exponent(const SplatC& s, double t, double x, double y, double z, double sp, double cp) {
-    double cx = s.x * cp - s.y * sp, cy = s.x * sp + s.y * cp;
-    double d0 = x - cx, d1 = y - cy, d2 = z - s.z;
-    double e0 = d0 * cp + d1 * sp, e1 = -d0 * sp + d1 * cp, e2 = d2;
-    double u0 = s.R[0] * e0 + s.R[3] * e1 + s.R[6] * e2;
-    double u1 = s.R[1] * e0 + s.R[4] * e1 + s.R[7] * e2;
-    double u2 = s.R[2] * e0 + s.R[5] * e1 + s.R[8] * e2;
-    double q0 = u0 * s.is[0], q1 = u1 * s.is[1], q2 = u2 * s.is[2];
-    double tt = (t - s.t0) * s.iw;
+template <class S>
+KS_DEV S gauss_exponent(const SplatCT<S>& s, S t, S x, S y, S z, S sp, S cp) {
+    S cx = s.x * cp - s.y * sp, cy = s.x * sp + s.y * cp;
+    S d0 = x - cx, d1 = y - cy, d2 = z - s.z;
+    S e0 = d0 * cp + d1 * sp, e1 = -d0 * sp + d1 * cp, e2 = d2;
+    S u0 = s.R[0] * e0 + s.R[3] * e1 + s.R[6] * e2;
+    S u1 = s.R[1] * e0 + s.R[4] * e1 + s.R[7] * e2;
+    S u2 = s.R[2] * e0 + s.R[5] * e1 + s.R[8] * e2;
+    S q0 = u0 * s.is[0], q1 = u1 * s.is[1], q2 = u2 * s.is[2];
+    S tt = (t - s.t0) * s.iw;
     return q0 * q0 + q1 * q1 + q2 * q2 + tt * tt;
 }
 
 // ---- metric, ZAMO tetrad, photon momentum ----------------------------------------------------------------------
-struct Geo {   // everything of a sample the splats share
-    double r, th, st, ct, S, Dl, Xi, lapse, omega, sgrr, sgthth, sgpp;
-    double pu[4];        // contravariant BL photon momentum
-    double pz[4];        // ZAMO components (t, r, phi, -theta)
+template <class T> struct GeoT {   // everything of a sample the splats share
+    T r, th, st, ct, S, Dl, Xi, lapse, omega, sgrr, sgthth, sgpp;
+    T pu[4];        // contravariant BL photon momentum
+    T pz[4];        // ZAMO components (t, r, phi, -theta)
 };
+using Geo = GeoT<double>;
 KS_DEV void make_geo(double a, double r, double th, double eta, double lam, bool nur, bool nuth, Geo& g) {
     g.r = r; g.th = th;
     sincos(th, &g.st, &g.ct);
@@ -106,14 +113,22 @@ KS_DEV void make_geo(double a, double r, double th, double eta, double lam, bool
     g.pz[2] = g.sgpp * (g.pu[3] - g.omega * g.pu[0]);
     g.pz[3] = -g.sgthth * g.pu[2];
 }
+// the FP64 geometry of a sample in another scalar (built once per sample, shared by the splats)
+template <class T> KS_DEV GeoT<T> geo_as(const Geo& g) {
+    GeoT<T> o;
+    o.r = T(g.r); o.th = T(g.th); o.st = T(g.st); o.ct = T(g.ct); o.S = T(g.S); o.Dl = T(g.Dl); o.Xi = T(g.Xi);
+    o.lapse = T(g.lapse); o.omega = T(g.omega); o.sgrr = T(g.sgrr); o.sgthth = T(g.sgthth); o.sgpp = T(g.sgpp);
+    for (int i = 0; i < 4; ++i) { o.pu[i] = T(g.pu[i]); o.pz[i] = T(g.pz[i]); }
+    return o;
+}
 
 template <class T> struct FrameT { T g, cosB, chi; };
 using Frame = FrameT<double>;
 
 // frames.jl:108-128 for one splat: its boost built from the fluid velocity u, the field direction bh
-// (unit); T = double, or duals in u and bh for the adjoint
-template <class T>
-KS_DEV FrameT<T> local_frame(const Geo& G, double a, double alpha, double beta, double thetao_sin, const T u[3], const T bh[3]) {
+// (unit); T = the scalar S of the geometry, or duals over it in u and bh for the adjoint
+template <class T, class S>
+KS_DEV FrameT<T> local_frame(const GeoT<S>& G, S a, S alpha, S beta, S thetao_sin, const T u[3], const T bh[3]) {
     T u2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
     T gam = sqrt(1.0 + u2);
     T q = 1.0 / (gam + 1.0);
@@ -140,11 +155,11 @@ KS_DEV FrameT<T> local_frame(const Geo& G, double a, double alpha, double beta, 
     T fr_ = fz1 * (1.0 / G.sgrr);
     T fth = fz3 * (-1.0 / G.sgthth);
     T fph = fz2 * (1.0 / G.sgpp) + G.omega * ft;
-    const double* p = G.pu;
+    const S* p = G.pu;
     T A = p[0] * fr_ - p[1] * ft + (a * G.st * G.st) * (p[1] * fph - p[3] * fr_);
     T Bw = ((G.r * G.r + a * a) * (p[3] * fth - p[2] * fph) - a * (p[0] * fth - p[2] * ft)) * G.st;
     T kk1 = A * G.r - Bw * (a * G.ct), kk2 = -(A * (a * G.ct) + Bw * G.r);
-    double mu = -(alpha + a * thetao_sin);
+    S mu = -(alpha + a * thetao_sin);
     T ea = beta * kk2 - mu * kk1, eb = beta * kk1 + mu * kk2;
     fr.chi = atan2(-ea, eb);
     return fr;
@@ -164,8 +179,8 @@ template <class T> KS_DEV T planck(T nu, T Th) {
 
 // thermal (coefficients.jl:185-229) with the splat's Bessel ratios k0/k2, k1/k2 (k2 > 0 in practice);
 // ne enters linearly and stays a plain number in the adjoint
-template <class T>
-KS_DEV CoefT<T> thermal(double ne, T Th, T k0k2, T k1k2, T B, T nu, T sth, T cth) {
+template <class T, class NE>
+KS_DEV CoefT<T> thermal(NE ne, T Th, T k0k2, T k1k2, T B, T nu, T sth, T cth) {
     CoefT<T> c;
     T w = 2.0 * PI * nu;
     T w0w = EE * B / (ME * CL) / w;
@@ -200,100 +215,104 @@ KS_DEV CoefT<T> thermal(double ne, T Th, T k0k2, T k1k2, T B, T nu, T sth, T cth
 }
 
 // power law (coefficients.jl:292-340); gPL = {Gamma((3p-1)/12) Gamma((3p+19)/12), Gamma((3p+2)/12) Gamma((3p+22)/12)}
-KS_DEV Coef powerlaw(double ne, double p, double gmin, double gmax, const double* gPL, double B, double nu, double sth, double cth) {
-    Coef c;
-    double nuc = EE * B / (2.0 * PI * ME * CL);
-    double gspan = pow(gmin, 1.0 - p) - pow(gmax, 1.0 - p);
-    double pref = ne * EE * EE * nuc / CL;
-    double X = nu / (nuc * sth);
+template <class T>
+KS_DEV CoefT<T> powerlaw(T ne, T p, T gmin, T gmax, const T* gPL, T B, T nu, T sth, T cth) {
+    CoefT<T> c;
+    T nuc = EE * B / (2.0 * PI * ME * CL);
+    T gspan = pow(gmin, 1.0 - p) - pow(gmax, 1.0 - p);
+    T pref = ne * EE * EE * nuc / CL;
+    T X = nu / (nuc * sth);
     c.jI = pref * (pow(3.0, p * 0.5) * (p - 1.0) * sth) / (2.0 * (p + 1.0) * gspan) * gPL[0] * pow(X, -(p - 1.0) * 0.5);
     c.jQ = (p + 1.0) / (p + 7.0 / 3.0) * c.jI;
     c.jV = (171.0 / 250.0) * pow(p, 0.49) * cth / sth * pow(nu / (3.0 * nuc * sth), -0.5) * c.jI;
-    double apref = ne * EE * EE / (nu * ME * CL);
+    T apref = ne * EE * EE / (nu * ME * CL);
     c.aI = apref * pow(3.0, (p + 1.0) * 0.5) * (p - 1.0) / (4.0 * gspan) * gPL[1] * pow(X, -(p + 2.0) * 0.5);
     c.aQ = c.aI * pow((17.0 / 500.0) * p - 43.0 / 1250.0, 43.0 / 500.0);
-    double term6 = pow(3.1 * pow(sth, -48.0 / 25.0) - 3.1, 64.0 / 125.0);
-    double sgn = cth > 0.0 ? 1.0 : (cth < 0.0 ? -1.0 : 0.0);
+    T term6 = pow(3.1 * pow(sth, -48.0 / 25.0) - 3.1, 64.0 / 125.0);
+    const double sgn = cth > 0.0 ? 1.0 : (cth < 0.0 ? -1.0 : 0.0);
     c.aV = c.aI * pow(0.71 * p + 22.0 / 625.0, 197.0 / 500.0) * term6 * pow(X, -0.5) * sgn;
-    double nuB = nuc * sth;
-    double rperp = ne * EE * EE * (p - 1.0) / (ME * CL * nuB * gspan);
-    double numin = gmin * gmin * nuB;
-    double rb = nuB / nu;
+    T nuB = nuc * sth;
+    T rperp = ne * EE * EE * (p - 1.0) / (ME * CL * nuB * gspan);
+    T numin = gmin * gmin * nuB;
+    T rb = nuB / nu;
     c.rQ = -rperp * rb * rb * rb * pow(gmin, 2.0 - p) * (1.0 - pow(numin / nu, p * 0.5 - 1.0)) / (p * 0.5 - 1.0);
     c.rV = 2.0 * (p + 2.0) / (p + 1.0) * rperp * rb * rb * pow(gmin, -(p + 1.0)) * log(gmin) * cth / sth;
     return c;
 }
 
-KS_DEV double kappa_rhoQ_fit(int i, double w, double X) {
-    double sw = sqrt(w), e5 = exp(-5.0 * w);
+template <class T>
+KS_DEV T kappa_rhoQ_fit(int i, T w, T X) {
+    T sw = sqrt(w), e5 = exp(-5.0 * w);
     if (i == 1) return (17.0 * w - 3.0 * sw + 7.0 * sw * e5) * (1.0 - exp(-pow(X, 0.84) / 30.0) - sin(X / 10.0) * exp(-1.5 * pow(X, 0.471)));
     if (i == 2) return (46.0 * w / 3.0 - 5.0 * sw / 3.0 + 17.0 * sw / 3.0 * e5) * (1.0 - exp(-pow(X, 0.84) / 18.0) - sin(X / 6.0) * exp(-1.75 * sqrt(X)));
     if (i == 3) return (14.0 * w - 13.0 * sw / 8.0 + 4.5 * sw * e5) * (1.0 - exp(-pow(X, 0.84) / 12.0) - sin(X / 4.0) * exp(-2.0 * pow(X, 0.525)));
     return (12.5 * w - sw + 5.0 * sw * e5) * (1.0 - exp(-pow(X, 0.84) / 8.0) - sin(0.375 * X) * exp(-2.25 * pow(X, 0.541)));
 }
-KS_DEV double kappa_rhoV_fit(int i, double w, double X) {
-    double xm = 1.0 / sqrt(X);
+template <class T>
+KS_DEV T kappa_rhoV_fit(int i, T w, T X) {
+    T xm = 1.0 / sqrt(X);
     if (i == 1) return (w * w + 2.0 * w + 1.0) / (3.125 * w * w + 4.0 * w + 1.0) * (1.0 - 0.17 * log1p(0.447 * xm));
     if (i == 2) return (w * w + 54.0 * w + 50.0) / (30.0 * w * w / 11.0 + 134.0 * w + 50.0) * (1.0 - 0.17 * log1p(0.391 * xm));
     if (i == 3) return (w * w + 43.0 * w + 38.0) / (7.0 * w * w / 3.0 + 92.5 * w + 38.0) * (1.0 - 0.17 * log1p(0.348 * xm));
     return (w + 13.0 / 14.0) / (2.0 * w + 13.0 / 14.0) * (1.0 - 0.17 * log1p(0.313 * xm));
 }
 // kappa (kappa.jl:25-119); gK = {G(k-4/3), G(k-2), G(k/4-1/3), G(k/4+4/3), G(2+k/2), K0/(K2+1e-40) at 1/w}
-KS_DEV Coef kappa(double ne, double kap, double w, double hyp, const double* gK, double B, double nu, double sth, double cth) {
-    Coef c;
+template <class T>
+KS_DEV CoefT<T> kappa(T ne, T kap, T w, T hyp, const T* gK, T B, T nu, T sth, T cth) {
+    CoefT<T> c;
     const double small = 1e-40;
-    double nuc = EE * B / (2.0 * PI * ME * CL);
-    double nuw = (w * kap) * (w * kap) * nuc * sth;
-    double X = nu / nuw;
-    double pref = ne * EE * EE * nuc * sth / CL;
-    double g43 = gK[0], g2 = gK[1], gq = gK[2], gp = gK[3], gk2 = gK[4];
+    T nuc = EE * B / (2.0 * PI * ME * CL);
+    T nuw = (w * kap) * (w * kap) * nuc * sth;
+    T X = nu / nuw;
+    T pref = ne * EE * EE * nuc * sth / CL;
+    T g43 = gK[0], g2 = gK[1], gq = gK[2], gp = gK[3], gk2 = gK[4];
     const double c37 = 12.980246132766677;      // 3^(7/3)
-    double NlowI = 4.0 * PI * g43 / (c37 * g2);
-    double hi = 0.25 * pow(3.0, (kap - 1.0) * 0.5) * (kap - 2.0) * (kap - 1.0) * gq * gp;
-    double NhighI = hi + small;
-    double X13 = cbrt(X);
-    double xI = 3.0 * pow(kap, -1.5);
+    T NlowI = 4.0 * PI * g43 / (c37 * g2);
+    T hi = 0.25 * pow(3.0, (kap - 1.0) * 0.5) * (kap - 2.0) * (kap - 1.0) * gq * gp;
+    T NhighI = hi + small;
+    T X13 = cbrt(X);
+    T xI = 3.0 * pow(kap, -1.5);
     c.jI = pref * NlowI * X13 * pow(1.0 + pow(X, xI * (3.0 * kap - 4.0) / 6.0) * pow(NlowI / NhighI, xI), -1.0 / xI);
-    double NlowQ = -0.5 * 4.0 * PI * g43 / (c37 * g2);
-    double NhighQ = -(0.64 + kap / 50.0) * hi + small;
-    double xQ = 3.7 * pow(kap, -1.6);
+    T NlowQ = -0.5 * 4.0 * PI * g43 / (c37 * g2);
+    T NhighQ = -(0.64 + kap / 50.0) * hi + small;
+    T xQ = 3.7 * pow(kap, -1.6);
     c.jQ = -(pref * NlowQ * X13 * pow(1.0 + pow(X, xQ * (3.0 * kap - 4.0) / 6.0) * pow(NlowQ / NhighQ, xQ), -1.0 / xQ));
-    double NlowV = -0.5625 * pow(pow(sth, -2.4) - 1.0, 0.48) * (pow(kap, -0.528) / w) * pow(X, -0.35) * 4.0 * PI * g43 / (c37 * g2);
-    double NhighV = -0.765625 * pow(pow(sth, -2.5) - 1.0, 0.44) * (pow(kap, -0.44) / w) / sqrt(X) * hi - small;
-    double xV = xI;
-    double sgn = cth > 0.0 ? 1.0 : (cth < 0.0 ? -1.0 : 0.0);
-    c.jV = fabs(NhighV) < small * small ? 0.0
+    T NlowV = -0.5625 * pow(pow(sth, -2.4) - 1.0, 0.48) * (pow(kap, -0.528) / w) * pow(X, -0.35) * 4.0 * PI * g43 / (c37 * g2);
+    T NhighV = -0.765625 * pow(pow(sth, -2.5) - 1.0, 0.44) * (pow(kap, -0.44) / w) / sqrt(X) * hi - small;
+    T xV = xI;
+    const double sgn = cth > 0.0 ? 1.0 : (cth < 0.0 ? -1.0 : 0.0);
+    c.jV = fabs(NhighV) < small * small ? T(0.0)
          : -(pref * NlowV * X13 * pow(1.0 + pow(X, xV * (3.0 * kap - 4.0) / 6.0) * pow(NlowV / NhighV, xV), -1.0 / xV)) * sgn;
-    double apref = ne * EE / (B * sth);
+    T apref = ne * EE / (B * sth);
     const double g53 = 0.9027452929509338;     // Gamma(5/3)
-    double ck = (kap - 2.0) * (kap - 1.0) * kap;
+    T ck = (kap - 2.0) * (kap - 1.0) * kap;
     const double c16 = 1.2009369551760027;     // 3^(1/6)
     const double tpi2 = 39.47841760435743;     // (2 pi)^2
-    double wk = w * kap;
-    double lowA = c16 * (10.0 / 41.0) * tpi2 / pow(wk, 16.0 / 3.0 - kap) * ck / (3.0 * kap - 1.0) * g53 * hyp;
-    double highA = 2.0 * 17.493418327624862 / 3.0 * ck / pow(wk, 5.0) * (2.0 * gk2 / (2.0 + kap) - 1.0);   // 2 pi^(5/2)/3
-    double NlowA = lowA;
-    double NhighA = highA * (pow(3.0 / kap, 4.75) + 0.6) + small;
-    double xA = pow(-1.75 + 1.6 * kap, -0.86);
-    double Xm53 = pow(X, -5.0 / 3.0);
+    T wk = w * kap;
+    T lowA = c16 * (10.0 / 41.0) * tpi2 / pow(wk, 16.0 / 3.0 - kap) * ck / (3.0 * kap - 1.0) * g53 * hyp;
+    T highA = 2.0 * 17.493418327624862 / 3.0 * ck / pow(wk, 5.0) * (2.0 * gk2 / (2.0 + kap) - 1.0);   // 2 pi^(5/2)/3
+    T NlowA = lowA;
+    T NhighA = highA * (pow(3.0 / kap, 4.75) + 0.6) + small;
+    T xA = pow(-1.75 + 1.6 * kap, -0.86);
+    T Xm53 = pow(X, -5.0 / 3.0);
     c.aI = apref * NlowA * Xm53 * pow(1.0 + pow(X, xA * (3.0 * kap - 1.0) / 6.0) * pow(NlowA / NhighA, xA), -1.0 / xA);
-    double NlowQA = -(25.0 / 48.0) * lowA;
-    double NhighQA = -(441.0 * pow(kap, -5.76) + 0.55) * highA + small;
-    double xQA = 1.4 * pow(kap, -1.15);
+    T NlowQA = -(25.0 / 48.0) * lowA;
+    T NhighQA = -(441.0 * pow(kap, -5.76) + 0.55) * highA + small;
+    T xQA = 1.4 * pow(kap, -1.15);
     c.aQ = -(apref * NlowQA * Xm53 * pow(1.0 + pow(X, xQA * (3.0 * kap - 1.0) / 6.0) * pow(NlowQA / NhighQA, xQA), -1.0 / xQA));
-    double NlowVA = -(0.77 / w) * pow(pow(sth, -2.28) - 1.0, 0.446) * pow(X, -0.35) * pow(kap, -0.7) * lowA;
-    double NhighVA = -(14.3 * pow(w, -0.928)) * sqrt(pow(sth, -2.05) - 1.0) *
+    T NlowVA = -(0.77 / w) * pow(pow(sth, -2.28) - 1.0, 0.446) * pow(X, -0.35) * pow(kap, -0.7) * lowA;
+    T NhighVA = -(14.3 * pow(w, -0.928)) * sqrt(pow(sth, -2.05) - 1.0) *
                      (169.0 * pow(kap, -8.0) + 13.0 / 2500.0 * kap - 263.0 / 5000.0 + 47.0 / (200.0 * kap)) / sqrt(X) * highA + small;
-    double xVA = 1.22 * pow(kap, -1.136) + 0.007;
+    T xVA = 1.22 * pow(kap, -1.136) + 0.007;
     c.aV = -(apref * NlowVA * Xm53 * pow(1.0 + pow(X, xVA * (3.0 * kap - 1.0) / 6.0) * pow(NlowVA / NhighVA, xVA), -1.0 / xVA)) * sgn;
     // rotativities (Marszewski+ 2021), linear in kappa between the fits
-    double kc = dmin(dmax(kap, 3.5), 5.0);
+    T kc = dmin(dmax(kap, 3.5), 5.0);
     int i = kc < 4.0 ? 1 : (kc < 4.5 ? 2 : 3);
-    double klo = 3.5 + (i - 1) * 0.5;
-    double f = (kc - klo) * 2.0;
-    double Xr = nu / ((w * kap) * (w * kap) * nuc * sth);
-    double fQ = (1.0 - f) * kappa_rhoQ_fit(i, w, Xr) + f * kappa_rhoQ_fit(i + 1, w, Xr);
-    double fV = (1.0 - f) * kappa_rhoV_fit(i, w, Xr) + f * kappa_rhoV_fit(i + 1, w, Xr);
+    const double klo = 3.5 + (i - 1) * 0.5;
+    T f = (kc - klo) * 2.0;
+    T Xr = nu / ((w * kap) * (w * kap) * nuc * sth);
+    T fQ = (1.0 - f) * kappa_rhoQ_fit<T>(i, w, Xr) + f * kappa_rhoQ_fit<T>(i + 1, w, Xr);
+    T fV = (1.0 - f) * kappa_rhoV_fit<T>(i, w, Xr) + f * kappa_rhoV_fit<T>(i + 1, w, Xr);
     c.rQ = -(ne * EE * EE * nuc * nuc * sth * sth) / (ME * CL * nu * nu * nu) * fQ;
     c.rV = 2.0 * (ne * EE * EE * nuc * cth) / (ME * CL * nu * nu) * gK[5] * fV;
     return c;
@@ -304,6 +323,11 @@ KS_DEV double pow2_below(double x) {
     if (!(isfinite(x) && x > 0.0)) return 1.0;
     int e; frexp(x, &e);
     return ldexp(1.0, e - 1);
+}
+KS_DEV f32 pow2_below(f32 x) {
+    if (!(isfinite(x.v) && x.v > 0.0f)) return f32(1.0f);
+    int e; frexpf(x.v, &e);
+    return f32(ldexpf(1.0f, e - 1));
 }
 template <class T> KS_DEV T cap_factor(T I, T Q, T V) {
     const double fmax = 0.99;
@@ -321,30 +345,28 @@ template <class T> KS_DEV T phi1(T a) {
     if (fabs(val(a)) < 1e-3) return 1.0 - a * (0.5 - a * (1.0 / 6.0 - a * (1.0 / 24.0 - a / 120.0)));
     return -expm1(-a) / a;
 }
-// M_n(a) = int_0^1 t^n e^{-at} dt, n = 0..9
-template <class T> KS_DEV void moments(T a, T M[10]) {
-    const double av = val(a);
-    if (fabs(av) < 1.0) {
-        for (int n = 0; n < 10; ++n) {
-            T term = T(1.0), sum = T(1.0 / (n + 1.0));
-            for (int m = 1; m < 24; ++m) { term = term * (-a / (double)m); sum = sum + term / (n + m + 1.0); }
-            M[n] = sum;
-        }
-    } else if (av < 40.0) {
-        T e = exp(-a);
-        for (int n = 0; n < 10; ++n) {
-            T term = T(1.0 / (n + 1.0)), sum = term;
-            for (int m = 1; m < 200; ++m) {
-                term = term * (a / (n + m + 1.0));
-                sum = sum + term;
-                if (val(term) < 1e-18 * val(sum)) break;
-            }
-            M[n] = e * sum;
+// M_n(a) = int_0^1 t^n e^{-at} dt for n = 0..NMAX (step.jl:239-268): for |a| < 1 the 24-term power
+// series sum_m (-1)^m a^m / (m! (n + m + 1)) by Horner's rule (the coefficients fold to constants in
+// the unrolled loops), otherwise the upward recurrence M_0 = (1 - e^{-a})/a, M_n = (n M_{n-1} - e^{-a})/a.
+KS_DEV constexpr double moment_coef(int n, int m) {
+    double f = 1.0;
+    for (int i = 2; i <= m; ++i) f *= i;
+    return ((m & 1) ? -1.0 : 1.0) / (f * (n + m + 1));
+}
+template <int NMAX, class T> KS_DEV void moments(T a, T M[10]) {
+    if (fabs(val(a)) < 1.0) {
+#pragma unroll
+        for (int n = 0; n <= NMAX; ++n) {
+            T s = T(moment_coef(n, 23));
+#pragma unroll
+            for (int m = 22; m >= 0; --m) s = s * a + moment_coef(n, m);
+            M[n] = s;
         }
     } else {
         T e = exp(-a);
         M[0] = (1.0 - e) / a;
-        for (int n = 1; n < 10; ++n) M[n] = ((double)n * M[n - 1] - e) / a;
+#pragma unroll
+        for (int n = 1; n <= NMAX; ++n) M[n] = ((double)n * M[n - 1] - e) / a;
     }
 }
 template <class T> KS_DEV T sinhc(T b) {
@@ -358,11 +380,11 @@ template <class T> KS_DEV T sinc_(T b) {
 
 // The step of one sample (step.jl:164-243) as polynomials in the scaled K' (K = alpha_I + K'):
 // O = o0 + o1 K' + o2 K'^2 + o3 K'^3 and Ej = Dp (e0 + e1 K' + e2 K'^2 + e3 K'^3), with
-// K'6 = (aQ, aU, aV, rQ, rU, rV) times Dp = pow2_below(D); D > 0.
-template <class T>
-KS_DEV void step_coefs(const T al[4], const T rh[3], double D, double& Dp, T K6[6], T o[4], T e4[4]) {
+// K'6 = (aQ, aU, aV, rQ, rU, rV) times Dp = pow2_below(D); D > 0. T is the scalar S or duals over it.
+template <class T, class S>
+KS_DEV void step_coefs(const T al[4], const T rh[3], S D, S& Dp, T K6[6], T o[4], T e4[4]) {
     Dp = pow2_below(D);
-    const double f = D / Dp;
+    const S f = D / Dp;
     T aI = al[0] * Dp;
     T aQ = al[1] * Dp, aU = al[2] * Dp, aV = al[3] * Dp;
     T rQ = rh[0] * Dp, rU = rh[1] * Dp, rV = rh[2] * Dp;
@@ -371,7 +393,7 @@ KS_DEV void step_coefs(const T al[4], const T rh[3], double D, double& Dp, T K6[
     T a = aI * f;
     T e = exp(-a);
     if (val(a2 + r2) * f * f < 1e-200) {
-        T M[10]; moments(a, M);
+        T M[10]; moments<1>(a, M);
         o[0] = e; o[1] = -f * e; o[2] = o[3] = T(0.0);
         e4[0] = f * phi1(a); e4[1] = -f * f * M[1]; e4[2] = e4[3] = T(0.0);
         return;
@@ -385,8 +407,8 @@ KS_DEV void step_coefs(const T al[4], const T rh[3], double D, double& Dp, T K6[
     T L1 = L1sq > 0.0 ? sqrt(L1sq) : T(0.0), L2 = L2sq > 0.0 ? sqrt(L2sq) : T(0.0);
     T b1 = L1 * f, b2 = L2 * f;
     T Th = L1sq + L2sq;
-    T M[10]; moments(a, M);
     if (val(Th) * f * f * (1.0 + val(a2 + r2) * f * f) < 1e-17) {
+        T M[10]; moments<3>(a, M);
         o[0] = e; o[1] = -f * e; o[2] = f * f * 0.5 * e; o[3] = -f * f * f / 6.0 * e;
         e4[0] = f * M[0]; e4[1] = -f * f * M[1]; e4[2] = f * f * f * M[2] * 0.5; e4[3] = -f * f * f * f * M[3] / 6.0;
         return;
@@ -409,6 +431,10 @@ KS_DEV void step_coefs(const T al[4], const T rh[3], double D, double& Dp, T K6[
     o[1] = -(s3 * L2sq + s2 * L1sq) * iTh;
     o[3] = -(s3 - s2) * iTh;
     T pa = phi1(a);
+    // the moments: M_1 always, M_0..M_9 for the small-b series
+    T M[10];
+    if (b1 < 0.1 || b2 < 0.1) moments<9>(a, M);
+    else moments<1>(a, M);
     // int_0^1 e^{-at}(cosh bt - 1), (1 - cos bt), sin(bt)/b, (sinh(bt)/b - t)
     T icm1, ivs, isc, ish;
     if (b1 < 0.1) {
@@ -459,29 +485,31 @@ KS_DEV void kprime_tmul(const T K6[6], const U v[4], R y[4]) {
 }
 
 // O and E = Ej j for K = alpha_I + K' (step.jl:164-243), D > 0. Writes O row-major.
-KS_DEV void transfer_step(const double j4[4], const double al[4], const double rh[3], double D, double O[16], double E[4]) {
-    double Dp, K6[6], o[4], e4[4];
+template <class S>
+KS_DEV void transfer_step(const S j4[4], const S al[4], const S rh[3], S D, S O[16], S E[4]) {
+    S Dp, K6[6], o[4], e4[4];
     step_coefs(al, rh, D, Dp, K6, o, e4);
-    const double aQ = K6[0], aU = K6[1], aV = K6[2], rQ = K6[3], rU = K6[4], rV = K6[5];
-    const double K[16] = {0.0, aQ, aU, aV, aQ, 0.0, rV, -rU, aU, -rV, 0.0, rQ, aV, rU, -rQ, 0.0};
-    double K2[16], K3[16];
+    const S aQ = K6[0], aU = K6[1], aV = K6[2], rQ = K6[3], rU = K6[4], rV = K6[5];
+    const S zero(0.0);
+    const S K[16] = {zero, aQ, aU, aV, aQ, zero, rV, -rU, aU, -rV, zero, rQ, aV, rU, -rQ, zero};
+    S K2[16], K3[16];
     for (int i = 0; i < 4; ++i) for (int k = 0; k < 4; ++k) {
-        double s = 0.0;
+        S s(0.0);
         for (int q = 0; q < 4; ++q) s += K[4 * i + q] * K[4 * q + k];
         K2[4 * i + k] = s;
     }
     for (int i = 0; i < 4; ++i) for (int k = 0; k < 4; ++k) {
-        double s = 0.0;
+        S s(0.0);
         for (int q = 0; q < 4; ++q) s += K2[4 * i + q] * K[4 * q + k];
         K3[4 * i + k] = s;
     }
     for (int i = 0; i < 4; ++i) {
-        double ei = 0.0;
+        S ei(0.0);
         for (int k = 0; k < 4; ++k) {
-            double id = i == k ? 1.0 : 0.0;
+            const double id = i == k ? 1.0 : 0.0;
             int ix = 4 * i + k;
             O[ix] = o[0] * id + o[1] * K[ix] + o[2] * K2[ix] + o[3] * K3[ix];
-            double ej = e4[0] * id + e4[1] * K[ix] + e4[2] * K2[ix] + e4[3] * K3[ix];
+            S ej = e4[0] * id + e4[1] * K[ix] + e4[2] * K2[ix] + e4[3] * K3[ix];
             ei += ej * j4[k];
         }
         E[i] = Dp * ei;
@@ -490,10 +518,11 @@ KS_DEV void transfer_step(const double j4[4], const double al[4], const double r
 
 // T_out = E + O T_in for one sample from its polynomial coefficients (the tails recursion,
 // dual_gradient.jl:129-147), by repeated products with K' instead of the 4x4 matrices
-KS_DEV void step_apply(const double j4[4], const double al[4], const double rh[3], double D, const double Tin[4], double Tout[4]) {
-    double Dp, K6[6], o[4], e4[4];
+template <class S>
+KS_DEV void step_apply(const S j4[4], const S al[4], const S rh[3], S D, const S Tin[4], S Tout[4]) {
+    S Dp, K6[6], o[4], e4[4];
     step_coefs(al, rh, D, Dp, K6, o, e4);
-    double v[4], u[4], y[4], z[4];
+    S v[4], u[4], y[4], z[4];
     for (int i = 0; i < 4; ++i) { v[i] = Tin[i]; u[i] = j4[i]; Tout[i] = o[0] * Tin[i] + Dp * e4[0] * j4[i]; }
     for (int n = 1; n < 4; ++n) {
         kprime_mul(K6, v, y); kprime_mul(K6, u, z);

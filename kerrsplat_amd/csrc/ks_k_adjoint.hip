@@ -1,0 +1,232 @@
+// Adjoint kernels: the dual sweep over stored samples (polarized_tails!, polarized_dual_sweep!,
+// dual_gradient.jl:125-249): tails and adjoints at the chunk boundaries from the chunk maps, the
+// per-sample adjoints inside each chunk, the per-splat element adjoints with a wavefront sum, and a
+// deterministic reduction over the item blocks.
+#include "ks_kernels.hpp"
+
+namespace ksd {
+
+// ---- the dual-sweep adjoint (dual_gradient.jl:125-249) over stored samples -------------------------------------
+// Per item: from the chunk maps (P_m, S_m), the tail behind each chunk T_out[m] = S_{m+1} + P_{m+1} T_out[m+1]
+// (T_out[last] = 0), the image nu-hat^3 T_in[0], and the adjoint in front of each chunk
+// w_in[m] = (P_0 ... P_{m-1})^T w_1 with w_1 = nu-hat^3 dstokes (dual_gradient.jl:162).
+// wT[(8 m + q) nitems + item] = w_in, wT[(8 m + 4 + q) nitems + item] = T_out.
+__global__ void ks_adj_combine(int64_t npix, int N, int nt, const double* __restrict__ nus, int nnu, const double* __restrict__ seg,
+                               const double* __restrict__ dstokes, double* __restrict__ wT, double* __restrict__ out) {
+    const int64_t nitems = npix * nt * nnu;
+    const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= nitems) return;
+    const int inu = (int)((item / npix) / nt);
+    const int nseg = (N + CHUNK - 1) / CHUNK;
+    const double nh = nus[inu] * (1.0 / NU0);
+    const double s3 = nh * nh * nh;
+    double T[4] = {0, 0, 0, 0};
+    for (int m = nseg - 1; m >= 0; --m) {
+        const double* g = seg + (int64_t)SEG_FIELDS * m * nitems + item;
+        for (int q = 0; q < 4; ++q) wT[(8 * (int64_t)m + 4 + q) * nitems + item] = T[q];
+        double Tn[4];
+        for (int i = 0; i < 4; ++i) {
+            double v = g[(16 + i) * nitems];
+            for (int q = 0; q < 4; ++q) v += g[(4 * i + q) * nitems] * T[q];
+            Tn[i] = v;
+        }
+        for (int i = 0; i < 4; ++i) T[i] = Tn[i];
+    }
+    for (int q = 0; q < 4; ++q) out[4 * item + q] = T[q] * s3;
+    double w[4];
+    for (int q = 0; q < 4; ++q) w[q] = dstokes[4 * item + q] * s3;
+    for (int m = 0; m < nseg; ++m) {
+        const double* g = seg + (int64_t)SEG_FIELDS * m * nitems + item;
+        for (int q = 0; q < 4; ++q) wT[(8 * (int64_t)m + q) * nitems + item] = w[q];
+        double wn[4];
+        for (int c = 0; c < 4; ++c) {
+            double v = 0.0;
+            for (int i = 0; i < 4; ++i) v += g[(4 * i + c) * nitems] * w[i];
+            wn[c] = v;
+        }
+        for (int q = 0; q < 4; ++q) w[q] = wn[q];
+    }
+}
+
+// The adjoint of one sample's step in the transfer's scalar; in FP32 the samples with a Faraday depth
+// |rho| Delta above FARADAY_FP64 take the FP64 step and its adjoint, as the forward pass does.
+template <class S>
+KS_DEV void step_adjoint_s(const S j4[4], const S a4[4], const S r3[3], S D, const S w[4], const S R[4], S Otw[4], S jb[4], S ab[4], S rb[3]) {
+    if constexpr (std::is_same<S, double>::value) {
+        sample_adjoint(j4, a4, r3, D, w, R, Otw, jb, ab, rb);
+    } else {
+        const S fd = sqrt(r3[0] * r3[0] + r3[1] * r3[1] + r3[2] * r3[2]) * D;
+        if (fd > FARADAY_FP64) {
+            double jd[4], ad[4], rd[3], wd[4], Rd[4], Od[4], jbd[4], abd[4], rbd[3];
+            for (int i = 0; i < 4; ++i) { jd[i] = val(j4[i]); ad[i] = val(a4[i]); wd[i] = val(w[i]); Rd[i] = val(R[i]); }
+            for (int i = 0; i < 3; ++i) rd[i] = val(r3[i]);
+            sample_adjoint(jd, ad, rd, val(D), wd, Rd, Od, jbd, abd, rbd);
+            for (int i = 0; i < 4; ++i) { Otw[i] = S(Od[i]); jb[i] = S(jbd[i]); ab[i] = S(abd[i]); }
+            for (int i = 0; i < 3; ++i) rb[i] = S(rbd[i]);
+        } else {
+            sample_adjoint(j4, a4, r3, D, w, R, Otw, jb, ab, rb);
+        }
+    }
+}
+template <class S>
+KS_DEV void step_apply_s(const S j4[4], const S a4[4], const S r3[3], S D, const S Tin[4], S Tout[4]) {
+    if constexpr (std::is_same<S, double>::value) {
+        step_apply(j4, a4, r3, D, Tin, Tout);
+    } else {
+        const S fd = sqrt(r3[0] * r3[0] + r3[1] * r3[1] + r3[2] * r3[2]) * D;
+        if (fd > FARADAY_FP64) {
+            double jd[4], ad[4], rd[3], Ti[4], To[4];
+            for (int i = 0; i < 4; ++i) { jd[i] = val(j4[i]); ad[i] = val(a4[i]); Ti[i] = val(Tin[i]); }
+            for (int i = 0; i < 3; ++i) rd[i] = val(r3[i]);
+            step_apply(jd, ad, rd, val(D), Ti, To);
+            for (int i = 0; i < 4; ++i) Tout[i] = S(To[i]);
+        } else {
+            step_apply(j4, a4, r3, D, Tin, Tout);
+        }
+    }
+}
+
+// Per (chunk, item): the tails T_{k+1} of the chunk's samples back to front from T_out[m], then the
+// adjoint front to back from w_in[m]: per sample the adjoints of its j, alpha, rho (written over them
+// in the record) and w <- O^T w (polarized_dual_kernel!'s ray loop, dual_gradient.jl:151-178).
+template <class S>
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(Waves<S>::adjstep))) ks_adj_segment(int N, int64_t nitems, int nch, const double* __restrict__ wT, S* __restrict__ rec,
+                                                      const uint8_t* __restrict__ aflg, S* __restrict__ tails) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= nitems * nch) return;
+    const int64_t item = tid % nitems;
+    const int m = (int)(tid / nitems);
+    const int kbeg = m * CHUNK + 1, kend = kbeg + CHUNK - 1 < N ? kbeg + CHUNK - 1 : N;
+    const int64_t st = (int64_t)N * nitems;
+    S T[4], w[4];
+    for (int q = 0; q < 4; ++q) { w[q] = S(wT[(8 * (int64_t)m + q) * nitems + item]); T[q] = S(wT[(8 * (int64_t)m + 4 + q) * nitems + item]); }
+    for (int k = kend; k >= kbeg; --k) {
+        const int64_t fo = (int64_t)(k - 1) * nitems + item;
+        if (!aflg[fo]) continue;
+        const S* o = rec + fo;
+        S j4[4], a4[4], r3[3];
+        for (int i = 0; i < 4; ++i) { j4[i] = o[(REC_J + i) * st]; a4[i] = o[(REC_A + i) * st]; }
+        for (int i = 0; i < 3; ++i) r3[i] = o[(REC_RHO + i) * st];
+        for (int q = 0; q < 4; ++q) tails[(int64_t)q * st + fo] = T[q];
+        S Tn[4];
+        step_apply_s(j4, a4, r3, o[REC_D * st], T, Tn);
+        for (int q = 0; q < 4; ++q) T[q] = Tn[q];
+    }
+    for (int k = kbeg; k <= kend; ++k) {
+        const int64_t fo = (int64_t)(k - 1) * nitems + item;
+        if (!aflg[fo]) continue;
+        S* o = rec + fo;
+        S j4[4], a4[4], r3[3], R[4];
+        for (int i = 0; i < 4; ++i) { j4[i] = o[(REC_J + i) * st]; a4[i] = o[(REC_A + i) * st]; R[i] = tails[(int64_t)i * st + fo]; }
+        for (int i = 0; i < 3; ++i) r3[i] = o[(REC_RHO + i) * st];
+        S Otw[4], jb[4], ab[4], rb[3];
+        step_adjoint_s(j4, a4, r3, o[REC_D * st], w, R, Otw, jb, ab, rb);
+        for (int i = 0; i < 4; ++i) { o[(REC_BAR + i) * st] = jb[i]; o[(REC_BAR + 4 + i) * st] = ab[i]; }
+        for (int i = 0; i < 3; ++i) o[(REC_BAR + 8 + i) * st] = rb[i];
+        for (int q = 0; q < 4; ++q) w[q] = Otw[q];
+    }
+}
+
+// Per (64 consecutive items, splat): every step-taking sample of the items through the splat's
+// element adjoint, 21 partial sums per lane, summed over the wavefront; part[(ib ns + i) 21 + r].
+template <class S>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(Waves<S>::adjelem))) ks_adj_splats(double a, double thetao, const double* __restrict__ soa, int64_t npix, int N, Samples smp,
+                                                    const SplatCT<S>* __restrict__ sp, const SplatAdjT<S>* __restrict__ sadj, int ns,
+                                                    const double* __restrict__ times, int nt, const double* __restrict__ nus, int64_t nitems,
+                                                    const S* __restrict__ rec, const uint8_t* __restrict__ aflg, double* __restrict__ part) {
+    const int i = (int)(blockIdx.x % ns);
+    const int64_t ib = blockIdx.x / ns;
+    const int64_t item = ib * 64 + threadIdx.x;
+    S grad[ADJ_ROWS];
+    for (int r = 0; r < ADJ_ROWS; ++r) grad[r] = S(0.0);
+    if (item < nitems) {
+        const int64_t j = item % npix;
+        const int64_t fr = item / npix;
+        const int it = (int)(fr % nt), inu = (int)(fr / nt);
+        const S nu = S(nus[inu]);
+        const double t_obs = times[it];
+        const SplatCT<S> c = sp[i];
+        const SplatAdjT<S> sa = sadj[i];
+        const double alpha = soa[(int64_t)PF_ALPHA * npix + j], beta = soa[(int64_t)PF_BETA * npix + j];
+        const double eta = soa[(int64_t)PF_ETA * npix + j], lam = soa[(int64_t)PF_LAM * npix + j];
+        const S tos = S(sin(thetao)), aS = S(a), alS = S(alpha), beS = S(beta);
+        const int64_t st = (int64_t)N * nitems;
+        for (int k = 1; k <= N; ++k) {
+            const int64_t fo = (int64_t)(k - 1) * nitems + item;
+            if (!aflg[fo]) continue;
+            const int64_t so = (int64_t)(k - 1) * npix + j;
+            S dl[13];
+            const S Gw = weight_dlog(c, sa, S(t_obs - smp.t[so]), S(smp.x[so]), S(smp.y[so]), S(smp.z[so]), dl);
+            if (!(Gw > 0.0)) continue;
+            const S* o = rec + fo;
+            S bars[11];
+            for (int q = 0; q < 11; ++q) bars[q] = o[(REC_BAR + q) * st];
+            const uint8_t fl = smp.fl[so];
+            Geo Gd;
+            make_geo(a, smp.r[so], smp.th[so], eta, lam, (fl & 2) != 0, (fl & 4) != 0, Gd);
+            const GeoT<S> G = geo_as<S>(Gd);
+            element_adjoint_thermal(c, sa, G, aS, alS, beS, tos, nu, bars, Gw, dl, grad);
+        }
+    }
+    // wavefront sum (blockDim = 64 = one wavefront), in FP64
+    double gsum[ADJ_ROWS];
+    for (int r = 0; r < ADJ_ROWS; ++r) {
+        double v = val(grad[r]);
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        gsum[r] = v;
+    }
+    if (threadIdx.x == 0) {
+        double* o = part + ((int64_t)ib * ns + i) * ADJ_ROWS;
+        for (int r = 0; r < ADJ_ROWS; ++r) o[r] = gsum[r];
+    }
+}
+
+// dparams[r + 21 i] = sum over item blocks in ascending order (deterministic)
+__global__ void ks_adj_reduce(const double* __restrict__ part, int64_t nib, int ns, double* __restrict__ dparams) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)ns * ADJ_ROWS) return;
+    const int i = (int)(t / ADJ_ROWS), r = (int)(t % ADJ_ROWS);
+    double s = 0.0;
+    for (int64_t b = 0; b < nib; ++b) s += part[(b * ns + i) * ADJ_ROWS + r];
+    dparams[t] = s;
+}
+
+// ---- launchers ----
+hipError_t launch_adj_combine(hipStream_t s, int64_t npix, int N, int nt, const double* nus, int nnu, const double* seg,
+                              const double* dstokes, double* wT, double* out) {
+    const int64_t items = npix * nt * nnu;
+    hipLaunchKernelGGL(ks_adj_combine, dim3(grid_for(items, 128)), dim3(128), 0, s, npix, N, nt, nus, nnu, seg, dstokes, wT, out);
+    return hipGetLastError();
+}
+hipError_t launch_adj_segment(hipStream_t s, int N, int64_t nitems, const double* wT, double* rec, const uint8_t* aflg, double* tails) {
+    const int nch = (N + CHUNK - 1) / CHUNK;
+    hipLaunchKernelGGL(ks_adj_segment<double>, dim3(grid_for(nitems * nch, 128)), dim3(128), 0, s, N, nitems, nch, wT, rec, aflg, tails);
+    return hipGetLastError();
+}
+hipError_t launch_adj_segment_f32(hipStream_t s, int N, int64_t nitems, const double* wT, f32* rec, const uint8_t* aflg, f32* tails) {
+    const int nch = (N + CHUNK - 1) / CHUNK;
+    hipLaunchKernelGGL(ks_adj_segment<f32>, dim3(grid_for(nitems * nch, 128)), dim3(128), 0, s, N, nitems, nch, wT, rec, aflg, tails);
+    return hipGetLastError();
+}
+hipError_t launch_adj_splats(hipStream_t s, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
+                             const SplatC* sp, const SplatAdj* sadj, int ns, const double* times, int nt, const double* nus,
+                             int64_t nitems, const double* rec, const uint8_t* aflg, double* part) {
+    const int64_t nib = (nitems + 63) / 64;
+    hipLaunchKernelGGL(ks_adj_splats<double>, dim3((unsigned)(nib * ns)), dim3(64), 0, s, a, thetao, soa, npix, N, smp, sp, sadj, ns, times, nt, nus,
+                       nitems, rec, aflg, part);
+    return hipGetLastError();
+}
+hipError_t launch_adj_splats_f32(hipStream_t s, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
+                                 const SplatCf* sp, const SplatAdjf* sadj, int ns, const double* times, int nt, const double* nus,
+                                 int64_t nitems, const f32* rec, const uint8_t* aflg, double* part) {
+    const int64_t nib = (nitems + 63) / 64;
+    hipLaunchKernelGGL(ks_adj_splats<f32>, dim3((unsigned)(nib * ns)), dim3(64), 0, s, a, thetao, soa, npix, N, smp, sp, sadj, ns, times, nt, nus,
+                       nitems, rec, aflg, part);
+    return hipGetLastError();
+}
+hipError_t launch_adj_reduce(hipStream_t s, const double* part, int64_t nib, int ns, double* dparams) {
+    hipLaunchKernelGGL(ks_adj_reduce, dim3(grid_for((int64_t)ns * ADJ_ROWS, 64)), dim3(64), 0, s, part, nib, ns, dparams);
+    return hipGetLastError();
+}
+
+}  // namespace ksd

@@ -1,0 +1,454 @@
+// Transfer kernels: the polarized transport of every (pixel, time, frequency) item as per-segment
+// 4x4 transfer maps, from stored samples (CHUNK samples per work item) or fused with the march (one
+// anchor segment per work item), their composition along each ray, and thin (Stokes I) images.
+// Reference: transport.jl:99-316, polarized.jl:41-198, Splats.jl:58-147, populations.jl:13-104,
+// trajectories.jl:80-128, step.jl:342-398.
+#include "ks_kernels.hpp"
+
+namespace ksd {
+
+// ---- splat sums ----------------------------------------------------------------------------------------------
+// What a sample's splat sum needs besides the sample itself, in the transfer's scalar S (double or
+// f32); the geometry of a sample (metric, tetrad, photon momentum) is always built in FP64.
+template <class S>
+struct TransCtxT {
+    const SplatCT<S>* __restrict__ sp; const S* __restrict__ knots; const S* __restrict__ ktimes;
+    int ns, nknots;
+    double a, rp, rm, eta, lam, t_obs;          // FP64: geometry and emission times
+    S aS, alpha, beta, thetao_sin, nu, L;
+};
+using TransCtx = TransCtxT<double>;
+template <class S>
+KS_DEV TransCtxT<S> make_ctx(double a, double rp, double rm, double thetao_sin, double alpha, double beta, double eta, double lam,
+                             const SplatCT<S>* sp, int ns, const S* knots, const S* ktimes, int nknots, double t_obs, double nu, double L) {
+    TransCtxT<S> c;
+    c.sp = sp; c.knots = knots; c.ktimes = ktimes; c.ns = ns; c.nknots = nknots;
+    c.a = a; c.rp = rp; c.rm = rm; c.eta = eta; c.lam = lam; c.t_obs = t_obs;
+    c.aS = S(a); c.alpha = S(alpha); c.beta = S(beta); c.thetao_sin = S(thetao_sin); c.nu = S(nu); c.L = S(L);
+    return c;
+}
+
+// The splat sum of one sample (accumulate_elements, transport.jl:254-276): invariant j, alpha, rho in
+// screen axes; false when no splat contributes. One instantiation per population and scalar.
+template <int KIND, class S>
+KS_DEV bool sample_coefficients(const TransCtxT<S>& c, double st, double r, double th, bool nur, bool nuth, S x, S y, S z,
+                                S j4[4], S a4[4], S r3[3]) {
+    const S t = S(c.t_obs - st);
+    for (int i = 0; i < 4; ++i) { j4[i] = S(0.0); a4[i] = S(0.0); }
+    r3[0] = r3[1] = r3[2] = S(0.0);
+    bool active = false, geo_ready = false;
+    GeoT<S> G;
+    for (int i = 0; i < c.ns; ++i) {
+        const SplatCT<S>& sc = c.sp[i];
+        S gexp;
+        if constexpr (KIND == MODEL_KNOT) {
+            // centre from the knots (trajectories.jl:197-233): no pattern rotation
+            const S* K = c.knots + 3 * (int64_t)c.nknots * i;
+            const int nk = c.nknots;
+            S cx, cy, cz;
+            if (t <= c.ktimes[0]) { cx = K[0]; cy = K[1]; cz = K[2]; }
+            else if (t >= c.ktimes[nk - 1]) { cx = K[3 * (nk - 1)]; cy = K[3 * (nk - 1) + 1]; cz = K[3 * (nk - 1) + 2]; }
+            else {
+                int kk = 0;
+                while (kk < nk - 2 && c.ktimes[kk + 1] <= t) ++kk;
+                S w = (t - c.ktimes[kk]) / (c.ktimes[kk + 1] - c.ktimes[kk]);
+                cx = (1.0 - w) * K[3 * kk] + w * K[3 * (kk + 1)];
+                cy = (1.0 - w) * K[3 * kk + 1] + w * K[3 * (kk + 1) + 1];
+                cz = (1.0 - w) * K[3 * kk + 2] + w * K[3 * (kk + 1) + 2];
+            }
+            S d0 = x - cx, d1 = y - cy, d2 = z - cz;
+            S u0 = sc.R[0] * d0 + sc.R[3] * d1 + sc.R[6] * d2;
+            S u1 = sc.R[1] * d0 + sc.R[4] * d1 + sc.R[7] * d2;
+            S u2 = sc.R[2] * d0 + sc.R[5] * d1 + sc.R[8] * d2;
+            S q0 = u0 * sc.is[0], q1 = u1 * sc.is[1], q2 = u2 * sc.is[2];
+            S tt = (t - sc.t0) * sc.iw;
+            gexp = q0 * q0 + q1 * q1 + q2 * q2 + tt * tt;
+        } else {
+            S sph, cph;
+            if (outside_support(sc, t, x, y, z, sph, cph)) continue;
+            gexp = gauss_exponent(sc, t, x, y, z, sph, cph);
+        }
+        S G_ = exp(-0.5 * gexp);
+        if (!(G_ > WEIGHT_CUTOFF)) continue;
+        if (!geo_ready) {
+            Geo Gd;
+            make_geo(c.a, r, th, c.eta, c.lam, nur, nuth, Gd);
+            G = geo_as<S>(Gd);
+            geo_ready = true;
+        }
+        FrameT<S> fr = local_frame(G, c.aS, c.alpha, c.beta, c.thetao_sin, sc.u, sc.bh);
+        S nuf = c.nu / fr.g;
+        S cb = dmin(dmax(fr.cosB, -1.0), 1.0);
+        S sth, cth;
+        sincos(acos(cb), &sth, &cth);    // the pitch angle through acos, as the reference's safe_acos
+        S ne = exp(sc.lnA) * G_;
+        CoefT<S> cf;
+        if constexpr (KIND == MODEL_THERMAL || KIND == MODEL_KNOT) cf = thermal(ne, sc.p1, sc.p2, sc.p3, sc.Bmag, nuf, sth, cth);
+        else if constexpr (KIND == MODEL_POWERLAW) cf = powerlaw(ne, sc.p1, sc.p2, sc.p3, sc.g, sc.Bmag, nuf, sth, cth);
+        else cf = kappa(ne, sc.p1, sc.p2, sc.p3, sc.g, sc.Bmag, nuf, sth, cth);
+        if (!(cf.jI > 0.0 || cf.aI > 0.0 || cf.rQ != 0.0 || cf.rV != 0.0)) continue;
+        S fe = cap_factor(cf.jI, cf.jQ, cf.jV), fa = cap_factor(cf.aI, cf.aQ, cf.aV);
+        S nh = nuf * (1.0 / NU0);
+        S i2 = 1.0 / (nh * nh);
+        S s2c, c2c; sincos(2.0 * fr.chi, &s2c, &c2c);
+        S jQ = fe * cf.jQ * i2, aQ = fa * cf.aQ * nh, rQ = cf.rQ * nh;
+        j4[0] += cf.jI * i2; j4[1] += jQ * c2c; j4[2] += jQ * s2c; j4[3] += fe * cf.jV * i2;
+        a4[0] += cf.aI * nh; a4[1] += aQ * c2c; a4[2] += aQ * s2c; a4[3] += fa * cf.aV * nh;
+        r3[0] += rQ * c2c; r3[1] += rQ * s2c; r3[2] += cf.rV * nh;
+        active = true;
+    }
+    return active;
+}
+
+// thin splats (Splats.jl:113-128): sum_i j_i g^2 Sigma dtau with the ZAMO redshift g
+KS_DEV double thin_sample(const SplatC* __restrict__ sp, int ns, double a, double lam, double t_obs, double st, double r, double th,
+                          double x, double y, double z, double dtau) {
+    const double t = t_obs - st;
+    double jt = 0.0;
+    for (int i = 0; i < ns; ++i) {
+        const SplatC& c = sp[i];
+        double sph, cph; sincos(c.omega * (t - c.t0), &sph, &cph);
+        jt += exp(c.lnA - 0.5 * gauss_exponent(c, t, x, y, z, sph, cph));
+    }
+    double s_, c_; sincos(th, &s_, &c_);
+    double S_ = r * r + a * a * c_ * c_, Dl = r * r - 2.0 * r + a * a;
+    double Xi = (r * r + a * a) * (r * r + a * a) - a * a * Dl * s_ * s_;
+    double g = sqrt(Dl * S_ / Xi) / (1.0 - 2.0 * a * r / Xi * lam);
+    return g * g * jt * S_ * dtau;
+}
+
+// A segment's composite transfer map: for its samples, front to back, P = O_k0 ... O_k1 and
+// Sv = sum_k O_k0 ... O_{k-1} E_k (the RadiativeState of step.jl:342-398 started at the identity).
+// In FP32 the Stokes sum is compensated (Kahan), so its error does not grow with the segment length.
+template <class S>
+struct SegMapT {
+    S P[16], Sv[4], comp[4];
+    KS_DEV void init() {
+        for (int i = 0; i < 16; ++i) P[i] = S((i % 5 == 0) ? 1.0 : 0.0);
+        for (int i = 0; i < 4; ++i) { Sv[i] = S(0.0); comp[i] = S(0.0); }
+    }
+    KS_DEV void advance(const S O[16], const S E[4]) {
+        for (int i = 0; i < 4; ++i) {
+            S pe = P[4 * i] * E[0] + P[4 * i + 1] * E[1] + P[4 * i + 2] * E[2] + P[4 * i + 3] * E[3];
+            if constexpr (std::is_same<S, double>::value) {
+                Sv[i] += pe;
+            } else {
+                S y = pe - comp[i];
+                S t = Sv[i] + y;
+                comp[i] = (t - Sv[i]) - y;
+                Sv[i] = t;
+            }
+        }
+        S Pn[16];
+        for (int i = 0; i < 4; ++i)
+            for (int q = 0; q < 4; ++q) Pn[4 * i + q] = P[4 * i] * O[q] + P[4 * i + 1] * O[4 + q] + P[4 * i + 2] * O[8 + q] + P[4 * i + 3] * O[12 + q];
+        for (int i = 0; i < 16; ++i) P[i] = Pn[i];
+    }
+    KS_DEV void store(double* seg, int64_t m, int64_t nitems, int64_t item) const {
+        double* o = seg + SEG_FIELDS * m * nitems + item;
+        for (int q = 0; q < 16; ++q) o[q * nitems] = val(P[q]);
+        for (int q = 0; q < 4; ++q) o[(16 + q) * nitems] = val(Sv[q]);
+    }
+};
+using SegMap = SegMapT<double>;
+
+// The step of one sample in the transfer's scalar; in FP32 the samples with a Faraday depth
+// |rho| Delta above FARADAY_FP64 take the FP64 step.
+template <class S>
+KS_DEV void sample_step(const S j4[4], const S a4[4], const S r3[3], S D, S O[16], S E[4]) {
+    if constexpr (std::is_same<S, double>::value) {
+        transfer_step(j4, a4, r3, D, O, E);
+    } else {
+        const S fd = sqrt(r3[0] * r3[0] + r3[1] * r3[1] + r3[2] * r3[2]) * D;
+        if (fd > FARADAY_FP64) {
+            double jd[4], ad[4], rd[3], Od[16], Ed[4];
+            for (int i = 0; i < 4; ++i) { jd[i] = val(j4[i]); ad[i] = val(a4[i]); }
+            for (int i = 0; i < 3; ++i) rd[i] = val(r3[i]);
+            transfer_step(jd, ad, rd, val(D), Od, Ed);
+            for (int i = 0; i < 16; ++i) O[i] = S(Od[i]);
+            for (int i = 0; i < 4; ++i) E[i] = S(Ed[i]);
+        } else {
+            transfer_step(j4, a4, r3, D, O, E);
+        }
+    }
+}
+
+// Delta = (L / nu-hat) Sigma dtau (transport.jl:282-284)
+KS_DEV double path_weight(double L, double nu, double a, double r, double th, double dtau) {
+    double ct = cos(th);
+    return L / (nu * (1.0 / NU0)) * (r * r + a * a * ct * ct) * dtau;
+}
+
+// ---- polarized transport from stored samples (transport.jl:263-290) ----------------------------------------
+// Work items (chunk m, item), item = j + npix (it + nt inu) fastest: samples m CHUNK + 1 .. of the
+// item's ray, composed into the chunk's map seg[(20 m + f) nitems + item]. With rec != nullptr the
+// adjoint records (RecField) of every step-taking sample are written, and aflg[(k - 1) nitems + item]
+// = 1 when the sample takes a step.
+
+template <int KIND, class S>
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(Waves<S>::transfer))) ks_chunk_transfer(double a, double thetao, const double* __restrict__ soa, int64_t npix, int N,
+                                                         Samples smp, const SplatCT<S>* __restrict__ sp, int ns,
+                                                         const S* __restrict__ knots, const S* __restrict__ ktimes, int nknots,
+                                                         const double* __restrict__ times, int nt, const double* __restrict__ nus, int nnu,
+                                                         double L, const int* __restrict__ kstop, int nch,
+                                                         double* __restrict__ seg, S* __restrict__ rec, uint8_t* __restrict__ aflg) {
+    const int64_t nitems = npix * nt * nnu;
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= nitems * nch) return;
+    const int64_t item = tid % nitems;
+    const int m = (int)(tid / nitems);
+    const int64_t j = item % npix;
+    const int64_t fr = item / npix;
+    const int it = (int)(fr % nt), inu = (int)(fr / nt);
+    const Spin Sp = make_spin(a);
+    const double dtau = soa[(int64_t)PF_TAUTOT * npix + j] / (double)(N + 1);
+    const TransCtxT<S> c = make_ctx<S>(a, Sp.rp, Sp.rm, sin(thetao), soa[(int64_t)PF_ALPHA * npix + j], soa[(int64_t)PF_BETA * npix + j],
+                                       soa[(int64_t)PF_ETA * npix + j], soa[(int64_t)PF_LAM * npix + j], sp, ns, knots, ktimes, nknots,
+                                       times[it], nus[inu], L);
+    const int ks = kstop ? kstop[j] : N + 1;
+    const double hor = Sp.rp * (1.0 + 1e-3);
+    const int kbeg = m * CHUNK + 1, kend = kbeg + CHUNK - 1 < N ? kbeg + CHUNK - 1 : N;
+    const int64_t st = (int64_t)N * nitems;
+    SegMapT<S> sm;
+    sm.init();
+    for (int k = kbeg; k <= kend; ++k) {
+        const int64_t o = (int64_t)(k - 1) * npix + j;
+        const uint8_t fl = smp.fl[o];
+        bool step = false;
+        const double r = smp.r[o];
+        if ((fl & 1) && r > hor && k < ks) {
+            const double th = smp.th[o];
+            S j4[4], a4[4], r3[3];
+            if (sample_coefficients<KIND, S>(c, smp.t[o], r, th, (fl & 2) != 0, (fl & 4) != 0, S(smp.x[o]), S(smp.y[o]), S(smp.z[o]),
+                                             j4, a4, r3)) {
+                const S D = S(path_weight(L, nus[inu], a, r, th, dtau));
+                S O[16], E[4];
+                sample_step(j4, a4, r3, D, O, E);
+                sm.advance(O, E);
+                step = true;
+                if (rec) {
+                    S* w = rec + (int64_t)(k - 1) * nitems + item;
+                    for (int i = 0; i < 4; ++i) { w[(REC_J + i) * st] = j4[i]; w[(REC_A + i) * st] = a4[i]; }
+                    for (int i = 0; i < 3; ++i) w[(REC_RHO + i) * st] = r3[i];
+                    w[REC_D * st] = D;
+                }
+            }
+        }
+        if (rec) aflg[(int64_t)(k - 1) * nitems + item] = step ? 1 : 0;
+    }
+    sm.store(seg, m, nitems, item);
+}
+
+// ---- the fused march + transport (caches without stored samples) --------------------------------------------
+template <int KIND>
+struct FusedF {
+    TransCtx c;
+    int kstop;
+    SegMap sm;
+    KS_DEV void operator()(int k, const Samp& s, double dtau) {
+        if (!(s.ok && s.r > c.rp * (1.0 + 1e-3)) || k >= kstop) return;
+        double x, y, z;
+        quasi_cartesian_ks(c.a, c.rp, c.rm, s.r, s.th, s.ph, x, y, z);
+        double j4[4], a4[4], r3[3];
+        if (!sample_coefficients<KIND, double>(c, s.t, s.r, s.th, s.nur, s.nuth, x, y, z, j4, a4, r3)) return;
+        double O[16], E[4];
+        transfer_step(j4, a4, r3, path_weight(c.L, c.nu, c.a, s.r, s.th, dtau), O, E);
+        sm.advance(O, E);
+    }
+};
+// Work items (anchor segment m, item), item fastest; segment m of pixel j covers samples m L + 1 ..
+// min((m + 1) L, N), L = segment_length of the pixel; items past the pixel's last segment exit.
+// Output: seg as ks_chunk_transfer; the segment's anchor residuals res[2 (m npix + j) + {0, 1}] from frame 0.
+template <int KIND>
+__global__ void __launch_bounds__(128) ks_seg_polarized(double a, double thetao, const double* __restrict__ soa, int64_t npix, int N, int M,
+                                                        const SplatC* __restrict__ sp, int ns,
+                                                        const double* __restrict__ knots, const double* __restrict__ ktimes, int nknots,
+                                                        const double* __restrict__ times, int nt, const double* __restrict__ nus, int nnu,
+                                                        double L, const int* __restrict__ kstop, int nseg_max,
+                                                        double* __restrict__ seg, double* __restrict__ res) {
+    const int64_t nitems = npix * nt * nnu;
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= nitems * nseg_max) return;
+    const int64_t item = tid % nitems;
+    const int m = (int)(tid / nitems);
+    const int64_t j = item % npix;
+    const int64_t fr = item / npix;
+    const int it = (int)(fr % nt), inu = (int)(fr / nt);
+    const PixC p = load_pixel(soa, npix, j);
+    const int Lseg = segment_length(M, 1.0 - p.k);
+    const int kbeg = m * Lseg + 1;
+    double r1 = 0.0, r2 = 0.0;
+    if (kbeg <= N) {
+        const int kend = kbeg + Lseg - 1 < N ? kbeg + Lseg - 1 : N;
+        const Spin S = make_spin(a);
+        FusedF<KIND> f;
+        f.c = make_ctx<double>(a, S.rp, S.rm, sin(thetao), p.alpha, p.beta, p.eta, p.lam, sp, ns, knots, ktimes, nknots, times[it], nus[inu], L);
+        f.kstop = kstop ? kstop[j] : N + 1;
+        f.sm.init();
+        const double dtau = p.tau_total / (double)(N + 1);
+        // a segment that starts past the outer horizon holds no valid sample and no valid anchor
+        if (kbeg * dtau <= p.tau_h) march_range(p, S, thetao, N, M, kbeg, kend, f, r1, r2);
+        f.sm.store(seg, m, nitems, item);
+    }
+    if (fr == 0) { res[2 * (m * npix + j)] = r1; res[2 * (m * npix + j) + 1] = r2; }
+}
+
+// ---- thin splats: per chunk (stored samples) or anchor segment (fused) sums --------------------------------------
+struct ThinF {
+    const SplatC* sp; int ns; double a, rp, rm, t_obs, lam, acc;
+    KS_DEV void operator()(int, const Samp& s, double dtau) {
+        if (!(s.ok && rp * (1.0 + 1e-3) < s.r && s.r < 1e3)) return;
+        double x, y, z;
+        quasi_cartesian_ks(a, rp, rm, s.r, s.th, s.ph, x, y, z);
+        acc += thin_sample(sp, ns, a, lam, t_obs, s.t, s.r, s.th, x, y, z, dtau);
+    }
+};
+// part[m npix + j] = the thin sum over segment m of pixel j (0 past its last segment); res as above
+__global__ void __launch_bounds__(128) ks_thin_parts(double a, double thetao, const double* __restrict__ soa, int64_t npix, int N, int M,
+                                                     Samples smp, const SplatC* __restrict__ sp, int ns, double t_obs, int nseg_max,
+                                                     double* __restrict__ part, double* __restrict__ res) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= npix * nseg_max) return;
+    const int64_t j = tid % npix;
+    const int m = (int)(tid / npix);
+    const Spin S = make_spin(a);
+    double acc = 0.0, r1 = 0.0, r2 = 0.0;
+    if (smp.r) {
+        const double lam = soa[(int64_t)PF_LAM * npix + j];
+        const double dtau = soa[(int64_t)PF_TAUTOT * npix + j] / (double)(N + 1);
+        const int kbeg = m * CHUNK + 1, kend = kbeg + CHUNK - 1 < N ? kbeg + CHUNK - 1 : N;
+        for (int k = kbeg; k <= kend; ++k) {
+            const int64_t o = (int64_t)(k - 1) * npix + j;
+            const double r = smp.r[o];
+            if (!((smp.fl[o] & 1) && S.rp * (1.0 + 1e-3) < r && r < 1e3)) continue;
+            acc += thin_sample(sp, ns, a, lam, t_obs, smp.t[o], r, smp.th[o], smp.x[o], smp.y[o], smp.z[o], dtau);
+        }
+    } else {
+        const PixC p = load_pixel(soa, npix, j);
+        const int Lseg = segment_length(M, 1.0 - p.k);
+        const int kbeg = m * Lseg + 1;
+        if (kbeg <= N && kbeg * (p.tau_total / (double)(N + 1)) <= p.tau_h) {
+            const int kend = kbeg + Lseg - 1 < N ? kbeg + Lseg - 1 : N;
+            ThinF f{sp, ns, a, S.rp, S.rm, t_obs, p.lam, 0.0};
+            march_range(p, S, thetao, N, M, kbeg, kend, f, r1, r2);
+            acc = f.acc;
+        }
+        res[2 * (m * npix + j)] = r1; res[2 * (m * npix + j) + 1] = r2;
+    }
+    part[m * npix + j] = acc;
+}
+__global__ void ks_thin_combine(int64_t npix, int nseg_max, const double* __restrict__ part, double* __restrict__ out) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= npix) return;
+    double s = 0.0;
+    for (int m = 0; m < nseg_max; ++m) s += part[m * npix + j];
+    out[j] = s;
+}
+
+// ---- composition of the segment maps ---------------------------------------------------------------------------
+// The number of segments of pixel j: fixed chunks (chunk > 0) or the pixel's anchor segments.
+KS_DEV int item_segments(const double* soa, int64_t npix, int64_t j, int N, int M, int chunk) {
+    const int L = chunk > 0 ? chunk : segment_length(M, 1.0 - soa[(int64_t)PF_K * npix + j]);
+    return (N + L - 1) / L;
+}
+// S = sum_m P_0 ... P_{m-1} S_m, the observed Stokes vector nu-hat^3 S (observed_stokes, transport.jl:311-316)
+__global__ void ks_seg_combine(const double* __restrict__ soa, int64_t npix, int N, int M, int chunk, int nt, const double* __restrict__ nus,
+                               int nnu, const double* __restrict__ seg, double* __restrict__ out) {
+    const int64_t nitems = npix * nt * nnu;
+    const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= nitems) return;
+    const int64_t j = item % npix;
+    const int inu = (int)((item / npix) / nt);
+    const int nseg = item_segments(soa, npix, j, N, M, chunk);
+    double P[16], Sv[4] = {0, 0, 0, 0};
+    for (int i = 0; i < 16; ++i) P[i] = (i % 5 == 0) ? 1.0 : 0.0;
+    for (int m = 0; m < nseg; ++m) {
+        const double* g = seg + (int64_t)SEG_FIELDS * m * nitems + item;
+        double Pm[16], Sm[4];
+        for (int q = 0; q < 16; ++q) Pm[q] = g[q * nitems];
+        for (int q = 0; q < 4; ++q) Sm[q] = g[(16 + q) * nitems];
+        for (int i = 0; i < 4; ++i) Sv[i] += P[4 * i] * Sm[0] + P[4 * i + 1] * Sm[1] + P[4 * i + 2] * Sm[2] + P[4 * i + 3] * Sm[3];
+        double Pn[16];
+        for (int i = 0; i < 4; ++i)
+            for (int q = 0; q < 4; ++q) Pn[4 * i + q] = P[4 * i] * Pm[q] + P[4 * i + 1] * Pm[4 + q] + P[4 * i + 2] * Pm[8 + q] + P[4 * i + 3] * Pm[12 + q];
+        for (int i = 0; i < 16; ++i) P[i] = Pn[i];
+    }
+    const double nh = nus[inu] * (1.0 / NU0);
+    const double s3 = nh * nh * nh;
+    for (int q = 0; q < 4; ++q) out[4 * item + q] = Sv[q] * s3;
+}
+
+// ---- launchers ----
+template <int KIND, class S>
+static void chunk_transfer_k(hipStream_t s, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
+                             const SplatCT<S>* sp, int ns, const S* knots, const S* ktimes, int nknots, const double* times, int nt,
+                             const double* nus, int nnu, double L, const int* kstop, double* seg, S* rec, uint8_t* aflg) {
+    const int nch = (N + CHUNK - 1) / CHUNK;
+    const int64_t items = npix * nt * nnu;
+    hipLaunchKernelGGL((ks_chunk_transfer<KIND, S>), dim3(grid_for(items * nch, 128)), dim3(128), 0, s, a, thetao, soa, npix, N, smp, sp, ns,
+                       knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, nch, seg, rec, aflg);
+}
+template <class S>
+static void chunk_transfer_s(hipStream_t s, int kind, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
+                             const SplatCT<S>* sp, int ns, const S* knots, const S* ktimes, int nknots, const double* times, int nt,
+                             const double* nus, int nnu, double L, const int* kstop, double* seg, S* rec, uint8_t* aflg) {
+    switch (kind) {
+        case MODEL_THERMAL: chunk_transfer_k<MODEL_THERMAL, S>(s, a, thetao, soa, npix, N, smp, sp, ns, knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, seg, rec, aflg); break;
+        case MODEL_POWERLAW: chunk_transfer_k<MODEL_POWERLAW, S>(s, a, thetao, soa, npix, N, smp, sp, ns, knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, seg, rec, aflg); break;
+        case MODEL_KAPPA: chunk_transfer_k<MODEL_KAPPA, S>(s, a, thetao, soa, npix, N, smp, sp, ns, knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, seg, rec, aflg); break;
+        default: chunk_transfer_k<MODEL_KNOT, S>(s, a, thetao, soa, npix, N, smp, sp, ns, knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, seg, rec, aflg); break;
+    }
+}
+hipError_t launch_chunk_transfer(hipStream_t s, int kind, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
+                                 const SplatC* sp, int ns, const double* knots, const double* ktimes, int nknots,
+                                 const double* times, int nt, const double* nus, int nnu, double L, const int* kstop,
+                                 double* seg, double* rec, uint8_t* aflg) {
+    chunk_transfer_s<double>(s, kind, a, thetao, soa, npix, N, smp, sp, ns, knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, seg, rec, aflg);
+    return hipGetLastError();
+}
+hipError_t launch_chunk_transfer_f32(hipStream_t s, int kind, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
+                                     const SplatCf* sp, int ns, const f32* knots, const f32* ktimes, int nknots,
+                                     const double* times, int nt, const double* nus, int nnu, double L, const int* kstop,
+                                     double* seg, f32* rec, uint8_t* aflg) {
+    chunk_transfer_s<f32>(s, kind, a, thetao, soa, npix, N, smp, sp, ns, knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, seg, rec, aflg);
+    return hipGetLastError();
+}
+template <int KIND>
+static void seg_polarized_k(hipStream_t s, double a, double thetao, const double* soa, int64_t npix, int N, int M, const SplatC* sp, int ns,
+                            const double* knots, const double* ktimes, int nknots, const double* times, int nt, const double* nus, int nnu,
+                            double L, const int* kstop, double* seg, double* res) {
+    const int nseg = max_segments(N);
+    const int64_t items = npix * nt * nnu;
+    hipLaunchKernelGGL(ks_seg_polarized<KIND>, dim3(grid_for(items * nseg, 128)), dim3(128), 0, s, a, thetao, soa, npix, N, M, sp, ns,
+                       knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, nseg, seg, res);
+}
+hipError_t launch_seg_polarized(hipStream_t s, int kind, double a, double thetao, const double* soa, int64_t npix, int N, int M,
+                                const SplatC* sp, int ns, const double* knots, const double* ktimes, int nknots,
+                                const double* times, int nt, const double* nus, int nnu, double L, const int* kstop,
+                                double* seg, double* res) {
+    switch (kind) {
+        case MODEL_THERMAL: seg_polarized_k<MODEL_THERMAL>(s, a, thetao, soa, npix, N, M, sp, ns, knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, seg, res); break;
+        case MODEL_POWERLAW: seg_polarized_k<MODEL_POWERLAW>(s, a, thetao, soa, npix, N, M, sp, ns, knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, seg, res); break;
+        case MODEL_KAPPA: seg_polarized_k<MODEL_KAPPA>(s, a, thetao, soa, npix, N, M, sp, ns, knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, seg, res); break;
+        default: seg_polarized_k<MODEL_KNOT>(s, a, thetao, soa, npix, N, M, sp, ns, knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, seg, res); break;
+    }
+    return hipGetLastError();
+}
+hipError_t launch_thin_parts(hipStream_t s, double a, double thetao, const double* soa, int64_t npix, int N, int M, Samples smp,
+                             const SplatC* sp, int ns, double t_obs, int nseg_max, double* part, double* res) {
+    hipLaunchKernelGGL(ks_thin_parts, dim3(grid_for(npix * nseg_max, 128)), dim3(128), 0, s, a, thetao, soa, npix, N, M, smp, sp, ns, t_obs,
+                       nseg_max, part, res);
+    return hipGetLastError();
+}
+hipError_t launch_thin_combine(hipStream_t s, int64_t npix, int nseg_max, const double* part, double* out) {
+    hipLaunchKernelGGL(ks_thin_combine, dim3(grid_for(npix, 256)), dim3(256), 0, s, npix, nseg_max, part, out);
+    return hipGetLastError();
+}
+hipError_t launch_seg_combine(hipStream_t s, const double* soa, int64_t npix, int N, int M, int chunk, int nt, const double* nus, int nnu,
+                              const double* seg, double* out) {
+    const int64_t items = npix * nt * nnu;
+    hipLaunchKernelGGL(ks_seg_combine, dim3(grid_for(items, 128)), dim3(128), 0, s, soa, npix, N, M, chunk, nt, nus, nnu, seg, out);
+    return hipGetLastError();
+}
+
+}  // namespace ksd

@@ -1,0 +1,120 @@
+// Shared layouts of the device buffers and the host launchers of every kernel of libkerrsplat_hip.
+//
+// The kernels live in four translation units compiled separately for gfx950:
+//   ks_k_geo.hip       K0 root cases, the stable case partition, K1 pixel constants, the stored-sample
+//                      march (anchor segments in parallel), the half-orbit cutoff, sorted -> screen
+//   ks_k_transfer.hip  the polarized transfer from stored samples (chunks) or fused with the march
+//                      (anchor segments), thin images, the composition of the segment maps
+//   ks_k_adjoint.hip   the dual-sweep adjoint over stored samples
+//   ks_k_vis.hip       visibilities
+// ks_lib.hip holds the C-ABI and calls the launchers declared here.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "ks_march.hpp"
+#include "ks_dtransfer.hpp"
+#include "ks_dadjoint.hpp"
+
+namespace ksd {
+
+constexpr int PART_BLOCK = 256;    // block of the case partition
+constexpr int CHUNK = 32;          // samples per work item of the transfer from stored samples
+constexpr int SEG_FIELDS = 20;     // a segment map: P (16, row-major), then Sv (4)
+constexpr int ADJ_ROWS = 21;       // parameter rows of a thermal splat
+// Minimum waves per SIMD requested for the heavy kernels (the register budget the compiler may use:
+// 512 / waves VGPRs per lane), per precision; tuned by measurement on MI355X.
+#ifndef KS_WAVES_MARCH
+#define KS_WAVES_MARCH 1
+#endif
+#ifndef KS_WAVES_TRANSFER64
+#define KS_WAVES_TRANSFER64 1
+#endif
+#ifndef KS_WAVES_TRANSFER32
+#define KS_WAVES_TRANSFER32 1
+#endif
+#ifndef KS_WAVES_ADJSTEP64
+#define KS_WAVES_ADJSTEP64 1
+#endif
+#ifndef KS_WAVES_ADJSTEP32
+#define KS_WAVES_ADJSTEP32 1
+#endif
+#ifndef KS_WAVES_ADJELEM64
+#define KS_WAVES_ADJELEM64 1
+#endif
+#ifndef KS_WAVES_ADJELEM32
+#define KS_WAVES_ADJELEM32 1
+#endif
+template <class S> struct Waves;
+template <> struct Waves<double> { static constexpr int transfer = KS_WAVES_TRANSFER64, adjstep = KS_WAVES_ADJSTEP64, adjelem = KS_WAVES_ADJELEM64; };
+template <> struct Waves<f32> { static constexpr int transfer = KS_WAVES_TRANSFER32, adjstep = KS_WAVES_ADJSTEP32, adjelem = KS_WAVES_ADJELEM32; };
+
+// FP32 transfer and adjoint: samples whose Faraday depth |rho| Delta exceeds this take the FP64 step
+// (the step's polynomial in K' cancels strongly at large Faraday depth)
+constexpr double FARADAY_FP64 = 10.0;
+
+// Stored samples, sample-major o = (k - 1) npix + j: t~, r, theta, phi and flags (bit 0 ok, 1 nu_r,
+// 2 nu_theta) as the reference stores them (direct_march.jl:14-17), plus the sample's quasi-Cartesian
+// Kerr-Schild position (coordinates.jl:22-31), which every render of the samples needs.
+struct Samples {
+    double *t, *r, *th, *ph, *x, *y, *z;
+    uint8_t* fl;
+};
+
+// Adjoint records of a step-taking sample: rec[(f N + k - 1) nitems + item], f = the invariant j (4),
+// alpha (4), rho (3), Delta; the adjoint sweep writes j_bar, alpha_bar, rho_bar over j, alpha, rho.
+enum RecField { REC_J, REC_A = REC_J + 4, REC_RHO = REC_A + 4, REC_D = REC_RHO + 3, REC_FIELDS };
+constexpr int REC_BAR = REC_J;
+
+// ---- ks_k_geo.hip ----
+hipError_t launch_root_case(hipStream_t s, double a, double thetao, const double* al, const double* be, int64_t npix, int8_t* numreals);
+hipError_t launch_case_count(hipStream_t s, int nblocks, const int8_t* numreals, int64_t npix, int* bcount);
+hipError_t launch_case_scatter(hipStream_t s, int nblocks, const int8_t* numreals, int64_t npix, const int* boff, int* perm);
+hipError_t launch_pixel_constants(hipStream_t s, double a, double thetao, const double* al, const double* be, const int* perm, int64_t npix,
+                                  double* soa);
+hipError_t launch_march_store(hipStream_t s, double a, double thetao, const double* soa, int64_t npix, int N, int M, int marcher, int nseg_max,
+                              Samples smp, double* res);
+hipError_t launch_res_reduce(hipStream_t s, int64_t npix, int nseg_max, const double* res, double* rt, double* rp);
+hipError_t launch_wind_cutoff(hipStream_t s, double a, double thetao, const double* soa, int64_t npix, int N, int M, int nmax, double slab,
+                              Samples smp, int* kstop);
+hipError_t launch_unsort4(hipStream_t s, const double* src, const int* perm, int64_t npix, int64_t nframes, double* dst);
+hipError_t launch_unsort1(hipStream_t s, const double* src, const int* perm, int64_t npix, double* dst);
+
+// ---- ks_k_transfer.hip ----
+hipError_t launch_chunk_transfer(hipStream_t s, int kind, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
+                                 const SplatC* sp, int ns, const double* knots, const double* ktimes, int nknots,
+                                 const double* times, int nt, const double* nus, int nnu, double L, const int* kstop,
+                                 double* seg, double* rec, uint8_t* aflg);
+hipError_t launch_chunk_transfer_f32(hipStream_t s, int kind, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
+                                     const SplatCf* sp, int ns, const f32* knots, const f32* ktimes, int nknots,
+                                     const double* times, int nt, const double* nus, int nnu, double L, const int* kstop,
+                                     double* seg, f32* rec, uint8_t* aflg);
+hipError_t launch_seg_polarized(hipStream_t s, int kind, double a, double thetao, const double* soa, int64_t npix, int N, int M,
+                                const SplatC* sp, int ns, const double* knots, const double* ktimes, int nknots,
+                                const double* times, int nt, const double* nus, int nnu, double L, const int* kstop,
+                                double* seg, double* res);
+hipError_t launch_thin_parts(hipStream_t s, double a, double thetao, const double* soa, int64_t npix, int N, int M, Samples smp,
+                             const SplatC* sp, int ns, double t_obs, int nseg_max, double* part, double* res);
+hipError_t launch_thin_combine(hipStream_t s, int64_t npix, int nseg_max, const double* part, double* out);
+hipError_t launch_seg_combine(hipStream_t s, const double* soa, int64_t npix, int N, int M, int chunk, int nt, const double* nus, int nnu,
+                              const double* seg, double* out);
+
+// ---- ks_k_adjoint.hip ----
+hipError_t launch_adj_combine(hipStream_t s, int64_t npix, int N, int nt, const double* nus, int nnu, const double* seg,
+                              const double* dstokes, double* wT, double* out);
+hipError_t launch_adj_segment(hipStream_t s, int N, int64_t nitems, const double* wT, double* rec, const uint8_t* aflg, double* tails);
+hipError_t launch_adj_segment_f32(hipStream_t s, int N, int64_t nitems, const double* wT, f32* rec, const uint8_t* aflg, f32* tails);
+hipError_t launch_adj_splats_f32(hipStream_t s, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
+                                 const SplatCf* sp, const SplatAdjf* sadj, int ns, const double* times, int nt, const double* nus,
+                                 int64_t nitems, const f32* rec, const uint8_t* aflg, double* part);
+hipError_t launch_adj_splats(hipStream_t s, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
+                             const SplatC* sp, const SplatAdj* sadj, int ns, const double* times, int nt, const double* nus,
+                             int64_t nitems, const double* rec, const uint8_t* aflg, double* part);
+hipError_t launch_adj_reduce(hipStream_t s, const double* part, int64_t nib, int ns, double* dparams);
+
+// ---- ks_k_vis.hip ----
+hipError_t launch_vis_direct(hipStream_t s, const double* img, const int* perm, int64_t npix, int nalpha, int nbeta, int nframes,
+                             double psize, double scale, const double* u, const double* v, const double* tap, int nuv, double* V);
+
+inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+
+}  // namespace ksd

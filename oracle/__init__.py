@@ -226,13 +226,28 @@ def vis_seed(W, nalpha, nbeta, psize, u, v, taper=None):
     return seed
 
 
-def polarized_gradient(a, thetao, alphas, betas, N, params, t_obs, nu, L, dstokes, nmax=-1, slab=0.0, M=64):
-    """The dual-sweep gradient (thermal splats): returns (dparams 21 x ns, image npix x 4)."""
+def count_gradient(a, thetao, alphas, betas, N, params, t_obs, nu, L, dstokes, M=64):
+    """Operation counts of the dual sweep on the Counted scalar, per phase (march, forward steps,
+    tails, adjoint sweep, and the sweep's per-sample step adjoints): returns (flops[5], special[5])
+    summed over the rays."""
+    al, be = _f64(alphas), _f64(betas)
+    P = np.asfortranarray(params, dtype=np.float64)
+    ds = _f64(dstokes)
+    c, sp = np.zeros(5), np.zeros(5)
+    lib().kso_count_gradient(_d(a), _d(thetao), _i(len(al)), _p(al), _p(be), _i(N), _i(M), _i(P.shape[1]), _p(P),
+                             _d(t_obs), _d(nu), _d(L), _p(ds), _p(c), _p(sp))
+    return c, sp
+
+
+def polarized_gradient(a, thetao, alphas, betas, N, params, t_obs, nu, L, dstokes, nmax=-1, slab=0.0, M=64, analytic_weight=False):
+    """The dual-sweep gradient (thermal splats): returns (dparams 21 x ns, image npix x 4). The splat
+    weight's derivative by 13-partial duals, or written out by hand with analytic_weight."""
     al, be = _f64(alphas), _f64(betas)
     P = np.asfortranarray(params, dtype=np.float64)
     ds = _f64(dstokes)
     g = np.zeros(P.shape, order="F")
     img = np.zeros((len(al), 4))
     lib().kso_polarized_gradient(_d(a), _d(thetao), _i(len(al)), _p(al), _p(be), _i(N), _i(M), _i(P.shape[1]), _p(P),
-                                 _d(t_obs), _d(nu), _d(L), _i(nmax), _d(slab), _p(ds), g.ctypes.data_as(_P), _p(img))
+                                 _d(t_obs), _d(nu), _d(L), _i(nmax), _d(slab), _p(ds), g.ctypes.data_as(_P), _p(img),
+                                 _i(int(analytic_weight)))
     return g, img

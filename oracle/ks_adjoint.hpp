@@ -16,53 +16,131 @@
 
 namespace kso {
 
-using D7 = Dual<7>;
-using D13 = Dual<13>;
-
-// sample_adjoint (transport.jl:332-341)
-inline void sample_adjoint(const double j4[4], const double a4[4], const double r3[3], double D, const double w[4], const double R[4],
-                           M4<double>& O, double E[4], double jbar[4], double abar[4], double rbar[3]) {
+// sample_adjoint (transport.jl:332-341); S = double, or Counted to measure the work. The 7 partials of
+// q = w . (O R + Ej j) with respect to (alpha, rho) come from duals through the step's polynomial
+// coefficients, contracted as sum_n o_n w.(K'^n R) + Dp e_n w.(K'^n j) (the same function as the
+// ForwardDiff derivative of the assembled matrices, without forming 4x4 dual matrices). Returns
+// O^T w (the next adjoint), E = Ej j, j_bar = Ej^T w, alpha_bar and rho_bar.
+template <class S>
+inline void sample_adjoint(const S j4[4], const S a4[4], const S r3[3], S D, const S w[4], const S R[4],
+                           S Otw[4], S E[4], S jbar[4], S abar[4], S rbar[3]) {
+    using D7 = Dual<7, S>;
     D7 ad[4], rd[3];
     for (int i = 0; i < 4; ++i) ad[i] = D7::seed(a4[i], i);
     for (int i = 0; i < 3; ++i) rd[i] = D7::seed(r3[i], 4 + i);
-    M4<D7> Od, Ejd;
-    step_operator<D7>(ad, rd, D7(D), Od, Ejd);
-    D7 q(0.0);
-    for (int i = 0; i < 4; ++i) {
-        D7 oR(0.0), ej(0.0);
-        for (int k = 0; k < 4; ++k) { oR = oR + Od.m[i][k] * R[k]; ej = ej + Ejd.m[i][k] * j4[k]; }
-        q = q + w[i] * (oR + ej);
-    }
-    for (int i = 0; i < 4; ++i) for (int k = 0; k < 4; ++k) O.m[i][k] = Od.m[i][k].v;
-    for (int i = 0; i < 4; ++i) {
-        double e = 0.0;
-        for (int k = 0; k < 4; ++k) e += Ejd.m[i][k].v * j4[k];
-        E[i] = e;
-    }
-    for (int k = 0; k < 4; ++k) {
-        double s = 0.0;
-        for (int i = 0; i < 4; ++i) s += Ejd.m[i][k].v * w[i];
-        jbar[k] = s;
+    D7 Dp, K6[6], o[4], e4[4];
+    step_coefficients<D7>(ad, rd, D7(D), Dp, K6, o, e4);
+    S wR(0.0), wj(0.0);
+    for (int i = 0; i < 4; ++i) { wR = wR + w[i] * R[i]; wj = wj + w[i] * j4[i]; }
+    D7 q = o[0] * D7(wR) + Dp * e4[0] * D7(wj);
+    D7 vR[4], vj[4], t4[4];
+    for (int i = 0; i < 4; ++i) { vR[i] = D7(R[i]); vj[i] = D7(j4[i]); }
+    for (int n = 1; n < 4; ++n) {
+        kprime_mul(K6, vR, t4); for (int i = 0; i < 4; ++i) vR[i] = t4[i];
+        kprime_mul(K6, vj, t4); for (int i = 0; i < 4; ++i) vj[i] = t4[i];
+        D7 sR = vR[0] * D7(w[0]) + vR[1] * D7(w[1]) + vR[2] * D7(w[2]) + vR[3] * D7(w[3]);
+        D7 sj = vj[0] * D7(w[0]) + vj[1] * D7(w[1]) + vj[2] * D7(w[2]) + vj[3] * D7(w[3]);
+        q = q + o[n] * sR + Dp * e4[n] * sj;
     }
     for (int i = 0; i < 4; ++i) abar[i] = q.d[i];
     for (int i = 0; i < 3; ++i) rbar[i] = q.d[4 + i];
+    // values: O^T w, Ej^T w and E = Ej j by repeated products with K'
+    S K6v[6], u[4], y[4], z[4], v[4];
+    for (int i = 0; i < 6; ++i) K6v[i] = K6[i].v;
+    const S Dpv = Dp.v;
+    for (int i = 0; i < 4; ++i) {
+        u[i] = w[i]; v[i] = j4[i];
+        Otw[i] = o[0].v * w[i]; jbar[i] = Dpv * e4[0].v * w[i]; E[i] = Dpv * e4[0].v * j4[i];
+    }
+    for (int n = 1; n < 4; ++n) {
+        kprime_tmul(K6v, u, y); kprime_mul(K6v, v, z);
+        for (int i = 0; i < 4; ++i) {
+            u[i] = y[i]; v[i] = z[i];
+            Otw[i] = Otw[i] + o[n].v * u[i]; jbar[i] = jbar[i] + Dpv * e4[n].v * u[i]; E[i] = E[i] + Dpv * e4[n].v * v[i];
+        }
+    }
 }
 
-// one thermal splat's share of sample s (dual_gradient.jl:46-76); grad is the splat's 21-row column
-inline void element_adjoint(const Model& m, int i, const RayCtx& rc, const Sample<double>& s, double t_obs, double nu,
-                            const double jbar[4], const double abar[4], const double rbar[3], double* grad) {
-    double x, y, z;
-    quasi_cartesian_ks(rc.a, s.r, s.th, s.ph, x, y, z);
-    double t = t_obs - s.t;
-    if (outside_support(m, i, t, x, y, z)) return;
-    // weight with 13 partials: rows 0..11 and the pattern rate (last row)
+// d ln G of the splat weight (Splats.jl:58-93) with respect to rows 1-12 and the pattern rate written
+// out by hand: ln G = -(|q|^2 + tt^2)/2 with q_b = u_b e^{-s_b}, u = R(quaternion)^T e,
+// e = Rz(-ph) (X - c(ph)), c(ph) = Rz(ph) (c0, c1) + c2 e_z, ph = omega (t - t0), tt = (t - t0) e^{-s_t}.
+// The same function as the 13-partial duals (checked against them in tests/test_oracle_adjoint.py);
+// it is how the GPU evaluates it, and the work model counts this form. Returns G (0 when culled).
+template <class S>
+inline S weight_dlog_analytic(const Model& m, int i, S t, S x, S y, S z, S dl[13]) {
+    if (outside_support(m, i, t, x, y, z)) return S(0.0);
+    const int nr = m.nrows;
+    const double c0 = m.P(0, i), c1 = m.P(1, i), c2 = m.P(2, i), t0 = m.P(10, i), om = m.P(nr - 1, i);
+    S ph = om * (t - t0);
+    S sp = ksin(ph), cp = kcos(ph);
+    S cx = c0 * cp - c1 * sp, cy = c0 * sp + c1 * cp;
+    S d0 = x - cx, d1 = y - cy, d2 = z - c2;
+    S e[3] = {d0 * cp + d1 * sp, -d0 * sp + d1 * cp, d2};
+    S R[3][3];
+    quat_rot(S(m.P(6, i)), S(m.P(7, i)), S(m.P(8, i)), S(m.P(9, i)), R);
+    S q[3], is[3];
+    for (int b = 0; b < 3; ++b) {
+        is[b] = kexp(S(-m.P(3 + b, i)));
+        q[b] = (R[0][b] * e[0] + R[1][b] * e[1] + R[2][b] * e[2]) * is[b];
+    }
+    S iw = kexp(S(-m.P(11, i)));
+    S dt = t - t0, tt = dt * iw;
+    S G = kexp(-(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + tt * tt) / 2.0);
+    if (!(G > WEIGHT_CUTOFF)) return S(0.0);
+    S gu[3], ge[3];
+    for (int b = 0; b < 3; ++b) { gu[b] = -q[b] * is[b]; dl[3 + b] = q[b] * q[b]; }
+    for (int a = 0; a < 3; ++a) ge[a] = R[a][0] * gu[0] + R[a][1] * gu[1] + R[a][2] * gu[2];
+    // rotation: d ln G / dR[a][b] = e_a gu_b, through the normalised quaternion (w, x, y, z)
+    S GR[9];
+    for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) GR[3 * a + b] = e[a] * gu[b];
+    double qq[4] = {m.P(6, i), m.P(7, i), m.P(8, i), m.P(9, i)};
+    double nq = std::sqrt(qq[0] * qq[0] + qq[1] * qq[1] + qq[2] * qq[2] + qq[3] * qq[3]);
+    S qw = S(qq[0] / nq), qx = S(qq[1] / nq), qy = S(qq[2] / nq), qz = S(qq[3] / nq);
+    S gn[4];
+    gn[0] = 2.0 * (-qz * GR[1] + qy * GR[2] + qz * GR[3] - qx * GR[5] - qy * GR[6] + qx * GR[7]);
+    gn[1] = 2.0 * (qy * GR[1] + qz * GR[2] + qy * GR[3] - 2.0 * qx * GR[4] - qw * GR[5] + qz * GR[6] + qw * GR[7] - 2.0 * qx * GR[8]);
+    gn[2] = 2.0 * (-2.0 * qy * GR[0] + qx * GR[1] + qw * GR[2] + qx * GR[3] + qz * GR[5] - qw * GR[6] + qz * GR[7] - 2.0 * qy * GR[8]);
+    gn[3] = 2.0 * (-2.0 * qz * GR[0] - qw * GR[1] + qx * GR[2] + qw * GR[3] - 2.0 * qz * GR[4] + qy * GR[5] + qx * GR[6] + qy * GR[7]);
+    S ng = qw * gn[0] + qx * gn[1] + qy * gn[2] + qz * gn[3];
+    S qn[4] = {qw, qx, qy, qz};
+    for (int k = 0; k < 4; ++k) dl[6 + k] = (gn[k] - qn[k] * ng) / nq;
+    // centre: d ln G / dc = -Rz(ph) ge
+    S gd0 = ge[0] * cp - ge[1] * sp, gd1 = ge[0] * sp + ge[1] * cp;
+    dl[0] = -(gd0 * cp + gd1 * sp);
+    dl[1] = -(gd1 * cp - gd0 * sp);
+    dl[2] = -ge[2];
+    // pattern phase: de/dph = (e1 + c1, -(e0 + c0), 0)
+    S dph = ge[0] * (e[1] + c1) - ge[1] * (e[0] + c0);
+    dl[10] = -om * dph + tt * iw;
+    dl[11] = tt * tt;
+    dl[12] = dph * dt;
+    return G;
+}
+
+// one thermal splat's share of sample s (dual_gradient.jl:46-76); grad is the splat's 21-row column.
+// ANALYTIC selects the hand-written d ln G instead of the 13-partial duals.
+template <class S, bool ANALYTIC = false>
+inline void element_adjoint(const Model& m, int i, const RayCtx& rc, const Sample<S>& s, S t_obs, S nu,
+                            const S jbar[4], const S abar[4], const S rbar[3], S* grad) {
+    using D7 = Dual<7, S>;
+    using D13 = Dual<13, S>;
+    S x, y, z;
+    quasi_cartesian_ks(S(rc.a), s.r, s.th, s.ph, x, y, z);
+    S t = t_obs - s.t;
     const int nr = m.nrows;
     std::vector<double> colv(nr);
     for (int r = 0; r < nr; ++r) colv[r] = m.P(r, i);
+    S G, dlnG[13];
+    if constexpr (ANALYTIC) {
+        G = weight_dlog_analytic(m, i, t, x, y, z, dlnG);
+        if (!(G > 0.0)) return;
+    } else {
+    if (outside_support(m, i, t, x, y, z)) return;
+    // weight with 13 partials: rows 0..11 and the pattern rate (last row)
     D13 pr[32];
     for (int r = 0; r < nr; ++r) {
         int k = r <= 11 ? r : (r == nr - 1 ? 12 : -1);
-        pr[r] = k >= 0 ? D13::seed(colv[r], k) : D13(colv[r]);
+        pr[r] = k >= 0 ? D13::seed(S(colv[r]), k) : D13(colv[r]);
     }
     // pattern offset and Gaussian in duals (Splats.jl:58-93)
     D13 ph = pr[nr - 1] * (D13(t) - pr[10]);
@@ -78,13 +156,13 @@ inline void element_adjoint(const Model& m, int i, const RayCtx& rc, const Sampl
     D13 q0 = u0 * kexp(-pr[3]), q1 = u1 * kexp(-pr[4]), q2 = u2 * kexp(-pr[5]);
     D13 tt = (D13(t) - pr[10]) * kexp(-pr[11]);
     D13 Gd = kexp(-(q0 * q0 + q1 * q1 + q2 * q2 + tt * tt) / 2.0);
-    double G = Gd.v;
+    G = Gd.v;
     if (!(G > WEIGHT_CUTOFF)) return;
-    double dlnG[13];
     for (int k = 0; k < 13; ++k) dlnG[k] = Gd.d[k] / G;
+    }
     // fluid rows with 7 partials (rows 13..19)
     D7 fl[7];
-    for (int k = 0; k < 7; ++k) fl[k] = D7::seed(colv[13 + k], k);
+    for (int k = 0; k < 7; ++k) fl[k] = D7::seed(S(colv[13 + k]), k);
     D7 Th = kexp(fl[0]);
     D7 Bmag = kexp(fl[1]);
     D7 st = ksin(fl[2]), ct = kcos(fl[2]), sph = ksin(fl[3]), cph = kcos(fl[3]);
@@ -97,84 +175,94 @@ inline void element_adjoint(const Model& m, int i, const RayCtx& rc, const Sampl
     // Bessel K0, K1 at 1/Theta_e with their derivatives (K0' = -K1, K1' = -K0 - K1/x)
     const double* kb = m.bessel + 3 * i;
     D7 xb = D7(1.0) / Th;
-    D7 k0 = chain(kb[0], -kb[1], xb), k1 = chain(kb[1], -kb[0] - kb[1] / xb.v, xb);
+    D7 k0 = chain(S(kb[0]), S(-kb[1]), xb), k1 = chain(S(kb[1]), S(-kb[0] - kb[1] / xb.v), xb);
     D7 k2 = k0 + 2.0 * k1 / xb;
-    double ne = std::exp(colv[12]) * G;
+    S ne = kexp(S(colv[12])) * G;
     Coef<D7> cf = thermal_synchrotron_k<D7>(D7(ne), Th, Bmag, nuf, thB, k0, k1, k2);
     Screen<D7> sc;
     if (!screen_coefficients<D7>(cf, fr, D7(nu), sc)) return;
     D7 q(0.0);
-    for (int k = 0; k < 4; ++k) q = q + jbar[k] * sc.j[k] + abar[k] * sc.a[k];
-    for (int k = 0; k < 3; ++k) q = q + rbar[k] * sc.r[k];
-    double sv = q.v;
-    for (int r = 0; r < 12; ++r) grad[r] += sv * dlnG[r];
-    grad[12] += sv;
-    for (int k = 0; k < 7; ++k) grad[13 + k] += q.d[k];
-    grad[nr - 1] += sv * dlnG[12];
+    for (int k = 0; k < 4; ++k) q = q + D7(jbar[k]) * sc.j[k] + D7(abar[k]) * sc.a[k];
+    for (int k = 0; k < 3; ++k) q = q + D7(rbar[k]) * sc.r[k];
+    S sv = q.v;
+    for (int r = 0; r < 12; ++r) grad[r] = grad[r] + sv * dlnG[r];
+    grad[12] = grad[12] + sv;
+    for (int k = 0; k < 7; ++k) grad[13 + k] = grad[13 + k] + q.d[k];
+    grad[nr - 1] = grad[nr - 1] + sv * dlnG[12];
 }
 
 // Gradient of sum_pixels dstokes . observed_stokes over one ray (thermal splats); adds into grad
-// (21 x ns, column-major) and returns the ray's observed Stokes vector.
+// (21 x ns, column-major) and returns the ray's observed Stokes vector. With S = Counted, phases (if
+// given) receives the operation counts at the ends of the march, the forward steps, the tails and the
+// adjoint sweep, and phases[4] the part of the sweep spent in sample_adjoint.
+template <class S, bool ANALYTIC = false>
 inline void polarized_gradient_ray(const Model& m, double a, double thetao, double alpha, double beta, int N, int M, double t_obs, double nu,
-                                   double L, int nmax, double slab, const double dstokes[4], double* grad, double image[4]) {
-    Pixel<double> p = pixel_constants<double>(a, alpha, beta, thetao);
-    RayCtx rc{a, alpha, beta, thetao, p.eta, p.lam};
-    std::vector<Sample<double>> S(N);
-    double dtau = 0, rt, rp;
-    auto store = [&](int k, const Sample<double>& s, double dt) { S[k - 1] = s; dtau = dt; };
+                                   double L, int nmax, double slab, const double dstokes[4], S* grad, S image[4], OpCounts* phases = nullptr) {
+    Pixel<S> p = pixel_constants<S>(S(a), S(alpha), S(beta), S(thetao));
+    RayCtx rc{a, alpha, beta, thetao, val(p.eta), val(p.lam)};
+    std::vector<Sample<S>> Sm(N);
+    S dtau(0.0), rt, rp;
+    auto store = [&](int k, const Sample<S>& s, S dt) { Sm[k - 1] = s; dtau = dt; };
     march_ray(p, N, M, store, rt, rp);
+    if (phases) phases[0] = op_counts();
     double hor = (1.0 + std::sqrt(1.0 - a * a)) * (1.0 + 1e-3);
     // half-orbit cutoff: first sample beyond the nmax-th passage (dual_gradient.jl:81-97)
     int kstop = N + 1;
     if (nmax >= 0) {
-        Winding<double> w = winding_zero<double>();
+        Winding<S> w = winding_zero<S>();
         for (int k = 1; k <= N; ++k) {
-            const Sample<double>& s = S[k - 1];
+            const Sample<S>& s = Sm[k - 1];
             if (!(s.ok && s.r > hor)) continue;
-            wind(w, s.r * std::cos(s.th), slab);
+            wind(w, s.r * kcos(s.th), S(slab));
             if (w.n > nmax) { kstop = k; break; }
         }
     }
     // per-sample steps, then tails back to front (dual_gradient.jl:129-147)
-    struct Step { bool on; double j4[4], a4[4], r3[3], D; M4<double> O; double E[4]; };
+    struct Step { bool on; S j4[4], a4[4], r3[3], D; M4<S> O; S E[4]; };
     std::vector<Step> st(N);
+    const S tS(t_obs), nuS(nu);
     for (int k = 1; k <= N; ++k) {
         Step& q = st[k - 1];
         q.on = false;
         if (k >= kstop) continue;
-        const Sample<double>& s = S[k - 1];
+        const Sample<S>& s = Sm[k - 1];
         if (!(s.ok && s.r > hor)) continue;
-        if (!accumulate_elements(m, rc, s, t_obs, nu, q.j4, q.a4, q.r3)) continue;
-        double c = std::cos(s.th);
+        if (!accumulate_elements(m, rc, s, tS, nuS, q.j4, q.a4, q.r3)) continue;
+        S c = kcos(s.th);
         q.D = L / (nu / NU0) * (s.r * s.r + a * a * c * c) * dtau;
-        M4<double> Ej;
+        M4<S> Ej;
         step_operator(q.a4, q.r3, q.D, q.O, Ej);
         for (int i = 0; i < 4; ++i) q.E[i] = Ej.m[i][0] * q.j4[0] + Ej.m[i][1] * q.j4[1] + Ej.m[i][2] * q.j4[2] + Ej.m[i][3] * q.j4[3];
         q.on = true;
     }
-    std::vector<double> tails(4 * (size_t)(N + 1), 0.0);
+    if (phases) phases[1] = op_counts();
+    std::vector<S> tails(4 * (size_t)(N + 1), S(0.0));
     for (int k = N; k >= 1; --k) {
-        double* Tk = &tails[4 * (k - 1)];
-        const double* Tn = &tails[4 * k];
+        S* Tk = &tails[4 * (k - 1)];
+        const S* Tn = &tails[4 * k];
         const Step& q = st[k - 1];
         if (q.on) for (int i = 0; i < 4; ++i) Tk[i] = q.E[i] + q.O.m[i][0] * Tn[0] + q.O.m[i][1] * Tn[1] + q.O.m[i][2] * Tn[2] + q.O.m[i][3] * Tn[3];
         else for (int i = 0; i < 4; ++i) Tk[i] = Tn[i];
     }
     double nh3 = std::pow(nu / NU0, 3);
     for (int i = 0; i < 4; ++i) image[i] = tails[i] * nh3;
+    if (phases) phases[2] = op_counts();
     // forward adjoint sweep (dual_gradient.jl:151-178)
-    double w[4];
-    for (int i = 0; i < 4; ++i) w[i] = dstokes[i] * nh3;
+    S w[4];
+    for (int i = 0; i < 4; ++i) w[i] = S(dstokes[i] * nh3);
+    OpCounts in_sample;
     for (int k = 1; k <= N; ++k) {
         const Step& q = st[k - 1];
         if (!q.on) continue;
-        M4<double> O; double E[4], jb[4], ab[4], rb[3];
-        sample_adjoint(q.j4, q.a4, q.r3, q.D, w, &tails[4 * k], O, E, jb, ab, rb);
-        for (int i = 0; i < m.ns; ++i) element_adjoint(m, i, rc, S[k - 1], t_obs, nu, jb, ab, rb, grad + (size_t)m.nrows * i);
-        double wn[4];
-        for (int c = 0; c < 4; ++c) wn[c] = O.m[0][c] * w[0] + O.m[1][c] * w[1] + O.m[2][c] * w[2] + O.m[3][c] * w[3];
-        for (int c = 0; c < 4; ++c) w[c] = wn[c];
+        S Otw[4], E[4], jb[4], ab[4], rb[3];
+        const OpCounts before = op_counts();
+        sample_adjoint<S>(q.j4, q.a4, q.r3, q.D, w, &tails[4 * k], Otw, E, jb, ab, rb);
+        in_sample.flops += op_counts().flops - before.flops;
+        in_sample.special += op_counts().special - before.special;
+        for (int i = 0; i < m.ns; ++i) element_adjoint<S, ANALYTIC>(m, i, rc, Sm[k - 1], tS, nuS, jb, ab, rb, grad + (size_t)m.nrows * i);
+        for (int c = 0; c < 4; ++c) w[c] = Otw[c];
     }
+    if (phases) { phases[3] = op_counts(); phases[4] = in_sample; }
 }
 
 }  // namespace kso

@@ -1,46 +1,53 @@
 // Forward-mode dual numbers for the oracle's adjoint (test infrastructure; see ks_scalar.hpp).
-// Dual<N> carries a value and N partials; the oracle's templated functions (step operator,
-// coefficients, frames, splat weight) run on it unchanged, which is how the reference computes the
-// per-sample local derivatives of its dual sweep (ForwardDiff duals, dual_gradient.jl:1-19).
+// Dual<N, S> carries a value and N partials of the base scalar S (double, or Counted to measure the
+// adjoint's algorithmic work); the oracle's templated functions (step operator, coefficients, frames,
+// splat weight) run on it unchanged, which is how the reference computes the per-sample local
+// derivatives of its dual sweep (ForwardDiff duals, dual_gradient.jl:1-19).
 #pragma once
+#include <type_traits>
 #include "ks_scalar.hpp"
 
 namespace kso {
 
-template <int N> struct Dual {
-    double v = 0;
-    double d[N] = {};
-    Dual() = default;
-    Dual(double x) : v(x) {}
-    static Dual seed(double x, int i) { Dual r(x); r.d[i] = 1.0; return r; }
+template <class S> struct ident { using type = S; };
+
+template <int N, class S = double> struct Dual {
+    S v;
+    S d[N];
+    Dual() : v(0.0) { for (int i = 0; i < N; ++i) d[i] = S(0.0); }
+    Dual(double x) : v(x) { for (int i = 0; i < N; ++i) d[i] = S(0.0); }
+    template <class X, class = std::enable_if_t<std::is_same_v<X, S> && !std::is_same_v<S, double>>>
+    Dual(X x) : v(x) { for (int i = 0; i < N; ++i) d[i] = S(0.0); }
+    static Dual seed(S x, int i) { Dual r; r.v = x; r.d[i] = S(1.0); return r; }
 };
-template <int N> double val(const Dual<N>& x) { return x.v; }
+template <int N, class S> double val(const Dual<N, S>& x) { return val(x.v); }
 
 #define KSD_LOOP for (int i = 0; i < N; ++i)
-template <int N> Dual<N> operator+(const Dual<N>& a, const Dual<N>& b) { Dual<N> r(a.v + b.v); KSD_LOOP r.d[i] = a.d[i] + b.d[i]; return r; }
-template <int N> Dual<N> operator-(const Dual<N>& a, const Dual<N>& b) { Dual<N> r(a.v - b.v); KSD_LOOP r.d[i] = a.d[i] - b.d[i]; return r; }
-template <int N> Dual<N> operator*(const Dual<N>& a, const Dual<N>& b) { Dual<N> r(a.v * b.v); KSD_LOOP r.d[i] = a.d[i] * b.v + a.v * b.d[i]; return r; }
-template <int N> Dual<N> operator/(const Dual<N>& a, const Dual<N>& b) {
-    Dual<N> r(a.v / b.v); double ib = 1.0 / b.v;
+#define KSD_T template <int N, class S>
+KSD_T Dual<N, S> operator+(const Dual<N, S>& a, const Dual<N, S>& b) { Dual<N, S> r; r.v = a.v + b.v; KSD_LOOP r.d[i] = a.d[i] + b.d[i]; return r; }
+KSD_T Dual<N, S> operator-(const Dual<N, S>& a, const Dual<N, S>& b) { Dual<N, S> r; r.v = a.v - b.v; KSD_LOOP r.d[i] = a.d[i] - b.d[i]; return r; }
+KSD_T Dual<N, S> operator*(const Dual<N, S>& a, const Dual<N, S>& b) { Dual<N, S> r; r.v = a.v * b.v; KSD_LOOP r.d[i] = a.d[i] * b.v + a.v * b.d[i]; return r; }
+KSD_T Dual<N, S> operator/(const Dual<N, S>& a, const Dual<N, S>& b) {
+    Dual<N, S> r; r.v = a.v / b.v; S ib = 1.0 / b.v;
     KSD_LOOP r.d[i] = (a.d[i] - r.v * b.d[i]) * ib;
     return r;
 }
-template <int N> Dual<N> operator+(const Dual<N>& a, double b) { Dual<N> r = a; r.v += b; return r; }
-template <int N> Dual<N> operator+(double a, const Dual<N>& b) { Dual<N> r = b; r.v += a; return r; }
-template <int N> Dual<N> operator-(const Dual<N>& a, double b) { Dual<N> r = a; r.v -= b; return r; }
-template <int N> Dual<N> operator-(double a, const Dual<N>& b) { Dual<N> r(a - b.v); KSD_LOOP r.d[i] = -b.d[i]; return r; }
-template <int N> Dual<N> operator*(const Dual<N>& a, double b) { Dual<N> r(a.v * b); KSD_LOOP r.d[i] = a.d[i] * b; return r; }
-template <int N> Dual<N> operator*(double a, const Dual<N>& b) { return b * a; }
-template <int N> Dual<N> operator/(const Dual<N>& a, double b) { return a * (1.0 / b); }
-template <int N> Dual<N> operator/(double a, const Dual<N>& b) { Dual<N> r(a / b.v); double f = -r.v / b.v; KSD_LOOP r.d[i] = f * b.d[i]; return r; }
-template <int N> Dual<N> operator-(const Dual<N>& a) { Dual<N> r(-a.v); KSD_LOOP r.d[i] = -a.d[i]; return r; }
-template <int N> Dual<N>& operator+=(Dual<N>& a, const Dual<N>& b) { a = a + b; return a; }
-template <int N> Dual<N>& operator-=(Dual<N>& a, const Dual<N>& b) { a = a - b; return a; }
-template <int N> Dual<N>& operator*=(Dual<N>& a, const Dual<N>& b) { a = a * b; return a; }
-#define KSD_CMP(op)                                                                                  \
-    template <int N> bool operator op(const Dual<N>& a, const Dual<N>& b) { return a.v op b.v; }    \
-    template <int N> bool operator op(const Dual<N>& a, double b) { return a.v op b; }              \
-    template <int N> bool operator op(double a, const Dual<N>& b) { return a op b.v; }
+KSD_T Dual<N, S> operator+(const Dual<N, S>& a, double b) { Dual<N, S> r = a; r.v = r.v + b; return r; }
+KSD_T Dual<N, S> operator+(double a, const Dual<N, S>& b) { Dual<N, S> r = b; r.v = r.v + a; return r; }
+KSD_T Dual<N, S> operator-(const Dual<N, S>& a, double b) { Dual<N, S> r = a; r.v = r.v - b; return r; }
+KSD_T Dual<N, S> operator-(double a, const Dual<N, S>& b) { Dual<N, S> r; r.v = a - b.v; KSD_LOOP r.d[i] = -b.d[i]; return r; }
+KSD_T Dual<N, S> operator*(const Dual<N, S>& a, double b) { Dual<N, S> r; r.v = a.v * b; KSD_LOOP r.d[i] = a.d[i] * b; return r; }
+KSD_T Dual<N, S> operator*(double a, const Dual<N, S>& b) { return b * a; }
+KSD_T Dual<N, S> operator/(const Dual<N, S>& a, double b) { return a * (1.0 / b); }
+KSD_T Dual<N, S> operator/(double a, const Dual<N, S>& b) { Dual<N, S> r; r.v = a / b.v; S f = -r.v / b.v; KSD_LOOP r.d[i] = f * b.d[i]; return r; }
+KSD_T Dual<N, S> operator-(const Dual<N, S>& a) { Dual<N, S> r; r.v = -a.v; KSD_LOOP r.d[i] = -a.d[i]; return r; }
+KSD_T Dual<N, S>& operator+=(Dual<N, S>& a, const Dual<N, S>& b) { a = a + b; return a; }
+KSD_T Dual<N, S>& operator-=(Dual<N, S>& a, const Dual<N, S>& b) { a = a - b; return a; }
+KSD_T Dual<N, S>& operator*=(Dual<N, S>& a, const Dual<N, S>& b) { a = a * b; return a; }
+#define KSD_CMP(op)                                                                              \
+    KSD_T bool operator op(const Dual<N, S>& a, const Dual<N, S>& b) { return val(a.v) op val(b.v); } \
+    KSD_T bool operator op(const Dual<N, S>& a, double b) { return val(a.v) op b; }                   \
+    KSD_T bool operator op(double a, const Dual<N, S>& b) { return a op val(b.v); }
 KSD_CMP(<)
 KSD_CMP(>)
 KSD_CMP(<=)
@@ -50,46 +57,49 @@ KSD_CMP(!=)
 #undef KSD_CMP
 
 // chain rule helper
-template <int N> Dual<N> chain(double f, double df, const Dual<N>& x) { Dual<N> r(f); KSD_LOOP r.d[i] = df * x.d[i]; return r; }
-template <int N> Dual<N> ksqrt(const Dual<N>& x) { double s = std::sqrt(x.v); return chain(s, s > 0 ? 0.5 / s : 0.0, x); }
-template <int N> Dual<N> kexp(const Dual<N>& x) { double e = std::exp(x.v); return chain(e, e, x); }
-template <int N> Dual<N> kexpm1(const Dual<N>& x) { return chain(std::expm1(x.v), std::exp(x.v), x); }
-template <int N> Dual<N> klog(const Dual<N>& x) { return chain(std::log(x.v), 1.0 / x.v, x); }
-template <int N> Dual<N> ksin(const Dual<N>& x) { return chain(std::sin(x.v), std::cos(x.v), x); }
-template <int N> Dual<N> kcos(const Dual<N>& x) { return chain(std::cos(x.v), -std::sin(x.v), x); }
-template <int N> Dual<N> ktan(const Dual<N>& x) { double t = std::tan(x.v); return chain(t, 1.0 + t * t, x); }
-template <int N> Dual<N> katan(const Dual<N>& x) { return chain(std::atan(x.v), 1.0 / (1.0 + x.v * x.v), x); }
-template <int N> Dual<N> kasin(const Dual<N>& x) { return chain(std::asin(x.v), 1.0 / std::sqrt(1.0 - x.v * x.v), x); }
-// acos with a zero derivative at the ends (the reference's safe_acos, step.jl:285-291)
-template <int N> Dual<N> kacos(const Dual<N>& x) {
-    double v = x.v, s = 1.0 - v * v;
-    return chain(std::acos(v), s > 0 ? -1.0 / std::sqrt(s) : 0.0, x);
+KSD_T Dual<N, S> chain(typename ident<S>::type f, typename ident<S>::type df, const Dual<N, S>& x) {
+    Dual<N, S> r; r.v = f; KSD_LOOP r.d[i] = df * x.d[i]; return r;
 }
-template <int N> Dual<N> ktanh(const Dual<N>& x) { double t = std::tanh(x.v); return chain(t, 1.0 - t * t, x); }
-template <int N> Dual<N> ksinh(const Dual<N>& x) { return chain(std::sinh(x.v), std::cosh(x.v), x); }
-template <int N> Dual<N> kcosh(const Dual<N>& x) { return chain(std::cosh(x.v), std::sinh(x.v), x); }
-template <int N> Dual<N> kcbrt(const Dual<N>& x) { double c = std::cbrt(x.v); return chain(c, c / (3.0 * x.v), x); }
-template <int N> Dual<N> kfloor(const Dual<N>& x) { return Dual<N>(std::floor(x.v)); }
-template <int N> Dual<N> kabs(const Dual<N>& x) { return x.v < 0 ? -x : x; }
-template <int N> Dual<N> kcopysign(const Dual<N>& a, const Dual<N>& b) { return (std::signbit(a.v) != std::signbit(b.v)) ? -a : a; }
-template <int N> Dual<N> katan2(const Dual<N>& y, const Dual<N>& x) {
-    double r2 = x.v * x.v + y.v * y.v;
-    Dual<N> r(std::atan2(y.v, x.v));
+KSD_T Dual<N, S> ksqrt(const Dual<N, S>& x) { S s = ksqrt(x.v); return chain(s, s > 0.0 ? S(0.5 / s) : S(0.0), x); }
+KSD_T Dual<N, S> kexp(const Dual<N, S>& x) { S e = kexp(x.v); return chain(e, e, x); }
+KSD_T Dual<N, S> kexpm1(const Dual<N, S>& x) { return chain(kexpm1(x.v), kexp(x.v), x); }
+KSD_T Dual<N, S> klog(const Dual<N, S>& x) { return chain(klog(x.v), S(1.0 / x.v), x); }
+KSD_T Dual<N, S> ksin(const Dual<N, S>& x) { return chain(ksin(x.v), kcos(x.v), x); }
+KSD_T Dual<N, S> kcos(const Dual<N, S>& x) { return chain(kcos(x.v), S(-ksin(x.v)), x); }
+KSD_T Dual<N, S> ktan(const Dual<N, S>& x) { S t = ktan(x.v); return chain(t, S(1.0 + t * t), x); }
+KSD_T Dual<N, S> katan(const Dual<N, S>& x) { return chain(katan(x.v), S(1.0 / (1.0 + x.v * x.v)), x); }
+KSD_T Dual<N, S> kasin(const Dual<N, S>& x) { return chain(kasin(x.v), S(1.0 / ksqrt(1.0 - x.v * x.v)), x); }
+// acos with a zero derivative at the ends (the reference's safe_acos, step.jl:285-291)
+KSD_T Dual<N, S> kacos(const Dual<N, S>& x) {
+    S s = 1.0 - x.v * x.v;
+    return chain(kacos(x.v), s > 0.0 ? S(-1.0 / ksqrt(s)) : S(0.0), x);
+}
+KSD_T Dual<N, S> ktanh(const Dual<N, S>& x) { S t = ktanh(x.v); return chain(t, S(1.0 - t * t), x); }
+KSD_T Dual<N, S> ksinh(const Dual<N, S>& x) { return chain(ksinh(x.v), kcosh(x.v), x); }
+KSD_T Dual<N, S> kcosh(const Dual<N, S>& x) { return chain(kcosh(x.v), ksinh(x.v), x); }
+KSD_T Dual<N, S> kcbrt(const Dual<N, S>& x) { S c = kcbrt(x.v); return chain(c, S(c / (3.0 * x.v)), x); }
+KSD_T Dual<N, S> kfloor(const Dual<N, S>& x) { return Dual<N, S>(kfloor(x.v)); }
+KSD_T Dual<N, S> kabs(const Dual<N, S>& x) { return val(x.v) < 0 ? -x : x; }
+KSD_T Dual<N, S> kcopysign(const Dual<N, S>& a, const Dual<N, S>& b) { return (std::signbit(val(a.v)) != std::signbit(val(b.v))) ? -a : a; }
+KSD_T Dual<N, S> katan2(const Dual<N, S>& y, const Dual<N, S>& x) {
+    S r2 = x.v * x.v + y.v * y.v;
+    Dual<N, S> r; r.v = katan2(y.v, x.v);
     KSD_LOOP r.d[i] = (x.v * y.d[i] - y.v * x.d[i]) / r2;
     return r;
 }
-template <int N> Dual<N> kpow(const Dual<N>& x, double y) {
-    double p = std::pow(x.v, y);
-    return chain(p, y == 0.0 ? 0.0 : y * std::pow(x.v, y - 1.0), x);
+KSD_T Dual<N, S> kpow(const Dual<N, S>& x, double y) {
+    S p = kpow(x.v, y);
+    return chain(p, y == 0.0 ? S(0.0) : S(y * kpow(x.v, y - 1.0)), x);
 }
-template <int N> Dual<N> kpow(const Dual<N>& x, const Dual<N>& y) {
-    double p = std::pow(x.v, y.v);
-    Dual<N> r(p);
-    double lx = x.v > 0 ? std::log(x.v) : 0.0;
-    double dx = y.v == 0.0 ? 0.0 : y.v * std::pow(x.v, y.v - 1.0);
+KSD_T Dual<N, S> kpow(const Dual<N, S>& x, const Dual<N, S>& y) {
+    S p = kpow(x.v, y.v);
+    Dual<N, S> r; r.v = p;
+    S lx = x.v > 0.0 ? S(klog(x.v)) : S(0.0);
+    S dx = y.v == 0.0 ? S(0.0) : S(y.v * kpow(x.v, y.v - 1.0));
     KSD_LOOP r.d[i] = dx * x.d[i] + p * lx * y.d[i];
     return r;
 }
+#undef KSD_T
 #undef KSD_LOOP
 
 }  // namespace kso

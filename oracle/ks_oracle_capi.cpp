@@ -178,7 +178,7 @@ int kso_count_polarized(double a, double thetao, int npix, const double* alphas,
 // dual_gradient.jl:225-245): dparams (21 x ns, column-major) accumulates; image (npix x 4) is written
 int kso_polarized_gradient(double a, double thetao, int npix, const double* alphas, const double* betas, int N, int M,
                            int ns, const double* params, double t_obs, double nu, double L, int nmax, double slab,
-                           const double* dstokes, double* dparams, double* image) {
+                           const double* dstokes, double* dparams, double* image, int analytic_weight) {
     ModelHolder mh = make_model<double>(MODEL_THERMAL, 21, ns, params, nullptr, 0, nullptr, nullptr, 0);
     const Model& m = mh.m;
     int nth = omp_get_max_threads();
@@ -186,11 +186,41 @@ int kso_polarized_gradient(double a, double thetao, int npix, const double* alph
 #pragma omp parallel for schedule(dynamic, 4)
     for (int i = 0; i < npix; ++i) {
         std::vector<double>& g = part[omp_get_thread_num()];
-        polarized_gradient_ray(m, a, thetao, alphas[i], betas[i], N, M, t_obs, nu, L, nmax, slab, dstokes + 4 * (size_t)i, g.data(),
-                               image + 4 * (size_t)i);
+        if (analytic_weight)
+            polarized_gradient_ray<double, true>(m, a, thetao, alphas[i], betas[i], N, M, t_obs, nu, L, nmax, slab, dstokes + 4 * (size_t)i,
+                                                 g.data(), image + 4 * (size_t)i);
+        else
+            polarized_gradient_ray<double, false>(m, a, thetao, alphas[i], betas[i], N, M, t_obs, nu, L, nmax, slab, dstokes + 4 * (size_t)i,
+                                                  g.data(), image + 4 * (size_t)i);
     }
     for (int t = 0; t < nth; ++t)
         for (size_t k = 0; k < 21 * (size_t)ns; ++k) dparams[k] += part[t][k];
+    return 0;
+}
+
+// the work of the dual sweep per phase on the Counted scalar (with d ln G written out, as the GPU does):
+// counts[0..3] = the march, the forward
+// steps (splat sums + step operators), the tails, the adjoint sweep, and counts[4] = the part of the
+// sweep spent in the per-sample step adjoints (summed over the rays)
+int kso_count_gradient(double a, double thetao, int npix, const double* alphas, const double* betas, int N, int M,
+                       int ns, const double* params, double t_obs, double nu, double L, const double* dstokes,
+                       double* counts, double* special) {
+    ModelHolder mh = make_model<Counted>(MODEL_THERMAL, 21, ns, params, nullptr, 0, nullptr, nullptr, 0);
+    const Model& m = mh.m;
+    double c[5] = {0, 0, 0, 0, 0}, sp[5] = {0, 0, 0, 0, 0};
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : c[:5], sp[:5])
+    for (int i = 0; i < npix; ++i) {
+        op_counts().reset();
+        std::vector<Counted> g(21 * (size_t)ns, Counted(0.0));
+        Counted img[4];
+        OpCounts ph[5];
+        polarized_gradient_ray<Counted, true>(m, a, thetao, alphas[i], betas[i], N, M, t_obs, nu, L, -1, 0.0, dstokes + 4 * (size_t)i, g.data(),
+                                              img, ph);
+        double pf = 0, ps = 0;
+        for (int q = 0; q < 4; ++q) { c[q] += ph[q].flops - pf; sp[q] += ph[q].special - ps; pf = ph[q].flops; ps = ph[q].special; }
+        c[4] += ph[4].flops; sp[4] += ph[4].special;
+    }
+    for (int q = 0; q < 5; ++q) { counts[q] = c[q]; special[q] = sp[q]; }
     return 0;
 }
 

@@ -336,47 +336,49 @@ template <class T> T phi1(T a) {
     return -kexpm1(-a) / a;
 }
 // M_n(a) = int_0^1 t^n e^{-at} dt, n = 0..9
-template <class T> void moments(T a, T M[10]) {
-    if (val(kabs(a)) < 1.0) {
+// Power-series coefficients of the moments, c[n][m] = (-1)^m / (m! (n + m + 1)), m = 0..23
+struct MomentSeries {
+    double c[10][24];
+    constexpr MomentSeries() : c{} {
         for (int n = 0; n < 10; ++n) {
-            T term = 1.0, sum = 1.0 / (n + 1.0);
-            for (int mm = 1; mm < 40; ++mm) {
-                term = term * (-a) / (double)mm;
-                sum = sum + term / (double)(n + mm + 1);
+            double fact = 1.0;
+            for (int m = 0; m < 24; ++m) {
+                if (m > 0) fact *= m;
+                c[n][m] = ((m % 2) ? -1.0 : 1.0) / (fact * (n + m + 1));
             }
-            M[n] = sum;
         }
-    } else if (val(a) < 40.0) {
-        // M_n = e^{-a} sum_m a^m n!/(n+m+1)!   (positive terms)
-        T e = kexp(-a);
-        for (int n = 0; n < 10; ++n) {
-            T term = 1.0 / (n + 1.0), sum = term;
-            for (int mm = 1; mm < 200; ++mm) {
-                term = term * a / (double)(n + mm + 1);
-                sum = sum + term;
-                if (val(term) < 1e-18 * val(sum)) break;
-            }
-            M[n] = e * sum;
+    }
+};
+inline constexpr MomentSeries MOMENT_SERIES{};
+
+// M_n(a) = int_0^1 t^n e^{-at} dt for n = 0..nmax (step.jl:239-268): for |a| < 1 the 24-term power
+// series by Horner's rule, otherwise the upward recurrence M_0 = (1 - e^{-a})/a,
+// M_n = (n M_{n-1} - e^{-a})/a.
+template <class T> void moments(T a, T M[10], int nmax = 9) {
+    if (val(kabs(a)) < 1.0) {
+        for (int n = 0; n <= nmax; ++n) {
+            T s = T(MOMENT_SERIES.c[n][23]);
+            for (int m = 22; m >= 0; --m) s = s * a + MOMENT_SERIES.c[n][m];
+            M[n] = s;
         }
     } else {
         T e = kexp(-a);
         M[0] = (1.0 - e) / a;
-        for (int n = 1; n < 10; ++n) M[n] = ((double)n * M[n - 1] - e) / a;
+        for (int n = 1; n <= nmax; ++n) M[n] = ((double)n * M[n - 1] - e) / a;
     }
 }
+// the integrals of the projector form given the moments M_0..M_9 when b < 0.1 (M_1 otherwise)
 // int_0^1 e^{-at}(cosh bt - 1) dt
-template <class T> T int_coshm1(T a, T b) {
+template <class T> T int_coshm1(T a, T b, const T M[10]) {
     if (b < 0.1) {
-        T M[10]; moments(a, M);
         T b2 = b * b;
         return b2 * (M[2] / 2.0 + b2 * (M[4] / 24.0 + b2 * (M[6] / 720.0 + b2 * M[8] / 40320.0)));
     }
     return (phi1(a - b) + phi1(a + b)) / 2.0 - phi1(a);
 }
 // int_0^1 e^{-at}(1 - cos bt) dt
-template <class T> T int_versin(T a, T b) {
+template <class T> T int_versin(T a, T b, const T M[10]) {
     if (b < 0.1) {
-        T M[10]; moments(a, M);
         T b2 = b * b;
         return b2 * (M[2] / 2.0 - b2 * (M[4] / 24.0 - b2 * (M[6] / 720.0 - b2 * M[8] / 40320.0)));
     }
@@ -384,10 +386,9 @@ template <class T> T int_versin(T a, T b) {
     return phi1(a) - (a - kexp(-a) * (a * c - b * s)) / (a * a + b * b);
 }
 // int_0^1 e^{-at} cos(bt) dt
-template <class T> T int_cos(T a, T b) { return phi1(a) - int_versin(a, b); }
+template <class T> T int_cos(T a, T b, const T M[10]) { return phi1(a) - int_versin(a, b, M); }
 // int_0^1 e^{-at} sin(bt)/b dt
-template <class T> T int_sinc(T a, T b) {
-    T M[10]; moments(a, M);
+template <class T> T int_sinc(T a, T b, const T M[10]) {
     if (b < 0.1) {
         T b2 = b * b;
         return M[1] - b2 * (M[3] / 6.0 - b2 * (M[5] / 120.0 - b2 * (M[7] / 5040.0 - b2 * M[9] / 362880.0)));
@@ -396,8 +397,7 @@ template <class T> T int_sinc(T a, T b) {
     return (1.0 - kexp(-a) * (a * s / b + c)) / (a * a + b * b);
 }
 // int_0^1 e^{-at}(sinh(bt)/b - t) dt
-template <class T> T int_sinhc_excess(T a, T b) {
-    T M[10]; moments(a, M);
+template <class T> T int_sinhc_excess(T a, T b, const T M[10]) {
     if (b < 0.1) {
         T b2 = b * b;
         return b2 * (M[3] / 6.0 + b2 * (M[5] / 120.0 + b2 * (M[7] / 5040.0 + b2 * M[9] / 362880.0)));
@@ -408,28 +408,25 @@ template <class T> T sinhc(T b) { return val(kabs(b)) < 0.1 ? 1.0 + b * b / 6.0 
 template <class T> T sinc_(T b) { return val(kabs(b)) < 0.1 ? 1.0 - b * b / 6.0 * (1.0 - b * b / 20.0 * (1.0 - b * b / 42.0 * (1.0 - b * b / 72.0))) : ksin(b) / b; }
 
 // O = exp(-K D), Ej = int_0^D exp(-K u) du for K = alpha_I + K', K' as in step.jl:190
-template <class T> void step_operator(const T al[4], const T rh[3], T D, M4<T>& O, M4<T>& Ej) {
+// The step as polynomials in K' (K = alpha_I + K'), the coefficients of step.jl:164-243:
+// O = o0 + o1 K' + o2 K'^2 + o3 K'^3, Ej = Dp (e0 + e1 K' + e2 K'^2 + e3 K'^3), with
+// K6 = (aQ, aU, aV, rQ, rU, rV) Dp the entries of K' after the power-of-two rescaling of the length.
+template <class T> void step_coefficients(const T al[4], const T rh[3], T D, T& Dp, T K6[6], T o[4], T e4[4]) {
     // exact power-of-two rescaling of the length (step.jl:164-173)
-    T Dp = pow2_below(D);
+    Dp = pow2_below(D);
     T f = D / Dp;
     T aI = al[0] * Dp, aQ = al[1] * Dp, aU = al[2] * Dp, aV = al[3] * Dp;
     T rQ = rh[0] * Dp, rU = rh[1] * Dp, rV = rh[2] * Dp;
-    M4<T> K = m4_zero<T>();
-    K.m[0][1] = aQ; K.m[0][2] = aU; K.m[0][3] = aV;
-    K.m[1][0] = aQ; K.m[1][2] = rV; K.m[1][3] = -rU;
-    K.m[2][0] = aU; K.m[2][1] = -rV; K.m[2][3] = rQ;
-    K.m[3][0] = aV; K.m[3][1] = rU; K.m[3][2] = -rQ;
+    K6[0] = aQ; K6[1] = aU; K6[2] = aV; K6[3] = rQ; K6[4] = rU; K6[5] = rV;
     T a2 = aQ * aQ + aU * aU + aV * aV, r2 = rQ * rQ + rU * rU + rV * rV;
     T a = aI * f;
     T e = kexp(-a);
     if ((a2 + r2) * f * f < 1e-200) {
-        T M[10]; moments(a, M);
-        O = m4_comb(e, -f * e, K, T(0.0), K, T(0.0), K);
-        Ej = m4_comb(f * phi1(a), -f * f * M[1], K, T(0.0), K, T(0.0), K);
-        for (int i = 0; i < 4; ++i) for (int j = 0; j < 4; ++j) Ej.m[i][j] = Dp * Ej.m[i][j];
+        T M[10]; moments(a, M, 1);
+        o[0] = e; o[1] = -f * e; o[2] = T(0.0); o[3] = T(0.0);
+        e4[0] = f * phi1(a); e4[1] = -f * f * M[1]; e4[2] = T(0.0); e4[3] = T(0.0);
         return;
     }
-    M4<T> K2 = m4_mul(K, K), K3 = m4_mul(K2, K);
     T ar = aQ * rQ + aU * rU + aV * rV;
     T h = (a2 - r2) / 2.0;
     T root = ksqrt(h * h + ar * ar);
@@ -440,10 +437,9 @@ template <class T> void step_operator(const T al[4], const T rh[3], T D, M4<T>& 
     T b1 = L1 * f, b2 = L2 * f;
     T Th = L1sq + L2sq;
     if (Th * f * f * (1.0 + (a2 + r2) * f * f) < 1e-17) {
-        T M[10]; moments(a, M);
-        O = m4_comb(e, -f * e, K, f * f / 2.0 * e, K2, -f * f * f / 6.0 * e, K3);
-        Ej = m4_comb(f * M[0], -f * f * M[1], K, f * f * f * M[2] / 2.0, K2, -f * f * f * f * M[3] / 6.0, K3);
-        for (int i = 0; i < 4; ++i) for (int j = 0; j < 4; ++j) Ej.m[i][j] = Dp * Ej.m[i][j];
+        T M[10]; moments(a, M, 3);
+        o[0] = e; o[1] = -f * e; o[2] = f * f / 2.0 * e; o[3] = -f * f * f / 6.0 * e;
+        e4[0] = f * M[0]; e4[1] = -f * f * M[1]; e4[2] = f * f * f * M[2] / 2.0; e4[3] = -f * f * f * f * M[3] / 6.0;
         return;
     }
     // projector form: exp(-K'f) = [(K'^2 + L2^2) (cosh b1 - K' sinh(b1)/L1) + (L1^2 - K'^2)(cos b2 - K' sin(b2)/L2)]/Th
@@ -457,20 +453,49 @@ template <class T> void step_operator(const T al[4], const T rh[3], T D, M4<T>& 
     T cb2 = kcos(b2), sb2h = ksin(b2 / 2.0);
     T versin2 = 2.0 * sb2h * sb2h;
     // O = c0 1 + c2 K'^2 - f esh M3 - f e sinc(b2) M2, M3 = (L2^2 K' + K'^3)/Th, M2 = (L1^2 K' - K'^3)/Th
-    T c0 = (L2sq * ech + L1sq * e * cb2) / Th;
-    T c2 = (echm1 + e * versin2) / Th;
     T s3 = f * esh, s2 = f * e * sinc_(b2);
-    // K' coefficient: -(s3 L2^2 + s2 L1^2)/Th ; K'^3 coefficient: -(s3 - s2)/Th
-    O = m4_comb(c0, -(s3 * L2sq + s2 * L1sq) / Th, K, c2, K2, -(s3 - s2) / Th, K3);
+    o[0] = (L2sq * ech + L1sq * e * cb2) / Th;
+    o[1] = -(s3 * L2sq + s2 * L1sq) / Th;
+    o[2] = (echm1 + e * versin2) / Th;
+    o[3] = -(s3 - s2) / Th;
     T pa = phi1(a);
-    T Ich = f * (pa + int_coshm1(a, b1));
-    T Ic = f * int_cos(a, b2);
-    T M[10]; moments(a, M);
-    T Ish = f * f * (M[1] + int_sinhc_excess(a, b1));
-    T Is = f * f * int_sinc(a, b2);
-    T C0 = (L2sq * Ich + L1sq * Ic) / Th;
-    T C2 = f * (int_coshm1(a, b1) + int_versin(a, b2)) / Th;
-    Ej = m4_comb(C0, -(Ish * L2sq + Is * L1sq) / Th, K, C2, K2, -(Ish - Is) / Th, K3);
+    T M[10]; moments(a, M, (b1 < 0.1 || b2 < 0.1) ? 9 : 1);
+    T Ich = f * (pa + int_coshm1(a, b1, M));
+    T Ic = f * int_cos(a, b2, M);
+    T Ish = f * f * (M[1] + int_sinhc_excess(a, b1, M));
+    T Is = f * f * int_sinc(a, b2, M);
+    e4[0] = (L2sq * Ich + L1sq * Ic) / Th;
+    e4[1] = -(Ish * L2sq + Is * L1sq) / Th;
+    e4[2] = f * (int_coshm1(a, b1, M) + int_versin(a, b2, M)) / Th;
+    e4[3] = -(Ish - Is) / Th;
+}
+
+// y = K' v and y = K'^T v (K'6 as above), for any mix of scalar types
+template <class T, class U, class R> void kprime_mul(const T K6[6], const U v[4], R y[4]) {
+    y[0] = K6[0] * v[1] + K6[1] * v[2] + K6[2] * v[3];
+    y[1] = K6[0] * v[0] + K6[5] * v[2] - K6[4] * v[3];
+    y[2] = K6[1] * v[0] - K6[5] * v[1] + K6[3] * v[3];
+    y[3] = K6[2] * v[0] + K6[4] * v[1] - K6[3] * v[2];
+}
+template <class T, class U, class R> void kprime_tmul(const T K6[6], const U v[4], R y[4]) {
+    y[0] = K6[0] * v[1] + K6[1] * v[2] + K6[2] * v[3];
+    y[1] = K6[0] * v[0] - K6[5] * v[2] + K6[4] * v[3];
+    y[2] = K6[1] * v[0] + K6[5] * v[1] - K6[3] * v[3];
+    y[3] = K6[2] * v[0] - K6[4] * v[1] + K6[3] * v[2];
+}
+
+// O = exp(-K D) and Ej = int_0^D exp(-K u) du (step.jl:164-243) assembled from the coefficients
+template <class T> void step_operator(const T al[4], const T rh[3], T D, M4<T>& O, M4<T>& Ej) {
+    T Dp, K6[6], o[4], e4[4];
+    step_coefficients(al, rh, D, Dp, K6, o, e4);
+    M4<T> K = m4_zero<T>();
+    K.m[0][1] = K6[0]; K.m[0][2] = K6[1]; K.m[0][3] = K6[2];
+    K.m[1][0] = K6[0]; K.m[1][2] = K6[5]; K.m[1][3] = -K6[4];
+    K.m[2][0] = K6[1]; K.m[2][1] = -K6[5]; K.m[2][3] = K6[3];
+    K.m[3][0] = K6[2]; K.m[3][1] = K6[4]; K.m[3][2] = -K6[3];
+    M4<T> K2 = m4_mul(K, K), K3 = m4_mul(K2, K);
+    O = m4_comb(o[0], o[1], K, o[2], K2, o[3], K3);
+    Ej = m4_comb(e4[0], e4[1], K, e4[2], K2, e4[3], K3);
     for (int i = 0; i < 4; ++i) for (int j = 0; j < 4; ++j) Ej.m[i][j] = Dp * Ej.m[i][j];
 }
 

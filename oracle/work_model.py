@@ -23,18 +23,25 @@ from kerrsplat_amd import synthetic as syn  # noqa: E402
 from kerrsplat_amd.api import Camera  # noqa: E402
 
 
-def polarized_work(cfg_id, stride=8):
+def polarized_work(cfg_id, stride=16):
+    """Per ray-sample work of the fwd+adjoint step, by phase: the march (t~, r, theta, phi), the
+    forward transfer (cull + splat sums + step operators), the tails, and the adjoint sweep split
+    into its per-sample step adjoints and its per-splat element adjoints."""
     cfg = syn.CONFIGS[cfg_id]
     a, th, res, N = cfg["a"], math.radians(cfg["theta_o_deg"]), cfg["res"], cfg["N"]
     cam = Camera((-cfg["fov"], cfg["fov"]), (-cfg["fov"], cfg["fov"]), res)
     idx = np.arange(0, cam.npix, stride)
     P = syn.thermal_splats(cfg_id, a, cfg["ns"])
-    flops, special = oracle.count_polarized(a, th, cam.alphas[idx], cam.betas[idx], N, P, 0.0, cfg["nus"][0],
-                                            syn.length_unit())
+    ds = np.random.default_rng(20261202).normal(size=(len(idx), 4))
+    c, sp = oracle.count_gradient(a, th, cam.alphas[idx], cam.betas[idx], N, P, 0.0, cfg["nus"][0], syn.length_unit(), ds)
     n = len(idx) * N
-    return {"flops_per_ray_sample": flops / n, "special_per_ray_sample": special / n,
+    phases = {"march": (c[0], sp[0]), "transfer": (c[1], sp[1]), "tails": (c[2], sp[2]),
+              "adjoint_steps": (c[4], sp[4]), "adjoint_elements": (c[3] - c[4], sp[3] - sp[4])}
+    out = {k: {"flops_per_ray_sample": v[0] / n, "special_per_ray_sample": v[1] / n} for k, v in phases.items()}
+    fwd = (c[0] + c[1]) / n
+    return {"flops_per_ray_sample": fwd, "flops_per_ray_sample_fwd_adjoint": c[:4].sum() / n, "phases": out,
             "rays_counted": int(len(idx)), "samples_per_ray": N, "splats": cfg["ns"],
-            "method": "oracle on the Counted scalar type, every %d-th ray of the screen" % stride,
+            "method": "oracle dual sweep on the Counted scalar type, every %d-th ray of the screen" % stride,
             "hbm_bytes_per_launch": None}
 
 

@@ -52,26 +52,40 @@ def test_samples_vs_oracle(oracle, marcher):
         assert e.max() < 1e-10, (f, e.max())
 
 
-def test_thin_image_vs_oracle(oracle):
+def make_cache(cam, N, a, th, path):
+    """A regenerated cache: 'stored' marches Recurrence(64) into sample storage (the transfer reads the
+    samples), 'fused' marches inside every render (Fused(64), no storage)."""
+    if path == "stored":
+        cache = K.GeodesicCache(cam, N, store_samples=True)
+        K.regenerate(cache, a, th, marcher=K.Recurrence(64))
+    else:
+        cache = K.GeodesicCache(cam, N, store_samples=False)
+        K.regenerate(cache, a, th, marcher=K.Fused(64))
+    return cache
+
+
+PATHS = ["stored", "fused"]
+
+
+@pytest.mark.parametrize("path", PATHS)
+def test_thin_image_vs_oracle(oracle, path):
     cfg = syn.CONFIGS[0]
     a, th = cfg["a"], math.radians(cfg["theta_o_deg"])
     cam = camera(cfg["fov"], 24)
     N = 256
     P = syn.thin_splats(0, a, cfg["ns"])
-    cache = K.GeodesicCache(cam, N, store_samples=False)
-    K.regenerate(cache, a, th, marcher=K.Fused(64))
+    cache = make_cache(cam, N, a, th, path)
     img = K.thin_image(cache, P, 0.0)
     ref = oracle.thin_image(a, th, cam.alphas, cam.betas, N, P, 0.0).reshape(cam.size, order="F")
     assert np.max(np.abs(img - ref)) <= 1e-10 * np.max(np.abs(ref))
     assert np.max(ref) > 0
 
 
-def _polarized_case(oracle, kind, params, res=16, N=256, a=0.5, thdeg=60.0, fov=15.0, nu=230e9, nmax=-1, slab=0.0, **kw):
+def _polarized_case(oracle, kind, params, path, res=16, N=256, a=0.5, thdeg=60.0, fov=15.0, nu=230e9, nmax=-1, slab=0.0, **kw):
     th = math.radians(thdeg)
     cam = camera(fov, res)
     L = syn.length_unit()
-    cache = K.GeodesicCache(cam, N, store_samples=False)
-    K.regenerate(cache, a, th, marcher=K.Fused(64))
+    cache = make_cache(cam, N, a, th, path)
     model = K.Model(params, kind=kind, **kw)
     img = K.polarized_image(cache, model, 0.0, nu, L, nmax=nmax, slab=slab, kind=kind)
     hyp = None
@@ -89,27 +103,32 @@ def _polarized_case(oracle, kind, params, res=16, N=256, a=0.5, thdeg=60.0, fov=
     return img
 
 
-def test_polarized_thermal_vs_oracle(oracle):
+@pytest.mark.parametrize("path", PATHS)
+def test_polarized_thermal_vs_oracle(oracle, path):
     P = syn.thermal_splats(1, 0.5, 6)
-    _polarized_case(oracle, "thermal", P)
+    _polarized_case(oracle, "thermal", P, path)
 
 
-def test_polarized_thermal_winding_vs_oracle(oracle):
+@pytest.mark.parametrize("path", PATHS)
+def test_polarized_thermal_winding_vs_oracle(oracle, path):
     P = syn.thermal_splats(1, 0.5, 6)
-    _polarized_case(oracle, "thermal", P, nmax=0, slab=0.5)
+    _polarized_case(oracle, "thermal", P, path, nmax=0, slab=0.5)
 
 
-def test_polarized_powerlaw_vs_oracle(oracle):
+@pytest.mark.parametrize("path", PATHS)
+def test_polarized_powerlaw_vs_oracle(oracle, path):
     P = syn.powerlaw_splats(2, 0.9, 4)
-    _polarized_case(oracle, "powerlaw", P, a=0.9, thdeg=45.0, fov=12.0)
+    _polarized_case(oracle, "powerlaw", P, path, a=0.9, thdeg=45.0, fov=12.0)
 
 
-def test_polarized_kappa_vs_oracle(oracle):
+@pytest.mark.parametrize("path", PATHS)
+def test_polarized_kappa_vs_oracle(oracle, path):
     P = syn.kappa_splats(2, 0.9, 4)
-    _polarized_case(oracle, "kappa", P, a=0.9, thdeg=45.0, fov=12.0)
+    _polarized_case(oracle, "kappa", P, path, a=0.9, thdeg=45.0, fov=12.0)
 
 
-def test_polarized_knot_vs_oracle(oracle):
+@pytest.mark.parametrize("path", PATHS)
+def test_polarized_knot_vs_oracle(oracle, path):
     P = syn.thermal_splats(4, 0.9, 3)
     times = np.array([-200.0, -100.0, 0.0, 100.0])
     knots = np.zeros((3, len(times), P.shape[1]))
@@ -117,17 +136,17 @@ def test_polarized_knot_vs_oracle(oracle):
         for k, t in enumerate(times):
             ph = P[20, i] * t
             knots[:, k, i] = [P[0, i] * math.cos(ph) - P[1, i] * math.sin(ph), P[0, i] * math.sin(ph) + P[1, i] * math.cos(ph), P[2, i]]
-    _polarized_case(oracle, "knot", P, a=0.9, thdeg=163.0, fov=12.0, knots=knots, knot_times=times)
+    _polarized_case(oracle, "knot", P, path, a=0.9, thdeg=163.0, fov=12.0, knots=knots, knot_times=times)
 
 
-def test_cube_frames_match_images(oracle):
+@pytest.mark.parametrize("path", PATHS)
+def test_cube_frames_match_images(oracle, path):
     """One launch over (frame, frequency) items equals per-frame images (polarized.jl:187-198)."""
     a, th = 0.9, math.radians(45.0)
     cam = camera(12.0, 12)
     L = syn.length_unit()
     P = syn.thermal_splats(2, a, 4, single_frame=False)
-    cache = K.GeodesicCache(cam, 128, store_samples=False)
-    K.regenerate(cache, a, th, marcher=K.Fused(64))
+    cache = make_cache(cam, 128, a, th, path)
     m = K.Model(P)
     times, nus = [0.0, 20.0, 40.0], [86e9, 345e9]
     cube = K.polarized_cube(cache, m, times, nus, L)
@@ -154,6 +173,83 @@ def test_visibilities_vs_oracle(oracle):
     assert np.max(np.abs(V - Vref)) <= 1e-12 * np.max(np.abs(Vref))
 
 
+def test_stored_and_fused_renders_agree():
+    """The transfer from stored Recurrence(64) samples and the fused march + transfer see the same samples."""
+    a, th = 0.5, math.radians(60.0)
+    cam = camera(15.0, 16)
+    L = syn.length_unit()
+    P = syn.thermal_splats(1, a, 6)
+    c1, c2 = make_cache(cam, 256, a, th, "stored"), make_cache(cam, 256, a, th, "fused")
+    i1, i2 = K.polarized_image(c1, P, 0.0, 230e9, L), K.polarized_image(c2, P, 0.0, 230e9, L)
+    assert np.abs(i1 - i2).max() <= 1e-13 * np.abs(i1[..., 0]).max()
+    s1 = K.samples(c1)
+    K.march(c1)
+    s2 = K.samples(c1)
+    for f in ("t", "r", "theta", "phi", "flags"):
+        assert np.array_equal(s1[f], s2[f]), f
+
+
+def _gradient_case(oracle, res=12, N=128, ns=3, dens=30.0, nmax=-1, slab=0.0, times=(0.0,), nus=(230e9,), seed=11):
+    """polarized_gradient (GPU dual sweep over stored samples) against the oracle's dual sweep on the
+    same seeded inputs: every entry within 1e-9 of the largest, and the forward image at the image bar.
+    The device contracts the step adjoint with matrix-vector products and writes d ln G by hand where the
+    oracle forms 4x4 dual matrices and 13-partial duals; per sample the two agree to about 7e-10 at
+    degenerate spectra of K' (and 1e-12 elsewhere), hence 1e-9 rather than the image bar of 1e-10."""
+    a, th = 0.5, math.radians(60.0)
+    cam = camera(15.0, res)
+    L = syn.length_unit()
+    P = syn.thermal_splats(1, a, ns)
+    P[12, :] += math.log(dens)
+    cache = make_cache(cam, N, a, th, "stored")
+    na, nb = cam.size
+    ds = np.random.default_rng(seed).normal(size=(na, nb, len(times), len(nus), 4))
+    g, cube = K.polarized_gradient(cache, P, list(times), list(nus), L, ds, nmax=nmax, slab=slab)
+    gref = np.zeros_like(g)
+    for it, t in enumerate(times):
+        for inu, nu in enumerate(nus):
+            dso = ds[:, :, it, inu, :].reshape(cam.npix, 4, order="F")
+            gi, img = oracle.polarized_gradient(a, th, cam.alphas, cam.betas, N, P, t, nu, L, dso, nmax=nmax, slab=slab)
+            gref += gi
+            ref = img.reshape(cam.size + (4,), order="F")
+            got = cube[:, :, it, inu, :]
+            assert np.abs(got - ref).max() <= 1e-10 * np.abs(ref[..., 0]).max()
+    scale = np.abs(gref).max()
+    assert scale > 0
+    err = np.abs(g - gref).max() / scale
+    assert err < 1e-9, err
+    return g
+
+
+def test_gradient_vs_oracle(oracle):
+    _gradient_case(oracle)
+
+
+def test_gradient_thin_parcels_vs_oracle(oracle):
+    _gradient_case(oracle, dens=1.0, seed=12)
+
+
+def test_gradient_winding_vs_oracle(oracle):
+    _gradient_case(oracle, nmax=0, slab=0.5, seed=13)
+
+
+def test_gradient_frames_and_frequencies_vs_oracle(oracle):
+    _gradient_case(oracle, times=(0.0, 15.0), nus=(230e9, 345e9), seed=14)
+
+
+def test_gradient_accumulates():
+    """dparams accumulates (dual_gradient.jl:240: dparams .+= ...)."""
+    a, th = 0.5, math.radians(60.0)
+    cam = camera(15.0, 8)
+    L = syn.length_unit()
+    P = syn.thermal_splats(1, a, 2)
+    cache = make_cache(cam, 64, a, th, "stored")
+    ds = np.random.default_rng(3).normal(size=(8, 8, 1, 1, 4))
+    g1, _ = K.polarized_gradient(cache, P, [0.0], [230e9], L, ds)
+    base = np.asfortranarray(np.full(P.shape, 2.5))
+    g2, _ = K.polarized_gradient(cache, P, [0.0], [230e9], L, ds, dparams=base)
+    assert np.array_equal(g2, g1 + 2.5)
+
+
 def test_errors_mirror_reference():
     cam = camera(10.0, 4)
     c = K.GeodesicCache(cam, 8, store_samples=False)
@@ -165,3 +261,67 @@ def test_errors_mirror_reference():
         K.regenerate(c, 0.9, 0.5, camera=camera(10.0, 5), marcher=K.Fused(64))
     with pytest.raises(ValueError):
         K.GeodesicCache(cam, 0)
+    K.regenerate(c, 0.9, 0.5, marcher=K.Fused(64))
+    with pytest.raises(ValueError, match="no stored samples"):
+        K.polarized_gradient(c, syn.thermal_splats(1, 0.9, 1), [0.0], [230e9], 1e14, np.zeros((4, 4, 1, 1, 4)))
+
+
+# ---- mixed precision (FP64 geodesics, FP32 transfer and adjoint) ------------------------------------------------
+# Bars from the target of the mixed path: Stokes error <= 1e-4 of the peak intensity and gradient relative
+# error <= 1e-3 of the largest entry, both against the plain FP64 path on identical inputs (itself at parity
+# with the oracle above).
+MIXED_CASES = [("thermal", syn.thermal_splats(1, 0.5, 6), 0.5, 60.0, 15.0, {}),
+               ("powerlaw", syn.powerlaw_splats(2, 0.9, 4), 0.9, 45.0, 12.0, {}),
+               ("kappa", syn.kappa_splats(2, 0.9, 4), 0.9, 45.0, 12.0, {})]
+
+
+@pytest.mark.parametrize("kind,P,a,thdeg,fov,kw", MIXED_CASES, ids=[c[0] for c in MIXED_CASES])
+def test_mixed_image_error(kind, P, a, thdeg, fov, kw):
+    th = math.radians(thdeg)
+    cam = camera(fov, 24)
+    L = syn.length_unit()
+    cache = make_cache(cam, 512, a, th, "stored")
+    m = K.Model(P, kind=kind, **kw)
+    ref = K.polarized_cube(cache, m, [0.0, 20.0], [230e9, 345e9], L)
+    K.precision(cache, "mixed")
+    got = K.polarized_cube(cache, m, [0.0, 20.0], [230e9, 345e9], L)
+    for it in range(2):
+        for inu in range(2):
+            peak = np.abs(ref[:, :, it, inu, 0]).max()
+            assert peak > 0
+            err = np.abs(got[:, :, it, inu] - ref[:, :, it, inu]).max() / peak
+            assert err <= 1e-4, (kind, it, inu, err)
+            assert err > 0          # the FP32 path really ran
+
+
+def test_mixed_dense_parcels_error():
+    """Optically thick, Faraday-rotating parcels: the FP64 fallback of the step keeps the FP32 path at its bar."""
+    a, th = 0.5, math.radians(60.0)
+    cam = camera(15.0, 24)
+    L = syn.length_unit()
+    P = syn.thermal_splats(1, a, 6)
+    P[12, :] += math.log(300.0)
+    cache = make_cache(cam, 512, a, th, "stored")
+    ref = K.polarized_image(cache, P, 0.0, 230e9, L)
+    K.precision(cache, "mixed")
+    got = K.polarized_image(cache, P, 0.0, 230e9, L)
+    err = np.abs(got - ref).max() / np.abs(ref[..., 0]).max()
+    assert 0 < err <= 1e-4, err
+
+
+@pytest.mark.parametrize("dens", [1.0, 30.0])
+def test_mixed_gradient_error(dens):
+    a, th = 0.5, math.radians(60.0)
+    cam = camera(15.0, 16)
+    L = syn.length_unit()
+    P = syn.thermal_splats(1, a, 4)
+    P[12, :] += math.log(dens)
+    cache = make_cache(cam, 256, a, th, "stored")
+    ds = np.random.default_rng(21).normal(size=(16, 16, 1, 1, 4))
+    g64, img64 = K.polarized_gradient(cache, P, [0.0], [230e9], L, ds)
+    K.precision(cache, "mixed")
+    g32, img32 = K.polarized_gradient(cache, P, [0.0], [230e9], L, ds)
+    gerr = np.abs(g32 - g64).max() / np.abs(g64).max()
+    ierr = np.abs(img32 - img64).max() / np.abs(img64[..., 0]).max()
+    assert 0 < gerr <= 1e-3, gerr
+    assert ierr <= 1e-4, ierr

@@ -65,3 +65,19 @@ def test_gradient_is_linear_in_seed(oracle):
     g2, _ = oracle.polarized_gradient(A, THO, cam.alphas, cam.betas, N, P, 0.0, NU, L, ds2)
     g12, _ = oracle.polarized_gradient(A, THO, cam.alphas, cam.betas, N, P, 0.0, NU, L, 2.0 * ds - 3.0 * ds2)
     assert np.abs(g12 - (2.0 * g1 - 3.0 * g2)).max() <= 1e-12 * np.abs(g12).max()
+
+
+@pytest.mark.parametrize("nmax,slab", [(-1, 0.0), (0, 0.5)])
+def test_analytic_weight_derivative_matches_duals(oracle, nmax, slab):
+    """d ln G of the splat weight written out by hand (centre under the pattern rotation, widths,
+    normalised quaternion, t0, temporal width, pattern rate) equals the 13-partial dual derivative."""
+    cam, P, ds = _setup(30.0, 9)
+    P = P.copy()
+    P[20, :] = [0.05, -0.03, 0.08]          # a visible pattern rotation
+    P[11, :] = np.log([40.0, 80.0, 25.0])   # finite temporal envelopes
+    P[10, :] = [5.0, -10.0, 2.0]
+    L = syn.length_unit()
+    g_dual, _ = oracle.polarized_gradient(A, THO, cam.alphas, cam.betas, N, P, 3.0, NU, L, ds, nmax=nmax, slab=slab)
+    g_an, _ = oracle.polarized_gradient(A, THO, cam.alphas, cam.betas, N, P, 3.0, NU, L, ds, nmax=nmax, slab=slab, analytic_weight=True)
+    assert np.abs(g_dual).max() > 0
+    assert np.abs(g_an - g_dual).max() <= 1e-12 * np.abs(g_dual).max()
