@@ -33,12 +33,14 @@ template <class S>
 KS_DEV void sample_adjoint(const S j4[4], const S a4[4], const S r3[3], S D, const S w[4], const S R[4],
                            S Otw[4], S jbar[4], S abar[4], S rbar[3]) {
     using D7 = DN<7, S>;
+    // seeded on the Dp-scaled coefficients (O(1) derivatives, no FP32 overflow for near-transparent
+    // steps); the bars are scaled back by the power of two Dp at the end, which is exact
+    const S Dp = pow2_below(D);
     D7 al[4], rh[3];
-    for (int i = 0; i < 4; ++i) al[i] = D7::seed(a4[i], i);
-    for (int i = 0; i < 3; ++i) rh[i] = D7::seed(r3[i], 4 + i);
-    S Dp;
+    for (int i = 0; i < 4; ++i) al[i] = D7::seed(a4[i] * Dp, i);
+    for (int i = 0; i < 3; ++i) rh[i] = D7::seed(r3[i] * Dp, 4 + i);
     D7 K6[6], o[4], e4[4];
-    step_coefs(al, rh, D, Dp, K6, o, e4);
+    step_coefs_scaled(al, rh, S(D / Dp), K6, o, e4);
     // q = w . (O R + Ej j)
     S wR(0.0), wj(0.0);
     for (int i = 0; i < 4; ++i) { wR += w[i] * R[i]; wj += w[i] * j4[i]; }
@@ -56,8 +58,8 @@ KS_DEV void sample_adjoint(const S j4[4], const S a4[4], const S r3[3], S D, con
             kprime_mul(K6, vj, t4); for (int i = 0; i < 4; ++i) vj[i] = t4[i];
         }
     }
-    for (int i = 0; i < 4; ++i) abar[i] = q.d[i];
-    for (int i = 0; i < 3; ++i) rbar[i] = q.d[4 + i];
+    for (int i = 0; i < 4; ++i) abar[i] = q.d[i] * Dp;
+    for (int i = 0; i < 3; ++i) rbar[i] = q.d[4 + i] * Dp;
     // values: O^T w = sum o_n K'^T^n w, Ej^T w = Dp sum e_n K'^T^n w
     S K6v[6];
     for (int i = 0; i < 6; ++i) K6v[i] = K6[i].v;

@@ -378,16 +378,21 @@ template <class T> KS_DEV T sinc_(T b) {
     return sin(b) / b;
 }
 
+// Below this |K' D|^2-sized gate the step takes its cubic Taylor form, whose first dropped term is
+// under gate / 24 relative: 1e-17 in FP64; 1e-6 in FP32, which also keeps the general form (its 1/Th
+// and the s3 - s2 difference) away from the range where FP32 cancels or overflows.
+KS_DEV double taylor_gate(double) { return 1e-17; }
+KS_DEV double taylor_gate(f32) { return 1e-6; }
+
 // The step of one sample (step.jl:164-243) as polynomials in the scaled K' (K = alpha_I + K'):
 // O = o0 + o1 K' + o2 K'^2 + o3 K'^3 and Ej = Dp (e0 + e1 K' + e2 K'^2 + e3 K'^3), with
 // K'6 = (aQ, aU, aV, rQ, rU, rV) times Dp = pow2_below(D); D > 0. T is the scalar S or duals over it.
+// step_coefs_scaled takes the coefficients already multiplied by Dp (exact: Dp is a power of two)
+// and f = D / Dp in [1, 2), so duals seeded on the scaled values carry O(1) derivatives.
 template <class T, class S>
-KS_DEV void step_coefs(const T al[4], const T rh[3], S D, S& Dp, T K6[6], T o[4], T e4[4]) {
-    Dp = pow2_below(D);
-    const S f = D / Dp;
-    T aI = al[0] * Dp;
-    T aQ = al[1] * Dp, aU = al[2] * Dp, aV = al[3] * Dp;
-    T rQ = rh[0] * Dp, rU = rh[1] * Dp, rV = rh[2] * Dp;
+KS_DEV void step_coefs_scaled(const T alS[4], const T rhS[3], S f, T K6[6], T o[4], T e4[4]) {
+    const T &aI = alS[0], &aQ = alS[1], &aU = alS[2], &aV = alS[3];
+    const T &rQ = rhS[0], &rU = rhS[1], &rV = rhS[2];
     K6[0] = aQ; K6[1] = aU; K6[2] = aV; K6[3] = rQ; K6[4] = rU; K6[5] = rV;
     T a2 = aQ * aQ + aU * aU + aV * aV, r2 = rQ * rQ + rU * rU + rV * rV;
     T a = aI * f;
@@ -407,7 +412,7 @@ KS_DEV void step_coefs(const T al[4], const T rh[3], S D, S& Dp, T K6[6], T o[4]
     T L1 = L1sq > 0.0 ? sqrt(L1sq) : T(0.0), L2 = L2sq > 0.0 ? sqrt(L2sq) : T(0.0);
     T b1 = L1 * f, b2 = L2 * f;
     T Th = L1sq + L2sq;
-    if (val(Th) * f * f * (1.0 + val(a2 + r2) * f * f) < 1e-17) {
+    if (val(Th) * f * f * (1.0 + val(a2 + r2) * f * f) < taylor_gate(S())) {
         T M[10]; moments<3>(a, M);
         o[0] = e; o[1] = -f * e; o[2] = f * f * 0.5 * e; o[3] = -f * f * f / 6.0 * e;
         e4[0] = f * M[0]; e4[1] = -f * f * M[1]; e4[2] = f * f * f * M[2] * 0.5; e4[3] = -f * f * f * f * M[3] / 6.0;
@@ -464,6 +469,14 @@ KS_DEV void step_coefs(const T al[4], const T rh[3], S D, S& Dp, T K6[6], T o[4]
     e4[2] = f * (icm1 + ivs) * iTh;
     e4[1] = -(Ish * L2sq + Is * L1sq) * iTh;
     e4[3] = -(Ish - Is) * iTh;
+}
+template <class T, class S>
+KS_DEV void step_coefs(const T al[4], const T rh[3], S D, S& Dp, T K6[6], T o[4], T e4[4]) {
+    Dp = pow2_below(D);
+    T alS[4], rhS[3];
+    for (int i = 0; i < 4; ++i) alS[i] = al[i] * Dp;
+    for (int i = 0; i < 3; ++i) rhS[i] = rh[i] * Dp;
+    step_coefs_scaled(alS, rhS, S(D / Dp), K6, o, e4);
 }
 
 // y = K' v and y = K'^T v with K' = [0 aQ aU aV; aQ 0 rV -rU; aU -rV 0 rQ; aV rU -rQ 0]

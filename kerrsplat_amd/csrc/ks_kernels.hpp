@@ -27,16 +27,16 @@ constexpr int ADJ_ROWS = 21;       // parameter rows of a thermal splat
 #define KS_WAVES_MARCH 1
 #endif
 #ifndef KS_WAVES_TRANSFER64
-#define KS_WAVES_TRANSFER64 1
+#define KS_WAVES_TRANSFER64 2
 #endif
 #ifndef KS_WAVES_TRANSFER32
-#define KS_WAVES_TRANSFER32 1
+#define KS_WAVES_TRANSFER32 2
 #endif
 #ifndef KS_WAVES_ADJSTEP64
 #define KS_WAVES_ADJSTEP64 1
 #endif
 #ifndef KS_WAVES_ADJSTEP32
-#define KS_WAVES_ADJSTEP32 1
+#define KS_WAVES_ADJSTEP32 2
 #endif
 #ifndef KS_WAVES_ADJELEM64
 #define KS_WAVES_ADJELEM64 1
