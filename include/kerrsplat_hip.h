@@ -126,17 +126,44 @@ const double* ks_stokes_device_ptr(const ks_cache* cache);
  * thermal splat parameters, ADDED into dparams (21 x ns, column-major, as the reference's
  * dparams .+= ...). dstokes and the optional image (the observed Stokes vectors of the same
  * forward pass) are 4 x npix x nt x nν, in screen order when screen_order, else in the cache's
- * sorted order. Geodesic samples are not stored: the forward segment march records each sample's
- * coefficients, and the adjoint re-uses them. Thermal splats only (KS_ERR_ARG otherwise). */
+ * sorted order. Needs stored samples (store_samples and a storing marcher): the forward transfer
+ * records each sample's coefficients and the reverse pass re-uses them. Thermal splats only
+ * (KS_ERR_ARG otherwise). Equal to ks_gradient_forward followed by ks_gradient_reverse. */
 int ks_polarized_gradient(ks_cache* cache, const ks_model* model, const double* times, int nt, const double* nus, int nnu,
                           double L, int nmax, double slab, const double* dstokes, int screen_order,
                           double* dparams, double* image);
+
+/* The two passes of sweep_passes(:dual) (Fit.jl:451-474). forward(t, ν): the transfer with the
+ * records the reverse pass needs, the observed images left on the device (and copied to image when
+ * given, screen or sorted order). reverse!(dstokes, t, ν): the gradient of Σ dstokes · I_obs for
+ * the same model, ADDED into dparams; dstokes NULL takes the seeds that ks_vis_chi2_seed or
+ * ks_image_chi2_seed wrote on the device for every frame of the forward pass. */
+int ks_gradient_forward(ks_cache* cache, const ks_model* model, const double* times, int nt, const double* nus, int nnu,
+                        double L, int nmax, double slab, double* image, int screen_order);
+int ks_gradient_reverse(ks_cache* cache, const ks_model* model, const double* dstokes, int screen_order, double* dparams);
 
 /* visibilities / vis_kernel! - visibilities.jl:21-40, device_visibilities.jl:61-73: V of the
  * rendered frames at nuv (u, v) points [wavelengths], psize = Δα L / D [rad], taper per point
  * (NULL = 1); V = nframes x nuv x 4 complex as (re, im) pairs, Jy. */
 int ks_visibilities(ks_cache* cache, int64_t frames, double psize, const double* u, const double* v,
                     const double* taper, int nuv, double* V);
+
+/* frame_chi2_seed!(seed, c, images, fs, cache, Δα, L, D) - device_visibilities.jl:97-111: the χ² of
+ * frame `frame` (it + nt*inu) of the last ks_gradient_forward against visibility data,
+ * Σ |(g V - d) / σ|² over baselines and Stokes parameters, returned in chi2, and the adjoint seed
+ * of the weighted residuals 2 conj(g) r / σ written into the device seed of that frame for
+ * ks_gradient_reverse(dstokes = NULL). gains: nuv complex (re, im) g_{s1} conj(g_{s2}), NULL = 1;
+ * vis: nuv x 4 complex (re, im), Jy; sigma: nuv x 4, Jy; V (optional): the model visibilities. */
+int ks_vis_chi2_seed(ks_cache* cache, int64_t frame, double psize, const double* u, const double* v, const double* taper,
+                     const double* gains, const double* vis, const double* sigma, int nuv, double* chi2, double* V);
+
+/* _frame_seed! without a Binning (Fit.jl:537-577; the terms of chi2/_chi2_frames, Fit.jl:66-93):
+ * the χ² of frame `frame` of the last ks_gradient_forward against an observed Stokes image,
+ * Σ ((I - d) / σ)² over the masked pixels, returned in chi2, and its seed 2 r / σ (0 outside the
+ * mask) written into the device seed of that frame. data: 4 x npix, screen order, cgs; sigma: 4
+ * values, or 4 x npix when sigma_per_pixel; mask: npix bytes (nonzero = in χ²), NULL = all. */
+int ks_image_chi2_seed(ks_cache* cache, int64_t frame, const double* data, const double* sigma, int sigma_per_pixel,
+                       const uint8_t* mask, double* chi2);
 
 #ifdef __cplusplus
 }

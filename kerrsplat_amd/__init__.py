@@ -10,6 +10,7 @@ C-ABI of libkerrsplat_hip.so (include/kerrsplat_hip.h):
     st    = polarized_image(cache, params, t_obs, nu, L)          # (n_alpha, n_beta, 4)
     cube  = polarized_cube(cache, params, times, nus, L)          # (n_alpha, n_beta, nt, nnu, 4)
     dp, cube = polarized_gradient(cache, params, times, nus, L, dstokes)   # dual sweep (stored samples)
+    chi = fit.chi2_gradient(dp, params, StokesMovie(...), cache, L)       # χ² and its gradient, seeds on the device
 
 Every call goes to the GPU library; there is no CPU fallback. Loading fails loudly when the library
 is missing or no GPU is present.
@@ -17,9 +18,13 @@ is missing or no GPU is present.
 from .api import (Camera, Direct, Recurrence, Fused, GeodesicCache, Model, regenerate, thin_image, polarized_image,
                   polarized_cube, polarized_gradient, march, precision, stage_ms, render_device, visibilities, unsort, to_screen, samples, residuals, KernelError,
                   NSPLATPARAMS, NPOLARIZEDPARAMS, NPOWERLAWPARAMS, NKAPPAPARAMS, gravitational_radius, grid_axis,
-                  library_path)
+                  library_path, gradient_forward, gradient_reverse, vis_chi2_seed, image_chi2_seed)
+from . import fit
+from .fit import StokesMovie, FrameScans, chi2, chi2_gradient, sweep_passes, frame_chi2_seed, image_loss_gradient
 
 __all__ = ["Camera", "Direct", "Recurrence", "Fused", "GeodesicCache", "Model", "regenerate", "thin_image",
            "polarized_image", "polarized_cube", "polarized_gradient", "march", "precision", "stage_ms", "render_device", "visibilities", "unsort", "to_screen", "samples",
            "residuals", "KernelError", "NSPLATPARAMS", "NPOLARIZEDPARAMS", "NPOWERLAWPARAMS", "NKAPPAPARAMS",
-           "gravitational_radius", "grid_axis", "library_path"]
+           "gravitational_radius", "grid_axis", "library_path", "gradient_forward", "gradient_reverse", "vis_chi2_seed",
+           "image_chi2_seed", "fit", "StokesMovie", "FrameScans", "chi2", "chi2_gradient", "sweep_passes", "frame_chi2_seed",
+           "image_loss_gradient"]

@@ -76,6 +76,10 @@ def _load():
     lib.ks_timing_end.argtypes = [_P, ctypes.POINTER(ctypes.c_float)]
     lib.ks_stage_ms.argtypes = [_P, _i, ctypes.POINTER(ctypes.c_float)]
     lib.ks_polarized_gradient.argtypes = [_P, _P, _P, _i, _P, _i, _d, _i, _d, _P, _i, _P, _P]
+    lib.ks_gradient_forward.argtypes = [_P, _P, _P, _i, _P, _i, _d, _i, _d, _P, _i]
+    lib.ks_gradient_reverse.argtypes = [_P, _P, _P, _i, _P]
+    lib.ks_vis_chi2_seed.argtypes = [_P, _i64, _d, _P, _P, _P, _P, _P, _P, _i, _P, _P]
+    lib.ks_image_chi2_seed.argtypes = [_P, _i64, _P, _P, _i, _P, _P]
     lib.ks_march.argtypes = [_P]
     lib.ks_cache_precision.argtypes = [_P, _i]
     _lib = lib
@@ -414,3 +418,83 @@ def visibilities(cache, frames, psize, u, v, taper=None):
     V = np.zeros((frames, len(uu), 4, 2))
     _check(cache._lib.ks_visibilities(cache._h, _i64(frames), _d(psize), _ptr(uu), _ptr(vv), _ptr(tp), _i(len(uu)), _ptr(V)))
     return V[..., 0] + 1j * V[..., 1]
+
+
+def gradient_forward(cache, model, times, nus, L, nmax=-1, slab=0.0, copy=True):
+    """The forward pass of sweep_passes(:dual) (Fit.jl:451-474): the transfer with the records the
+    reverse pass needs; the observed images stay on the device. Returns them in sorted order,
+    (nnu, nt, npix, 4), when copy, else None."""
+    if model.kind != "thermal":
+        raise ValueError("the dual sweep takes thermal splats (PolarizedSplats)")
+    t, n = _f64(np.atleast_1d(times)), _f64(np.atleast_1d(nus))
+    out = np.zeros((len(n), len(t), cache.npix, 4)) if copy else None
+    _check(cache._lib.ks_gradient_forward(cache._h, model._h, _ptr(t), _i(len(t)), _ptr(n), _i(len(n)), _d(L), _i(nmax), _d(slab),
+                                          _ptr(out), _i(0)))
+    cache._fw = (len(t), len(n))
+    return out
+
+
+def gradient_reverse(cache, model, dparams, dstokes=None):
+    """The reverse pass of sweep_passes(:dual): adds d(sum dstokes . I_obs)/dparams of the last
+    forward pass into dparams (Fortran-ordered, shaped like the parameters). dstokes in sorted order,
+    (nnu, nt, npix, 4); None takes the device seeds that frame_chi2_seed / image_chi2_seed wrote."""
+    if dparams.shape != model.params.shape or not dparams.flags.f_contiguous or dparams.dtype != np.float64:
+        raise DimensionMismatch("dparams must be a Fortran-ordered float64 array shaped like the parameters")
+    ds = None
+    if dstokes is not None:
+        nt, nnu = getattr(cache, "_fw", (0, 0))
+        ds = np.ascontiguousarray(np.asarray(dstokes, dtype=np.float64))
+        if ds.size != nnu * nt * cache.npix * 4:
+            raise DimensionMismatch(f"dstokes must hold {(nnu, nt, cache.npix, 4)} values, got {ds.shape}")
+    _check(cache._lib.ks_gradient_reverse(cache._h, model._h, _ptr(ds), _i(0), _ptr(dparams)))
+    return dparams
+
+
+def vis_chi2_seed(cache, frame, psize, u, v, vis, sigma, taper=None, gains=None, return_model=False):
+    """frame_chi2_seed! (device_visibilities.jl:97-111) for frame `frame` (it + nt inu) of the last
+    forward pass: chi2 = sum |(g V - d) / sigma|^2 over baselines and Stokes parameters, and the
+    adjoint seed of the weighted residuals left on the device for gradient_reverse(dstokes=None).
+    vis: complex (nuv, 4) [Jy]; sigma: (nuv, 4) or a 4-vector [Jy]; gains: complex (nuv,) or None."""
+    uu, vv = _f64(u), _f64(v)
+    nuv = len(uu)
+    d = np.asarray(vis, dtype=np.complex128)
+    if d.shape != (nuv, 4):
+        raise DimensionMismatch(f"vis must be ({nuv}, 4), got {d.shape}")
+    dv = np.ascontiguousarray(np.stack([d.real, d.imag], axis=-1))
+    sg = np.broadcast_to(np.asarray(sigma, dtype=np.float64), (nuv, 4)).copy()
+    tp = _f64(taper) if taper is not None else None
+    g = None
+    if gains is not None:
+        gc = np.asarray(gains, dtype=np.complex128)
+        g = np.ascontiguousarray(np.stack([gc.real, gc.imag], axis=-1))
+    chi = np.zeros(1)
+    V = np.zeros((nuv, 4, 2)) if return_model else None
+    _check(cache._lib.ks_vis_chi2_seed(cache._h, _i64(frame), _d(psize), _ptr(uu), _ptr(vv), _ptr(tp), _ptr(g), _ptr(dv), _ptr(sg),
+                                       _i(nuv), _ptr(chi), _ptr(V)))
+    if return_model:
+        return float(chi[0]), V[..., 0] + 1j * V[..., 1]
+    return float(chi[0])
+
+
+def image_chi2_seed(cache, frame, data, sigma, mask=None):
+    """_frame_seed! without a binning (Fit.jl:537-577) for frame `frame` of the last forward pass:
+    chi2 = sum ((I - d) / sigma)^2 over the masked pixels and the seed 2 r / sigma on the device.
+    data: (n_alpha, n_beta, 4) observed Stokes [cgs]; sigma: a 4-vector or (n_alpha, n_beta, 4);
+    mask: (n_alpha, n_beta) booleans or None."""
+    na, nb = cache.screen_size
+    dd = np.asarray(data, dtype=np.float64)
+    if dd.shape != (na, nb, 4):
+        raise DimensionMismatch(f"data must be {(na, nb, 4)}, got {dd.shape}")
+    flat = lambda x: np.ascontiguousarray(np.transpose(x, (1, 0, 2)).reshape(na * nb, 4))   # pixel = i + na j
+    sg = np.asarray(sigma, dtype=np.float64)
+    per_pixel = sg.shape != (4,)
+    sgf = flat(np.broadcast_to(sg, (na, nb, 4))) if per_pixel else np.ascontiguousarray(sg)
+    mk = None
+    if mask is not None:
+        mm = np.asarray(mask, dtype=bool)
+        if mm.shape != (na, nb):
+            raise DimensionMismatch(f"mask must be {(na, nb)}, got {mm.shape}")
+        mk = np.ascontiguousarray(mm.T.reshape(na * nb).astype(np.uint8))
+    chi = np.zeros(1)
+    _check(cache._lib.ks_image_chi2_seed(cache._h, _i64(frame), _ptr(flat(dd)), _ptr(sgf), _i(int(per_pixel)), _ptr(mk), _ptr(chi)))
+    return float(chi[0])

@@ -71,6 +71,14 @@ KS_DEV void sample_adjoint(const S j4[4], const S a4[4], const S r3[3], S D, con
     }
 }
 
+// whether the splat's weight at the sample passes the cutoff, by the same arithmetic as weight_dlog
+template <class S>
+KS_DEV bool weight_active(const SplatCT<S>& c, S t, S x, S y, S z) {
+    S sp, cp;
+    if (outside_support(c, t, x, y, z, sp, cp)) return false;
+    return exp(-0.5 * gauss_exponent(c, t, x, y, z, sp, cp)) > WEIGHT_CUTOFF;
+}
+
 // d ln G of the splat weight (Splats.jl:58-93) with respect to rows 1-12 and the pattern rate; the
 // Gaussian itself is returned (0 when culled). dl[0..11] = rows 1..12, dl[12] = omega.
 template <class S>

@@ -18,6 +18,7 @@ namespace ksd {
 
 constexpr double ME = 9.1093826e-28, CL = 2.99792458e10, EE = 4.80320680e-10, HPL = 6.6260693e-27;
 constexpr double NU0 = 1e11;
+constexpr double JY = 1e-23;                  // erg s^-1 cm^-2 Hz^-1
 constexpr double PLANCK0 = 2.0 * HPL * NU0 * NU0 * NU0 / (CL * CL);
 constexpr double WEIGHT_CUTOFF = 1e-6;
 

@@ -10,7 +10,7 @@ namespace ksd {
 // Per item: from the chunk maps (P_m, S_m), the tail behind each chunk T_out[m] = S_{m+1} + P_{m+1} T_out[m+1]
 // (T_out[last] = 0), the image nu-hat^3 T_in[0], and the adjoint in front of each chunk
 // w_in[m] = (P_0 ... P_{m-1})^T w_1 with w_1 = nu-hat^3 dstokes (dual_gradient.jl:162).
-// wT[(8 m + q) nitems + item] = w_in, wT[(8 m + 4 + q) nitems + item] = T_out.
+// wT[(8 m + q) nitems + item] = w_in, wT[(8 m + 4 + q) nitems + item] = T_out; the image only when out is given.
 __global__ void ks_adj_combine(int64_t npix, int N, int nt, const double* __restrict__ nus, int nnu, const double* __restrict__ seg,
                                const double* __restrict__ dstokes, double* __restrict__ wT, double* __restrict__ out) {
     const int64_t nitems = npix * nt * nnu;
@@ -32,7 +32,8 @@ __global__ void ks_adj_combine(int64_t npix, int N, int nt, const double* __rest
         }
         for (int i = 0; i < 4; ++i) T[i] = Tn[i];
     }
-    for (int q = 0; q < 4; ++q) out[4 * item + q] = T[q] * s3;
+    if (out)
+        for (int q = 0; q < 4; ++q) out[4 * item + q] = T[q] * s3;
     double w[4];
     for (int q = 0; q < 4; ++q) w[q] = dstokes[4 * item + q] * s3;
     for (int m = 0; m < nseg; ++m) {
@@ -127,46 +128,94 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(Waves<
     }
 }
 
-// Per (64 consecutive items, splat): every step-taking sample of the items through the splat's
-// element adjoint, 21 partial sums per lane, summed over the wavefront; part[(ib ns + i) 21 + r].
+// Per (64 consecutive items, splat): the element adjoints of every (item, sample) pair inside the
+// splat's weight cutoff, 21 partial sums per lane, summed over the wavefront; part[(ib ns + i) 21 + r].
+// The wavefront first walks its items' samples with the cheap weight test and queues the pairs that
+// pass (ballot + mbcnt into an LDS ring), then runs the element adjoint 64 pairs at a time, so that
+// lanes outside the splat's support do not idle through it. Every pair of a wavefront belongs to
+// splat i, so the sums stay per lane in registers; the queue order is fixed, so the result is
+// deterministic.
 template <class S>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(Waves<S>::adjelem))) ks_adj_splats(double a, double thetao, const double* __restrict__ soa, int64_t npix, int N, Samples smp,
                                                     const SplatCT<S>* __restrict__ sp, const SplatAdjT<S>* __restrict__ sadj, int ns,
                                                     const double* __restrict__ times, int nt, const double* __restrict__ nus, int64_t nitems,
                                                     const S* __restrict__ rec, const uint8_t* __restrict__ aflg, double* __restrict__ part) {
+    constexpr int QCAP = 128;   // ring capacity: < 64 pending + 64 pushed by one sample step
+    __shared__ double s_alpha[64], s_beta[64], s_eta[64], s_lam[64], s_tobs[64], s_nu[64];
+    __shared__ int s_q[QCAP];   // entry = lane | k << 6
     const int i = (int)(blockIdx.x % ns);
     const int64_t ib = blockIdx.x / ns;
-    const int64_t item = ib * 64 + threadIdx.x;
+    const int lane = (int)threadIdx.x;
+    const int64_t item = ib * 64 + lane;
+    const bool valid = item < nitems;
+    const SplatCT<S> c = sp[i];
+    const SplatAdjT<S> sa = sadj[i];
+    int64_t j = 0;
+    double t_obs = 0.0;
+    if (valid) {
+        j = item % npix;
+        const int64_t fr = item / npix;
+        t_obs = times[fr % nt];
+        s_alpha[lane] = soa[(int64_t)PF_ALPHA * npix + j];
+        s_beta[lane] = soa[(int64_t)PF_BETA * npix + j];
+        s_eta[lane] = soa[(int64_t)PF_ETA * npix + j];
+        s_lam[lane] = soa[(int64_t)PF_LAM * npix + j];
+        s_tobs[lane] = t_obs;
+        s_nu[lane] = nus[fr / nt];
+    }
+    __syncthreads();
+    const S tos = S(sin(thetao)), aS = S(a);
+    const int64_t st = (int64_t)N * nitems;
     S grad[ADJ_ROWS];
     for (int r = 0; r < ADJ_ROWS; ++r) grad[r] = S(0.0);
-    if (item < nitems) {
-        const int64_t j = item % npix;
-        const int64_t fr = item / npix;
-        const int it = (int)(fr % nt), inu = (int)(fr / nt);
-        const S nu = S(nus[inu]);
-        const double t_obs = times[it];
-        const SplatCT<S> c = sp[i];
-        const SplatAdjT<S> sa = sadj[i];
-        const double alpha = soa[(int64_t)PF_ALPHA * npix + j], beta = soa[(int64_t)PF_BETA * npix + j];
-        const double eta = soa[(int64_t)PF_ETA * npix + j], lam = soa[(int64_t)PF_LAM * npix + j];
-        const S tos = S(sin(thetao)), aS = S(a), alS = S(alpha), beS = S(beta);
-        const int64_t st = (int64_t)N * nitems;
-        for (int k = 1; k <= N; ++k) {
-            const int64_t fo = (int64_t)(k - 1) * nitems + item;
-            if (!aflg[fo]) continue;
-            const int64_t so = (int64_t)(k - 1) * npix + j;
+    int qhead = 0, qn = 0;   // wavefront-uniform ring state
+    auto run_batch = [&](int cnt) {
+        if (lane < cnt) {
+            const int e = s_q[(qhead + lane) & (QCAP - 1)];
+            const int l = e & 63, k = e >> 6;
+            const int64_t it2 = ib * 64 + l;
+            const int64_t fo = (int64_t)(k - 1) * nitems + it2;
+            const int64_t so = (int64_t)(k - 1) * npix + it2 % npix;
             S dl[13];
-            const S Gw = weight_dlog(c, sa, S(t_obs - smp.t[so]), S(smp.x[so]), S(smp.y[so]), S(smp.z[so]), dl);
-            if (!(Gw > 0.0)) continue;
-            const S* o = rec + fo;
-            S bars[11];
-            for (int q = 0; q < 11; ++q) bars[q] = o[(REC_BAR + q) * st];
-            const uint8_t fl = smp.fl[so];
-            Geo Gd;
-            make_geo(a, smp.r[so], smp.th[so], eta, lam, (fl & 2) != 0, (fl & 4) != 0, Gd);
-            const GeoT<S> G = geo_as<S>(Gd);
-            element_adjoint_thermal(c, sa, G, aS, alS, beS, tos, nu, bars, Gw, dl, grad);
+            const S Gw = weight_dlog(c, sa, S(s_tobs[l] - smp.t[so]), S(smp.x[so]), S(smp.y[so]), S(smp.z[so]), dl);
+            if (Gw > 0.0) {
+                const S* o = rec + fo;
+                S bars[11];
+                for (int q = 0; q < 11; ++q) bars[q] = o[(REC_BAR + q) * st];
+                const uint8_t fl = smp.fl[so];
+                Geo Gd;
+                make_geo(a, smp.r[so], smp.th[so], s_eta[l], s_lam[l], (fl & 2) != 0, (fl & 4) != 0, Gd);
+                const GeoT<S> G = geo_as<S>(Gd);
+                element_adjoint_thermal(c, sa, G, aS, S(s_alpha[l]), S(s_beta[l]), tos, S(s_nu[l]), bars, Gw, dl, grad);
+            }
         }
+        qhead = (qhead + cnt) & (QCAP - 1);
+        qn -= cnt;
+    };
+    for (int k = 1; k <= N; ++k) {
+        bool act = false;
+        if (valid) {
+            const int64_t fo = (int64_t)(k - 1) * nitems + item;
+            if (aflg[fo]) {
+                const int64_t so = (int64_t)(k - 1) * npix + j;
+                act = weight_active(c, S(t_obs - smp.t[so]), S(smp.x[so]), S(smp.y[so]), S(smp.z[so]));
+            }
+        }
+        const uint64_t m = __ballot(act);
+        if (act) {
+            const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            s_q[(qhead + qn + rank) & (QCAP - 1)] = lane | (k << 6);
+        }
+        qn += __popcll(m);
+        if (qn >= 64) {
+            __syncthreads();
+            run_batch(64);
+            __syncthreads();
+        }
+    }
+    if (qn > 0) {
+        __syncthreads();
+        run_batch(qn);
     }
     // wavefront sum (blockDim = 64 = one wavefront), in FP64
     double gsum[ADJ_ROWS];

@@ -1,34 +1,254 @@
-// Visibilities of rendered Stokes images (visibilities.jl:21-40, device_visibilities.jl:53-111).
+// Visibilities of rendered Stokes images, their χ² against data and the adjoint seeds
+// (visibilities.jl:21-40, device_visibilities.jl:53-111; the image χ² of _frame_seed!, Fit.jl:537-577).
+//
+// On Krang's pixel grid the phase of pixel (i, j) at baseline k separates,
+//   2 pi (u_k l_i + v_k m_j) = 2 pi u_k l_i + 2 pi v_k m_j,
+// so V_k = tap_k scale sum_j By[k, j] sum_i Ax[k, i] I[i, j] with Ax = e^{2 pi i u l}, By = e^{2 pi i v m}.
+// The inner sum over i is a real matrix product with the phase table stacked as [cos; sin]
+// (2 nuv x n_alpha) against the image as an n_alpha x 4 n_beta matrix; the adjoint seed
+// Re sum_k w_k conj(Ax By) is the transposed product of the same table with the weights rotated by
+// conj(By). Both products run through one FP64 tiled GEMM; everything else is O(nuv (n_alpha + n_beta)).
 #include "ks_kernels.hpp"
 
 namespace ksd {
 
-// ---- visibilities: one work item per (uv point, frame); images in sorted order with perm ---------------------------
-__global__ void ks_vis_direct(const double* __restrict__ img, const int* __restrict__ perm, int64_t npix, int nalpha, int nbeta, int nframes,
-                              double psize, double scale, const double* __restrict__ u, const double* __restrict__ v, const double* __restrict__ tap,
-                              int nuv, double* __restrict__ V /* nframes x nuv x 4 complex (re, im) */) {
-    int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (item >= (int64_t)nuv * nframes) return;
-    int k = (int)(item % nuv);
-    int64_t f = item / nuv;
-    double uk = u[k], vk = v[k];
-    double re[4] = {0, 0, 0, 0}, im[4] = {0, 0, 0, 0};
-    const double* I = img + 4 * npix * f;
-    for (int64_t j = 0; j < npix; ++j) {
-        int pix = perm[j];
-        int ia = pix % nalpha, ib = pix / nalpha;
-        double l = -((ia + 1) - (nalpha + 1) * 0.5) * psize, m = ((ib + 1) - (nbeta + 1) * 0.5) * psize;
-        double s, c; sincos(2.0 * PI * (uk * l + vk * m), &s, &c);
-        for (int q = 0; q < 4; ++q) { double x = I[4 * j + q] * scale; re[q] += x * c; im[q] += x * s; }
+// ---- phase tables: A2[k][i] = cos(2 pi u_k l_i), A2[nuv + k][i] = sin(...); B2 likewise for v_k m_j ---------------
+// l_i = -((i + 1) - (n_alpha + 1) / 2) psize (east = -alpha), m_j = ((j + 1) - (n_beta + 1) / 2) psize
+__global__ void ks_vis_tables(int nuv, int na, int nb, double psize, const double* __restrict__ u, const double* __restrict__ v,
+                              double* __restrict__ A2, double* __restrict__ B2) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int row = na + nb;
+    if (t >= (int64_t)nuv * row) return;
+    const int k = (int)(t / row), q = (int)(t % row);
+    double s, c;
+    if (q < na) {
+        const double l = -((q + 1) - (na + 1) * 0.5) * psize;
+        sincos(2.0 * PI * (u[k] * l), &s, &c);
+        A2[(int64_t)k * na + q] = c;
+        A2[(int64_t)(nuv + k) * na + q] = s;
+    } else {
+        const int j = q - na;
+        const double m = ((j + 1) - (nb + 1) * 0.5) * psize;
+        sincos(2.0 * PI * (v[k] * m), &s, &c);
+        B2[(int64_t)k * nb + j] = c;
+        B2[(int64_t)(nuv + k) * nb + j] = s;
     }
-    double tk = tap ? tap[k] : 1.0;
-    for (int q = 0; q < 4; ++q) { V[2 * (4 * item + q)] = re[q] * tk; V[2 * (4 * item + q) + 1] = im[q] * tk; }
 }
 
-hipError_t launch_vis_direct(hipStream_t s, const double* img, const int* perm, int64_t npix, int nalpha, int nbeta, int nframes,
-                             double psize, double scale, const double* u, const double* v, const double* tap, int nuv, double* V) {
-    hipLaunchKernelGGL(ks_vis_direct, dim3(grid_for((int64_t)nuv * nframes, 64)), dim3(64), 0, s, img, perm, npix, nalpha, nbeta, nframes,
-                       psize, scale, u, v, tap, nuv, V);
+// ---- the image as n_alpha x 4 n_beta row-major matrices, one per frame, from the sorted cube ----------------------
+// Im[f][i][4 j + q] = out[4 (s + npix f) + q] with perm[s] = i + n_alpha j
+__global__ void ks_vis_gather(const double* __restrict__ out, const int* __restrict__ perm, int64_t npix, int na, int nb, int64_t f0, int nf,
+                              double* __restrict__ Im) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npix * nf) return;
+    const int64_t s = t % npix, f = t / npix;
+    const int p = perm[s];
+    const int i = p % na, j = p / na;
+    const double* src = out + 4 * (s + npix * (f0 + f));
+    double* dst = Im + (int64_t)npix * 4 * f + (int64_t)i * 4 * nb + 4 * j;
+    for (int q = 0; q < 4; ++q) dst[q] = src[q];
+}
+
+// ---- FP64 tiled GEMM: C[M x N] = op(A) B, row-major; A is M x K, or K x M with TA; batched over blockIdx.z ---------
+// 64 x 64 tile per 256-thread block, K in steps of 16 through LDS, 4 x 4 outputs per thread (rows ty + 16 r,
+// columns tx + 16 c, so that a wavefront's LDS reads fall in distinct banks).
+constexpr int GT = 64, GK = 16;
+template <bool TA>
+__global__ void __launch_bounds__(256) ks_dgemm(int M, int N, int K, const double* __restrict__ A, int64_t lda, int64_t sA,
+                                                const double* __restrict__ B, int64_t ldb, int64_t sB, double* __restrict__ C, int64_t ldc,
+                                                int64_t sC) {
+    __shared__ double As[GK][GT + 1];
+    __shared__ double Bs[GK][GT];
+    A += sA * blockIdx.z; B += sB * blockIdx.z; C += sC * blockIdx.z;
+    const int tid = threadIdx.x, tx = tid % 16, ty = tid / 16;
+    const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
+    double acc[4][4] = {};
+    for (int k0 = 0; k0 < K; k0 += GK) {
+        for (int e = tid; e < GT * GK; e += 256) {
+            int mm, kk;
+            if (TA) { kk = e / GT; mm = e % GT; }
+            else { mm = e / GK; kk = e % GK; }
+            const int gm = m0 + mm, gk = k0 + kk;
+            double a = 0.0;
+            if (gm < M && gk < K) a = TA ? A[(int64_t)gk * lda + gm] : A[(int64_t)gm * lda + gk];
+            As[kk][mm] = a;
+            const int bk = e / GT, bn = e % GT;
+            const int gk2 = k0 + bk, gn = n0 + bn;
+            Bs[bk][bn] = (gk2 < K && gn < N) ? B[(int64_t)gk2 * ldb + gn] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < GK; ++kk) {
+            double a[4], b[4];
+            for (int r = 0; r < 4; ++r) a[r] = As[kk][ty + 16 * r];
+            for (int c = 0; c < 4; ++c) b[c] = Bs[kk][tx + 16 * c];
+            for (int r = 0; r < 4; ++r)
+                for (int c = 0; c < 4; ++c) acc[r][c] = fma(a[r], b[c], acc[r][c]);
+        }
+        __syncthreads();
+    }
+    for (int r = 0; r < 4; ++r) {
+        const int gm = m0 + ty + 16 * r;
+        if (gm >= M) continue;
+        for (int c = 0; c < 4; ++c) {
+            const int gn = n0 + tx + 16 * c;
+            if (gn < N) C[(int64_t)gm * ldc + gn] = acc[r][c];
+        }
+    }
+}
+
+// ---- V[f][k][q] = tap_k scale sum_j (Cr + i Ci)[k][4 j + q] (Br + i Bi)[k][j]; V as (re, im) pairs ------------------
+// C2 per frame: 2 nuv x 4 n_beta (rows k: real part, rows nuv + k: imaginary part)
+__global__ void ks_vis_combine(int nuv, int nb, int nf, const double* __restrict__ C2, const double* __restrict__ B2,
+                               const double* __restrict__ tap, double scale, double* __restrict__ V) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nuv * 4 * nf) return;
+    const int q = (int)(t % 4), k = (int)((t / 4) % nuv);
+    const int64_t f = t / (4 * (int64_t)nuv);
+    const double* Cr = C2 + (int64_t)f * 2 * nuv * 4 * nb + (int64_t)k * 4 * nb + q;
+    const double* Ci = Cr + (int64_t)nuv * 4 * nb;
+    const double* Br = B2 + (int64_t)k * nb;
+    const double* Bi = Br + (int64_t)nuv * nb;
+    double re = 0.0, im = 0.0;
+    for (int j = 0; j < nb; ++j) {
+        const double cr = Cr[4 * j], ci = Ci[4 * j], br = Br[j], bi = Bi[j];
+        re += cr * br - ci * bi;
+        im += cr * bi + ci * br;
+    }
+    const double w = (tap ? tap[k] : 1.0) * scale;
+    V[2 * t] = re * w;
+    V[2 * t + 1] = im * w;
+}
+
+// ---- residuals r = (g V - d) / sigma per Stokes parameter, chi2 per baseline, and the weights rotated for the seed --
+// w_k = 2 conj(g_k) r_k / sigma_k (device_visibilities.jl:106), times tap_k scale (dV/dI); the seed product
+// then takes E2[k][4 j + q] = Re(w_kq conj(By[k][j])), E2[nuv + k][4 j + q] = Im(...).
+// V, vis: nuv x 4 complex (re, im); sigma: nuv x 4; gains: nuv complex or null.
+__global__ void ks_vis_residual(int nuv, const double* __restrict__ V, const double* __restrict__ vis, const double* __restrict__ sigma,
+                               const double* __restrict__ gains, const double* __restrict__ tap, double scale, double* __restrict__ wq,
+                               double* __restrict__ chi_k) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nuv) return;
+    const double gr = gains ? gains[2 * k] : 1.0, gi = gains ? gains[2 * k + 1] : 0.0;
+    const double t = (tap ? tap[k] : 1.0) * scale;
+    double chi = 0.0;
+    for (int q = 0; q < 4; ++q) {
+        const int64_t o = 4 * (int64_t)k + q;
+        const double vr = V[2 * o], vi = V[2 * o + 1];
+        const double is = 1.0 / sigma[o];
+        const double rr = (gr * vr - gi * vi - vis[2 * o]) * is, ri = (gr * vi + gi * vr - vis[2 * o + 1]) * is;
+        chi += rr * rr + ri * ri;
+        // 2 conj(g) r / sigma
+        wq[2 * o] = 2.0 * (gr * rr + gi * ri) * is * t;
+        wq[2 * o + 1] = 2.0 * (gr * ri - gi * rr) * is * t;
+    }
+    chi_k[k] = chi;
+}
+__global__ void ks_vis_rotate(int nuv, int nb, const double* __restrict__ wq, const double* __restrict__ B2, double* __restrict__ E2) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nuv * nb) return;
+    const int k = (int)(t / nb), j = (int)(t % nb);
+    const double br = B2[(int64_t)k * nb + j], bi = B2[(int64_t)(nuv + k) * nb + j];
+    double* er = E2 + (int64_t)k * 4 * nb + 4 * j;
+    double* ei = er + (int64_t)nuv * 4 * nb;
+    for (int q = 0; q < 4; ++q) {
+        const double wr = wq[2 * (4 * (int64_t)k + q)], wi = wq[2 * (4 * (int64_t)k + q) + 1];
+        er[q] = wr * br + wi * bi;      // w conj(b)
+        ei[q] = wi * br - wr * bi;
+    }
+}
+
+// seed (n_alpha x 4 n_beta, screen) -> the dual sweep's seed slot of frame f in sorted order
+__global__ void ks_vis_scatter_seed(const double* __restrict__ Sd, const int* __restrict__ perm, int64_t npix, int na, int nb, int64_t f,
+                                    double* __restrict__ ads) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= npix) return;
+    const int p = perm[s];
+    const int i = p % na, j = p / na;
+    const double* src = Sd + (int64_t)i * 4 * nb + 4 * j;
+    double* dst = ads + 4 * (s + npix * f);
+    for (int q = 0; q < 4; ++q) dst[q] = src[q];
+}
+
+// ---- the image χ² of one frame (_frame_seed!, unbinned): r = (I - d) / sigma over masked pixels, seed 2 r / sigma ---
+// data, sigma (4 or 4 per pixel) and mask (per pixel, or null) in screen order; image and seed in sorted order.
+__global__ void ks_image_residual(const double* __restrict__ out, const int* __restrict__ perm, int64_t npix, int64_t f,
+                                  const double* __restrict__ data, const double* __restrict__ sigma, int sigma_per_pixel,
+                                  const uint8_t* __restrict__ mask, double* __restrict__ ads, double* __restrict__ chi_p) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= npix) return;
+    const int p = perm[s];
+    const double* img = out + 4 * (s + npix * f);
+    double* seed = ads + 4 * (s + npix * f);
+    double chi = 0.0;
+    if (!mask || mask[p]) {
+        const double* sg = sigma + (sigma_per_pixel ? 4 * (int64_t)p : 0);
+        for (int q = 0; q < 4; ++q) {
+            const double is = 1.0 / sg[q];
+            const double r = (img[q] - data[4 * (int64_t)p + q]) * is;
+            chi += r * r;
+            seed[q] = 2.0 * r * is;
+        }
+    } else {
+        for (int q = 0; q < 4; ++q) seed[q] = 0.0;
+    }
+    chi_p[s] = chi;
+}
+
+// ---- deterministic sum of n values by one block: strided per-thread sums, then a fixed tree ----------------------
+__global__ void __launch_bounds__(256) ks_sum_fixed(const double* __restrict__ x, int64_t n, double* __restrict__ out) {
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 256) s += x[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+// ---- launchers ----------------------------------------------------------------------------------------------------
+hipError_t launch_vis_tables(hipStream_t s, int nuv, int na, int nb, double psize, const double* u, const double* v, double* A2, double* B2) {
+    hipLaunchKernelGGL(ks_vis_tables, dim3(grid_for((int64_t)nuv * (na + nb), 256)), dim3(256), 0, s, nuv, na, nb, psize, u, v, A2, B2);
+    return hipGetLastError();
+}
+hipError_t launch_vis_gather(hipStream_t s, const double* out, const int* perm, int64_t npix, int na, int nb, int64_t f0, int nf, double* Im) {
+    hipLaunchKernelGGL(ks_vis_gather, dim3(grid_for(npix * nf, 256)), dim3(256), 0, s, out, perm, npix, na, nb, f0, nf, Im);
+    return hipGetLastError();
+}
+hipError_t launch_dgemm(hipStream_t s, bool transA, int M, int N, int K, const double* A, int64_t lda, int64_t sA, const double* B,
+                        int64_t ldb, int64_t sB, double* C, int64_t ldc, int64_t sC, int batch) {
+    const dim3 grid((N + GT - 1) / GT, (M + GT - 1) / GT, batch);
+    if (transA) hipLaunchKernelGGL(ks_dgemm<true>, grid, dim3(256), 0, s, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC);
+    else hipLaunchKernelGGL(ks_dgemm<false>, grid, dim3(256), 0, s, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC);
+    return hipGetLastError();
+}
+hipError_t launch_vis_combine(hipStream_t s, int nuv, int nb, int nf, const double* C2, const double* B2, const double* tap, double scale, double* V) {
+    hipLaunchKernelGGL(ks_vis_combine, dim3(grid_for((int64_t)nuv * 4 * nf, 256)), dim3(256), 0, s, nuv, nb, nf, C2, B2, tap, scale, V);
+    return hipGetLastError();
+}
+hipError_t launch_vis_residual(hipStream_t s, int nuv, const double* V, const double* vis, const double* sigma, const double* gains,
+                               const double* tap, double scale, double* wq, double* chi_k) {
+    hipLaunchKernelGGL(ks_vis_residual, dim3(grid_for(nuv, 256)), dim3(256), 0, s, nuv, V, vis, sigma, gains, tap, scale, wq, chi_k);
+    return hipGetLastError();
+}
+hipError_t launch_vis_rotate(hipStream_t s, int nuv, int nb, const double* wq, const double* B2, double* E2) {
+    hipLaunchKernelGGL(ks_vis_rotate, dim3(grid_for((int64_t)nuv * nb, 256)), dim3(256), 0, s, nuv, nb, wq, B2, E2);
+    return hipGetLastError();
+}
+hipError_t launch_vis_scatter_seed(hipStream_t s, const double* Sd, const int* perm, int64_t npix, int na, int nb, int64_t f, double* ads) {
+    hipLaunchKernelGGL(ks_vis_scatter_seed, dim3(grid_for(npix, 256)), dim3(256), 0, s, Sd, perm, npix, na, nb, f, ads);
+    return hipGetLastError();
+}
+hipError_t launch_image_residual(hipStream_t s, const double* out, const int* perm, int64_t npix, int64_t f, const double* data,
+                                 const double* sigma, int sigma_per_pixel, const uint8_t* mask, double* ads, double* chi_p) {
+    hipLaunchKernelGGL(ks_image_residual, dim3(grid_for(npix, 256)), dim3(256), 0, s, out, perm, npix, f, data, sigma, sigma_per_pixel, mask, ads, chi_p);
+    return hipGetLastError();
+}
+hipError_t launch_sum_fixed(hipStream_t s, const double* x, int64_t n, double* out) {
+    hipLaunchKernelGGL(ks_sum_fixed, dim3(1), dim3(256), 0, s, x, n, out);
     return hipGetLastError();
 }
 

@@ -112,8 +112,18 @@ hipError_t launch_adj_splats(hipStream_t s, double a, double thetao, const doubl
 hipError_t launch_adj_reduce(hipStream_t s, const double* part, int64_t nib, int ns, double* dparams);
 
 // ---- ks_k_vis.hip ----
-hipError_t launch_vis_direct(hipStream_t s, const double* img, const int* perm, int64_t npix, int nalpha, int nbeta, int nframes,
-                             double psize, double scale, const double* u, const double* v, const double* tap, int nuv, double* V);
+hipError_t launch_vis_tables(hipStream_t s, int nuv, int na, int nb, double psize, const double* u, const double* v, double* A2, double* B2);
+hipError_t launch_vis_gather(hipStream_t s, const double* out, const int* perm, int64_t npix, int na, int nb, int64_t f0, int nf, double* Im);
+hipError_t launch_dgemm(hipStream_t s, bool transA, int M, int N, int K, const double* A, int64_t lda, int64_t sA, const double* B,
+                        int64_t ldb, int64_t sB, double* C, int64_t ldc, int64_t sC, int batch);
+hipError_t launch_vis_combine(hipStream_t s, int nuv, int nb, int nf, const double* C2, const double* B2, const double* tap, double scale, double* V);
+hipError_t launch_vis_residual(hipStream_t s, int nuv, const double* V, const double* vis, const double* sigma, const double* gains,
+                               const double* tap, double scale, double* wq, double* chi_k);
+hipError_t launch_vis_rotate(hipStream_t s, int nuv, int nb, const double* wq, const double* B2, double* E2);
+hipError_t launch_vis_scatter_seed(hipStream_t s, const double* Sd, const int* perm, int64_t npix, int na, int nb, int64_t f, double* ads);
+hipError_t launch_image_residual(hipStream_t s, const double* out, const int* perm, int64_t npix, int64_t f, const double* data,
+                                 const double* sigma, int sigma_per_pixel, const uint8_t* mask, double* ads, double* chi_p);
+hipError_t launch_sum_fixed(hipStream_t s, const double* x, int64_t n, double* out);
 
 inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 

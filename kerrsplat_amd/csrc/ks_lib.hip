@@ -39,6 +39,19 @@ template <class T> int dalloc(T** p, size_t n) {
     return 0;
 }
 template <class T> void dfree(T*& p) { if (p) { (void)hipFree(p); p = nullptr; } }
+// a grow-only device buffer
+template <class T> struct Buf {
+    T* p = nullptr;
+    size_t cap = 0;
+    int grow(size_t n) {
+        if (cap >= n && p) return 0;
+        dfree(p); cap = 0;
+        if (dalloc(&p, n)) return KS_ERR_HIP;
+        cap = n;
+        return 0;
+    }
+    void release() { dfree(p); cap = 0; }
+};
 
 }  // namespace
 
@@ -76,8 +89,12 @@ struct ks_cache {
     double* out = nullptr; int64_t out_frames = 0;    // sorted-order Stokes images
     double* out_screen = nullptr;
     double* img1 = nullptr;                            // thin image (sorted)
-    double *vu = nullptr, *vv = nullptr, *vt = nullptr, *vV = nullptr;   // visibility buffers
-    int64_t vis_cap = 0, visV_cap = 0;
+    // visibility and χ² workspace (grow-only): (u, v, taper, gains, data, sigma), the phase tables
+    // A2 (2 nuv x n_alpha) and B2 (2 nuv x n_beta), the images as n_alpha x 4 n_beta matrices, the
+    // partial products C2 (2 nuv x 4 n_beta per frame), V, the rotated weights E2, the screen seed,
+    // per-baseline or per-pixel χ² terms and their sum, the image data, noise and mask
+    Buf<double> vu, vv, vt, vg, vd, vs, vA2, vB2, vIm, vC2, vV, vwq, vE2, vSd, vchi, vsum, idata, isig;
+    Buf<uint8_t> imask;
     double* seg = nullptr; int64_t seg_cap = 0;        // per-segment transfer maps
     double* dtn = nullptr; int64_t dtn_cap = 0;        // observation times then frequencies (device)
     std::vector<double> htn;                           // their host staging copy (outlives the async copy)
@@ -87,6 +104,10 @@ struct ks_cache {
     double* arec = nullptr; uint8_t* aflg = nullptr; double* atail = nullptr; double* awT = nullptr; double* ads = nullptr;
     double* apart = nullptr; double* adpar = nullptr;
     int64_t a_items = 0, a_part = 0, a_dpar = 0;
+    // the last forward pass of the dual sweep (ks_gradient_forward): what ks_gradient_reverse needs
+    const ks_model* fw_model = nullptr;
+    int fw_nt = 0, fw_nnu = 0, fw_generation = -1, fw_precision = -1;
+    std::vector<uint8_t> fw_seeded;                    // frames whose device seed a *_chi2_seed call wrote
     std::vector<int> perm_host;
 };
 
@@ -164,6 +185,7 @@ int ks_stage_ms(ks_ctx* c, int stage, float* ms) {
 int ks_cache_create(ks_ctx* ctx, int64_t npix, int N, int store_samples, ks_cache** out) {
     if (!ctx || !out) return fail(KS_ERR_ARG, "ks_cache_create: null argument");
     if (N < 1) return fail(KS_ERR_ARG, "need at least one sample per ray");
+    if (N > (1 << 24)) return fail(KS_ERR_ARG, "N = %d above the supported 2^24 samples per ray", N);
     if (npix < 1) return fail(KS_ERR_ARG, "need at least one pixel");
     if (npix > ((int64_t)1 << 30)) return fail(KS_ERR_ARG, "npix %lld above the supported 2^30", (long long)npix);
     HIPCHK(hipSetDevice(ctx->device));
@@ -198,7 +220,10 @@ int ks_cache_destroy(ks_cache* c) {
     dfree(c->st); dfree(c->sr); dfree(c->sth); dfree(c->sph); dfree(c->sfl); dfree(c->rt); dfree(c->rp);
     dfree(c->sx); dfree(c->sy); dfree(c->sz); dfree(c->thin_part);
     dfree(c->out); dfree(c->out_screen); dfree(c->img1);
-    dfree(c->vu); dfree(c->vv); dfree(c->vt); dfree(c->vV);
+    for (Buf<double>* b : {&c->vu, &c->vv, &c->vt, &c->vg, &c->vd, &c->vs, &c->vA2, &c->vB2, &c->vIm, &c->vC2, &c->vV, &c->vwq, &c->vE2,
+                           &c->vSd, &c->vchi, &c->vsum, &c->idata, &c->isig})
+        b->release();
+    c->imask.release();
     dfree(c->seg); dfree(c->segres); dfree(c->kstop); dfree(c->dtn);
     dfree(c->arec); dfree(c->aflg); dfree(c->atail); dfree(c->awT); dfree(c->ads); dfree(c->apart); dfree(c->adpar);
     delete c;
@@ -599,9 +624,10 @@ static int ensure_adj(ks_cache* c, int64_t items, int ns) {
     return KS_OK;
 }
 
-int ks_polarized_gradient(ks_cache* c, const ks_model* m, const double* times, int nt, const double* nus, int nnu, double L, int nmax, double slab,
-                          const double* dstokes, int screen_order, double* dparams, double* image) {
-    if (!c || !m || !times || !nus || !dstokes || !dparams) return fail(KS_ERR_ARG, "ks_polarized_gradient: null argument");
+// forward pass of the dual sweep: the transfer with the per-sample records, and the images
+int ks_gradient_forward(ks_cache* c, const ks_model* m, const double* times, int nt, const double* nus, int nnu, double L, int nmax, double slab,
+                        double* image, int screen_order) {
+    if (!c || !m || !times || !nus) return fail(KS_ERR_ARG, "ks_gradient_forward: null argument");
     if (c->generation < 1) return fail(KS_ERR_STATE, "regenerate! the cache first");
     if (m->kind != KS_MODEL_THERMAL) return fail(KS_ERR_ARG, "the dual sweep takes thermal splats (PolarizedSplats)");
     if (!has_samples(c)) return fail(KS_ERR_ARG, "the cache holds no stored samples: build it with store_samples = true and a storing marcher");
@@ -610,25 +636,51 @@ int ks_polarized_gradient(ks_cache* c, const ks_model* m, const double* times, i
     for (int i = 0; i < nnu; ++i) if (!(nus[i] > 0.0)) return fail(KS_ERR_ARG, "frequencies must be positive");
     ks_ctx* x = c->ctx;
     HIPCHK(hipSetDevice(x->device));
+    c->fw_model = nullptr;
     const int64_t frames = (int64_t)nt * nnu;
     const int64_t items = c->npix * frames;
     const int ns = m->ns;
     if (ensure_out(c, frames) || ensure_adj(c, items, ns > 0 ? ns : 1)) return KS_ERR_HIP;
     const double* d_tn = stage_times(c, times, nt, nus, nnu);
     if (!d_tn) return KS_ERR_HIP;
-    // seeds in sorted order: dstokes[4 (j + npix f) + q]
-    std::vector<double> ds((size_t)4 * items);
-    for (int64_t f = 0; f < frames; ++f)
-        for (int64_t j = 0; j < c->npix; ++j) {
-            const int64_t src = screen_order ? c->perm_host[j] + c->npix * f : j + c->npix * f;
-            for (int q = 0; q < 4; ++q) ds[4 * (j + c->npix * f) + q] = dstokes[4 * src + q];
-        }
-    HIPCHK(hipMemcpyAsync(c->ads, ds.data(), ds.size() * sizeof(double), hipMemcpyHostToDevice, x->stream));
-    HIPCHK(hipStreamSynchronize(x->stream));         // ds is released on every return path below
-    // forward: chunk maps plus the per-sample records
     int rc = launch_transfer(c, m, d_tn, nt, nnu, L, nmax, slab, c->arec, c->aflg);
     if (rc) return rc;
-    HIPCHK(launch_adj_combine(x->stream, c->npix, c->N, nt, d_tn + nt, nnu, c->seg, c->ads, c->awT, c->out));
+    HIPCHK(launch_seg_combine(x->stream, c->soa, c->npix, c->N, c->M, CHUNK, nt, d_tn + nt, nnu, c->seg, c->out));
+    c->fw_model = m; c->fw_nt = nt; c->fw_nnu = nnu; c->fw_generation = c->generation; c->fw_precision = c->precision;
+    c->fw_seeded.assign((size_t)frames, 0);
+    if (image) return ks_copy_stokes(c, frames, image, screen_order);
+    return KS_OK;
+}
+
+// reverse pass: the seeds, the tails and adjoints per chunk, the element adjoints, the gradient
+int ks_gradient_reverse(ks_cache* c, const ks_model* m, const double* dstokes, int screen_order, double* dparams) {
+    if (!c || !m || !dparams) return fail(KS_ERR_ARG, "ks_gradient_reverse: null argument");
+    if (!c->fw_model) return fail(KS_ERR_STATE, "no forward pass to reverse: call ks_gradient_forward first");
+    if (m != c->fw_model) return fail(KS_ERR_ARG, "the reverse pass takes the model of its forward pass");
+    if (c->generation != c->fw_generation || c->precision != c->fw_precision)
+        return fail(KS_ERR_STATE, "the cache changed (regenerate! or precision) since the forward pass");
+    ks_ctx* x = c->ctx;
+    HIPCHK(hipSetDevice(x->device));
+    const int nt = c->fw_nt, nnu = c->fw_nnu;
+    const int64_t frames = (int64_t)nt * nnu;
+    const int64_t items = c->npix * frames;
+    const int ns = m->ns;
+    if (dstokes) {
+        // seeds in sorted order: ads[4 (j + npix f) + q]
+        std::vector<double> ds((size_t)4 * items);
+        for (int64_t f = 0; f < frames; ++f)
+            for (int64_t j = 0; j < c->npix; ++j) {
+                const int64_t src = screen_order ? c->perm_host[j] + c->npix * f : j + c->npix * f;
+                for (int q = 0; q < 4; ++q) ds[4 * (j + c->npix * f) + q] = dstokes[4 * src + q];
+            }
+        HIPCHK(hipMemcpyAsync(c->ads, ds.data(), ds.size() * sizeof(double), hipMemcpyHostToDevice, x->stream));
+        HIPCHK(hipStreamSynchronize(x->stream));     // ds is released on every return path below
+    } else {
+        for (int64_t f = 0; f < frames; ++f)
+            if (!c->fw_seeded[f]) return fail(KS_ERR_STATE, "frame %lld has no device seed: give dstokes or call a *_chi2_seed for every frame", (long long)f);
+    }
+    const double* d_tn = c->dtn;
+    HIPCHK(launch_adj_combine(x->stream, c->npix, c->N, nt, d_tn + nt, nnu, c->seg, c->ads, c->awT, nullptr));
     HIPCHK(x->stage_begin(KS_STAGE_ADJ_STEPS));
     const bool mixed = c->precision == KS_PRECISION_MIXED;
     if (mixed) HIPCHK(launch_adj_segment_f32(x->stream, c->N, items, c->awT, (f32*)c->arec, c->aflg, (f32*)c->atail));
@@ -648,17 +700,17 @@ int ks_polarized_gradient(ks_cache* c, const ks_model* m, const double* times, i
     }
     std::vector<double> g((size_t)ns * ADJ_ROWS);
     if (ns > 0) HIPCHK(hipMemcpyAsync(g.data(), c->adpar, g.size() * sizeof(double), hipMemcpyDeviceToHost, x->stream));
-    if (image) {
-        const double* src = c->out;
-        if (screen_order) {
-            HIPCHK(launch_unsort4(x->stream, c->out, c->perm, c->npix, frames, c->out_screen));
-            src = c->out_screen;
-        }
-        HIPCHK(hipMemcpyAsync(image, src, (size_t)4 * items * sizeof(double), hipMemcpyDeviceToHost, x->stream));
-    }
     HIPCHK(hipStreamSynchronize(x->stream));
     for (size_t i = 0; i < g.size(); ++i) dparams[i] += g[i];
     return KS_OK;
+}
+
+int ks_polarized_gradient(ks_cache* c, const ks_model* m, const double* times, int nt, const double* nus, int nnu, double L, int nmax, double slab,
+                          const double* dstokes, int screen_order, double* dparams, double* image) {
+    if (!c || !m || !times || !nus || !dstokes || !dparams) return fail(KS_ERR_ARG, "ks_polarized_gradient: null argument");
+    int rc = ks_gradient_forward(c, m, times, nt, nus, nnu, L, nmax, slab, image, screen_order);
+    if (rc) return rc;
+    return ks_gradient_reverse(c, m, dstokes, screen_order, dparams);
 }
 
 int ks_copy_stokes(ks_cache* c, int64_t frames, double* out_stokes, int screen_order) {
@@ -702,32 +754,114 @@ int ks_thin_image(ks_cache* c, const ks_model* m, double t_obs, double* out, int
     return KS_OK;
 }
 
+// ---- visibilities and χ² (visibilities.jl:21-40, device_visibilities.jl:53-111, Fit.jl:537-577) ----
+// host (u, v, taper) to the device and the phase tables of the cache's screen
+static int vis_setup(ks_cache* c, double psize, const double* u, const double* v, const double* taper, int nuv) {
+    ks_ctx* x = c->ctx;
+    const int na = c->nalpha, nb = c->nbeta;
+    if (c->vu.grow(nuv) || c->vv.grow(nuv) || c->vt.grow(nuv) || c->vA2.grow((size_t)2 * nuv * na) || c->vB2.grow((size_t)2 * nuv * nb))
+        return KS_ERR_HIP;
+    HIPCHK(hipMemcpyAsync(c->vu.p, u, nuv * sizeof(double), hipMemcpyHostToDevice, x->stream));
+    HIPCHK(hipMemcpyAsync(c->vv.p, v, nuv * sizeof(double), hipMemcpyHostToDevice, x->stream));
+    if (taper) HIPCHK(hipMemcpyAsync(c->vt.p, taper, nuv * sizeof(double), hipMemcpyHostToDevice, x->stream));
+    HIPCHK(launch_vis_tables(x->stream, nuv, na, nb, psize, c->vu.p, c->vv.p, c->vA2.p, c->vB2.p));
+    return KS_OK;
+}
+
+// V of frames [f0, f0 + nf) of the images in the cache into c->vV (nf x nuv x 4 complex)
+static int vis_forward(ks_cache* c, int64_t f0, int nf, int nuv, const double* d_tap, double scale) {
+    ks_ctx* x = c->ctx;
+    const int na = c->nalpha, nb = c->nbeta;
+    const int64_t mat = (int64_t)na * 4 * nb, cmat = (int64_t)2 * nuv * 4 * nb;
+    if (c->vIm.grow((size_t)mat * nf) || c->vC2.grow((size_t)cmat * nf) || c->vV.grow((size_t)8 * nuv * nf)) return KS_ERR_HIP;
+    HIPCHK(launch_vis_gather(x->stream, c->out, c->perm, c->npix, na, nb, f0, nf, c->vIm.p));
+    HIPCHK(launch_dgemm(x->stream, false, 2 * nuv, 4 * nb, na, c->vA2.p, na, 0, c->vIm.p, 4 * nb, mat, c->vC2.p, 4 * nb, cmat, nf));
+    HIPCHK(launch_vis_combine(x->stream, nuv, nb, nf, c->vC2.p, c->vB2.p, d_tap, scale, c->vV.p));
+    return KS_OK;
+}
+
+static int screen_ready(const ks_cache* c) {
+    if ((int64_t)c->nalpha * c->nbeta != c->npix) return fail(KS_ERR_STATE, "the cache's screen shape is not set");
+    return KS_OK;
+}
+
 // visibilities of the last rendered frames (images in the cache, sorted order), EHT sign convention
 int ks_visibilities(ks_cache* c, int64_t frames, double psize, const double* u, const double* v, const double* taper, int nuv, double* V) {
     if (!c || !u || !v || !V) return fail(KS_ERR_ARG, "ks_visibilities: null argument");
     if (frames > c->out_frames || frames < 1) return fail(KS_ERR_DIM, "only %lld frames rendered", (long long)c->out_frames);
     if (nuv < 1) return fail(KS_ERR_ARG, "need at least one (u, v) point");
-    if ((int64_t)c->nalpha * c->nbeta != c->npix) return fail(KS_ERR_STATE, "the cache's screen shape is not set");
+    if (int rc = screen_ready(c)) return rc;
     ks_ctx* x = c->ctx;
     HIPCHK(hipSetDevice(x->device));
-    if (c->vis_cap < nuv) {
-        dfree(c->vu); dfree(c->vv); dfree(c->vt);
-        if (dalloc(&c->vu, nuv) || dalloc(&c->vv, nuv) || dalloc(&c->vt, nuv)) return KS_ERR_HIP;
-        c->vis_cap = nuv;
-    }
-    if (c->visV_cap < (int64_t)8 * nuv * frames) {
-        dfree(c->vV);
-        if (dalloc(&c->vV, (size_t)8 * nuv * frames)) return KS_ERR_HIP;
-        c->visV_cap = (int64_t)8 * nuv * frames;
-    }
-    HIPCHK(hipMemcpyAsync(c->vu, u, nuv * sizeof(double), hipMemcpyHostToDevice, x->stream));
-    HIPCHK(hipMemcpyAsync(c->vv, v, nuv * sizeof(double), hipMemcpyHostToDevice, x->stream));
-    if (taper) HIPCHK(hipMemcpyAsync(c->vt, taper, nuv * sizeof(double), hipMemcpyHostToDevice, x->stream));
-    double scale = psize * psize / 1e-23;
-    HIPCHK(launch_vis_direct(x->stream, c->out, c->perm, c->npix, c->nalpha, c->nbeta, (int)frames, psize, scale, c->vu, c->vv,
-                             taper ? c->vt : nullptr, nuv, c->vV));
-    HIPCHK(hipMemcpyAsync(V, c->vV, (size_t)8 * nuv * frames * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+    if (int rc = vis_setup(c, psize, u, v, taper, nuv)) return rc;
+    const double scale = psize * psize / JY;
+    if (int rc = vis_forward(c, 0, (int)frames, nuv, taper ? c->vt.p : nullptr, scale)) return rc;
+    HIPCHK(hipMemcpyAsync(V, c->vV.p, (size_t)8 * nuv * frames * sizeof(double), hipMemcpyDeviceToHost, x->stream));
     HIPCHK(hipStreamSynchronize(x->stream));
+    return KS_OK;
+}
+
+static int seed_frame_ok(ks_cache* c, int64_t frame) {
+    if (!c->fw_model) return fail(KS_ERR_STATE, "no forward pass: call ks_gradient_forward first");
+    if (frame < 0 || frame >= (int64_t)c->fw_seeded.size())
+        return fail(KS_ERR_DIM, "frame %lld outside the forward pass's %zu frames", (long long)frame, c->fw_seeded.size());
+    return KS_OK;
+}
+
+// frame_chi2_seed!: χ² of one frame's visibilities and the dual sweep's seed of that frame on the device
+int ks_vis_chi2_seed(ks_cache* c, int64_t frame, double psize, const double* u, const double* v, const double* taper, const double* gains,
+                     const double* vis, const double* sigma, int nuv, double* chi2, double* V) {
+    if (!c || !u || !v || !vis || !sigma || !chi2) return fail(KS_ERR_ARG, "ks_vis_chi2_seed: null argument");
+    if (nuv < 1) return fail(KS_ERR_ARG, "need at least one (u, v) point");
+    if (int rc = seed_frame_ok(c, frame)) return rc;
+    if (int rc = screen_ready(c)) return rc;
+    for (int k = 0; k < 4 * nuv; ++k) if (!(sigma[k] > 0.0)) return fail(KS_ERR_ARG, "sigma must be positive (entry %d)", k);
+    ks_ctx* x = c->ctx;
+    HIPCHK(hipSetDevice(x->device));
+    const int na = c->nalpha, nb = c->nbeta;
+    if (int rc = vis_setup(c, psize, u, v, taper, nuv)) return rc;
+    if (c->vg.grow((size_t)2 * nuv) || c->vd.grow((size_t)8 * nuv) || c->vs.grow((size_t)4 * nuv) || c->vwq.grow((size_t)8 * nuv) ||
+        c->vE2.grow((size_t)2 * nuv * 4 * nb) || c->vSd.grow((size_t)na * 4 * nb) || c->vchi.grow(nuv) || c->vsum.grow(1))
+        return KS_ERR_HIP;
+    if (gains) HIPCHK(hipMemcpyAsync(c->vg.p, gains, (size_t)2 * nuv * sizeof(double), hipMemcpyHostToDevice, x->stream));
+    HIPCHK(hipMemcpyAsync(c->vd.p, vis, (size_t)8 * nuv * sizeof(double), hipMemcpyHostToDevice, x->stream));
+    HIPCHK(hipMemcpyAsync(c->vs.p, sigma, (size_t)4 * nuv * sizeof(double), hipMemcpyHostToDevice, x->stream));
+    const double scale = psize * psize / JY;
+    const double* d_tap = taper ? c->vt.p : nullptr;
+    if (int rc = vis_forward(c, frame, 1, nuv, d_tap, scale)) return rc;
+    HIPCHK(launch_vis_residual(x->stream, nuv, c->vV.p, c->vd.p, c->vs.p, gains ? c->vg.p : nullptr, d_tap, scale, c->vwq.p, c->vchi.p));
+    HIPCHK(launch_vis_rotate(x->stream, nuv, nb, c->vwq.p, c->vB2.p, c->vE2.p));
+    HIPCHK(launch_dgemm(x->stream, true, na, 4 * nb, 2 * nuv, c->vA2.p, na, 0, c->vE2.p, 4 * nb, 0, c->vSd.p, 4 * nb, 0, 1));
+    HIPCHK(launch_vis_scatter_seed(x->stream, c->vSd.p, c->perm, c->npix, na, nb, frame, c->ads));
+    HIPCHK(launch_sum_fixed(x->stream, c->vchi.p, nuv, c->vsum.p));
+    HIPCHK(hipMemcpyAsync(chi2, c->vsum.p, sizeof(double), hipMemcpyDeviceToHost, x->stream));
+    if (V) HIPCHK(hipMemcpyAsync(V, c->vV.p, (size_t)8 * nuv * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    c->fw_seeded[frame] = 1;
+    return KS_OK;
+}
+
+// _frame_seed! (unbinned): χ² of one frame's image and the dual sweep's seed of that frame on the device
+int ks_image_chi2_seed(ks_cache* c, int64_t frame, const double* data, const double* sigma, int sigma_per_pixel, const uint8_t* mask,
+                       double* chi2) {
+    if (!c || !data || !sigma || !chi2) return fail(KS_ERR_ARG, "ks_image_chi2_seed: null argument");
+    if (int rc = seed_frame_ok(c, frame)) return rc;
+    const int64_t nsig = sigma_per_pixel ? 4 * c->npix : 4;
+    for (int64_t k = 0; k < nsig; ++k)
+        if (!(sigma[k] > 0.0) && (!mask || !sigma_per_pixel || mask[k / 4])) return fail(KS_ERR_ARG, "sigma must be positive (entry %lld)", (long long)k);
+    ks_ctx* x = c->ctx;
+    HIPCHK(hipSetDevice(x->device));
+    if (c->idata.grow((size_t)4 * c->npix) || c->isig.grow((size_t)nsig) || c->vchi.grow(c->npix) || c->vsum.grow(1)) return KS_ERR_HIP;
+    if (mask && c->imask.grow(c->npix)) return KS_ERR_HIP;
+    HIPCHK(hipMemcpyAsync(c->idata.p, data, (size_t)4 * c->npix * sizeof(double), hipMemcpyHostToDevice, x->stream));
+    HIPCHK(hipMemcpyAsync(c->isig.p, sigma, (size_t)nsig * sizeof(double), hipMemcpyHostToDevice, x->stream));
+    if (mask) HIPCHK(hipMemcpyAsync(c->imask.p, mask, (size_t)c->npix, hipMemcpyHostToDevice, x->stream));
+    HIPCHK(launch_image_residual(x->stream, c->out, c->perm, c->npix, frame, c->idata.p, c->isig.p, sigma_per_pixel, mask ? c->imask.p : nullptr,
+                                 c->ads, c->vchi.p));
+    HIPCHK(launch_sum_fixed(x->stream, c->vchi.p, c->npix, c->vsum.p));
+    HIPCHK(hipMemcpyAsync(chi2, c->vsum.p, sizeof(double), hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    c->fw_seeded[frame] = 1;
     return KS_OK;
 }
 
