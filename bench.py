@@ -4,10 +4,12 @@ Workload (the configuration the headline metric is quoted on that fits one GPU):
 theta_o = 60 deg, 128 x 128 screen over +-15 M, 1024 Mino-time samples per ray, 32 thermal splats,
 full IQUV at 230 GHz, visibilities on 512 uv points (seeded synthetic draws). The per-pixel
 constants (K0/K1) are built once beforehand and the splat parameters are resident on the device.
-One fwd+adjoint step = the Recurrence(64) march of every ray into sample storage, the polarized
-transfer with the adjoint records, the dual sweep (the gradient of sum(dstokes * I) for a fixed
-seeded dstokes with respect to all 21 x 32 splat parameters) and the visibilities of the image.
-The forward-only step (march + transfer + visibilities) is timed beside it.
+One fwd+adjoint step is the device pipeline of a visibility fit iteration (frame_chi2_seed! between
+the two passes of the dual sweep): the Recurrence(64) march of every ray into sample storage, the
+polarized transfer with the adjoint records, the visibilities of the image with their χ² against
+fixed synthetic data and the adjoint seed of the weighted residuals (left on the device), and the
+reverse pass (the gradient with respect to all 21 x 32 splat parameters). The forward-only step
+(march + transfer + visibilities) is timed beside it.
 
 Prints one JSON line; value = ray-samples per second (pixels x samples x frames x frequencies / s)
 of the fwd+adjoint step.
@@ -37,7 +39,27 @@ def parse():
     p.add_argument("--config", type=int, default=1)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
+    p.add_argument("--copy-probe", action="store_true",
+                   help="also run device-to-device copies of 256 MiB (the reference for scaling HBM byte counters)")
     return p.parse_args()
+
+
+def copy_probe(nbytes=1 << 28, reps=3):
+    """Device-to-device copies of nbytes through the HIP runtime (its blit kernel): the kernel of known
+    traffic (nbytes read + nbytes written per copy) that scales the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    n = ctypes.c_size_t(nbytes)
+    for ptr in (a, b):
+        if hip.hipMalloc(ctypes.byref(ptr), n) != 0:
+            raise RuntimeError("hipMalloc failed in the copy probe")
+    hip.hipMemset(a, 1, n)
+    for _ in range(reps):
+        hip.hipMemcpy(b, a, n, 3)                      # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
 
 
 def cpu_baseline(cfg, P, L, target_s, ds):
@@ -86,18 +108,32 @@ def main():
     psize = syn.pixel_size_rad(-cfg["fov"], cfg["fov"], res)
     lib, ctx = api._load(), api.context()
     times, nus = cfg["times"], cfg["nus"]
+    nframes = len(times) * len(nus)
+    # fixed synthetic visibility data: the FP64 model's visibilities, perturbed by 5 % and thermal noise
+    rng = np.random.default_rng(20261203)
+    K.precision(cache, "fp64")
+    K.render_device(cache, model, times, nus, L)
+    V0 = K.visibilities(cache, nframes, psize, u, v) if cfg["nuv"] else None
+    if cfg["nuv"]:
+        sig = 0.01 * np.abs(V0[..., 0]).max() * np.ones(V0.shape)
+        vdata = V0 * (1 + 0.05 * rng.normal(size=V0.shape)) + sig * (rng.normal(size=V0.shape) + 1j * rng.normal(size=V0.shape))
+    dparams = np.zeros(P.shape, order="F")
+    chis = []
 
     def forward_step():
         K.march(cache)
         K.render_device(cache, model, times, nus, L)
         if cfg["nuv"]:
-            K.visibilities(cache, 1, psize, u, v)
+            K.visibilities(cache, nframes, psize, u, v)
 
     def adjoint_step():
         K.march(cache)
-        K.polarized_gradient(cache, model, times, nus, L, ds)
-        if cfg["nuv"]:
-            K.visibilities(cache, 1, psize, u, v)
+        if not cfg["nuv"]:
+            K.polarized_gradient(cache, model, times, nus, L, ds, dparams=dparams)
+            return
+        K.gradient_forward(cache, model, times, nus, L, copy=False)
+        chis.append(sum(K.vis_chi2_seed(cache, f, psize, u, v, vdata[f], sig[f]) for f in range(nframes)))
+        K.gradient_reverse(cache, model, dparams)
 
     def timed(step, stages):
         """W untimed warmups, then K steps bracketed by stream synchronizations; per step, each listed
@@ -121,8 +157,11 @@ def main():
         K.precision(cache, prec)
         fwd_s, fwd_ms = timed(forward_step, STG_FWD)
         adj_s, adj_ms = timed(adjoint_step, STG_ADJ)
-        g, cube = K.polarized_gradient(cache, model, times, nus, L, ds)
-        runs[prec] = {"fwd_s": fwd_s, "fwd_ms": fwd_ms, "adj_s": adj_s, "adj_ms": adj_ms, "grad": g.copy(), "cube": cube}
+        dparams[:] = 0.0
+        adjoint_step()
+        cube = K.gradient_forward(cache, model, times, nus, L)
+        runs[prec] = {"fwd_s": fwd_s, "fwd_ms": fwd_ms, "adj_s": adj_s, "adj_ms": adj_ms, "grad": dparams.copy(), "cube": cube,
+                      "chi2": chis[-1] if chis else None}
     img_err = float(np.abs(runs["mixed"]["cube"] - runs["fp64"]["cube"]).max() / np.abs(runs["fp64"]["cube"][..., 0]).max())
     grad_err = float(np.abs(runs["mixed"]["grad"] - runs["fp64"]["grad"]).max() / np.abs(runs["fp64"]["grad"]).max())
     head = "mixed" if img_err <= 1e-4 else "fp64"        # the mixed path counts only within its error bar
@@ -167,6 +206,8 @@ def main():
                   "forward": {"value": work / r["fwd_s"], "unit": "ray-samples/s", "ms_per_step": 1e3 * r["fwd_s"], "stage_ms": r["fwd_ms"]}}
              for p_, r in runs.items()}
 
+    if args.copy_probe:
+        copy_probe()
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(cfg, P, L, args.cpu_seconds, ds)
@@ -178,12 +219,14 @@ def main():
                       "samples_per_ray": N, "splats": cfg["ns"], "kind": cfg["kind"], "frames": len(times),
                       "frequencies": len(nus), "uv_points": cfg["nuv"],
                       "path": ("mixed: FP64 geodesics (Recurrence(64) into stored samples, anchor segments in parallel), "
-                               "FP32 transfer and dual-sweep adjoint in 32-sample chunks (FP64 step above Faraday depth 10), visibilities")
+                               "FP32 transfer and dual-sweep adjoint in 32-sample chunks (FP64 step above Faraday depth 10), "
+                               "visibility chi2 and seed on the device (FP64 GEMMs)")
                               if head == "mixed" else
                               "plain FP64: Recurrence(64) march into stored samples (anchor segments in parallel), "
-                              "transfer in 32-sample chunks, dual-sweep adjoint, visibilities"},
+                              "transfer in 32-sample chunks, dual-sweep adjoint, visibility chi2 and seed on the device"},
            "mixed_over_fp64": paths["fp64"]["fwd_adjoint"]["ms_per_step"] / paths["mixed"]["fwd_adjoint"]["ms_per_step"],
            "mixed_error_vs_fp64": {"stokes_over_peak_I": img_err, "gradient_over_max": grad_err},
+           "vis_chi2": {p_: runs[p_]["chi2"] for p_ in runs},
            "paths": paths, "gradient_evals_per_s": 1.0 / adj_s,
            "stages": stages,
            "roofline": roofline, "cpu_baseline": cpu}
