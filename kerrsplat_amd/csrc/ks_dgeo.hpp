@@ -438,7 +438,7 @@ KS_DEV double radius_jac(const PixC& p, double sn, double cn) {
 }
 
 // closed-form t~ and phi at Mino time tau (the anchors)
-KS_DEV void t_phi_direct(const PixC& p, const Spin& S, double tau, double& t, double& ph) {
+KS_COLD void t_phi_direct(const PixC& p, const Spin& S, double tau, double& t, double& ph) {
     double X = p.c * (p.fo - tau);
     double psi = am(X, p.k);
     double g = radial_G2p2G1(p, X, psi);

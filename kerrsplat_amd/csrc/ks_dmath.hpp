@@ -12,6 +12,9 @@
 #include <stdint.h>
 
 #define KS_DEV __host__ __device__ __forceinline__
+// Kept out of line: heavy code on rare paths (closed-form anchors, turning points), so that the hot
+// loops around the calls keep their registers.
+#define KS_COLD __host__ __device__ __attribute__((noinline)) inline
 
 namespace ksd {
 

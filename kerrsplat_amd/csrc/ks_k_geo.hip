@@ -68,8 +68,9 @@ struct StoreF {
 // Work items (segment m, pixel j), pixel fastest. Recurrence: the pixel's anchor segments (march_range);
 // Direct: the closed forms at every sample of chunk m (SEG_MIN samples). Anchor residuals per segment
 // res[2 (m npix + j) + {0, 1}] (zero for Direct and past the pixel's last segment).
+template <int MARCHER>
 __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(KS_WAVES_MARCH))) ks_march_store(double a, double thetao, const double* __restrict__ soa, int64_t npix, int N, int M,
-                                                      int marcher, int nseg_max, Samples smp, double* __restrict__ res) {
+                                                      int nseg_max, Samples smp, double* __restrict__ res) {
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= npix * nseg_max) return;
     const int64_t j = tid % npix;
@@ -78,11 +79,11 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(KS_WAV
     const PixC p = load_pixel(soa, npix, j);
     StoreF f{smp, j, npix, a, S.rp, S.rm};
     double r1 = 0.0, r2 = 0.0;
-    const int L = marcher == 0 ? SEG_MIN : segment_length(M, 1.0 - p.k);
+    const int L = MARCHER == 0 ? SEG_MIN : segment_length(M, 1.0 - p.k);
     const int kbeg = m * L + 1;
     if (kbeg <= N) {
         const int kend = kbeg + L - 1 < N ? kbeg + L - 1 : N;
-        if (marcher == 0) {
+        if constexpr (MARCHER == 0) {
             const VisC vis = vis_constants(p, a, thetao);
             const double dtau = p.tau_total / (double)(N + 1);
             for (int k = kbeg; k <= kend; ++k) f(k, direct_sample(p, S, vis, k * dtau), dtau);
@@ -183,7 +184,8 @@ hipError_t launch_pixel_constants(hipStream_t s, double a, double thetao, const 
 }
 hipError_t launch_march_store(hipStream_t s, double a, double thetao, const double* soa, int64_t npix, int N, int M, int marcher, int nseg_max,
                               Samples smp, double* res) {
-    hipLaunchKernelGGL(ks_march_store, dim3(grid_for(npix * nseg_max, 128)), dim3(128), 0, s, a, thetao, soa, npix, N, M, marcher, nseg_max, smp, res);
+    if (marcher == 0) hipLaunchKernelGGL(ks_march_store<0>, dim3(grid_for(npix * nseg_max, 128)), dim3(128), 0, s, a, thetao, soa, npix, N, M, nseg_max, smp, res);
+    else hipLaunchKernelGGL(ks_march_store<1>, dim3(grid_for(npix * nseg_max, 128)), dim3(128), 0, s, a, thetao, soa, npix, N, M, nseg_max, smp, res);
     return hipGetLastError();
 }
 hipError_t launch_res_reduce(hipStream_t s, int64_t npix, int nseg_max, const double* res, double* rt, double* rp) {

@@ -78,10 +78,10 @@ KS_DEV RP ray_point(const PixC& p, const Spin& S, const QC& q, const JS& xr, con
     smooth_integrands(p, S, q, o.r, o.cth, o.s, o.sn, o.cn, idn, o.gt, o.gph);
     return o;
 }
-KS_DEV JS radial_state(const PixC& p, double tau) { return js_at(p.c * (p.fo - tau), p.k); }
-KS_DEV JS polar_state(const PixC& p, double tau) { return js_at(p.scale * (tau + p.off) / p.f, p.mu); }
+KS_COLD JS radial_state(const PixC& p, double tau) { return js_at(p.c * (p.fo - tau), p.k); }
+KS_COLD JS polar_state(const PixC& p, double tau) { return js_at(p.scale * (tau + p.off) / p.f, p.mu); }
 
-KS_DEV RP ray_point_turning(const PixC& p, const Spin& S, const QC& q) {
+KS_COLD RP ray_point_turning(const PixC& p, const Spin& S, const QC& q) {
     JS xt = polar_state(p, p.fo);
     RP o;
     o.r = p.r4;
@@ -114,6 +114,19 @@ KS_DEV void interval_increment(const PixC& p, const Spin& S, const QC& q, const 
     It = h * (1.0 / 6.0) * (pa.gt + 4.0 * pm.gt + pb.gt) + fabs(dFr) + q.Qt * dFp + q.Qtm * dFm;
     Iph = h * (1.0 / 6.0) * (pa.gph + 4.0 * pm.gph + pb.gph) + q.Qph * dFp + q.Qphm * dFm +
           polar_closed_form(p, q, tb, pb.sn, pb.cn) - polar_closed_form(p, q, ta, pa.sn, pa.cn);
+}
+
+// the interval holding the radial turning point of a scattering ray, split there (quadrature.jl:255-270)
+KS_COLD void turning_increment(const PixC& p, const Spin& S, const QC& q, const RP& pa, const RP& pb, double ta, double tb, double& t, double& ph) {
+    RP pt = ray_point_turning(p, S, q);
+    double t1 = 0.5 * (ta + p.fo), t2 = 0.5 * (p.fo + tb);
+    RP p1 = ray_point(p, S, q, radial_state(p, t1), polar_state(p, t1), t1);
+    RP p2 = ray_point(p, S, q, radial_state(p, t2), polar_state(p, t2), t2);
+    double It1, Ip1, It2, Ip2;
+    interval_increment(p, S, q, pa, p1, pt, ta, p.fo, It1, Ip1);
+    interval_increment(p, S, q, pt, p2, pb, p.fo, tb, It2, Ip2);
+    t += It1 + It2;
+    ph += Ip1 + Ip2;
 }
 
 KS_DEV int anchor_interval(int M, double one_minus_k) {
@@ -175,15 +188,7 @@ KS_DEV void march_range(const PixC& p, const Spin& S, double thetao, int N, int 
         else { xr = js_step(xr, Dr, p.k); xt = js_step(xt, Dt, p.mu); }
         RP pb = ray_point(p, S, q, xr, xt, tb);
         if (q.scattering && ta < p.fo && p.fo < tb) {
-            RP pt = ray_point_turning(p, S, q);
-            double t1 = 0.5 * (ta + p.fo), t2 = 0.5 * (p.fo + tb);
-            RP p1 = ray_point(p, S, q, radial_state(p, t1), polar_state(p, t1), t1);
-            RP p2 = ray_point(p, S, q, radial_state(p, t2), polar_state(p, t2), t2);
-            double It1, Ip1, It2, Ip2;
-            interval_increment(p, S, q, pa, p1, pt, ta, p.fo, It1, Ip1);
-            interval_increment(p, S, q, pt, p2, pb, p.fo, tb, It2, Ip2);
-            t += It1 + It2;
-            ph += Ip1 + Ip2;
+            turning_increment(p, S, q, pa, pb, ta, tb, t, ph);
         } else {
             double It, Ip;
             interval_increment(p, S, q, pa, pmid, pb, ta, tb, It, Ip);
