@@ -347,21 +347,38 @@ template <class T> KS_DEV T phi1(T a) {
     return -expm1(-a) / a;
 }
 // M_n(a) = int_0^1 t^n e^{-at} dt for n = 0..NMAX (step.jl:239-268): for |a| < 1 the 24-term power
-// series sum_m (-1)^m a^m / (m! (n + m + 1)) by Horner's rule (the coefficients fold to constants in
-// the unrolled loops), otherwise the upward recurrence M_0 = (1 - e^{-a})/a, M_n = (n M_{n-1} - e^{-a})/a.
+// series sum_m t_m / (n + m + 1), t_m = (-a)^m / m!, otherwise the upward recurrence
+// M_0 = (1 - e^{-a})/a, M_n = (n M_{n-1} - e^{-a})/a. The series is summed forward from one running
+// term t_m for all n at once in FP64 (the reference and the oracle use Horner's rule per n): the same
+// truncated series, with 33 reciprocals as constants instead of 24 (NMAX + 1) FP64 Horner coefficients,
+// which the compiler would otherwise keep in registers across the sample loops. Its terms fall as
+// 1/m!, so the forward sum carries a few ulp of rounding. FP32 keeps Horner's rule.
 KS_DEV constexpr double moment_coef(int n, int m) {
     double f = 1.0;
     for (int i = 2; i <= m; ++i) f *= i;
     return ((m & 1) ? -1.0 : 1.0) / (f * (n + m + 1));
 }
+template <class T> struct base_scalar { using type = T; };
+template <int N, class S> struct base_scalar<DN<N, S>> { using type = S; };
 template <int NMAX, class T> KS_DEV void moments(T a, T M[10]) {
     if (fabs(val(a)) < 1.0) {
+        if constexpr (std::is_same<typename base_scalar<T>::type, double>::value) {
+            for (int n = 0; n <= NMAX; ++n) M[n] = T(1.0 / (n + 1));
+            T tm = T(1.0);
+            for (int m = 1; m <= 23; ++m) {
+                tm = tm * (-a) * (1.0 / m);
 #pragma unroll
-        for (int n = 0; n <= NMAX; ++n) {
-            T s = T(moment_coef(n, 23));
+                for (int n = 0; n <= NMAX; ++n) M[n] += tm * (1.0 / (n + m + 1));
+            }
+        } else {
+            // FP32: Horner per n (its coefficients are cheap 32-bit constants)
 #pragma unroll
-            for (int m = 22; m >= 0; --m) s = s * a + moment_coef(n, m);
-            M[n] = s;
+            for (int n = 0; n <= NMAX; ++n) {
+                T s = T(moment_coef(n, 23));
+#pragma unroll
+                for (int m = 22; m >= 0; --m) s = s * a + moment_coef(n, m);
+                M[n] = s;
+            }
         }
     } else {
         T e = exp(-a);

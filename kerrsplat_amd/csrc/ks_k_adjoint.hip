@@ -49,42 +49,42 @@ __global__ void ks_adj_combine(int64_t npix, int N, int nt, const double* __rest
     }
 }
 
-// The adjoint of one sample's step in the transfer's scalar; in FP32 the samples with a Faraday depth
-// |rho| Delta above FARADAY_FP64 take the FP64 step and its adjoint, as the forward pass does.
+// The FP64 step and step adjoint of an FP32 sample, out of line (the rare samples with a Faraday depth
+// |rho| Delta above FARADAY_FP64, where the forward pass takes the FP64 step too).
+KS_COLD void step_adjoint_f64_for_f32(const f32 j4[4], const f32 a4[4], const f32 r3[3], f32 D, const f32 w[4], const f32 R[4], f32 Otw[4],
+                                      f32 jb[4], f32 ab[4], f32 rb[3]) {
+    double jd[4], ad[4], rd[3], wd[4], Rd[4], Od[4], jbd[4], abd[4], rbd[3];
+    for (int i = 0; i < 4; ++i) { jd[i] = val(j4[i]); ad[i] = val(a4[i]); wd[i] = val(w[i]); Rd[i] = val(R[i]); }
+    for (int i = 0; i < 3; ++i) rd[i] = val(r3[i]);
+    sample_adjoint(jd, ad, rd, val(D), wd, Rd, Od, jbd, abd, rbd);
+    for (int i = 0; i < 4; ++i) { Otw[i] = f32(Od[i]); jb[i] = f32(jbd[i]); ab[i] = f32(abd[i]); }
+    for (int i = 0; i < 3; ++i) rb[i] = f32(rbd[i]);
+}
+KS_COLD void step_apply_f64_for_f32(const f32 j4[4], const f32 a4[4], const f32 r3[3], f32 D, const f32 Tin[4], f32 Tout[4]) {
+    double jd[4], ad[4], rd[3], Ti[4], To[4];
+    for (int i = 0; i < 4; ++i) { jd[i] = val(j4[i]); ad[i] = val(a4[i]); Ti[i] = val(Tin[i]); }
+    for (int i = 0; i < 3; ++i) rd[i] = val(r3[i]);
+    step_apply(jd, ad, rd, val(D), Ti, To);
+    for (int i = 0; i < 4; ++i) Tout[i] = f32(To[i]);
+}
+
+// The adjoint of one sample's step, and the step applied to a tail, in the transfer's scalar; in FP32
+// the samples with a Faraday depth |rho| Delta above FARADAY_FP64 take the FP64 step and its adjoint.
 template <class S>
 KS_DEV void step_adjoint_s(const S j4[4], const S a4[4], const S r3[3], S D, const S w[4], const S R[4], S Otw[4], S jb[4], S ab[4], S rb[3]) {
-    if constexpr (std::is_same<S, double>::value) {
-        sample_adjoint(j4, a4, r3, D, w, R, Otw, jb, ab, rb);
-    } else {
+    if constexpr (!std::is_same<S, double>::value) {
         const S fd = sqrt(r3[0] * r3[0] + r3[1] * r3[1] + r3[2] * r3[2]) * D;
-        if (fd > FARADAY_FP64) {
-            double jd[4], ad[4], rd[3], wd[4], Rd[4], Od[4], jbd[4], abd[4], rbd[3];
-            for (int i = 0; i < 4; ++i) { jd[i] = val(j4[i]); ad[i] = val(a4[i]); wd[i] = val(w[i]); Rd[i] = val(R[i]); }
-            for (int i = 0; i < 3; ++i) rd[i] = val(r3[i]);
-            sample_adjoint(jd, ad, rd, val(D), wd, Rd, Od, jbd, abd, rbd);
-            for (int i = 0; i < 4; ++i) { Otw[i] = S(Od[i]); jb[i] = S(jbd[i]); ab[i] = S(abd[i]); }
-            for (int i = 0; i < 3; ++i) rb[i] = S(rbd[i]);
-        } else {
-            sample_adjoint(j4, a4, r3, D, w, R, Otw, jb, ab, rb);
-        }
+        if (fd > FARADAY_FP64) { step_adjoint_f64_for_f32(j4, a4, r3, D, w, R, Otw, jb, ab, rb); return; }
     }
+    sample_adjoint(j4, a4, r3, D, w, R, Otw, jb, ab, rb);
 }
 template <class S>
 KS_DEV void step_apply_s(const S j4[4], const S a4[4], const S r3[3], S D, const S Tin[4], S Tout[4]) {
-    if constexpr (std::is_same<S, double>::value) {
-        step_apply(j4, a4, r3, D, Tin, Tout);
-    } else {
+    if constexpr (!std::is_same<S, double>::value) {
         const S fd = sqrt(r3[0] * r3[0] + r3[1] * r3[1] + r3[2] * r3[2]) * D;
-        if (fd > FARADAY_FP64) {
-            double jd[4], ad[4], rd[3], Ti[4], To[4];
-            for (int i = 0; i < 4; ++i) { jd[i] = val(j4[i]); ad[i] = val(a4[i]); Ti[i] = val(Tin[i]); }
-            for (int i = 0; i < 3; ++i) rd[i] = val(r3[i]);
-            step_apply(jd, ad, rd, val(D), Ti, To);
-            for (int i = 0; i < 4; ++i) Tout[i] = S(To[i]);
-        } else {
-            step_apply(j4, a4, r3, D, Tin, Tout);
-        }
+        if (fd > FARADAY_FP64) { step_apply_f64_for_f32(j4, a4, r3, D, Tin, Tout); return; }
     }
+    step_apply(j4, a4, r3, D, Tin, Tout);
 }
 
 // Per (chunk, item): the tails T_{k+1} of the chunk's samples back to front from T_out[m], then the

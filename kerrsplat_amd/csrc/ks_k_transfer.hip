@@ -128,21 +128,44 @@ struct SegMapT {
         for (int i = 0; i < 4; ++i) { Sv[i] = S(0.0); comp[i] = S(0.0); }
     }
     KS_DEV void advance(const S O[16], const S E[4]) {
-        for (int i = 0; i < 4; ++i) {
-            S pe = P[4 * i] * E[0] + P[4 * i + 1] * E[1] + P[4 * i + 2] * E[2] + P[4 * i + 3] * E[3];
-            if constexpr (std::is_same<S, double>::value) {
-                Sv[i] += pe;
-            } else {
-                S y = pe - comp[i];
-                S t = Sv[i] + y;
-                comp[i] = (t - Sv[i]) - y;
-                Sv[i] = t;
-            }
-        }
+        for (int i = 0; i < 4; ++i) add_stokes(P[4 * i] * E[0] + P[4 * i + 1] * E[1] + P[4 * i + 2] * E[2] + P[4 * i + 3] * E[3], i);
         S Pn[16];
         for (int i = 0; i < 4; ++i)
             for (int q = 0; q < 4; ++q) Pn[4 * i + q] = P[4 * i] * O[q] + P[4 * i + 1] * O[4 + q] + P[4 * i + 2] * O[8 + q] + P[4 * i + 3] * O[12 + q];
         for (int i = 0; i < 16; ++i) P[i] = Pn[i];
+    }
+    KS_DEV void add_stokes(const S pe, int i) {
+        if constexpr (std::is_same<S, double>::value) {
+            Sv[i] += pe;
+        } else {
+            S y = pe - comp[i];
+            S t = Sv[i] + y;
+            comp[i] = (t - Sv[i]) - y;
+            Sv[i] = t;
+        }
+    }
+    // One sample's step without forming O: E = Dp sum_n e_n K'^n j, Sv += P E, then every row of P
+    // becomes sum_n o_n (row K'^n) (products with K'^T), which keeps 4 + 16 values live instead of the
+    // 64 of O, K'^2 and K'^3.
+    KS_DEV void step(const S j4[4], const S a4[4], const S r3[3], S D) {
+        S Dp, K6[6], o[4], e4[4];
+        step_coefs(a4, r3, D, Dp, K6, o, e4);
+        S u[4], y[4], E[4];
+        for (int i = 0; i < 4; ++i) { u[i] = j4[i]; E[i] = e4[0] * u[i]; }
+        for (int n = 1; n < 4; ++n) {
+            kprime_mul(K6, u, y);
+            for (int i = 0; i < 4; ++i) { u[i] = y[i]; E[i] += e4[n] * u[i]; }
+        }
+        for (int i = 0; i < 4; ++i) add_stokes(Dp * (P[4 * i] * E[0] + P[4 * i + 1] * E[1] + P[4 * i + 2] * E[2] + P[4 * i + 3] * E[3]), i);
+        for (int i = 0; i < 4; ++i) {
+            S v[4], acc[4];
+            for (int q = 0; q < 4; ++q) { v[q] = P[4 * i + q]; acc[q] = o[0] * v[q]; }
+            for (int n = 1; n < 4; ++n) {
+                kprime_tmul(K6, v, y);
+                for (int q = 0; q < 4; ++q) { v[q] = y[q]; acc[q] += o[n] * v[q]; }
+            }
+            for (int q = 0; q < 4; ++q) P[4 * i + q] = acc[q];
+        }
     }
     KS_DEV void store(double* seg, int64_t m, int64_t nitems, int64_t item) const {
         double* o = seg + SEG_FIELDS * m * nitems + item;
@@ -152,25 +175,29 @@ struct SegMapT {
 };
 using SegMap = SegMapT<double>;
 
-// The step of one sample in the transfer's scalar; in FP32 the samples with a Faraday depth
-// |rho| Delta above FARADAY_FP64 take the FP64 step.
+// The FP64 step of an FP32 sample, out of line (the rare samples with a Faraday depth |rho| Delta above
+// FARADAY_FP64, where the step's polynomial in K' cancels strongly): O and E in FP64, rounded to FP32.
+KS_COLD void step_f64_for_f32(const f32 j4[4], const f32 a4[4], const f32 r3[3], f32 D, f32 O[16], f32 E[4]) {
+    double jd[4], ad[4], rd[3], Od[16], Ed[4];
+    for (int i = 0; i < 4; ++i) { jd[i] = val(j4[i]); ad[i] = val(a4[i]); }
+    for (int i = 0; i < 3; ++i) rd[i] = val(r3[i]);
+    transfer_step(jd, ad, rd, val(D), Od, Ed);
+    for (int i = 0; i < 16; ++i) O[i] = f32(Od[i]);
+    for (int i = 0; i < 4; ++i) E[i] = f32(Ed[i]);
+}
+// One sample's step applied to a segment map, in the transfer's scalar.
 template <class S>
-KS_DEV void sample_step(const S j4[4], const S a4[4], const S r3[3], S D, S O[16], S E[4]) {
-    if constexpr (std::is_same<S, double>::value) {
-        transfer_step(j4, a4, r3, D, O, E);
-    } else {
+KS_DEV void map_step(SegMapT<S>& sm, const S j4[4], const S a4[4], const S r3[3], S D) {
+    if constexpr (!std::is_same<S, double>::value) {
         const S fd = sqrt(r3[0] * r3[0] + r3[1] * r3[1] + r3[2] * r3[2]) * D;
         if (fd > FARADAY_FP64) {
-            double jd[4], ad[4], rd[3], Od[16], Ed[4];
-            for (int i = 0; i < 4; ++i) { jd[i] = val(j4[i]); ad[i] = val(a4[i]); }
-            for (int i = 0; i < 3; ++i) rd[i] = val(r3[i]);
-            transfer_step(jd, ad, rd, val(D), Od, Ed);
-            for (int i = 0; i < 16; ++i) O[i] = S(Od[i]);
-            for (int i = 0; i < 4; ++i) E[i] = S(Ed[i]);
-        } else {
-            transfer_step(j4, a4, r3, D, O, E);
+            S O[16], E[4];
+            step_f64_for_f32(j4, a4, r3, D, O, E);
+            sm.advance(O, E);
+            return;
         }
     }
+    sm.step(j4, a4, r3, D);
 }
 
 // Delta = (L / nu-hat) Sigma dtau (transport.jl:282-284)
@@ -222,9 +249,7 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(Waves<
             if (sample_coefficients<KIND, S>(c, smp.t[o], r, th, (fl & 2) != 0, (fl & 4) != 0, S(smp.x[o]), S(smp.y[o]), S(smp.z[o]),
                                              j4, a4, r3)) {
                 const S D = S(path_weight(L, nus[inu], a, r, th, dtau));
-                S O[16], E[4];
-                sample_step(j4, a4, r3, D, O, E);
-                sm.advance(O, E);
+                map_step(sm, j4, a4, r3, D);
                 step = true;
                 if (rec) {
                     S* w = rec + (int64_t)(k - 1) * nitems + item;
@@ -251,9 +276,7 @@ struct FusedF {
         quasi_cartesian_ks(c.a, c.rp, c.rm, s.r, s.th, s.ph, x, y, z);
         double j4[4], a4[4], r3[3];
         if (!sample_coefficients<KIND, double>(c, s.t, s.r, s.th, s.nur, s.nuth, x, y, z, j4, a4, r3)) return;
-        double O[16], E[4];
-        transfer_step(j4, a4, r3, path_weight(c.L, c.nu, c.a, s.r, s.th, dtau), O, E);
-        sm.advance(O, E);
+        sm.step(j4, a4, r3, path_weight(c.L, c.nu, c.a, s.r, s.th, dtau));
     }
 };
 // Work items (anchor segment m, item), item fastest; segment m of pixel j covers samples m L + 1 ..
