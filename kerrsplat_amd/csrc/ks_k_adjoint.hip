@@ -128,11 +128,15 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(Waves<
     }
 }
 
-// Per (64 consecutive items, splat): the element adjoints of every (item, sample) pair inside the
-// splat's weight cutoff, 21 partial sums per lane, summed over the wavefront; part[(ib ns + i) 21 + r].
-// The wavefront first walks its items' samples with the cheap weight test and queues the pairs that
-// pass (ballot + mbcnt into an LDS ring), then runs the element adjoint 64 pairs at a time, so that
-// lanes outside the splat's support do not idle through it. Every pair of a wavefront belongs to
+// Per (64 consecutive items, slice of ADJ_SLICE samples, splat): the element adjoints of every
+// (item, sample) pair inside the splat's weight cutoff, 21 partial sums per lane, summed over the
+// wavefront; part[(u ns + i) 21 + r] for the unit u = ib nslice + slice. The slices bound the work of
+// one wavefront: the pairs of a splat crowd into a few item blocks and stretches of the rays, and
+// whole-ray units left most of the chip idle behind the heaviest ones.
+// The wavefront first walks its items' samples with the cheap weight test, WALK samples at a time with
+// their loads issued together, and queues the pairs that pass (ballot + mbcnt into an LDS ring), then
+// runs the element adjoint 64 pairs at a time, so that lanes outside the splat's support do not idle
+// through it. Every pair of a wavefront belongs to
 // splat i, so the sums stay per lane in registers; the queue order is fixed, so the result is
 // deterministic.
 template <class S>
@@ -140,16 +144,27 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(Waves<S
                                                     const SplatCT<S>* __restrict__ sp, const SplatAdjT<S>* __restrict__ sadj, int ns,
                                                     const double* __restrict__ times, int nt, const double* __restrict__ nus, int64_t nitems,
                                                     const S* __restrict__ rec, const uint8_t* __restrict__ aflg, double* __restrict__ part) {
-    constexpr int QCAP = 128;   // ring capacity: < 64 pending + 64 pushed by one sample step
+    constexpr int WALK = 8;                 // samples tested per group, their loads issued together
+    constexpr int QCAP = 1024;              // ring capacity: < 64 pending + 64 per sample of a group
+    static_assert(QCAP >= 64 * (WALK + 1) && (QCAP & (QCAP - 1)) == 0, "the ring must hold a group's pushes");
     __shared__ double s_alpha[64], s_beta[64], s_eta[64], s_lam[64], s_tobs[64], s_nu[64];
     __shared__ int s_q[QCAP];   // entry = lane | k << 6
     const int i = (int)(blockIdx.x % ns);
-    const int64_t ib = blockIdx.x / ns;
+    const int64_t unit = blockIdx.x / ns;
+    const int nslice = (N + ADJ_SLICE - 1) / ADJ_SLICE;
+    const int64_t ib = unit / nslice;
+    const int kfirst = (int)(unit % nslice) * ADJ_SLICE + 1;
+    const int klast = kfirst + ADJ_SLICE - 1 < N ? kfirst + ADJ_SLICE - 1 : N;
     const int lane = (int)threadIdx.x;
     const int64_t item = ib * 64 + lane;
     const bool valid = item < nitems;
-    const SplatCT<S> c = sp[i];
-    const SplatAdjT<S> sa = sadj[i];
+    // the splat's constants in LDS, read where they are used: held in registers for the whole kernel
+    // they outgrow the scalar file and spill into vector lanes
+    __shared__ SplatCT<S> s_c;
+    __shared__ SplatAdjT<S> s_sa;
+    if (lane == 0) { s_c = sp[i]; s_sa = sadj[i]; }
+    const SplatCT<S>& c = s_c;
+    const SplatAdjT<S>& sa = s_sa;
     int64_t j = 0;
     double t_obs = 0.0;
     if (valid) {
@@ -192,22 +207,31 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(Waves<S
         qhead = (qhead + cnt) & (QCAP - 1);
         qn -= cnt;
     };
-    for (int k = 1; k <= N; ++k) {
-        bool act = false;
-        if (valid) {
-            const int64_t fo = (int64_t)(k - 1) * nitems + item;
-            if (aflg[fo]) {
+    for (int k0 = kfirst; k0 <= klast; k0 += WALK) {
+        uint8_t fl[WALK];
+        double tk[WALK], xk[WALK], yk[WALK], zk[WALK];
+#pragma unroll
+        for (int u = 0; u < WALK; ++u) {
+            const int k = k0 + u;
+            fl[u] = 0;
+            tk[u] = xk[u] = yk[u] = zk[u] = 0.0;
+            if (valid && k <= klast) {
                 const int64_t so = (int64_t)(k - 1) * npix + j;
-                act = weight_active(c, S(t_obs - smp.t[so]), S(smp.x[so]), S(smp.y[so]), S(smp.z[so]));
+                fl[u] = aflg[(int64_t)(k - 1) * nitems + item];
+                tk[u] = smp.t[so]; xk[u] = smp.x[so]; yk[u] = smp.y[so]; zk[u] = smp.z[so];
             }
         }
-        const uint64_t m = __ballot(act);
-        if (act) {
-            const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-            s_q[(qhead + qn + rank) & (QCAP - 1)] = lane | (k << 6);
+#pragma unroll
+        for (int u = 0; u < WALK; ++u) {
+            const bool act = fl[u] && weight_active(c, S(t_obs - tk[u]), S(xk[u]), S(yk[u]), S(zk[u]));
+            const uint64_t m = __ballot(act);
+            if (act) {
+                const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                s_q[(qhead + qn + rank) & (QCAP - 1)] = lane | ((k0 + u) << 6);
+            }
+            qn += __popcll(m);
         }
-        qn += __popcll(m);
-        if (qn >= 64) {
+        while (qn >= 64) {
             __syncthreads();
             run_batch(64);
             __syncthreads();
@@ -225,18 +249,18 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(Waves<S
         gsum[r] = v;
     }
     if (threadIdx.x == 0) {
-        double* o = part + ((int64_t)ib * ns + i) * ADJ_ROWS;
+        double* o = part + (unit * ns + i) * ADJ_ROWS;
         for (int r = 0; r < ADJ_ROWS; ++r) o[r] = gsum[r];
     }
 }
 
-// dparams[r + 21 i] = sum over item blocks in ascending order (deterministic)
-__global__ void ks_adj_reduce(const double* __restrict__ part, int64_t nib, int ns, double* __restrict__ dparams) {
+// dparams[r + 21 i] = sum over the work units in ascending order (deterministic)
+__global__ void ks_adj_reduce(const double* __restrict__ part, int64_t nunits, int ns, double* __restrict__ dparams) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)ns * ADJ_ROWS) return;
     const int i = (int)(t / ADJ_ROWS), r = (int)(t % ADJ_ROWS);
     double s = 0.0;
-    for (int64_t b = 0; b < nib; ++b) s += part[(b * ns + i) * ADJ_ROWS + r];
+    for (int64_t b = 0; b < nunits; ++b) s += part[(b * ns + i) * ADJ_ROWS + r];
     dparams[t] = s;
 }
 
@@ -260,21 +284,21 @@ hipError_t launch_adj_segment_f32(hipStream_t s, int N, int64_t nitems, const do
 hipError_t launch_adj_splats(hipStream_t s, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
                              const SplatC* sp, const SplatAdj* sadj, int ns, const double* times, int nt, const double* nus,
                              int64_t nitems, const double* rec, const uint8_t* aflg, double* part) {
-    const int64_t nib = (nitems + 63) / 64;
-    hipLaunchKernelGGL(ks_adj_splats<double>, dim3((unsigned)(nib * ns)), dim3(64), 0, s, a, thetao, soa, npix, N, smp, sp, sadj, ns, times, nt, nus,
+    const int64_t nunits = adj_units(nitems, N);
+    hipLaunchKernelGGL(ks_adj_splats<double>, dim3((unsigned)(nunits * ns)), dim3(64), 0, s, a, thetao, soa, npix, N, smp, sp, sadj, ns, times, nt, nus,
                        nitems, rec, aflg, part);
     return hipGetLastError();
 }
 hipError_t launch_adj_splats_f32(hipStream_t s, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
                                  const SplatCf* sp, const SplatAdjf* sadj, int ns, const double* times, int nt, const double* nus,
                                  int64_t nitems, const f32* rec, const uint8_t* aflg, double* part) {
-    const int64_t nib = (nitems + 63) / 64;
-    hipLaunchKernelGGL(ks_adj_splats<f32>, dim3((unsigned)(nib * ns)), dim3(64), 0, s, a, thetao, soa, npix, N, smp, sp, sadj, ns, times, nt, nus,
+    const int64_t nunits = adj_units(nitems, N);
+    hipLaunchKernelGGL(ks_adj_splats<f32>, dim3((unsigned)(nunits * ns)), dim3(64), 0, s, a, thetao, soa, npix, N, smp, sp, sadj, ns, times, nt, nus,
                        nitems, rec, aflg, part);
     return hipGetLastError();
 }
-hipError_t launch_adj_reduce(hipStream_t s, const double* part, int64_t nib, int ns, double* dparams) {
-    hipLaunchKernelGGL(ks_adj_reduce, dim3(grid_for((int64_t)ns * ADJ_ROWS, 64)), dim3(64), 0, s, part, nib, ns, dparams);
+hipError_t launch_adj_reduce(hipStream_t s, const double* part, int64_t nunits, int ns, double* dparams) {
+    hipLaunchKernelGGL(ks_adj_reduce, dim3(grid_for((int64_t)ns * ADJ_ROWS, 64)), dim3(64), 0, s, part, nunits, ns, dparams);
     return hipGetLastError();
 }
 

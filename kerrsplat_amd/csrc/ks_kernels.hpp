@@ -21,6 +21,9 @@ constexpr int PART_BLOCK = 256;    // block of the case partition
 constexpr int CHUNK = 32;          // samples per work item of the transfer from stored samples
 constexpr int SEG_FIELDS = 20;     // a segment map: P (16, row-major), then Sv (4)
 constexpr int ADJ_ROWS = 21;       // parameter rows of a thermal splat
+constexpr int ADJ_SLICE = 128;     // samples per work unit of the element adjoints
+// element-adjoint work units (64 items x ADJ_SLICE samples) of nitems items of N samples
+inline int64_t adj_units(int64_t nitems, int N) { return ((nitems + 63) / 64) * ((N + ADJ_SLICE - 1) / ADJ_SLICE); }
 // Minimum waves per SIMD requested for the heavy kernels (the register budget the compiler may use:
 // 512 / waves VGPRs per lane), per precision; tuned by measurement on MI355X.
 #ifndef KS_WAVES_MARCH
@@ -109,7 +112,7 @@ hipError_t launch_adj_splats_f32(hipStream_t s, double a, double thetao, const d
 hipError_t launch_adj_splats(hipStream_t s, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
                              const SplatC* sp, const SplatAdj* sadj, int ns, const double* times, int nt, const double* nus,
                              int64_t nitems, const double* rec, const uint8_t* aflg, double* part);
-hipError_t launch_adj_reduce(hipStream_t s, const double* part, int64_t nib, int ns, double* dparams);
+hipError_t launch_adj_reduce(hipStream_t s, const double* part, int64_t nunits, int ns, double* dparams);
 
 // ---- ks_k_vis.hip ----
 hipError_t launch_vis_tables(hipStream_t s, int nuv, int na, int nb, double psize, const double* u, const double* v, double* A2, double* B2);

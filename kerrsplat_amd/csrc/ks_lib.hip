@@ -610,11 +610,11 @@ static int ensure_adj(ks_cache* c, int64_t items, int ns) {
             return KS_ERR_HIP;
         c->a_items = items;
     }
-    const int64_t nib = (items + 63) / 64;
-    if (c->a_part < nib * ns * ADJ_ROWS) {
+    const int64_t nunits = adj_units(items, c->N);
+    if (c->a_part < nunits * ns * ADJ_ROWS) {
         dfree(c->apart);
-        if (dalloc(&c->apart, (size_t)nib * ns * ADJ_ROWS)) return KS_ERR_HIP;
-        c->a_part = nib * ns * ADJ_ROWS;
+        if (dalloc(&c->apart, (size_t)nunits * ns * ADJ_ROWS)) return KS_ERR_HIP;
+        c->a_part = nunits * ns * ADJ_ROWS;
     }
     if (c->a_dpar < (int64_t)ns * ADJ_ROWS) {
         dfree(c->adpar);
@@ -686,7 +686,6 @@ int ks_gradient_reverse(ks_cache* c, const ks_model* m, const double* dstokes, i
     if (mixed) HIPCHK(launch_adj_segment_f32(x->stream, c->N, items, c->awT, (f32*)c->arec, c->aflg, (f32*)c->atail));
     else HIPCHK(launch_adj_segment(x->stream, c->N, items, c->awT, c->arec, c->aflg, c->atail));
     HIPCHK(x->stage_end(KS_STAGE_ADJ_STEPS));
-    const int64_t nib = (items + 63) / 64;
     if (ns > 0) {
         HIPCHK(x->stage_begin(KS_STAGE_ADJ_ELEMENTS));
         if (mixed)
@@ -696,7 +695,7 @@ int ks_gradient_reverse(ks_cache* c, const ks_model* m, const double* dstokes, i
             HIPCHK(launch_adj_splats(x->stream, c->spin, c->thetao, c->soa, c->npix, c->N, samples_of(c), m->d_splats, m->d_adj, ns, d_tn, nt,
                                      d_tn + nt, items, c->arec, c->aflg, c->apart));
         HIPCHK(x->stage_end(KS_STAGE_ADJ_ELEMENTS));
-        HIPCHK(launch_adj_reduce(x->stream, c->apart, nib, ns, c->adpar));
+        HIPCHK(launch_adj_reduce(x->stream, c->apart, adj_units(items, c->N), ns, c->adpar));
     }
     std::vector<double> g((size_t)ns * ADJ_ROWS);
     if (ns > 0) HIPCHK(hipMemcpyAsync(g.data(), c->adpar, g.size() * sizeof(double), hipMemcpyDeviceToHost, x->stream));
