@@ -99,8 +99,9 @@ template <int N, class S> KS_DEV double val(const DN<N, S>& x) { return val(x.v)
 KSDD_T KS_DEV DN<N, S> operator+(const DN<N, S>& a, const DN<N, S>& b) { DN<N, S> r; r.v = a.v + b.v; KSDD_LOOP r.d[i] = a.d[i] + b.d[i]; return r; }
 KSDD_T KS_DEV DN<N, S> operator-(const DN<N, S>& a, const DN<N, S>& b) { DN<N, S> r; r.v = a.v - b.v; KSDD_LOOP r.d[i] = a.d[i] - b.d[i]; return r; }
 KSDD_T KS_DEV DN<N, S> operator*(const DN<N, S>& a, const DN<N, S>& b) { DN<N, S> r; r.v = a.v * b.v; KSDD_LOOP r.d[i] = a.d[i] * b.v + a.v * b.d[i]; return r; }
+// one reciprocal per dual division (the value then carries at most one more rounding)
 KSDD_T KS_DEV DN<N, S> operator/(const DN<N, S>& a, const DN<N, S>& b) {
-    DN<N, S> r; r.v = a.v / b.v; const S ib = 1.0 / b.v;
+    DN<N, S> r; const S ib = 1.0 / b.v; r.v = a.v * ib;
     KSDD_LOOP r.d[i] = (a.d[i] - r.v * b.d[i]) * ib;
     return r;
 }
@@ -111,7 +112,7 @@ KSDD_T KS_DEV DN<N, S> operator-(double a, const DN<N, S>& b) { DN<N, S> r; r.v 
 KSDD_T KS_DEV DN<N, S> operator*(const DN<N, S>& a, double b) { DN<N, S> r; r.v = a.v * b; KSDD_LOOP r.d[i] = a.d[i] * b; return r; }
 KSDD_T KS_DEV DN<N, S> operator*(double a, const DN<N, S>& b) { return b * a; }
 KSDD_T KS_DEV DN<N, S> operator/(const DN<N, S>& a, double b) { return a * (1.0 / b); }
-KSDD_T KS_DEV DN<N, S> operator/(double a, const DN<N, S>& b) { DN<N, S> r; r.v = a / b.v; const S f = -r.v / b.v; KSDD_LOOP r.d[i] = f * b.d[i]; return r; }
+KSDD_T KS_DEV DN<N, S> operator/(double a, const DN<N, S>& b) { DN<N, S> r; const S ib = 1.0 / b.v; r.v = a * ib; const S f = -r.v * ib; KSDD_LOOP r.d[i] = f * b.d[i]; return r; }
 KSDD_T KS_DEV DN<N, S> operator-(const DN<N, S>& a) { DN<N, S> r; r.v = -a.v; KSDD_LOOP r.d[i] = -a.d[i]; return r; }
 KSDD_T KS_DEV DN<N, S>& operator+=(DN<N, S>& a, const DN<N, S>& b) { a = a + b; return a; }
 KSDD_T KS_DEV DN<N, S>& operator*=(DN<N, S>& a, const DN<N, S>& b) { a = a * b; return a; }
@@ -125,7 +126,7 @@ KSDD_TS KS_DEV DN<N, S> operator+(S a, const DN<N, S>& b) { DN<N, S> r = b; r.v 
 KSDD_TS KS_DEV DN<N, S> operator-(const DN<N, S>& a, S b) { DN<N, S> r = a; r.v = r.v - b; return r; }
 KSDD_TS KS_DEV DN<N, S> operator-(S a, const DN<N, S>& b) { DN<N, S> r; r.v = a - b.v; KSDD_LOOP r.d[i] = -b.d[i]; return r; }
 KSDD_TS KS_DEV DN<N, S> operator/(const DN<N, S>& a, S b) { const S ib = 1.0 / b; return a * ib; }
-KSDD_TS KS_DEV DN<N, S> operator/(S a, const DN<N, S>& b) { DN<N, S> r; r.v = a / b.v; const S f = -r.v / b.v; KSDD_LOOP r.d[i] = f * b.d[i]; return r; }
+KSDD_TS KS_DEV DN<N, S> operator/(S a, const DN<N, S>& b) { DN<N, S> r; const S ib = 1.0 / b.v; r.v = a * ib; const S f = -r.v * ib; KSDD_LOOP r.d[i] = f * b.d[i]; return r; }
 #undef KSDD_TS
 #define KSDD_CMP(op)                                                                                     \
     KSDD_T KS_DEV bool operator op(const DN<N, S>& a, const DN<N, S>& b) { return a.v op b.v; }         \
