@@ -92,10 +92,10 @@ KS_DEV void step_apply_s(const S j4[4], const S a4[4], const S r3[3], S D, const
 // in the record) and w <- O^T w (polarized_dual_kernel!'s ray loop, dual_gradient.jl:151-178).
 template <class S>
 __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(Waves<S>::adjstep))) ks_adj_segment(int N, int64_t nitems, int nch, const double* __restrict__ wT, S* __restrict__ rec,
-                                                      const uint8_t* __restrict__ aflg, S* __restrict__ tails) {
+                                                      const uint8_t* __restrict__ aflg, S* __restrict__ tails, Samples smp, int64_t npix) {
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= nitems * nch) return;
-    const int64_t item = tid % nitems;
+    const int64_t item = slot_item(smp, tid % nitems, npix);     // wavefronts over compact patches of sky
     const int m = (int)(tid / nitems);
     const int kbeg = m * CHUNK + 1, kend = kbeg + CHUNK - 1 < N ? kbeg + CHUNK - 1 : N;
     const int64_t st = (int64_t)N * nitems;
@@ -148,6 +148,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(Waves<S
     constexpr int QCAP = 1024;              // ring capacity: < 64 pending + 64 per sample of a group
     static_assert(QCAP >= 64 * (WALK + 1) && (QCAP & (QCAP - 1)) == 0, "the ring must hold a group's pushes");
     __shared__ double s_alpha[64], s_beta[64], s_eta[64], s_lam[64], s_tobs[64], s_nu[64];
+    __shared__ int64_t s_item[64];
     __shared__ int s_q[QCAP];   // entry = lane | k << 6
     const int i = (int)(blockIdx.x % ns);
     const int64_t unit = blockIdx.x / ns;
@@ -156,8 +157,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(Waves<S
     const int kfirst = (int)(unit % nslice) * ADJ_SLICE + 1;
     const int klast = kfirst + ADJ_SLICE - 1 < N ? kfirst + ADJ_SLICE - 1 : N;
     const int lane = (int)threadIdx.x;
-    const int64_t item = ib * 64 + lane;
-    const bool valid = item < nitems;
+    const bool valid = ib * 64 + lane < nitems;
+    const int64_t item = ib * 64 + lane;      // sorted order: the pair queue already packs the lanes
     // the splat's constants in LDS, read where they are used: held in registers for the whole kernel
     // they outgrow the scalar file and spill into vector lanes
     __shared__ SplatCT<S> s_c;
@@ -177,6 +178,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(Waves<S
         s_lam[lane] = soa[(int64_t)PF_LAM * npix + j];
         s_tobs[lane] = t_obs;
         s_nu[lane] = nus[fr / nt];
+        s_item[lane] = item;
     }
     __syncthreads();
     const S tos = S(sin(thetao)), aS = S(a);
@@ -188,7 +190,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(Waves<S
         if (lane < cnt) {
             const int e = s_q[(qhead + lane) & (QCAP - 1)];
             const int l = e & 63, k = e >> 6;
-            const int64_t it2 = ib * 64 + l;
+            const int64_t it2 = s_item[l];
             const int64_t fo = (int64_t)(k - 1) * nitems + it2;
             const int64_t so = (int64_t)(k - 1) * npix + it2 % npix;
             S dl[13];
@@ -271,14 +273,16 @@ hipError_t launch_adj_combine(hipStream_t s, int64_t npix, int N, int nt, const 
     hipLaunchKernelGGL(ks_adj_combine, dim3(grid_for(items, 128)), dim3(128), 0, s, npix, N, nt, nus, nnu, seg, dstokes, wT, out);
     return hipGetLastError();
 }
-hipError_t launch_adj_segment(hipStream_t s, int N, int64_t nitems, const double* wT, double* rec, const uint8_t* aflg, double* tails) {
+hipError_t launch_adj_segment(hipStream_t s, int N, int64_t nitems, const double* wT, double* rec, const uint8_t* aflg, double* tails,
+                              Samples smp, int64_t npix) {
     const int nch = (N + CHUNK - 1) / CHUNK;
-    hipLaunchKernelGGL(ks_adj_segment<double>, dim3(grid_for(nitems * nch, 128)), dim3(128), 0, s, N, nitems, nch, wT, rec, aflg, tails);
+    hipLaunchKernelGGL(ks_adj_segment<double>, dim3(grid_for(nitems * nch, 128)), dim3(128), 0, s, N, nitems, nch, wT, rec, aflg, tails, smp, npix);
     return hipGetLastError();
 }
-hipError_t launch_adj_segment_f32(hipStream_t s, int N, int64_t nitems, const double* wT, f32* rec, const uint8_t* aflg, f32* tails) {
+hipError_t launch_adj_segment_f32(hipStream_t s, int N, int64_t nitems, const double* wT, f32* rec, const uint8_t* aflg, f32* tails,
+                                  Samples smp, int64_t npix) {
     const int nch = (N + CHUNK - 1) / CHUNK;
-    hipLaunchKernelGGL(ks_adj_segment<f32>, dim3(grid_for(nitems * nch, 128)), dim3(128), 0, s, N, nitems, nch, wT, rec, aflg, tails);
+    hipLaunchKernelGGL(ks_adj_segment<f32>, dim3(grid_for(nitems * nch, 128)), dim3(128), 0, s, N, nitems, nch, wT, rec, aflg, tails, smp, npix);
     return hipGetLastError();
 }
 hipError_t launch_adj_splats(hipStream_t s, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,

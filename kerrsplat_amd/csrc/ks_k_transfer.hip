@@ -222,7 +222,7 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(Waves<
     const int64_t nitems = npix * nt * nnu;
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= nitems * nch) return;
-    const int64_t item = tid % nitems;
+    const int64_t item = slot_item(smp, tid % nitems, npix);     // wavefronts over compact patches of sky
     const int m = (int)(tid / nitems);
     const int64_t j = item % npix;
     const int64_t fr = item / npix;

@@ -61,7 +61,12 @@ constexpr double FARADAY_FP64 = 10.0;
 struct Samples {
     double *t, *r, *th, *ph, *x, *y, *z;
     uint8_t* fl;
+    const int* tmap;      // the sorted pixels in wavefront order (8 x 8 screen tiles), or null for sorted order
 };
+// the item of work slot t (t = j + npix f): the pixel at slot t % npix of the wavefront order, same frame
+KS_DEV int64_t slot_item(const Samples& smp, int64_t t, int64_t npix) {
+    return smp.tmap ? (int64_t)smp.tmap[t % npix] + npix * (t / npix) : t;
+}
 
 // Adjoint records of a step-taking sample: rec[(f N + k - 1) nitems + item], f = the invariant j (4),
 // alpha (4), rho (3), Delta; the adjoint sweep writes j_bar, alpha_bar, rho_bar over j, alpha, rho.
@@ -104,8 +109,10 @@ hipError_t launch_seg_combine(hipStream_t s, const double* soa, int64_t npix, in
 // ---- ks_k_adjoint.hip ----
 hipError_t launch_adj_combine(hipStream_t s, int64_t npix, int N, int nt, const double* nus, int nnu, const double* seg,
                               const double* dstokes, double* wT, double* out);
-hipError_t launch_adj_segment(hipStream_t s, int N, int64_t nitems, const double* wT, double* rec, const uint8_t* aflg, double* tails);
-hipError_t launch_adj_segment_f32(hipStream_t s, int N, int64_t nitems, const double* wT, f32* rec, const uint8_t* aflg, f32* tails);
+hipError_t launch_adj_segment(hipStream_t s, int N, int64_t nitems, const double* wT, double* rec, const uint8_t* aflg, double* tails,
+                              Samples smp, int64_t npix);
+hipError_t launch_adj_segment_f32(hipStream_t s, int N, int64_t nitems, const double* wT, f32* rec, const uint8_t* aflg, f32* tails,
+                                  Samples smp, int64_t npix);
 hipError_t launch_adj_splats_f32(hipStream_t s, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
                                  const SplatCf* sp, const SplatAdjf* sadj, int ns, const double* times, int nt, const double* nus,
                                  int64_t nitems, const f32* rec, const uint8_t* aflg, double* part);

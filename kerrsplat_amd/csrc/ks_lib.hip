@@ -80,6 +80,7 @@ struct ks_cache {
     double *al = nullptr, *be = nullptr;
     int8_t* numreals = nullptr;
     int* perm = nullptr;
+    int* tmap = nullptr;                               // wavefront grouping of the sorted pixels (8 x 8 screen tiles)
     int* bcount = nullptr; int nblocks = 0;
     double* soa = nullptr;
     double *st = nullptr, *sr = nullptr, *sth = nullptr, *sph = nullptr; uint8_t* sfl = nullptr;
@@ -195,6 +196,7 @@ int ks_cache_create(ks_ctx* ctx, int64_t npix, int N, int store_samples, ks_cach
     c->nblocks = (int)grid_for(npix, PART_BLOCK);
     int rc = 0;
     rc |= dalloc(&c->al, npix); rc |= dalloc(&c->be, npix); rc |= dalloc(&c->numreals, npix); rc |= dalloc(&c->perm, npix);
+    rc |= dalloc(&c->tmap, npix);
     rc |= dalloc(&c->bcount, 3 * (size_t)c->nblocks);
     rc |= dalloc(&c->soa, (size_t)PF_COUNT * npix);
     rc |= dalloc(&c->rt, npix); rc |= dalloc(&c->rp, npix);
@@ -216,7 +218,7 @@ int ks_cache_create(ks_ctx* ctx, int64_t npix, int N, int store_samples, ks_cach
 int ks_cache_destroy(ks_cache* c) {
     if (!c) return KS_OK;
     (void)hipSetDevice(c->ctx->device);
-    dfree(c->al); dfree(c->be); dfree(c->numreals); dfree(c->perm); dfree(c->bcount); dfree(c->soa);
+    dfree(c->al); dfree(c->be); dfree(c->numreals); dfree(c->perm); dfree(c->tmap); dfree(c->bcount); dfree(c->soa);
     dfree(c->st); dfree(c->sr); dfree(c->sth); dfree(c->sph); dfree(c->sfl); dfree(c->rt); dfree(c->rp);
     dfree(c->sx); dfree(c->sy); dfree(c->sz); dfree(c->thin_part);
     dfree(c->out); dfree(c->out_screen); dfree(c->img1);
@@ -242,11 +244,13 @@ int ks_cache_info(const ks_cache* c, int64_t* npix, int* N, int* generation, int
 static Samples samples_of(const ks_cache* c) {
     Samples S;
     S.t = c->st; S.r = c->sr; S.th = c->sth; S.ph = c->sph; S.x = c->sx; S.y = c->sy; S.z = c->sz; S.fl = c->sfl;
+    S.tmap = c->tmap;
     return S;
 }
 static Samples no_samples() {
     Samples S;
     S.t = S.r = S.th = S.ph = S.x = S.y = S.z = nullptr; S.fl = nullptr;
+    S.tmap = nullptr;
     return S;
 }
 // the cache's marcher into its sample storage (march!, cache.jl:182-190), anchor segments in parallel
@@ -302,6 +306,19 @@ int ks_regenerate(ks_cache* c, const double* alpha, const double* beta, int nalp
     }
     HIPCHK(hipMemcpyAsync(c->perm_host.data(), c->perm, n * sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    // Wavefront grouping for the kernels over stored samples: the sorted pixels in 8 x 8 screen tiles, so
+    // that the 64 lanes of a wavefront look at one compact patch of sky (a splat then covers most or none
+    // of them); a tile row of 8 pixels stays contiguous in the case-sorted order.
+    {
+        std::vector<int> inv(n), tm;
+        tm.reserve(n);
+        for (int64_t j = 0; j < n; ++j) inv[c->perm_host[j]] = (int)j;
+        for (int tb = 0; tb < nbeta; tb += 8)
+            for (int ta = 0; ta < nalpha; ta += 8)
+                for (int ib = tb; ib < std::min(tb + 8, nbeta); ++ib)
+                    for (int ia = ta; ia < std::min(ta + 8, nalpha); ++ia) tm.push_back(inv[ia + (int64_t)nalpha * ib]);
+        HIPCHK(hipMemcpy(c->tmap, tm.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    }
     c->generation += 1;
     return KS_OK;
 }
@@ -683,8 +700,8 @@ int ks_gradient_reverse(ks_cache* c, const ks_model* m, const double* dstokes, i
     HIPCHK(launch_adj_combine(x->stream, c->npix, c->N, nt, d_tn + nt, nnu, c->seg, c->ads, c->awT, nullptr));
     HIPCHK(x->stage_begin(KS_STAGE_ADJ_STEPS));
     const bool mixed = c->precision == KS_PRECISION_MIXED;
-    if (mixed) HIPCHK(launch_adj_segment_f32(x->stream, c->N, items, c->awT, (f32*)c->arec, c->aflg, (f32*)c->atail));
-    else HIPCHK(launch_adj_segment(x->stream, c->N, items, c->awT, c->arec, c->aflg, c->atail));
+    if (mixed) HIPCHK(launch_adj_segment_f32(x->stream, c->N, items, c->awT, (f32*)c->arec, c->aflg, (f32*)c->atail, samples_of(c), c->npix));
+    else HIPCHK(launch_adj_segment(x->stream, c->N, items, c->awT, c->arec, c->aflg, c->atail, samples_of(c), c->npix));
     HIPCHK(x->stage_end(KS_STAGE_ADJ_STEPS));
     if (ns > 0) {
         HIPCHK(x->stage_begin(KS_STAGE_ADJ_ELEMENTS));
