@@ -143,15 +143,23 @@ template <class S>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(Waves<S>::adjelem))) ks_adj_splats(double a, double thetao, const double* __restrict__ soa, int64_t npix, int N, Samples smp,
                                                     const SplatCT<S>* __restrict__ sp, const SplatAdjT<S>* __restrict__ sadj, int ns,
                                                     const double* __restrict__ times, int nt, const double* __restrict__ nus, int64_t nitems,
-                                                    const S* __restrict__ rec, const uint8_t* __restrict__ aflg, double* __restrict__ part) {
+                                                    const S* __restrict__ rec, const uint8_t* __restrict__ aflg, int64_t nunits, double* __restrict__ part) {
     constexpr int WALK = 8;                 // samples tested per group, their loads issued together
     constexpr int QCAP = 1024;              // ring capacity: < 64 pending + 64 per sample of a group
     static_assert(QCAP >= 64 * (WALK + 1) && (QCAP & (QCAP - 1)) == 0, "the ring must hold a group's pushes");
     __shared__ double s_alpha[64], s_beta[64], s_eta[64], s_lam[64], s_tobs[64], s_nu[64];
     __shared__ int64_t s_item[64];
     __shared__ int s_q[QCAP];   // entry = lane | k << 6
-    const int i = (int)(blockIdx.x % ns);
-    const int64_t unit = blockIdx.x / ns;
+    // XCD-aware order: the hardware deals consecutive blocks round-robin over the 8 XCDs, so block b runs on
+    // XCD b % 8; all ns splats of a unit go to one XCD (and its L2, which then reads the unit's samples once)
+    int i;
+    int64_t unit;
+    {
+        const int64_t b = blockIdx.x, q = b / 8, x = b % 8;
+        i = (int)(q % ns);
+        unit = (q / ns) * 8 + x;
+        if (unit >= nunits) return;       // the padding of the last group of 8 units
+    }
     const int nslice = (N + ADJ_SLICE - 1) / ADJ_SLICE;
     const int64_t ib = unit / nslice;
     const int kfirst = (int)(unit % nslice) * ADJ_SLICE + 1;
@@ -256,14 +264,16 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(Waves<S
     }
 }
 
-// dparams[r + 21 i] = sum over the work units in ascending order (deterministic)
-__global__ void ks_adj_reduce(const double* __restrict__ part, int64_t nunits, int ns, double* __restrict__ dparams) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// dparams[r + 21 i] = the sum over the work units in a fixed order (deterministic): one wavefront per
+// (splat, row), lane l summing units l, l + 64, ..., then a fixed butterfly over the lanes
+__global__ void __launch_bounds__(64) ks_adj_reduce(const double* __restrict__ part, int64_t nunits, int ns, double* __restrict__ dparams) {
+    const int64_t t = blockIdx.x;
     if (t >= (int64_t)ns * ADJ_ROWS) return;
     const int i = (int)(t / ADJ_ROWS), r = (int)(t % ADJ_ROWS);
     double s = 0.0;
-    for (int64_t b = 0; b < nunits; ++b) s += part[(b * ns + i) * ADJ_ROWS + r];
-    dparams[t] = s;
+    for (int64_t b = threadIdx.x; b < nunits; b += 64) s += part[(b * ns + i) * ADJ_ROWS + r];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (threadIdx.x == 0) dparams[t] = s;
 }
 
 // ---- launchers ----
@@ -289,20 +299,20 @@ hipError_t launch_adj_splats(hipStream_t s, double a, double thetao, const doubl
                              const SplatC* sp, const SplatAdj* sadj, int ns, const double* times, int nt, const double* nus,
                              int64_t nitems, const double* rec, const uint8_t* aflg, double* part) {
     const int64_t nunits = adj_units(nitems, N);
-    hipLaunchKernelGGL(ks_adj_splats<double>, dim3((unsigned)(nunits * ns)), dim3(64), 0, s, a, thetao, soa, npix, N, smp, sp, sadj, ns, times, nt, nus,
-                       nitems, rec, aflg, part);
+    hipLaunchKernelGGL(ks_adj_splats<double>, dim3((unsigned)((nunits + 7) / 8 * 8 * ns)), dim3(64), 0, s, a, thetao, soa, npix, N, smp, sp, sadj, ns, times,
+                       nt, nus, nitems, rec, aflg, nunits, part);
     return hipGetLastError();
 }
 hipError_t launch_adj_splats_f32(hipStream_t s, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
                                  const SplatCf* sp, const SplatAdjf* sadj, int ns, const double* times, int nt, const double* nus,
                                  int64_t nitems, const f32* rec, const uint8_t* aflg, double* part) {
     const int64_t nunits = adj_units(nitems, N);
-    hipLaunchKernelGGL(ks_adj_splats<f32>, dim3((unsigned)(nunits * ns)), dim3(64), 0, s, a, thetao, soa, npix, N, smp, sp, sadj, ns, times, nt, nus,
-                       nitems, rec, aflg, part);
+    hipLaunchKernelGGL(ks_adj_splats<f32>, dim3((unsigned)((nunits + 7) / 8 * 8 * ns)), dim3(64), 0, s, a, thetao, soa, npix, N, smp, sp, sadj, ns, times,
+                       nt, nus, nitems, rec, aflg, nunits, part);
     return hipGetLastError();
 }
 hipError_t launch_adj_reduce(hipStream_t s, const double* part, int64_t nunits, int ns, double* dparams) {
-    hipLaunchKernelGGL(ks_adj_reduce, dim3(grid_for((int64_t)ns * ADJ_ROWS, 64)), dim3(64), 0, s, part, nunits, ns, dparams);
+    hipLaunchKernelGGL(ks_adj_reduce, dim3((unsigned)(ns * ADJ_ROWS)), dim3(64), 0, s, part, nunits, ns, dparams);
     return hipGetLastError();
 }
 
