@@ -173,14 +173,20 @@ def main():
     # against the FP64 vector peak on the FP64 path and the FP32 vector peak on the mixed path
     wm_path = os.path.join(ROOT, "profiles", "work_model.json")
     wm = json.load(open(wm_path)).get(cfg["name"]) if os.path.exists(wm_path) else None
-    kernel_of = {"march": "ks_march_store", "transfer": "ks_chunk_transfer", "adjoint_steps": "ks_adj_segment",
-                 "adjoint_elements": "ks_adj_splats"}
+    tpl = {"fp64": "double", "mixed": "f32"}
+    kernel_of = {"march": lambda p_: "ks_march_store<1>", "transfer": lambda p_: f"ks_chunk_transfer<1, {tpl[p_]}>",
+                 "adjoint_steps": lambda p_: f"ks_adj_segment<{tpl[p_]}>", "adjoint_elements": lambda p_: f"ks_adj_splats<{tpl[p_]}>"}
+    # HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE summary (rocprofv3 --pmc, scaled by a copy probe)
+    tr_path = os.path.join(ROOT, "profiles", "traffic_128x128_N1024_32thermal.json")
+    traffic = json.load(open(tr_path))["kernels"] if (os.path.exists(tr_path) and args.config == 1) else {}
 
     def stage_table(prec):
         peak = FP64_PEAK_TFLOPS if prec == "fp64" else FP32_PEAK_TFLOPS
         out = {}
         for k, t_ms in runs[prec]["adj_ms"].items():
-            st = {"kernel": kernel_of[k], "ms": t_ms}
+            st = {"kernel": kernel_of[k](prec), "ms": t_ms}
+            if st["kernel"] in traffic:
+                st["hbm_bytes_per_launch"] = traffic[st["kernel"]]["bytes"]
             if wm and "phases" in wm:
                 ph = wm["phases"]
                 per = {"march": ph["march"]["flops_per_ray_sample"], "transfer": ph["transfer"]["flops_per_ray_sample"],
@@ -199,7 +205,7 @@ def main():
     d = stages[head][dom]
     if "achieved_tflops" in d:
         roofline = {"bound": "valu-fp64" if (head == "fp64" or dom == "march") else "valu-fp32", "achieved": d["achieved_tflops"],
-                    "peak": d["peak_tflops"], "unit": "TFLOP/s", "frac": d["frac"], "traffic": None, "kernel": d["kernel"],
+                    "peak": d["peak_tflops"], "unit": "TFLOP/s", "frac": d["frac"], "traffic": d.get("hbm_bytes_per_launch"), "kernel": d["kernel"],
                     "kernel_ms": d["ms"], "flops_per_ray_sample": d["flops_per_ray_sample"]}
     paths = {p_: {"fwd_adjoint": {"value": work / r["adj_s"], "unit": "ray-samples/s", "ms_per_step": 1e3 * r["adj_s"],
                                   "gradient_evals_per_s": 1.0 / r["adj_s"], "stage_ms": r["adj_ms"]},
