@@ -86,6 +86,29 @@ def cpu_baseline(cfg, P, L, target_s, ds):
             "sample": f"{n2} of {cam.npix} rays (random subset) x {N} samples, fwd+adjoint (dual sweep), FP64 oracle, {dt2:.1f} s"}
 
 
+def visibility_stage(K, syn, cfg, model, L, res=512, nuv=8192, reps=3):
+    """The visibility transform's matrix-core product at the largest configuration's size (512 x 512
+    screen, 8192 uv points, one frame; the image rendered from the same splats over a fused 64-sample
+    march): HIP-event duration of the FP64 GEMM and its rate against the FP64 matrix peak."""
+    a, th = cfg["a"], math.radians(cfg["theta_o_deg"])
+    cam = K.Camera((-cfg["fov"], cfg["fov"]), (-cfg["fov"], cfg["fov"]), res)
+    cache = K.GeodesicCache(cam, 64, store_samples=False)
+    K.regenerate(cache, a, th, marcher=K.Fused(64))
+    K.render_device(cache, model, cfg["times"][:1], cfg["nus"][:1], L)
+    u, v = syn.uv_disc(4, nuv)
+    psize = syn.pixel_size_rad(-cfg["fov"], cfg["fov"], res)
+    K.visibilities(cache, 1, psize, u, v)
+    ms = []
+    for _ in range(reps):
+        K.visibilities(cache, 1, psize, u, v)
+        ms.append(K.stage_ms("vis"))
+    t = float(np.median(ms))
+    flops = 2.0 * (2 * nuv) * res * (4 * res)          # C2 = [cos; sin](2 nuv x n_alpha) . image (n_alpha x 4 n_beta)
+    achieved = flops / (t * 1e-3) / 1e12
+    return {"workload": f"{res}x{res} screen, {nuv} uv, 1 frame", "kernel": "ks_dgemm<false>", "bound": "mfma-fp64",
+            "ms": t, "achieved_tflops": achieved, "peak_tflops": FP64_PEAK_TFLOPS, "frac": achieved / FP64_PEAK_TFLOPS}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -212,6 +235,7 @@ def main():
                   "forward": {"value": work / r["fwd_s"], "unit": "ray-samples/s", "ms_per_step": 1e3 * r["fwd_s"], "stage_ms": r["fwd_ms"]}}
              for p_, r in runs.items()}
 
+    vis_stage = visibility_stage(K, syn, cfg, model, L) if cfg["nuv"] else None
     if args.copy_probe:
         copy_probe()
     cpu = None
@@ -235,7 +259,7 @@ def main():
            "vis_chi2": {p_: runs[p_]["chi2"] for p_ in runs},
            "paths": paths, "gradient_evals_per_s": 1.0 / adj_s,
            "stages": stages,
-           "roofline": roofline, "cpu_baseline": cpu}
+           "roofline": roofline, "cpu_baseline": cpu, "vis_stage": vis_stage}
     print(json.dumps(out), flush=True)
 
 

@@ -49,12 +49,14 @@ int ks_timing_end(ks_ctx* ctx, float* ms);
 /* Duration of the last launch of a pipeline stage on this context, bracketed by HIP events on its
  * stream; synchronizes. Stages: KS_STAGE_TRANSFER (the polarized transfer kernel of a render or of
  * the gradient's forward pass), KS_STAGE_ADJ_STEPS (the per-sample step adjoints), KS_STAGE_ADJ_ELEMENTS
- * (the per-splat element adjoints), KS_STAGE_MARCH (the stored-sample march). No reference
- * counterpart (measurement hook for bench.py). */
+ * (the per-splat element adjoints), KS_STAGE_MARCH (the stored-sample march), KS_STAGE_VIS (the
+ * visibility transform's matrix-core product, ks_visibilities). No reference counterpart
+ * (measurement hook for bench.py). */
 #define KS_STAGE_TRANSFER 0
 #define KS_STAGE_ADJ_STEPS 1
 #define KS_STAGE_ADJ_ELEMENTS 2
 #define KS_STAGE_MARCH 3
+#define KS_STAGE_VIS 4
 int ks_stage_ms(ks_ctx* ctx, int stage, float* ms);
 
 /* GeodesicCache(backend, camera, Val(N); store_samples) - cache.jl:106-125. */

@@ -344,12 +344,13 @@ def polarized_image(cache, params, t_obs, nu, L, nmax=-1, slab=0.0, kind="therma
     return c[:, :, 0, 0, :] if screen else c[0, 0]
 
 
-STAGES = {"transfer": 0, "adjoint_steps": 1, "adjoint_elements": 2, "march": 3}
+STAGES = {"transfer": 0, "adjoint_steps": 1, "adjoint_elements": 2, "march": 3, "vis": 4}
 
 
 def stage_ms(stage, device=0):
     """Duration [ms] of the last launch of a pipeline stage ("transfer", "adjoint_steps",
-    "adjoint_elements", "march") on the device's context, from HIP events on its stream."""
+    "adjoint_elements", "march", "vis": the visibility transform's matrix-core product) on the device's
+    context, from HIP events on its stream."""
     ms = ctypes.c_float()
     _check(_load().ks_stage_ms(context(device), _i(STAGES[stage]), ctypes.byref(ms)))
     return ms.value

@@ -7,7 +7,8 @@
 // The inner sum over i is a real matrix product with the phase table stacked as [cos; sin]
 // (2 nuv x n_alpha) against the image as an n_alpha x 4 n_beta matrix; the adjoint seed
 // Re sum_k w_k conj(Ax By) is the transposed product of the same table with the weights rotated by
-// conj(By). Both products run through one FP64 tiled GEMM; everything else is O(nuv (n_alpha + n_beta)).
+// conj(By). Both products run through one FP64 GEMM on the matrix cores (v_mfma_f64_16x16x4f64);
+// everything else is O(nuv (n_alpha + n_beta)).
 #include "ks_kernels.hpp"
 
 namespace ksd {
@@ -49,20 +50,24 @@ __global__ void ks_vis_gather(const double* __restrict__ out, const int* __restr
     for (int q = 0; q < 4; ++q) dst[q] = src[q];
 }
 
-// ---- FP64 tiled GEMM: C[M x N] = op(A) B, row-major; A is M x K, or K x M with TA; batched over blockIdx.z ---------
-// 64 x 64 tile per 256-thread block, K in steps of 16 through LDS, 4 x 4 outputs per thread (rows ty + 16 r,
-// columns tx + 16 c, so that a wavefront's LDS reads fall in distinct banks).
+// ---- FP64 GEMM on the matrix cores: C[M x N] = op(A) B, row-major; A is M x K, or K x M with TA; batched over z ----
+// 64 x 64 tile per 256-thread block, K in steps of 16 through LDS. Wavefront w owns rows 16 w .. 16 w + 15
+// and the four 16 x 16 output tiles along N, each a v_mfma_f64_16x16x4f64 accumulator; per step of 4 in K
+// lane l supplies A[16 w + l % 16][k + l / 16] and B[k + l / 16][16 t + l % 16], and holds
+// D[4 r + l / 16][l % 16] in its register r (the instruction's lane layout, checked on the device).
+typedef double dbl4 __attribute__((ext_vector_type(4)));
 constexpr int GT = 64, GK = 16;
 template <bool TA>
 __global__ void __launch_bounds__(256) ks_dgemm(int M, int N, int K, const double* __restrict__ A, int64_t lda, int64_t sA,
                                                 const double* __restrict__ B, int64_t ldb, int64_t sB, double* __restrict__ C, int64_t ldc,
                                                 int64_t sC) {
     __shared__ double As[GK][GT + 1];
-    __shared__ double Bs[GK][GT];
+    __shared__ double Bs[GK][GT + 1];
     A += sA * blockIdx.z; B += sB * blockIdx.z; C += sC * blockIdx.z;
-    const int tid = threadIdx.x, tx = tid % 16, ty = tid / 16;
+    const int tid = threadIdx.x, w = tid / 64, l = tid % 64;
     const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
-    double acc[4][4] = {};
+    dbl4 acc[4];
+    for (int t = 0; t < 4; ++t) acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
     for (int k0 = 0; k0 < K; k0 += GK) {
         for (int e = tid; e < GT * GK; e += 256) {
             int mm, kk;
@@ -78,23 +83,30 @@ __global__ void __launch_bounds__(256) ks_dgemm(int M, int N, int K, const doubl
         }
         __syncthreads();
 #pragma unroll
-        for (int kk = 0; kk < GK; ++kk) {
-            double a[4], b[4];
-            for (int r = 0; r < 4; ++r) a[r] = As[kk][ty + 16 * r];
-            for (int c = 0; c < 4; ++c) b[c] = Bs[kk][tx + 16 * c];
-            for (int r = 0; r < 4; ++r)
-                for (int c = 0; c < 4; ++c) acc[r][c] = fma(a[r], b[c], acc[r][c]);
+        for (int k4 = 0; k4 < GK; k4 += 4) {
+            const double a = As[k4 + l / 16][16 * w + l % 16];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Bs[k4 + l / 16][16 * t + l % 16], acc[t], 0, 0, 0);
         }
         __syncthreads();
     }
-    for (int r = 0; r < 4; ++r) {
-        const int gm = m0 + ty + 16 * r;
-        if (gm >= M) continue;
-        for (int c = 0; c < 4; ++c) {
-            const int gn = n0 + tx + 16 * c;
-            if (gn < N) C[(int64_t)gm * ldc + gn] = acc[r][c];
+    for (int t = 0; t < 4; ++t) {
+        const int gn = n0 + 16 * t + l % 16;
+        if (gn >= N) continue;
+        for (int r = 0; r < 4; ++r) {
+            const int gm = m0 + 16 * w + 4 * r + l / 16;
+            if (gm < M) C[(int64_t)gm * ldc + gn] = acc[t][r];
         }
     }
+}
+
+// C[i] = sum over the nsplit partial products P[z][i] in order (the split-K GEMM's fixed-order reduction)
+__global__ void ks_sum_partials(const double* __restrict__ P, int nsplit, int64_t n, double* __restrict__ C) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int z = 0; z < nsplit; ++z) s += P[(int64_t)z * n + i];
+    C[i] = s;
 }
 
 // ---- V[f][k][q] = tap_k scale sum_j (Cr + i Ci)[k][4 j + q] (Br + i Bi)[k][j]; V as (re, im) pairs ------------------
@@ -223,6 +235,19 @@ hipError_t launch_dgemm(hipStream_t s, bool transA, int M, int N, int K, const d
     const dim3 grid((N + GT - 1) / GT, (M + GT - 1) / GT, batch);
     if (transA) hipLaunchKernelGGL(ks_dgemm<true>, grid, dim3(256), 0, s, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC);
     else hipLaunchKernelGGL(ks_dgemm<false>, grid, dim3(256), 0, s, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC);
+    return hipGetLastError();
+}
+// C = A^T B with A stored K x M, K cut into nsplit equal ranges (nsplit halved until it divides K) whose
+// products land in P (nsplit x M x N) in one launch and are summed in order into C: the split gives a
+// short-M, long-K product enough blocks to fill the chip
+hipError_t launch_dgemm_tn_splitk(hipStream_t s, int M, int N, int K, const double* A, int64_t lda, const double* B, int64_t ldb,
+                                  double* C, int nsplit, double* P) {
+    while (nsplit > 1 && K % nsplit) nsplit /= 2;
+    const int kc = K / nsplit;
+    const dim3 grid((N + GT - 1) / GT, (M + GT - 1) / GT, nsplit);
+    hipLaunchKernelGGL(ks_dgemm<true>, grid, dim3(256), 0, s, M, N, kc, A, lda, (int64_t)kc * lda, B, ldb, (int64_t)kc * ldb, P, (int64_t)N,
+                       (int64_t)M * N);
+    hipLaunchKernelGGL(ks_sum_partials, dim3(grid_for((int64_t)M * N, 256)), dim3(256), 0, s, P, nsplit, (int64_t)M * N, C);
     return hipGetLastError();
 }
 hipError_t launch_vis_combine(hipStream_t s, int nuv, int nb, int nf, const double* C2, const double* B2, const double* tap, double scale, double* V) {

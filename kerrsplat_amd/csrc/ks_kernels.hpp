@@ -126,6 +126,9 @@ hipError_t launch_vis_tables(hipStream_t s, int nuv, int na, int nb, double psiz
 hipError_t launch_vis_gather(hipStream_t s, const double* out, const int* perm, int64_t npix, int na, int nb, int64_t f0, int nf, double* Im);
 hipError_t launch_dgemm(hipStream_t s, bool transA, int M, int N, int K, const double* A, int64_t lda, int64_t sA, const double* B,
                         int64_t ldb, int64_t sB, double* C, int64_t ldc, int64_t sC, int batch);
+hipError_t launch_dgemm_tn_splitk(hipStream_t s, int M, int N, int K, const double* A, int64_t lda, const double* B, int64_t ldb,
+                                  double* C, int nsplit, double* P);
+constexpr int VIS_SPLITK = 8;      // K ranges of the seed product (n_alpha x 4 n_beta from 2 nuv)
 hipError_t launch_vis_combine(hipStream_t s, int nuv, int nb, int nf, const double* C2, const double* B2, const double* tap, double scale, double* V);
 hipError_t launch_vis_residual(hipStream_t s, int nuv, const double* V, const double* vis, const double* sigma, const double* gains,
                                const double* tap, double scale, double* wq, double* chi_k);

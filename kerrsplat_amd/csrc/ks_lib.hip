@@ -59,7 +59,7 @@ struct ks_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    static constexpr int NSTAGE = 4;
+    static constexpr int NSTAGE = 5;
     hipEvent_t evs[NSTAGE][2] = {};                // around the last launch of each stage
     bool staged[NSTAGE] = {};
     hipError_t stage_begin(int k) { return hipEventRecord(evs[k][0], stream); }
@@ -94,7 +94,7 @@ struct ks_cache {
     // A2 (2 nuv x n_alpha) and B2 (2 nuv x n_beta), the images as n_alpha x 4 n_beta matrices, the
     // partial products C2 (2 nuv x 4 n_beta per frame), V, the rotated weights E2, the screen seed,
     // per-baseline or per-pixel χ² terms and their sum, the image data, noise and mask
-    Buf<double> vu, vv, vt, vg, vd, vs, vA2, vB2, vIm, vC2, vV, vwq, vE2, vSd, vchi, vsum, idata, isig;
+    Buf<double> vu, vv, vt, vg, vd, vs, vA2, vB2, vIm, vC2, vV, vwq, vE2, vSd, vSp, vchi, vsum, idata, isig;
     Buf<uint8_t> imask;
     double* seg = nullptr; int64_t seg_cap = 0;        // per-segment transfer maps
     double* dtn = nullptr; int64_t dtn_cap = 0;        // observation times then frequencies (device)
@@ -223,7 +223,7 @@ int ks_cache_destroy(ks_cache* c) {
     dfree(c->sx); dfree(c->sy); dfree(c->sz); dfree(c->thin_part);
     dfree(c->out); dfree(c->out_screen); dfree(c->img1);
     for (Buf<double>* b : {&c->vu, &c->vv, &c->vt, &c->vg, &c->vd, &c->vs, &c->vA2, &c->vB2, &c->vIm, &c->vC2, &c->vV, &c->vwq, &c->vE2,
-                           &c->vSd, &c->vchi, &c->vsum, &c->idata, &c->isig})
+                           &c->vSd, &c->vSp, &c->vchi, &c->vsum, &c->idata, &c->isig})
         b->release();
     c->imask.release();
     dfree(c->seg); dfree(c->segres); dfree(c->kstop); dfree(c->dtn);
@@ -791,7 +791,9 @@ static int vis_forward(ks_cache* c, int64_t f0, int nf, int nuv, const double* d
     const int64_t mat = (int64_t)na * 4 * nb, cmat = (int64_t)2 * nuv * 4 * nb;
     if (c->vIm.grow((size_t)mat * nf) || c->vC2.grow((size_t)cmat * nf) || c->vV.grow((size_t)8 * nuv * nf)) return KS_ERR_HIP;
     HIPCHK(launch_vis_gather(x->stream, c->out, c->perm, c->npix, na, nb, f0, nf, c->vIm.p));
+    HIPCHK(x->stage_begin(KS_STAGE_VIS));
     HIPCHK(launch_dgemm(x->stream, false, 2 * nuv, 4 * nb, na, c->vA2.p, na, 0, c->vIm.p, 4 * nb, mat, c->vC2.p, 4 * nb, cmat, nf));
+    HIPCHK(x->stage_end(KS_STAGE_VIS));
     HIPCHK(launch_vis_combine(x->stream, nuv, nb, nf, c->vC2.p, c->vB2.p, d_tap, scale, c->vV.p));
     return KS_OK;
 }
@@ -837,7 +839,8 @@ int ks_vis_chi2_seed(ks_cache* c, int64_t frame, double psize, const double* u, 
     const int na = c->nalpha, nb = c->nbeta;
     if (int rc = vis_setup(c, psize, u, v, taper, nuv)) return rc;
     if (c->vg.grow((size_t)2 * nuv) || c->vd.grow((size_t)8 * nuv) || c->vs.grow((size_t)4 * nuv) || c->vwq.grow((size_t)8 * nuv) ||
-        c->vE2.grow((size_t)2 * nuv * 4 * nb) || c->vSd.grow((size_t)na * 4 * nb) || c->vchi.grow(nuv) || c->vsum.grow(1))
+        c->vE2.grow((size_t)2 * nuv * 4 * nb) || c->vSd.grow((size_t)na * 4 * nb) || c->vSp.grow((size_t)VIS_SPLITK * na * 4 * nb) ||
+        c->vchi.grow(nuv) || c->vsum.grow(1))
         return KS_ERR_HIP;
     if (gains) HIPCHK(hipMemcpyAsync(c->vg.p, gains, (size_t)2 * nuv * sizeof(double), hipMemcpyHostToDevice, x->stream));
     HIPCHK(hipMemcpyAsync(c->vd.p, vis, (size_t)8 * nuv * sizeof(double), hipMemcpyHostToDevice, x->stream));
@@ -847,7 +850,7 @@ int ks_vis_chi2_seed(ks_cache* c, int64_t frame, double psize, const double* u, 
     if (int rc = vis_forward(c, frame, 1, nuv, d_tap, scale)) return rc;
     HIPCHK(launch_vis_residual(x->stream, nuv, c->vV.p, c->vd.p, c->vs.p, gains ? c->vg.p : nullptr, d_tap, scale, c->vwq.p, c->vchi.p));
     HIPCHK(launch_vis_rotate(x->stream, nuv, nb, c->vwq.p, c->vB2.p, c->vE2.p));
-    HIPCHK(launch_dgemm(x->stream, true, na, 4 * nb, 2 * nuv, c->vA2.p, na, 0, c->vE2.p, 4 * nb, 0, c->vSd.p, 4 * nb, 0, 1));
+    HIPCHK(launch_dgemm_tn_splitk(x->stream, na, 4 * nb, 2 * nuv, c->vA2.p, na, c->vE2.p, 4 * nb, c->vSd.p, VIS_SPLITK, c->vSp.p));
     HIPCHK(launch_vis_scatter_seed(x->stream, c->vSd.p, c->perm, c->npix, na, nb, frame, c->ads));
     HIPCHK(launch_sum_fixed(x->stream, c->vchi.p, nuv, c->vsum.p));
     HIPCHK(hipMemcpyAsync(chi2, c->vsum.p, sizeof(double), hipMemcpyDeviceToHost, x->stream));
