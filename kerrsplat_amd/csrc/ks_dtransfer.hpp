@@ -65,6 +65,29 @@ KS_DEV bool outside_support(const SplatCT<S>& s, S t, S x, S y, S z, S& sp, S& c
     S dx = x - cx, dy = y - cy, dz = z - s.z;
     return dx * dx + dy * dy + dz * dz > s.rsup2;
 }
+// Whether a splat can reach a region of samples (the per-tile cull): the region is a sphere (centre p,
+// radius R) around its sample positions, with emission times in [tmin, tmax]. Over those times the
+// splat's centre sweeps an arc of the circle about the spin axis; every point of an arc of half-angle h
+// is within 2 rho sin(min(h, pi) / 2) of the arc's midpoint. False only when no sample of the region
+// can be inside the support sphere (outside_support), with a margin for rounding.
+template <class S>
+KS_DEV bool splat_may_reach(const SplatCT<S>& c, double px, double py, double pz, double R, double tmin, double tmax) {
+    const double x = val(c.x), y = val(c.y), z = val(c.z), w = val(c.omega);
+    const double tm = 0.5 * (tmin + tmax), h = 0.5 * fabs(w) * (tmax - tmin);
+    double sp, cp;
+    sincos(w * (tm - val(c.t0)), &sp, &cp);
+    const double cx = x * cp - y * sp, cy = x * sp + y * cp;
+    const double arc = 2.0 * sqrt(x * x + y * y) * sin(0.5 * dmin(h, PI));
+    const double dx = px - cx, dy = py - cy, dz = pz - z;
+    const double d = sqrt(dx * dx + dy * dy + dz * dz);
+    const double rho = sqrt(x * x + y * y);
+    const double reach = R + sqrt(val(c.rsup2)) + arc;
+    // margin: the per-sample test in FP32 rounds the emission time (|t| 6e-8, turned by omega into a phase
+    // error) and the positions; 1e-4 relative covers the rest
+    const double margin = 1e-4 * (reach + d) + 4e-7 * rho * fabs(w) * dmax(fabs(tmin), fabs(tmax)) + 1e-6;
+    return d <= reach + margin;
+}
+
 // the Gaussian exponent q^2 + tau^2 given the rotation phase (Splats.jl:58-93)
 template <class S>
 KS_DEV S gauss_exponent(const SplatCT<S>& s, S t, S x, S y, S z, S sp, S cp) {

@@ -128,6 +128,35 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(Waves<
     }
 }
 
+// The region of each element-adjoint unit (64 items x ADJ_SLICE samples): the bounding sphere of its
+// step-taking samples and their emission times, ub[UB_FIELDS unit + f] (wave_region); one wavefront per unit.
+__global__ void __launch_bounds__(64) ks_unit_bounds(int64_t npix, int N, Samples smp, const double* __restrict__ times, int nt,
+                                                     int64_t nitems, const uint8_t* __restrict__ aflg, double* __restrict__ ub) {
+    const int64_t unit = blockIdx.x;
+    const int nslice = (N + ADJ_SLICE - 1) / ADJ_SLICE;
+    const int64_t ib = unit / nslice;
+    const int kfirst = (int)(unit % nslice) * ADJ_SLICE + 1;
+    const int klast = kfirst + ADJ_SLICE - 1 < N ? kfirst + ADJ_SLICE - 1 : N;
+    const int64_t item = ib * 64 + threadIdx.x;
+    double lo[4] = {INFINITY, INFINITY, INFINITY, INFINITY}, hi[4] = {-INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    if (item < nitems) {
+        const int64_t j = item % npix;
+        const double t_obs = times[(item / npix) % nt];
+        for (int k = kfirst; k <= klast; ++k) {
+            if (!aflg[(int64_t)(k - 1) * nitems + item]) continue;
+            const int64_t so = (int64_t)(k - 1) * npix + j;
+            const double v[4] = {smp.x[so], smp.y[so], smp.z[so], t_obs - smp.t[so]};
+            for (int q = 0; q < 4; ++q) { lo[q] = fmin(lo[q], v[q]); hi[q] = fmax(hi[q], v[q]); }
+        }
+    }
+    double px = 0.0, py = 0.0, pz = 0.0, R = 0.0, tmin = 0.0, tmax = 0.0;
+    const bool any = wave_region(lo, hi, px, py, pz, R, tmin, tmax);
+    if (threadIdx.x == 0) {
+        double* o = ub + UB_FIELDS * unit;
+        o[0] = px; o[1] = py; o[2] = pz; o[3] = R; o[4] = tmin; o[5] = tmax; o[6] = any ? 1.0 : 0.0; o[7] = 0.0;
+    }
+}
+
 // Per (64 consecutive items, slice of ADJ_SLICE samples, splat): the element adjoints of every
 // (item, sample) pair inside the splat's weight cutoff, 21 partial sums per lane, summed over the
 // wavefront; part[(u ns + i) 21 + r] for the unit u = ib nslice + slice. The slices bound the work of
@@ -143,7 +172,8 @@ template <class S>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(Waves<S>::adjelem))) ks_adj_splats(double a, double thetao, const double* __restrict__ soa, int64_t npix, int N, Samples smp,
                                                     const SplatCT<S>* __restrict__ sp, const SplatAdjT<S>* __restrict__ sadj, int ns,
                                                     const double* __restrict__ times, int nt, const double* __restrict__ nus, int64_t nitems,
-                                                    const S* __restrict__ rec, const uint8_t* __restrict__ aflg, int64_t nunits, double* __restrict__ part) {
+                                                    const S* __restrict__ rec, const uint8_t* __restrict__ aflg, int64_t nunits,
+                                                    const double* __restrict__ ub, double* __restrict__ part) {
     constexpr int WALK = 8;                 // samples tested per group, their loads issued together
     constexpr int QCAP = 1024;              // ring capacity: < 64 pending + 64 per sample of a group
     static_assert(QCAP >= 64 * (WALK + 1) && (QCAP & (QCAP - 1)) == 0, "the ring must hold a group's pushes");
@@ -159,6 +189,14 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(Waves<S
         i = (int)(q % ns);
         unit = (q / ns) * 8 + x;
         if (unit >= nunits) return;       // the padding of the last group of 8 units
+    }
+    // the per-unit cull: a splat that cannot reach any step-taking sample of the unit adds zeros
+    {
+        const double* u = ub + UB_FIELDS * unit;
+        if (!(u[6] > 0.0) || !splat_may_reach(sp[i], u[0], u[1], u[2], u[3], u[4], u[5])) {
+            if (threadIdx.x < ADJ_ROWS) part[(unit * ns + i) * ADJ_ROWS + threadIdx.x] = 0.0;
+            return;
+        }
     }
     const int nslice = (N + ADJ_SLICE - 1) / ADJ_SLICE;
     const int64_t ib = unit / nslice;
@@ -297,18 +335,23 @@ hipError_t launch_adj_segment_f32(hipStream_t s, int N, int64_t nitems, const do
 }
 hipError_t launch_adj_splats(hipStream_t s, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
                              const SplatC* sp, const SplatAdj* sadj, int ns, const double* times, int nt, const double* nus,
-                             int64_t nitems, const double* rec, const uint8_t* aflg, double* part) {
+                             int64_t nitems, const double* rec, const uint8_t* aflg, const double* ub, double* part) {
     const int64_t nunits = adj_units(nitems, N);
     hipLaunchKernelGGL(ks_adj_splats<double>, dim3((unsigned)((nunits + 7) / 8 * 8 * ns)), dim3(64), 0, s, a, thetao, soa, npix, N, smp, sp, sadj, ns, times,
-                       nt, nus, nitems, rec, aflg, nunits, part);
+                       nt, nus, nitems, rec, aflg, nunits, ub, part);
     return hipGetLastError();
 }
 hipError_t launch_adj_splats_f32(hipStream_t s, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
                                  const SplatCf* sp, const SplatAdjf* sadj, int ns, const double* times, int nt, const double* nus,
-                                 int64_t nitems, const f32* rec, const uint8_t* aflg, double* part) {
+                                 int64_t nitems, const f32* rec, const uint8_t* aflg, const double* ub, double* part) {
     const int64_t nunits = adj_units(nitems, N);
     hipLaunchKernelGGL(ks_adj_splats<f32>, dim3((unsigned)((nunits + 7) / 8 * 8 * ns)), dim3(64), 0, s, a, thetao, soa, npix, N, smp, sp, sadj, ns, times,
-                       nt, nus, nitems, rec, aflg, nunits, part);
+                       nt, nus, nitems, rec, aflg, nunits, ub, part);
+    return hipGetLastError();
+}
+hipError_t launch_unit_bounds(hipStream_t s, int64_t npix, int N, Samples smp, const double* times, int nt, int64_t nitems,
+                              const uint8_t* aflg, double* ub) {
+    hipLaunchKernelGGL(ks_unit_bounds, dim3((unsigned)adj_units(nitems, N)), dim3(64), 0, s, npix, N, smp, times, nt, nitems, aflg, ub);
     return hipGetLastError();
 }
 hipError_t launch_adj_reduce(hipStream_t s, const double* part, int64_t nunits, int ns, double* dparams) {

@@ -30,15 +30,19 @@ KS_DEV TransCtxT<S> make_ctx(double a, double rp, double rm, double thetao_sin, 
 
 // The splat sum of one sample (accumulate_elements, transport.jl:254-276): invariant j, alpha, rho in
 // screen axes; false when no splat contributes. One instantiation per population and scalar.
+// With a candidate list (ascending splat indices, the per-tile cull) only those splats are visited; the
+// others lie outside their support at every sample of the tile and would add exact zeros.
 template <int KIND, class S>
 KS_DEV bool sample_coefficients(const TransCtxT<S>& c, double st, double r, double th, bool nur, bool nuth, S x, S y, S z,
-                                S j4[4], S a4[4], S r3[3]) {
+                                S j4[4], S a4[4], S r3[3], const uint16_t* cand = nullptr, int ncand = 0) {
     const S t = S(c.t_obs - st);
     for (int i = 0; i < 4; ++i) { j4[i] = S(0.0); a4[i] = S(0.0); }
     r3[0] = r3[1] = r3[2] = S(0.0);
     bool active = false, geo_ready = false;
     GeoT<S> G;
-    for (int i = 0; i < c.ns; ++i) {
+    const int nloop = cand ? ncand : c.ns;
+    for (int ci = 0; ci < nloop; ++ci) {
+        const int i = cand ? (int)cand[ci] : ci;
         const SplatCT<S>& sc = c.sp[i];
         S gexp;
         if constexpr (KIND == MODEL_KNOT) {
@@ -212,7 +216,7 @@ KS_DEV double path_weight(double L, double nu, double a, double r, double th, do
 // adjoint records (RecField) of every step-taking sample are written, and aflg[(k - 1) nitems + item]
 // = 1 when the sample takes a step.
 
-template <int KIND, class S>
+template <int KIND, class S, bool CULL>
 __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(Waves<S>::transfer))) ks_chunk_transfer(double a, double thetao, const double* __restrict__ soa, int64_t npix, int N,
                                                          Samples smp, const SplatCT<S>* __restrict__ sp, int ns,
                                                          const S* __restrict__ knots, const S* __restrict__ ktimes, int nknots,
@@ -221,9 +225,10 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(Waves<
                                                          double* __restrict__ seg, S* __restrict__ rec, uint8_t* __restrict__ aflg) {
     const int64_t nitems = npix * nt * nnu;
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (tid >= nitems * nch) return;
-    const int64_t item = slot_item(smp, tid % nitems, npix);     // wavefronts over compact patches of sky
-    const int m = (int)(tid / nitems);
+    const bool live = tid < nitems * nch;       // the tail's lanes take slot 0 and write nothing
+    const int64_t slot = live ? tid : 0;
+    const int64_t item = slot_item(smp, slot % nitems, npix);     // wavefronts over compact patches of sky
+    const int m = (int)(slot / nitems);
     const int64_t j = item % npix;
     const int64_t fr = item / npix;
     const int it = (int)(fr % nt), inu = (int)(fr / nt);
@@ -236,6 +241,34 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(Waves<
     const double hor = Sp.rp * (1.0 + 1e-3);
     const int kbeg = m * CHUNK + 1, kend = kbeg + CHUNK - 1 < N ? kbeg + CHUNK - 1 : N;
     const int64_t st = (int64_t)N * nitems;
+    // the per-tile cull: the splats that can reach the wavefront's samples of this chunk, in ascending
+    // order, as a candidate list in LDS (the rest add exact zeros to every sample of the tile)
+    constexpr bool cull = CULL && KIND != MODEL_KNOT;     // the launcher picks CULL for CULL_MIN_NS <= ns <= CAND_MAX
+    __shared__ uint16_t s_cand[2][cull ? CAND_MAX : 1];
+    const int wv = (int)(threadIdx.x / 64), lane = (int)(threadIdx.x % 64);
+    int ncand = 0;
+    if constexpr (cull) {
+        double lo[4] = {INFINITY, INFINITY, INFINITY, INFINITY}, hi[4] = {-INFINITY, -INFINITY, -INFINITY, -INFINITY};
+        if (live)
+            for (int k = kbeg; k <= kend; ++k) {
+                const int64_t o = (int64_t)(k - 1) * npix + j;
+                if ((smp.fl[o] & 1) && smp.r[o] > hor && k < ks) {
+                    const double v[4] = {smp.x[o], smp.y[o], smp.z[o], c.t_obs - smp.t[o]};
+                    for (int q = 0; q < 4; ++q) { lo[q] = fmin(lo[q], v[q]); hi[q] = fmax(hi[q], v[q]); }
+                }
+            }
+        double px = 0.0, py = 0.0, pz = 0.0, R = 0.0, tmin = 0.0, tmax = 0.0;
+        const bool any = wave_region(lo, hi, px, py, pz, R, tmin, tmax);
+        for (int i0 = 0; i0 < ns; i0 += 64) {
+            const int i = i0 + lane;
+            const bool hit = any && i < ns && splat_may_reach(sp[i], px, py, pz, R, tmin, tmax);
+            const uint64_t mk = __ballot(hit);
+            if (hit) s_cand[wv][ncand + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk, 0u))] = (uint16_t)i;
+            ncand += __popcll(mk);
+        }
+    }
+    if constexpr (cull) __syncthreads();
+    if (!live) return;
     SegMapT<S> sm;
     sm.init();
     for (int k = kbeg; k <= kend; ++k) {
@@ -247,7 +280,7 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(Waves<
             const double th = smp.th[o];
             S j4[4], a4[4], r3[3];
             if (sample_coefficients<KIND, S>(c, smp.t[o], r, th, (fl & 2) != 0, (fl & 4) != 0, S(smp.x[o]), S(smp.y[o]), S(smp.z[o]),
-                                             j4, a4, r3)) {
+                                             j4, a4, r3, cull ? s_cand[wv] : nullptr, ncand)) {
                 const S D = S(path_weight(L, nus[inu], a, r, th, dtau));
                 map_step(sm, j4, a4, r3, D);
                 step = true;
@@ -408,8 +441,12 @@ static void chunk_transfer_k(hipStream_t s, double a, double thetao, const doubl
                              const double* nus, int nnu, double L, const int* kstop, double* seg, S* rec, uint8_t* aflg) {
     const int nch = (N + CHUNK - 1) / CHUNK;
     const int64_t items = npix * nt * nnu;
-    hipLaunchKernelGGL((ks_chunk_transfer<KIND, S>), dim3(grid_for(items * nch, 128)), dim3(128), 0, s, a, thetao, soa, npix, N, smp, sp, ns,
-                       knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, nch, seg, rec, aflg);
+    if (ns >= CULL_MIN_NS && ns <= CAND_MAX)
+        hipLaunchKernelGGL((ks_chunk_transfer<KIND, S, true>), dim3(grid_for(items * nch, 128)), dim3(128), 0, s, a, thetao, soa, npix, N, smp, sp,
+                           ns, knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, nch, seg, rec, aflg);
+    else
+        hipLaunchKernelGGL((ks_chunk_transfer<KIND, S, false>), dim3(grid_for(items * nch, 128)), dim3(128), 0, s, a, thetao, soa, npix, N, smp, sp,
+                           ns, knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, nch, seg, rec, aflg);
 }
 template <class S>
 static void chunk_transfer_s(hipStream_t s, int kind, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,

@@ -22,6 +22,23 @@ constexpr int CHUNK = 32;          // samples per work item of the transfer from
 constexpr int SEG_FIELDS = 20;     // a segment map: P (16, row-major), then Sv (4)
 constexpr int ADJ_ROWS = 21;       // parameter rows of a thermal splat
 constexpr int ADJ_SLICE = 128;     // samples per work unit of the element adjoints
+constexpr int CAND_MAX = 1024;     // splats a tile's candidate list holds (more splats: no per-tile cull)
+constexpr int CULL_MIN_NS = 64;    // the transfer culls per tile from this many splats on (below, the per-sample test is cheaper)
+constexpr int UB_FIELDS = 8;       // a unit's region: centre (3), radius, emission times [min, max], any sample, pad
+// the minimum and maximum over a wavefront (butterfly: every lane ends with the wavefront's value)
+KS_DEV double wave_min(double v) { for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64)); return v; }
+KS_DEV double wave_max(double v) { for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64)); return v; }
+// A region's bounding sphere from per-lane boxes lo/hi (x, y, z) and emission times (index 3), reduced over
+// the wavefront; false when no lane saw a sample.
+KS_DEV bool wave_region(double lo[4], double hi[4], double& px, double& py, double& pz, double& R, double& tmin, double& tmax) {
+    for (int q = 0; q < 4; ++q) { lo[q] = wave_min(lo[q]); hi[q] = wave_max(hi[q]); }
+    if (!(hi[0] >= lo[0])) return false;
+    px = 0.5 * (lo[0] + hi[0]); py = 0.5 * (lo[1] + hi[1]); pz = 0.5 * (lo[2] + hi[2]);
+    const double ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
+    R = 0.5 * sqrt(ex * ex + ey * ey + ez * ez);
+    tmin = lo[3]; tmax = hi[3];
+    return true;
+}
 // element-adjoint work units (64 items x ADJ_SLICE samples) of nitems items of N samples
 inline int64_t adj_units(int64_t nitems, int N) { return ((nitems + 63) / 64) * ((N + ADJ_SLICE - 1) / ADJ_SLICE); }
 // Minimum waves per SIMD requested for the heavy kernels (the register budget the compiler may use:
@@ -115,11 +132,13 @@ hipError_t launch_adj_segment_f32(hipStream_t s, int N, int64_t nitems, const do
                                   Samples smp, int64_t npix);
 hipError_t launch_adj_splats_f32(hipStream_t s, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
                                  const SplatCf* sp, const SplatAdjf* sadj, int ns, const double* times, int nt, const double* nus,
-                                 int64_t nitems, const f32* rec, const uint8_t* aflg, double* part);
+                                 int64_t nitems, const f32* rec, const uint8_t* aflg, const double* ub, double* part);
 hipError_t launch_adj_splats(hipStream_t s, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
                              const SplatC* sp, const SplatAdj* sadj, int ns, const double* times, int nt, const double* nus,
-                             int64_t nitems, const double* rec, const uint8_t* aflg, double* part);
+                             int64_t nitems, const double* rec, const uint8_t* aflg, const double* ub, double* part);
 hipError_t launch_adj_reduce(hipStream_t s, const double* part, int64_t nunits, int ns, double* dparams);
+hipError_t launch_unit_bounds(hipStream_t s, int64_t npix, int N, Samples smp, const double* times, int nt, int64_t nitems,
+                              const uint8_t* aflg, double* ub);
 
 // ---- ks_k_vis.hip ----
 hipError_t launch_vis_tables(hipStream_t s, int nuv, int na, int nb, double psize, const double* u, const double* v, double* A2, double* B2);

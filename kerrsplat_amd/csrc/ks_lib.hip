@@ -103,8 +103,8 @@ struct ks_cache {
     int* kstop = nullptr;                              // half-orbit cutoff per pixel
     // adjoint workspace (grow-only): records, flags, tails, per-segment (w_in, T_out), seeds, partial sums
     double* arec = nullptr; uint8_t* aflg = nullptr; double* atail = nullptr; double* awT = nullptr; double* ads = nullptr;
-    double* apart = nullptr; double* adpar = nullptr;
-    int64_t a_items = 0, a_part = 0, a_dpar = 0;
+    double* apart = nullptr; double* adpar = nullptr; double* aub = nullptr;
+    int64_t a_items = 0, a_part = 0, a_dpar = 0, a_ub = 0;
     // the last forward pass of the dual sweep (ks_gradient_forward): what ks_gradient_reverse needs
     const ks_model* fw_model = nullptr;
     int fw_nt = 0, fw_nnu = 0, fw_generation = -1, fw_precision = -1;
@@ -227,7 +227,7 @@ int ks_cache_destroy(ks_cache* c) {
         b->release();
     c->imask.release();
     dfree(c->seg); dfree(c->segres); dfree(c->kstop); dfree(c->dtn);
-    dfree(c->arec); dfree(c->aflg); dfree(c->atail); dfree(c->awT); dfree(c->ads); dfree(c->apart); dfree(c->adpar);
+    dfree(c->arec); dfree(c->aflg); dfree(c->atail); dfree(c->awT); dfree(c->ads); dfree(c->apart); dfree(c->adpar); dfree(c->aub);
     delete c;
     return KS_OK;
 }
@@ -633,6 +633,11 @@ static int ensure_adj(ks_cache* c, int64_t items, int ns) {
         if (dalloc(&c->apart, (size_t)nunits * ns * ADJ_ROWS)) return KS_ERR_HIP;
         c->a_part = nunits * ns * ADJ_ROWS;
     }
+    if (c->a_ub < nunits) {
+        dfree(c->aub);
+        if (dalloc(&c->aub, (size_t)nunits * UB_FIELDS)) return KS_ERR_HIP;
+        c->a_ub = nunits;
+    }
     if (c->a_dpar < (int64_t)ns * ADJ_ROWS) {
         dfree(c->adpar);
         if (dalloc(&c->adpar, (size_t)ns * ADJ_ROWS)) return KS_ERR_HIP;
@@ -705,12 +710,13 @@ int ks_gradient_reverse(ks_cache* c, const ks_model* m, const double* dstokes, i
     HIPCHK(x->stage_end(KS_STAGE_ADJ_STEPS));
     if (ns > 0) {
         HIPCHK(x->stage_begin(KS_STAGE_ADJ_ELEMENTS));
+        HIPCHK(launch_unit_bounds(x->stream, c->npix, c->N, samples_of(c), d_tn, nt, items, c->aflg, c->aub));
         if (mixed)
             HIPCHK(launch_adj_splats_f32(x->stream, c->spin, c->thetao, c->soa, c->npix, c->N, samples_of(c), m->d_splats_f, m->d_adj_f, ns,
-                                         d_tn, nt, d_tn + nt, items, (const f32*)c->arec, c->aflg, c->apart));
+                                         d_tn, nt, d_tn + nt, items, (const f32*)c->arec, c->aflg, c->aub, c->apart));
         else
             HIPCHK(launch_adj_splats(x->stream, c->spin, c->thetao, c->soa, c->npix, c->N, samples_of(c), m->d_splats, m->d_adj, ns, d_tn, nt,
-                                     d_tn + nt, items, c->arec, c->aflg, c->apart));
+                                     d_tn + nt, items, c->arec, c->aflg, c->aub, c->apart));
         HIPCHK(x->stage_end(KS_STAGE_ADJ_ELEMENTS));
         HIPCHK(launch_adj_reduce(x->stream, c->apart, adj_units(items, c->N), ns, c->adpar));
     }
