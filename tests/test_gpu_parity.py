@@ -139,6 +139,27 @@ def test_polarized_knot_vs_oracle(oracle, path):
     _polarized_case(oracle, "knot", P, path, a=0.9, thdeg=163.0, fov=12.0, knots=knots, knot_times=times)
 
 
+def test_polarized_many_splats_tile_cull_vs_oracle(oracle):
+    """72 thermal splats: the transfer over stored samples takes the per-tile cull (candidate lists from 64
+    splats on) and the element adjoints the per-unit cull; both must leave the images and the gradient
+    at the FP64 bars, and the mixed path within its own."""
+    a = 0.5
+    P = syn.thermal_splats(1, a, 72)
+    _polarized_case(oracle, "thermal", P, "stored", res=12, N=128)
+    cam = camera(15.0, 12)
+    L = syn.length_unit()
+    cache = make_cache(cam, 128, a, math.radians(60.0), "stored")
+    ds = np.random.default_rng(31).normal(size=(12, 12, 1, 1, 4))
+    g, cube = K.polarized_gradient(cache, P, [0.0], [230e9], L, ds)
+    gref, _ = oracle.polarized_gradient(a, math.radians(60.0), cam.alphas, cam.betas, 128, P, 0.0, 230e9, L,
+                                        ds[:, :, 0, 0, :].reshape(cam.npix, 4, order="F"))
+    assert np.abs(g - gref).max() <= 1e-9 * np.abs(gref).max()
+    K.precision(cache, "mixed")
+    g32, cube32 = K.polarized_gradient(cache, P, [0.0], [230e9], L, ds)
+    assert np.abs(cube32 - cube).max() <= 1e-4 * np.abs(cube[..., 0]).max()
+    assert np.abs(g32 - g).max() <= 1e-3 * np.abs(g).max()
+
+
 @pytest.mark.parametrize("path", PATHS)
 def test_cube_frames_match_images(oracle, path):
     """One launch over (frame, frequency) items equals per-frame images (polarized.jl:187-198)."""
