@@ -159,6 +159,15 @@ int ks_visibilities(ks_cache* cache, int64_t frames, double psize, const double*
 int ks_vis_chi2_seed(ks_cache* cache, int64_t frame, double psize, const double* u, const double* v, const double* taper,
                      const double* gains, const double* vis, const double* sigma, int nuv, double* chi2, double* V);
 
+/* seed_kernel! (device_visibilities.jl:76-88) for any visibility-domain loss l(V): the adjoint transform
+ * Σ_k Re[w_k conj(∂V_k/∂I)] of w = ∂l/∂Re V + i ∂l/∂Im V (nuv x 4 complex as (re, im) pairs), written
+ * into the device seed of frame `frame` of the last ks_gradient_forward (added to it when add), for
+ * ks_gradient_reverse(dstokes = NULL): the closure-phase and log-closure-amplitude χ² of
+ * chi2_closures (visibilities.jl:141-199), which the reference differentiates with host Enzyme, get a
+ * device gradient this way. */
+int ks_vis_seed(ks_cache* cache, int64_t frame, double psize, const double* u, const double* v, const double* taper,
+                const double* w, int nuv, int add);
+
 /* _frame_seed! without a Binning (Fit.jl:537-577; the terms of chi2/_chi2_frames, Fit.jl:66-93):
  * the χ² of frame `frame` of the last ks_gradient_forward against an observed Stokes image,
  * Σ ((I - d) / σ)² over the masked pixels, returned in chi2, and its seed 2 r / σ (0 outside the

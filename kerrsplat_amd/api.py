@@ -80,6 +80,7 @@ def _load():
     lib.ks_gradient_reverse.argtypes = [_P, _P, _P, _i, _P]
     lib.ks_vis_chi2_seed.argtypes = [_P, _i64, _d, _P, _P, _P, _P, _P, _P, _i, _P, _P]
     lib.ks_image_chi2_seed.argtypes = [_P, _i64, _P, _P, _i, _P, _P]
+    lib.ks_vis_seed.argtypes = [_P, _i64, _d, _P, _P, _P, _P, _i, _i]
     lib.ks_march.argtypes = [_P]
     lib.ks_cache_precision.argtypes = [_P, _i]
     _lib = lib
@@ -499,3 +500,16 @@ def image_chi2_seed(cache, frame, data, sigma, mask=None):
     chi = np.zeros(1)
     _check(cache._lib.ks_image_chi2_seed(cache._h, _i64(frame), _ptr(flat(dd)), _ptr(sgf), _i(int(per_pixel)), _ptr(mk), _ptr(chi)))
     return float(chi[0])
+
+
+def vis_seed(cache, frame, psize, u, v, w, taper=None, add=False):
+    """seed_kernel! (device_visibilities.jl:76-88) for any visibility-domain loss: the adjoint transform of
+    w = dl/dRe V + i dl/dIm V (complex (nuv, 4)) into the device seed of frame `frame` of the last
+    forward pass (added to the existing seed with add=True), for gradient_reverse(dstokes=None)."""
+    uu, vv = _f64(u), _f64(v)
+    ww = np.asarray(w, dtype=np.complex128)
+    if ww.shape != (len(uu), 4):
+        raise DimensionMismatch(f"w must be ({len(uu)}, 4), got {ww.shape}")
+    wv = np.ascontiguousarray(np.stack([ww.real, ww.imag], axis=-1))
+    tp = _f64(taper) if taper is not None else None
+    _check(cache._lib.ks_vis_seed(cache._h, _i64(frame), _d(psize), _ptr(uu), _ptr(vv), _ptr(tp), _ptr(wv), _i(len(uu)), _i(int(add))))

@@ -170,16 +170,24 @@ __global__ void ks_vis_rotate(int nuv, int nb, const double* __restrict__ wq, co
     }
 }
 
-// seed (n_alpha x 4 n_beta, screen) -> the dual sweep's seed slot of frame f in sorted order
+// seed (n_alpha x 4 n_beta, screen) -> the dual sweep's seed slot of frame f in sorted order (added to it with add)
 __global__ void ks_vis_scatter_seed(const double* __restrict__ Sd, const int* __restrict__ perm, int64_t npix, int na, int nb, int64_t f,
-                                    double* __restrict__ ads) {
+                                    int add, double* __restrict__ ads) {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= npix) return;
     const int p = perm[s];
     const int i = p % na, j = p / na;
     const double* src = Sd + (int64_t)i * 4 * nb + 4 * j;
     double* dst = ads + 4 * (s + npix * f);
-    for (int q = 0; q < 4; ++q) dst[q] = src[q];
+    for (int q = 0; q < 4; ++q) dst[q] = add ? dst[q] + src[q] : src[q];
+}
+// weights w (nuv x 4 complex, the derivative of a loss with respect to the model visibilities) times
+// tap_k scale: the seed_kernel! input of an arbitrary visibility-domain loss
+__global__ void ks_vis_scale_weights(int nuv, const double* __restrict__ w, const double* __restrict__ tap, double scale, double* __restrict__ wq) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nuv) return;
+    const double t = (tap ? tap[k] : 1.0) * scale;
+    for (int q = 0; q < 8; ++q) wq[8 * (int64_t)k + q] = w[8 * (int64_t)k + q] * t;
 }
 
 // ---- the image χ² of one frame (_frame_seed!, unbinned): r = (I - d) / sigma over masked pixels, seed 2 r / sigma ---
@@ -263,8 +271,12 @@ hipError_t launch_vis_rotate(hipStream_t s, int nuv, int nb, const double* wq, c
     hipLaunchKernelGGL(ks_vis_rotate, dim3(grid_for((int64_t)nuv * nb, 256)), dim3(256), 0, s, nuv, nb, wq, B2, E2);
     return hipGetLastError();
 }
-hipError_t launch_vis_scatter_seed(hipStream_t s, const double* Sd, const int* perm, int64_t npix, int na, int nb, int64_t f, double* ads) {
-    hipLaunchKernelGGL(ks_vis_scatter_seed, dim3(grid_for(npix, 256)), dim3(256), 0, s, Sd, perm, npix, na, nb, f, ads);
+hipError_t launch_vis_scatter_seed(hipStream_t s, const double* Sd, const int* perm, int64_t npix, int na, int nb, int64_t f, int add, double* ads) {
+    hipLaunchKernelGGL(ks_vis_scatter_seed, dim3(grid_for(npix, 256)), dim3(256), 0, s, Sd, perm, npix, na, nb, f, add, ads);
+    return hipGetLastError();
+}
+hipError_t launch_vis_scale_weights(hipStream_t s, int nuv, const double* w, const double* tap, double scale, double* wq) {
+    hipLaunchKernelGGL(ks_vis_scale_weights, dim3(grid_for(nuv, 256)), dim3(256), 0, s, nuv, w, tap, scale, wq);
     return hipGetLastError();
 }
 hipError_t launch_image_residual(hipStream_t s, const double* out, const int* perm, int64_t npix, int64_t f, const double* data,

@@ -152,7 +152,8 @@ hipError_t launch_vis_combine(hipStream_t s, int nuv, int nb, int nf, const doub
 hipError_t launch_vis_residual(hipStream_t s, int nuv, const double* V, const double* vis, const double* sigma, const double* gains,
                                const double* tap, double scale, double* wq, double* chi_k);
 hipError_t launch_vis_rotate(hipStream_t s, int nuv, int nb, const double* wq, const double* B2, double* E2);
-hipError_t launch_vis_scatter_seed(hipStream_t s, const double* Sd, const int* perm, int64_t npix, int na, int nb, int64_t f, double* ads);
+hipError_t launch_vis_scatter_seed(hipStream_t s, const double* Sd, const int* perm, int64_t npix, int na, int nb, int64_t f, int add, double* ads);
+hipError_t launch_vis_scale_weights(hipStream_t s, int nuv, const double* w, const double* tap, double scale, double* wq);
 hipError_t launch_image_residual(hipStream_t s, const double* out, const int* perm, int64_t npix, int64_t f, const double* data,
                                  const double* sigma, int sigma_per_pixel, const uint8_t* mask, double* ads, double* chi_p);
 hipError_t launch_sum_fixed(hipStream_t s, const double* x, int64_t n, double* out);

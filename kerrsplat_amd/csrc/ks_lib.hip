@@ -857,10 +857,36 @@ int ks_vis_chi2_seed(ks_cache* c, int64_t frame, double psize, const double* u, 
     HIPCHK(launch_vis_residual(x->stream, nuv, c->vV.p, c->vd.p, c->vs.p, gains ? c->vg.p : nullptr, d_tap, scale, c->vwq.p, c->vchi.p));
     HIPCHK(launch_vis_rotate(x->stream, nuv, nb, c->vwq.p, c->vB2.p, c->vE2.p));
     HIPCHK(launch_dgemm_tn_splitk(x->stream, na, 4 * nb, 2 * nuv, c->vA2.p, na, c->vE2.p, 4 * nb, c->vSd.p, VIS_SPLITK, c->vSp.p));
-    HIPCHK(launch_vis_scatter_seed(x->stream, c->vSd.p, c->perm, c->npix, na, nb, frame, c->ads));
+    HIPCHK(launch_vis_scatter_seed(x->stream, c->vSd.p, c->perm, c->npix, na, nb, frame, 0, c->ads));
     HIPCHK(launch_sum_fixed(x->stream, c->vchi.p, nuv, c->vsum.p));
     HIPCHK(hipMemcpyAsync(chi2, c->vsum.p, sizeof(double), hipMemcpyDeviceToHost, x->stream));
     if (V) HIPCHK(hipMemcpyAsync(V, c->vV.p, (size_t)8 * nuv * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    c->fw_seeded[frame] = 1;
+    return KS_OK;
+}
+
+// seed_kernel! for an arbitrary visibility-domain loss: the adjoint transform of w = dloss/dV into the
+// device seed of one frame of the last forward pass (overwriting it, or added to it with add)
+int ks_vis_seed(ks_cache* c, int64_t frame, double psize, const double* u, const double* v, const double* taper, const double* w, int nuv,
+                int add) {
+    if (!c || !u || !v || !w) return fail(KS_ERR_ARG, "ks_vis_seed: null argument");
+    if (nuv < 1) return fail(KS_ERR_ARG, "need at least one (u, v) point");
+    if (int rc = seed_frame_ok(c, frame)) return rc;
+    if (int rc = screen_ready(c)) return rc;
+    if (add && !c->fw_seeded[frame]) return fail(KS_ERR_STATE, "frame %lld has no seed to add to", (long long)frame);
+    ks_ctx* x = c->ctx;
+    HIPCHK(hipSetDevice(x->device));
+    const int na = c->nalpha, nb = c->nbeta;
+    if (int rc = vis_setup(c, psize, u, v, taper, nuv)) return rc;
+    if (c->vd.grow((size_t)8 * nuv) || c->vwq.grow((size_t)8 * nuv) || c->vE2.grow((size_t)2 * nuv * 4 * nb) || c->vSd.grow((size_t)na * 4 * nb) ||
+        c->vSp.grow((size_t)VIS_SPLITK * na * 4 * nb))
+        return KS_ERR_HIP;
+    HIPCHK(hipMemcpyAsync(c->vd.p, w, (size_t)8 * nuv * sizeof(double), hipMemcpyHostToDevice, x->stream));
+    HIPCHK(launch_vis_scale_weights(x->stream, nuv, c->vd.p, taper ? c->vt.p : nullptr, psize * psize / JY, c->vwq.p));
+    HIPCHK(launch_vis_rotate(x->stream, nuv, nb, c->vwq.p, c->vB2.p, c->vE2.p));
+    HIPCHK(launch_dgemm_tn_splitk(x->stream, na, 4 * nb, 2 * nuv, c->vA2.p, na, c->vE2.p, 4 * nb, c->vSd.p, VIS_SPLITK, c->vSp.p));
+    HIPCHK(launch_vis_scatter_seed(x->stream, c->vSd.p, c->perm, c->npix, na, nb, frame, add, c->ads));
     HIPCHK(hipStreamSynchronize(x->stream));
     c->fw_seeded[frame] = 1;
     return KS_OK;

@@ -142,3 +142,92 @@ def image_loss_gradient(dparams, loss_and_grad, cache, params, t_obs, nu, L, nma
     dflat = np.transpose(np.asarray(dimg, dtype=np.float64), (1, 0, 2)).reshape(cache.npix, 4)
     api.gradient_reverse(cache, m, dparams, dflat[perm][None, None])
     return value
+
+
+# ---- closure quantities and station gains (visibilities.jl:141-214) --------------------------------------
+def _signed_vis(vis, k):
+    """Stokes I visibility of 1-based row k, conjugated for a negative k (the stored baseline runs the other way)."""
+    return np.conj(vis[-k - 1, 0]) if k < 0 else vis[k - 1, 0]
+
+
+def closure_phases(vis, triangles):
+    """closure_phases(vis, triangles) (visibilities.jl:141-150): arg(V1 V2 V3) of Stokes I for triangles of
+    1-based visibility indices closing (ij, jk, ki); a negative index conjugates. vis: complex (nuv, 4)."""
+    vis = np.asarray(vis)
+    return np.array([np.angle(_signed_vis(vis, t[0]) * _signed_vis(vis, t[1]) * _signed_vis(vis, t[2])) for t in triangles])
+
+
+def log_closure_amplitudes(vis, quadrangles):
+    """log_closure_amplitudes(vis, quadrangles) (visibilities.jl:152-161): ln(|V12||V34| / (|V13||V24|)) of
+    Stokes I for quadrangles of 1-based indices (12, 34, 13, 24); signs are ignored."""
+    vis = np.asarray(vis)
+    a = lambda k: np.abs(vis[abs(k) - 1, 0])
+    return np.array([np.log(a(q[0]) * a(q[1]) / (a(q[2]) * a(q[3]))) for q in quadrangles])
+
+
+class ClosureData:
+    """ClosureData(u, v, triangles, phases, σ_phase, quadrangles, logamps, σ_logamp) (visibilities.jl:163-178)."""
+
+    def __init__(self, u, v, triangles=(), phases=(), sigma_phase=(), quadrangles=(), logamps=(), sigma_logamp=()):
+        self.u = np.asarray(u, dtype=np.float64).ravel()
+        self.v = np.asarray(v, dtype=np.float64).ravel()
+        self.triangles = [tuple(int(x) for x in t) for t in triangles]
+        self.phases = np.asarray(phases, dtype=np.float64)
+        self.sigma_phase = np.asarray(sigma_phase, dtype=np.float64)
+        self.quadrangles = [tuple(int(x) for x in q) for q in quadrangles]
+        self.logamps = np.asarray(logamps, dtype=np.float64)
+        self.sigma_logamp = np.asarray(sigma_logamp, dtype=np.float64)
+
+
+def closure_chi2(V, data, grad=False):
+    """The closure χ² of model visibilities V (complex (nuv, 4)) as chi2_closures computes it
+    (visibilities.jl:180-199: closure-phase differences wrapped to (-pi, pi]); with grad=True also
+    w = dchi2/dRe V + i dchi2/dIm V, the input of vis_seed (Stokes I only)."""
+    V = np.asarray(V, dtype=np.complex128)
+    total = 0.0
+    w = np.zeros_like(V)
+    if data.triangles:
+        cp = closure_phases(V, data.triangles)
+        d = np.remainder(cp - data.phases + np.pi, 2 * np.pi) - np.pi
+        total += float(np.sum((d / data.sigma_phase) ** 2))
+        if grad:
+            coef = 2.0 * d / data.sigma_phase ** 2
+            for c, t in zip(coef, data.triangles):
+                for k in t:      # d arg(V) = Im(dV / V): w = i V / |V|^2, negated for a conjugated leg
+                    z = V[abs(k) - 1, 0]
+                    w[abs(k) - 1, 0] += c * (1j if k > 0 else -1j) * z / abs(z) ** 2
+    if data.quadrangles:
+        la = log_closure_amplitudes(V, data.quadrangles)
+        r = (la - data.logamps) / data.sigma_logamp
+        total += float(np.sum(r ** 2))
+        if grad:
+            coef = 2.0 * r / data.sigma_logamp
+            for c, q in zip(coef, data.quadrangles):
+                for sgn, k in zip((1.0, 1.0, -1.0, -1.0), q):     # d ln|V| = Re(dV / V): w = V / |V|^2
+                    z = V[abs(k) - 1, 0]
+                    w[abs(k) - 1, 0] += c * sgn * z / abs(z) ** 2
+    return (total, w) if grad else total
+
+
+def frame_chi2_closures(cache, c, data, dalpha, L, D, taper=None, add=False):
+    """chi2_closures of frame c of the last forward pass (visibilities on the device, closures on the host,
+    as the reference computes them) with its adjoint seed written on the device through vis_seed
+    (added to the frame's seed with add=True: a visibility and a closure term in one likelihood)."""
+    psize = dalpha * L / D
+    V = api.visibilities(cache, c + 1, psize, data.u, data.v, taper=taper)[c]
+    chi, w = closure_chi2(V, data, grad=True)
+    api.vis_seed(cache, c, psize, data.u, data.v, w, taper=taper, add=add)
+    return chi
+
+
+def gain_factors(gains, s1, s2):
+    """g_{s1} conj(g_{s2}) per baseline from a 2 x nstations gain matrix (log-amplitude, phase) and 1-based
+    station indices (visibilities.jl:92)."""
+    g = np.asarray(gains, dtype=np.float64)
+    s1, s2 = np.asarray(s1) - 1, np.asarray(s2) - 1
+    return np.exp(g[0, s1] + g[0, s2]) * np.exp(1j * (g[1, s1] - g[1, s2]))
+
+
+def apply_gains(vis, gains, s1, s2):
+    """apply_gains(vis, gains, s1, s2) (visibilities.jl:201-214): V'_k = g_{s1[k]} conj(g_{s2[k]}) V_k."""
+    return np.asarray(vis) * gain_factors(gains, s1, s2)[:, None]

@@ -157,3 +157,46 @@ def test_seed_errors():
     other = K.Model(syn.thermal_splats(1, A, 1))
     with pytest.raises(ValueError, match="model of its forward"):
         K.gradient_reverse(cache, other, dp, np.zeros((1, 2, cache.npix, 4)))
+
+
+def test_closure_seed_vs_oracle(oracle):
+    """frame_chi2_closures: closure χ² of a frame's device visibilities and its seed through vis_seed (the
+    adjoint transform of dχ²/dV on the device), added to a visibility χ² seed of the same frame; the
+    gradient it drives equals the host-seeded dual sweep with the oracle's adjoint transform."""
+    import itertools
+    from kerrsplat_amd import fit
+    na = 14
+    cam, cache = stored_cache(na, 128)
+    L = syn.length_unit()
+    P = syn.thermal_splats(1, A, 3)
+    P[12, :] += math.log(30.0)
+    m = K.Model(P)
+    img = K.unsort(cache, K.gradient_forward(cache, m, [0.0], [NU], L)[0, 0])
+    psize = syn.pixel_size_rad(-15.0, 15.0, na)
+    rng = np.random.default_rng(13)
+    nst = 6
+    pairs = list(itertools.combinations(range(1, nst + 1), 2))
+    index = {p: n + 1 for n, p in enumerate(pairs)}
+    leg = lambda a, b: index[(a, b)] if a < b else -index[(b, a)]
+    pos = rng.normal(scale=3e9, size=(nst, 2))
+    u = np.array([pos[i - 1, 0] - pos[j - 1, 0] for i, j in pairs]); v = np.array([pos[i - 1, 1] - pos[j - 1, 1] for i, j in pairs])
+    tri = [(leg(i, j), leg(j, k), leg(k, i)) for i, j, k in itertools.combinations(range(1, nst + 1), 3)]
+    quad = [(leg(i, j), leg(k, l), leg(i, k), leg(j, l)) for i, j, k, l in itertools.combinations(range(1, nst + 1), 4)]
+    V0 = oracle.visibilities(img[None], na, na, psize, u, v)[0]
+    data = fit.ClosureData(u, v, tri, fit.closure_phases(V0, tri) + rng.normal(scale=0.2, size=len(tri)), np.full(len(tri), 0.1),
+                           quad, fit.log_closure_amplitudes(V0, quad) + rng.normal(scale=0.1, size=len(quad)), np.full(len(quad), 0.1))
+    sig = 0.05 * np.abs(V0[:, 0]).max() * np.ones((len(u), 4))
+    vdata = V0 * 1.1
+    chi_v = K.vis_chi2_seed(cache, 0, psize, u, v, vdata, sig)
+    dalpha = 30.0 / (na - 1)
+    chi_c = fit.frame_chi2_closures(cache, 0, data, dalpha, L, syn.DISTANCE_CM, add=True)
+    chi_c_ref, w = fit.closure_chi2(V0, data, grad=True)
+    assert abs(chi_c - chi_c_ref) <= 1e-9 * chi_c_ref
+    dp_dev = np.zeros(P.shape, order="F")
+    K.gradient_reverse(cache, m, dp_dev)
+    wv = 2 * (V0 - vdata) / sig ** 2
+    seed = oracle.vis_seed((w + wv)[None], na, na, psize, u, v)[0]
+    ds = np.transpose(seed.reshape(na, na, 4), (1, 0, 2))[:, :, None, None, :]
+    dp_host, _ = K.polarized_gradient(cache, m, [0.0], [NU], L, ds)
+    assert chi_v > 0 and np.abs(dp_host).max() > 0
+    assert np.abs(dp_dev - dp_host).max() <= 1e-9 * np.abs(dp_host).max()
