@@ -28,7 +28,7 @@ enum { KS_MARCHER_DIRECT = 0, KS_MARCHER_RECURRENCE = 1, KS_MARCHER_FUSED = 2 };
 
 /* splat models: ThinRenderer (Splats.jl:105), PolarizedSplats (polarized.jl:41), PowerLawSplats,
  * KappaSplats (populations.jl:34-60), KnotSplats (trajectories.jl:184) */
-enum { KS_MODEL_THIN = 0, KS_MODEL_THERMAL = 1, KS_MODEL_POWERLAW = 2, KS_MODEL_KAPPA = 3, KS_MODEL_KNOT = 4 };
+enum { KS_MODEL_THIN = 0, KS_MODEL_THERMAL = 1, KS_MODEL_POWERLAW = 2, KS_MODEL_KAPPA = 3, KS_MODEL_KNOT = 4, KS_MODEL_COMPOSITE = 5 };
 
 typedef struct ks_ctx ks_ctx;
 typedef struct ks_cache ks_cache;
@@ -104,6 +104,12 @@ int ks_copy_samples(const ks_cache* cache, double* t, double* r, double* theta, 
  * gamma_max: power law; knots: 3 x nknots x ns, knot_times: nknots (knot splats). */
 int ks_model_create(ks_ctx* ctx, int kind, int nrows, int ns, const double* params, double gamma_max,
                     const double* knots, const double* knot_times, int nknots, ks_model** model);
+/* CompositeModel(a, b) - transport.jl:298-316: the elements of parts[0], then parts[1], ...; the
+ * coefficients of overlapping elements add. Parts are thermal, power-law, kappa or composite models
+ * of the same context (thin and knot splats are not composed); the splat constants are copied, so
+ * the parts may be destroyed afterwards. Renders through ks_render like any population; the dual
+ * sweep takes thermal splats only. */
+int ks_model_composite(ks_ctx* ctx, int nparts, const ks_model* const* parts, ks_model** model);
 int ks_model_destroy(ks_model* model);
 
 /* thin_image!(out, cache, params, t_obs) - Splats.jl:137-147: Stokes I, screen or sorted order. */

@@ -15,7 +15,7 @@ C-ABI of libkerrsplat_hip.so (include/kerrsplat_hip.h):
 Every call goes to the GPU library; there is no CPU fallback. Loading fails loudly when the library
 is missing or no GPU is present.
 """
-from .api import (Camera, Direct, Recurrence, Fused, GeodesicCache, Model, regenerate, thin_image, polarized_image,
+from .api import (Camera, Direct, Recurrence, Fused, GeodesicCache, Model, CompositeModel, regenerate, thin_image, polarized_image,
                   polarized_cube, polarized_gradient, march, precision, stage_ms, render_device, visibilities, unsort, to_screen, samples, residuals, KernelError,
                   NSPLATPARAMS, NPOLARIZEDPARAMS, NPOWERLAWPARAMS, NKAPPAPARAMS, gravitational_radius, grid_axis,
                   library_path, gradient_forward, gradient_reverse, vis_chi2_seed, image_chi2_seed, vis_seed)
@@ -23,7 +23,7 @@ from . import fit
 from .fit import (StokesMovie, FrameScans, chi2, chi2_gradient, sweep_passes, frame_chi2_seed, image_loss_gradient, closure_phases,
                   log_closure_amplitudes, ClosureData, closure_chi2, frame_chi2_closures, gain_factors, apply_gains)
 
-__all__ = ["Camera", "Direct", "Recurrence", "Fused", "GeodesicCache", "Model", "regenerate", "thin_image",
+__all__ = ["Camera", "Direct", "Recurrence", "Fused", "GeodesicCache", "Model", "CompositeModel", "regenerate", "thin_image",
            "polarized_image", "polarized_cube", "polarized_gradient", "march", "precision", "stage_ms", "render_device", "visibilities", "unsort", "to_screen", "samples",
            "residuals", "KernelError", "NSPLATPARAMS", "NPOLARIZEDPARAMS", "NPOWERLAWPARAMS", "NKAPPAPARAMS",
            "gravitational_radius", "grid_axis", "library_path", "gradient_forward", "gradient_reverse", "vis_chi2_seed",

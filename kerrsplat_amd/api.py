@@ -68,6 +68,7 @@ def _load():
     lib.ks_cache_create.argtypes = [_P, _i64, _i, _i, ctypes.POINTER(_P)]
     lib.ks_regenerate.argtypes = [_P, _P, _P, _i, _i, _d, _d, _i, _i]
     lib.ks_model_create.argtypes = [_P, _i, _i, _i, _P, _d, _P, _P, _i, ctypes.POINTER(_P)]
+    lib.ks_model_composite.argtypes = [_P, _i, _P, ctypes.POINTER(_P)]
     lib.ks_render.argtypes = [_P, _P, _P, _i, _P, _i, _d, _i, _d, _P, _i]
     lib.ks_render_device.argtypes = [_P, _P, _P, _i, _P, _i, _d, _i, _d]
     lib.ks_copy_stokes.argtypes = [_P, _i64, _P, _i]
@@ -303,6 +304,26 @@ class Model:
                 self._h = None
         except Exception:
             pass
+
+
+class CompositeModel(Model):
+    """CompositeModel(a, b, ...) (transport.jl:298-316): the elements of a, then of b, ...; the
+    coefficients of overlapping elements add. Parts are Model objects of kind thermal, powerlaw or
+    kappa, or other composites; the splat constants are copied to the composite on the device."""
+
+    def __init__(self, *parts, device=0):
+        if not parts:
+            raise ValueError("CompositeModel needs at least one part")
+        for p in parts:
+            if not isinstance(p, Model) or p.kind not in ("thermal", "powerlaw", "kappa", "composite"):
+                raise ValueError("CompositeModel parts are thermal, powerlaw, kappa or composite Model objects")
+        self.kind, self.params, self.parts = "composite", None, parts
+        self._lib = _load()
+        arr = (_P * len(parts))(*[p._h.value for p in parts])
+        h = _P()
+        _check(self._lib.ks_model_composite(context(device), _i(len(parts)), arr, ctypes.byref(h)))
+        self._h = h
+        self.ns = sum(p.ns for p in parts)
 
 
 def _as_model(params, kind):

@@ -22,7 +22,9 @@ constexpr double JY = 1e-23;                  // erg s^-1 cm^-2 Hz^-1
 constexpr double PLANCK0 = 2.0 * HPL * NU0 * NU0 * NU0 / (CL * CL);
 constexpr double WEIGHT_CUTOFF = 1e-6;
 
-enum ModelKind : int { MODEL_THIN = 0, MODEL_THERMAL = 1, MODEL_POWERLAW = 2, MODEL_KAPPA = 3, MODEL_KNOT = 4 };
+// MODEL_COMPOSITE: concatenated thermal, power-law and kappa splats (CompositeModel, transport.jl:298-316),
+// each splat's population in SplatCT::kind
+enum ModelKind : int { MODEL_THIN = 0, MODEL_THERMAL = 1, MODEL_POWERLAW = 2, MODEL_KAPPA = 3, MODEL_KNOT = 4, MODEL_COMPOSITE = 5 };
 
 // Per-splat constants, built on the host by ks_model_create in both precisions (SplatC, SplatCf);
 // a kernel reads them with wave-uniform addresses.
@@ -41,7 +43,7 @@ template <class S> struct SplatCT {
     S g[6];               // power law: Gamma((3p-1)/12) Gamma((3p+19)/12), Gamma((3p+2)/12) Gamma((3p+22)/12);
                           // kappa: Gamma(k-4/3), Gamma(k-2), Gamma(k/4-1/3), Gamma(k/4+4/3), Gamma(2+k/2), K0/(K2+1e-40) at 1/w
     int knot0;            // knot splats: index of the splat's first knot
-    int pad;
+    int kind;             // the splat's population (MODEL_THERMAL, MODEL_POWERLAW, MODEL_KAPPA, ...)
 };
 using SplatC = SplatCT<double>;
 using SplatCf = SplatCT<f32>;

@@ -89,6 +89,9 @@ KS_DEV bool sample_coefficients(const TransCtxT<S>& c, double st, double r, doub
         CoefT<S> cf;
         if constexpr (KIND == MODEL_THERMAL || KIND == MODEL_KNOT) cf = thermal(ne, sc.p1, sc.p2, sc.p3, sc.Bmag, nuf, sth, cth);
         else if constexpr (KIND == MODEL_POWERLAW) cf = powerlaw(ne, sc.p1, sc.p2, sc.p3, sc.g, sc.Bmag, nuf, sth, cth);
+        else if constexpr (KIND == MODEL_KAPPA) cf = kappa(ne, sc.p1, sc.p2, sc.p3, sc.g, sc.Bmag, nuf, sth, cth);
+        else if (sc.kind == MODEL_THERMAL) cf = thermal(ne, sc.p1, sc.p2, sc.p3, sc.Bmag, nuf, sth, cth);
+        else if (sc.kind == MODEL_POWERLAW) cf = powerlaw(ne, sc.p1, sc.p2, sc.p3, sc.g, sc.Bmag, nuf, sth, cth);
         else cf = kappa(ne, sc.p1, sc.p2, sc.p3, sc.g, sc.Bmag, nuf, sth, cth);
         if (!(cf.jI > 0.0 || cf.aI > 0.0 || cf.rQ != 0.0 || cf.rV != 0.0)) continue;
         S fe = cap_factor(cf.jI, cf.jQ, cf.jV), fa = cap_factor(cf.aI, cf.aQ, cf.aV);
@@ -456,6 +459,7 @@ static void chunk_transfer_s(hipStream_t s, int kind, double a, double thetao, c
         case MODEL_THERMAL: chunk_transfer_k<MODEL_THERMAL, S>(s, a, thetao, soa, npix, N, smp, sp, ns, knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, seg, rec, aflg); break;
         case MODEL_POWERLAW: chunk_transfer_k<MODEL_POWERLAW, S>(s, a, thetao, soa, npix, N, smp, sp, ns, knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, seg, rec, aflg); break;
         case MODEL_KAPPA: chunk_transfer_k<MODEL_KAPPA, S>(s, a, thetao, soa, npix, N, smp, sp, ns, knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, seg, rec, aflg); break;
+        case MODEL_COMPOSITE: chunk_transfer_k<MODEL_COMPOSITE, S>(s, a, thetao, soa, npix, N, smp, sp, ns, knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, seg, rec, aflg); break;
         default: chunk_transfer_k<MODEL_KNOT, S>(s, a, thetao, soa, npix, N, smp, sp, ns, knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, seg, rec, aflg); break;
     }
 }
@@ -490,6 +494,7 @@ hipError_t launch_seg_polarized(hipStream_t s, int kind, double a, double thetao
         case MODEL_THERMAL: seg_polarized_k<MODEL_THERMAL>(s, a, thetao, soa, npix, N, M, sp, ns, knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, seg, res); break;
         case MODEL_POWERLAW: seg_polarized_k<MODEL_POWERLAW>(s, a, thetao, soa, npix, N, M, sp, ns, knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, seg, res); break;
         case MODEL_KAPPA: seg_polarized_k<MODEL_KAPPA>(s, a, thetao, soa, npix, N, M, sp, ns, knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, seg, res); break;
+        case MODEL_COMPOSITE: seg_polarized_k<MODEL_COMPOSITE>(s, a, thetao, soa, npix, N, M, sp, ns, knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, seg, res); break;
         default: seg_polarized_k<MODEL_KNOT>(s, a, thetao, soa, npix, N, M, sp, ns, knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, seg, res); break;
     }
     return hipGetLastError();

@@ -391,7 +391,7 @@ static void to_f32(const SplatC& a, SplatCf& b) {
     b.t0 = a.t0; b.iw = a.iw; b.omega = a.omega; b.rsup2 = a.rsup2; b.lnA = a.lnA; b.Bmag = a.Bmag;
     b.p1 = a.p1; b.p2 = a.p2; b.p3 = a.p3;
     for (int i = 0; i < 6; ++i) b.g[i] = a.g[i];
-    b.knot0 = a.knot0; b.pad = a.pad;
+    b.knot0 = a.knot0; b.kind = a.kind;
 }
 static void to_f32(const SplatAdj& a, SplatAdjf& b) {
     std::memset(&b, 0, sizeof b);
@@ -430,6 +430,7 @@ int ks_model_create(ks_ctx* ctx, int kind, int nrows, int ns, const double* para
         double smax = std::max(std::max(p[3], p[4]), p[5]);
         c.rsup2 = SR2 * std::exp(2.0 * smax);
         c.lnA = p[12];
+        c.kind = kind;
         if (kind == KS_MODEL_THIN) continue;
         int rB = (kind == KS_MODEL_THERMAL || kind == KS_MODEL_KNOT) ? 14 : 15;
         c.Bmag = std::exp(p[rB]);
@@ -500,6 +501,46 @@ int ks_model_create(ks_ctx* ctx, int kind, int nrows, int ns, const double* para
     return KS_OK;
 }
 
+static_assert(KS_MODEL_THIN == MODEL_THIN && KS_MODEL_THERMAL == MODEL_THERMAL && KS_MODEL_POWERLAW == MODEL_POWERLAW &&
+              KS_MODEL_KAPPA == MODEL_KAPPA && KS_MODEL_KNOT == MODEL_KNOT && KS_MODEL_COMPOSITE == MODEL_COMPOSITE,
+              "the C-ABI model kinds are the device population tags");
+
+int ks_model_composite(ks_ctx* ctx, int nparts, const ks_model* const* parts, ks_model** out) {
+    if (!ctx || !out || !parts || nparts < 1) return fail(KS_ERR_ARG, "ks_model_composite: null argument or no parts");
+    int64_t ns = 0;
+    for (int p = 0; p < nparts; ++p) {
+        const ks_model* q = parts[p];
+        if (!q) return fail(KS_ERR_ARG, "ks_model_composite: part %d is null", p);
+        if (q->ctx != ctx) return fail(KS_ERR_ARG, "ks_model_composite: part %d belongs to another context", p);
+        if (q->kind != KS_MODEL_THERMAL && q->kind != KS_MODEL_POWERLAW && q->kind != KS_MODEL_KAPPA && q->kind != KS_MODEL_COMPOSITE)
+            return fail(KS_ERR_ARG, "ks_model_composite: part %d has kind %d (parts are thermal, power-law, kappa or composite)", p, q->kind);
+        ns += q->ns;
+    }
+    if (ns > INT32_MAX) return fail(KS_ERR_DIM, "ks_model_composite: %lld splats", (long long)ns);
+    HIPCHK(hipSetDevice(ctx->device));
+    ks_model* m = new ks_model;
+    m->ctx = ctx; m->kind = KS_MODEL_COMPOSITE; m->nrows = 0; m->ns = (int)ns;
+    const size_t n = ns > 0 ? (size_t)ns : 1;
+    if (dalloc(&m->d_splats, n) || dalloc(&m->d_adj, n) || dalloc(&m->d_splats_f, n) || dalloc(&m->d_adj_f, n)) {
+        ks_model_destroy(m); return KS_ERR_HIP;
+    }
+    size_t off = 0;
+    for (int p = 0; p < nparts; ++p) {
+        const ks_model* q = parts[p];
+        if (q->ns < 1) continue;
+        const hipError_t e[4] = {
+            hipMemcpy(m->d_splats + off, q->d_splats, q->ns * sizeof(SplatC), hipMemcpyDeviceToDevice),
+            hipMemcpy(m->d_adj + off, q->d_adj, q->ns * sizeof(SplatAdj), hipMemcpyDeviceToDevice),
+            hipMemcpy(m->d_splats_f + off, q->d_splats_f, q->ns * sizeof(SplatCf), hipMemcpyDeviceToDevice),
+            hipMemcpy(m->d_adj_f + off, q->d_adj_f, q->ns * sizeof(SplatAdjf), hipMemcpyDeviceToDevice)};
+        for (hipError_t x : e)
+            if (x != hipSuccess) { ks_model_destroy(m); return fail(KS_ERR_HIP, "ks_model_composite: %s", hipGetErrorString(x)); }
+        off += q->ns;
+    }
+    *out = m;
+    return KS_OK;
+}
+
 int ks_model_destroy(ks_model* m) {
     if (!m) return KS_OK;
     (void)hipSetDevice(m->ctx->device);
@@ -555,8 +596,7 @@ static int launch_transfer(ks_cache* c, const ks_model* m, const double* d_tn, i
         kst = c->kstop;
     }
     HIPCHK(x->stage_begin(KS_STAGE_TRANSFER));
-    const int kind = m->kind == KS_MODEL_THERMAL ? MODEL_THERMAL : m->kind == KS_MODEL_POWERLAW ? MODEL_POWERLAW
-                   : m->kind == KS_MODEL_KAPPA ? MODEL_KAPPA : MODEL_KNOT;
+    const int kind = m->kind;
     if (has_samples(c) && c->precision == KS_PRECISION_MIXED)
         HIPCHK(launch_chunk_transfer_f32(x->stream, kind, c->spin, c->thetao, c->soa, c->npix, c->N, samples_of(c), m->d_splats_f, m->ns,
                                          m->d_knots_f, m->d_ktimes_f, m->nknots, d_tn, nt, d_tn + nt, nnu, L, kst, c->seg, (f32*)rec, aflg));

@@ -156,6 +156,35 @@ int kso_polarized_image(double a, double thetao, int npix, const double* alphas,
     return 0;
 }
 
+// polarized Stokes image of a CompositeModel (transport.jl:298-316): part p has kinds[p], nrows[p] x ns[p]
+// parameters params[p], kappa factors hyp[p] (or null) and power-law cutoff gamma_max[p]; the elements
+// run over part 0, then part 1, ... Knot parts are not composed here (their knots stay per model).
+int kso_polarized_composite(double a, double thetao, int npix, const double* alphas, const double* betas, int N, int M, int marcher,
+                            int nparts, const int* kinds, const int* nrows, const int* ns, const double* const* params,
+                            const double* const* hyp, const double* gamma_max, double t_obs, double nu, double L, int nmax, double slab,
+                            double* out, double* resid_t, double* resid_ph) {
+    if (nparts < 1) return 1;
+    std::vector<ModelHolder> parts;
+    parts.reserve(nparts);
+    for (int p = 0; p < nparts; ++p) {
+        if (kinds[p] == MODEL_THIN || kinds[p] == MODEL_KNOT) return 2;
+        parts.push_back(make_model<double>(kinds[p], nrows[p], ns[p], params[p], hyp ? hyp[p] : nullptr, gamma_max[p], nullptr, nullptr, 0));
+    }
+    for (int p = 0; p < nparts; ++p) {
+        parts[p].m.bessel = parts[p].bessel.data();
+        parts[p].m.next = p + 1 < nparts ? &parts[p + 1].m : nullptr;
+    }
+    const Model& m = parts[0].m;
+#pragma omp parallel for schedule(dynamic, 8)
+    for (int i = 0; i < npix; ++i) {
+        double rt, rp;
+        polarized_ray<double>(m, a, thetao, alphas[i], betas[i], N, M, marcher, t_obs, nu, L, nmax, slab, out + 4 * (size_t)i, &rt, &rp);
+        if (resid_t) resid_t[i] = rt;
+        if (resid_ph) resid_ph[i] = rp;
+    }
+    return 0;
+}
+
 // algorithmic operation count of the polarized forward transport over a set of rays (Counted<double>)
 int kso_count_polarized(double a, double thetao, int npix, const double* alphas, const double* betas, int N, int M,
                         int kind, int nrows, int ns, const double* params, const double* hyp, double gamma_max,

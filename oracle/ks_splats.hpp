@@ -30,6 +30,9 @@ struct Model {
     // per-splat Bessel values that depend only on the splat (K0, K1, K2 at 1/Theta_e for thermal
     // and knot splats, at 1/w for kappa splats), filled once per model
     const double* bessel = nullptr;
+    // CompositeModel (transport.jl:298-316): the elements of this model are followed by those of
+    // `next` (and its successors); coefficients of overlapping elements add
+    const Model* next = nullptr;
     double P(int row, int i) const { return params[row + nrows * i]; }
 };
 
@@ -159,19 +162,24 @@ template <class T> void wind(Winding<T>& w, T z, T h) {
     w.zprev = z;
 }
 
-// sum of all elements' screen coefficients at a sample; returns whether any element is active
-template <class T> bool accumulate_elements(const Model& m, const RayCtx& rc, const Sample<T>& s, T t_obs, T nu_obs, T j4[4], T a4[4], T r3[3]) {
+// sum of all elements' screen coefficients at a sample (transport.jl:254-276); returns whether any
+// element is active. A composite model visits its parts in order, each part's elements in order
+// (element(::CompositeModel), transport.jl:312-316), so the sum runs over the same sequence.
+template <class T> bool accumulate_elements(const Model& m0, const RayCtx& rc, const Sample<T>& s, T t_obs, T nu_obs, T j4[4], T a4[4], T r3[3]) {
     for (int q = 0; q < 4; ++q) { j4[q] = 0.0; a4[q] = 0.0; }
     for (int q = 0; q < 3; ++q) r3[q] = 0.0;
     bool active = false;
-    for (int i = 0; i < m.ns; ++i) {
-        Coef<T> cf; Frame<T> fr;
-        if (!element(m, i, rc, s, t_obs, nu_obs, cf, fr)) continue;
-        Screen<T> sc;
-        if (!screen_coefficients(cf, fr, nu_obs, sc)) continue;
-        for (int q = 0; q < 4; ++q) { j4[q] = j4[q] + sc.j[q]; a4[q] = a4[q] + sc.a[q]; }
-        for (int q = 0; q < 3; ++q) r3[q] = r3[q] + sc.r[q];
-        active = true;
+    for (const Model* pm = &m0; pm; pm = pm->next) {
+        const Model& m = *pm;
+        for (int i = 0; i < m.ns; ++i) {
+            Coef<T> cf; Frame<T> fr;
+            if (!element(m, i, rc, s, t_obs, nu_obs, cf, fr)) continue;
+            Screen<T> sc;
+            if (!screen_coefficients(cf, fr, nu_obs, sc)) continue;
+            for (int q = 0; q < 4; ++q) { j4[q] = j4[q] + sc.j[q]; a4[q] = a4[q] + sc.a[q]; }
+            for (int q = 0; q < 3; ++q) r3[q] = r3[q] + sc.r[q];
+            active = true;
+        }
     }
     return active;
 }

@@ -160,6 +160,61 @@ def test_polarized_many_splats_tile_cull_vs_oracle(oracle):
     assert np.abs(g32 - g).max() <= 1e-3 * np.abs(g).max()
 
 
+def _composite_parts(oracle, a, nt, npl, nk):
+    T, P, Kp = syn.thermal_splats(1, a, nt), syn.powerlaw_splats(2, a, npl), syn.kappa_splats(3, a, nk)
+    hyp = [oracle.kappa_hyp(Kp[13, i], math.exp(Kp[14, i])) for i in range(Kp.shape[1])]
+    return [("thermal", T), ("powerlaw", P), ("kappa", Kp, hyp)]
+
+
+def _composite_case(oracle, path, nt, npl, nk, res=16, N=256):
+    """CompositeModel(PolarizedSplats, CompositeModel(PowerLawSplats, KappaSplats)) (transport.jl:298-316)
+    against the oracle's composite at the FP64 image bars; the stored path also in mixed precision."""
+    a, th = 0.9, math.radians(45.0)
+    cam = camera(12.0, res)
+    L = syn.length_unit()
+    cache = make_cache(cam, N, a, th, path)
+    ps = _composite_parts(oracle, a, nt, npl, nk)
+    model = K.CompositeModel(K.Model(ps[0][1]), K.CompositeModel(K.Model(ps[1][1], kind="powerlaw"), K.Model(ps[2][1], kind="kappa")))
+    assert model.ns == nt + npl + nk
+    img = K.polarized_image(cache, model, 0.0, 230e9, L)
+    ref = oracle.polarized_composite(a, th, cam.alphas, cam.betas, N, ps, 0.0, 230e9, L)[0].reshape(cam.size + (4,), order="F")
+    I = np.abs(ref[..., 0])
+    assert I.max() > 0
+    err = np.abs(img - ref).max(axis=-1) / np.maximum(I, 1e-300)
+    bright = I > 1e-6 * I.max()
+    assert err[bright].max() < 1e-10, err[bright].max()
+    assert np.abs(img - ref).max() <= 1e-10 * I.max()
+    # every population reaches the image: the thermal part alone differs from the composite
+    alone = K.polarized_image(cache, K.Model(ps[0][1]), 0.0, 230e9, L)
+    assert np.abs(alone - img).max() > 1e-3 * I.max()
+    if path == "stored":
+        K.precision(cache, "mixed")
+        got = K.polarized_image(cache, model, 0.0, 230e9, L)
+        assert 0 < np.abs(got - img).max() <= 1e-4 * I.max()
+
+
+@pytest.mark.parametrize("path", PATHS)
+def test_polarized_composite_vs_oracle(oracle, path):
+    _composite_case(oracle, path, 3, 2, 2)
+
+
+def test_polarized_composite_tile_cull_vs_oracle(oracle):
+    """72 splats of three populations: the transfer takes the per-tile cull with mixed population tags."""
+    _composite_case(oracle, "stored", 40, 16, 16, res=12, N=128)
+
+
+def test_composite_errors():
+    th = K.Model(syn.thermal_splats(1, 0.9, 2))
+    with pytest.raises(ValueError, match="parts"):
+        K.CompositeModel(th, K.Model(syn.thin_splats(0, 0.9, 2), kind="thin"))
+    with pytest.raises(ValueError):
+        K.CompositeModel()
+    cam = camera(10.0, 4)
+    c = make_cache(cam, 16, 0.9, 0.5, "stored")
+    with pytest.raises(ValueError, match="thermal"):
+        K.polarized_gradient(c, K.CompositeModel(th, th), [0.0], [230e9], 1e14, np.zeros((4, 4, 1, 1, 4)))
+
+
 @pytest.mark.parametrize("path", PATHS)
 def test_cube_frames_match_images(oracle, path):
     """One launch over (frame, frequency) items equals per-frame images (polarized.jl:187-198)."""
