@@ -197,7 +197,8 @@ def main():
     wm_path = os.path.join(ROOT, "profiles", "work_model.json")
     wm = json.load(open(wm_path)).get(cfg["name"]) if os.path.exists(wm_path) else None
     tpl = {"fp64": "double", "mixed": "f32"}
-    kernel_of = {"march": lambda p_: "ks_march_store<1>", "transfer": lambda p_: f"ks_chunk_transfer<1, {tpl[p_]}>",
+    cull = "true" if 64 <= cfg["ns"] <= 1024 else "false"      # the transfer's per-tile cull (from 64 to 1024 splats)
+    kernel_of = {"march": lambda p_: "ks_march_store<1>", "transfer": lambda p_: f"ks_chunk_transfer<1, {tpl[p_]}, {cull}>",
                  "adjoint_steps": lambda p_: f"ks_adj_segment<{tpl[p_]}>", "adjoint_elements": lambda p_: f"ks_adj_splats<{tpl[p_]}>"}
     # HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE summary (rocprofv3 --pmc, scaled by a copy probe)
     tr_path = os.path.join(ROOT, "profiles", "traffic_128x128_N1024_32thermal.json")

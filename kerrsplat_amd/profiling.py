@@ -1,10 +1,13 @@
 """Summaries of rocprofv3 output for the kernels of this package (text for profiles/, no GPU needed).
 
     python -m kerrsplat_amd.profiling trace  <kernel_stats.csv>                      > summary.md
+    python -m kerrsplat_amd.profiling launches <kernel_trace.csv>                     > summary.md
     python -m kerrsplat_amd.profiling traffic <fetch counter_collection.csv> <write counter_collection.csv> \
         [--copy-bytes N] [--json out.json]                                           > summary.md
 
 `trace` tabulates the kernel-trace statistics (calls, average and total duration per kernel).
+`launches` does the same from the per-dispatch trace, one row per (kernel, grid size), with the
+registers and scratch (spill) bytes per lane the compiler gave the kernel.
 `traffic` turns FETCH_SIZE and WRITE_SIZE (KiB per dispatch) into HBM bytes per launch: each counter is
 scaled by what it reads for the runtime's device-to-device copy of a known size (bench.py
 --copy-probe copies 256 MiB, so FETCH and WRITE should each read 256 MiB for that dispatch); the
@@ -34,29 +37,51 @@ def trace_summary(stats_csv):
     return out
 
 
+def launch_summary(trace_csv):
+    per = collections.OrderedDict()
+    for r in csv.DictReader(open(trace_csv)):
+        grid = int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"])
+        key = (short_name(r["Kernel_Name"]), grid)
+        d = per.setdefault(key, {"ns": [], "vgpr": int(r["VGPR_Count"]), "agpr": int(r["Accum_VGPR_Count"]),
+                                 "scratch": int(r["Scratch_Size"]), "lds": int(r["LDS_Block_Size"])})
+        d["ns"].append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    out = []
+    for (k, grid), d in per.items():
+        out.append({"kernel": k, "grid": grid, "calls": len(d["ns"]), "avg_us": sum(d["ns"]) / len(d["ns"]) / 1e3,
+                    "total_ms": sum(d["ns"]) / 1e6, "vgpr": d["vgpr"], "agpr": d["agpr"], "scratch": d["scratch"], "lds": d["lds"]})
+    return sorted(out, key=lambda r: -r["total_ms"])
+
+
 def counter_means(path, counter):
+    """Counter values per (kernel, grid size): a kernel launched at several sizes gets one entry per size."""
     per = collections.defaultdict(list)
     for r in csv.DictReader(open(path)):
         if r["Counter_Name"] == counter:
-            per[short_name(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+            per[(short_name(r["Kernel_Name"]), int(r["Grid_Size"]))].append(float(r["Counter_Value"]))
     return per
 
 
 def traffic_summary(fetch_csv, write_csv, copy_bytes=1 << 28):
+    """HBM bytes per launch, keyed by kernel name, or 'name@grid' for a kernel launched at several grid sizes."""
     fetch, write = counter_means(fetch_csv, "FETCH_SIZE"), counter_means(write_csv, "WRITE_SIZE")
-    if COPY_KERNEL not in fetch or COPY_KERNEL not in write:
+    fc = [v for (k, _), vs in fetch.items() if k == COPY_KERNEL for v in vs]
+    wc = [v for (k, _), vs in write.items() if k == COPY_KERNEL for v in vs]
+    if not fc or not wc:
         raise SystemExit("no copy-probe dispatch in the counter files (run bench.py with --copy-probe)")
     kib = copy_bytes / 1024.0
-    f_scale = kib / max(fetch[COPY_KERNEL])
-    w_scale = kib / max(write[COPY_KERNEL])
+    f_scale, w_scale = kib / max(fc), kib / max(wc)
     out = {"copy_probe_bytes": copy_bytes, "fetch_scale": f_scale, "write_scale": w_scale, "kernels": {}}
-    for k in sorted(set(fetch) | set(write)):
+    keys = sorted(set(fetch) | set(write))
+    sizes = collections.Counter(k for k, _ in keys)
+    for k, grid in keys:
         if k == COPY_KERNEL:
             continue
-        fk = sum(fetch.get(k, [0.0])) / max(len(fetch.get(k, [])), 1)
-        wk = sum(write.get(k, [0.0])) / max(len(write.get(k, [])), 1)
-        out["kernels"][k] = {"fetch_bytes": fk * 1024.0 * f_scale, "write_bytes": wk * 1024.0 * w_scale,
-                             "bytes": (fk * f_scale + wk * w_scale) * 1024.0, "dispatches": len(fetch.get(k, []))}
+        fv, wv = fetch.get((k, grid), []), write.get((k, grid), [])
+        fk = sum(fv) / max(len(fv), 1)
+        wk = sum(wv) / max(len(wv), 1)
+        name = k if sizes[k] == 1 else f"{k}@{grid}"
+        out["kernels"][name] = {"fetch_bytes": fk * 1024.0 * f_scale, "write_bytes": wk * 1024.0 * w_scale,
+                                "bytes": (fk * f_scale + wk * w_scale) * 1024.0, "dispatches": len(fv), "grid": grid}
     return out
 
 
@@ -65,6 +90,8 @@ def main(argv=None):
     sub = ap.add_subparsers(dest="cmd", required=True)
     t = sub.add_parser("trace")
     t.add_argument("stats")
+    l_ = sub.add_parser("launches")
+    l_.add_argument("trace")
     c = sub.add_parser("traffic")
     c.add_argument("fetch")
     c.add_argument("write")
@@ -76,12 +103,18 @@ def main(argv=None):
         print("| kernel | calls | avg (us) | total (ms) | % |\n|---|---|---|---|---|")
         for r in rows:
             print(f"| `{r['kernel']}` | {r['calls']} | {r['avg_us']:.1f} | {r['total_ms']:.2f} | {r['percent']:.2f} |")
+    elif a.cmd == "launches":
+        print("| kernel | grid (threads) | calls | avg (us) | total (ms) | VGPR | AGPR | scratch (B/lane) | LDS (B) |\n"
+              "|---|---|---|---|---|---|---|---|---|")
+        for r in launch_summary(a.trace):
+            print(f"| `{r['kernel']}` | {r['grid']} | {r['calls']} | {r['avg_us']:.1f} | {r['total_ms']:.2f} | {r['vgpr']} | "
+                  f"{r['agpr']} | {r['scratch']} | {r['lds']} |")
     else:
         s = traffic_summary(a.fetch, a.write, a.copy_bytes)
         print(f"copy probe: {s['copy_probe_bytes']} bytes; FETCH_SIZE scale {s['fetch_scale']:.3f}, WRITE_SIZE scale {s['write_scale']:.3f}\n")
-        print("| kernel | read (MB) | written (MB) | HBM bytes per launch (MB) |\n|---|---|---|---|")
+        print("| kernel | grid (threads) | read (MB) | written (MB) | HBM bytes per launch (MB) |\n|---|---|---|---|---|")
         for k, v in sorted(s["kernels"].items(), key=lambda kv: -kv[1]["bytes"]):
-            print(f"| `{k}` | {v['fetch_bytes'] / 1e6:.1f} | {v['write_bytes'] / 1e6:.1f} | {v['bytes'] / 1e6:.1f} |")
+            print(f"| `{k.split('@')[0]}` | {v['grid']} | {v['fetch_bytes'] / 1e6:.1f} | {v['write_bytes'] / 1e6:.1f} | {v['bytes'] / 1e6:.1f} |")
         if a.json:
             json.dump(s, open(a.json, "w"), indent=1)
 
