@@ -33,6 +33,7 @@ enum { KS_MODEL_THIN = 0, KS_MODEL_THERMAL = 1, KS_MODEL_POWERLAW = 2, KS_MODEL_
 typedef struct ks_ctx ks_ctx;
 typedef struct ks_cache ks_cache;
 typedef struct ks_model ks_model;
+typedef struct ks_binning ks_binning;
 
 /* Description of the last error on this thread (empty string if none). */
 const char* ks_last_error(void);
@@ -145,7 +146,9 @@ int ks_polarized_gradient(ks_cache* cache, const ks_model* model, const double* 
  * records the reverse pass needs, the observed images left on the device (and copied to image when
  * given, screen or sorted order). reverse!(dstokes, t, ν): the gradient of Σ dstokes · I_obs for
  * the same model, ADDED into dparams; dstokes NULL takes the seeds that ks_vis_chi2_seed or
- * ks_image_chi2_seed wrote on the device for every frame of the forward pass. */
+ * ks_image_chi2_seed wrote on the device for every frame of the forward pass (a reverse pass with
+ * dstokes replaces them). As in the reference, reverse! may run any number of times on one forward
+ * pass; a render (ks_render, ks_render_device) in between ends the forward pass (KS_ERR_STATE). */
 int ks_gradient_forward(ks_cache* cache, const ks_model* model, const double* times, int nt, const double* nus, int nnu,
                         double L, int nmax, double slab, double* image, int screen_order);
 int ks_gradient_reverse(ks_cache* cache, const ks_model* model, const double* dstokes, int screen_order, double* dparams);
@@ -181,6 +184,22 @@ int ks_vis_seed(ks_cache* cache, int64_t frame, double psize, const double* u, c
  * values, or 4 x npix when sigma_per_pixel; mask: npix bytes (nonzero = in χ²), NULL = all. */
 int ks_image_chi2_seed(ks_cache* cache, int64_t frame, const double* data, const double* sigma, int sigma_per_pixel,
                        const uint8_t* mask, double* chi2);
+
+/* Binning(pixel, size) - camera.jl:62-80: pixel integration of a camera whose points sub-sample
+ * pixels. Point m (screen order of the cache's camera) belongs to pixel pixel[m], a 0-based linear
+ * index i + n_alpha j into an n_alpha x n_beta screen; a pixel's value is the mean over its points
+ * (bin, camera.jl:91-98), and every pixel needs at least one point. */
+int ks_binning_create(ks_ctx* ctx, int64_t npoints, const int64_t* pixel, int n_alpha, int n_beta, ks_binning** binning);
+int ks_binning_destroy(ks_binning* binning);
+
+/* _frame_seed! through a Binning (Fit.jl:555-574): the χ² of the binned frame `frame` of the last
+ * ks_gradient_forward, Σ ((bin(I) - d) / σ)² over the masked pixels of the binned screen, returned in
+ * chi2, and its seed, 2 r / σ / count shared by the points of each pixel (0 outside the mask),
+ * written into the device seed of that frame. The binning's points must be the cache's pixels.
+ * data: 4 x npixels(binning), screen order of the binned screen; sigma: 4 values, or 4 per binned
+ * pixel when sigma_per_pixel; mask: one byte per binned pixel, NULL = all. */
+int ks_image_chi2_seed_binned(ks_cache* cache, int64_t frame, const ks_binning* binning, const double* data, const double* sigma,
+                              int sigma_per_pixel, const uint8_t* mask, double* chi2);
 
 #ifdef __cplusplus
 }

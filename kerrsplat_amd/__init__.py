@@ -18,7 +18,8 @@ is missing or no GPU is present.
 from .api import (Camera, Direct, Recurrence, Fused, GeodesicCache, Model, CompositeModel, regenerate, thin_image, polarized_image,
                   polarized_cube, polarized_gradient, march, precision, stage_ms, render_device, visibilities, unsort, to_screen, samples, residuals, KernelError,
                   NSPLATPARAMS, NPOLARIZEDPARAMS, NPOWERLAWPARAMS, NKAPPAPARAMS, gravitational_radius, grid_axis,
-                  library_path, gradient_forward, gradient_reverse, vis_chi2_seed, image_chi2_seed, vis_seed)
+                  library_path, gradient_forward, gradient_reverse, vis_chi2_seed, image_chi2_seed, vis_seed, tiles, Binning,
+                  bin, binned_grid, binned_polar, concatenate)
 from . import fit
 from .fit import (StokesMovie, FrameScans, chi2, chi2_gradient, sweep_passes, frame_chi2_seed, image_loss_gradient, closure_phases,
                   log_closure_amplitudes, ClosureData, closure_chi2, frame_chi2_closures, gain_factors, apply_gains)
@@ -29,4 +30,4 @@ __all__ = ["Camera", "Direct", "Recurrence", "Fused", "GeodesicCache", "Model", 
            "gravitational_radius", "grid_axis", "library_path", "gradient_forward", "gradient_reverse", "vis_chi2_seed",
            "image_chi2_seed", "fit", "StokesMovie", "FrameScans", "chi2", "chi2_gradient", "sweep_passes", "frame_chi2_seed",
            "image_loss_gradient", "vis_seed", "closure_phases", "log_closure_amplitudes", "ClosureData", "closure_chi2",
-           "frame_chi2_closures", "gain_factors", "apply_gains"]
+           "frame_chi2_closures", "gain_factors", "apply_gains", "tiles", "Binning", "bin", "binned_grid", "binned_polar", "concatenate"]

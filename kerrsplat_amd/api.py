@@ -81,6 +81,9 @@ def _load():
     lib.ks_gradient_reverse.argtypes = [_P, _P, _P, _i, _P]
     lib.ks_vis_chi2_seed.argtypes = [_P, _i64, _d, _P, _P, _P, _P, _P, _P, _i, _P, _P]
     lib.ks_image_chi2_seed.argtypes = [_P, _i64, _P, _P, _i, _P, _P]
+    lib.ks_binning_create.argtypes = [_P, _i64, _P, _i, _i, ctypes.POINTER(_P)]
+    lib.ks_binning_destroy.argtypes = [_P]
+    lib.ks_image_chi2_seed_binned.argtypes = [_P, _i64, _P, _P, _P, _i, _P, _P]
     lib.ks_vis_seed.argtypes = [_P, _i64, _d, _P, _P, _P, _P, _i, _i]
     lib.ks_march.argtypes = [_P]
     lib.ks_cache_precision.argtypes = [_P, _i]
@@ -158,6 +161,124 @@ class Camera:
     @property
     def npix(self):
         return len(self.alphas)
+
+
+def tiles(camera, ntiles):
+    """tiles(camera, ntiles) (cache.jl:234-246): `ntiles` cameras of consecutive pixels (screen order)
+    with the index range each covers, so that a screen too large for stored samples is regenerated
+    tile by tile with a smaller cache and the results written back into the full screen."""
+    n = camera.npix
+    bounds = np.round(np.linspace(0, n, int(ntiles) + 1)).astype(np.int64)
+    return [(Camera.points(camera.alphas[b0:b1], camera.betas[b0:b1]), range(int(b0), int(b1)))
+            for b0, b1 in zip(bounds[:-1], bounds[1:]) if b1 > b0]
+
+
+class Binning:
+    """Binning(pixel, size) (camera.jl:62-80): pixel integration of a camera whose points sub-sample
+    pixels. pixel[m] is the pixel of point m (screen order), a 0-based linear index i + n_alpha j into
+    a screen of shape size = (n_alpha, n_beta) (the reference's 1-based index minus one); a pixel's
+    value is the mean over its points (`bin`). Every pixel needs at least one point."""
+
+    def __init__(self, pixel, size):
+        self.pixel = np.asarray(pixel, dtype=np.int64).ravel()
+        self.size = (int(size[0]), int(size[1]))
+        npix = self.size[0] * self.size[1]
+        bad = (self.pixel < 0) | (self.pixel >= npix)
+        if bad.any():
+            raise ValueError(f"pixel index {int(self.pixel[bad][0])} outside 0:{npix - 1}")
+        self.count = np.bincount(self.pixel, minlength=npix)
+        if (self.count == 0).any():
+            raise ValueError("every pixel needs at least one point")
+        self._h = None
+
+    @property
+    def npixels(self):
+        return self.size[0] * self.size[1]
+
+    @property
+    def npoints(self):
+        return len(self.pixel)
+
+    def handle(self, device=0):
+        """The binning on the device (ks_binning_create), made on first use."""
+        if self._h is None:
+            self._lib = _load()
+            h = _P()
+            _check(self._lib.ks_binning_create(context(device), _i64(self.npoints), _ptr(self.pixel), _i(self.size[0]), _i(self.size[1]),
+                                               ctypes.byref(h)))
+            self._h = h
+        return self._h
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None) is not None:
+                self._lib.ks_binning_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+def bin(binning, values):
+    """bin(binning, values) (camera.jl:82-105): the mean of the values of each pixel's points. values
+    has one entry per point along its leading axis (screen order; trailing axes, e.g. Stokes, are kept),
+    or is a cube (npoints, 1, nt, nnu[, 4]) as polarized_cube returns for a point camera; returns
+    (n_alpha, n_beta, ...)."""
+    x = np.asarray(values, dtype=np.float64)
+    if x.ndim >= 2 and x.shape[:2] == (binning.npoints, 1):
+        x = x.reshape((binning.npoints,) + x.shape[2:])
+    if x.shape[0] != binning.npoints:
+        raise DimensionMismatch(f"expected {binning.npoints} values, got {x.shape[0]}")
+    acc = np.zeros((binning.npixels,) + x.shape[1:])
+    np.add.at(acc, binning.pixel, x)                  # in point order, as the reference's loop
+    acc /= binning.count.reshape((-1,) + (1,) * (x.ndim - 1))
+    na, nb = binning.size
+    return acc.reshape((nb, na) + x.shape[1:]).swapaxes(0, 1)
+
+
+def binned_grid(arange, brange, size, subsamples=1):
+    """binned_grid((alo, ahi), (blo, bhi), (na, nb); subsamples) (camera.jl:107-128): na x nb square pixels
+    covering the given edges, each integrated over subsamples^2 points at the centres of its sub-cells;
+    points listed pixel by pixel in screen order. Returns (camera, binning)."""
+    na, nb = (int(size), int(size)) if np.ndim(size) == 0 else (int(size[0]), int(size[1]))
+    K = int(subsamples)
+    if K < 1:
+        raise ValueError("subsamples must be at least 1")
+    da, db = (arange[1] - arange[0]) / na, (brange[1] - brange[0]) / nb
+    s = (np.arange(1, K + 1) - 0.5) / K
+    j, i, sj, si = np.meshgrid(np.arange(nb), np.arange(na), s, s, indexing="ij")     # j slowest, si fastest
+    al = arange[0] + (i + si) * da
+    be = brange[0] + (j + sj) * db
+    pixel = i + j * na
+    return Camera.points(al.ravel(), be.ravel()), Binning(pixel.ravel(), (na, nb))
+
+
+def binned_polar(rho_edges, npsi, subsamples=(1, 1), psi0=0.0):
+    """binned_polar(ρedges, nψ; subsamples, ψ0) (camera.jl:130-149): an annulus of polar pixels, radial
+    cells between consecutive edges and npsi azimuthal cells from psi0, pixel (iρ, iψ) at index
+    iρ + nρ iψ, each integrated over subsamples[0] x subsamples[1] (ρ, ψ) sub-cells (α = ρ cos ψ,
+    β = ρ sin ψ). Returns (camera, binning)."""
+    edges = np.asarray(rho_edges, dtype=np.float64)
+    nr = len(edges) - 1
+    if nr < 1:
+        raise ValueError("need at least two radial edges")
+    Kr, Kp = int(subsamples[0]), int(subsamples[1])
+    dpsi = 2 * np.pi / int(npsi)
+    ip, ir, sp, sr = np.meshgrid(np.arange(int(npsi)), np.arange(nr), np.arange(1, Kp + 1), np.arange(1, Kr + 1), indexing="ij")
+    rho = edges[ir] + (sr - 0.5) / Kr * (edges[ir + 1] - edges[ir])
+    psi = psi0 + (ip + (sp - 0.5) / Kp) * dpsi
+    return Camera.points((rho * np.cos(psi)).ravel(), (rho * np.sin(psi)).ravel()), Binning((ir + ip * nr).ravel(), (nr, int(npsi)))
+
+
+def concatenate(*parts):
+    """concatenate((camera, binning), ...) (camera.jl:151-164): the points and the pixels in order (the
+    pixels of the second screen follow those of the first), binned screen shape (npixels, 1)."""
+    al = np.concatenate([c.alphas for c, _ in parts])
+    be = np.concatenate([c.betas for c, _ in parts])
+    pixel, offset = [], 0
+    for _, b in parts:
+        pixel.append(b.pixel + offset)
+        offset += b.npixels
+    return Camera.points(al, be), Binning(np.concatenate(pixel), (offset, 1))
 
 
 class Direct:
@@ -499,12 +620,13 @@ def vis_chi2_seed(cache, frame, psize, u, v, vis, sigma, taper=None, gains=None,
     return float(chi[0])
 
 
-def image_chi2_seed(cache, frame, data, sigma, mask=None):
-    """_frame_seed! without a binning (Fit.jl:537-577) for frame `frame` of the last forward pass:
-    chi2 = sum ((I - d) / sigma)^2 over the masked pixels and the seed 2 r / sigma on the device.
-    data: (n_alpha, n_beta, 4) observed Stokes [cgs]; sigma: a 4-vector or (n_alpha, n_beta, 4);
-    mask: (n_alpha, n_beta) booleans or None."""
-    na, nb = cache.screen_size
+def image_chi2_seed(cache, frame, data, sigma, mask=None, binning=None):
+    """_frame_seed! (Fit.jl:537-577) for frame `frame` of the last forward pass: chi2 = sum ((I - d) /
+    sigma)^2 over the masked pixels and the seed 2 r / sigma on the device; with a Binning, I is the
+    binned image and each point receives its pixel's seed divided by the pixel's point count.
+    data: (n_alpha, n_beta, 4) observed Stokes [cgs] of the (binned) screen; sigma: a 4-vector or
+    (n_alpha, n_beta, 4); mask: (n_alpha, n_beta) booleans or None."""
+    na, nb = cache.screen_size if binning is None else binning.size
     dd = np.asarray(data, dtype=np.float64)
     if dd.shape != (na, nb, 4):
         raise DimensionMismatch(f"data must be {(na, nb, 4)}, got {dd.shape}")
@@ -519,7 +641,11 @@ def image_chi2_seed(cache, frame, data, sigma, mask=None):
             raise DimensionMismatch(f"mask must be {(na, nb)}, got {mm.shape}")
         mk = np.ascontiguousarray(mm.T.reshape(na * nb).astype(np.uint8))
     chi = np.zeros(1)
-    _check(cache._lib.ks_image_chi2_seed(cache._h, _i64(frame), _ptr(flat(dd)), _ptr(sgf), _i(int(per_pixel)), _ptr(mk), _ptr(chi)))
+    if binning is None:
+        _check(cache._lib.ks_image_chi2_seed(cache._h, _i64(frame), _ptr(flat(dd)), _ptr(sgf), _i(int(per_pixel)), _ptr(mk), _ptr(chi)))
+    else:
+        _check(cache._lib.ks_image_chi2_seed_binned(cache._h, _i64(frame), binning.handle(), _ptr(flat(dd)), _ptr(sgf), _i(int(per_pixel)),
+                                                    _ptr(mk), _ptr(chi)))
     return float(chi[0])
 
 

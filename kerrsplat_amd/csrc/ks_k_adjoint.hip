@@ -88,11 +88,13 @@ KS_DEV void step_apply_s(const S j4[4], const S a4[4], const S r3[3], S D, const
 }
 
 // Per (chunk, item): the tails T_{k+1} of the chunk's samples back to front from T_out[m], then the
-// adjoint front to back from w_in[m]: per sample the adjoints of its j, alpha, rho (written over them
-// in the record) and w <- O^T w (polarized_dual_kernel!'s ray loop, dual_gradient.jl:151-178).
+// adjoint front to back from w_in[m]: per sample the adjoints of its j, alpha, rho (into bar, beside the
+// records, which stay intact so that the reverse pass can run again on the same forward pass) and
+// w <- O^T w (polarized_dual_kernel!'s ray loop, dual_gradient.jl:151-178).
 template <class S>
-__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(Waves<S>::adjstep))) ks_adj_segment(int N, int64_t nitems, int nch, const double* __restrict__ wT, S* __restrict__ rec,
-                                                      const uint8_t* __restrict__ aflg, S* __restrict__ tails, Samples smp, int64_t npix) {
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(Waves<S>::adjstep))) ks_adj_segment(int N, int64_t nitems, int nch, const double* __restrict__ wT, const S* __restrict__ rec,
+                                                      S* __restrict__ bar, const uint8_t* __restrict__ aflg, S* __restrict__ tails, Samples smp,
+                                                      int64_t npix) {
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= nitems * nch) return;
     const int64_t item = slot_item(smp, tid % nitems, npix);     // wavefronts over compact patches of sky
@@ -116,14 +118,15 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(Waves<
     for (int k = kbeg; k <= kend; ++k) {
         const int64_t fo = (int64_t)(k - 1) * nitems + item;
         if (!aflg[fo]) continue;
-        S* o = rec + fo;
+        const S* o = rec + fo;
         S j4[4], a4[4], r3[3], R[4];
         for (int i = 0; i < 4; ++i) { j4[i] = o[(REC_J + i) * st]; a4[i] = o[(REC_A + i) * st]; R[i] = tails[(int64_t)i * st + fo]; }
         for (int i = 0; i < 3; ++i) r3[i] = o[(REC_RHO + i) * st];
         S Otw[4], jb[4], ab[4], rb[3];
         step_adjoint_s(j4, a4, r3, o[REC_D * st], w, R, Otw, jb, ab, rb);
-        for (int i = 0; i < 4; ++i) { o[(REC_BAR + i) * st] = jb[i]; o[(REC_BAR + 4 + i) * st] = ab[i]; }
-        for (int i = 0; i < 3; ++i) o[(REC_BAR + 8 + i) * st] = rb[i];
+        S* b = bar + fo;
+        for (int i = 0; i < 4; ++i) { b[i * st] = jb[i]; b[(4 + i) * st] = ab[i]; }
+        for (int i = 0; i < 3; ++i) b[(8 + i) * st] = rb[i];
         for (int q = 0; q < 4; ++q) w[q] = Otw[q];
     }
 }
@@ -172,7 +175,7 @@ template <class S>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(Waves<S>::adjelem))) ks_adj_splats(double a, double thetao, const double* __restrict__ soa, int64_t npix, int N, Samples smp,
                                                     const SplatCT<S>* __restrict__ sp, const SplatAdjT<S>* __restrict__ sadj, int ns,
                                                     const double* __restrict__ times, int nt, const double* __restrict__ nus, int64_t nitems,
-                                                    const S* __restrict__ rec, const uint8_t* __restrict__ aflg, int64_t nunits,
+                                                    const S* __restrict__ bar, const uint8_t* __restrict__ aflg, int64_t nunits,
                                                     const double* __restrict__ ub, double* __restrict__ part) {
     constexpr int WALK = 8;                 // samples tested per group, their loads issued together
     constexpr int QCAP = 1024;              // ring capacity: < 64 pending + 64 per sample of a group
@@ -242,9 +245,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(Waves<S
             S dl[13];
             const S Gw = weight_dlog(c, sa, S(s_tobs[l] - smp.t[so]), S(smp.x[so]), S(smp.y[so]), S(smp.z[so]), dl);
             if (Gw > 0.0) {
-                const S* o = rec + fo;
-                S bars[11];
-                for (int q = 0; q < 11; ++q) bars[q] = o[(REC_BAR + q) * st];
+                const S* o = bar + fo;
+                S bars[BAR_FIELDS];
+                for (int q = 0; q < BAR_FIELDS; ++q) bars[q] = o[q * st];
                 const uint8_t fl = smp.fl[so];
                 Geo Gd;
                 make_geo(a, smp.r[so], smp.th[so], s_eta[l], s_lam[l], (fl & 2) != 0, (fl & 4) != 0, Gd);
@@ -321,32 +324,33 @@ hipError_t launch_adj_combine(hipStream_t s, int64_t npix, int N, int nt, const 
     hipLaunchKernelGGL(ks_adj_combine, dim3(grid_for(items, 128)), dim3(128), 0, s, npix, N, nt, nus, nnu, seg, dstokes, wT, out);
     return hipGetLastError();
 }
-hipError_t launch_adj_segment(hipStream_t s, int N, int64_t nitems, const double* wT, double* rec, const uint8_t* aflg, double* tails,
-                              Samples smp, int64_t npix) {
+hipError_t launch_adj_segment(hipStream_t s, int N, int64_t nitems, const double* wT, const double* rec, double* bar, const uint8_t* aflg,
+                              double* tails, Samples smp, int64_t npix) {
     const int nch = (N + CHUNK - 1) / CHUNK;
-    hipLaunchKernelGGL(ks_adj_segment<double>, dim3(grid_for(nitems * nch, 128)), dim3(128), 0, s, N, nitems, nch, wT, rec, aflg, tails, smp, npix);
+    hipLaunchKernelGGL(ks_adj_segment<double>, dim3(grid_for(nitems * nch, 128)), dim3(128), 0, s, N, nitems, nch, wT, rec, bar, aflg, tails, smp,
+                       npix);
     return hipGetLastError();
 }
-hipError_t launch_adj_segment_f32(hipStream_t s, int N, int64_t nitems, const double* wT, f32* rec, const uint8_t* aflg, f32* tails,
-                                  Samples smp, int64_t npix) {
+hipError_t launch_adj_segment_f32(hipStream_t s, int N, int64_t nitems, const double* wT, const f32* rec, f32* bar, const uint8_t* aflg,
+                                  f32* tails, Samples smp, int64_t npix) {
     const int nch = (N + CHUNK - 1) / CHUNK;
-    hipLaunchKernelGGL(ks_adj_segment<f32>, dim3(grid_for(nitems * nch, 128)), dim3(128), 0, s, N, nitems, nch, wT, rec, aflg, tails, smp, npix);
+    hipLaunchKernelGGL(ks_adj_segment<f32>, dim3(grid_for(nitems * nch, 128)), dim3(128), 0, s, N, nitems, nch, wT, rec, bar, aflg, tails, smp, npix);
     return hipGetLastError();
 }
 hipError_t launch_adj_splats(hipStream_t s, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
                              const SplatC* sp, const SplatAdj* sadj, int ns, const double* times, int nt, const double* nus,
-                             int64_t nitems, const double* rec, const uint8_t* aflg, const double* ub, double* part) {
+                             int64_t nitems, const double* bar, const uint8_t* aflg, const double* ub, double* part) {
     const int64_t nunits = adj_units(nitems, N);
     hipLaunchKernelGGL(ks_adj_splats<double>, dim3((unsigned)((nunits + 7) / 8 * 8 * ns)), dim3(64), 0, s, a, thetao, soa, npix, N, smp, sp, sadj, ns, times,
-                       nt, nus, nitems, rec, aflg, nunits, ub, part);
+                       nt, nus, nitems, bar, aflg, nunits, ub, part);
     return hipGetLastError();
 }
 hipError_t launch_adj_splats_f32(hipStream_t s, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
                                  const SplatCf* sp, const SplatAdjf* sadj, int ns, const double* times, int nt, const double* nus,
-                                 int64_t nitems, const f32* rec, const uint8_t* aflg, const double* ub, double* part) {
+                                 int64_t nitems, const f32* bar, const uint8_t* aflg, const double* ub, double* part) {
     const int64_t nunits = adj_units(nitems, N);
     hipLaunchKernelGGL(ks_adj_splats<f32>, dim3((unsigned)((nunits + 7) / 8 * 8 * ns)), dim3(64), 0, s, a, thetao, soa, npix, N, smp, sp, sadj, ns, times,
-                       nt, nus, nitems, rec, aflg, nunits, ub, part);
+                       nt, nus, nitems, bar, aflg, nunits, ub, part);
     return hipGetLastError();
 }
 hipError_t launch_unit_bounds(hipStream_t s, int64_t npix, int N, Samples smp, const double* times, int nt, int64_t nitems,

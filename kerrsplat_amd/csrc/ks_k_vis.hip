@@ -215,6 +215,45 @@ __global__ void ks_image_residual(const double* __restrict__ out, const int* __r
     chi_p[s] = chi;
 }
 
+// _frame_seed! through a Binning (Fit.jl:555-574), per binned pixel q: the mean of its points' images (the
+// points in ascending screen order, as bin() adds them, camera.jl:91-98), the weighted residual, its χ² term
+// and the seed 2 r / sigma / count that each of its points receives. iperm: screen index -> sorted slot.
+__global__ void ks_bin_residual(const double* __restrict__ out, const int* __restrict__ iperm, int64_t npix, int64_t f,
+                                const int* __restrict__ off, const int* __restrict__ pts, int64_t nbins, const double* __restrict__ data,
+                                const double* __restrict__ sigma, int sigma_per_pixel, const uint8_t* __restrict__ mask,
+                                double* __restrict__ pseed, double* __restrict__ chi_p) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nbins) return;
+    const int b = off[q], e = off[q + 1];
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int m = b; m < e; ++m) {
+        const double* img = out + 4 * ((int64_t)iperm[pts[m]] + npix * f);
+        for (int k = 0; k < 4; ++k) acc[k] += img[k];
+    }
+    const double cnt = (double)(e - b);
+    double chi = 0.0;
+    if (!mask || mask[q]) {
+        const double* sg = sigma + (sigma_per_pixel ? 4 * q : 0);
+        for (int k = 0; k < 4; ++k) {
+            const double r = (acc[k] / cnt - data[4 * q + k]) / sg[k];
+            chi += r * r;
+            pseed[4 * q + k] = 2.0 * r / sg[k] / cnt;
+        }
+    } else {
+        for (int k = 0; k < 4; ++k) pseed[4 * q + k] = 0.0;
+    }
+    chi_p[q] = chi;
+}
+
+// each sorted slot takes the seed of the binned pixel its point belongs to
+__global__ void ks_bin_seed(const int* __restrict__ perm, const int* __restrict__ pix, int64_t npix, int64_t f,
+                            const double* __restrict__ pseed, double* __restrict__ ads) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= npix) return;
+    const int64_t q = pix[perm[s]];
+    for (int k = 0; k < 4; ++k) ads[4 * (s + npix * f) + k] = pseed[4 * q + k];
+}
+
 // ---- deterministic sum of n values by one block: strided per-thread sums, then a fixed tree ----------------------
 __global__ void __launch_bounds__(256) ks_sum_fixed(const double* __restrict__ x, int64_t n, double* __restrict__ out) {
     __shared__ double sh[256];
@@ -286,6 +325,17 @@ hipError_t launch_image_residual(hipStream_t s, const double* out, const int* pe
 }
 hipError_t launch_sum_fixed(hipStream_t s, const double* x, int64_t n, double* out) {
     hipLaunchKernelGGL(ks_sum_fixed, dim3(1), dim3(256), 0, s, x, n, out);
+    return hipGetLastError();
+}
+hipError_t launch_bin_residual(hipStream_t s, const double* out, const int* iperm, int64_t npix, int64_t f, const int* off, const int* pts,
+                               int64_t nbins, const double* data, const double* sigma, int sigma_per_pixel, const uint8_t* mask,
+                               double* pseed, double* chi_p) {
+    hipLaunchKernelGGL(ks_bin_residual, dim3(grid_for(nbins, 256)), dim3(256), 0, s, out, iperm, npix, f, off, pts, nbins, data, sigma,
+                       sigma_per_pixel, mask, pseed, chi_p);
+    return hipGetLastError();
+}
+hipError_t launch_bin_seed(hipStream_t s, const int* perm, const int* pix, int64_t npix, int64_t f, const double* pseed, double* ads) {
+    hipLaunchKernelGGL(ks_bin_seed, dim3(grid_for(npix, 256)), dim3(256), 0, s, perm, pix, npix, f, pseed, ads);
     return hipGetLastError();
 }
 

@@ -88,7 +88,7 @@ KS_DEV int64_t slot_item(const Samples& smp, int64_t t, int64_t npix) {
 // Adjoint records of a step-taking sample: rec[(f N + k - 1) nitems + item], f = the invariant j (4),
 // alpha (4), rho (3), Delta; the adjoint sweep writes j_bar, alpha_bar, rho_bar over j, alpha, rho.
 enum RecField { REC_J, REC_A = REC_J + 4, REC_RHO = REC_A + 4, REC_D = REC_RHO + 3, REC_FIELDS };
-constexpr int REC_BAR = REC_J;
+constexpr int BAR_FIELDS = 11;    // the adjoints of j (4), alpha (4), rho (3) of every step-taking sample
 
 // ---- ks_k_geo.hip ----
 hipError_t launch_root_case(hipStream_t s, double a, double thetao, const double* al, const double* be, int64_t npix, int8_t* numreals);
@@ -126,16 +126,16 @@ hipError_t launch_seg_combine(hipStream_t s, const double* soa, int64_t npix, in
 // ---- ks_k_adjoint.hip ----
 hipError_t launch_adj_combine(hipStream_t s, int64_t npix, int N, int nt, const double* nus, int nnu, const double* seg,
                               const double* dstokes, double* wT, double* out);
-hipError_t launch_adj_segment(hipStream_t s, int N, int64_t nitems, const double* wT, double* rec, const uint8_t* aflg, double* tails,
-                              Samples smp, int64_t npix);
-hipError_t launch_adj_segment_f32(hipStream_t s, int N, int64_t nitems, const double* wT, f32* rec, const uint8_t* aflg, f32* tails,
-                                  Samples smp, int64_t npix);
+hipError_t launch_adj_segment(hipStream_t s, int N, int64_t nitems, const double* wT, const double* rec, double* bar, const uint8_t* aflg,
+                              double* tails, Samples smp, int64_t npix);
+hipError_t launch_adj_segment_f32(hipStream_t s, int N, int64_t nitems, const double* wT, const f32* rec, f32* bar, const uint8_t* aflg,
+                                  f32* tails, Samples smp, int64_t npix);
 hipError_t launch_adj_splats_f32(hipStream_t s, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
                                  const SplatCf* sp, const SplatAdjf* sadj, int ns, const double* times, int nt, const double* nus,
-                                 int64_t nitems, const f32* rec, const uint8_t* aflg, const double* ub, double* part);
+                                 int64_t nitems, const f32* bar, const uint8_t* aflg, const double* ub, double* part);
 hipError_t launch_adj_splats(hipStream_t s, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
                              const SplatC* sp, const SplatAdj* sadj, int ns, const double* times, int nt, const double* nus,
-                             int64_t nitems, const double* rec, const uint8_t* aflg, const double* ub, double* part);
+                             int64_t nitems, const double* bar, const uint8_t* aflg, const double* ub, double* part);
 hipError_t launch_adj_reduce(hipStream_t s, const double* part, int64_t nunits, int ns, double* dparams);
 hipError_t launch_unit_bounds(hipStream_t s, int64_t npix, int N, Samples smp, const double* times, int nt, int64_t nitems,
                               const uint8_t* aflg, double* ub);
@@ -157,6 +157,10 @@ hipError_t launch_vis_scale_weights(hipStream_t s, int nuv, const double* w, con
 hipError_t launch_image_residual(hipStream_t s, const double* out, const int* perm, int64_t npix, int64_t f, const double* data,
                                  const double* sigma, int sigma_per_pixel, const uint8_t* mask, double* ads, double* chi_p);
 hipError_t launch_sum_fixed(hipStream_t s, const double* x, int64_t n, double* out);
+hipError_t launch_bin_residual(hipStream_t s, const double* out, const int* iperm, int64_t npix, int64_t f, const int* off, const int* pts,
+                               int64_t nbins, const double* data, const double* sigma, int sigma_per_pixel, const uint8_t* mask,
+                               double* pseed, double* chi_p);
+hipError_t launch_bin_seed(hipStream_t s, const int* perm, const int* pix, int64_t npix, int64_t f, const double* pseed, double* ads);
 
 inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 

@@ -81,6 +81,7 @@ struct ks_cache {
     int8_t* numreals = nullptr;
     int* perm = nullptr;
     int* tmap = nullptr;                               // wavefront grouping of the sorted pixels (8 x 8 screen tiles)
+    int* iperm = nullptr;                              // screen index -> sorted slot
     int* bcount = nullptr; int nblocks = 0;
     double* soa = nullptr;
     double *st = nullptr, *sr = nullptr, *sth = nullptr, *sph = nullptr; uint8_t* sfl = nullptr;
@@ -96,13 +97,14 @@ struct ks_cache {
     // per-baseline or per-pixel χ² terms and their sum, the image data, noise and mask
     Buf<double> vu, vv, vt, vg, vd, vs, vA2, vB2, vIm, vC2, vV, vwq, vE2, vSd, vSp, vchi, vsum, idata, isig;
     Buf<uint8_t> imask;
+    Buf<double> bseed;                                 // binned χ²: the seed of every binned pixel
     double* seg = nullptr; int64_t seg_cap = 0;        // per-segment transfer maps
     double* dtn = nullptr; int64_t dtn_cap = 0;        // observation times then frequencies (device)
     std::vector<double> htn;                           // their host staging copy (outlives the async copy)
     double* segres = nullptr;                          // per-segment anchor residuals (2 x nseg_max x npix)
     int* kstop = nullptr;                              // half-orbit cutoff per pixel
     // adjoint workspace (grow-only): records, flags, tails, per-segment (w_in, T_out), seeds, partial sums
-    double* arec = nullptr; uint8_t* aflg = nullptr; double* atail = nullptr; double* awT = nullptr; double* ads = nullptr;
+    double* arec = nullptr; double* abar = nullptr; uint8_t* aflg = nullptr; double* atail = nullptr; double* awT = nullptr; double* ads = nullptr;
     double* apart = nullptr; double* adpar = nullptr; double* aub = nullptr;
     int64_t a_items = 0, a_part = 0, a_dpar = 0, a_ub = 0;
     // the last forward pass of the dual sweep (ks_gradient_forward): what ks_gradient_reverse needs
@@ -110,6 +112,15 @@ struct ks_cache {
     int fw_nt = 0, fw_nnu = 0, fw_generation = -1, fw_precision = -1;
     std::vector<uint8_t> fw_seeded;                    // frames whose device seed a *_chi2_seed call wrote
     std::vector<int> perm_host;
+};
+
+// Binning (camera.jl:62-80) on the device: the points of every pixel as a list in ascending screen order
+// (off[q] .. off[q+1]), and the pixel of every point
+struct ks_binning {
+    ks_ctx* ctx = nullptr;
+    int64_t npoints = 0, nbins = 0;
+    int na = 0, nb = 0;
+    int* off = nullptr; int* pts = nullptr; int* pix = nullptr;
 };
 
 struct ks_model {
@@ -197,6 +208,7 @@ int ks_cache_create(ks_ctx* ctx, int64_t npix, int N, int store_samples, ks_cach
     int rc = 0;
     rc |= dalloc(&c->al, npix); rc |= dalloc(&c->be, npix); rc |= dalloc(&c->numreals, npix); rc |= dalloc(&c->perm, npix);
     rc |= dalloc(&c->tmap, npix);
+    rc |= dalloc(&c->iperm, npix);
     rc |= dalloc(&c->bcount, 3 * (size_t)c->nblocks);
     rc |= dalloc(&c->soa, (size_t)PF_COUNT * npix);
     rc |= dalloc(&c->rt, npix); rc |= dalloc(&c->rp, npix);
@@ -218,16 +230,16 @@ int ks_cache_create(ks_ctx* ctx, int64_t npix, int N, int store_samples, ks_cach
 int ks_cache_destroy(ks_cache* c) {
     if (!c) return KS_OK;
     (void)hipSetDevice(c->ctx->device);
-    dfree(c->al); dfree(c->be); dfree(c->numreals); dfree(c->perm); dfree(c->tmap); dfree(c->bcount); dfree(c->soa);
+    dfree(c->al); dfree(c->be); dfree(c->numreals); dfree(c->perm); dfree(c->tmap); dfree(c->iperm); dfree(c->bcount); dfree(c->soa);
     dfree(c->st); dfree(c->sr); dfree(c->sth); dfree(c->sph); dfree(c->sfl); dfree(c->rt); dfree(c->rp);
     dfree(c->sx); dfree(c->sy); dfree(c->sz); dfree(c->thin_part);
     dfree(c->out); dfree(c->out_screen); dfree(c->img1);
     for (Buf<double>* b : {&c->vu, &c->vv, &c->vt, &c->vg, &c->vd, &c->vs, &c->vA2, &c->vB2, &c->vIm, &c->vC2, &c->vV, &c->vwq, &c->vE2,
-                           &c->vSd, &c->vSp, &c->vchi, &c->vsum, &c->idata, &c->isig})
+                           &c->vSd, &c->vSp, &c->vchi, &c->vsum, &c->idata, &c->isig, &c->bseed})
         b->release();
     c->imask.release();
     dfree(c->seg); dfree(c->segres); dfree(c->kstop); dfree(c->dtn);
-    dfree(c->arec); dfree(c->aflg); dfree(c->atail); dfree(c->awT); dfree(c->ads); dfree(c->apart); dfree(c->adpar); dfree(c->aub);
+    dfree(c->arec); dfree(c->abar); dfree(c->aflg); dfree(c->atail); dfree(c->awT); dfree(c->ads); dfree(c->apart); dfree(c->adpar); dfree(c->aub);
     delete c;
     return KS_OK;
 }
@@ -318,6 +330,7 @@ int ks_regenerate(ks_cache* c, const double* alpha, const double* beta, int nalp
                 for (int ib = tb; ib < std::min(tb + 8, nbeta); ++ib)
                     for (int ia = ta; ia < std::min(ta + 8, nalpha); ++ia) tm.push_back(inv[ia + (int64_t)nalpha * ib]);
         HIPCHK(hipMemcpy(c->tmap, tm.data(), n * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->iperm, inv.data(), n * sizeof(int), hipMemcpyHostToDevice));
     }
     c->generation += 1;
     return KS_OK;
@@ -622,6 +635,7 @@ int ks_render_device(ks_cache* c, const ks_model* m, const double* times, int nt
     for (int i = 0; i < nnu; ++i) if (!(nus[i] > 0.0)) return fail(KS_ERR_ARG, "frequencies must be positive");
     ks_ctx* x = c->ctx;
     HIPCHK(hipSetDevice(x->device));
+    c->fw_model = nullptr;             // the render reuses the segment maps, times and images of a forward pass
     const int64_t frames = (int64_t)nt * nnu;
     if (ensure_out(c, frames)) return KS_ERR_HIP;
     const double* d_tn = stage_times(c, times, nt, nus, nnu);
@@ -659,10 +673,10 @@ int ks_render(ks_cache* c, const ks_model* m, const double* times, int nt, const
 static int ensure_adj(ks_cache* c, int64_t items, int ns) {
     const int nseg = (c->N + CHUNK - 1) / CHUNK;
     if (c->a_items < items) {
-        dfree(c->arec); dfree(c->aflg); dfree(c->atail); dfree(c->awT); dfree(c->ads);
+        dfree(c->arec); dfree(c->abar); dfree(c->aflg); dfree(c->atail); dfree(c->awT); dfree(c->ads);
         c->a_items = 0;
         const size_t per = (size_t)c->N * items;
-        if (dalloc(&c->arec, (size_t)REC_FIELDS * per) || dalloc(&c->aflg, per) || dalloc(&c->atail, 4 * per) ||
+        if (dalloc(&c->arec, (size_t)REC_FIELDS * per) || dalloc(&c->abar, (size_t)BAR_FIELDS * per) || dalloc(&c->aflg, per) || dalloc(&c->atail, 4 * per) ||
             dalloc(&c->awT, (size_t)8 * nseg * items) || dalloc(&c->ads, (size_t)4 * items))
             return KS_ERR_HIP;
         c->a_items = items;
@@ -737,6 +751,7 @@ int ks_gradient_reverse(ks_cache* c, const ks_model* m, const double* dstokes, i
             }
         HIPCHK(hipMemcpyAsync(c->ads, ds.data(), ds.size() * sizeof(double), hipMemcpyHostToDevice, x->stream));
         HIPCHK(hipStreamSynchronize(x->stream));     // ds is released on every return path below
+        std::fill(c->fw_seeded.begin(), c->fw_seeded.end(), 0);     // the device seeds are replaced
     } else {
         for (int64_t f = 0; f < frames; ++f)
             if (!c->fw_seeded[f]) return fail(KS_ERR_STATE, "frame %lld has no device seed: give dstokes or call a *_chi2_seed for every frame", (long long)f);
@@ -745,18 +760,19 @@ int ks_gradient_reverse(ks_cache* c, const ks_model* m, const double* dstokes, i
     HIPCHK(launch_adj_combine(x->stream, c->npix, c->N, nt, d_tn + nt, nnu, c->seg, c->ads, c->awT, nullptr));
     HIPCHK(x->stage_begin(KS_STAGE_ADJ_STEPS));
     const bool mixed = c->precision == KS_PRECISION_MIXED;
-    if (mixed) HIPCHK(launch_adj_segment_f32(x->stream, c->N, items, c->awT, (f32*)c->arec, c->aflg, (f32*)c->atail, samples_of(c), c->npix));
-    else HIPCHK(launch_adj_segment(x->stream, c->N, items, c->awT, c->arec, c->aflg, c->atail, samples_of(c), c->npix));
+    if (mixed) HIPCHK(launch_adj_segment_f32(x->stream, c->N, items, c->awT, (const f32*)c->arec, (f32*)c->abar, c->aflg, (f32*)c->atail, samples_of(c),
+                                             c->npix));
+    else HIPCHK(launch_adj_segment(x->stream, c->N, items, c->awT, c->arec, c->abar, c->aflg, c->atail, samples_of(c), c->npix));
     HIPCHK(x->stage_end(KS_STAGE_ADJ_STEPS));
     if (ns > 0) {
         HIPCHK(x->stage_begin(KS_STAGE_ADJ_ELEMENTS));
         HIPCHK(launch_unit_bounds(x->stream, c->npix, c->N, samples_of(c), d_tn, nt, items, c->aflg, c->aub));
         if (mixed)
             HIPCHK(launch_adj_splats_f32(x->stream, c->spin, c->thetao, c->soa, c->npix, c->N, samples_of(c), m->d_splats_f, m->d_adj_f, ns,
-                                         d_tn, nt, d_tn + nt, items, (const f32*)c->arec, c->aflg, c->aub, c->apart));
+                                         d_tn, nt, d_tn + nt, items, (const f32*)c->abar, c->aflg, c->aub, c->apart));
         else
             HIPCHK(launch_adj_splats(x->stream, c->spin, c->thetao, c->soa, c->npix, c->N, samples_of(c), m->d_splats, m->d_adj, ns, d_tn, nt,
-                                     d_tn + nt, items, c->arec, c->aflg, c->aub, c->apart));
+                                     d_tn + nt, items, c->abar, c->aflg, c->aub, c->apart));
         HIPCHK(x->stage_end(KS_STAGE_ADJ_ELEMENTS));
         HIPCHK(launch_adj_reduce(x->stream, c->apart, adj_units(items, c->N), ns, c->adpar));
     }
@@ -950,6 +966,70 @@ int ks_image_chi2_seed(ks_cache* c, int64_t frame, const double* data, const dou
     HIPCHK(launch_image_residual(x->stream, c->out, c->perm, c->npix, frame, c->idata.p, c->isig.p, sigma_per_pixel, mask ? c->imask.p : nullptr,
                                  c->ads, c->vchi.p));
     HIPCHK(launch_sum_fixed(x->stream, c->vchi.p, c->npix, c->vsum.p));
+    HIPCHK(hipMemcpyAsync(chi2, c->vsum.p, sizeof(double), hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    c->fw_seeded[frame] = 1;
+    return KS_OK;
+}
+
+int ks_binning_create(ks_ctx* ctx, int64_t npoints, const int64_t* pixel, int n_alpha, int n_beta, ks_binning** out) {
+    if (!ctx || !out || (npoints > 0 && !pixel)) return fail(KS_ERR_ARG, "ks_binning_create: null argument");
+    if (n_alpha < 1 || n_beta < 1) return fail(KS_ERR_ARG, "binned screen %d x %d", n_alpha, n_beta);
+    const int64_t nbins = (int64_t)n_alpha * n_beta;
+    if (npoints > (int64_t)1 << 24 || nbins > npoints) return fail(KS_ERR_DIM, "%lld points for %lld pixels", (long long)npoints, (long long)nbins);
+    std::vector<int> off(nbins + 1, 0), pts(npoints), pix(npoints);
+    for (int64_t m = 0; m < npoints; ++m) {
+        if (pixel[m] < 0 || pixel[m] >= nbins) return fail(KS_ERR_ARG, "pixel index %lld outside 0:%lld", (long long)pixel[m], (long long)(nbins - 1));
+        off[pixel[m] + 1] += 1;
+        pix[m] = (int)pixel[m];
+    }
+    for (int64_t q = 0; q < nbins; ++q) {
+        if (off[q + 1] == 0) return fail(KS_ERR_ARG, "every pixel needs at least one point (pixel %lld has none)", (long long)q);
+        off[q + 1] += off[q];
+    }
+    std::vector<int> fill(off.begin(), off.end() - 1);
+    for (int64_t m = 0; m < npoints; ++m) pts[fill[pix[m]]++] = (int)m;       // ascending m within each pixel
+    HIPCHK(hipSetDevice(ctx->device));
+    ks_binning* b = new ks_binning;
+    b->ctx = ctx; b->npoints = npoints; b->nbins = nbins; b->na = n_alpha; b->nb = n_beta;
+    if (dalloc(&b->off, off.size()) || dalloc(&b->pts, pts.size()) || dalloc(&b->pix, pix.size())) { ks_binning_destroy(b); return KS_ERR_HIP; }
+    HIPCHK(hipMemcpy(b->off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->pts, pts.data(), pts.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->pix, pix.data(), pix.size() * sizeof(int), hipMemcpyHostToDevice));
+    *out = b;
+    return KS_OK;
+}
+
+int ks_binning_destroy(ks_binning* b) {
+    if (!b) return KS_OK;
+    (void)hipSetDevice(b->ctx->device);
+    dfree(b->off); dfree(b->pts); dfree(b->pix);
+    delete b;
+    return KS_OK;
+}
+
+int ks_image_chi2_seed_binned(ks_cache* c, int64_t frame, const ks_binning* b, const double* data, const double* sigma, int sigma_per_pixel,
+                              const uint8_t* mask, double* chi2) {
+    if (!c || !b || !data || !sigma || !chi2) return fail(KS_ERR_ARG, "ks_image_chi2_seed_binned: null argument");
+    if (b->ctx != c->ctx) return fail(KS_ERR_ARG, "the binning belongs to another context");
+    if (b->npoints != c->npix) return fail(KS_ERR_DIM, "binning has %lld points, cache has %lld pixels", (long long)b->npoints, (long long)c->npix);
+    if (int rc = seed_frame_ok(c, frame)) return rc;
+    const int64_t nq = b->nbins;
+    const int64_t nsig = sigma_per_pixel ? 4 * nq : 4;
+    for (int64_t k = 0; k < nsig; ++k)
+        if (!(sigma[k] > 0.0) && (!mask || !sigma_per_pixel || mask[k / 4])) return fail(KS_ERR_ARG, "sigma must be positive (entry %lld)", (long long)k);
+    ks_ctx* x = c->ctx;
+    HIPCHK(hipSetDevice(x->device));
+    if (c->idata.grow((size_t)4 * nq) || c->isig.grow((size_t)nsig) || c->vchi.grow(nq) || c->vsum.grow(1) || c->bseed.grow((size_t)4 * nq))
+        return KS_ERR_HIP;
+    if (mask && c->imask.grow(nq)) return KS_ERR_HIP;
+    HIPCHK(hipMemcpyAsync(c->idata.p, data, (size_t)4 * nq * sizeof(double), hipMemcpyHostToDevice, x->stream));
+    HIPCHK(hipMemcpyAsync(c->isig.p, sigma, (size_t)nsig * sizeof(double), hipMemcpyHostToDevice, x->stream));
+    if (mask) HIPCHK(hipMemcpyAsync(c->imask.p, mask, (size_t)nq, hipMemcpyHostToDevice, x->stream));
+    HIPCHK(launch_bin_residual(x->stream, c->out, c->iperm, c->npix, frame, b->off, b->pts, nq, c->idata.p, c->isig.p, sigma_per_pixel,
+                               mask ? c->imask.p : nullptr, c->bseed.p, c->vchi.p));
+    HIPCHK(launch_bin_seed(x->stream, c->perm, b->pix, c->npix, frame, c->bseed.p, c->ads));
+    HIPCHK(launch_sum_fixed(x->stream, c->vchi.p, nq, c->vsum.p));
     HIPCHK(hipMemcpyAsync(chi2, c->vsum.p, sizeof(double), hipMemcpyDeviceToHost, x->stream));
     HIPCHK(hipStreamSynchronize(x->stream));
     c->fw_seeded[frame] = 1;

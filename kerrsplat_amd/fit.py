@@ -49,12 +49,15 @@ def _frames(movie, frames, freqs):
     return ks, ls
 
 
-def chi2(params, movie, cache, L, frames=None, freqs=None, nmax=-1, slab=0.0):
-    """chi2(params, movie, cache, L; frames, freqs, nmax, slab) (Fit.jl:66-93): Σ ((model - data)/σ)²
-    over the masked pixels of the selected frames and frequencies; the model cube is rendered in one
-    launch on the device (the cache's precision policy applies), the sum is taken on the host."""
+def chi2(params, movie, cache, L, frames=None, freqs=None, nmax=-1, slab=0.0, binning=None):
+    """chi2(params, movie, cache, L; frames, freqs, nmax, slab, binning) (Fit.jl:66-93): Σ ((model -
+    data)/σ)² over the masked pixels of the selected frames and frequencies, the model integrated over
+    the pixels of a Binning when one is given (pixel_stokes, Fit.jl:73-75); the model cube is rendered
+    in one launch on the device (the cache's precision policy applies), the sum is taken on the host."""
     ks, ls = _frames(movie, frames, freqs)
     cube = api.polarized_cube(cache, params, movie.times[ks], movie.nus[ls], L, nmax=nmax, slab=slab)
+    if binning is not None:
+        cube = api.bin(binning, cube)
     total = 0.0
     for b, l in enumerate(ls):
         for a, k in enumerate(ks):
@@ -86,11 +89,12 @@ def sweep_passes(dparams, cache, params, L, nmax=-1, slab=0.0):
     return forward, reverse
 
 
-def chi2_gradient(dparams, params, movie, cache, L, frames=None, freqs=None, nmax=-1, slab=0.0, batch_frames=4):
-    """chi2_gradient!(dparams, params, movie, cache, L; frames, freqs, nmax, slab, batch_frames)
+def chi2_gradient(dparams, params, movie, cache, L, frames=None, freqs=None, nmax=-1, slab=0.0, batch_frames=4, binning=None):
+    """chi2_gradient!(dparams, params, movie, cache, L; frames, freqs, nmax, slab, batch_frames, binning)
     (Fit.jl:476-534): the movie χ² and, added into dparams, its gradient with respect to the thermal
     splat parameters. Per frequency and batch of frames: one forward launch, the image χ² and its
-    seeds 2 r / σ² on the device (_frame_seed!, Fit.jl:537-577), one reverse launch."""
+    seeds 2 r / σ² on the device (_frame_seed!, Fit.jl:537-577, through the Binning when given), one
+    reverse launch."""
     m = params if isinstance(params, Model) else Model(params)
     ks, ls = _frames(movie, frames, freqs)
     total = 0.0
@@ -99,7 +103,7 @@ def chi2_gradient(dparams, params, movie, cache, L, frames=None, freqs=None, nma
             chunk = ks[c0:c0 + max(int(batch_frames), 1)]
             api.gradient_forward(cache, m, movie.times[chunk], [movie.nus[l]], L, nmax=nmax, slab=slab, copy=False)
             for c, k in enumerate(chunk):
-                total += api.image_chi2_seed(cache, c, movie.data[:, :, k, l, :], movie.noise(k, l), movie.mask[:, :, k, l])
+                total += api.image_chi2_seed(cache, c, movie.data[:, :, k, l, :], movie.noise(k, l), movie.mask[:, :, k, l], binning=binning)
             api.gradient_reverse(cache, m, dparams, None)
     return total
 
@@ -127,19 +131,26 @@ def frame_chi2_seed(cache, c, fs, dalpha, L, D, return_model=False):
     return api.vis_chi2_seed(cache, c, psize, fs.u, fs.v, fs.vis, fs.sigma, taper=fs.tap, gains=fs.gains, return_model=return_model)
 
 
-def image_loss_gradient(dparams, loss_and_grad, cache, params, t_obs, nu, L, nmax=-1, slab=0.0):
-    """image_loss_gradient!(dparams, loss, cache, params, t_obs, ν, L) (Fit.jl:586-615) with the
+def image_loss_gradient(dparams, loss_and_grad, cache, params, t_obs, nu, L, nmax=-1, slab=0.0, binning=None):
+    """image_loss_gradient!(dparams, loss, cache, params, t_obs, ν, L; binning) (Fit.jl:586-615) with the
     image derivative supplied by the caller: loss_and_grad(image) takes the screen-shaped observed
-    image (n_alpha, n_beta, 4) and returns (value, d value / d image); the reverse pass on the device
-    turns that seed into d loss / d params, added into dparams. Returns the loss value."""
+    image (n_alpha, n_beta, 4), binned over the pixels of a Binning when one is given, and returns
+    (value, d value / d image); the reverse pass on the device turns that seed into d loss / d params,
+    added into dparams. Returns the loss value."""
     m = params if isinstance(params, Model) else Model(params)
     sorted_img = api.gradient_forward(cache, m, [t_obs], [nu], L, nmax=nmax, slab=slab)[0, 0]
     na, nb = cache.screen_size
     perm = cache.perm_host - 1                      # screen index of every sorted pixel
     screen = np.zeros((cache.npix, 4))
     screen[perm] = sorted_img
-    value, dimg = loss_and_grad(np.transpose(screen.reshape(nb, na, 4), (1, 0, 2)))
-    dflat = np.transpose(np.asarray(dimg, dtype=np.float64), (1, 0, 2)).reshape(cache.npix, 4)
+    if binning is None:
+        value, dimg = loss_and_grad(np.transpose(screen.reshape(nb, na, 4), (1, 0, 2)))
+        dflat = np.transpose(np.asarray(dimg, dtype=np.float64), (1, 0, 2)).reshape(cache.npix, 4)
+    else:
+        value, dimg = loss_and_grad(api.bin(binning, screen))
+        bna, bnb = binning.size
+        dpix = np.transpose(np.asarray(dimg, dtype=np.float64), (1, 0, 2)).reshape(bna * bnb, 4)
+        dflat = dpix[binning.pixel] / binning.count[binning.pixel][:, None]      # the mean's adjoint
     api.gradient_reverse(cache, m, dparams, dflat[perm][None, None])
     return value
 

@@ -122,6 +122,75 @@ def test_image_chi2_seed_and_movie_gradient(oracle):
     assert np.abs(dp1 - dp).max() <= 1e-12 * np.abs(dp).max()
 
 
+def test_binned_chi2_seed_and_movie_gradient():
+    """_frame_seed! through a Binning on the device (2 x 2 points per pixel) against the host restatement
+    (tests/test_binning.py, pinned by central differences): the movie χ² against the host χ² of the binned
+    cube, the gradient against the dual sweep seeded on the host; image_loss_gradient through the binning."""
+    from test_binning import frame_seed_binned
+    cam, b = K.binned_grid((-15.0, 15.0), (-15.0, 15.0), (8, 8), subsamples=2)
+    cache = K.GeodesicCache(cam, 128, store_samples=True)
+    K.regenerate(cache, A, THO, marcher=K.Recurrence(64))
+    L = syn.length_unit()
+    P = syn.thermal_splats(1, A, 3)
+    P[12, :] += math.log(30.0)
+    m = K.Model(P)
+    times = np.array([0.0, 25.0])
+    truth = K.bin(b, K.polarized_cube(cache, P * 1.0 + 0.02, times, [NU], L))       # (8, 8, 2, 1, 4)
+    rng = np.random.default_rng(12)
+    sigma = 0.05 * np.abs(truth[..., 0]).max() * rng.uniform(0.5, 1.5, size=truth.shape)
+    data = truth + sigma * rng.normal(size=truth.shape)
+    mask = rng.uniform(size=truth.shape[:4]) > 0.2
+    movie = K.StokesMovie(data, times, [NU], sigma, mask)
+    dp = np.zeros(P.shape, order="F")
+    chi = K.chi2_gradient(dp, m, movie, cache, L, batch_frames=2, binning=b)
+    chi_ref = K.chi2(m, movie, cache, L, binning=b)
+    assert abs(chi - chi_ref) <= 1e-12 * chi_ref
+    cube = K.polarized_cube(cache, m, times, [NU], L)                                 # (256, 1, 2, 1, 4)
+    seeds = np.zeros(cube.shape)
+    chi_host = 0.0
+    for it in range(2):
+        c_, s_ = frame_seed_binned(cube[:, 0, it, 0], screen_flat(data[:, :, it, 0]), screen_flat(sigma[:, :, it, 0]),
+                                   mask[:, :, it, 0].T.ravel(), b)
+        seeds[:, 0, it, 0] = s_
+        chi_host += c_
+    assert abs(chi - chi_host) <= 1e-12 * chi_host
+    dp_ref, _ = K.polarized_gradient(cache, m, list(times), [NU], L, seeds)
+    assert np.abs(dp - dp_ref).max() <= 1e-9 * np.abs(dp_ref).max()
+    # a linear loss of the binned image: its seed through the binning is w[pixel] / count per point
+    w = rng.normal(size=(8, 8, 4))
+    dl = np.zeros(P.shape, order="F")
+    val = K.image_loss_gradient(dl, lambda img: (float(np.sum(w * img)), w), cache, m, 0.0, NU, L, binning=b)
+    assert abs(val - np.sum(w * K.bin(b, cube[:, :, 0, 0]))) <= 1e-12 * abs(val)
+    pw = screen_flat(w)[b.pixel] / b.count[b.pixel][:, None]
+    dl_ref, _ = K.polarized_gradient(cache, m, [0.0], [NU], L, pw[:, None, None, None, :])
+    assert np.abs(dl - dl_ref).max() <= 1e-9 * np.abs(dl_ref).max()
+
+
+def test_binning_of_one_point_per_pixel_is_the_unbinned_seed():
+    na = 10
+    cam, cache = stored_cache(na, 96)
+    L = syn.length_unit()
+    m = K.Model(syn.thermal_splats(1, A, 2))
+    img = K.polarized_cube(cache, m, [0.0], [NU], L)[:, :, 0, 0]
+    rng = np.random.default_rng(4)
+    data, sigma = img * rng.uniform(0.9, 1.1, size=img.shape), 0.1 * np.abs(img[..., 0]).max() * np.ones(4)
+    K.gradient_forward(cache, m, [0.0], [NU], L, copy=False)
+    b = K.Binning(np.arange(na * na), (na, na))
+    chi_b = K.image_chi2_seed(cache, 0, data, sigma, binning=b)
+    g_b = np.zeros((21, 2), order="F")
+    K.gradient_reverse(cache, m, g_b, None)
+    chi = K.image_chi2_seed(cache, 0, data, sigma)
+    g = np.zeros((21, 2), order="F")
+    K.gradient_reverse(cache, m, g, None)
+    g2 = np.zeros((21, 2), order="F")
+    K.gradient_reverse(cache, m, g2, None)                         # reverse! again on the same forward pass
+    assert np.all(np.isfinite(g)) and np.array_equal(g2, g)
+    assert abs(chi_b - chi) <= 1e-13 * chi                       # r / sigma against r * (1 / sigma): rounding only
+    assert np.abs(g_b - g).max() <= 1e-12 * np.abs(g).max()
+    with pytest.raises(K.api.DimensionMismatch, match="points"):
+        K.image_chi2_seed(cache, 0, data[:5, :5], sigma, binning=K.Binning(np.arange(25), (5, 5)))
+
+
 def test_forward_reverse_equal_polarized_gradient():
     na = 10
     cam, cache = stored_cache(na, 96)
@@ -157,6 +226,12 @@ def test_seed_errors():
     other = K.Model(syn.thermal_splats(1, A, 1))
     with pytest.raises(ValueError, match="model of its forward"):
         K.gradient_reverse(cache, other, dp, np.zeros((1, 2, cache.npix, 4)))
+    K.gradient_reverse(cache, m, dp, np.zeros((1, 2, cache.npix, 4)))     # host seeds replace the device seeds
+    with pytest.raises(ValueError, match="no device seed"):
+        K.gradient_reverse(cache, m, dp)
+    K.polarized_image(cache, m, 0.0, NU, L)                               # a render ends the forward pass
+    with pytest.raises(K.api.StateError, match="forward"):
+        K.gradient_reverse(cache, m, dp, np.zeros((1, 2, cache.npix, 4)))
 
 
 def test_closure_seed_vs_oracle(oracle):
