@@ -185,6 +185,16 @@ int ks_vis_seed(ks_cache* cache, int64_t frame, double psize, const double* u, c
 int ks_image_chi2_seed(ks_cache* cache, int64_t frame, const double* data, const double* sigma, int sigma_per_pixel,
                        const uint8_t* mask, double* chi2);
 
+/* spacetime_valgrad / spacetime_jacobian - joint.jl:74-154: the observed Stokes vectors of the cache's
+ * camera at its spin and inclination for every (time, frequency), and their forward-mode partials with
+ * respect to x = (a, theta_o, ln L): three-partial duals seeded on the spin, the inclination and the log
+ * length unit, carried through the pixel constants, the closed-form (Direct) samples tau_k = k tau_total /
+ * (N + 1) and the polarized transport. out_stokes: 4 x npix x nt x nnu, screen order, frame it + nt*inu;
+ * out_tangent: 3 x 4 x npix x nt x nnu (partial fastest). Thermal, power-law, kappa and composite models.
+ * Currently returns KS_ERR_STATE without launching: the dual kernel is disabled (see ks_lib.hip). */
+int ks_spacetime_tangent(ks_cache* cache, const ks_model* model, const double* times, int nt, const double* nus, int nnu,
+                         double L, int nmax, double slab, double* out_stokes, double* out_tangent);
+
 /* Binning(pixel, size) - camera.jl:62-80: pixel integration of a camera whose points sub-sample
  * pixels. Point m (screen order of the cache's camera) belongs to pixel pixel[m], a 0-based linear
  * index i + n_alpha j into an n_alpha x n_beta screen; a pixel's value is the mean over its points

@@ -70,6 +70,7 @@ def _load():
     lib.ks_model_create.argtypes = [_P, _i, _i, _i, _P, _d, _P, _P, _i, ctypes.POINTER(_P)]
     lib.ks_model_composite.argtypes = [_P, _i, _P, ctypes.POINTER(_P)]
     lib.ks_render.argtypes = [_P, _P, _P, _i, _P, _i, _d, _i, _d, _P, _i]
+    lib.ks_spacetime_tangent.argtypes = [_P, _P, _P, _i, _P, _i, _d, _i, _d, _P, _P]
     lib.ks_render_device.argtypes = [_P, _P, _P, _i, _P, _i, _d, _i, _d]
     lib.ks_copy_stokes.argtypes = [_P, _i64, _P, _i]
     lib.ks_thin_image.argtypes = [_P, _P, _d, _P, _i]
@@ -463,6 +464,25 @@ def render_device(cache, model, times, nus, L, nmax=-1, slab=0.0):
     """The fused polarized launch with the result left on the device (ks_render_device)."""
     t, n = _f64(np.atleast_1d(times)), _f64(np.atleast_1d(nus))
     _check(cache._lib.ks_render_device(cache._h, model._h, _ptr(t), _i(len(t)), _ptr(n), _i(len(n)), _d(L), _i(nmax), _d(slab)))
+
+
+def spacetime_tangent(cache, params, times, nus, L, nmax=-1, slab=0.0, kind="thermal", screen=True):
+    """The observed Stokes cube of the cache's camera at its spin and inclination and its forward-mode partials
+    with respect to x = (a, theta_o, ln L) (the dual cache of spacetime_valgrad / spacetime_jacobian,
+    joint.jl:74-154): three-partial duals through the pixel constants, the closed-form samples and the
+    transport. Returns (cube, dcube): (n_alpha, n_beta, nt, nnu, 4) and (n_alpha, n_beta, nt, nnu, 4, 3) with
+    screen=True, (nnu, nt, npix, 4[, 3]) in screen pixel order otherwise."""
+    m = _as_model(params, kind)
+    t, n = _f64(np.atleast_1d(times)), _f64(np.atleast_1d(nus))
+    out = np.zeros((len(n), len(t), cache.npix, 4))
+    dout = np.zeros((len(n), len(t), cache.npix, 4, 3))
+    _check(cache._lib.ks_spacetime_tangent(cache._h, m._h, _ptr(t), _i(len(t)), _ptr(n), _i(len(n)), _d(L), _i(nmax), _d(slab),
+                                           _ptr(out), _ptr(dout)))
+    if not screen:
+        return out, dout
+    na, nb = cache.screen_size
+    return (np.transpose(out.reshape(len(n), len(t), nb, na, 4), (3, 2, 1, 0, 4)),
+            np.transpose(dout.reshape(len(n), len(t), nb, na, 4, 3), (3, 2, 1, 0, 4, 5)))
 
 
 def polarized_cube(cache, params, times, nus, L, nmax=-1, slab=0.0, kind="thermal", screen=True):

@@ -165,6 +165,16 @@ KSDD_T KS_DEV DN<N, S> acos(const DN<N, S>& x) {
     return chain(acos(x.v), s > 0.0 ? S(-1.0 / sqrt(s)) : S(0.0), x);
 }
 KSDD_T KS_DEV DN<N, S> fabs(const DN<N, S>& x) { return x.v < 0.0 ? -x : x; }
+// (the base-scalar calls are qualified: these names have no f32 overload here to keep ::atan etc. visible)
+KSDD_T KS_DEV DN<N, S> atan(const DN<N, S>& x) { return chain(::atan(x.v), S(1.0 / (1.0 + x.v * x.v)), x); }
+KSDD_T KS_DEV DN<N, S> asin(const DN<N, S>& x) { return chain(::asin(x.v), S(1.0 / sqrt(1.0 - x.v * x.v)), x); }
+KSDD_T KS_DEV DN<N, S> tan(const DN<N, S>& x) { const S t = ::tan(x.v); return chain(t, S(1.0 + t * t), x); }
+KSDD_T KS_DEV DN<N, S> floor(const DN<N, S>& x) { return DN<N, S>(::floor(x.v)); }
+KSDD_T KS_DEV DN<N, S> scale2(const DN<N, S>& x, int e) { return x * ldexp(1.0, e); }
+KSDD_T KS_DEV DN<N, S> copysign(const DN<N, S>& a, const DN<N, S>& b) { return (signbit(a.v) != signbit(b.v)) ? -a : a; }
+KSDD_T KS_DEV DN<N, S>& operator-=(DN<N, S>& a, const DN<N, S>& b) { a = a - b; return a; }
+KSDD_T KS_DEV DN<N, S>& operator/=(DN<N, S>& a, const DN<N, S>& b) { a = a / b; return a; }
+KSDD_T KS_DEV bool isfinite_(const DN<N, S>& x) { return isfinite_(x.v); }
 KSDD_T KS_DEV DN<N, S> atan2(const DN<N, S>& y, const DN<N, S>& x) {
     const S r2 = x.v * x.v + y.v * y.v;
     DN<N, S> r; r.v = atan2(y.v, x.v);
@@ -174,6 +184,17 @@ KSDD_T KS_DEV DN<N, S> atan2(const DN<N, S>& y, const DN<N, S>& x) {
 KSDD_T KS_DEV DN<N, S> pow(const DN<N, S>& x, double y) {
     const S p = pow(x.v, y);
     return chain(p, y == 0.0 ? S(0.0) : S(y * pow(x.v, y - 1.0)), x);
+}
+KSDD_T KS_DEV DN<N, S> pow(const DN<N, S>& x, const DN<N, S>& y) {
+    const S p = ::pow(x.v, y.v);
+    const S dx = y.v == 0.0 ? S(0.0) : S(y.v * ::pow(x.v, y.v - 1.0)), dy = x.v > 0.0 ? S(p * ::log(x.v)) : S(0.0);
+    DN<N, S> r; r.v = p;
+    KSDD_LOOP r.d[i] = dx * x.d[i] + dy * y.d[i];
+    return r;
+}
+KSDD_T KS_DEV DN<N, S> pow(double x, const DN<N, S>& y) {
+    const S p = ::pow(x, y.v);
+    return chain(p, x > 0.0 ? S(p * ::log(x)) : S(0.0), y);
 }
 KSDD_T KS_DEV DN<N, S> dmax(const DN<N, S>& a, const DN<N, S>& b) { return a.v > b.v ? a : b; }
 KSDD_T KS_DEV DN<N, S> dmin(const DN<N, S>& a, const DN<N, S>& b) { return a.v < b.v ? a : b; }

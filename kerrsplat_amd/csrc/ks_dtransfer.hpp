@@ -49,10 +49,11 @@ using SplatC = SplatCT<double>;
 using SplatCf = SplatCT<f32>;
 
 // ---- geometry ---------------------------------------------------------------------------------------------
-KS_DEV void quasi_cartesian_ks(double a, double rp, double rm, double r, double th, double ph, double& x, double& y, double& z) {
-    double temp = sqrt(1.0 - a * a);
-    double phks = ph + a / (2.0 * temp) * log(fabs((r - rp + EPS) / (r - rm + EPS))) - atan2(a, r);
-    double st, ct, sp, cp;
+template <class Q>
+KS_DEV void quasi_cartesian_ks(Q a, Q rp, Q rm, Q r, Q th, Q ph, Q& x, Q& y, Q& z) {
+    Q temp = sqrt(1.0 - a * a);
+    Q phks = ph + a / (2.0 * temp) * log(fabs((r - rp + EPS) / (r - rm + EPS))) - atan2(a, r);
+    Q st, ct, sp, cp;
     sincos(th, &st, &ct);
     sincos(phks, &sp, &cp);
     x = r * st * cp; y = r * st * sp; z = r * ct;
@@ -111,7 +112,8 @@ template <class T> struct GeoT {   // everything of a sample the splats share
     T pz[4];        // ZAMO components (t, r, phi, -theta)
 };
 using Geo = GeoT<double>;
-KS_DEV void make_geo(double a, double r, double th, double eta, double lam, bool nur, bool nuth, Geo& g) {
+template <class Q>
+KS_DEV void make_geo(Q a, Q r, Q th, Q eta, Q lam, bool nur, bool nuth, GeoT<Q>& g) {
     g.r = r; g.th = th;
     sincos(th, &g.st, &g.ct);
     g.S = r * r + a * a * g.ct * g.ct;
@@ -122,14 +124,14 @@ KS_DEV void make_geo(double a, double r, double th, double eta, double lam, bool
     g.sgrr = sqrt(g.S / g.Dl);
     g.sgthth = sqrt(g.S);
     g.sgpp = g.st * sqrt(g.Xi / g.S);
-    double w = r * r + a * a - a * lam;
-    double R = w * w - g.Dl * (eta + (lam - a) * (lam - a));
-    double Th = eta + a * a * g.ct * g.ct - lam * lam * g.ct * g.ct / (g.st * g.st);
-    double pr = (nur ? 1.0 : -1.0) * (R > 0.0 ? sqrt(R) : 0.0) / g.Dl;
-    double pth = (nuth ? 1.0 : -1.0) * (Th > 0.0 ? sqrt(Th) : 0.0);
+    Q w = r * r + a * a - a * lam;
+    Q R = w * w - g.Dl * (eta + (lam - a) * (lam - a));
+    Q Th = eta + a * a * g.ct * g.ct - lam * lam * g.ct * g.ct / (g.st * g.st);
+    Q pr = (nur ? 1.0 : -1.0) * (R > 0.0 ? sqrt(R) : Q(0.0)) / g.Dl;
+    Q pth = (nuth ? 1.0 : -1.0) * (Th > 0.0 ? sqrt(Th) : Q(0.0));
     // p^mu = g^{mu nu} p_nu with p_t = -1, p_phi = lam
-    double DS = g.Dl * g.S;
-    double gtt = -g.Xi / DS, gtp = -2.0 * a * r / DS, gpp = (g.Dl - a * a * g.st * g.st) / (DS * g.st * g.st);
+    Q DS = g.Dl * g.S;
+    Q gtt = -g.Xi / DS, gtp = -2.0 * a * r / DS, gpp = (g.Dl - a * a * g.st * g.st) / (DS * g.st * g.st);
     g.pu[0] = -gtt + gtp * lam;
     g.pu[1] = g.Dl / g.S * pr;
     g.pu[2] = pth / g.S;
@@ -355,6 +357,8 @@ KS_DEV f32 pow2_below(f32 x) {
     int e; frexpf(x.v, &e);
     return f32(ldexpf(1.0f, e - 1));
 }
+// a power of two has no derivative: the duals' partials stay on f = D / Dp
+template <int N, class S> KS_DEV DN<N, S> pow2_below(const DN<N, S>& x) { return DN<N, S>(pow2_below(x.v)); }
 template <class T> KS_DEV T cap_factor(T I, T Q, T V) {
     const double fmax = 0.99;
     if (!(I > 0.0)) return (val(Q) == 0.0 && val(V) == 0.0) ? T(1.0) : T(0.0);
@@ -426,6 +430,7 @@ template <class T> KS_DEV T sinc_(T b) {
 // and the s3 - s2 difference) away from the range where FP32 cancels or overflows.
 KS_DEV double taylor_gate(double) { return 1e-17; }
 KS_DEV double taylor_gate(f32) { return 1e-6; }
+template <int N, class S> KS_DEV double taylor_gate(const DN<N, S>&) { return taylor_gate(S()); }
 
 // The step of one sample (step.jl:164-243) as polynomials in the scaled K' (K = alpha_I + K'):
 // O = o0 + o1 K' + o2 K'^2 + o3 K'^3 and Ej = Dp (e0 + e1 K' + e2 K'^2 + e3 K'^3), with

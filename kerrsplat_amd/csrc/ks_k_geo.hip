@@ -103,31 +103,6 @@ __global__ void ks_res_reduce(int64_t npix, int nseg_max, const double* __restri
     rt[j] = r1; rp[j] = r2;
 }
 
-// The half-orbit counter (WindingState, step.jl:498-517) over the valid samples outside the horizon
-// guard; kstop = the first sample beyond the nmax-th passage (N + 1 when there is none).
-struct WindF {
-    double rp, slab; int nmax, kstop;
-    double zprev; int wn; bool inside, crossed;
-    KS_DEV void operator()(int k, const Samp& s, double) {
-        if (kstop <= k || !(s.ok && s.r > rp * (1.0 + 1e-3))) return;
-        double z = s.r * cos(s.th);
-        if (isnan(zprev)) { zprev = z; inside = fabs(z) < slab; crossed = false; }
-        else {
-            bool inside_now = fabs(z) < slab;
-            bool crossing = zprev * z < 0.0;
-            if (inside || inside_now) {
-                crossed = crossed || crossing;
-                if (inside && !inside_now) { if (crossed) wn += 1; crossed = false; }
-                else if (!inside && inside_now) { crossed = crossing; }
-                inside = inside_now;
-            } else if (crossing) {
-                wn += 1;
-            }
-            zprev = z;
-        }
-        if (wn > nmax) kstop = k;
-    }
-};
 // from the march (fused caches) or from the stored samples (smp.r != nullptr)
 __global__ void __launch_bounds__(128) ks_wind_cutoff(double a, double thetao, const double* __restrict__ soa, int64_t npix, int N, int M,
                                                       int nmax, double slab, Samples smp, int* __restrict__ kstop) {

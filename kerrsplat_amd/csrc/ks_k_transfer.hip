@@ -124,63 +124,6 @@ KS_DEV double thin_sample(const SplatC* __restrict__ sp, int ns, double a, doubl
     return g * g * jt * S_ * dtau;
 }
 
-// A segment's composite transfer map: for its samples, front to back, P = O_k0 ... O_k1 and
-// Sv = sum_k O_k0 ... O_{k-1} E_k (the RadiativeState of step.jl:342-398 started at the identity).
-// In FP32 the Stokes sum is compensated (Kahan), so its error does not grow with the segment length.
-template <class S>
-struct SegMapT {
-    S P[16], Sv[4], comp[4];
-    KS_DEV void init() {
-        for (int i = 0; i < 16; ++i) P[i] = S((i % 5 == 0) ? 1.0 : 0.0);
-        for (int i = 0; i < 4; ++i) { Sv[i] = S(0.0); comp[i] = S(0.0); }
-    }
-    KS_DEV void advance(const S O[16], const S E[4]) {
-        for (int i = 0; i < 4; ++i) add_stokes(P[4 * i] * E[0] + P[4 * i + 1] * E[1] + P[4 * i + 2] * E[2] + P[4 * i + 3] * E[3], i);
-        S Pn[16];
-        for (int i = 0; i < 4; ++i)
-            for (int q = 0; q < 4; ++q) Pn[4 * i + q] = P[4 * i] * O[q] + P[4 * i + 1] * O[4 + q] + P[4 * i + 2] * O[8 + q] + P[4 * i + 3] * O[12 + q];
-        for (int i = 0; i < 16; ++i) P[i] = Pn[i];
-    }
-    KS_DEV void add_stokes(const S pe, int i) {
-        if constexpr (std::is_same<S, double>::value) {
-            Sv[i] += pe;
-        } else {
-            S y = pe - comp[i];
-            S t = Sv[i] + y;
-            comp[i] = (t - Sv[i]) - y;
-            Sv[i] = t;
-        }
-    }
-    // One sample's step without forming O: E = Dp sum_n e_n K'^n j, Sv += P E, then every row of P
-    // becomes sum_n o_n (row K'^n) (products with K'^T), which keeps 4 + 16 values live instead of the
-    // 64 of O, K'^2 and K'^3.
-    KS_DEV void step(const S j4[4], const S a4[4], const S r3[3], S D) {
-        S Dp, K6[6], o[4], e4[4];
-        step_coefs(a4, r3, D, Dp, K6, o, e4);
-        S u[4], y[4], E[4];
-        for (int i = 0; i < 4; ++i) { u[i] = j4[i]; E[i] = e4[0] * u[i]; }
-        for (int n = 1; n < 4; ++n) {
-            kprime_mul(K6, u, y);
-            for (int i = 0; i < 4; ++i) { u[i] = y[i]; E[i] += e4[n] * u[i]; }
-        }
-        for (int i = 0; i < 4; ++i) add_stokes(Dp * (P[4 * i] * E[0] + P[4 * i + 1] * E[1] + P[4 * i + 2] * E[2] + P[4 * i + 3] * E[3]), i);
-        for (int i = 0; i < 4; ++i) {
-            S v[4], acc[4];
-            for (int q = 0; q < 4; ++q) { v[q] = P[4 * i + q]; acc[q] = o[0] * v[q]; }
-            for (int n = 1; n < 4; ++n) {
-                kprime_tmul(K6, v, y);
-                for (int q = 0; q < 4; ++q) { v[q] = y[q]; acc[q] += o[n] * v[q]; }
-            }
-            for (int q = 0; q < 4; ++q) P[4 * i + q] = acc[q];
-        }
-    }
-    KS_DEV void store(double* seg, int64_t m, int64_t nitems, int64_t item) const {
-        double* o = seg + SEG_FIELDS * m * nitems + item;
-        for (int q = 0; q < 16; ++q) o[q * nitems] = val(P[q]);
-        for (int q = 0; q < 4; ++q) o[(16 + q) * nitems] = val(Sv[q]);
-    }
-};
-using SegMap = SegMapT<double>;
 
 // The FP64 step of an FP32 sample, out of line (the rare samples with a Faraday depth |rho| Delta above
 // FARADAY_FP64, where the step's polynomial in K' cancels strongly): O and E in FP64, rounded to FP32.

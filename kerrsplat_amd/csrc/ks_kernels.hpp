@@ -90,6 +90,89 @@ KS_DEV int64_t slot_item(const Samples& smp, int64_t t, int64_t npix) {
 enum RecField { REC_J, REC_A = REC_J + 4, REC_RHO = REC_A + 4, REC_D = REC_RHO + 3, REC_FIELDS };
 constexpr int BAR_FIELDS = 11;    // the adjoints of j (4), alpha (4), rho (3) of every step-taking sample
 
+// A segment's composite transfer map: for its samples, front to back, P = O_k0 ... O_k1 and
+// Sv = sum_k O_k0 ... O_{k-1} E_k (the RadiativeState of step.jl:342-398 started at the identity).
+// In FP32 the Stokes sum is compensated (Kahan), so its error does not grow with the segment length.
+template <class S>
+struct SegMapT {
+    S P[16], Sv[4], comp[4];
+    KS_DEV void init() {
+        for (int i = 0; i < 16; ++i) P[i] = S((i % 5 == 0) ? 1.0 : 0.0);
+        for (int i = 0; i < 4; ++i) { Sv[i] = S(0.0); comp[i] = S(0.0); }
+    }
+    KS_DEV void advance(const S O[16], const S E[4]) {
+        for (int i = 0; i < 4; ++i) add_stokes(P[4 * i] * E[0] + P[4 * i + 1] * E[1] + P[4 * i + 2] * E[2] + P[4 * i + 3] * E[3], i);
+        S Pn[16];
+        for (int i = 0; i < 4; ++i)
+            for (int q = 0; q < 4; ++q) Pn[4 * i + q] = P[4 * i] * O[q] + P[4 * i + 1] * O[4 + q] + P[4 * i + 2] * O[8 + q] + P[4 * i + 3] * O[12 + q];
+        for (int i = 0; i < 16; ++i) P[i] = Pn[i];
+    }
+    KS_DEV void add_stokes(const S pe, int i) {
+        if constexpr (!std::is_same<S, f32>::value) {          // compensated in FP32 only
+            Sv[i] += pe;
+        } else {
+            S y = pe - comp[i];
+            S t = Sv[i] + y;
+            comp[i] = (t - Sv[i]) - y;
+            Sv[i] = t;
+        }
+    }
+    // One sample's step without forming O: E = Dp sum_n e_n K'^n j, Sv += P E, then every row of P
+    // becomes sum_n o_n (row K'^n) (products with K'^T), which keeps 4 + 16 values live instead of the
+    // 64 of O, K'^2 and K'^3.
+    KS_DEV void step(const S j4[4], const S a4[4], const S r3[3], S D) {
+        S Dp, K6[6], o[4], e4[4];
+        step_coefs(a4, r3, D, Dp, K6, o, e4);
+        S u[4], y[4], E[4];
+        for (int i = 0; i < 4; ++i) { u[i] = j4[i]; E[i] = e4[0] * u[i]; }
+        for (int n = 1; n < 4; ++n) {
+            kprime_mul(K6, u, y);
+            for (int i = 0; i < 4; ++i) { u[i] = y[i]; E[i] += e4[n] * u[i]; }
+        }
+        for (int i = 0; i < 4; ++i) add_stokes(Dp * (P[4 * i] * E[0] + P[4 * i + 1] * E[1] + P[4 * i + 2] * E[2] + P[4 * i + 3] * E[3]), i);
+        for (int i = 0; i < 4; ++i) {
+            S v[4], acc[4];
+            for (int q = 0; q < 4; ++q) { v[q] = P[4 * i + q]; acc[q] = o[0] * v[q]; }
+            for (int n = 1; n < 4; ++n) {
+                kprime_tmul(K6, v, y);
+                for (int q = 0; q < 4; ++q) { v[q] = y[q]; acc[q] += o[n] * v[q]; }
+            }
+            for (int q = 0; q < 4; ++q) P[4 * i + q] = acc[q];
+        }
+    }
+    KS_DEV void store(double* seg, int64_t m, int64_t nitems, int64_t item) const {
+        double* o = seg + SEG_FIELDS * m * nitems + item;
+        for (int q = 0; q < 16; ++q) o[q * nitems] = val(P[q]);
+        for (int q = 0; q < 4; ++q) o[(16 + q) * nitems] = val(Sv[q]);
+    }
+};
+using SegMap = SegMapT<double>;
+
+// The half-orbit counter (WindingState, step.jl:498-517) over the valid samples outside the horizon
+// guard; kstop = the first sample beyond the nmax-th passage (N + 1 when there is none).
+struct WindF {
+    double rp, slab; int nmax, kstop;
+    double zprev; int wn; bool inside, crossed;
+    KS_DEV void operator()(int k, const Samp& s, double) {
+        if (kstop <= k || !(s.ok && s.r > rp * (1.0 + 1e-3))) return;
+        double z = s.r * cos(s.th);
+        if (isnan(zprev)) { zprev = z; inside = fabs(z) < slab; crossed = false; }
+        else {
+            bool inside_now = fabs(z) < slab;
+            bool crossing = zprev * z < 0.0;
+            if (inside || inside_now) {
+                crossed = crossed || crossing;
+                if (inside && !inside_now) { if (crossed) wn += 1; crossed = false; }
+                else if (!inside && inside_now) { crossed = crossing; }
+                inside = inside_now;
+            } else if (crossing) {
+                wn += 1;
+            }
+            zprev = z;
+        }
+        if (wn > nmax) kstop = k;
+    }
+};
 // ---- ks_k_geo.hip ----
 hipError_t launch_root_case(hipStream_t s, double a, double thetao, const double* al, const double* be, int64_t npix, int8_t* numreals);
 hipError_t launch_case_count(hipStream_t s, int nblocks, const int8_t* numreals, int64_t npix, int* bcount);
@@ -161,6 +244,11 @@ hipError_t launch_bin_residual(hipStream_t s, const double* out, const int* iper
                                int64_t nbins, const double* data, const double* sigma, int sigma_per_pixel, const uint8_t* mask,
                                double* pseed, double* chi_p);
 hipError_t launch_bin_seed(hipStream_t s, const int* perm, const int* pix, int64_t npix, int64_t f, const double* pseed, double* ads);
+
+// ---- ks_k_tangent.hip ----
+hipError_t launch_tangent(hipStream_t s, int kind, double a, double thetao, double lnL, const double* al, const double* be, int64_t npix, int N,
+                          const SplatC* sp, int ns, const double* times, int nt, const double* nus, int nnu, int nmax, double slab,
+                          double* out, double* dout);
 
 inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 

@@ -242,3 +242,86 @@ def gain_factors(gains, s1, s2):
 def apply_gains(vis, gains, s1, s2):
     """apply_gains(vis, gains, s1, s2) (visibilities.jl:201-214): V'_k = g_{s1[k]} conj(g_{s2[k]}) V_k."""
     return np.asarray(vis) * gain_factors(gains, s1, s2)[:, None]
+
+
+# ---- the spacetime block (joint.jl:74-154) ---------------------------------------------------------------
+def _spacetime_cache(x, camera, N):
+    """A cache of `camera` with N samples per ray at (a, theta_o) = x[:2] (pixel constants only: the tangent
+    kernel marches the closed-form samples itself)."""
+    cache = api.GeodesicCache(camera, N, store_samples=False)
+    api.regenerate(cache, float(x[0]), float(x[1]), marcher=api.Fused(64))
+    return cache
+
+
+def spacetime_valgrad(loss_and_grad, x, camera, N, params, times, nus, nmax=-1, slab=0.0, kind="thermal"):
+    """spacetime_valgrad(loss, x, camera, backend; N) (joint.jl:74-82) for a loss of the rendered cube: the
+    value and gradient with respect to x = (a, theta_o, ln L). The cube of `params` at `times`, `nus` and its
+    forward-mode partials come from spacetime_tangent; loss_and_grad(cube) takes the screen-shaped cube
+    (n_alpha, n_beta, nt, nnu, 4) and returns (value, d value / d cube) (spacetime_movie_loss builds the
+    movie χ²). Returns (value, gradient (3,))."""
+    cache = _spacetime_cache(x, camera, N)
+    cube, dcube = api.spacetime_tangent(cache, params, times, nus, float(np.exp(x[2])), nmax=nmax, slab=slab, kind=kind)
+    value, dv = loss_and_grad(cube)
+    g = np.einsum("abtfs,abtfsk->k", np.asarray(dv, dtype=np.float64), dcube)
+    return float(value), g
+
+
+def spacetime_movie_loss(movie, frames=None, freqs=None, binning=None):
+    """The movie χ² as a loss of the rendered cube for spacetime_valgrad (spacetime_movie_loss, joint.jl:90-96):
+    the cube must cover the movie's selected times and frequencies in order; with a Binning the cube is
+    integrated over its pixels first. Returns loss_and_grad(cube) -> (χ², dχ²/dcube)."""
+    ks, ls = _frames(movie, frames, freqs)
+
+    def loss_and_grad(cube):
+        model = api.bin(binning, cube) if binning is not None else cube
+        total = 0.0
+        dmodel = np.zeros(model.shape)
+        for b, l in enumerate(ls):
+            for a, k in enumerate(ks):
+                sg = movie.noise(k, l)
+                r = (model[:, :, a, b, :] - movie.data[:, :, k, l, :]) / sg
+                mk = movie.mask[:, :, k, l][..., None]
+                total += float(np.sum((r * r) * mk))
+                dmodel[:, :, a, b, :] = np.where(mk, 2 * r / sg, 0.0)
+        if binning is None:
+            return total, dmodel
+        flat = np.transpose(dmodel, (1, 0) + tuple(range(2, dmodel.ndim))).reshape((binning.npixels,) + dmodel.shape[2:])
+        pts = flat[binning.pixel] / binning.count[binning.pixel].reshape((-1,) + (1,) * (flat.ndim - 1))
+        return total, pts.reshape((binning.npoints, 1) + dmodel.shape[2:])       # the cube of a point camera
+
+    return loss_and_grad
+
+
+def spacetime_jacobian(x, camera, N, params, movie, frames=None, freqs=None, nmax=-1, slab=0.0, binning=None, kind="thermal"):
+    """spacetime_jacobian with the movie residuals (joint.jl:98-154, spacetime_movie_residuals): the scaled
+    residuals (model - data)/σ of the masked pixels, ordered frequency, frame, β, α, Stokes as the reference
+    appends them, and their Jacobian with respect to x = (a, theta_o, ln L). Rows with a finite residual and
+    non-finite partials are zeroed, with a warning above one per thousand. Returns (r, J)."""
+    if len(x) != 3:
+        raise ValueError("the spacetime block is (a, theta_o, ln L)")
+    ks, ls = _frames(movie, frames, freqs)
+    cache = _spacetime_cache(x, camera, N)
+    cube, dcube = api.spacetime_tangent(cache, params, movie.times[ks], movie.nus[ls], float(np.exp(x[2])), nmax=nmax, slab=slab, kind=kind)
+    if binning is not None:
+        cube = api.bin(binning, cube)
+        dcube = api.bin(binning, dcube)
+    rs, Js = [], []
+    for b, l in enumerate(ls):
+        for a, k in enumerate(ks):
+            sg = np.broadcast_to(movie.noise(k, l), movie.data.shape[:2] + (4,))
+            r = (cube[:, :, a, b, :] - movie.data[:, :, k, l, :]) / sg
+            J = dcube[:, :, a, b, :, :] / sg[..., None]
+            mk = movie.mask[:, :, k, l]
+            rr = np.transpose(r, (1, 0, 2))[mk.T]                     # j (beta) outer, i (alpha) inner
+            JJ = np.transpose(J, (1, 0, 2, 3))[mk.T]
+            rs.append(rr.reshape(-1))
+            Js.append(JJ.reshape(-1, 3))
+    r = np.concatenate(rs) if rs else np.zeros(0)
+    J = np.concatenate(Js) if Js else np.zeros((0, 3))
+    bad = np.isfinite(r) & ~np.all(np.isfinite(J), axis=1)
+    if bad.any():
+        if bad.sum() > 1e-3 * len(r):
+            import warnings
+            warnings.warn(f"spacetime Jacobian: {int(bad.sum())} of {len(r)} residuals have non-finite partials; their rows are zeroed")
+        J[bad] = 0.0
+    return r, J

@@ -117,6 +117,23 @@ def polarized_image(a, thetao, alphas, betas, N, params, t_obs, nu, L, kind="the
     return out, rt, rp
 
 
+def spacetime_tangent(a, thetao, alphas, betas, N, params, t_obs, nu, L, kind="thermal", hyp=None, gamma_max=1e5, nmax=-1,
+                      slab=0.0, M=64, marcher=1):
+    """Observed Stokes (npix, 4) and their forward-mode partials (npix, 4, 3) with respect to (a, theta_o, ln L)
+    (spacetime_valgrad / spacetime_jacobian, joint.jl:74-154)."""
+    al, be = _f64(alphas), _f64(betas)
+    P = np.asfortranarray(params, dtype=np.float64)
+    hy = _f64(hyp) if hyp is not None else None
+    n = len(al)
+    out, dout = np.zeros((n, 4)), np.zeros((n, 4, 3))
+    rc = lib().kso_spacetime_tangent(_d(a), _d(thetao), _i(n), _p(al), _p(be), _i(N), _i(M), _i(marcher), _i(KIND[kind]),
+                                     _i(P.shape[0]), _i(P.shape[1]), _p(P), _p(hy), _d(gamma_max), _d(t_obs), _d(nu),
+                                     _d(np.log(L)), _i(nmax), _d(slab), _p(out), _p(dout))
+    if rc:
+        raise ValueError(f"kso_spacetime_tangent: unsupported model (code {rc})")
+    return out, dout
+
+
 def polarized_composite(a, thetao, alphas, betas, N, parts, t_obs, nu, L, nmax=-1, slab=0.0, M=64, marcher=1):
     """Stokes image of CompositeModel(parts...) (transport.jl:298-316). parts: a list of
     (kind, params) or (kind, params, hyp, gamma_max) with kind thermal, powerlaw or kappa."""

@@ -41,14 +41,22 @@ template <class T, class F> void march(const Pixel<T>& p, int N, int M, int marc
     else march_ray(p, N, M, f, rt, rp);
 }
 
-// the polarized transport of one ray (transport.jl:263-290, with the half-orbit counter when nmax >= 0)
-template <class T> void polarized_ray(const Model& m, double a, double thetao, double alpha, double beta, int N, int M, int marcher,
-                                      double t_obs, double nu, double L, int nmax, double slab, T out[4], T* rt, T* rp) {
+// a ray constant in the spacetime scalar R: the value alone for R = double, else the scalar itself
+template <class R, class T> R as_spacetime(const T& x) {
+    if constexpr (std::is_same_v<R, double>) return val(x);
+    else return R(x);
+}
+
+// the polarized transport of one ray (transport.jl:263-290, with the half-orbit counter when nmax >= 0); the
+// spin, inclination and length unit in R (double, or the duals of the spacetime tangent with T = R)
+template <class T, class R = double> void polarized_ray(const Model& m, R a, R thetao, double alpha, double beta, int N, int M, int marcher,
+                                                        double t_obs, double nu, R L, int nmax, double slab, T out[4], T* rt, T* rp) {
     Pixel<T> p = pixel_constants<T>(T(a), T(alpha), T(beta), T(thetao));
-    RayCtx rc{a, alpha, beta, thetao, val(p.eta), val(p.lam)};
+    RayCtxT<R> rc{a, R(alpha), R(beta), thetao, as_spacetime<R>(p.eta), as_spacetime<R>(p.lam)};
     RadState<T> st = radstate_zero<T>();
     Winding<T> w = winding_zero<T>();
-    double hor = (1.0 + std::sqrt(1.0 - a * a)) * (1.0 + 1e-3);
+    const double av = val(a);
+    double hor = (1.0 + std::sqrt(1.0 - av * av)) * (1.0 + 1e-3);
     auto f = [&](int k, const Sample<T>& s, T dtau) {
         if (nmax >= 0) {
             if (!(s.ok && s.r > hor)) return;
@@ -152,6 +160,30 @@ int kso_polarized_image(double a, double thetao, int npix, const double* alphas,
         polarized_ray<double>(m, a, thetao, alphas[i], betas[i], N, M, marcher, t_obs, nu, L, nmax, slab, out + 4 * (size_t)i, &rt, &rp);
         if (resid_t) resid_t[i] = rt;
         if (resid_ph) resid_ph[i] = rp;
+    }
+    return 0;
+}
+
+// spacetime_valgrad / spacetime_jacobian (joint.jl:74-154): the observed Stokes vector of every ray and its
+// forward-mode partials with respect to (a, theta_o, ln L), the three duals seeded on the spin, the inclination
+// and the log length unit and carried through the pixel constants, the march and the transport.
+// out: npix x 4; dout: npix x 4 x 3 (partial fastest).
+int kso_spacetime_tangent(double a, double thetao, int npix, const double* alphas, const double* betas, int N, int M, int marcher,
+                          int kind, int nrows, int ns, const double* params, const double* hyp, double gamma_max,
+                          double t_obs, double nu, double lnL, int nmax, double slab, double* out, double* dout) {
+    using D3 = Dual<3, double>;
+    ModelHolder mh = make_model<double>(kind, nrows, ns, params, hyp, gamma_max, nullptr, nullptr, 0);
+    const Model& m = mh.m;
+    if (kind == MODEL_THIN || kind == MODEL_KNOT) return 2;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int i = 0; i < npix; ++i) {
+        const D3 ad = D3::seed(a, 0), td = D3::seed(thetao, 1), Ld = kexp(D3::seed(lnL, 2));
+        D3 o[4];
+        polarized_ray<D3, D3>(m, ad, td, alphas[i], betas[i], N, M, marcher, t_obs, nu, Ld, nmax, slab, o, nullptr, nullptr);
+        for (int q = 0; q < 4; ++q) {
+            out[4 * (size_t)i + q] = o[q].v;
+            for (int k = 0; k < 3; ++k) dout[12 * (size_t)i + 3 * q + k] = o[q].d[k];
+        }
     }
     return 0;
 }

@@ -89,9 +89,11 @@ template <class T> void knot_centre(const Model& m, int i, T t, T c[3]) {
 }
 
 // ---- one element's fluid coefficients and frame -----------------------------------------------------
-struct RayCtx { double a, alpha, beta, thetao, eta, lam; };
+// the ray's spacetime and screen constants; R = double, or a dual scalar for the spacetime tangent
+template <class R> struct RayCtxT { R a, alpha, beta, thetao, eta, lam; };
+using RayCtx = RayCtxT<double>;
 
-template <class T> bool element(const Model& m, int i, const RayCtx& rc, const Sample<T>& s, T t_obs, T nu_obs, Coef<T>& cf, Frame<T>& fr) {
+template <class T, class RS> bool element(const Model& m, int i, const RayCtxT<RS>& rc, const Sample<T>& s, T t_obs, T nu_obs, Coef<T>& cf, Frame<T>& fr) {
     T x, y, z;
     quasi_cartesian_ks(T(rc.a), s.r, s.th, s.ph, x, y, z);
     T t = t_obs - s.t;
@@ -165,7 +167,7 @@ template <class T> void wind(Winding<T>& w, T z, T h) {
 // sum of all elements' screen coefficients at a sample (transport.jl:254-276); returns whether any
 // element is active. A composite model visits its parts in order, each part's elements in order
 // (element(::CompositeModel), transport.jl:312-316), so the sum runs over the same sequence.
-template <class T> bool accumulate_elements(const Model& m0, const RayCtx& rc, const Sample<T>& s, T t_obs, T nu_obs, T j4[4], T a4[4], T r3[3]) {
+template <class T, class RS> bool accumulate_elements(const Model& m0, const RayCtxT<RS>& rc, const Sample<T>& s, T t_obs, T nu_obs, T j4[4], T a4[4], T r3[3]) {
     for (int q = 0; q < 4; ++q) { j4[q] = 0.0; a4[q] = 0.0; }
     for (int q = 0; q < 3; ++q) r3[q] = 0.0;
     bool active = false;
@@ -185,9 +187,10 @@ template <class T> bool accumulate_elements(const Model& m0, const RayCtx& rc, c
 }
 
 // one sample of the polarized transport (transport.jl:263-290); returns true when a step was taken
-template <class T> bool sample_step(const Model& m, const RayCtx& rc, const Sample<T>& s, T dtau, T t_obs, T nu_obs, T L, M4<T>& O, T E[4]) {
-    T a = rc.a;
-    T rh = 1.0 + std::sqrt(1.0 - rc.a * rc.a);
+template <class T, class RS> bool sample_step(const Model& m, const RayCtxT<RS>& rc, const Sample<T>& s, T dtau, T t_obs, T nu_obs, T L, M4<T>& O, T E[4]) {
+    T a = T(rc.a);
+    const double av = val(rc.a);
+    T rh = 1.0 + std::sqrt(1.0 - av * av);
     if (!(s.ok && s.r > rh * (1.0 + 1e-3))) return false;
     T j4[4], a4[4], r3[3];
     if (!accumulate_elements(m, rc, s, t_obs, nu_obs, j4, a4, r3)) return false;
