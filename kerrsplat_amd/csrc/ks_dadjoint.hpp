@@ -126,42 +126,50 @@ KS_DEV S weight_dlog(const SplatCT<S>& c, const SplatAdjT<S>& sa, S t, S x, S y,
 // One thermal splat's share of one sample (dual_gradient.jl:46-76), given the splat's weight Gw > 0 at
 // the sample and d ln G (weight_dlog): adds into grad (the splat's 21-row column).
 // bars = (j_bar[4], alpha_bar[4], rho_bar[3]).
+// The seven fluid partials (ln Theta_e, ln B, theta_B, phi_B, u1..u3) are taken in two five-partial passes
+// joined by the chain rule: the frame (g, cos B, chi) with respect to (theta_B, phi_B, u1..u3), and the
+// sample's contribution q with respect to (ln Theta_e, ln B, g, cos B, chi). The frame enters q only
+// through those three values, so the products are the seven-partial derivatives at 5/7 of the width.
 template <class S>
 KS_DEV void element_adjoint_thermal(const SplatCT<S>& c, const SplatAdjT<S>& sa, const GeoT<S>& G, S a, S alpha, S beta,
                                     S thetao_sin, S nu, const S bars[11], S Gw, const S dl[13], S grad[21]) {
-    using D7 = DN<7, S>;
-    // fluid rows as 7 partials: ln Theta_e, ln B, theta_B, phi_B, u1, u2, u3
-    D7 u[3] = {D7::seed(c.u[0], 4), D7::seed(c.u[1], 5), D7::seed(c.u[2], 6)};
-    D7 bh[3];
-    bh[0] = D7(sa.sthB * sa.cphB); bh[0].d[2] = sa.cthB * sa.cphB; bh[0].d[3] = -sa.sthB * sa.sphB;
-    bh[1] = D7(sa.sthB * sa.sphB); bh[1].d[2] = sa.cthB * sa.sphB; bh[1].d[3] = sa.sthB * sa.cphB;
-    bh[2] = D7(sa.cthB);           bh[2].d[2] = -sa.sthB;
-    FrameT<D7> fr = local_frame(G, a, alpha, beta, thetao_sin, u, bh);
-    D7 Th(c.p1); Th.d[0] = c.p1;
-    D7 k0k2(c.p2); k0k2.d[0] = sa.dk0k2;
-    D7 k1k2(c.p3); k1k2.d[0] = sa.dk1k2;
-    D7 Bm(c.Bmag); Bm.d[1] = c.Bmag;
-    D7 nuf = nu / fr.g;
-    D7 cb = dmin(dmax(fr.cosB, -1.0), 1.0);
-    D7 thB = acos(cb);
-    D7 sth, cth;
+    using D5 = DN<5, S>;
+    // the frame: partials theta_B, phi_B, u1, u2, u3
+    D5 u[3] = {D5::seed(c.u[0], 2), D5::seed(c.u[1], 3), D5::seed(c.u[2], 4)};
+    D5 bh[3];
+    bh[0] = D5(sa.sthB * sa.cphB); bh[0].d[0] = sa.cthB * sa.cphB; bh[0].d[1] = -sa.sthB * sa.sphB;
+    bh[1] = D5(sa.sthB * sa.sphB); bh[1].d[0] = sa.cthB * sa.sphB; bh[1].d[1] = sa.sthB * sa.cphB;
+    bh[2] = D5(sa.cthB);           bh[2].d[0] = -sa.sthB;
+    const FrameT<D5> fr = local_frame(G, a, alpha, beta, thetao_sin, u, bh);
+    // the contribution: partials ln Theta_e, ln B, g, cos B, chi
+    D5 Th(c.p1); Th.d[0] = c.p1;
+    D5 k0k2(c.p2); k0k2.d[0] = sa.dk0k2;
+    D5 k1k2(c.p3); k1k2.d[0] = sa.dk1k2;
+    D5 Bm(c.Bmag); Bm.d[1] = c.Bmag;
+    const D5 g = D5::seed(fr.g.v, 2), cosB = D5::seed(fr.cosB.v, 3), chi = D5::seed(fr.chi.v, 4);
+    D5 nuf = nu / g;
+    D5 cb = dmin(dmax(cosB, -1.0), 1.0);
+    D5 thB = acos(cb);
+    D5 sth, cth;
     sincos(thB, &sth, &cth);
     const S ne = exp(c.lnA) * Gw;
-    CoefT<D7> cf = thermal(ne, Th, k0k2, k1k2, Bm, nuf, sth, cth);
+    CoefT<D5> cf = thermal(ne, Th, k0k2, k1k2, Bm, nuf, sth, cth);
     if (!(cf.jI > 0.0 || cf.aI > 0.0 || cf.rQ.v != 0.0 || cf.rV.v != 0.0)) return;
-    D7 fe = cap_factor(cf.jI, cf.jQ, cf.jV), fa = cap_factor(cf.aI, cf.aQ, cf.aV);
-    D7 nh = nuf * (1.0 / NU0);
-    D7 i2 = 1.0 / (nh * nh);
-    D7 s2c, c2c;
-    sincos(2.0 * fr.chi, &s2c, &c2c);
-    D7 jQ = fe * cf.jQ * i2, aQ = fa * cf.aQ * nh, rQ = cf.rQ * nh;
-    D7 q = bars[0] * (cf.jI * i2) + bars[1] * (jQ * c2c) + bars[2] * (jQ * s2c) + bars[3] * (fe * cf.jV * i2);
+    D5 fe = cap_factor(cf.jI, cf.jQ, cf.jV), fa = cap_factor(cf.aI, cf.aQ, cf.aV);
+    D5 nh = nuf * (1.0 / NU0);
+    D5 i2 = 1.0 / (nh * nh);
+    D5 s2c, c2c;
+    sincos(2.0 * chi, &s2c, &c2c);
+    D5 jQ = fe * cf.jQ * i2, aQ = fa * cf.aQ * nh, rQ = cf.rQ * nh;
+    D5 q = bars[0] * (cf.jI * i2) + bars[1] * (jQ * c2c) + bars[2] * (jQ * s2c) + bars[3] * (fe * cf.jV * i2);
     q = q + bars[4] * (cf.aI * nh) + bars[5] * (aQ * c2c) + bars[6] * (aQ * s2c) + bars[7] * (fa * cf.aV * nh);
     q = q + bars[8] * (rQ * c2c) + bars[9] * (rQ * s2c) + bars[10] * (cf.rV * nh);
     const S sv = q.v;
     for (int r = 0; r < 12; ++r) grad[r] += sv * dl[r];
     grad[12] += sv;
-    for (int k = 0; k < 7; ++k) grad[13 + k] += q.d[k];
+    grad[13] += q.d[0];
+    grad[14] += q.d[1];
+    for (int k = 0; k < 5; ++k) grad[15 + k] += q.d[2] * fr.g.d[k] + q.d[3] * fr.cosB.d[k] + q.d[4] * fr.chi.d[k];
     grad[20] += sv * dl[12];
 }
 

@@ -118,7 +118,8 @@ inline S weight_dlog_analytic(const Model& m, int i, S t, S x, S y, S z, S dl[13
 }
 
 // one thermal splat's share of sample s (dual_gradient.jl:46-76); grad is the splat's 21-row column.
-// ANALYTIC selects the hand-written d ln G instead of the 13-partial duals.
+// ANALYTIC selects the device's form: the hand-written d ln G instead of the 13-partial duals, and the
+// seven fluid partials in two five-partial passes joined by the chain rule instead of one 7-partial pass.
 template <class S, bool ANALYTIC = false>
 inline void element_adjoint(const Model& m, int i, const RayCtx& rc, const Sample<S>& s, S t_obs, S nu,
                             const S jbar[4], const S abar[4], const S rbar[3], S* grad) {
@@ -160,35 +161,69 @@ inline void element_adjoint(const Model& m, int i, const RayCtx& rc, const Sampl
     if (!(G > WEIGHT_CUTOFF)) return;
     for (int k = 0; k < 13; ++k) dlnG[k] = Gd.d[k] / G;
     }
-    // fluid rows with 7 partials (rows 13..19)
-    D7 fl[7];
-    for (int k = 0; k < 7; ++k) fl[k] = D7::seed(S(colv[13 + k]), k);
-    D7 Th = kexp(fl[0]);
-    D7 Bmag = kexp(fl[1]);
-    D7 st = ksin(fl[2]), ct = kcos(fl[2]), sph = ksin(fl[3]), cph = kcos(fl[3]);
-    D7 B[3] = {Bmag * st * cph, Bmag * st * sph, Bmag * ct};
-    D7 u[3] = {fl[4], fl[5], fl[6]};
-    Frame<D7> fr = local_frame<D7>(D7(rc.a), D7(s.r), D7(s.th), D7(rc.eta), D7(rc.lam), s.nur, s.nuth, D7(rc.alpha), D7(rc.beta),
-                                   D7(rc.thetao), u, B);
-    D7 nuf = D7(nu) / fr.g;
-    D7 thB = kacos(kmin(kmax(fr.cosB, D7(-1.0)), D7(1.0)));
-    // Bessel K0, K1 at 1/Theta_e with their derivatives (K0' = -K1, K1' = -K0 - K1/x)
     const double* kb = m.bessel + 3 * i;
-    D7 xb = D7(1.0) / Th;
-    D7 k0 = chain(S(kb[0]), S(-kb[1]), xb), k1 = chain(S(kb[1]), S(-kb[0] - kb[1] / xb.v), xb);
-    D7 k2 = k0 + 2.0 * k1 / xb;
     S ne = kexp(S(colv[12])) * G;
-    Coef<D7> cf = thermal_synchrotron_k<D7>(D7(ne), Th, Bmag, nuf, thB, k0, k1, k2);
-    Screen<D7> sc;
-    if (!screen_coefficients<D7>(cf, fr, D7(nu), sc)) return;
-    D7 q(0.0);
-    for (int k = 0; k < 4; ++k) q = q + D7(jbar[k]) * sc.j[k] + D7(abar[k]) * sc.a[k];
-    for (int k = 0; k < 3; ++k) q = q + D7(rbar[k]) * sc.r[k];
-    S sv = q.v;
-    for (int r = 0; r < 12; ++r) grad[r] = grad[r] + sv * dlnG[r];
-    grad[12] = grad[12] + sv;
-    for (int k = 0; k < 7; ++k) grad[13 + k] = grad[13 + k] + q.d[k];
-    grad[nr - 1] = grad[nr - 1] + sv * dlnG[12];
+    if constexpr (ANALYTIC) {
+        // the device's form: the frame (g, cos B, chi) with five partials (theta_B, phi_B, u1..u3), the
+        // contribution with five (ln Theta_e, ln B, g, cos B, chi), joined by the chain rule
+        using D5 = Dual<5, S>;
+        D5 st = ksin(D5::seed(S(colv[15]), 0)), ct = kcos(D5::seed(S(colv[15]), 0));
+        D5 sph = ksin(D5::seed(S(colv[16]), 1)), cph = kcos(D5::seed(S(colv[16]), 1));
+        D5 Bh[3] = {st * cph, st * sph, ct};
+        D5 u[3] = {D5::seed(S(colv[17]), 2), D5::seed(S(colv[18]), 3), D5::seed(S(colv[19]), 4)};
+        Frame<D5> fr = local_frame<D5>(D5(rc.a), D5(s.r), D5(s.th), D5(rc.eta), D5(rc.lam), s.nur, s.nuth, D5(rc.alpha), D5(rc.beta),
+                                       D5(rc.thetao), u, Bh);
+        D5 Th = kexp(D5::seed(S(colv[13]), 0));
+        D5 Bmag = kexp(D5::seed(S(colv[14]), 1));
+        Frame<D5> fq;
+        fq.g = D5::seed(fr.g.v, 2); fq.cosB = D5::seed(fr.cosB.v, 3); fq.chi = D5::seed(fr.chi.v, 4);
+        D5 nuf = D5(nu) / fq.g;
+        D5 thB = kacos(kmin(kmax(fq.cosB, D5(-1.0)), D5(1.0)));
+        D5 xb = D5(1.0) / Th;
+        D5 k0 = chain(S(kb[0]), S(-kb[1]), xb), k1 = chain(S(kb[1]), S(-kb[0] - kb[1] / xb.v), xb);
+        D5 k2 = k0 + 2.0 * k1 / xb;
+        Coef<D5> cf = thermal_synchrotron_k<D5>(D5(ne), Th, Bmag, nuf, thB, k0, k1, k2);
+        Screen<D5> sc;
+        if (!screen_coefficients<D5>(cf, fq, D5(nu), sc)) return;
+        D5 q(0.0);
+        for (int k = 0; k < 4; ++k) q = q + D5(jbar[k]) * sc.j[k] + D5(abar[k]) * sc.a[k];
+        for (int k = 0; k < 3; ++k) q = q + D5(rbar[k]) * sc.r[k];
+        S sv = q.v;
+        for (int r = 0; r < 12; ++r) grad[r] = grad[r] + sv * dlnG[r];
+        grad[12] = grad[12] + sv;
+        grad[13] = grad[13] + q.d[0];
+        grad[14] = grad[14] + q.d[1];
+        for (int k = 0; k < 5; ++k) grad[15 + k] = grad[15 + k] + q.d[2] * fr.g.d[k] + q.d[3] * fr.cosB.d[k] + q.d[4] * fr.chi.d[k];
+        grad[nr - 1] = grad[nr - 1] + sv * dlnG[12];
+    } else {
+        // fluid rows with 7 partials (rows 13..19)
+        D7 fl[7];
+        for (int k = 0; k < 7; ++k) fl[k] = D7::seed(S(colv[13 + k]), k);
+        D7 Th = kexp(fl[0]);
+        D7 Bmag = kexp(fl[1]);
+        D7 st = ksin(fl[2]), ct = kcos(fl[2]), sph = ksin(fl[3]), cph = kcos(fl[3]);
+        D7 B[3] = {Bmag * st * cph, Bmag * st * sph, Bmag * ct};
+        D7 u[3] = {fl[4], fl[5], fl[6]};
+        Frame<D7> fr = local_frame<D7>(D7(rc.a), D7(s.r), D7(s.th), D7(rc.eta), D7(rc.lam), s.nur, s.nuth, D7(rc.alpha), D7(rc.beta),
+                                       D7(rc.thetao), u, B);
+        D7 nuf = D7(nu) / fr.g;
+        D7 thB = kacos(kmin(kmax(fr.cosB, D7(-1.0)), D7(1.0)));
+        // Bessel K0, K1 at 1/Theta_e with their derivatives (K0' = -K1, K1' = -K0 - K1/x)
+        D7 xb = D7(1.0) / Th;
+        D7 k0 = chain(S(kb[0]), S(-kb[1]), xb), k1 = chain(S(kb[1]), S(-kb[0] - kb[1] / xb.v), xb);
+        D7 k2 = k0 + 2.0 * k1 / xb;
+        Coef<D7> cf = thermal_synchrotron_k<D7>(D7(ne), Th, Bmag, nuf, thB, k0, k1, k2);
+        Screen<D7> sc;
+        if (!screen_coefficients<D7>(cf, fr, D7(nu), sc)) return;
+        D7 q(0.0);
+        for (int k = 0; k < 4; ++k) q = q + D7(jbar[k]) * sc.j[k] + D7(abar[k]) * sc.a[k];
+        for (int k = 0; k < 3; ++k) q = q + D7(rbar[k]) * sc.r[k];
+        S sv = q.v;
+        for (int r = 0; r < 12; ++r) grad[r] = grad[r] + sv * dlnG[r];
+        grad[12] = grad[12] + sv;
+        for (int k = 0; k < 7; ++k) grad[13 + k] = grad[13 + k] + q.d[k];
+        grad[nr - 1] = grad[nr - 1] + sv * dlnG[12];
+    }
 }
 
 // Gradient of sum_pixels dstokes . observed_stokes over one ray (thermal splats); adds into grad
