@@ -126,10 +126,11 @@ KS_DEV S weight_dlog(const SplatCT<S>& c, const SplatAdjT<S>& sa, S t, S x, S y,
 // One thermal splat's share of one sample (dual_gradient.jl:46-76), given the splat's weight Gw > 0 at
 // the sample and d ln G (weight_dlog): adds into grad (the splat's 21-row column).
 // bars = (j_bar[4], alpha_bar[4], rho_bar[3]).
-// The seven fluid partials (ln Theta_e, ln B, theta_B, phi_B, u1..u3) are taken in two five-partial passes
-// joined by the chain rule: the frame (g, cos B, chi) with respect to (theta_B, phi_B, u1..u3), and the
-// sample's contribution q with respect to (ln Theta_e, ln B, g, cos B, chi). The frame enters q only
-// through those three values, so the products are the seven-partial derivatives at 5/7 of the width.
+// The seven fluid partials (ln Theta_e, ln B, theta_B, phi_B, u1..u3) are taken in two passes joined by the
+// chain rule: the frame (g, cos B, chi) with respect to (theta_B, phi_B, u1..u3), and the sample's
+// contribution q with respect to (ln Theta_e, ln B, g, cos B), with dq/dchi written out. The frame enters q
+// only through those three values, so the products are the seven-partial derivatives at a fraction of the
+// width.
 template <class S>
 KS_DEV void element_adjoint_thermal(const SplatCT<S>& c, const SplatAdjT<S>& sa, const GeoT<S>& G, S a, S alpha, S beta,
                                     S thetao_sin, S nu, const S bars[11], S Gw, const S dl[13], S grad[21]) {
@@ -141,35 +142,39 @@ KS_DEV void element_adjoint_thermal(const SplatCT<S>& c, const SplatAdjT<S>& sa,
     bh[1] = D5(sa.sthB * sa.sphB); bh[1].d[0] = sa.cthB * sa.sphB; bh[1].d[1] = sa.sthB * sa.cphB;
     bh[2] = D5(sa.cthB);           bh[2].d[0] = -sa.sthB;
     const FrameT<D5> fr = local_frame(G, a, alpha, beta, thetao_sin, u, bh);
-    // the contribution: partials ln Theta_e, ln B, g, cos B, chi
-    D5 Th(c.p1); Th.d[0] = c.p1;
-    D5 k0k2(c.p2); k0k2.d[0] = sa.dk0k2;
-    D5 k1k2(c.p3); k1k2.d[0] = sa.dk1k2;
-    D5 Bm(c.Bmag); Bm.d[1] = c.Bmag;
-    const D5 g = D5::seed(fr.g.v, 2), cosB = D5::seed(fr.cosB.v, 3), chi = D5::seed(fr.chi.v, 4);
-    D5 nuf = nu / g;
-    D5 cb = dmin(dmax(cosB, -1.0), 1.0);
-    D5 thB = acos(cb);
-    D5 sth, cth;
+    // the contribution: partials ln Theta_e, ln B, g, cos B; chi enters only through the screen rotation,
+    // q = A + Bc cos 2chi + Bs sin 2chi with A, Bc, Bs free of chi, so its derivative is written out
+    using D4 = DN<4, S>;
+    D4 Th(c.p1); Th.d[0] = c.p1;
+    D4 k0k2(c.p2); k0k2.d[0] = sa.dk0k2;
+    D4 k1k2(c.p3); k1k2.d[0] = sa.dk1k2;
+    D4 Bm(c.Bmag); Bm.d[1] = c.Bmag;
+    const D4 g = D4::seed(fr.g.v, 2), cosB = D4::seed(fr.cosB.v, 3);
+    D4 nuf = nu / g;
+    D4 cb = dmin(dmax(cosB, -1.0), 1.0);
+    D4 thB = acos(cb);
+    D4 sth, cth;
     sincos(thB, &sth, &cth);
     const S ne = exp(c.lnA) * Gw;
-    CoefT<D5> cf = thermal(ne, Th, k0k2, k1k2, Bm, nuf, sth, cth);
+    CoefT<D4> cf = thermal(ne, Th, k0k2, k1k2, Bm, nuf, sth, cth);
     if (!(cf.jI > 0.0 || cf.aI > 0.0 || cf.rQ.v != 0.0 || cf.rV.v != 0.0)) return;
-    D5 fe = cap_factor(cf.jI, cf.jQ, cf.jV), fa = cap_factor(cf.aI, cf.aQ, cf.aV);
-    D5 nh = nuf * (1.0 / NU0);
-    D5 i2 = 1.0 / (nh * nh);
-    D5 s2c, c2c;
-    sincos(2.0 * chi, &s2c, &c2c);
-    D5 jQ = fe * cf.jQ * i2, aQ = fa * cf.aQ * nh, rQ = cf.rQ * nh;
-    D5 q = bars[0] * (cf.jI * i2) + bars[1] * (jQ * c2c) + bars[2] * (jQ * s2c) + bars[3] * (fe * cf.jV * i2);
-    q = q + bars[4] * (cf.aI * nh) + bars[5] * (aQ * c2c) + bars[6] * (aQ * s2c) + bars[7] * (fa * cf.aV * nh);
-    q = q + bars[8] * (rQ * c2c) + bars[9] * (rQ * s2c) + bars[10] * (cf.rV * nh);
+    D4 fe = cap_factor(cf.jI, cf.jQ, cf.jV), fa = cap_factor(cf.aI, cf.aQ, cf.aV);
+    D4 nh = nuf * (1.0 / NU0);
+    D4 i2 = 1.0 / (nh * nh);
+    S s2c, c2c;
+    sincos(2.0 * fr.chi.v, &s2c, &c2c);
+    D4 jQ = fe * cf.jQ * i2, aQ = fa * cf.aQ * nh, rQ = cf.rQ * nh;
+    D4 A = bars[0] * (cf.jI * i2) + bars[3] * (fe * cf.jV * i2) + bars[4] * (cf.aI * nh) + bars[7] * (fa * cf.aV * nh) + bars[10] * (cf.rV * nh);
+    D4 Bc = bars[1] * jQ + bars[5] * aQ + bars[8] * rQ;
+    D4 Bs = bars[2] * jQ + bars[6] * aQ + bars[9] * rQ;
+    D4 q = A + Bc * c2c + Bs * s2c;
+    const S qchi = 2.0 * (Bs.v * c2c - Bc.v * s2c);
     const S sv = q.v;
     for (int r = 0; r < 12; ++r) grad[r] += sv * dl[r];
     grad[12] += sv;
     grad[13] += q.d[0];
     grad[14] += q.d[1];
-    for (int k = 0; k < 5; ++k) grad[15 + k] += q.d[2] * fr.g.d[k] + q.d[3] * fr.cosB.d[k] + q.d[4] * fr.chi.d[k];
+    for (int k = 0; k < 5; ++k) grad[15 + k] += q.d[2] * fr.g.d[k] + q.d[3] * fr.cosB.d[k] + qchi * fr.chi.d[k];
     grad[20] += sv * dl[12];
 }
 

@@ -119,7 +119,7 @@ inline S weight_dlog_analytic(const Model& m, int i, S t, S x, S y, S z, S dl[13
 
 // one thermal splat's share of sample s (dual_gradient.jl:46-76); grad is the splat's 21-row column.
 // ANALYTIC selects the device's form: the hand-written d ln G instead of the 13-partial duals, and the
-// seven fluid partials in two five-partial passes joined by the chain rule instead of one 7-partial pass.
+// seven fluid partials in two passes joined by the chain rule instead of one 7-partial pass.
 template <class S, bool ANALYTIC = false>
 inline void element_adjoint(const Model& m, int i, const RayCtx& rc, const Sample<S>& s, S t_obs, S nu,
                             const S jbar[4], const S abar[4], const S rbar[3], S* grad) {
@@ -165,7 +165,8 @@ inline void element_adjoint(const Model& m, int i, const RayCtx& rc, const Sampl
     S ne = kexp(S(colv[12])) * G;
     if constexpr (ANALYTIC) {
         // the device's form: the frame (g, cos B, chi) with five partials (theta_B, phi_B, u1..u3), the
-        // contribution with five (ln Theta_e, ln B, g, cos B, chi), joined by the chain rule
+        // contribution with four (ln Theta_e, ln B, g, cos B) and its chi derivative written out, joined by
+        // the chain rule
         using D5 = Dual<5, S>;
         D5 st = ksin(D5::seed(S(colv[15]), 0)), ct = kcos(D5::seed(S(colv[15]), 0));
         D5 sph = ksin(D5::seed(S(colv[16]), 1)), cph = kcos(D5::seed(S(colv[16]), 1));
@@ -173,27 +174,31 @@ inline void element_adjoint(const Model& m, int i, const RayCtx& rc, const Sampl
         D5 u[3] = {D5::seed(S(colv[17]), 2), D5::seed(S(colv[18]), 3), D5::seed(S(colv[19]), 4)};
         Frame<D5> fr = local_frame<D5>(D5(rc.a), D5(s.r), D5(s.th), D5(rc.eta), D5(rc.lam), s.nur, s.nuth, D5(rc.alpha), D5(rc.beta),
                                        D5(rc.thetao), u, Bh);
-        D5 Th = kexp(D5::seed(S(colv[13]), 0));
-        D5 Bmag = kexp(D5::seed(S(colv[14]), 1));
-        Frame<D5> fq;
-        fq.g = D5::seed(fr.g.v, 2); fq.cosB = D5::seed(fr.cosB.v, 3); fq.chi = D5::seed(fr.chi.v, 4);
-        D5 nuf = D5(nu) / fq.g;
-        D5 thB = kacos(kmin(kmax(fq.cosB, D5(-1.0)), D5(1.0)));
-        D5 xb = D5(1.0) / Th;
-        D5 k0 = chain(S(kb[0]), S(-kb[1]), xb), k1 = chain(S(kb[1]), S(-kb[0] - kb[1] / xb.v), xb);
-        D5 k2 = k0 + 2.0 * k1 / xb;
-        Coef<D5> cf = thermal_synchrotron_k<D5>(D5(ne), Th, Bmag, nuf, thB, k0, k1, k2);
-        Screen<D5> sc;
-        if (!screen_coefficients<D5>(cf, fq, D5(nu), sc)) return;
-        D5 q(0.0);
-        for (int k = 0; k < 4; ++k) q = q + D5(jbar[k]) * sc.j[k] + D5(abar[k]) * sc.a[k];
-        for (int k = 0; k < 3; ++k) q = q + D5(rbar[k]) * sc.r[k];
+        using D4 = Dual<4, S>;
+        D4 Th = kexp(D4::seed(S(colv[13]), 0));
+        D4 Bmag = kexp(D4::seed(S(colv[14]), 1));
+        D4 g = D4::seed(fr.g.v, 2), cosB = D4::seed(fr.cosB.v, 3);
+        D4 nuf = D4(nu) / g;
+        D4 thB = kacos(kmin(kmax(cosB, D4(-1.0)), D4(1.0)));
+        D4 xb = D4(1.0) / Th;
+        D4 k0 = chain(S(kb[0]), S(-kb[1]), xb), k1 = chain(S(kb[1]), S(-kb[0] - kb[1] / xb.v), xb);
+        D4 k2 = k0 + 2.0 * k1 / xb;
+        Coef<D4> cf = thermal_synchrotron_k<D4>(D4(ne), Th, Bmag, nuf, thB, k0, k1, k2);
+        if (!(cf.jI > 0.0 || cf.aI > 0.0 || val(cf.rQ) != 0.0 || val(cf.rV) != 0.0)) return;
+        // the screen coefficients before the rotation by 2 chi, whose derivative is written out
+        Coef<D4> ci = invariants(cap_polarization(cf), nuf);
+        D4 A = D4(jbar[0]) * ci.jI + D4(jbar[3]) * ci.jV + D4(abar[0]) * ci.aI + D4(abar[3]) * ci.aV + D4(rbar[2]) * ci.rV;
+        D4 Bc = D4(jbar[1]) * ci.jQ + D4(abar[1]) * ci.aQ + D4(rbar[0]) * ci.rQ;
+        D4 Bs = D4(jbar[2]) * ci.jQ + D4(abar[2]) * ci.aQ + D4(rbar[1]) * ci.rQ;
+        S s2 = ksin(2.0 * fr.chi.v), c2 = kcos(2.0 * fr.chi.v);
+        D4 q = A + Bc * D4(c2) + Bs * D4(s2);
+        S qchi = 2.0 * (Bs.v * c2 - Bc.v * s2);
         S sv = q.v;
         for (int r = 0; r < 12; ++r) grad[r] = grad[r] + sv * dlnG[r];
         grad[12] = grad[12] + sv;
         grad[13] = grad[13] + q.d[0];
         grad[14] = grad[14] + q.d[1];
-        for (int k = 0; k < 5; ++k) grad[15 + k] = grad[15 + k] + q.d[2] * fr.g.d[k] + q.d[3] * fr.cosB.d[k] + q.d[4] * fr.chi.d[k];
+        for (int k = 0; k < 5; ++k) grad[15 + k] = grad[15 + k] + q.d[2] * fr.g.d[k] + q.d[3] * fr.cosB.d[k] + qchi * fr.chi.d[k];
         grad[nr - 1] = grad[nr - 1] + sv * dlnG[12];
     } else {
         // fluid rows with 7 partials (rows 13..19)
