@@ -50,7 +50,7 @@ inline int64_t adj_units(int64_t nitems, int N) { return ((nitems + 63) / 64) * 
 #define KS_WAVES_TRANSFER64 2
 #endif
 #ifndef KS_WAVES_TRANSFER32
-#define KS_WAVES_TRANSFER32 2
+#define KS_WAVES_TRANSFER32 3
 #endif
 #ifndef KS_WAVES_ADJSTEP64
 #define KS_WAVES_ADJSTEP64 1
