@@ -325,3 +325,36 @@ def spacetime_jacobian(x, camera, N, params, movie, frames=None, freqs=None, nma
             warnings.warn(f"spacetime Jacobian: {int(bad.sum())} of {len(r)} residuals have non-finite partials; their rows are zeroed")
         J[bad] = 0.0
     return r, J
+
+
+# ---- the diffractive scattering kernel of Sgr A* (scattering.jl:1-40) ------------------------------------
+class ScatteringKernel:
+    """ScatteringKernel(ν; fwhm_maj = 1.380, fwhm_min = 0.703, pa = 81.9) (scattering.jl:5-34): the
+    ensemble-average scattering of Sgr A* at frequency ν [Hz] as a Gaussian in the visibility domain, FWHM
+    fwhm_maj mas λ_cm² along the position angle pa (degrees east of north) and fwhm_min mas λ_cm² across it
+    (Johnson et al. 2018; ehtim's Bower et al. 2006 constants are 1.309, 0.640, 78). kernel(u, v) at
+    (u, v) in wavelengths is exp(-2π² (a u² + 2 c u v + b v²)), ehtim's covariance form; it is the taper
+    per baseline that FrameScans and the device χ² take."""
+
+    def __init__(self, nu, fwhm_maj=1.380, fwhm_min=0.703, pa=81.9):
+        lam_cm = 2.99792458e10 / nu
+        mas = 1e-3 * np.pi / (180 * 3600)
+        to_sigma = lam_cm ** 2 * mas / (2 * np.sqrt(2 * np.log(2)))
+        smaj, smin = fwhm_maj * to_sigma, fwhm_min * to_sigma
+        th = -np.deg2rad(pa)                      # the angle enters negated in ehtim's convention
+        ct, st = np.cos(th), np.sin(th)
+        self.a = (smin * ct) ** 2 + (smaj * st) ** 2
+        self.b = (smaj * ct) ** 2 + (smin * st) ** 2
+        self.c = (smin ** 2 - smaj ** 2) * ct * st
+
+    def __call__(self, u, v):
+        u, v = np.asarray(u, dtype=np.float64), np.asarray(v, dtype=np.float64)
+        return np.exp(-2 * np.pi ** 2 * (self.a * u * u + 2 * self.c * u * v + self.b * v * v))
+
+
+def taper(kernel, model, u, v):
+    """taper(kernel, model, u, v) (scattering.jl:36-38): model visibilities (nuv, 4) times the kernel at
+    each baseline; kernel None leaves them unchanged."""
+    if kernel is None:
+        return model
+    return np.asarray(model) * kernel(u, v)[:, None]
