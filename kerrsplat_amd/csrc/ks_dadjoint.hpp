@@ -3,9 +3,10 @@
 // Reference: sample_adjoint (src/Transfer/transport.jl:332-341) and element_adjoint! / ParamColumn
 // (src/Splats/dual_gradient.jl:22-76). Along a ray, l = w_1 . T_1 with T_k = E_k + O_k T_{k+1}; the
 // adjoints of sample k are dl/dO_k = w_k T_{k+1}^T and dl/dE_k = w_k, w_{k+1} = O_k^T w_k.
-//   * sample_adjoint: j_bar = Ej^T w exactly; alpha_bar, rho_bar from 7-partial duals through the
-//     step's polynomial coefficients (O = sum o_n K'^n, Ej = Dp sum e_n K'^n), contracted as
-//     sum_n o_n w.(K'^n T) + Dp e_n w.(K'^n j) so that no 4x4 dual matrix is formed.
+//   * sample_adjoint: j_bar = Ej^T w exactly; alpha_bar, rho_bar through the step's polynomial
+//     coefficients (O = sum o_n K'^n, Ej = Dp sum e_n K'^n), contracted as sum_n o_n w.(K'^n T) +
+//     Dp e_n w.(K'^n j) so that no 4x4 dual matrix is formed: four-partial duals over the invariants the
+//     coefficients depend on, six-partial duals over K' in the products, joined by the chain rule.
 //   * element_adjoint_thermal: one splat's share of one sample. Every coefficient is proportional to
 //     the density n_e G, so the 13 geometric rows (centre, widths, orientation, t0, temporal width,
 //     pattern rate) and ln n_e enter through (c_bar . c) d ln G, with d ln G written out by hand
@@ -32,37 +33,60 @@ using SplatAdjf = SplatAdjT<f32>;
 template <class S>
 KS_DEV void sample_adjoint(const S j4[4], const S a4[4], const S r3[3], S D, const S w[4], const S R[4],
                            S Otw[4], S jbar[4], S abar[4], S rbar[3]) {
-    using D7 = DN<7, S>;
-    // seeded on the Dp-scaled coefficients (O(1) derivatives, no FP32 overflow for near-transparent
-    // steps); the bars are scaled back by the power of two Dp at the end, which is exact
+    // q = w . (O R + Ej j) = sum_n o_n w.(K'^n R) + Dp e_n w.(K'^n j). The coefficients o_n, e_n depend on the
+    // seven (alpha, rho) only through four invariants (alpha_I, a2, r2, ar): four-partial duals through
+    // step_coefs_inv; the products w.(K'^n v) are polynomials in the six entries of K': six-partial duals
+    // through three cheap matrix-vector products. The chain rule joins them. Everything is seeded on the
+    // Dp-scaled coefficients (O(1) derivatives, no FP32 overflow for near-transparent steps); the bars are
+    // scaled back by the power of two Dp at the end, which is exact.
+    using D4 = DN<4, S>;
+    using D6 = DN<6, S>;
     const S Dp = pow2_below(D);
-    D7 al[4], rh[3];
-    for (int i = 0; i < 4; ++i) al[i] = D7::seed(a4[i] * Dp, i);
-    for (int i = 0; i < 3; ++i) rh[i] = D7::seed(r3[i] * Dp, 4 + i);
-    D7 K6[6], o[4], e4[4];
-    step_coefs_scaled(al, rh, S(D / Dp), K6, o, e4);
-    // q = w . (O R + Ej j)
-    S wR(0.0), wj(0.0);
-    for (int i = 0; i < 4; ++i) { wR += w[i] * R[i]; wj += w[i] * j4[i]; }
-    D7 q = o[0] * wR + (Dp * wj) * e4[0];
-    D7 vR[4], vj[4];
+    const S f = D / Dp;
+    S K6v[6];
+    const S aIs = a4[0] * Dp;
+    for (int i = 0; i < 3; ++i) { K6v[i] = a4[1 + i] * Dp; K6v[3 + i] = r3[i] * Dp; }
+    const S a2 = K6v[0] * K6v[0] + K6v[1] * K6v[1] + K6v[2] * K6v[2];
+    const S r2 = K6v[3] * K6v[3] + K6v[4] * K6v[4] + K6v[5] * K6v[5];
+    const S ar = K6v[0] * K6v[3] + K6v[1] * K6v[4] + K6v[2] * K6v[5];
+    D4 o[4], e4[4];
+    step_coefs_inv(D4::seed(aIs, 0), D4::seed(a2, 1), D4::seed(r2, 2), D4::seed(ar, 3), f, o, e4);
+    // sR[n] = w.(K'^n R), sj[n] = w.(K'^n j)
+    D6 K6[6];
+    for (int i = 0; i < 6; ++i) K6[i] = D6::seed(K6v[i], i);
+    S sR0(0.0), sj0(0.0);
+    for (int i = 0; i < 4; ++i) { sR0 += w[i] * R[i]; sj0 += w[i] * j4[i]; }
+    D6 vR[4], vj[4], sR[4], sj[4];
+    sR[0] = D6(sR0); sj[0] = D6(sj0);
     kprime_mul(K6, R, vR);
     kprime_mul(K6, j4, vj);
     for (int n = 1; n < 4; ++n) {
-        D7 sR = vR[0] * w[0] + vR[1] * w[1] + vR[2] * w[2] + vR[3] * w[3];
-        D7 sj = vj[0] * w[0] + vj[1] * w[1] + vj[2] * w[2] + vj[3] * w[3];
-        q = q + o[n] * sR + Dp * (e4[n] * sj);
+        sR[n] = vR[0] * w[0] + vR[1] * w[1] + vR[2] * w[2] + vR[3] * w[3];
+        sj[n] = vj[0] * w[0] + vj[1] * w[1] + vj[2] * w[2] + vj[3] * w[3];
         if (n < 3) {
-            D7 t4[4];
+            D6 t4[4];
             kprime_mul(K6, vR, t4); for (int i = 0; i < 4; ++i) vR[i] = t4[i];
             kprime_mul(K6, vj, t4); for (int i = 0; i < 4; ++i) vj[i] = t4[i];
         }
     }
-    for (int i = 0; i < 4; ++i) abar[i] = q.d[i] * Dp;
-    for (int i = 0; i < 3; ++i) rbar[i] = q.d[4 + i] * Dp;
+    // dq/d(invariant k) and dq/d(K' entry m) at fixed invariants
+    S gi[4], hm[6];
+    for (int k = 0; k < 4; ++k) {
+        S acc(0.0);
+        for (int n = 0; n < 4; ++n) acc += o[n].d[k] * sR[n].v + Dp * (e4[n].d[k] * sj[n].v);
+        gi[k] = acc;
+    }
+    for (int m = 0; m < 6; ++m) {
+        S acc(0.0);
+        for (int n = 1; n < 4; ++n) acc += o[n].v * sR[n].d[m] + Dp * (e4[n].v * sj[n].d[m]);
+        hm[m] = acc;
+    }
+    abar[0] = gi[0] * Dp;
+    for (int i = 0; i < 3; ++i) {
+        abar[1 + i] = (hm[i] + 2.0 * K6v[i] * gi[1] + K6v[3 + i] * gi[3]) * Dp;        // aQ, aU, aV
+        rbar[i] = (hm[3 + i] + 2.0 * K6v[3 + i] * gi[2] + K6v[i] * gi[3]) * Dp;        // rQ, rU, rV
+    }
     // values: O^T w = sum o_n K'^T^n w, Ej^T w = Dp sum e_n K'^T^n w
-    S K6v[6];
-    for (int i = 0; i < 6; ++i) K6v[i] = K6[i].v;
     S u[4], y[4];
     for (int i = 0; i < 4; ++i) { u[i] = w[i]; Otw[i] = o[0].v * w[i]; jbar[i] = Dp * e4[0].v * w[i]; }
     for (int n = 1; n < 4; ++n) {

@@ -437,12 +437,10 @@ template <int N, class S> KS_DEV double taylor_gate(const DN<N, S>&) { return ta
 // K'6 = (aQ, aU, aV, rQ, rU, rV) times Dp = pow2_below(D); D > 0. T is the scalar S or duals over it.
 // step_coefs_scaled takes the coefficients already multiplied by Dp (exact: Dp is a power of two)
 // and f = D / Dp in [1, 2), so duals seeded on the scaled values carry O(1) derivatives.
+// The coefficients depend on K' only through its invariants: step_coefs_inv takes the scaled alpha_I, the
+// squared norms a2 = |(aQ, aU, aV)|^2, r2 = |(rQ, rU, rV)|^2 and ar = (aQ, aU, aV).(rQ, rU, rV).
 template <class T, class S>
-KS_DEV void step_coefs_scaled(const T alS[4], const T rhS[3], S f, T K6[6], T o[4], T e4[4]) {
-    const T &aI = alS[0], &aQ = alS[1], &aU = alS[2], &aV = alS[3];
-    const T &rQ = rhS[0], &rU = rhS[1], &rV = rhS[2];
-    K6[0] = aQ; K6[1] = aU; K6[2] = aV; K6[3] = rQ; K6[4] = rU; K6[5] = rV;
-    T a2 = aQ * aQ + aU * aU + aV * aV, r2 = rQ * rQ + rU * rU + rV * rV;
+KS_DEV void step_coefs_inv(T aI, T a2, T r2, T ar, S f, T o[4], T e4[4]) {
     T a = aI * f;
     T e = exp(-a);
     if (val(a2 + r2) * f * f < 1e-200) {
@@ -451,7 +449,6 @@ KS_DEV void step_coefs_scaled(const T alS[4], const T rhS[3], S f, T K6[6], T o[
         e4[0] = f * phi1(a); e4[1] = -f * f * M[1]; e4[2] = e4[3] = T(0.0);
         return;
     }
-    T ar = aQ * rQ + aU * rU + aV * rV;
     T h = (a2 - r2) * 0.5;
     T root = sqrt(h * h + ar * ar);
     T L1sq, L2sq;
@@ -517,6 +514,15 @@ KS_DEV void step_coefs_scaled(const T alS[4], const T rhS[3], S f, T K6[6], T o[
     e4[2] = f * (icm1 + ivs) * iTh;
     e4[1] = -(Ish * L2sq + Is * L1sq) * iTh;
     e4[3] = -(Ish - Is) * iTh;
+}
+template <class T, class S>
+KS_DEV void step_coefs_scaled(const T alS[4], const T rhS[3], S f, T K6[6], T o[4], T e4[4]) {
+    const T &aQ = alS[1], &aU = alS[2], &aV = alS[3];
+    const T &rQ = rhS[0], &rU = rhS[1], &rV = rhS[2];
+    K6[0] = aQ; K6[1] = aU; K6[2] = aV; K6[3] = rQ; K6[4] = rU; K6[5] = rV;
+    const T a2 = aQ * aQ + aU * aU + aV * aV, r2 = rQ * rQ + rU * rU + rV * rV;
+    const T ar = aQ * rQ + aU * rU + aV * rV;
+    step_coefs_inv(alS[0], a2, r2, ar, f, o, e4);
 }
 template <class T, class S>
 KS_DEV void step_coefs(const T al[4], const T rh[3], S D, S& Dp, T K6[6], T o[4], T e4[4]) {

@@ -21,9 +21,66 @@ namespace kso {
 // coefficients, contracted as sum_n o_n w.(K'^n R) + Dp e_n w.(K'^n j) (the same function as the
 // ForwardDiff derivative of the assembled matrices, without forming 4x4 dual matrices). Returns
 // O^T w (the next adjoint), E = Ej j, j_bar = Ej^T w, alpha_bar and rho_bar.
-template <class S>
+template <class S, bool SPLIT = false>
 inline void sample_adjoint(const S j4[4], const S a4[4], const S r3[3], S D, const S w[4], const S R[4],
                            S Otw[4], S E[4], S jbar[4], S abar[4], S rbar[3]) {
+    if constexpr (SPLIT) {
+        // the device's form: four-partial duals over the invariants (alpha_I, a2, r2, ar) of the scaled K
+        // through the coefficients, six-partial duals over the scaled K' in the products w.(K'^n v), joined
+        // by the chain rule; the bars are the derivatives with respect to the scaled entries times Dp
+        using D4 = Dual<4, S>;
+        using D6 = Dual<6, S>;
+        S Dp = pow2_below(D), f = D / Dp;
+        S K6v[6];
+        S aIs = a4[0] * Dp;
+        for (int i = 0; i < 3; ++i) { K6v[i] = a4[1 + i] * Dp; K6v[3 + i] = r3[i] * Dp; }
+        S a2 = K6v[0] * K6v[0] + K6v[1] * K6v[1] + K6v[2] * K6v[2];
+        S r2 = K6v[3] * K6v[3] + K6v[4] * K6v[4] + K6v[5] * K6v[5];
+        S ar = K6v[0] * K6v[3] + K6v[1] * K6v[4] + K6v[2] * K6v[5];
+        D4 o[4], e4[4];
+        step_coefficients_inv<D4>(D4::seed(aIs, 0), D4::seed(a2, 1), D4::seed(r2, 2), D4::seed(ar, 3), D4(f), o, e4);
+        D6 K6[6];
+        for (int i = 0; i < 6; ++i) K6[i] = D6::seed(K6v[i], i);
+        D6 vR[4], vj[4], t4[4], sR[4], sj[4];
+        S sR0(0.0), sj0(0.0);
+        for (int i = 0; i < 4; ++i) { sR0 = sR0 + w[i] * R[i]; sj0 = sj0 + w[i] * j4[i]; vR[i] = D6(R[i]); vj[i] = D6(j4[i]); }
+        sR[0] = D6(sR0); sj[0] = D6(sj0);
+        for (int n = 1; n < 4; ++n) {
+            kprime_mul(K6, vR, t4); for (int i = 0; i < 4; ++i) vR[i] = t4[i];
+            kprime_mul(K6, vj, t4); for (int i = 0; i < 4; ++i) vj[i] = t4[i];
+            sR[n] = vR[0] * D6(w[0]) + vR[1] * D6(w[1]) + vR[2] * D6(w[2]) + vR[3] * D6(w[3]);
+            sj[n] = vj[0] * D6(w[0]) + vj[1] * D6(w[1]) + vj[2] * D6(w[2]) + vj[3] * D6(w[3]);
+        }
+        S gi[4], hm[6];
+        for (int k = 0; k < 4; ++k) {
+            S acc(0.0);
+            for (int n = 0; n < 4; ++n) acc = acc + o[n].d[k] * sR[n].v + Dp * (e4[n].d[k] * sj[n].v);
+            gi[k] = acc;
+        }
+        for (int m = 0; m < 6; ++m) {
+            S acc(0.0);
+            for (int n = 1; n < 4; ++n) acc = acc + o[n].v * sR[n].d[m] + Dp * (e4[n].v * sj[n].d[m]);
+            hm[m] = acc;
+        }
+        abar[0] = gi[0] * Dp;
+        for (int i = 0; i < 3; ++i) {
+            abar[1 + i] = (hm[i] + 2.0 * K6v[i] * gi[1] + K6v[3 + i] * gi[3]) * Dp;
+            rbar[i] = (hm[3 + i] + 2.0 * K6v[3 + i] * gi[2] + K6v[i] * gi[3]) * Dp;
+        }
+        S u[4], y[4], z[4], v[4];
+        for (int i = 0; i < 4; ++i) {
+            u[i] = w[i]; v[i] = j4[i];
+            Otw[i] = o[0].v * w[i]; jbar[i] = Dp * e4[0].v * w[i]; E[i] = Dp * e4[0].v * j4[i];
+        }
+        for (int n = 1; n < 4; ++n) {
+            kprime_tmul(K6v, u, y); kprime_mul(K6v, v, z);
+            for (int i = 0; i < 4; ++i) {
+                u[i] = y[i]; v[i] = z[i];
+                Otw[i] = Otw[i] + o[n].v * u[i]; jbar[i] = jbar[i] + Dp * e4[n].v * u[i]; E[i] = E[i] + Dp * e4[n].v * v[i];
+            }
+        }
+        return;
+    }
     using D7 = Dual<7, S>;
     D7 ad[4], rd[3];
     for (int i = 0; i < 4; ++i) ad[i] = D7::seed(a4[i], i);
@@ -119,7 +176,8 @@ inline S weight_dlog_analytic(const Model& m, int i, S t, S x, S y, S z, S dl[13
 
 // one thermal splat's share of sample s (dual_gradient.jl:46-76); grad is the splat's 21-row column.
 // ANALYTIC selects the device's form: the hand-written d ln G instead of the 13-partial duals, and the
-// seven fluid partials in two passes joined by the chain rule instead of one 7-partial pass.
+// seven fluid partials in two passes joined by the chain rule instead of one 7-partial pass (the sweep
+// passes it on to sample_adjoint, whose device form splits the same way).
 template <class S, bool ANALYTIC = false>
 inline void element_adjoint(const Model& m, int i, const RayCtx& rc, const Sample<S>& s, S t_obs, S nu,
                             const S jbar[4], const S abar[4], const S rbar[3], S* grad) {
@@ -296,7 +354,7 @@ inline void polarized_gradient_ray(const Model& m, double a, double thetao, doub
         if (!q.on) continue;
         S Otw[4], E[4], jb[4], ab[4], rb[3];
         const OpCounts before = op_counts();
-        sample_adjoint<S>(q.j4, q.a4, q.r3, q.D, w, &tails[4 * k], Otw, E, jb, ab, rb);
+        sample_adjoint<S, ANALYTIC>(q.j4, q.a4, q.r3, q.D, w, &tails[4 * k], Otw, E, jb, ab, rb);
         in_sample.flops += op_counts().flops - before.flops;
         in_sample.special += op_counts().special - before.special;
         for (int i = 0; i < m.ns; ++i) element_adjoint<S, ANALYTIC>(m, i, rc, Sm[k - 1], tS, nuS, jb, ab, rb, grad + (size_t)m.nrows * i);

@@ -411,14 +411,9 @@ template <class T> T sinc_(T b) { return val(kabs(b)) < 0.1 ? 1.0 - b * b / 6.0 
 // The step as polynomials in K' (K = alpha_I + K'), the coefficients of step.jl:164-243:
 // O = o0 + o1 K' + o2 K'^2 + o3 K'^3, Ej = Dp (e0 + e1 K' + e2 K'^2 + e3 K'^3), with
 // K6 = (aQ, aU, aV, rQ, rU, rV) Dp the entries of K' after the power-of-two rescaling of the length.
-template <class T> void step_coefficients(const T al[4], const T rh[3], T D, T& Dp, T K6[6], T o[4], T e4[4]) {
-    // exact power-of-two rescaling of the length (step.jl:164-173)
-    Dp = pow2_below(D);
-    T f = D / Dp;
-    T aI = al[0] * Dp, aQ = al[1] * Dp, aU = al[2] * Dp, aV = al[3] * Dp;
-    T rQ = rh[0] * Dp, rU = rh[1] * Dp, rV = rh[2] * Dp;
-    K6[0] = aQ; K6[1] = aU; K6[2] = aV; K6[3] = rQ; K6[4] = rU; K6[5] = rV;
-    T a2 = aQ * aQ + aU * aU + aV * aV, r2 = rQ * rQ + rU * rU + rV * rV;
+// the coefficients from the invariants of the scaled K = alpha_I + K': aI (scaled), a2 = |(aQ, aU, aV)|^2,
+// r2 = |(rQ, rU, rV)|^2, ar = (aQ, aU, aV).(rQ, rU, rV), and f = D / Dp (all the step depends on)
+template <class T> void step_coefficients_inv(T aI, T a2, T r2, T ar, T f, T o[4], T e4[4]) {
     T a = aI * f;
     T e = kexp(-a);
     if ((a2 + r2) * f * f < 1e-200) {
@@ -427,7 +422,6 @@ template <class T> void step_coefficients(const T al[4], const T rh[3], T D, T& 
         e4[0] = f * phi1(a); e4[1] = -f * f * M[1]; e4[2] = T(0.0); e4[3] = T(0.0);
         return;
     }
-    T ar = aQ * rQ + aU * rU + aV * rV;
     T h = (a2 - r2) / 2.0;
     T root = ksqrt(h * h + ar * ar);
     T L1sq, L2sq;
@@ -468,6 +462,17 @@ template <class T> void step_coefficients(const T al[4], const T rh[3], T D, T& 
     e4[1] = -(Ish * L2sq + Is * L1sq) / Th;
     e4[2] = f * (int_coshm1(a, b1, M) + int_versin(a, b2, M)) / Th;
     e4[3] = -(Ish - Is) / Th;
+}
+template <class T> void step_coefficients(const T al[4], const T rh[3], T D, T& Dp, T K6[6], T o[4], T e4[4]) {
+    // exact power-of-two rescaling of the length (step.jl:164-173)
+    Dp = pow2_below(D);
+    T f = D / Dp;
+    T aQ = al[1] * Dp, aU = al[2] * Dp, aV = al[3] * Dp;
+    T rQ = rh[0] * Dp, rU = rh[1] * Dp, rV = rh[2] * Dp;
+    K6[0] = aQ; K6[1] = aU; K6[2] = aV; K6[3] = rQ; K6[4] = rU; K6[5] = rV;
+    T a2 = aQ * aQ + aU * aU + aV * aV, r2 = rQ * rQ + rU * rU + rV * rV;
+    T ar = aQ * rQ + aU * rU + aV * rV;
+    step_coefficients_inv(al[0] * Dp, a2, r2, ar, f, o, e4);
 }
 
 // y = K' v and y = K'^T v (K'6 as above), for any mix of scalar types
