@@ -147,25 +147,95 @@ KS_DEV S weight_dlog(const SplatCT<S>& c, const SplatAdjT<S>& sa, S t, S x, S y,
     return G;
 }
 
+// The frame of local_frame (g, cos B, chi) and its partials with respect to (theta_B, phi_B, u1, u2, u3):
+// the boost of the photon momentum (g, the wave vector k) forward in three-partial duals over u, cos B = k . bh
+// from those, and chi by one reverse sweep of its own chain (the polarisation direction f = k x bh, the
+// inverse boost, the Walker-Penrose constant, the screen angle) joined to the partials of k and of the boost's
+// q = 1 / (gamma + 1). chi depends on the unit direction h = f / |f| only, so the normalisation adds
+// nothing to its partials (hbar . h = 0, fbar = hbar / |f|).
+template <class S> struct FrameJ { S g, cosB, chi, dg[5], dcosB[5], dchi[5]; };
+
+template <class S>
+KS_DEV FrameJ<S> frame_jacobian(const GeoT<S>& G, S a, S alpha, S beta, S thetao_sin, const S uv[3], S sthB, S cthB, S sphB, S cphB) {
+    using D3 = DN<3, S>;
+    const S bh0 = sthB * cphB, bh1 = sthB * sphB, bh2 = cthB;
+    const S bt0 = cthB * cphB, bt1 = cthB * sphB, bt2 = -sthB;     // d bh / d theta_B
+    const S bp0 = -bh1, bp1 = bh0;                                  // d bh / d phi_B (third entry 0)
+    const D3 u0 = D3::seed(uv[0], 0), u1 = D3::seed(uv[1], 1), u2 = D3::seed(uv[2], 2);
+    const D3 gam = sqrt(1.0 + (u0 * u0 + u1 * u1 + u2 * u2));
+    const D3 q = 1.0 / (gam + 1.0);
+    const D3 ud = u0 * G.pz[1] + u1 * G.pz[2] + u2 * G.pz[3];
+    const D3 c1 = ud * q - G.pz[0];
+    const D3 g = 1.0 / (gam * G.pz[0] - ud);
+    const D3 k0 = (G.pz[1] + u0 * c1) * g, k1 = (G.pz[2] + u1 * c1) * g, k2 = (G.pz[3] + u2 * c1) * g;
+    const D3 cosB = k0 * bh0 + k1 * bh1 + k2 * bh2;
+    FrameJ<S> o;
+    o.g = g.v; o.cosB = cosB.v;
+    o.dg[0] = o.dg[1] = S(0.0);
+    o.dcosB[0] = k0.v * bt0 + k1.v * bt1 + k2.v * bt2;
+    o.dcosB[1] = k0.v * bp0 + k1.v * bp1;
+    for (int i = 0; i < 3; ++i) { o.dg[2 + i] = g.d[i]; o.dcosB[2 + i] = cosB.d[i]; }
+    for (int i = 0; i < 5; ++i) o.dchi[i] = S(0.0);
+    o.chi = S(0.0);
+    const S f0 = k1.v * bh2 - k2.v * bh1, f1 = k2.v * bh0 - k0.v * bh2, f2 = k0.v * bh1 - k1.v * bh0;
+    const S fn = sqrt(f0 * f0 + f1 * f1 + f2 * f2);
+    if (!(fn > 0.0)) return o;
+    // forward values (the expressions of local_frame)
+    const S ifn = 1.0 / fn;
+    const S h0 = f0 * ifn, h1 = f1 * ifn, h2 = f2 * ifn;
+    const S uf = u0.v * h0 + u1.v * h1 + u2.v * h2;
+    const S ufq = uf * q.v;
+    const S fz1 = h0 + u0.v * ufq, fz2 = h1 + u1.v * ufq, fz3 = h2 + u2.v * ufq;
+    const S ft = uf * (1.0 / G.lapse);
+    const S fr_ = fz1 * (1.0 / G.sgrr);
+    const S fth = fz3 * (-1.0 / G.sgthth);
+    const S fph = fz2 * (1.0 / G.sgpp) + G.omega * ft;
+    const S* p = G.pu;
+    const S C1 = a * G.st * G.st, C2 = G.r * G.r + a * a, act = a * G.ct;
+    const S A = p[0] * fr_ - p[1] * ft + C1 * (p[1] * fph - p[3] * fr_);
+    const S Bw = (C2 * (p[3] * fth - p[2] * fph) - a * (p[0] * fth - p[2] * ft)) * G.st;
+    const S kk1 = A * G.r - Bw * act, kk2 = -(A * act + Bw * G.r);
+    const S mu = -(alpha + a * thetao_sin);
+    const S ea = beta * kk2 - mu * kk1, eb = beta * kk1 + mu * kk2;
+    o.chi = atan2(-ea, eb);
+    // reverse: d chi = (ea d eb - eb d ea) / (ea^2 + eb^2)
+    const S den = ea * ea + eb * eb;
+    if (!(den > 0.0)) return o;
+    const S w = 1.0 / den;
+    const S eab = -eb * w, ebb = ea * w;
+    const S kk1b = beta * ebb - mu * eab, kk2b = beta * eab + mu * ebb;
+    const S Ab = G.r * kk1b - act * kk2b, Bws = -(act * kk1b + G.r * kk2b) * G.st;
+    const S fthb = Bws * (C2 * p[3] - a * p[0]);
+    const S fphb = Ab * (C1 * p[1]) - Bws * (C2 * p[2]);
+    const S ftb = Bws * (a * p[2]) - Ab * p[1] + fphb * G.omega;
+    const S frb = Ab * (p[0] - C1 * p[3]);
+    const S z1b = frb * (1.0 / G.sgrr), z2b = fphb * (1.0 / G.sgpp), z3b = fthb * (-1.0 / G.sgthth);
+    const S ufqb = z1b * u0.v + z2b * u1.v + z3b * u2.v;
+    const S ufb = ftb * (1.0 / G.lapse) + ufqb * q.v;
+    const S qb = ufqb * uf;
+    const S fb0 = (z1b + ufb * u0.v) * ifn, fb1 = (z2b + ufb * u1.v) * ifn, fb2 = (z3b + ufb * u2.v) * ifn;
+    // f = k x bh: kbar = bh x fbar, bhbar = fbar x k
+    const S kb0 = bh1 * fb2 - bh2 * fb1, kb1 = bh2 * fb0 - bh0 * fb2, kb2 = bh0 * fb1 - bh1 * fb0;
+    const S bb0 = fb1 * k2.v - fb2 * k1.v, bb1 = fb2 * k0.v - fb0 * k2.v, bb2 = fb0 * k1.v - fb1 * k0.v;
+    o.dchi[0] = bb0 * bt0 + bb1 * bt1 + bb2 * bt2;
+    o.dchi[1] = bb0 * bp0 + bb1 * bp1;
+    const S ub[3] = {z1b * ufq + ufb * h0, z2b * ufq + ufb * h1, z3b * ufq + ufb * h2};
+    for (int i = 0; i < 3; ++i) o.dchi[2 + i] = ub[i] + qb * q.d[i] + kb0 * k0.d[i] + kb1 * k1.d[i] + kb2 * k2.d[i];
+    return o;
+}
+
 // One thermal splat's share of one sample (dual_gradient.jl:46-76), given the splat's weight Gw > 0 at
 // the sample and d ln G (weight_dlog): adds into grad (the splat's 21-row column).
 // bars = (j_bar[4], alpha_bar[4], rho_bar[3]).
-// The seven fluid partials (ln Theta_e, ln B, theta_B, phi_B, u1..u3) are taken in two passes joined by the
-// chain rule: the frame (g, cos B, chi) with respect to (theta_B, phi_B, u1..u3), and the sample's
-// contribution q with respect to (ln Theta_e, ln B, g, cos B), with dq/dchi written out. The frame enters q
-// only through those three values, so the products are the seven-partial derivatives at a fraction of the
-// width.
+// The seven fluid partials (ln Theta_e, ln B, theta_B, phi_B, u1..u3) are joined by the chain rule from the
+// frame (g, cos B, chi) with respect to (theta_B, phi_B, u1..u3) (frame_jacobian) and the sample's
+// contribution q with respect to (ln Theta_e, ln B, g, cos B) in four-partial duals, with dq/dchi written
+// out. The frame enters q only through those three values, so the products are the seven-partial derivatives.
 template <class S>
 KS_DEV void element_adjoint_thermal(const SplatCT<S>& c, const SplatAdjT<S>& sa, const GeoT<S>& G, S a, S alpha, S beta,
                                     S thetao_sin, S nu, const S bars[11], S Gw, const S dl[13], S grad[21]) {
-    using D5 = DN<5, S>;
-    // the frame: partials theta_B, phi_B, u1, u2, u3
-    D5 u[3] = {D5::seed(c.u[0], 2), D5::seed(c.u[1], 3), D5::seed(c.u[2], 4)};
-    D5 bh[3];
-    bh[0] = D5(sa.sthB * sa.cphB); bh[0].d[0] = sa.cthB * sa.cphB; bh[0].d[1] = -sa.sthB * sa.sphB;
-    bh[1] = D5(sa.sthB * sa.sphB); bh[1].d[0] = sa.cthB * sa.sphB; bh[1].d[1] = sa.sthB * sa.cphB;
-    bh[2] = D5(sa.cthB);           bh[2].d[0] = -sa.sthB;
-    const FrameT<D5> fr = local_frame(G, a, alpha, beta, thetao_sin, u, bh);
+    const S uv[3] = {c.u[0], c.u[1], c.u[2]};
+    const FrameJ<S> fr = frame_jacobian(G, a, alpha, beta, thetao_sin, uv, sa.sthB, sa.cthB, sa.sphB, sa.cphB);
     // the contribution: partials ln Theta_e, ln B, g, cos B; chi enters only through the screen rotation,
     // q = A + Bc cos 2chi + Bs sin 2chi with A, Bc, Bs free of chi, so its derivative is written out
     using D4 = DN<4, S>;
@@ -173,7 +243,7 @@ KS_DEV void element_adjoint_thermal(const SplatCT<S>& c, const SplatAdjT<S>& sa,
     D4 k0k2(c.p2); k0k2.d[0] = sa.dk0k2;
     D4 k1k2(c.p3); k1k2.d[0] = sa.dk1k2;
     D4 Bm(c.Bmag); Bm.d[1] = c.Bmag;
-    const D4 g = D4::seed(fr.g.v, 2), cosB = D4::seed(fr.cosB.v, 3);
+    const D4 g = D4::seed(fr.g, 2), cosB = D4::seed(fr.cosB, 3);
     D4 nuf = nu / g;
     D4 cb = dmin(dmax(cosB, -1.0), 1.0);
     D4 thB = acos(cb);
@@ -186,7 +256,7 @@ KS_DEV void element_adjoint_thermal(const SplatCT<S>& c, const SplatAdjT<S>& sa,
     D4 nh = nuf * (1.0 / NU0);
     D4 i2 = 1.0 / (nh * nh);
     S s2c, c2c;
-    sincos(2.0 * fr.chi.v, &s2c, &c2c);
+    sincos(2.0 * fr.chi, &s2c, &c2c);
     D4 jQ = fe * cf.jQ * i2, aQ = fa * cf.aQ * nh, rQ = cf.rQ * nh;
     D4 A = bars[0] * (cf.jI * i2) + bars[3] * (fe * cf.jV * i2) + bars[4] * (cf.aI * nh) + bars[7] * (fa * cf.aV * nh) + bars[10] * (cf.rV * nh);
     D4 Bc = bars[1] * jQ + bars[5] * aQ + bars[8] * rQ;
@@ -198,7 +268,7 @@ KS_DEV void element_adjoint_thermal(const SplatCT<S>& c, const SplatAdjT<S>& sa,
     grad[12] += sv;
     grad[13] += q.d[0];
     grad[14] += q.d[1];
-    for (int k = 0; k < 5; ++k) grad[15 + k] += q.d[2] * fr.g.d[k] + q.d[3] * fr.cosB.d[k] + qchi * fr.chi.d[k];
+    for (int k = 0; k < 5; ++k) grad[15 + k] += q.d[2] * fr.dg[k] + q.d[3] * fr.dcosB[k] + qchi * fr.dchi[k];
     grad[20] += sv * dl[12];
 }
 

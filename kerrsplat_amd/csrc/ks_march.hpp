@@ -104,27 +104,29 @@ KS_DEV double polar_closed_form(const PixC& p, const QC& q, double tau, double s
     return p.lam * q.C * p.f * q.gc * ((1.0 + q.mpp * eps / (2.0 * q.nq)) * A - q.mpp / (2.0 * q.nq) * ph);
 }
 
+// Pa, Pb: polar_closed_form at the interval's ends (each end's value serves both intervals that share it)
 KS_DEV void interval_increment(const PixC& p, const Spin& S, const QC& q, const RP& pa, const RP& pm, const RP& pb, double ta, double tb,
-                               double& It, double& Iph) {
+                               double Pa, double Pb, double& It, double& Iph) {
     double h = tb - ta;
     double ra = pa.r, rb = pb.r;
     double dFr = (rb - ra) + 2.0 * log(rb / ra);
     double dFp = log((rb - S.rp) * ra / ((ra - S.rp) * rb)) / q.sp;
     double dFm = log((rb - S.rm) * ra / ((ra - S.rm) * rb)) / q.sm;
     It = h * (1.0 / 6.0) * (pa.gt + 4.0 * pm.gt + pb.gt) + fabs(dFr) + q.Qt * dFp + q.Qtm * dFm;
-    Iph = h * (1.0 / 6.0) * (pa.gph + 4.0 * pm.gph + pb.gph) + q.Qph * dFp + q.Qphm * dFm +
-          polar_closed_form(p, q, tb, pb.sn, pb.cn) - polar_closed_form(p, q, ta, pa.sn, pa.cn);
+    Iph = h * (1.0 / 6.0) * (pa.gph + 4.0 * pm.gph + pb.gph) + q.Qph * dFp + q.Qphm * dFm + Pb - Pa;
 }
 
 // the interval holding the radial turning point of a scattering ray, split there (quadrature.jl:255-270)
-KS_COLD void turning_increment(const PixC& p, const Spin& S, const QC& q, const RP& pa, const RP& pb, double ta, double tb, double& t, double& ph) {
+KS_COLD void turning_increment(const PixC& p, const Spin& S, const QC& q, const RP& pa, const RP& pb, double ta, double tb, double Pa, double Pb,
+                               double& t, double& ph) {
     RP pt = ray_point_turning(p, S, q);
+    const double Pt = polar_closed_form(p, q, p.fo, pt.sn, pt.cn);
     double t1 = 0.5 * (ta + p.fo), t2 = 0.5 * (p.fo + tb);
     RP p1 = ray_point(p, S, q, radial_state(p, t1), polar_state(p, t1), t1);
     RP p2 = ray_point(p, S, q, radial_state(p, t2), polar_state(p, t2), t2);
     double It1, Ip1, It2, Ip2;
-    interval_increment(p, S, q, pa, p1, pt, ta, p.fo, It1, Ip1);
-    interval_increment(p, S, q, pt, p2, pb, p.fo, tb, It2, Ip2);
+    interval_increment(p, S, q, pa, p1, pt, ta, p.fo, Pa, Pt, It1, Ip1);
+    interval_increment(p, S, q, pt, p2, pb, p.fo, tb, Pt, Pb, It2, Ip2);
     t += It1 + It2;
     ph += Ip1 + Ip2;
 }
@@ -171,6 +173,7 @@ KS_DEV void march_range(const PixC& p, const Spin& S, double thetao, int N, int 
     }
     double t, ph;
     t_phi_direct(p, S, (double)A * dtau, t, ph);
+    double Pa = polar_closed_form(p, q, (double)A * dtau, pa.sn, pa.cn);
     if (A == kbeg) {
         const double ta = (double)A * dtau;
         Samp s{t, pa.r, pa.th, ph, pa.nur, pa.nuth, (ta <= p.tau_h) && krang_visible(vis, pa.cth)};
@@ -187,11 +190,12 @@ KS_DEV void march_range(const PixC& p, const Spin& S, double thetao, int N, int 
         if (anchor) { xr = radial_state(p, tb); xt = polar_state(p, tb); }
         else { xr = js_step(xr, Dr, p.k); xt = js_step(xt, Dt, p.mu); }
         RP pb = ray_point(p, S, q, xr, xt, tb);
+        const double Pb = polar_closed_form(p, q, tb, pb.sn, pb.cn);
         if (q.scattering && ta < p.fo && p.fo < tb) {
-            turning_increment(p, S, q, pa, pb, ta, tb, t, ph);
+            turning_increment(p, S, q, pa, pb, ta, tb, Pa, Pb, t, ph);
         } else {
             double It, Ip;
-            interval_increment(p, S, q, pa, pmid, pb, ta, tb, It, Ip);
+            interval_increment(p, S, q, pa, pmid, pb, ta, tb, Pa, Pb, It, Ip);
             t += It;
             ph += Ip;
         }
@@ -211,6 +215,7 @@ KS_DEV void march_range(const PixC& p, const Spin& S, double thetao, int N, int 
             f(k, s, dtau);
         }
         pa = pb;
+        Pa = Pb;
     }
 }
 

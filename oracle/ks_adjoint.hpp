@@ -174,6 +174,82 @@ inline S weight_dlog_analytic(const Model& m, int i, S t, S x, S y, S z, S dl[13
     return G;
 }
 
+// The device's form of the frame partials: local_frame's (g, cos B, chi) by compact expressions (the boost
+// applied to the ZAMO momentum and, inverted, to the unit polarisation direction h = f / |f|) with their
+// partials over (theta_B, phi_B, u1, u2, u3): g and the wave vector k forward in three-partial duals over u,
+// cos B = k . bh from those, chi by one reverse sweep of its own chain joined to the partials of k and of
+// q = 1 / (gamma + 1). chi depends on h only, so hbar . h = 0 and fbar = hbar / |f|. Checked through the
+// whole gradient against the seven-partial duals of the other path.
+template <class S> struct FrameJ { S g, cosB, chi, dg[5], dcosB[5], dchi[5]; };
+
+template <class S>
+FrameJ<S> frame_jacobian(const RayCtx& rc, const Sample<S>& s, S thB, S phB, const S uv[3]) {
+    using D3 = Dual<3, S>;
+    const S a = S(rc.a);
+    S pd[4], pu[4], pz[4];
+    momentum_bl_d(a, s.r, s.th, S(rc.eta), S(rc.lam), s.nur, s.nuth, pd);
+    metric_uu_mul(a, s.r, s.th, pd, pu);
+    const Zamo<S> zm = zamo(a, s.r, s.th);
+    bl_to_zamo(zm, pu, pz);
+    const S st = ksin(s.th), ct = kcos(s.th);
+    const S sthB = ksin(thB), cthB = kcos(thB), sphB = ksin(phB), cphB = kcos(phB);
+    const S bh[3] = {sthB * cphB, sthB * sphB, cthB};
+    const S bt[3] = {cthB * cphB, cthB * sphB, -sthB}, bp[3] = {-sthB * sphB, sthB * cphB, S(0.0)};
+    const D3 u[3] = {D3::seed(uv[0], 0), D3::seed(uv[1], 1), D3::seed(uv[2], 2)};
+    const D3 gam = ksqrt(D3(1.0) + (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
+    const D3 q = D3(1.0) / (gam + D3(1.0));
+    const D3 ud = u[0] * D3(pz[1]) + u[1] * D3(pz[2]) + u[2] * D3(pz[3]);
+    const D3 c1 = ud * q - D3(pz[0]);
+    const D3 g = D3(1.0) / (gam * D3(pz[0]) - ud);
+    D3 k[3];
+    for (int i = 0; i < 3; ++i) k[i] = (D3(pz[1 + i]) + u[i] * c1) * g;
+    const D3 cosB = k[0] * D3(bh[0]) + k[1] * D3(bh[1]) + k[2] * D3(bh[2]);
+    FrameJ<S> o;
+    o.g = g.v; o.cosB = cosB.v; o.chi = S(0.0);
+    o.dg[0] = o.dg[1] = S(0.0);
+    o.dcosB[0] = k[0].v * bt[0] + k[1].v * bt[1] + k[2].v * bt[2];
+    o.dcosB[1] = k[0].v * bp[0] + k[1].v * bp[1] + k[2].v * bp[2];
+    for (int i = 0; i < 3; ++i) { o.dg[2 + i] = g.d[i]; o.dcosB[2 + i] = cosB.d[i]; }
+    for (int i = 0; i < 5; ++i) o.dchi[i] = S(0.0);
+    const S f[3] = {k[1].v * bh[2] - k[2].v * bh[1], k[2].v * bh[0] - k[0].v * bh[2], k[0].v * bh[1] - k[1].v * bh[0]};
+    const S fn = ksqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+    if (!(fn > 0.0)) return o;
+    const S ifn = 1.0 / fn;
+    const S h[3] = {f[0] * ifn, f[1] * ifn, f[2] * ifn};
+    const S uf = u[0].v * h[0] + u[1].v * h[1] + u[2].v * h[2];
+    const S ufq = uf * q.v;
+    const S C1 = a * st * st, C2 = s.r * s.r + a * a, act = a * ct;
+    const S mu = -(S(rc.alpha) + a * ksin(S(rc.thetao))), beta = S(rc.beta);
+    const S ft = uf / zm.lapse, fr = (h[0] + u[0].v * ufq) / zm.sgrr, fth = -(h[2] + u[2].v * ufq) / zm.sgthth;
+    const S fph = (h[1] + u[1].v * ufq) / zm.sgpp + zm.omega * ft;
+    const S A = pu[0] * fr - pu[1] * ft + C1 * (pu[1] * fph - pu[3] * fr);
+    const S Bw = (C2 * (pu[3] * fth - pu[2] * fph) - a * (pu[0] * fth - pu[2] * ft)) * st;
+    const S kk1 = A * s.r - Bw * act, kk2 = -(A * act + Bw * s.r);
+    const S ea = beta * kk2 - mu * kk1, eb = beta * kk1 + mu * kk2;
+    o.chi = katan2(-ea, eb);
+    const S den = ea * ea + eb * eb;
+    if (!(den > 0.0)) return o;
+    const S eab = -eb / den, ebb = ea / den;
+    const S kk1b = beta * ebb - mu * eab, kk2b = beta * eab + mu * ebb;
+    const S Ab = s.r * kk1b - act * kk2b, Bws = -(act * kk1b + s.r * kk2b) * st;
+    const S fthb = Bws * (C2 * pu[3] - a * pu[0]);
+    const S fphb = Ab * (C1 * pu[1]) - Bws * (C2 * pu[2]);
+    const S ftb = Bws * (a * pu[2]) - Ab * pu[1] + fphb * zm.omega;
+    const S zb[3] = {(Ab * (pu[0] - C1 * pu[3])) / zm.sgrr, fphb / zm.sgpp, -fthb / zm.sgthth};
+    const S ufqb = zb[0] * u[0].v + zb[1] * u[1].v + zb[2] * u[2].v;
+    const S ufb = ftb / zm.lapse + ufqb * q.v;
+    const S qb = ufqb * uf;
+    S fb[3], ub[3];
+    for (int i = 0; i < 3; ++i) { fb[i] = (zb[i] + ufb * u[i].v) * ifn; ub[i] = zb[i] * ufq + ufb * h[i]; }
+    // f = k x bh: kbar = bh x fbar, bhbar = fbar x k
+    const S kb[3] = {bh[1] * fb[2] - bh[2] * fb[1], bh[2] * fb[0] - bh[0] * fb[2], bh[0] * fb[1] - bh[1] * fb[0]};
+    const S bb[3] = {fb[1] * k[2].v - fb[2] * k[1].v, fb[2] * k[0].v - fb[0] * k[2].v, fb[0] * k[1].v - fb[1] * k[0].v};
+    o.dchi[0] = bb[0] * bt[0] + bb[1] * bt[1] + bb[2] * bt[2];
+    o.dchi[1] = bb[0] * bp[0] + bb[1] * bp[1] + bb[2] * bp[2];
+    for (int i = 0; i < 3; ++i) o.dchi[2 + i] = ub[i] + qb * q.d[i] + kb[0] * k[0].d[i] + kb[1] * k[1].d[i] + kb[2] * k[2].d[i];
+    return o;
+}
+
 // one thermal splat's share of sample s (dual_gradient.jl:46-76); grad is the splat's 21-row column.
 // ANALYTIC selects the device's form: the hand-written d ln G instead of the 13-partial duals, and the
 // seven fluid partials in two passes joined by the chain rule instead of one 7-partial pass (the sweep
@@ -225,17 +301,12 @@ inline void element_adjoint(const Model& m, int i, const RayCtx& rc, const Sampl
         // the device's form: the frame (g, cos B, chi) with five partials (theta_B, phi_B, u1..u3), the
         // contribution with four (ln Theta_e, ln B, g, cos B) and its chi derivative written out, joined by
         // the chain rule
-        using D5 = Dual<5, S>;
-        D5 st = ksin(D5::seed(S(colv[15]), 0)), ct = kcos(D5::seed(S(colv[15]), 0));
-        D5 sph = ksin(D5::seed(S(colv[16]), 1)), cph = kcos(D5::seed(S(colv[16]), 1));
-        D5 Bh[3] = {st * cph, st * sph, ct};
-        D5 u[3] = {D5::seed(S(colv[17]), 2), D5::seed(S(colv[18]), 3), D5::seed(S(colv[19]), 4)};
-        Frame<D5> fr = local_frame<D5>(D5(rc.a), D5(s.r), D5(s.th), D5(rc.eta), D5(rc.lam), s.nur, s.nuth, D5(rc.alpha), D5(rc.beta),
-                                       D5(rc.thetao), u, Bh);
+        const S uv[3] = {S(colv[17]), S(colv[18]), S(colv[19])};
+        const FrameJ<S> fr = frame_jacobian(rc, s, S(colv[15]), S(colv[16]), uv);
         using D4 = Dual<4, S>;
         D4 Th = kexp(D4::seed(S(colv[13]), 0));
         D4 Bmag = kexp(D4::seed(S(colv[14]), 1));
-        D4 g = D4::seed(fr.g.v, 2), cosB = D4::seed(fr.cosB.v, 3);
+        D4 g = D4::seed(fr.g, 2), cosB = D4::seed(fr.cosB, 3);
         D4 nuf = D4(nu) / g;
         D4 thB = kacos(kmin(kmax(cosB, D4(-1.0)), D4(1.0)));
         D4 xb = D4(1.0) / Th;
@@ -248,7 +319,7 @@ inline void element_adjoint(const Model& m, int i, const RayCtx& rc, const Sampl
         D4 A = D4(jbar[0]) * ci.jI + D4(jbar[3]) * ci.jV + D4(abar[0]) * ci.aI + D4(abar[3]) * ci.aV + D4(rbar[2]) * ci.rV;
         D4 Bc = D4(jbar[1]) * ci.jQ + D4(abar[1]) * ci.aQ + D4(rbar[0]) * ci.rQ;
         D4 Bs = D4(jbar[2]) * ci.jQ + D4(abar[2]) * ci.aQ + D4(rbar[1]) * ci.rQ;
-        S s2 = ksin(2.0 * fr.chi.v), c2 = kcos(2.0 * fr.chi.v);
+        S s2 = ksin(2.0 * fr.chi), c2 = kcos(2.0 * fr.chi);
         D4 q = A + Bc * D4(c2) + Bs * D4(s2);
         S qchi = 2.0 * (Bs.v * c2 - Bc.v * s2);
         S sv = q.v;
@@ -256,7 +327,7 @@ inline void element_adjoint(const Model& m, int i, const RayCtx& rc, const Sampl
         grad[12] = grad[12] + sv;
         grad[13] = grad[13] + q.d[0];
         grad[14] = grad[14] + q.d[1];
-        for (int k = 0; k < 5; ++k) grad[15 + k] = grad[15 + k] + q.d[2] * fr.g.d[k] + q.d[3] * fr.cosB.d[k] + qchi * fr.chi.d[k];
+        for (int k = 0; k < 5; ++k) grad[15 + k] = grad[15 + k] + q.d[2] * fr.dg[k] + q.d[3] * fr.dcosB[k] + qchi * fr.dchi[k];
         grad[nr - 1] = grad[nr - 1] + sv * dlnG[12];
     } else {
         // fluid rows with 7 partials (rows 13..19)
