@@ -246,9 +246,8 @@ KS_DEV void element_adjoint_thermal(const SplatCT<S>& c, const SplatAdjT<S>& sa,
     const D4 g = D4::seed(fr.g, 2), cosB = D4::seed(fr.cosB, 3);
     D4 nuf = nu / g;
     D4 cb = dmin(dmax(cosB, -1.0), 1.0);
-    D4 thB = acos(cb);
     D4 sth, cth;
-    sincos(thB, &sth, &cth);
+    pitch_sincos(cb, sth, cth);
     const S ne = exp(c.lnA) * Gw;
     CoefT<D4> cf = thermal(ne, Th, k0k2, k1k2, Bm, nuf, sth, cth);
     if (!(cf.jI > 0.0 || cf.aI > 0.0 || cf.rQ.v != 0.0 || cf.rV.v != 0.0)) return;

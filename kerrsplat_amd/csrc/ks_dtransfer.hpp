@@ -193,6 +193,13 @@ KS_DEV FrameT<T> local_frame(const GeoT<S>& G, S a, S alpha, S beta, S thetao_si
     return fr;
 }
 
+// sin and cos of the pitch angle acos(cb), cb in [-1, 1] (the reference's safe_acos, then its sin and cos):
+// cos = cb and sin = sqrt((1 - cb)(1 + cb)), whose product form stays exact to an ulp near |cb| = 1
+template <class T> KS_DEV void pitch_sincos(const T& cb, T& sth, T& cth) {
+    cth = cb;
+    sth = sqrt((1.0 - cb) * (1.0 + cb));
+}
+
 // ---- coefficients ------------------------------------------------------------------------------------------------
 template <class T> struct CoefT { T jI, jQ, jV, aI, aQ, aV, rQ, rV; };
 using Coef = CoefT<double>;

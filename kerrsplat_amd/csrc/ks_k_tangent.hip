@@ -46,7 +46,7 @@ KS_DEV bool tangent_coefficients(const SplatC* __restrict__ sp, int ns, const D3
         const D3 nuf = nu / fr.g;
         const D3 cb = dmin(dmax(fr.cosB, -1.0), 1.0);
         D3 sth, cth;
-        sincos(acos(cb), &sth, &cth);
+        pitch_sincos(cb, sth, cth);
         const D3 ne = exp(sc.lnA) * G_;
         CoefT<D3> cf;
         const int kind = KIND == MODEL_COMPOSITE ? sc.kind : KIND;
