@@ -129,6 +129,19 @@ int ks_render_device(ks_cache* cache, const ks_model* model, const double* times
 int ks_copy_stokes(ks_cache* cache, int64_t frames, double* out_stokes, int screen_order);
 const double* ks_stokes_device_ptr(const ks_cache* cache);
 
+/* ray_lists(cache, params, times; nmax, slab) -> RayLists - dual_gradient.jl:283-359: for every ray and
+ * frame the splats whose support sphere (outside_support) one of the ray's step-taking samples (ok and
+ * outside the horizon guard) enters at the frame's time, samples past the half-orbit cutoff excluded when
+ * nmax >= 0, in ascending splat order. Needs stored samples. count[p + npix f] (int32) receives the number
+ * of splats of ray p (screen order when screen_order, else the cache's sorted order), frame f. With ids NULL
+ * *capacity receives the largest count (at least 1); with ids, ids[s + capacity (p + npix f)] receives the
+ * s-th splat (0-based; -1 past the count) for the caller's *capacity, and a capacity below the largest count
+ * fails with KS_ERR_DIM and *capacity set to it. Thermal, power-law and kappa models. The transport over the
+ * lists is the same sum as over all splats (a splat outside its support adds exactly zero): the image and
+ * gradient entry points cull per tile of samples on the device and take no lists. */
+int ks_ray_lists(ks_cache* cache, const ks_model* model, const double* times, int nt, int nmax, double slab, int screen_order,
+                 int32_t* count, int* capacity, int32_t* ids);
+
 /* polarized_gradient!(dparams, dstokes, cache, params, t_obs, ν_obs, L; method = :dual, nmax, slab)
  * - polarized.jl:473-480 with polarized_tails! and polarized_dual_sweep! (dual_gradient.jl:180-249):
  * the gradient of l = Σ dstokes · I_obs over every (pixel, time, frequency) with respect to the

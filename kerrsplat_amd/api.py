@@ -74,6 +74,7 @@ def _load():
     lib.ks_render_device.argtypes = [_P, _P, _P, _i, _P, _i, _d, _i, _d]
     lib.ks_copy_stokes.argtypes = [_P, _i64, _P, _i]
     lib.ks_thin_image.argtypes = [_P, _P, _d, _P, _i]
+    lib.ks_ray_lists.argtypes = [_P, _P, _P, _i, _i, _d, _i, _P, ctypes.POINTER(_i), _P]
     lib.ks_visibilities.argtypes = [_P, _i64, _d, _P, _P, _P, _i, _P]
     lib.ks_timing_end.argtypes = [_P, ctypes.POINTER(ctypes.c_float)]
     lib.ks_stage_ms.argtypes = [_P, _i, ctypes.POINTER(ctypes.c_float)]
@@ -458,6 +459,79 @@ def thin_image(cache, params, t_obs, screen=True):
     out = np.zeros(cache.npix)
     _check(cache._lib.ks_thin_image(cache._h, m._h, _d(t_obs), _ptr(out), _i(int(screen))))
     return out.reshape(cache.screen_size, order="F") if screen else out
+
+
+class RayLists:
+    """RayLists(ids, count) (polarized.jl:53-75): for every ray the splats whose support its samples enter.
+    ids is (capacity, npix) or (capacity, npix, nframes) int32, 0-based splat indices in ascending order
+    with -1 past each ray's count; count is (npix,) or (npix, nframes); rays in the camera's pixel order."""
+
+    def __init__(self, ids, count):
+        self.ids, self.count = np.asarray(ids, dtype=np.int32), np.asarray(count, dtype=np.int32)
+        if self.ids.shape[1:] != self.count.shape:
+            raise DimensionMismatch(f"ids {self.ids.shape} and count {self.count.shape} disagree")
+
+    @property
+    def capacity(self):
+        return self.ids.shape[0]
+
+    @property
+    def nframes(self):
+        return self.ids.shape[2] if self.ids.ndim == 3 else 1
+
+    def ray(self, j, f=0):
+        """The splats of ray j in frame f."""
+        ids = self.ids if self.ids.ndim == 3 else self.ids[:, :, None]
+        cnt = self.count if self.count.ndim == 2 else self.count[:, None]
+        return ids[: cnt[j, f], j, f]
+
+
+def capacity(lists):
+    return lists.capacity
+
+
+def nframes(lists):
+    return lists.nframes
+
+
+def framed(lists):
+    """The lists as (capacity, npix, nframes) and (npix, nframes) (polarized.jl:68-70)."""
+    if lists is None or lists.ids.ndim == 3:
+        return lists
+    return RayLists(lists.ids[:, :, None], lists.count[:, None])
+
+
+def flattened(lists):
+    """The lists flattened over frames: (capacity, npix nframes) and a vector, frame-major (polarized.jl:71-72)."""
+    return RayLists(lists.ids.reshape(lists.capacity, -1, order="F"), lists.count.reshape(-1, order="F"))
+
+
+def frame_lists(lists, f):
+    """The lists of frame f of framed lists (polarized.jl:73-75)."""
+    if lists is None:
+        return None
+    return RayLists(lists.ids[:, :, f], lists.count[:, f])
+
+
+def ray_lists(cache, params, times, nmax=-1, slab=0.0, kind="thermal"):
+    """ray_lists(cache, params, t_obs | times; nmax, slab) -> RayLists (dual_gradient.jl:326-359): for every
+    ray the splats whose support sphere one of its step-taking samples enters at the observation time,
+    samples past the half-orbit cutoff excluded, in ascending order; built on the device (a hit byte per
+    splat and ray, the counts, then the lists filled into capacity = the largest count slots). A scalar
+    time gives (capacity, npix) ids and (npix,) counts, a vector of times (capacity, npix, nframes) and
+    (npix, nframes). The transport over the lists is the same sum as over all splats; this build's image
+    and gradient launches cull per tile of samples on the device and take no lists."""
+    m = _as_model(params, kind)
+    scalar = np.ndim(times) == 0
+    t = _f64(np.atleast_1d(times))
+    npix, nt = cache.npix, len(t)
+    count = np.zeros((nt, npix), dtype=np.int32)
+    cap = _i(0)
+    _check(cache._lib.ks_ray_lists(cache._h, m._h, _ptr(t), _i(nt), _i(nmax), _d(slab), _i(1), _ptr(count), ctypes.byref(cap), None))
+    ids = np.zeros((nt, npix, cap.value), dtype=np.int32)
+    _check(cache._lib.ks_ray_lists(cache._h, m._h, _ptr(t), _i(nt), _i(nmax), _d(slab), _i(1), _ptr(count), ctypes.byref(cap), _ptr(ids)))
+    ids, count = np.transpose(ids, (2, 1, 0)), count.T
+    return RayLists(ids[:, :, 0], count[:, 0]) if scalar else RayLists(ids, count)
 
 
 def render_device(cache, model, times, nus, L, nmax=-1, slab=0.0):

@@ -463,4 +463,57 @@ hipError_t launch_seg_combine(hipStream_t s, const double* soa, int64_t npix, in
     return hipGetLastError();
 }
 
+// ---- per-ray splat lists (ray_lists, dual_gradient.jl:283-359) ----------------------------------------------
+// One thread per (ray j in sorted order, frame f): hits[i + ns (j + npix f)] = 1 when a sample of ray j before its
+// half-orbit cutoff kstop[j] (none when kstop is null) that takes a step (ok, outside the horizon guard) lies inside
+// splat i's support sphere at the frame's time (outside_support), else 0; cnt[j + npix f] = the number of hits.
+// Splats outer: a splat's test stops at the ray's first sample inside its sphere, and each byte is written once.
+__global__ void __launch_bounds__(64) ks_ray_hits(double rp, int64_t npix, int N, Samples smp, const SplatC* __restrict__ sp, int ns,
+                                                  const double* __restrict__ times, int nt, const int* __restrict__ kstop,
+                                                  uint8_t* __restrict__ hits, int* __restrict__ cnt) {
+    const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= npix * nt) return;
+    const int64_t j = item % npix;
+    const double t_obs = times[item / npix];
+    const int kend = kstop ? (kstop[j] - 1 < N ? kstop[j] - 1 : N) : N;
+    const double hor = rp * (1.0 + 1e-3);
+    int n = 0;
+    for (int i = 0; i < ns; ++i) {
+        const SplatC& c = sp[i];
+        uint8_t hit = 0;
+        for (int k = 1; k <= kend; ++k) {
+            const int64_t o = (int64_t)(k - 1) * npix + j;
+            if (!(smp.fl[o] & 1) || !(smp.r[o] > hor)) continue;
+            double sph, cph;
+            if (!outside_support(c, t_obs - smp.t[o], smp.x[o], smp.y[o], smp.z[o], sph, cph)) { hit = 1; break; }
+        }
+        hits[(int64_t)i + (int64_t)ns * item] = hit;
+        n += hit;
+    }
+    cnt[item] = n;
+}
+// ids[s + cap t] = the (s+1)-th hit splat of output item t in ascending order (0-based), -1 past its count. Output
+// item t = p + npix f reads item iperm[p] + npix f of hits (the screen pixel p's sorted slot), or t itself when
+// iperm is null.
+__global__ void ks_ray_fill(int64_t items, int64_t npix, int ns, int cap, const uint8_t* __restrict__ hits, const int* __restrict__ iperm,
+                            int32_t* __restrict__ ids) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= items) return;
+    const int64_t it = iperm ? iperm[t % npix] + npix * (t / npix) : t;
+    int n = 0;
+    for (int i = 0; i < ns; ++i)
+        if (hits[(int64_t)i + (int64_t)ns * it] && n < cap) ids[(int64_t)cap * t + n++] = i;
+    for (; n < cap; ++n) ids[(int64_t)cap * t + n] = -1;
+}
+
+hipError_t launch_ray_hits(hipStream_t s, double rp, int64_t npix, int N, Samples smp, const SplatC* sp, int ns, const double* times, int nt,
+                           const int* kstop, uint8_t* hits, int* cnt) {
+    hipLaunchKernelGGL(ks_ray_hits, dim3(grid_for(npix * nt, 64)), dim3(64), 0, s, rp, npix, N, smp, sp, ns, times, nt, kstop, hits, cnt);
+    return hipGetLastError();
+}
+hipError_t launch_ray_fill(hipStream_t s, int64_t items, int64_t npix, int ns, int cap, const uint8_t* hits, const int* iperm, int32_t* ids) {
+    hipLaunchKernelGGL(ks_ray_fill, dim3(grid_for(items, 128)), dim3(128), 0, s, items, npix, ns, cap, hits, iperm, ids);
+    return hipGetLastError();
+}
+
 }  // namespace ksd

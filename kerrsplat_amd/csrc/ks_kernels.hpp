@@ -200,6 +200,9 @@ hipError_t launch_seg_polarized(hipStream_t s, int kind, double a, double thetao
                                 const SplatC* sp, int ns, const double* knots, const double* ktimes, int nknots,
                                 const double* times, int nt, const double* nus, int nnu, double L, const int* kstop,
                                 double* seg, double* res);
+hipError_t launch_ray_hits(hipStream_t s, double rp, int64_t npix, int N, Samples smp, const SplatC* sp, int ns, const double* times, int nt,
+                           const int* kstop, uint8_t* hits, int* cnt);
+hipError_t launch_ray_fill(hipStream_t s, int64_t items, int64_t npix, int ns, int cap, const uint8_t* hits, const int* iperm, int32_t* ids);
 hipError_t launch_thin_parts(hipStream_t s, double a, double thetao, const double* soa, int64_t npix, int N, int M, Samples smp,
                              const SplatC* sp, int ns, double t_obs, int nseg_max, double* part, double* res);
 hipError_t launch_thin_combine(hipStream_t s, int64_t npix, int nseg_max, const double* part, double* out);

@@ -832,6 +832,62 @@ int ks_thin_image(ks_cache* c, const ks_model* m, double t_obs, double* out, int
     return KS_OK;
 }
 
+// ---- per-ray splat lists (ray_lists, dual_gradient.jl:283-359) ----
+int ks_ray_lists(ks_cache* c, const ks_model* m, const double* times, int nt, int nmax, double slab, int screen_order, int32_t* count,
+                 int* capacity, int32_t* ids) {
+    if (!c || !m || !times || !count || !capacity) return fail(KS_ERR_ARG, "ks_ray_lists: null argument");
+    if (nt < 1) return fail(KS_ERR_ARG, "ks_ray_lists: no frames");
+    if (!has_samples(c))
+        return fail(KS_ERR_STATE, "the cache holds no stored samples: build it with store_samples and a storing marcher");
+    if (m->kind != KS_MODEL_THERMAL && m->kind != KS_MODEL_POWERLAW && m->kind != KS_MODEL_KAPPA)
+        return fail(KS_ERR_ARG, "ks_ray_lists takes thermal, power-law or kappa splats");
+    ks_ctx* x = c->ctx;
+    HIPCHK(hipSetDevice(x->device));
+    const int64_t npix = c->npix, items = npix * nt;
+    struct Scratch {   // freed on every return
+        Buf<uint8_t> hits; Buf<int> cnt; Buf<double> dt; Buf<int32_t> ids;
+        ~Scratch() { hits.release(); cnt.release(); dt.release(); ids.release(); }
+    } b;
+    Buf<uint8_t>& hits = b.hits;
+    Buf<int>& cnt = b.cnt;
+    Buf<double>& dt = b.dt;
+    Buf<int32_t>& dids = b.ids;
+    if (hits.grow((size_t)std::max(m->ns, 1) * items) || cnt.grow(items) || dt.grow(nt)) return KS_ERR_HIP;
+    HIPCHK(hipMemcpyAsync(dt.p, times, nt * sizeof(double), hipMemcpyHostToDevice, x->stream));
+    const int* kst = nullptr;
+    if (nmax >= 0) {
+        if (!c->kstop) { if (dalloc(&c->kstop, (size_t)npix)) return KS_ERR_HIP; }
+        HIPCHK(launch_wind_cutoff(x->stream, c->spin, c->thetao, c->soa, npix, c->N, c->M, nmax, slab, samples_of(c), c->kstop));
+        kst = c->kstop;
+    }
+    const double rp = 1.0 + sqrt(1.0 - c->spin * c->spin);
+    HIPCHK(launch_ray_hits(x->stream, rp, npix, c->N, samples_of(c), m->d_splats, m->ns, dt.p, nt, kst, hits.p, cnt.p));
+    std::vector<int> cs(items);
+    HIPCHK(hipMemcpyAsync(cs.data(), cnt.p, items * sizeof(int), hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    int cap = 1;
+    for (int64_t t = 0; t < items; ++t) {
+        cap = std::max(cap, cs[t]);
+        const int64_t j = t % npix, f = t / npix;
+        count[(screen_order ? c->perm_host[j] : j) + npix * f] = cs[t];
+    }
+    if (ids) {
+        if (*capacity < cap) {
+            const int have = *capacity;
+            *capacity = cap;
+            return fail(KS_ERR_DIM, "ks_ray_lists: capacity %d below the largest count %d", have, cap);
+        }
+        const int cap_out = *capacity;
+        if (dids.grow((size_t)cap_out * items)) return KS_ERR_HIP;
+        HIPCHK(launch_ray_fill(x->stream, items, npix, m->ns, cap_out, hits.p, screen_order ? c->iperm : nullptr, dids.p));
+        HIPCHK(hipMemcpyAsync(ids, dids.p, (size_t)cap_out * items * sizeof(int32_t), hipMemcpyDeviceToHost, x->stream));
+        HIPCHK(hipStreamSynchronize(x->stream));
+    } else {
+        *capacity = cap;
+    }
+    return KS_OK;
+}
+
 // ---- visibilities and χ² (visibilities.jl:21-40, device_visibilities.jl:53-111, Fit.jl:537-577) ----
 // host (u, v, taper) to the device and the phase tables of the cache's screen
 static int vis_setup(ks_cache* c, double psize, const double* u, const double* v, const double* taper, int nuv) {

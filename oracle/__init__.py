@@ -87,6 +87,19 @@ def samples(a, thetao, alphas, betas, N, M=64, marcher=1):
     return {"t": t, "r": r, "theta": th, "phi": ph, "flags": flags, "resid_t": rt, "resid_phi": rp, "tau_total": tt}
 
 
+def ray_hits(a, thetao, alphas, betas, N, params, times, nmax=-1, slab=0.0, M=64, marcher=1):
+    """ray_lists' hit table (dual_gradient.jl:283-324): hits[i, j, f] = 1 when a step-taking sample of ray j
+    (screen order; before the half-orbit cutoff when nmax >= 0) enters splat i's support sphere at times[f]."""
+    al, be = _f64(alphas), _f64(betas)
+    P = np.asfortranarray(params, dtype=np.float64)
+    tv = _f64(np.atleast_1d(times))
+    ns, n, nt = P.shape[1], len(al), len(tv)
+    hits = np.zeros(ns * n * nt, dtype=np.uint8)
+    lib().kso_ray_hits(_d(a), _d(thetao), _i(n), _p(al), _p(be), _i(N), _i(M), _i(marcher), _i(P.shape[0]), _i(ns), _p(P), _i(nt), _p(tv),
+                       _i(nmax), _d(slab), _p(hits))
+    return hits.reshape(nt, n, ns).transpose(2, 1, 0)
+
+
 def thin_image(a, thetao, alphas, betas, N, params, t_obs, M=64, marcher=1):
     al, be = _f64(alphas), _f64(betas)
     P = np.asfortranarray(params, dtype=np.float64)

@@ -130,6 +130,39 @@ int kso_samples(double a, double thetao, int npix, const double* alphas, const d
     return 0;
 }
 
+// ray_lists (dual_gradient.jl:283-359): hits[i + ns (j + npix f)] = 1 when a sample of ray j that takes a step
+// (ok, outside the horizon guard; before the nmax-th half-orbit passage through |z| < slab when nmax >= 0) lies
+// inside splat i's support sphere at times[f] (outside_support, polarized.jl:110-120), else 0; screen order.
+// Any model whose rows 0-5, 10 and last row are the geometry (thermal, power-law, kappa).
+int kso_ray_hits(double a, double thetao, int npix, const double* alphas, const double* betas, int N, int M, int marcher, int nrows, int ns,
+                 const double* params, int nt, const double* times, int nmax, double slab, uint8_t* hits) {
+    Model m;
+    m.nrows = nrows; m.ns = ns; m.params = params;
+    const double hor = (1.0 + std::sqrt(1.0 - a * a)) * (1.0 + 1e-3);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int j = 0; j < npix; ++j) {
+        Pixel<double> p = pixel_constants<double>(a, alphas[j], betas[j], thetao);
+        Winding<double> w = winding_zero<double>();
+        auto f = [&](int, const Sample<double>& s, double) {
+            if (!(s.ok && s.r > hor)) return;
+            if (nmax >= 0) {
+                wind(w, s.r * kcos(s.th), slab);
+                if (w.n > nmax) return;
+            }
+            double x, y, z;
+            quasi_cartesian_ks(a, s.r, s.th, s.ph, x, y, z);
+            for (int fr = 0; fr < nt; ++fr)
+                for (int i = 0; i < ns; ++i) {
+                    uint8_t& h = hits[(size_t)i + (size_t)ns * ((size_t)j + (size_t)npix * fr)];
+                    if (!h && !outside_support(m, i, times[fr] - s.t, x, y, z)) h = 1;
+                }
+        };
+        double rt, rp;
+        march(p, N, M, marcher, f, rt, rp);
+    }
+    return 0;
+}
+
 // optically thin Stokes I image (screen order)
 int kso_thin_image(double a, double thetao, int npix, const double* alphas, const double* betas, int N, int M, int marcher,
                    int ns, const double* params, double t_obs, double* out) {
