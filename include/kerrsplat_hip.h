@@ -106,9 +106,9 @@ int ks_copy_samples(const ks_cache* cache, double* t, double* r, double* theta, 
 int ks_model_create(ks_ctx* ctx, int kind, int nrows, int ns, const double* params, double gamma_max,
                     const double* knots, const double* knot_times, int nknots, ks_model** model);
 /* CompositeModel(a, b) - transport.jl:298-316: the elements of parts[0], then parts[1], ...; the
- * coefficients of overlapping elements add. Parts are thermal, power-law, kappa or composite models
- * of the same context (thin and knot splats are not composed); the splat constants are copied, so
- * the parts may be destroyed afterwards. Renders through ks_render like any population; the dual
+ * coefficients of overlapping elements add. Parts are thermal, power-law, kappa, knot or composite
+ * models of the same context (thin splats are not composed; knot parts keep their own knots and knot
+ * times); the splat constants and knots are copied, so the parts may be destroyed afterwards. Renders through ks_render like any population; the dual
  * sweep takes thermal splats only. */
 int ks_model_composite(ks_ctx* ctx, int nparts, const ks_model* const* parts, ks_model** model);
 int ks_model_destroy(ks_model* model);

@@ -431,15 +431,16 @@ class Model:
 
 class CompositeModel(Model):
     """CompositeModel(a, b, ...) (transport.jl:298-316): the elements of a, then of b, ...; the
-    coefficients of overlapping elements add. Parts are Model objects of kind thermal, powerlaw or
-    kappa, or other composites; the splat constants are copied to the composite on the device."""
+    coefficients of overlapping elements add. Parts are Model objects of kind thermal, powerlaw, kappa
+    or knot (each knot part with its own knots and knot times), or other composites; the splat constants
+    and knots are copied to the composite on the device."""
 
     def __init__(self, *parts, device=0):
         if not parts:
             raise ValueError("CompositeModel needs at least one part")
         for p in parts:
-            if not isinstance(p, Model) or p.kind not in ("thermal", "powerlaw", "kappa", "composite"):
-                raise ValueError("CompositeModel parts are thermal, powerlaw, kappa or composite Model objects")
+            if not isinstance(p, Model) or p.kind not in ("thermal", "powerlaw", "kappa", "knot", "composite"):
+                raise ValueError("CompositeModel parts are thermal, powerlaw, kappa, knot or composite Model objects")
         self.kind, self.params, self.parts = "composite", None, parts
         self._lib = _load()
         arr = (_P * len(parts))(*[p._h.value for p in parts])

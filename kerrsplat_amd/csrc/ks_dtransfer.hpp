@@ -42,7 +42,7 @@ template <class S> struct SplatCT {
                           // kappa: kappa, w, hyp
     S g[6];               // power law: Gamma((3p-1)/12) Gamma((3p+19)/12), Gamma((3p+2)/12) Gamma((3p+22)/12);
                           // kappa: Gamma(k-4/3), Gamma(k-2), Gamma(k/4-1/3), Gamma(k/4+4/3), Gamma(2+k/2), K0/(K2+1e-40) at 1/w
-    int knot0;            // knot splats: index of the splat's first knot
+    int knot0, nk, kt0;   // knot splats: the first of its nk knots (xyz triples) and of their times in the model's arrays
     int kind;             // the splat's population (MODEL_THERMAL, MODEL_POWERLAW, MODEL_KAPPA, ...)
 };
 using SplatC = SplatCT<double>;

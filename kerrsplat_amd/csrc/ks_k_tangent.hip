@@ -19,7 +19,7 @@ KS_DEV SplatCT<D3> splat_dual(const SplatC& s) {
     o.t0 = D3(s.t0); o.iw = D3(s.iw); o.omega = D3(s.omega); o.rsup2 = D3(s.rsup2); o.lnA = D3(s.lnA); o.Bmag = D3(s.Bmag);
     o.p1 = D3(s.p1); o.p2 = D3(s.p2); o.p3 = D3(s.p3);
     for (int i = 0; i < 6; ++i) o.g[i] = D3(s.g[i]);
-    o.knot0 = s.knot0; o.kind = s.kind;
+    o.knot0 = s.knot0; o.nk = s.nk; o.kt0 = s.kt0; o.kind = s.kind;
     return o;
 }
 

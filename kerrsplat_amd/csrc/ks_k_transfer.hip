@@ -28,6 +28,33 @@ KS_DEV TransCtxT<S> make_ctx(double a, double rp, double rm, double thetao_sin, 
     return c;
 }
 
+// A knot splat's Gaussian exponent at (t, x, y, z): its centre interpolated linearly between its knots
+// (trajectories.jl:197-233), held at the first and last knot outside their times; no pattern rotation.
+template <class S>
+KS_DEV S knot_gexp(const TransCtxT<S>& c, const SplatCT<S>& sc, S t, S x, S y, S z) {
+    const S* K = c.knots + 3 * (int64_t)sc.knot0;
+    const S* kt = c.ktimes + sc.kt0;
+    const int nk = sc.nk;
+    S cx, cy, cz;
+    if (t <= kt[0]) { cx = K[0]; cy = K[1]; cz = K[2]; }
+    else if (t >= kt[nk - 1]) { cx = K[3 * (nk - 1)]; cy = K[3 * (nk - 1) + 1]; cz = K[3 * (nk - 1) + 2]; }
+    else {
+        int kk = 0;
+        while (kk < nk - 2 && kt[kk + 1] <= t) ++kk;
+        S w = (t - kt[kk]) / (kt[kk + 1] - kt[kk]);
+        cx = (1.0 - w) * K[3 * kk] + w * K[3 * (kk + 1)];
+        cy = (1.0 - w) * K[3 * kk + 1] + w * K[3 * (kk + 1) + 1];
+        cz = (1.0 - w) * K[3 * kk + 2] + w * K[3 * (kk + 1) + 2];
+    }
+    S d0 = x - cx, d1 = y - cy, d2 = z - cz;
+    S u0 = sc.R[0] * d0 + sc.R[3] * d1 + sc.R[6] * d2;
+    S u1 = sc.R[1] * d0 + sc.R[4] * d1 + sc.R[7] * d2;
+    S u2 = sc.R[2] * d0 + sc.R[5] * d1 + sc.R[8] * d2;
+    S q0 = u0 * sc.is[0], q1 = u1 * sc.is[1], q2 = u2 * sc.is[2];
+    S tt = (t - sc.t0) * sc.iw;
+    return q0 * q0 + q1 * q1 + q2 * q2 + tt * tt;
+}
+
 // The splat sum of one sample (accumulate_elements, transport.jl:254-276): invariant j, alpha, rho in
 // screen axes; false when no splat contributes. One instantiation per population and scalar.
 // With a candidate list (ascending splat indices, the per-tile cull) only those splats are visited; the
@@ -45,28 +72,8 @@ KS_DEV bool sample_coefficients(const TransCtxT<S>& c, double st, double r, doub
         const int i = cand ? (int)cand[ci] : ci;
         const SplatCT<S>& sc = c.sp[i];
         S gexp;
-        if constexpr (KIND == MODEL_KNOT) {
-            // centre from the knots (trajectories.jl:197-233): no pattern rotation
-            const S* K = c.knots + 3 * (int64_t)c.nknots * i;
-            const int nk = c.nknots;
-            S cx, cy, cz;
-            if (t <= c.ktimes[0]) { cx = K[0]; cy = K[1]; cz = K[2]; }
-            else if (t >= c.ktimes[nk - 1]) { cx = K[3 * (nk - 1)]; cy = K[3 * (nk - 1) + 1]; cz = K[3 * (nk - 1) + 2]; }
-            else {
-                int kk = 0;
-                while (kk < nk - 2 && c.ktimes[kk + 1] <= t) ++kk;
-                S w = (t - c.ktimes[kk]) / (c.ktimes[kk + 1] - c.ktimes[kk]);
-                cx = (1.0 - w) * K[3 * kk] + w * K[3 * (kk + 1)];
-                cy = (1.0 - w) * K[3 * kk + 1] + w * K[3 * (kk + 1) + 1];
-                cz = (1.0 - w) * K[3 * kk + 2] + w * K[3 * (kk + 1) + 2];
-            }
-            S d0 = x - cx, d1 = y - cy, d2 = z - cz;
-            S u0 = sc.R[0] * d0 + sc.R[3] * d1 + sc.R[6] * d2;
-            S u1 = sc.R[1] * d0 + sc.R[4] * d1 + sc.R[7] * d2;
-            S u2 = sc.R[2] * d0 + sc.R[5] * d1 + sc.R[8] * d2;
-            S q0 = u0 * sc.is[0], q1 = u1 * sc.is[1], q2 = u2 * sc.is[2];
-            S tt = (t - sc.t0) * sc.iw;
-            gexp = q0 * q0 + q1 * q1 + q2 * q2 + tt * tt;
+        if (KIND == MODEL_KNOT || (KIND == MODEL_COMPOSITE && sc.kind == MODEL_KNOT)) {
+            gexp = knot_gexp(c, sc, t, x, y, z);
         } else {
             S sph, cph;
             if (outside_support(sc, t, x, y, z, sph, cph)) continue;
@@ -94,7 +101,7 @@ KS_DEV bool sample_coefficients(const TransCtxT<S>& c, double st, double r, doub
         if constexpr (KIND == MODEL_THERMAL || KIND == MODEL_KNOT) cf = thermal(ne, sc.p1, sc.p2, sc.p3, sc.Bmag, nuf, sth, cth);
         else if constexpr (KIND == MODEL_POWERLAW) cf = powerlaw(ne, sc.p1, sc.p2, sc.p3, sc.g, sc.Bmag, nuf, sth, cth);
         else if constexpr (KIND == MODEL_KAPPA) cf = kappa(ne, sc.p1, sc.p2, sc.p3, sc.g, sc.Bmag, nuf, sth, cth);
-        else if (sc.kind == MODEL_THERMAL) cf = thermal(ne, sc.p1, sc.p2, sc.p3, sc.Bmag, nuf, sth, cth);
+        else if (sc.kind == MODEL_THERMAL || sc.kind == MODEL_KNOT) cf = thermal(ne, sc.p1, sc.p2, sc.p3, sc.Bmag, nuf, sth, cth);
         else if (sc.kind == MODEL_POWERLAW) cf = powerlaw(ne, sc.p1, sc.p2, sc.p3, sc.g, sc.Bmag, nuf, sth, cth);
         else cf = kappa(ne, sc.p1, sc.p2, sc.p3, sc.g, sc.Bmag, nuf, sth, cth);
         if (!(cf.jI > 0.0 || cf.aI > 0.0 || cf.rQ != 0.0 || cf.rV != 0.0)) continue;
@@ -391,7 +398,8 @@ static void chunk_transfer_k(hipStream_t s, double a, double thetao, const doubl
                              const double* nus, int nnu, double L, const int* kstop, double* seg, S* rec, uint8_t* aflg) {
     const int nch = (N + CHUNK - 1) / CHUNK;
     const int64_t items = npix * nt * nnu;
-    if (ns >= CULL_MIN_NS && ns <= CAND_MAX)
+    // the per-tile cull bounds pattern-rotating splats: not for knot splats (alone, or in a composite: nknots > 0)
+    if (ns >= CULL_MIN_NS && ns <= CAND_MAX && !(KIND == MODEL_COMPOSITE && nknots > 0))
         hipLaunchKernelGGL((ks_chunk_transfer<KIND, S, true>), dim3(grid_for(items * nch, 128)), dim3(128), 0, s, a, thetao, soa, npix, N, smp, sp,
                            ns, knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, nch, seg, rec, aflg);
     else

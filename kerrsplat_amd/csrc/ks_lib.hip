@@ -129,6 +129,7 @@ struct ks_model {
     SplatC* d_splats = nullptr;
     SplatAdj* d_adj = nullptr;                         // thermal splats: constants of the adjoint
     double* d_knots = nullptr; double* d_ktimes = nullptr; int nknots = 0;
+    int64_t nkn = 0, nkt = 0;                          // knot triples and knot times held (knot models and composites with knot parts)
     // the same constants in FP32 (the mixed-precision transfer and adjoint)
     SplatCf* d_splats_f = nullptr; SplatAdjf* d_adj_f = nullptr; f32* d_knots_f = nullptr; f32* d_ktimes_f = nullptr;
 };
@@ -404,7 +405,7 @@ static void to_f32(const SplatC& a, SplatCf& b) {
     b.t0 = a.t0; b.iw = a.iw; b.omega = a.omega; b.rsup2 = a.rsup2; b.lnA = a.lnA; b.Bmag = a.Bmag;
     b.p1 = a.p1; b.p2 = a.p2; b.p3 = a.p3;
     for (int i = 0; i < 6; ++i) b.g[i] = a.g[i];
-    b.knot0 = a.knot0; b.kind = a.kind;
+    b.knot0 = a.knot0; b.nk = a.nk; b.kt0 = a.kt0; b.kind = a.kind;
 }
 static void to_f32(const SplatAdj& a, SplatAdjf& b) {
     std::memset(&b, 0, sizeof b);
@@ -444,6 +445,7 @@ int ks_model_create(ks_ctx* ctx, int kind, int nrows, int ns, const double* para
         c.rsup2 = SR2 * std::exp(2.0 * smax);
         c.lnA = p[12];
         c.kind = kind;
+        if (kind == KS_MODEL_KNOT) { c.knot0 = nknots * i; c.nk = nknots; c.kt0 = 0; }
         if (kind == KS_MODEL_THIN) continue;
         int rB = (kind == KS_MODEL_THERMAL || kind == KS_MODEL_KNOT) ? 14 : 15;
         c.Bmag = std::exp(p[rB]);
@@ -500,6 +502,7 @@ int ks_model_create(ks_ctx* ctx, int kind, int nrows, int ns, const double* para
     }
     if (kind == KS_MODEL_KNOT) {
         m->nknots = nknots;
+        m->nkn = (int64_t)nknots * ns; m->nkt = nknots;
         if (dalloc(&m->d_knots, (size_t)3 * nknots * ns) || dalloc(&m->d_ktimes, (size_t)nknots)) { ks_model_destroy(m); return KS_ERR_HIP; }
         HIPCHK(hipMemcpy(m->d_knots, knots, (size_t)3 * nknots * ns * sizeof(double), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(m->d_ktimes, knot_times, (size_t)nknots * sizeof(double), hipMemcpyHostToDevice));
@@ -520,16 +523,19 @@ static_assert(KS_MODEL_THIN == MODEL_THIN && KS_MODEL_THERMAL == MODEL_THERMAL &
 
 int ks_model_composite(ks_ctx* ctx, int nparts, const ks_model* const* parts, ks_model** out) {
     if (!ctx || !out || !parts || nparts < 1) return fail(KS_ERR_ARG, "ks_model_composite: null argument or no parts");
-    int64_t ns = 0;
+    int64_t ns = 0, nkn = 0, nkt = 0;    // splats, knot triples and knot times of all parts
     for (int p = 0; p < nparts; ++p) {
         const ks_model* q = parts[p];
         if (!q) return fail(KS_ERR_ARG, "ks_model_composite: part %d is null", p);
         if (q->ctx != ctx) return fail(KS_ERR_ARG, "ks_model_composite: part %d belongs to another context", p);
-        if (q->kind != KS_MODEL_THERMAL && q->kind != KS_MODEL_POWERLAW && q->kind != KS_MODEL_KAPPA && q->kind != KS_MODEL_COMPOSITE)
-            return fail(KS_ERR_ARG, "ks_model_composite: part %d has kind %d (parts are thermal, power-law, kappa or composite)", p, q->kind);
+        if (q->kind != KS_MODEL_THERMAL && q->kind != KS_MODEL_POWERLAW && q->kind != KS_MODEL_KAPPA && q->kind != KS_MODEL_KNOT &&
+            q->kind != KS_MODEL_COMPOSITE)
+            return fail(KS_ERR_ARG, "ks_model_composite: part %d has kind %d (parts are thermal, power-law, kappa, knot or composite)", p,
+                        q->kind);
         ns += q->ns;
+        nkn += q->nkn; nkt += q->nkt;
     }
-    if (ns > INT32_MAX) return fail(KS_ERR_DIM, "ks_model_composite: %lld splats", (long long)ns);
+    if (ns > INT32_MAX || nkn > INT32_MAX) return fail(KS_ERR_DIM, "ks_model_composite: %lld splats, %lld knots", (long long)ns, (long long)nkn);
     HIPCHK(hipSetDevice(ctx->device));
     ks_model* m = new ks_model;
     m->ctx = ctx; m->kind = KS_MODEL_COMPOSITE; m->nrows = 0; m->ns = (int)ns;
@@ -537,21 +543,46 @@ int ks_model_composite(ks_ctx* ctx, int nparts, const ks_model* const* parts, ks
     if (dalloc(&m->d_splats, n) || dalloc(&m->d_adj, n) || dalloc(&m->d_splats_f, n) || dalloc(&m->d_adj_f, n)) {
         ks_model_destroy(m); return KS_ERR_HIP;
     }
-    size_t off = 0;
+#define HIPCHK_M(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { ks_model_destroy(m); return fail(KS_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); } } while (0)
+    // the splats (with each knot splat's knot0 / kt0 moved by the offsets of its part's knots) and the knots, concatenated
+    std::vector<SplatC> h(n);
+    std::vector<SplatCf> hf(n);
+    std::vector<double> kn((size_t)3 * nkn), kt((size_t)nkt);
+    size_t off = 0, okn = 0, okt = 0;
     for (int p = 0; p < nparts; ++p) {
         const ks_model* q = parts[p];
-        if (q->ns < 1) continue;
-        const hipError_t e[4] = {
-            hipMemcpy(m->d_splats + off, q->d_splats, q->ns * sizeof(SplatC), hipMemcpyDeviceToDevice),
-            hipMemcpy(m->d_adj + off, q->d_adj, q->ns * sizeof(SplatAdj), hipMemcpyDeviceToDevice),
-            hipMemcpy(m->d_splats_f + off, q->d_splats_f, q->ns * sizeof(SplatCf), hipMemcpyDeviceToDevice),
-            hipMemcpy(m->d_adj_f + off, q->d_adj_f, q->ns * sizeof(SplatAdjf), hipMemcpyDeviceToDevice)};
-        for (hipError_t x : e)
-            if (x != hipSuccess) { ks_model_destroy(m); return fail(KS_ERR_HIP, "ks_model_composite: %s", hipGetErrorString(x)); }
-        off += q->ns;
+        if (q->ns > 0) {
+            HIPCHK_M(hipMemcpy(h.data() + off, q->d_splats, q->ns * sizeof(SplatC), hipMemcpyDeviceToHost));
+            HIPCHK_M(hipMemcpy(m->d_adj + off, q->d_adj, q->ns * sizeof(SplatAdj), hipMemcpyDeviceToDevice));
+            HIPCHK_M(hipMemcpy(m->d_adj_f + off, q->d_adj_f, q->ns * sizeof(SplatAdjf), hipMemcpyDeviceToDevice));
+            for (int i = 0; i < q->ns; ++i)
+                if (h[off + i].kind == KS_MODEL_KNOT) { h[off + i].knot0 += (int)okn; h[off + i].kt0 += (int)okt; }
+        }
+        if (q->nkn > 0) {
+            HIPCHK_M(hipMemcpy(kn.data() + 3 * okn, q->d_knots, (size_t)3 * q->nkn * sizeof(double), hipMemcpyDeviceToHost));
+            HIPCHK_M(hipMemcpy(kt.data() + okt, q->d_ktimes, (size_t)q->nkt * sizeof(double), hipMemcpyDeviceToHost));
+        }
+        off += q->ns; okn += q->nkn; okt += q->nkt;
+    }
+    for (size_t i = 0; i < n; ++i) to_f32(h[i], hf[i]);
+    HIPCHK_M(hipMemcpy(m->d_splats, h.data(), n * sizeof(SplatC), hipMemcpyHostToDevice));
+    HIPCHK_M(hipMemcpy(m->d_splats_f, hf.data(), n * sizeof(SplatCf), hipMemcpyHostToDevice));
+    if (nkn > 0) {
+        m->nknots = (int)nkt; m->nkn = nkn; m->nkt = nkt;
+        std::vector<f32> knf(kn.size()), ktf(kt.size());
+        for (size_t i = 0; i < kn.size(); ++i) knf[i] = f32(kn[i]);
+        for (size_t i = 0; i < kt.size(); ++i) ktf[i] = f32(kt[i]);
+        if (dalloc(&m->d_knots, kn.size()) || dalloc(&m->d_ktimes, kt.size()) || dalloc(&m->d_knots_f, kn.size()) || dalloc(&m->d_ktimes_f, kt.size())) {
+            ks_model_destroy(m); return KS_ERR_HIP;
+        }
+        HIPCHK_M(hipMemcpy(m->d_knots, kn.data(), kn.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK_M(hipMemcpy(m->d_ktimes, kt.data(), kt.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK_M(hipMemcpy(m->d_knots_f, knf.data(), knf.size() * sizeof(f32), hipMemcpyHostToDevice));
+        HIPCHK_M(hipMemcpy(m->d_ktimes_f, ktf.data(), ktf.size() * sizeof(f32), hipMemcpyHostToDevice));
     }
     *out = m;
     return KS_OK;
+#undef HIPCHK_M
 }
 
 int ks_model_destroy(ks_model* m) {

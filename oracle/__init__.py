@@ -149,26 +149,36 @@ def spacetime_tangent(a, thetao, alphas, betas, N, params, t_obs, nu, L, kind="t
 
 def polarized_composite(a, thetao, alphas, betas, N, parts, t_obs, nu, L, nmax=-1, slab=0.0, M=64, marcher=1):
     """Stokes image of CompositeModel(parts...) (transport.jl:298-316). parts: a list of
-    (kind, params) or (kind, params, hyp, gamma_max) with kind thermal, powerlaw or kappa."""
+    (kind, params) or (kind, params, hyp, gamma_max) with kind thermal, powerlaw or kappa, or
+    ("knot", params, knots, knot_times) with knots 3 x nknots x ns."""
     al, be = _f64(alphas), _f64(betas)
     n, k = len(al), len(parts)
-    kinds, rows, ns = (ctypes.c_int * k)(), (ctypes.c_int * k)(), (ctypes.c_int * k)()
+    kinds, rows, ns, nk = (ctypes.c_int * k)(), (ctypes.c_int * k)(), (ctypes.c_int * k)(), (ctypes.c_int * k)()
     Ps, Hs, gm = (ctypes.c_void_p * k)(), (ctypes.c_void_p * k)(), (ctypes.c_double * k)()
+    Ks, Ts = (ctypes.c_void_p * k)(), (ctypes.c_void_p * k)()
     keep = []
     for q, part in enumerate(parts):
         kind, params = part[0], part[1]
-        hyp = part[2] if len(part) > 2 else None
         P = np.asfortranarray(params, dtype=np.float64)
-        hy = _f64(hyp) if hyp is not None else None
-        keep += [P, hy]
         kinds[q], rows[q], ns[q] = KIND[kind], P.shape[0], P.shape[1]
         Ps[q] = P.ctypes.data
+        gm[q] = 1e5
+        keep.append(P)
+        if kind == "knot":
+            kn = np.asfortranarray(part[2], dtype=np.float64)
+            kt = _f64(part[3])
+            keep += [kn, kt]
+            Ks[q], Ts[q], nk[q] = kn.ctypes.data, kt.ctypes.data, len(kt)
+            continue
+        hyp = part[2] if len(part) > 2 else None
+        hy = _f64(hyp) if hyp is not None else None
+        keep.append(hy)
         Hs[q] = hy.ctypes.data if hy is not None else None
         gm[q] = part[3] if len(part) > 3 else 1e5
     out = np.zeros((n, 4))
     rt, rp = np.zeros(n), np.zeros(n)
     rc = lib().kso_polarized_composite(_d(a), _d(thetao), _i(n), _p(al), _p(be), _i(N), _i(M), _i(marcher), _i(k), kinds, rows, ns,
-                                       Ps, Hs, gm, _d(t_obs), _d(nu), _d(L), _i(nmax), _d(slab), _p(out), _p(rt), _p(rp))
+                                       Ps, Hs, gm, Ks, Ts, nk, _d(t_obs), _d(nu), _d(L), _i(nmax), _d(slab), _p(out), _p(rt), _p(rp))
     if rc:
         raise ValueError(f"kso_polarized_composite: bad parts (code {rc})")
     return out, rt, rp

@@ -226,14 +226,18 @@ int kso_spacetime_tangent(double a, double thetao, int npix, const double* alpha
 // run over part 0, then part 1, ... Knot parts are not composed here (their knots stay per model).
 int kso_polarized_composite(double a, double thetao, int npix, const double* alphas, const double* betas, int N, int M, int marcher,
                             int nparts, const int* kinds, const int* nrows, const int* ns, const double* const* params,
-                            const double* const* hyp, const double* gamma_max, double t_obs, double nu, double L, int nmax, double slab,
+                            const double* const* hyp, const double* gamma_max, const double* const* knots, const double* const* knot_times,
+                            const int* nknots, double t_obs, double nu, double L, int nmax, double slab,
                             double* out, double* resid_t, double* resid_ph) {
     if (nparts < 1) return 1;
     std::vector<ModelHolder> parts;
     parts.reserve(nparts);
     for (int p = 0; p < nparts; ++p) {
-        if (kinds[p] == MODEL_THIN || kinds[p] == MODEL_KNOT) return 2;
-        parts.push_back(make_model<double>(kinds[p], nrows[p], ns[p], params[p], hyp ? hyp[p] : nullptr, gamma_max[p], nullptr, nullptr, 0));
+        if (kinds[p] == MODEL_THIN) return 2;
+        if (kinds[p] == MODEL_KNOT && (!knots || !knots[p] || !knot_times[p] || nknots[p] < 1)) return 3;
+        const bool kn = kinds[p] == MODEL_KNOT;
+        parts.push_back(make_model<double>(kinds[p], nrows[p], ns[p], params[p], hyp ? hyp[p] : nullptr, gamma_max[p], kn ? knots[p] : nullptr,
+                                           kn ? knot_times[p] : nullptr, kn ? nknots[p] : 0));
     }
     for (int p = 0; p < nparts; ++p) {
         parts[p].m.bessel = parts[p].bessel.data();

@@ -203,6 +203,43 @@ def test_polarized_composite_tile_cull_vs_oracle(oracle):
     _composite_case(oracle, "stored", 40, 16, 16, res=12, N=128)
 
 
+@pytest.mark.parametrize("path,nk1,nth", [("stored", 3, 3), ("fused", 3, 3), ("stored", 30, 40)])
+def test_polarized_composite_knots_vs_oracle(oracle, path, nk1, nth):
+    """CompositeModel(KnotSplats, PolarizedSplats, KnotSplats) with the two knot parts on different knot-time
+    grids (each part's knots and times stay its own in the concatenated model) against the oracle's composite;
+    with 72 splats the transfer would take the per-tile cull, which bounds pattern-rotating splats only and
+    is left out for a composite holding knots."""
+    from test_oracle_composite import knot_set
+    a, th = 0.9, math.radians(45.0)
+    cam = camera(12.0, 16)
+    L = syn.length_unit()
+    cache = make_cache(cam, 256, a, th, path)
+    t1, t2 = np.array([-200.0, -100.0, 0.0, 100.0]), np.array([-150.0, -20.0, 60.0])
+    P1, k1 = knot_set(a, nk1, t1, seed=4)
+    P2, k2 = knot_set(a, 2, t2, seed=6)
+    T = syn.thermal_splats(1, a, nth)
+    ps = [("knot", P1, k1, t1), ("thermal", T), ("knot", P2, k2, t2)]
+    model = K.CompositeModel(K.Model(P1, kind="knot", knots=k1, knot_times=t1),
+                             K.CompositeModel(K.Model(T), K.Model(P2, kind="knot", knots=k2, knot_times=t2)))
+    assert model.ns == nk1 + nth + 2
+    for t_obs in (30.0, -170.0):
+        img = K.polarized_image(cache, model, t_obs, 230e9, L)
+        ref = oracle.polarized_composite(a, th, cam.alphas, cam.betas, 256, ps, t_obs, 230e9, L)[0].reshape(cam.size + (4,), order="F")
+        I = np.abs(ref[..., 0])
+        assert I.max() > 0
+        assert np.abs(img - ref).max() <= 1e-10 * I.max()
+        # the knot parts show: without both (the thermal part alone) and, with few splats, without the second
+        for sub in [ps[1:2]] + ([ps[:2]] if nk1 < 10 else []):
+            alone = oracle.polarized_composite(a, th, cam.alphas, cam.betas, 256, sub, t_obs, 230e9, L)[0]
+            assert np.abs(alone.reshape(cam.size + (4,), order="F") - ref).max() > 1e-3 * I.max()
+    if path == "stored":
+        K.precision(cache, "mixed")
+        got = K.polarized_image(cache, model, 30.0, 230e9, L)
+        K.precision(cache, "fp64")
+        img = K.polarized_image(cache, model, 30.0, 230e9, L)
+        assert 0 < np.abs(got - img).max() <= 1e-4 * np.abs(img[..., 0]).max()
+
+
 def test_composite_errors():
     th = K.Model(syn.thermal_splats(1, 0.9, 2))
     with pytest.raises(ValueError, match="parts"):

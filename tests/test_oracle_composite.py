@@ -73,3 +73,39 @@ def test_composite_order_is_rounding_only(oracle, pixels):
     ps = parts(oracle)
     comp = render(oracle, pixels, ps)
     assert np.abs(render(oracle, pixels, ps[::-1]) - comp).max() <= 1e-13 * comp[:, 0].max()
+
+
+def knot_set(a, n, times, seed=4):
+    """Thermal splats whose centres follow knots on their pattern circles at the given times."""
+    P = syn.thermal_splats(seed, a, n)
+    kn = np.zeros((3, len(times), n))
+    for i in range(n):
+        for k, t in enumerate(times):
+            ph = P[20, i] * t
+            kn[:, k, i] = [P[0, i] * math.cos(ph) - P[1, i] * math.sin(ph), P[0, i] * math.sin(ph) + P[1, i] * math.cos(ph), P[2, i]]
+    return P, kn
+
+
+def test_knot_parts_split_and_add(oracle, pixels):
+    """Knot parts (KnotSplats, trajectories.jl:71-90) in a composite: a knot set cut in two renders the single
+    set exactly, and a knot part beside a thermal part adds to it in the thin limit (first order)."""
+    times = np.array([-200.0, -100.0, 0.0, 100.0])
+    P, kn = knot_set(A, 4, times)
+    L = syn.length_unit()
+    ref = oracle.polarized_image(A, TH, pixels[0], pixels[1], N, P, 30.0, NU, L, kind="knot", knots=kn, knot_times=times)[0]
+    assert ref[:, 0].max() > 0
+    comp = oracle.polarized_composite(A, TH, pixels[0], pixels[1], N, [("knot", P[:, :2], kn[:, :, :2], times),
+                                                                       ("knot", P[:, 2:], kn[:, :, 2:], times)], 30.0, NU, L)[0]
+    assert np.array_equal(comp, ref)
+    rem = []
+    for scale in (1e-3, 1e-6):
+        Q, T = P.copy(), syn.thermal_splats(1, A, 3)
+        Q[12] += math.log(scale)
+        T[12] += math.log(scale)
+        parts_ = [("knot", Q, kn, times), ("thermal", T)]
+        both = oracle.polarized_composite(A, TH, pixels[0], pixels[1], N, parts_, 30.0, NU, L)[0]
+        each = [oracle.polarized_composite(A, TH, pixels[0], pixels[1], N, [p], 30.0, NU, L)[0] for p in parts_]
+        peak = both[:, 0].max()
+        assert min(e[:, 0].max() for e in each) >= 1e-2 * peak
+        rem.append(np.abs(both - sum(each)).max() / peak)
+    assert rem[1] <= 5e-5 and 1e2 <= rem[0] / rem[1] <= 1e4, rem
