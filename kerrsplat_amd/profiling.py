@@ -2,12 +2,17 @@
 
     python -m kerrsplat_amd.profiling trace  <kernel_stats.csv>                      > summary.md
     python -m kerrsplat_amd.profiling launches <kernel_trace.csv>                     > summary.md
+    python -m kerrsplat_amd.profiling valu <counter_collection.csv with SQ_INSTS_VALU, GRBM_GUI_ACTIVE> > summary.md
     python -m kerrsplat_amd.profiling traffic <fetch counter_collection.csv> <write counter_collection.csv> \
         [--copy-bytes N] [--json out.json]                                           > summary.md
 
 `trace` tabulates the kernel-trace statistics (calls, average and total duration per kernel).
 `launches` does the same from the per-dispatch trace, one row per (kernel, grid size), with the
 registers and scratch (spill) bytes per lane the compiler gave the kernel.
+`valu` turns SQ_INSTS_VALU and GRBM_GUI_ACTIVE into the VALU issue utilisation of each kernel: a wave64
+vector instruction holds a SIMD (16 lanes) for 4 cycles, so an XCD of 32 CUs x 4 SIMDs issues at most 32
+wave-instructions per cycle; GRBM_GUI_ACTIVE sums the busy cycles of the 8 XCDs, so the utilisation is
+SQ_INSTS_VALU / (32 GRBM_GUI_ACTIVE) (quarter-rate transcendentals make the true issue limit lower still).
 `traffic` turns FETCH_SIZE and WRITE_SIZE (KiB per dispatch) into HBM bytes per launch: each counter is
 scaled by what it reads for the runtime's device-to-device copy of a known size (bench.py
 --copy-probe copies 256 MiB, so FETCH and WRITE should each read 256 MiB for that dispatch); the
@@ -85,6 +90,23 @@ def traffic_summary(fetch_csv, write_csv, copy_bytes=1 << 28):
     return out
 
 
+def valu_summary(path, top=12):
+    """Per (kernel, grid): wave-level VALU and SALU instructions and busy cycles per dispatch, and the VALU issue
+    utilisation SQ_INSTS_VALU / (32 GRBM_GUI_ACTIVE), heaviest first."""
+    valu, salu, busy = (counter_means(path, c) for c in ("SQ_INSTS_VALU", "SQ_INSTS_SALU", "GRBM_GUI_ACTIVE"))
+    rows = []
+    for key, v in valu.items():
+        b = busy.get(key)
+        if not b:
+            continue
+        iv, ib = sum(v) / len(v), sum(b) / len(b)
+        isl = salu.get(key, [0.0])
+        rows.append({"kernel": key[0], "grid": key[1], "dispatches": len(v), "valu": iv, "salu": sum(isl) / len(isl), "cycles": ib,
+                     "util": iv / (32.0 * ib) if ib > 0 else 0.0})
+    rows.sort(key=lambda r: -r["valu"])
+    return rows[:top]
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -92,12 +114,20 @@ def main(argv=None):
     t.add_argument("stats")
     l_ = sub.add_parser("launches")
     l_.add_argument("trace")
+    v = sub.add_parser("valu")
+    v.add_argument("counters")
     c = sub.add_parser("traffic")
     c.add_argument("fetch")
     c.add_argument("write")
     c.add_argument("--copy-bytes", type=int, default=1 << 28)
     c.add_argument("--json")
     a = ap.parse_args(argv)
+    if a.cmd == "valu":
+        print("| kernel | grid (threads) | VALU instr. (wave) | SALU instr. (wave) | busy cycles (8 XCDs) | VALU issue utilisation |\n"
+              "|---|---|---|---|---|---|")
+        for r in valu_summary(a.counters):
+            print(f"| `{r['kernel']}` | {r['grid']} | {r['valu']:.3g} | {r['salu']:.3g} | {r['cycles']:.3g} | {100 * r['util']:.0f}% |")
+        return
     if a.cmd == "trace":
         rows = trace_summary(a.stats)
         print("| kernel | calls | avg (us) | total (ms) | % |\n|---|---|---|---|---|")
