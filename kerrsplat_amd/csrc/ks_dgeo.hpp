@@ -454,6 +454,17 @@ KS_DEV bool krang_visible(const VisCT<R>& v, R cth) {
     return !(v.absbe_eps < bb);
 }
 
+// Whether Mino time tau lies before the radial turning point fo (nu_r, direct_march.jl): tau < fo, with a sample
+// within 1e-12 fo of the turning point taken as at it. With tau_total = 2 fo and N + 1 even the turning point is a
+// sample in exact arithmetic, and the strict comparison would be decided by rounding (flipping the sign of the
+// radial momentum there, where it is sqrt of a rounding-sized R).
+KS_DEV double val(double x);     // ks_ddual.hpp (the dual scalars' val is found by argument-dependent lookup)
+template <class R>
+KS_DEV bool before_turning(const R& tau, const R& fo) {
+    const double t = val(tau), f = val(fo);
+    return t < f && (!isfinite(f) || f - t > 1e-12 * fabs(f));
+}
+
 template <class R>
 KS_DEV R radius_jac(const PixCT<R>& p, R sn, R cn) {
     if (p.rcase == 2) {
@@ -516,7 +527,7 @@ KS_DEV SampT<R> direct_sample(const PixCT<R>& p, const SpinT<R>& S, const VisCT<
     R cth;
     polar_from_jac<R>(p, jt.sn, jt.cn, jt.dn, s.th, cth, s.nuth);
     t_phi_direct<R>(p, S, tau, s.t, s.ph);
-    s.nur = tau < p.fo;
+    s.nur = before_turning<R>(tau, p.fo);
     s.ok = (tau <= p.tau_h) && krang_visible<R>(vis, cth);
     return s;
 }

@@ -127,7 +127,8 @@ KS_DEV void make_geo(Q a, Q r, Q th, Q eta, Q lam, bool nur, bool nuth, GeoT<Q>&
     Q w = r * r + a * a - a * lam;
     Q R = w * w - g.Dl * (eta + (lam - a) * (lam - a));
     Q Th = eta + a * a * g.ct * g.ct - lam * lam * g.ct * g.ct / (g.st * g.st);
-    Q pr = (nur ? 1.0 : -1.0) * (R > 0.0 ? sqrt(R) : Q(0.0)) / g.Dl;
+    // R below 1e-14 w^2 is rounding (a sample on the radial turning point): its square root would be noise
+    Q pr = (nur ? 1.0 : -1.0) * (R > 1e-14 * (w * w) ? sqrt(R) : Q(0.0)) / g.Dl;
     Q pth = (nuth ? 1.0 : -1.0) * (Th > 0.0 ? sqrt(Th) : Q(0.0));
     // p^mu = g^{mu nu} p_nu with p_t = -1, p_phi = lam
     Q DS = g.Dl * g.S;

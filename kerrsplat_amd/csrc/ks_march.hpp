@@ -69,9 +69,11 @@ KS_DEV RP ray_point(const PixC& p, const Spin& S, const QC& q, const JS& xr, con
     RP o;
     o.r = radius_jac(p, xr.sn, xr.cn);
     polar_from_jac(p, xt.sn, xt.cn, xt.dn, o.th, o.cth, o.nuth);
-    o.nur = tau < p.fo;
+    o.nur = before_turning<double>(tau, p.fo);
     double R = r_potential(S.a, p.lam, p.eta, o.r);
-    o.s = (o.nur ? -1.0 : 1.0) * sqrt(dmax(R, 0.0));
+    const double w = o.r * o.r + S.a * S.a - S.a * p.lam;
+    // R below 1e-14 w^2 is rounding (a sample on the radial turning point, where R = 0): s = 0 there
+    o.s = (o.nur ? -1.0 : 1.0) * (R > 1e-14 * (w * w) ? sqrt(R) : 0.0);
     double idn = 1.0 / xt.dn;
     o.sn = xt.sn * idn / p.scale;
     o.cn = xt.cn * idn;
@@ -155,6 +157,10 @@ KS_DEV void march_range(const PixC& p, const Spin& S, double thetao, int N, int 
     const VisC vis = vis_constants(p, S.a, thetao);
     const double dtau = p.tau_total / (double)(N + 1);
     const double h = 0.5 * dtau;
+    // the radial turning point splits the interval that holds it (ta < fo < tb). With N + 1 even, fo = tau_total / 2
+    // falls on a sample in exact arithmetic, and the strict comparison would be decided by the rounding of fo and
+    // k dtau; a turning point within 1e-9 dtau of a sample is taken as on it (no split, the exact-arithmetic case)
+    const double tol = 1e-9 * dtau;
     const int Man = anchor_interval(M, 1.0 - p.k);
     const JS Dr = js_step_constants(-p.c * h, p.k);
     const JS Dt = js_step_constants(p.scale * h / p.f, p.mu);
@@ -191,7 +197,7 @@ KS_DEV void march_range(const PixC& p, const Spin& S, double thetao, int N, int 
         else { xr = js_step(xr, Dr, p.k); xt = js_step(xt, Dt, p.mu); }
         RP pb = ray_point(p, S, q, xr, xt, tb);
         const double Pb = polar_closed_form(p, q, tb, pb.sn, pb.cn);
-        if (q.scattering && ta < p.fo && p.fo < tb) {
+        if (q.scattering && p.fo - ta > tol && tb - p.fo > tol) {
             turning_increment(p, S, q, pa, pb, ta, tb, Pa, Pb, t, ph);
         } else {
             double It, Ip;

@@ -442,6 +442,14 @@ template <class T> void radial_t_phi(const Pixel<T>& p, T tau, T& t, T& ph) {
     ph = (p.Qp * (p.Gpi - Gp) + p.Qm * (p.Gmi - Gm)) / p.c;
 }
 
+// tau before the radial turning point fo (nu_r): tau < fo, with a sample within 1e-12 fo of the turning point taken
+// as at it. With tau_total = 2 fo and N + 1 even the turning point is a sample in exact arithmetic, and the strict
+// comparison of direct_march.jl would be decided by rounding (the radial momentum's sign, sqrt of a rounding-sized R).
+template <class T> bool before_turning(const T& tau, const T& fo) {
+    const double t = val(tau), f = val(fo);
+    return t < f && (!std::isfinite(f) || f - t > 1e-12 * std::fabs(f));
+}
+
 template <class T> Sample<T> direct_sample(const Pixel<T>& p, T tau) {
     Sample<T> s;
     s.r = radius_direct(p, p.c * (p.fo - tau));
@@ -452,7 +460,7 @@ template <class T> Sample<T> direct_sample(const Pixel<T>& p, T tau) {
     polar_integrals(p, tau, Gt, Gphi);
     s.t = tr + p.a * p.a * Gt;
     s.ph = pr + p.lam * Gphi;
-    s.nur = tau < p.fo;
+    s.nur = before_turning(tau, p.fo);
     s.ok = (tau <= p.tau_h) && krang_visible(vis_constants(p), cth);
     return s;
 }
@@ -559,8 +567,11 @@ template <class T> RayPoint<T> ray_point(const Marcher<T>& m, const JState<T>& x
     RayPoint<T> q;
     q.r = radius_state(p, xr);
     polar_state(m, xt, q.th, q.nuth, q.cth);
-    q.nur = tau < p.fo;
-    q.s = (q.nur ? T(-1.0) : T(1.0)) * ksqrt(kmax(r_potential(p, q.r), T(0.0)));
+    q.nur = before_turning(tau, p.fo);
+    {   // R below 1e-14 w^2 is rounding (a sample on the radial turning point, where R = 0): s = 0 there
+        const T R = r_potential(p, q.r), w = q.r * q.r + p.a * p.a - p.a * p.lam;
+        q.s = (q.nur ? T(-1.0) : T(1.0)) * (R > 1e-14 * (w * w) ? ksqrt(R) : T(0.0));
+    }
     q.sn = xt.sn / (m.scale * xt.dn);
     q.cn = xt.cn / xt.dn;
     T dn = 1.0 / xt.dn;
@@ -624,6 +635,10 @@ template <class T, class F> void march_ray(const Pixel<T>& p, int N, int M, F&& 
     Marcher<T> m = make_marcher(p);
     T dtau = p.tau_total / (double)(N + 1);
     T h = dtau / 2.0;
+    // the split at the turning point (quadrature.jl:269, τa < fo < τb): with N + 1 even, fo = τ_total / 2 falls on a
+    // sample in exact arithmetic and the strict comparison is decided by rounding; a turning point within 1e-9 dtau
+    // of a sample is taken as on it (no split), the exact-arithmetic outcome
+    T tol = 1e-9 * dtau;
     int Man = anchor_interval(M, 1.0 - p.k);
     JState<T> Dr = jstep_constants(-p.c * h, m.k);
     JState<T> Dt = jstep_constants(m.scale * h / m.tempfac, m.mu);
@@ -643,7 +658,7 @@ template <class T, class F> void march_ray(const Pixel<T>& p, int N, int M, F&& 
         if (anchor) { xr = radial_state_at(m, tb); xt = polar_state_at(m, tb); }
         else { xr = jstep(xr, Dr, m.k); xt = jstep(xt, Dt, m.mu); }
         RayPoint<T> pb = ray_point(m, xr, xt, tb);
-        if (m.scattering && ta < p.fo && p.fo < tb) {
+        if (m.scattering && p.fo - ta > tol && tb - p.fo > tol) {
             RayPoint<T> pt = ray_point_turning(m);
             RayPoint<T> p1 = ray_point_direct(m, (ta + p.fo) / 2.0);
             RayPoint<T> p2 = ray_point_direct(m, (p.fo + tb) / 2.0);

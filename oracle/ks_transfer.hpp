@@ -514,7 +514,9 @@ template <class T> void momentum_bl_d(T a, T r, T th, T eta, T lam, bool nur, bo
     T R = w * w - Dl * (eta + (lam - a) * (lam - a));
     T c = kcos(th), s = ksin(th);
     T Th = eta + a * a * c * c - lam * lam * c * c / (s * s);
-    T sR = R > 0.0 ? ksqrt(R) : T(0.0), sT = Th > 0.0 ? ksqrt(Th) : T(0.0);
+    // R below 1e-14 w^2 is rounding (a sample on the radial turning point, where R = 0): its square root would be
+    // noise of order 1e-7 w, taken as 0
+    T sR = R > 1e-14 * (w * w) ? ksqrt(R) : T(0.0), sT = Th > 0.0 ? ksqrt(Th) : T(0.0);
     pd[0] = -1.0; pd[1] = (nur ? T(1.0) : T(-1.0)) * sR / Dl; pd[2] = (nuth ? T(1.0) : T(-1.0)) * sT; pd[3] = lam;
 }
 // inverse BL metric times a covariant vector (t, r, theta, phi)
