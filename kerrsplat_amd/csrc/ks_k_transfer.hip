@@ -92,8 +92,9 @@ KS_DEV bool sample_coefficients(const TransCtxT<S>& c, double st, double r, doub
         S cb = dmin(dmax(fr.cosB, -1.0), 1.0);
         S sth, cth;
         // FP64 takes cos = cb and the product form of the sine; the FP32 path keeps acos and sincos: with the
-        // product form its images moved further from the FP64 images (9.0e-7 against 6.2e-7 of peak I,
-        // 16 x 16 pixels, N = 256, 4 thermal splats), which the sine's accuracy alone does not explain
+        // product form, or with the sine taken in FP64 from cb, its images moved further from the FP64 images
+        // (9.0e-7 against 6.2e-7 of peak I, 16 x 16 pixels, N = 256, 4 thermal splats), which the sine's
+        // accuracy alone does not explain
         if constexpr (std::is_same<S, double>::value) pitch_sincos(cb, sth, cth);
         else sincos(acos(cb), &sth, &cth);
         S ne = exp(sc.lnA) * G_;
