@@ -21,7 +21,7 @@ constexpr int PART_BLOCK = 256;    // block of the case partition
 constexpr int CHUNK = 32;          // samples per work item of the transfer from stored samples
 constexpr int SEG_FIELDS = 20;     // a segment map: P (16, row-major), then Sv (4)
 constexpr int ADJ_ROWS = 21;       // parameter rows of a thermal splat
-constexpr int ADJ_SLICE = 128;     // samples per work unit of the element adjoints
+constexpr int ADJ_SLICE = 32;      // samples per work unit of the element adjoints
 constexpr int CAND_MAX = 1024;     // splats a tile's candidate list holds (more splats: no per-tile cull)
 constexpr int CULL_MIN_NS = 64;    // the transfer culls per tile from this many splats on (below, the per-sample test is cheaper)
 constexpr int UB_FIELDS = 8;       // a unit's region: centre (3), radius, emission times [min, max], any sample, pad
