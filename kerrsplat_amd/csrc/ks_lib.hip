@@ -12,6 +12,7 @@
 #include <vector>
 #include <algorithm>
 #include <cmath>
+#include <atomic>
 
 #include "ks_kernels.hpp"
 #include "../../include/kerrsplat_hip.h"
@@ -39,6 +40,10 @@ template <class T> int dalloc(T** p, size_t n) {
     return 0;
 }
 template <class T> void dfree(T*& p) { if (p) { (void)hipFree(p); p = nullptr; } }
+uint64_t next_model_id() {
+    static std::atomic<uint64_t> n{0};
+    return ++n;
+}
 // a grow-only device buffer
 template <class T> struct Buf {
     T* p = nullptr;
@@ -108,7 +113,7 @@ struct ks_cache {
     double* apart = nullptr; double* adpar = nullptr; double* aub = nullptr;
     int64_t a_items = 0, a_part = 0, a_dpar = 0, a_ub = 0;
     // the last forward pass of the dual sweep (ks_gradient_forward): what ks_gradient_reverse needs
-    const ks_model* fw_model = nullptr;
+    uint64_t fw_model = 0;                             // id of its model (0: no forward pass to reverse)
     int fw_nt = 0, fw_nnu = 0, fw_generation = -1, fw_precision = -1;
     std::vector<uint8_t> fw_seeded;                    // frames whose device seed a *_chi2_seed call wrote
     std::vector<int> perm_host;
@@ -125,6 +130,8 @@ struct ks_binning {
 
 struct ks_model {
     ks_ctx* ctx = nullptr;
+    uint64_t id = 0;                                   // unique per created model: a model freed and re-created at the same
+                                                       // address is not the model of an earlier forward pass
     int kind = KS_MODEL_THERMAL, nrows = 0, ns = 0;
     SplatC* d_splats = nullptr;
     SplatAdj* d_adj = nullptr;                         // thermal splats: constants of the adjoint
@@ -487,6 +494,7 @@ int ks_model_create(ks_ctx* ctx, int kind, int nrows, int ns, const double* para
         }
     }
     ks_model* m = new ks_model;
+    m->id = next_model_id();
     m->ctx = ctx; m->kind = kind; m->nrows = nrows; m->ns = ns;
     if (dalloc(&m->d_splats, h.size()) || dalloc(&m->d_adj, ha.size()) || dalloc(&m->d_splats_f, h.size()) || dalloc(&m->d_adj_f, ha.size())) {
         ks_model_destroy(m); return KS_ERR_HIP;
@@ -538,6 +546,7 @@ int ks_model_composite(ks_ctx* ctx, int nparts, const ks_model* const* parts, ks
     if (ns > INT32_MAX || nkn > INT32_MAX) return fail(KS_ERR_DIM, "ks_model_composite: %lld splats, %lld knots", (long long)ns, (long long)nkn);
     HIPCHK(hipSetDevice(ctx->device));
     ks_model* m = new ks_model;
+    m->id = next_model_id();
     m->ctx = ctx; m->kind = KS_MODEL_COMPOSITE; m->nrows = 0; m->ns = (int)ns;
     const size_t n = ns > 0 ? (size_t)ns : 1;
     if (dalloc(&m->d_splats, n) || dalloc(&m->d_adj, n) || dalloc(&m->d_splats_f, n) || dalloc(&m->d_adj_f, n)) {
@@ -666,7 +675,7 @@ int ks_render_device(ks_cache* c, const ks_model* m, const double* times, int nt
     for (int i = 0; i < nnu; ++i) if (!(nus[i] > 0.0)) return fail(KS_ERR_ARG, "frequencies must be positive");
     ks_ctx* x = c->ctx;
     HIPCHK(hipSetDevice(x->device));
-    c->fw_model = nullptr;             // the render reuses the segment maps, times and images of a forward pass
+    c->fw_model = 0;                   // the render reuses the segment maps, times and images of a forward pass
     const int64_t frames = (int64_t)nt * nnu;
     if (ensure_out(c, frames)) return KS_ERR_HIP;
     const double* d_tn = stage_times(c, times, nt, nus, nnu);
@@ -743,7 +752,7 @@ int ks_gradient_forward(ks_cache* c, const ks_model* m, const double* times, int
     for (int i = 0; i < nnu; ++i) if (!(nus[i] > 0.0)) return fail(KS_ERR_ARG, "frequencies must be positive");
     ks_ctx* x = c->ctx;
     HIPCHK(hipSetDevice(x->device));
-    c->fw_model = nullptr;
+    c->fw_model = 0;
     const int64_t frames = (int64_t)nt * nnu;
     const int64_t items = c->npix * frames;
     const int ns = m->ns;
@@ -753,7 +762,7 @@ int ks_gradient_forward(ks_cache* c, const ks_model* m, const double* times, int
     int rc = launch_transfer(c, m, d_tn, nt, nnu, L, nmax, slab, c->arec, c->aflg);
     if (rc) return rc;
     HIPCHK(launch_seg_combine(x->stream, c->soa, c->npix, c->N, c->M, CHUNK, nt, d_tn + nt, nnu, c->seg, c->out));
-    c->fw_model = m; c->fw_nt = nt; c->fw_nnu = nnu; c->fw_generation = c->generation; c->fw_precision = c->precision;
+    c->fw_model = m->id; c->fw_nt = nt; c->fw_nnu = nnu; c->fw_generation = c->generation; c->fw_precision = c->precision;
     c->fw_seeded.assign((size_t)frames, 0);
     if (image) return ks_copy_stokes(c, frames, image, screen_order);
     return KS_OK;
@@ -763,7 +772,7 @@ int ks_gradient_forward(ks_cache* c, const ks_model* m, const double* times, int
 int ks_gradient_reverse(ks_cache* c, const ks_model* m, const double* dstokes, int screen_order, double* dparams) {
     if (!c || !m || !dparams) return fail(KS_ERR_ARG, "ks_gradient_reverse: null argument");
     if (!c->fw_model) return fail(KS_ERR_STATE, "no forward pass to reverse: call ks_gradient_forward first");
-    if (m != c->fw_model) return fail(KS_ERR_ARG, "the reverse pass takes the model of its forward pass");
+    if (m->id != c->fw_model) return fail(KS_ERR_ARG, "the reverse pass takes the model of its forward pass");
     if (c->generation != c->fw_generation || c->precision != c->fw_precision)
         return fail(KS_ERR_STATE, "the cache changed (regenerate! or precision) since the forward pass");
     ks_ctx* x = c->ctx;

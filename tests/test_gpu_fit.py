@@ -234,6 +234,29 @@ def test_seed_errors():
         K.gradient_reverse(cache, m, dp, np.zeros((1, 2, cache.npix, 4)))
 
 
+def test_reverse_rejects_a_model_recreated_at_the_same_address():
+    """A model freed after the forward pass and a new one created (often at the freed address) is not the
+    forward pass's model: the library tells models apart by a creation id, not by their address."""
+    na = 6
+    cam, cache = stored_cache(na, 64)
+    L = syn.length_unit()
+    P = syn.thermal_splats(1, A, 1)
+    m = K.Model(P)
+    K.gradient_forward(cache, m, [0.0], [NU], L, copy=False)
+    old = m._h
+    m.__del__()
+    same_address = False
+    for _ in range(64):
+        m2 = K.Model(P)
+        if m2._h == old:
+            same_address = True
+            break
+    print(f"recreated at the freed address: {same_address}", flush=True)
+    dp = np.zeros(P.shape, order="F")
+    with pytest.raises(ValueError, match="model of its forward"):
+        K.gradient_reverse(cache, m2, dp, np.zeros((1, 1, cache.npix, 4)))
+
+
 def test_closure_seed_vs_oracle(oracle):
     """frame_chi2_closures: closure χ² of a frame's device visibilities and its seed through vis_seed (the
     adjoint transform of dχ²/dV on the device), added to a visibility χ² seed of the same frame; the
