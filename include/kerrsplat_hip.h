@@ -172,6 +172,19 @@ int ks_gradient_reverse(ks_cache* cache, const ks_model* model, const double* ds
 int ks_visibilities(ks_cache* cache, int64_t frames, double psize, const double* u, const double* v,
                     const double* taper, int nuv, double* V);
 
+/* No counterpart in the reference: ks_visibilities for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). Same arguments, result and errors as
+ * ks_visibilities, and it calls no hipMalloc, hipFree or other allocating routine: its workspace is one fixed
+ * device array of KS_NOALLOC_SCRATCH_BYTES per GPU inside the library, held until the process ends. It needs
+ * (3 + 2 n_alpha + 2 n_beta) nuv + 4 n_alpha n_beta frames + 8 n_beta nuv frames + 8 nuv frames doubles (each of
+ * the 8 parts rounded up to 256 bytes) and fails with KS_ERR_DIM above the limit (a 128 x 128 screen at 4096
+ * (u, v) points and one frame needs 49 MiB). Every call on one GPU shares that array and keeps the last call's
+ * contents: calls that overlap on one GPU (from several threads, or from several contexts of the same device)
+ * race. */
+#define KS_NOALLOC_SCRATCH_BYTES (64LL << 20)
+int ks_visibilities_noalloc(ks_cache* cache, int64_t frames, double psize, const double* u, const double* v,
+                            const double* taper, int nuv, double* V);
+
 /* frame_chi2_seed!(seed, c, images, fs, cache, Δα, L, D) - device_visibilities.jl:97-111: the χ² of
  * frame `frame` (it + nt*inu) of the last ks_gradient_forward against visibility data,
  * Σ |(g V - d) / σ|² over baselines and Stokes parameters, returned in chi2, and the adjoint seed

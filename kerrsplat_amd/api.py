@@ -76,6 +76,7 @@ def _load():
     lib.ks_thin_image.argtypes = [_P, _P, _d, _P, _i]
     lib.ks_ray_lists.argtypes = [_P, _P, _P, _i, _i, _d, _i, _P, ctypes.POINTER(_i), _P]
     lib.ks_visibilities.argtypes = [_P, _i64, _d, _P, _P, _P, _i, _P]
+    lib.ks_visibilities_noalloc.argtypes = [_P, _i64, _d, _P, _P, _P, _i, _P]
     lib.ks_timing_end.argtypes = [_P, ctypes.POINTER(ctypes.c_float)]
     lib.ks_stage_ms.argtypes = [_P, _i, ctypes.POINTER(ctypes.c_float)]
     lib.ks_polarized_gradient.argtypes = [_P, _P, _P, _i, _P, _i, _d, _i, _d, _P, _i, _P, _P]
@@ -649,13 +650,15 @@ def polarized_gradient(cache, params, times, nus, L, dstokes, dparams=None, nmax
     return dparams, np.transpose(img, (3, 2, 1, 0, 4))
 
 
-def visibilities(cache, frames, psize, u, v, taper=None):
+def visibilities(cache, frames, psize, u, v, taper=None, noalloc=False):
     """Visibilities of the rendered frames (visibilities.jl:21-40, device_visibilities.jl:61-73):
-    complex (frames, nuv, 4) in Jy for pixel size psize = Δα L / D [rad]."""
+    complex (frames, nuv, 4) in Jy for pixel size psize = Δα L / D [rad]. noalloc takes the entry point
+    that allocates no device memory (its workspace is the library's fixed per-GPU array)."""
     uu, vv = _f64(u), _f64(v)
     tp = _f64(taper) if taper is not None else None
     V = np.zeros((frames, len(uu), 4, 2))
-    _check(cache._lib.ks_visibilities(cache._h, _i64(frames), _d(psize), _ptr(uu), _ptr(vv), _ptr(tp), _i(len(uu)), _ptr(V)))
+    fn = cache._lib.ks_visibilities_noalloc if noalloc else cache._lib.ks_visibilities
+    _check(fn(cache._h, _i64(frames), _d(psize), _ptr(uu), _ptr(vv), _ptr(tp), _i(len(uu)), _ptr(V)))
     return V[..., 0] + 1j * V[..., 1]
 
 

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <atomic>
+#include <mutex>
 
 #include "ks_kernels.hpp"
 #include "../../include/kerrsplat_hip.h"
@@ -60,8 +61,34 @@ template <class T> struct Buf {
 
 }  // namespace
 
+// The workspace of ks_visibilities_noalloc: one fixed array per GPU, held until the process ends. In host code
+// `scratch` names a host-side stand-in, not device memory: its device address on each GPU is taken once, with
+// that GPU current, by hipGetSymbolAddress (scratch_address) and only that address is handed to kernels and copies.
+static __device__ __align__(256) unsigned char scratch[KS_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int KS_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[KS_MAX_DEVICES] = {};
+
+// the device address of `scratch` on `device` (which must be current), recorded on first use
+int scratch_address(int device, void** out) {
+    if (device < 0 || device >= KS_MAX_DEVICES) return fail(KS_ERR_ARG, "device %d beyond the %d the library tracks", device, KS_MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return fail(KS_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return KS_OK;
+}
+}  // namespace
+
 struct ks_ctx {
     int device = 0;
+    void* scratch_dev = nullptr;                       // device address of `scratch` on this device
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     static constexpr int NSTAGE = 5;
@@ -151,8 +178,11 @@ int ks_ctx_create(int device, ks_ctx** out) {
     HIPCHK(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) return fail(KS_ERR_ARG, "device %d outside 0:%d", device, n - 1);
     HIPCHK(hipSetDevice(device));
+    void* scr = nullptr;
+    if (int rc = scratch_address(device, &scr)) return rc;
     ks_ctx* c = new ks_ctx;
     c->device = device;
+    c->scratch_dev = scr;
     HIPCHK(hipStreamCreate(&c->stream));
     HIPCHK(hipEventCreate(&c->ev0));
     HIPCHK(hipEventCreate(&c->ev1));
@@ -929,31 +959,50 @@ int ks_ray_lists(ks_cache* c, const ks_model* m, const double* times, int nt, in
 }
 
 // ---- visibilities and χ² (visibilities.jl:21-40, device_visibilities.jl:53-111, Fit.jl:537-577) ----
+// device workspace of the visibility transform: (u, v, taper), the phase tables A2 (2 nuv x n_alpha) and
+// B2 (2 nuv x n_beta), the frames as n_alpha x 4 n_beta matrices, the partial products C2 (2 nuv x 4 n_beta per
+// frame) and V (nframes x nuv x 4 complex)
+struct VisWs { double *u, *v, *t, *A2, *B2, *Im, *C2, *V; };
+
 // host (u, v, taper) to the device and the phase tables of the cache's screen
-static int vis_setup(ks_cache* c, double psize, const double* u, const double* v, const double* taper, int nuv) {
+static int vis_tables_ws(ks_cache* c, const VisWs& w, double psize, const double* u, const double* v, const double* taper, int nuv) {
     ks_ctx* x = c->ctx;
-    const int na = c->nalpha, nb = c->nbeta;
-    if (c->vu.grow(nuv) || c->vv.grow(nuv) || c->vt.grow(nuv) || c->vA2.grow((size_t)2 * nuv * na) || c->vB2.grow((size_t)2 * nuv * nb))
-        return KS_ERR_HIP;
-    HIPCHK(hipMemcpyAsync(c->vu.p, u, nuv * sizeof(double), hipMemcpyHostToDevice, x->stream));
-    HIPCHK(hipMemcpyAsync(c->vv.p, v, nuv * sizeof(double), hipMemcpyHostToDevice, x->stream));
-    if (taper) HIPCHK(hipMemcpyAsync(c->vt.p, taper, nuv * sizeof(double), hipMemcpyHostToDevice, x->stream));
-    HIPCHK(launch_vis_tables(x->stream, nuv, na, nb, psize, c->vu.p, c->vv.p, c->vA2.p, c->vB2.p));
+    HIPCHK(hipMemcpyAsync(w.u, u, nuv * sizeof(double), hipMemcpyHostToDevice, x->stream));
+    HIPCHK(hipMemcpyAsync(w.v, v, nuv * sizeof(double), hipMemcpyHostToDevice, x->stream));
+    if (taper) HIPCHK(hipMemcpyAsync(w.t, taper, nuv * sizeof(double), hipMemcpyHostToDevice, x->stream));
+    HIPCHK(launch_vis_tables(x->stream, nuv, c->nalpha, c->nbeta, psize, w.u, w.v, w.A2, w.B2));
     return KS_OK;
 }
 
-// V of frames [f0, f0 + nf) of the images in the cache into c->vV (nf x nuv x 4 complex)
-static int vis_forward(ks_cache* c, int64_t f0, int nf, int nuv, const double* d_tap, double scale) {
+// V of frames [f0, f0 + nf) of the images in the cache into w.V
+static int vis_forward_ws(ks_cache* c, const VisWs& w, int64_t f0, int nf, int nuv, const double* d_tap, double scale) {
     ks_ctx* x = c->ctx;
     const int na = c->nalpha, nb = c->nbeta;
     const int64_t mat = (int64_t)na * 4 * nb, cmat = (int64_t)2 * nuv * 4 * nb;
-    if (c->vIm.grow((size_t)mat * nf) || c->vC2.grow((size_t)cmat * nf) || c->vV.grow((size_t)8 * nuv * nf)) return KS_ERR_HIP;
-    HIPCHK(launch_vis_gather(x->stream, c->out, c->perm, c->npix, na, nb, f0, nf, c->vIm.p));
+    HIPCHK(launch_vis_gather(x->stream, c->out, c->perm, c->npix, na, nb, f0, nf, w.Im));
     HIPCHK(x->stage_begin(KS_STAGE_VIS));
-    HIPCHK(launch_dgemm(x->stream, false, 2 * nuv, 4 * nb, na, c->vA2.p, na, 0, c->vIm.p, 4 * nb, mat, c->vC2.p, 4 * nb, cmat, nf));
+    HIPCHK(launch_dgemm(x->stream, false, 2 * nuv, 4 * nb, na, w.A2, na, 0, w.Im, 4 * nb, mat, w.C2, 4 * nb, cmat, nf));
     HIPCHK(x->stage_end(KS_STAGE_VIS));
-    HIPCHK(launch_vis_combine(x->stream, nuv, nb, nf, c->vC2.p, c->vB2.p, d_tap, scale, c->vV.p));
+    HIPCHK(launch_vis_combine(x->stream, nuv, nb, nf, w.C2, w.B2, d_tap, scale, w.V));
     return KS_OK;
+}
+
+static VisWs cache_ws(const ks_cache* c) { return VisWs{c->vu.p, c->vv.p, c->vt.p, c->vA2.p, c->vB2.p, c->vIm.p, c->vC2.p, c->vV.p}; }
+
+// the phase tables in the cache's grow-only workspace
+static int vis_setup(ks_cache* c, double psize, const double* u, const double* v, const double* taper, int nuv) {
+    const int na = c->nalpha, nb = c->nbeta;
+    if (c->vu.grow(nuv) || c->vv.grow(nuv) || c->vt.grow(nuv) || c->vA2.grow((size_t)2 * nuv * na) || c->vB2.grow((size_t)2 * nuv * nb))
+        return KS_ERR_HIP;
+    return vis_tables_ws(c, cache_ws(c), psize, u, v, taper, nuv);
+}
+
+// V of frames [f0, f0 + nf) into c->vV
+static int vis_forward(ks_cache* c, int64_t f0, int nf, int nuv, const double* d_tap, double scale) {
+    const int na = c->nalpha, nb = c->nbeta;
+    const int64_t mat = (int64_t)na * 4 * nb, cmat = (int64_t)2 * nuv * 4 * nb;
+    if (c->vIm.grow((size_t)mat * nf) || c->vC2.grow((size_t)cmat * nf) || c->vV.grow((size_t)8 * nuv * nf)) return KS_ERR_HIP;
+    return vis_forward_ws(c, cache_ws(c), f0, nf, nuv, d_tap, scale);
 }
 
 static int screen_ready(const ks_cache* c) {
@@ -973,6 +1022,38 @@ int ks_visibilities(ks_cache* c, int64_t frames, double psize, const double* u, 
     const double scale = psize * psize / JY;
     if (int rc = vis_forward(c, 0, (int)frames, nuv, taper ? c->vt.p : nullptr, scale)) return rc;
     HIPCHK(hipMemcpyAsync(V, c->vV.p, (size_t)8 * nuv * frames * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    return KS_OK;
+}
+
+// ks_visibilities without allocating: the same transform with its workspace carved from the device array
+// `scratch` (KS_NOALLOC_SCRATCH_BYTES) at the address recorded for the context's device; each part starts on a
+// 256-byte boundary. Needs nothing cleared beforehand: every kernel of the transform writes its whole output.
+int ks_visibilities_noalloc(ks_cache* c, int64_t frames, double psize, const double* u, const double* v, const double* taper, int nuv, double* V) {
+    if (!c || !u || !v || !V) return fail(KS_ERR_ARG, "ks_visibilities_noalloc: null argument");
+    if (frames > c->out_frames || frames < 1) return fail(KS_ERR_DIM, "only %lld frames rendered", (long long)c->out_frames);
+    if (nuv < 1) return fail(KS_ERR_ARG, "need at least one (u, v) point");
+    if (int rc = screen_ready(c)) return rc;
+    ks_ctx* x = c->ctx;
+    char* base = (char*)x->scratch_dev;
+    if (!base) return fail(KS_ERR_STATE, "the context's device has no scratch address");
+    const int na = c->nalpha, nb = c->nbeta;
+    const int64_t parts[8] = {nuv, nuv, nuv, (int64_t)2 * nuv * na, (int64_t)2 * nuv * nb, (int64_t)na * 4 * nb * frames,
+                              (int64_t)2 * nuv * 4 * nb * frames, (int64_t)8 * nuv * frames};
+    double* ptr[8];
+    int64_t off = 0;
+    for (int k = 0; k < 8; ++k) {
+        ptr[k] = (double*)(base + off);
+        off += (parts[k] * (int64_t)sizeof(double) + 255) / 256 * 256;
+    }
+    if (off > (int64_t)KS_NOALLOC_SCRATCH_BYTES)
+        return fail(KS_ERR_DIM, "ks_visibilities_noalloc: %lld frames of a %d x %d screen at %d (u, v) points need %lld bytes of workspace, "
+                    "above the fixed %lld", (long long)frames, na, nb, nuv, (long long)off, (long long)KS_NOALLOC_SCRATCH_BYTES);
+    const VisWs w{ptr[0], ptr[1], ptr[2], ptr[3], ptr[4], ptr[5], ptr[6], ptr[7]};
+    HIPCHK(hipSetDevice(x->device));
+    if (int rc = vis_tables_ws(c, w, psize, u, v, taper, nuv)) return rc;
+    if (int rc = vis_forward_ws(c, w, 0, (int)frames, nuv, taper ? w.t : nullptr, psize * psize / JY)) return rc;
+    HIPCHK(hipMemcpyAsync(V, w.V, (size_t)8 * nuv * frames * sizeof(double), hipMemcpyDeviceToHost, x->stream));
     HIPCHK(hipStreamSynchronize(x->stream));
     return KS_OK;
 }
