@@ -318,3 +318,38 @@ def polarized_gradient(a, thetao, alphas, betas, N, params, t_obs, nu, L, dstoke
                                  _d(t_obs), _d(nu), _d(L), _i(nmax), _d(slab), _p(ds), g.ctypes.data_as(_P), _p(img),
                                  _i(int(analytic_weight)))
     return g, img
+
+
+# ---- test-only fluid models (ks_fluid.hpp): end-to-end pins of the transport -------------------------------
+def riaf_image(a, thetao, alphas, betas, N, nu, L, pandya=True, M=64, marcher=1):
+    """Observed Stokes (npix, 4) of ipole's example RIAF (the reference's test/riaf_model.jl)."""
+    al, be = _f64(alphas), _f64(betas)
+    out = np.zeros((len(al), 4))
+    lib().kso_riaf_image(_d(a), _d(thetao), _i(len(al)), _p(al), _p(be), _i(N), _i(M), _i(marcher), _d(nu), _d(L),
+                         _i(int(pandya)), _p(out))
+    return out
+
+
+def slab_image(a, thetao, alphas, betas, N, nu, L, slabs, M=64, marcher=1):
+    """Observed Stokes (npix, 4) of shells with given fluid-frame coefficients: slabs is a list of
+    (r1, r2, (jI, jQ, jV, aI, aQ, aV, rQ, rV), u[3], B[3])."""
+    al, be = _f64(alphas), _f64(betas)
+    rows = [np.concatenate([[r1, r2], c, u, B]) for (r1, r2, c, u, B) in slabs]
+    sl = _f64(np.array(rows).reshape(-1, 16))
+    out = np.zeros((len(al), 4))
+    lib().kso_slab_image(_d(a), _d(thetao), _i(len(al)), _p(al), _p(be), _i(N), _i(M), _i(marcher), _d(nu), _d(L),
+                         _i(len(rows)), _p(sl), _p(out))
+    return out
+
+
+def gold_image(model, thetao, alphas, betas, N, nu, L, polarized=False, M=64, marcher=1):
+    """Gold et al. (2020) model `model` (1..5): (spin, observed intensity (npix,)), or with polarized the
+    Stokes (npix, 4) of the same model through the polarized transport."""
+    al, be = _f64(alphas), _f64(betas)
+    out = np.zeros((len(al), 4)) if polarized else np.zeros(len(al))
+    spin = _d(0.0)
+    rc = lib().kso_gold_image(_i(model - 1), _d(thetao), _i(len(al)), _p(al), _p(be), _i(N), _i(M), _i(marcher), _d(nu), _d(L),
+                              _i(int(polarized)), _p(out), ctypes.byref(spin))
+    if rc:
+        raise ValueError(f"no Gold model {model}")
+    return spin.value, out

@@ -7,6 +7,7 @@
 #include "ks_splats.hpp"
 #include "ks_vis.hpp"
 #include "ks_adjoint.hpp"
+#include "ks_fluid.hpp"
 #include <cstring>
 #include <vector>
 #include <numeric>
@@ -72,6 +73,41 @@ template <class T, class R = double> void polarized_ray(const Model& m, R a, R t
     for (int q = 0; q < 4; ++q) out[q] = st.S[q] * nh * nh * nh;
     if (rt) *rt = r1;
     if (rp) *rp = r2;
+}
+
+// the polarized transport of one ray through a fluid model whose elements at a sample are visited by
+// elems(rc, s, add) (transport.jl:263-290 with nelements/element of the model): the same screen coefficients,
+// step and front-to-back state as the splat models
+template <class Elems> void fluid_ray(Elems&& elems, double a, double thetao, double alpha, double beta, int N, int M, int marcher,
+                                      double nu, double L, double out[4]) {
+    const Pixel<double> p = pixel_constants<double>(a, alpha, beta, thetao);
+    const RayCtx rc{a, alpha, beta, thetao, p.eta, p.lam};
+    RadState<double> st = radstate_zero<double>();
+    const double hor = (1.0 + std::sqrt(1.0 - a * a)) * (1.0 + 1e-3);
+    auto f = [&](int, const Sample<double>& s, double dtau) {
+        if (!(s.ok && s.r > hor)) return;
+        double j4[4] = {0, 0, 0, 0}, a4[4] = {0, 0, 0, 0}, r3[3] = {0, 0, 0};
+        bool active = false;
+        elems(rc, s, [&](const Coef<double>& cf, const Frame<double>& fr) {
+            Screen<double> sc;
+            if (!screen_coefficients(cf, fr, nu, sc)) return;
+            for (int q = 0; q < 4; ++q) { j4[q] += sc.j[q]; a4[q] += sc.a[q]; }
+            for (int q = 0; q < 3; ++q) r3[q] += sc.r[q];
+            active = true;
+        });
+        if (!active) return;
+        const double c = std::cos(s.th);
+        const double D = L / (nu / NU0) * (s.r * s.r + a * a * c * c) * dtau;
+        M4<double> O, Ej;
+        step_operator(a4, r3, D, O, Ej);
+        double E[4];
+        for (int i = 0; i < 4; ++i) E[i] = Ej.m[i][0] * j4[0] + Ej.m[i][1] * j4[1] + Ej.m[i][2] * j4[2] + Ej.m[i][3] * j4[3];
+        advance(st, O, E);
+    };
+    double r1, r2;
+    march(p, N, M, marcher, f, r1, r2);
+    const double nh = nu / NU0;
+    for (int q = 0; q < 4; ++q) out[q] = st.S[q] * nh * nh * nh;
 }
 }  // namespace
 
@@ -389,3 +425,81 @@ int kso_vis_seed(int nalpha, int nbeta, int nframes, double psize, int nuv, cons
 }
 
 }  // extern "C"
+
+extern "C" {
+// ipole's example RIAF (test/riaf_model.jl): observed Stokes (I, Q, U, V) per pixel, screen order, out npix x 4
+int kso_riaf_image(double a, double thetao, int npix, const double* alphas, const double* betas, int N, int M, int marcher, double nu,
+                   double L, int pandya, double* out) {
+    const RIAF m = riaf_example(a, pandya != 0);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int i = 0; i < npix; ++i)
+        fluid_ray([&](const RayCtx& rc, const Sample<double>& s, auto&& add) {
+            Coef<double> cf; Frame<double> fr;
+            if (riaf_element(m, rc, s, nu, cf, fr)) add(cf, fr);
+        }, a, thetao, alphas[i], betas[i], N, M, marcher, nu, L, out + 4 * (size_t)i);
+    return 0;
+}
+
+// shells with given fluid-frame coefficients (test_polarized.jl:33-51): slab k is slabs[16 k ..] =
+// (r1, r2, jI, jQ, jV, aI, aQ, aV, rQ, rV, u[3], B[3]); observed Stokes per pixel, out npix x 4
+int kso_slab_image(double a, double thetao, int npix, const double* alphas, const double* betas, int N, int M, int marcher, double nu,
+                   double L, int nslab, const double* slabs, double* out) {
+    std::vector<Slab> els((size_t)nslab);
+    for (int k = 0; k < nslab; ++k) {
+        const double* q = slabs + 16 * (size_t)k;
+        els[k] = Slab{q[0], q[1], {q[2], q[3], q[4], q[5], q[6], q[7], q[8], q[9]}, {q[10], q[11], q[12]}, {q[13], q[14], q[15]}};
+    }
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int i = 0; i < npix; ++i)
+        fluid_ray([&](const RayCtx& rc, const Sample<double>& s, auto&& add) {
+            for (const Slab& e : els) {
+                Coef<double> cf; Frame<double> fr;
+                if (slab_element(e, rc, s, cf, fr)) add(cf, fr);
+            }
+        }, a, thetao, alphas[i], betas[i], N, M, marcher, nu, L, out + 4 * (size_t)i);
+    return 0;
+}
+
+// Gold et al. (2020) model `model` (0..4) at its spin (test_gold2020.jl): the observed intensity per pixel
+// through the unpolarized transport (transport.jl:36-75), out npix; with polarized the same model through the
+// polarized transport as one unpolarized element (test_polarized.jl:24-32), out npix x 4. *spin receives a.
+int kso_gold_image(int model, double thetao, int npix, const double* alphas, const double* betas, int N, int M, int marcher, double nu,
+                   double L, int polarized, double* out, double* spin) {
+    if (model < 0 || model > 4) return 1;
+    double a; Gold gm;
+    gold_model(model, a, gm);
+    if (spin) *spin = a;
+    const double nh = nu / NU0;
+    const double hor = (1.0 + std::sqrt(1.0 - a * a)) * (1.0 + 1e-3);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int i = 0; i < npix; ++i) {
+        if (polarized) {
+            fluid_ray([&](const RayCtx& rc, const Sample<double>& s, auto&& add) {
+                double j, al, g;
+                gold_coefficients(gm, rc, s, nu, j, al, g);
+                add(Coef<double>{j, 0.0, 0.0, al, 0.0, 0.0, 0.0, 0.0}, Frame<double>{g, 0.0, 0.0});
+            }, a, thetao, alphas[i], betas[i], N, M, marcher, nu, L, out + 4 * (size_t)i);
+            continue;
+        }
+        const Pixel<double> p = pixel_constants<double>(a, alphas[i], betas[i], thetao);
+        const RayCtx rc{a, alphas[i], betas[i], thetao, p.eta, p.lam};
+        double tau = 0.0, I = 0.0;
+        auto f = [&](int, const Sample<double>& s, double dtau) {
+            if (!(s.ok && s.r > hor)) return;
+            double j, al, g;
+            gold_coefficients(gm, rc, s, nu, j, al, g);
+            const double c = std::cos(s.th);
+            const double D = L / nh * (s.r * s.r + a * a * c * c) * dtau;
+            const double ji = j * g * g / (nh * nh), ai = al * nh / g;
+            const double ti = ai * D;
+            I += std::exp(-tau) * ji * D * phi1(ti);
+            tau += ti;
+        };
+        double r1, r2;
+        march(p, N, M, marcher, f, r1, r2);
+        out[i] = I * nh * nh * nh;
+    }
+    return 0;
+}
+}  // extern "C"
+

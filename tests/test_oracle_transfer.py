@@ -7,8 +7,12 @@
 * the 4x4 step against exp(-K D) and its integral by a 50-digit Taylor series with scaling and
   doubling (the reference's method, test/test_transfer_step.jl:22-39), on seeded random physical
   cases and the reference's thirteen special cases (test_transfer_step.jl:72-84);
-* physical identities: Kirchhoff, V and rho_V odd under theta -> pi - theta, the polarization cap,
-  unit propagation direction and the boost inverse in the local frame.
+* power-law rho_Q, rho_V against symphony's numerical rotativities and kappa j_V against symphony's numerical
+  j_V, at the reference's looser gates for those fits;
+* physical identities: Kirchhoff, V and rho_V odd under theta -> pi - theta, the polarization cap, the
+  ZAMO redshift of a fluid at rest and the field-sign symmetry of the local frame. The frame of a moving
+  fluid (boost, pitch angle, Walker-Penrose angle) is pinned end to end by ipole's RIAF images
+  (tests/test_oracle_endtoend.py).
 """
 import csv
 import math
@@ -66,6 +70,43 @@ def test_powerlaw_vs_symphony(oracle):
         assert w < 1e-12, (n, w)
 
 
+
+
+def test_powerlaw_rotativities_vs_symphony(oracle):
+    """Power-law rho_Q, rho_V (Jones & O'Dell via Dexter 2016) against symphony's numerical susceptibility
+    integration (powerlaw_rho_table.csv) inside the closed forms' validity window nu > 3 nu_min, at the
+    reference's gate (test_coefficients.jl:124-153): every sign agrees, rho_V within 5%, rho_Q within 35%."""
+    t = read_table("powerlaw_rho_table.csv")
+    ME, CL, EE = 9.1093826e-28, 2.99792458e10, 4.80320680e-10
+    used = 0
+    worstV = worstQ = 0.0
+    for i in range(len(t["nu"])):
+        nu, B, th, p, gmin, gmax, ne = (t[k][i] for k in ("nu", "B", "theta", "p", "gamma_min", "gamma_max", "ne"))
+        if not (np.isfinite(t["rhoQ"][i]) and np.isfinite(t["rhoV"][i])):
+            continue
+        if not nu > 3 * gmin ** 2 * EE * B * math.sin(th) / (2 * math.pi * ME * CL):
+            continue
+        used += 1
+        c = oracle.powerlaw(ne, p, gmin, gmax, B, nu, th)
+        rQ, rV = c[6], c[7]
+        assert np.sign(rQ) == np.sign(t["rhoQ"][i]) and np.sign(rV) == np.sign(t["rhoV"][i]), i
+        worstV = max(worstV, abs(rV / t["rhoV"][i] - 1))
+        worstQ = max(worstQ, abs(rQ / t["rhoQ"][i] - 1))
+    assert used >= 10
+    assert worstV < 0.05 and worstQ < 0.35, (worstV, worstQ)
+
+
+def test_kappa_jv_vs_symphony_numerical(oracle):
+    """kappa j_V (the fit) against symphony's numerical evaluation (kappa_jv_table.csv), the reference's gate
+    (test_coefficients.jl:184-196): signs agree and the fit is within 50%."""
+    t = read_table("kappa_jv_table.csv")
+    worst = 0.0
+    for i in range(len(t["nu"])):
+        nu, kap, w, B, ne, th = (t[k][i] for k in ("nu", "kappa", "w", "B", "ne", "theta"))
+        jV = oracle.kappa(ne, kap, w, B, nu, th, oracle.kappa_hyp(kap, w))[2]
+        assert np.sign(jV) == np.sign(t["jV"][i]), i
+        worst = max(worst, abs(jV / t["jV"][i] - 1))
+    assert len(t["nu"]) >= 10 and worst < 0.5, worst
 
 
 def test_kappa_vs_symphony(oracle):
