@@ -27,7 +27,7 @@ enum { KS_OK = 0, KS_ERR_ARG = 1, KS_ERR_DIM = 2, KS_ERR_STATE = 3, KS_ERR_HIP =
 enum { KS_MARCHER_DIRECT = 0, KS_MARCHER_RECURRENCE = 1, KS_MARCHER_FUSED = 2 };
 
 /* splat models: ThinRenderer (Splats.jl:105), PolarizedSplats (polarized.jl:41), PowerLawSplats,
- * KappaSplats (populations.jl:34-60), KnotSplats (trajectories.jl:184) */
+ * KappaSplats (populations.jl:34-60), KnotSplats (trajectories.jl:80-88) */
 enum { KS_MODEL_THIN = 0, KS_MODEL_THERMAL = 1, KS_MODEL_POWERLAW = 2, KS_MODEL_KAPPA = 3, KS_MODEL_KNOT = 4, KS_MODEL_COMPOSITE = 5 };
 
 typedef struct ks_ctx ks_ctx;
@@ -91,7 +91,7 @@ int ks_march(ks_cache* cache);
 int ks_get_perm(const ks_cache* cache, int64_t* perm, int64_t counts[3]);
 /* cache.residual_t, cache.residual_ϕ (sorted order) - cache.jl:98-99, quadrature.jl:283-291. */
 int ks_get_residuals(const ks_cache* cache, double* resid_t, double* resid_phi);
-/* host(cache.consts) - Geodesics.jl:66-73 over PixelConstants (pixel_constants.jl:271-296):
+/* host(cache.consts) - Geodesics.jl:66-73 over PixelConstants (pixel_constants.jl:29-62):
  * ks_num_const_fields() doubles per pixel, field-major, sorted order (layout in DESIGN.md). */
 int ks_num_const_fields(void);
 int ks_copy_consts(const ks_cache* cache, double* soa);
@@ -100,7 +100,7 @@ int ks_copy_consts(const ks_cache* cache, double* soa);
 int ks_copy_samples(const ks_cache* cache, double* t, double* r, double* theta, double* phi, uint8_t* flags);
 
 /* A splat parameter set on the device: the model argument of ThinRenderer / RadiativeTransport
- * (Splats.jl:105-111, polarized.jl:41-51, populations.jl:34-60, trajectories.jl:184-192).
+ * (Splats.jl:105-111, polarized.jl:41-51, populations.jl:34-60, trajectories.jl:80-88).
  * params: nrows x ns column-major (14 thin, 21 thermal/knot, 22 power law/kappa);
  * gamma_max: power law; knots: 3 x nknots x ns, knot_times: nknots (knot splats). */
 int ks_model_create(ks_ctx* ctx, int kind, int nrows, int ns, const double* params, double gamma_max,
@@ -119,7 +119,7 @@ int ks_thin_image(ks_cache* cache, const ks_model* model, double t_obs, double* 
 /* polarized_image! / polarized_cube - polarized.jl:160-198: one fused launch over all
  * (pixel, time, frequency) items; out_stokes = observed (I, Q, U, V) in cgs,
  * 4 x npix x nt x nnu (frame index it + nt*inu), screen or sorted order. nmax >= 0 truncates
- * the rays after their nmax-th passage through |z| < slab (step.jl:469-517). */
+ * the rays after their nmax-th passage through |z| < slab (WindingState and wind, step.jl:362-398; transport.jl:140-146). */
 int ks_render(ks_cache* cache, const ks_model* model, const double* times, int nt, const double* nus, int nnu,
               double L, int nmax, double slab, double* out_stokes, int screen_order);
 /* The same launch with the result left on the device (no host copy, no synchronization). */

@@ -16,7 +16,7 @@ Every call goes to the GPU library; there is no CPU fallback. Loading fails loud
 is missing or no GPU is present.
 """
 from .api import (Camera, Direct, Recurrence, Fused, GeodesicCache, Model, CompositeModel, regenerate, thin_image, polarized_image,
-                  polarized_cube, polarized_gradient, march, precision, stage_ms, render_device, visibilities, unsort, to_screen, samples, residuals, KernelError,
+                  polarized_cube, polarized_gradient, march, precision, stage_ms, render_device, visibilities, unsort, to_screen, samples, residuals, consts, KernelError,
                   NSPLATPARAMS, NPOLARIZEDPARAMS, NPOWERLAWPARAMS, NKAPPAPARAMS, gravitational_radius, grid_axis,
                   library_path, gradient_forward, gradient_reverse, vis_chi2_seed, image_chi2_seed, vis_seed, tiles, Binning,
                   bin, binned_grid, binned_polar, concatenate, spacetime_tangent, RayLists, ray_lists, framed, flattened, frame_lists)
@@ -25,7 +25,7 @@ from .fit import (ScatteringKernel, taper, spacetime_valgrad, spacetime_jacobian
                   log_closure_amplitudes, ClosureData, closure_chi2, frame_chi2_closures, gain_factors, apply_gains)
 
 __all__ = ["Camera", "Direct", "Recurrence", "Fused", "GeodesicCache", "Model", "CompositeModel", "regenerate", "thin_image",
-           "polarized_image", "polarized_cube", "polarized_gradient", "march", "precision", "stage_ms", "render_device", "visibilities", "unsort", "to_screen", "samples",
+           "polarized_image", "polarized_cube", "polarized_gradient", "march", "precision", "stage_ms", "render_device", "visibilities", "unsort", "to_screen", "samples", "consts",
            "residuals", "KernelError", "NSPLATPARAMS", "NPOLARIZEDPARAMS", "NPOWERLAWPARAMS", "NKAPPAPARAMS",
            "gravitational_radius", "grid_axis", "library_path", "gradient_forward", "gradient_reverse", "vis_chi2_seed",
            "image_chi2_seed", "fit", "StokesMovie", "FrameScans", "chi2", "chi2_gradient", "sweep_passes", "frame_chi2_seed",

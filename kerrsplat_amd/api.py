@@ -395,6 +395,22 @@ def samples(cache):
     return {"t": t.T.copy(), "r": r.T.copy(), "theta": th.T.copy(), "phi": ph.T.copy(), "flags": fl.T.copy()}
 
 
+PIXEL_FIELDS = ("alpha", "beta", "lam", "eta", "case", "k", "c", "fo", "tau_total", "tau_h", "r1", "r3", "r4", "n", "n0", "n1",
+                "d0", "d1", "a2", "b1", "g0", "psi0", "Pp", "Pm", "Qp", "Qm", "It0", "Iph0", "up", "um", "f", "mth", "off", "sgn",
+                "npol", "Xth0", "Gt0", "Gphi0", "vortical", "mu", "scale", "K2", "nreal")
+
+
+def consts(cache):
+    """host(cache.consts) (Geodesics.jl:66-73 over PixelConstants, pixel_constants.jl:29-62): the per-pixel
+    constants in the cache's sorted order as a dict of (npix,) arrays named as in PIXEL_FIELDS."""
+    nf = cache._lib.ks_num_const_fields()
+    if nf != len(PIXEL_FIELDS):
+        raise KernelError(f"the library holds {nf} constants per pixel, this module names {len(PIXEL_FIELDS)}")
+    soa = np.zeros((nf, cache.npix))
+    _check(cache._lib.ks_copy_consts(cache._h, _ptr(soa)))
+    return {name: soa[i].copy() for i, name in enumerate(PIXEL_FIELDS)}
+
+
 def residuals(cache):
     rt, rp = np.zeros(cache.npix), np.zeros(cache.npix)
     _check(cache._lib.ks_get_residuals(cache._h, _ptr(rt), _ptr(rp)))
@@ -403,7 +419,7 @@ def residuals(cache):
 
 class Model:
     """A splat parameter matrix on the device: ThinRenderer / PolarizedSplats / PowerLawSplats /
-    KappaSplats / KnotSplats (Splats.jl:105, polarized.jl:41, populations.jl:34-60, trajectories.jl:184)."""
+    KappaSplats / KnotSplats (Splats.jl:105, polarized.jl:41, populations.jl:34-60, trajectories.jl:80-88)."""
 
     def __init__(self, params, kind="thermal", gamma_max=1e5, knots=None, knot_times=None, device=0):
         if kind not in MODEL_KIND:

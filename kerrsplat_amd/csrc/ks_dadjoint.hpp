@@ -1,6 +1,6 @@
 // Device pieces of the dual-sweep adjoint (thermal splats).
 //
-// Reference: sample_adjoint (src/Transfer/transport.jl:332-341) and element_adjoint! / ParamColumn
+// Reference: sample_adjoint (src/Transfer/transport.jl:198-207) and element_adjoint! / ParamColumn
 // (src/Splats/dual_gradient.jl:22-76). Along a ray, l = w_1 . T_1 with T_k = E_k + O_k T_{k+1}; the
 // adjoints of sample k are dl/dO_k = w_k T_{k+1}^T and dl/dE_k = w_k, w_{k+1} = O_k^T w_k.
 //   * sample_adjoint: j_bar = Ej^T w exactly; alpha_bar, rho_bar through the step's polynomial

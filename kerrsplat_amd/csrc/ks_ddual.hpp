@@ -5,7 +5,7 @@
 //                  operation rounds to float and runs on the FP32 units and hardware transcendentals);
 //   * DN<N, S>   - forward-mode duals (a value and N partials of scalar S) for the per-sample local
 //                  derivatives of the adjoint, as the reference obtains them with ForwardDiff duals
-//                  (src/Splats/dual_gradient.jl:1-19, src/Transfer/transport.jl:332-341).
+//                  (src/Splats/dual_gradient.jl:1-19, src/Transfer/transport.jl:198-207).
 #pragma once
 #include <type_traits>
 #include "ks_dmath.hpp"

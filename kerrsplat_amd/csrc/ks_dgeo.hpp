@@ -266,7 +266,7 @@ __constant__ double GL16_W[16] = {0.0271524594117541, 0.0622535239386479, 0.0951
                                   0.1246289712555339, 0.0951585116824928, 0.0622535239386479, 0.0271524594117541};
 
 // K1: the constants of one pixel (the reference's Krang.SlowLightIntensityPixel + radial_marcher +
-// PolarMarcher + QuadratureConstants, pixel_constants.jl:121-157, recurrence.jl:273-446)
+// PolarMarcher + QuadratureConstants, pixel_constants.jl:121-157, recurrence.jl:195-308, quadrature.jl:42-108)
 template <class R>
 KS_DEV PixCT<R> pixel_constants(const SpinT<R>& S, R thetao, R alpha, R beta, const double* glx, const double* glw) {
     PixCT<R> p;
@@ -442,7 +442,7 @@ KS_DEV VisCT<R> vis_constants(const PixCT<R>& p, R a, R thetao) {
     R c = cos(thetao);
     return {a, fabs(p.alpha), p.alpha * p.alpha, fabs(p.beta) + EPS, fabs(c), c * c};
 }
-// Krang's screen-visibility rule (recurrence.jl:508-517)
+// Krang's screen-visibility rule (recurrence.jl:273-308)
 template <class R>
 KS_DEV bool krang_visible(const VisCT<R>& v, R cth) {
     if (!(cth > v.abscos)) return true;

@@ -213,7 +213,7 @@ template <class T> KS_DEV T planck(T nu, T Th) {
     return pref / expm1(x);
 }
 
-// thermal (coefficients.jl:185-229) with the splat's Bessel ratios k0/k2, k1/k2 (k2 > 0 in practice);
+// thermal (coefficients.jl:101-151) with the splat's Bessel ratios k0/k2, k1/k2 (k2 > 0 in practice);
 // ne enters linearly and stays a plain number in the adjoint
 template <class T, class NE>
 KS_DEV CoefT<T> thermal(NE ne, T Th, T k0k2, T k1k2, T B, T nu, T sth, T cth) {
@@ -250,7 +250,7 @@ KS_DEV CoefT<T> thermal(NE ne, T Th, T k0k2, T k1k2, T B, T nu, T sth, T cth) {
     return c;
 }
 
-// power law (coefficients.jl:292-340); gPL = {Gamma((3p-1)/12) Gamma((3p+19)/12), Gamma((3p+2)/12) Gamma((3p+22)/12)}
+// power law (coefficients.jl:208-256); gPL = {Gamma((3p-1)/12) Gamma((3p+19)/12), Gamma((3p+2)/12) Gamma((3p+22)/12)}
 template <class T>
 KS_DEV CoefT<T> powerlaw(T ne, T p, T gmin, T gmax, const T* gPL, T B, T nu, T sth, T cth) {
     CoefT<T> c;

@@ -29,7 +29,7 @@ KS_DEV TransCtxT<S> make_ctx(double a, double rp, double rm, double thetao_sin, 
 }
 
 // A knot splat's Gaussian exponent at (t, x, y, z): its centre interpolated linearly between its knots
-// (trajectories.jl:197-233), held at the first and last knot outside their times; no pattern rotation.
+// (trajectories.jl:93-128), held at the first and last knot outside their times; no pattern rotation.
 template <class S>
 KS_DEV S knot_gexp(const TransCtxT<S>& c, const SplatCT<S>& sc, S t, S x, S y, S z) {
     const S* K = c.knots + 3 * (int64_t)sc.knot0;

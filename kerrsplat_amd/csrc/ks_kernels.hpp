@@ -148,7 +148,7 @@ struct SegMapT {
 };
 using SegMap = SegMapT<double>;
 
-// The half-orbit counter (WindingState, step.jl:498-517) over the valid samples outside the horizon
+// The half-orbit counter (WindingState, step.jl:362-398) over the valid samples outside the horizon
 // guard; kstop = the first sample beyond the nmax-th passage (N + 1 when there is none).
 struct WindF {
     double rp, slab; int nmax, kstop;

@@ -1,7 +1,7 @@
 // Device Recurrence(M) marcher: the fused march of one ray with a consumer folded over its samples.
 //
 // Reference: march_ray (src/Geodesics/quadrature.jl:231-296) and the pieces it uses
-// (recurrence.jl:24-517, quadrature.jl:42-215, jacobi.jl:146-209). Per sample: two addition-theorem
+// (recurrence.jl:24-308, quadrature.jl:42-296, jacobi.jl:8-71). Per sample: two addition-theorem
 // steps of the radial and polar Jacobi states to the midpoint and the end of the interval, the
 // radius and polar angle from the states, Simpson on the desingularised Mino-time rates of t~ and
 // phi plus their closed-form singular pieces (large r, both horizon poles, the polar-axis spike),

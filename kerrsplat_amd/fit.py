@@ -327,7 +327,7 @@ def spacetime_jacobian(x, camera, N, params, movie, frames=None, freqs=None, nma
     return r, J
 
 
-# ---- the diffractive scattering kernel of Sgr A* (scattering.jl:1-40) ------------------------------------
+# ---- the diffractive scattering kernel of Sgr A* (scattering.jl:1-38) ------------------------------------
 class ScatteringKernel:
     """ScatteringKernel(ν; fwhm_maj = 1.380, fwhm_min = 0.703, pa = 81.9) (scattering.jl:5-34): the
     ensemble-average scattering of Sgr A* at frequency ν [Hz] as a Gaussian in the visibility domain, FWHM

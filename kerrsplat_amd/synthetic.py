@@ -25,7 +25,7 @@ def seed(cfg):
 
 
 def quasi_cartesian_ks(a, r, th, ph):
-    """BL -> quasi-Cartesian Kerr-Schild (coordinates.jl:389-397)."""
+    """BL -> quasi-Cartesian Kerr-Schild (coordinates.jl:22-31)."""
     temp = math.sqrt(1.0 - a * a)
     rp, rm = 1.0 + temp, 1.0 - temp
     eps = np.finfo(float).eps

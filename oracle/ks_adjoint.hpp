@@ -1,7 +1,7 @@
 // The dual sweep: the gradient of a linear functional of the polarized image with respect to the
 // thermal splat parameters, for the CPU oracle (test infrastructure; see ks_scalar.hpp).
 //
-// Restates src/Splats/dual_gradient.jl:1-249 and sample_adjoint (transport.jl:332-341). Along a ray
+// Restates src/Splats/dual_gradient.jl:1-249 and sample_adjoint (transport.jl:198-207). Along a ray
 // the invariant Stokes vector arriving at the observer is T_1 with T_{N+1} = 0,
 // T_k = E_k + O_k T_{k+1}; for l = w_1 . T_1 the adjoints of sample k are dl/dO_k = w_k T_{k+1}^T,
 // dl/dE_k = w_k with w_{k+1} = O_k^T w_k. Per sample: j_bar = Ej^T w exactly; alpha_bar, rho_bar by
@@ -16,7 +16,7 @@
 
 namespace kso {
 
-// sample_adjoint (transport.jl:332-341); S = double, or Counted to measure the work. The 7 partials of
+// sample_adjoint (transport.jl:198-207); S = double, or Counted to measure the work. The 7 partials of
 // q = w . (O R + Ej j) with respect to (alpha, rho) come from duals through the step's polynomial
 // coefficients, contracted as sum_n o_n w.(K'^n R) + Dp e_n w.(K'^n j) (the same function as the
 // ForwardDiff derivative of the assembled matrices, without forming 4x4 dual matrices). Returns

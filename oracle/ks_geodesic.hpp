@@ -21,7 +21,7 @@
 //     (r_o + 2 ln r_o)], phi(tau) = integral_0^tau dphi/dtau (phi = 0 at the observer); both grow
 //     with tau by the Mino-time rates, as the reference's quadrature does (quadrature.jl:276-281);
 //   * flags: nu_r = tau < fo, nu_theta = sign of p^theta along the photon, ok = inside the radial
-//     validity and Krang's screen-visibility rule (recurrence.jl:388-517).
+//     validity and Krang's screen-visibility rule (recurrence.jl:150-186, 273-308).
 //
 // The t~ and phi closed forms: with the Mino-time rates
 //   dt/dtau = r^2 + 2r + 4 + P+/(r - r+) + P-/(r - r-) + a^2 cos^2(theta),
@@ -210,7 +210,7 @@ template <class T> T radial_X_of_r(const Pixel<T>& p, T r) {
     return ellF(p.psi0 + katan((r + p.b1) / p.a2), p.k);
 }
 
-// radius from the radial argument (closed forms; recurrence.jl:378-386 for the cn^2 form)
+// radius from the radial argument (closed forms; recurrence.jl:160-172 for the cn^2 form)
 template <class T> T radius_direct(const Pixel<T>& p, T X) {
     Jac<T> j = jacobi(X, p.k);
     if (p.rcase == 2) {
@@ -395,7 +395,7 @@ template <class T> VisConst<T> vis_constants(const Pixel<T>& p) {
     T c = kcos(p.thetao);
     return {p.a, kabs(p.alpha), p.alpha * p.alpha, kabs(p.beta) + KS_EPS, kabs(c), c * c};
 }
-// Krang's screen-visibility rule (recurrence.jl:508-517)
+// Krang's screen-visibility rule (recurrence.jl:273-308)
 template <class T> bool krang_visible(const VisConst<T>& v, T cth) {
     if (!(cth > v.abscos)) return true;
     T c2 = cth * cth, s2 = 1.0 - c2;
@@ -468,7 +468,7 @@ template <class T> Sample<T> direct_sample(const Pixel<T>& p, T tau) {
 // ---- the Recurrence(M) marcher (src/Geodesics/recurrence.jl, quadrature.jl) -----------------------------
 template <class T> struct JState { T sn, cn, dn; };
 template <class T> JState<T> jstate(T u, T m) { Jac<T> j = jacobi(u, m); return {j.sn, j.cn, j.dn}; }
-// addition theorem (jacobi.jl:177-183)
+// addition theorem (jacobi.jl:39-51)
 template <class T> JState<T> jstep(const JState<T>& x, const JState<T>& d, T m) {
     T den = 1.0 - m * (x.sn * x.sn) * (d.sn * d.sn);
     return {(x.sn * d.cn * d.dn + d.sn * x.cn * x.dn) / den,
@@ -483,7 +483,7 @@ template <class T> JState<T> jstep_constants(T delta, T m) {
 template <class T> struct Marcher {
     const Pixel<T>* p;
     T k;                       // radial parameter
-    // polar, transformed parameter (negative_parameter_transform, jacobi.jl:207-209)
+    // polar, transformed parameter (negative_parameter_transform, jacobi.jl:69-71)
     T mu, scale, tempfac, offset, sqrt_up, um, up_m_um, sgn;
     bool vortical;
     T tau_valid;
@@ -699,7 +699,7 @@ inline double grid_axis(double lo, double hi, int i, int n) {
 }
 
 // ---- coordinates --------------------------------------------------------------------------------------
-// BL -> quasi-Cartesian Kerr-Schild (coordinates.jl:389-397)
+// BL -> quasi-Cartesian Kerr-Schild (coordinates.jl:22-31)
 template <class T> void quasi_cartesian_ks(T a, T r, T th, T ph, T& x, T& y, T& z) {
     T temp = ksqrt(1.0 - a * a);
     T rp = 1.0 + temp, rm = 1.0 - temp;

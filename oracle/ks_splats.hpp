@@ -9,7 +9,7 @@
 //                                                                          (polarized.jl:17-19)
 //   power law (22 rows):  ... logne p loggmin logB thB phB u1 u2 u3 omega (populations.jl:13-14)
 //   kappa (22 rows):      ... logne kappa logw_kappa logB thB phB u1 u2 u3 omega (populations.jl:24-25)
-//   knots: thermal rows, centre interpolated from 3 x nknots x ns knots    (trajectories.jl:184-233)
+//   knots: thermal rows, centre interpolated from 3 x nknots x ns knots    (trajectories.jl:80-128)
 #pragma once
 #include "ks_geodesic.hpp"
 #include "ks_transfer.hpp"
@@ -75,7 +75,7 @@ template <class T> bool outside_support(const Model& m, int i, T t, T x, T y, T 
     double smax = std::max(std::max(m.P(3, i), m.P(4, i)), m.P(5, i));
     return d2 > support_radius2() * std::exp(2.0 * smax);
 }
-// knot centre (trajectories.jl:197-215)
+// knot centre (trajectories.jl:93-111)
 template <class T> void knot_centre(const Model& m, int i, T t, T c[3]) {
     const double* ts = m.knot_times;
     int n = m.nknots;
@@ -134,7 +134,7 @@ template <class T, class RS> bool element(const Model& m, int i, const RayCtxT<R
     return true;
 }
 
-// screen-basis invariant coefficients of one element (transport.jl:357-362)
+// screen-basis invariant coefficients of one element (transport.jl:223-228)
 template <class T> bool screen_coefficients(const Coef<T>& cf, const Frame<T>& fr, T nu_obs, Screen<T>& sc) {
     if (!(cf.jI > 0.0 || cf.aI > 0.0 || val(cf.rQ) != 0.0 || val(cf.rV) != 0.0)) return false;
     Coef<T> ci = invariants(cap_polarization(cf), nu_obs / fr.g);
@@ -142,11 +142,11 @@ template <class T> bool screen_coefficients(const Coef<T>& cf, const Frame<T>& f
     return true;
 }
 
-// ---- radiative state (step.jl:461-466) --------------------------------------------------------------------
+// ---- radiative state (step.jl:342-360) --------------------------------------------------------------------
 template <class T> struct RadState { M4<T> P; T S[4]; };
 template <class T> RadState<T> radstate_zero() { RadState<T> s; s.P = m4_eye<T>(); for (int i = 0; i < 4; ++i) s.S[i] = 0.0; return s; }
 
-// the winding counter (step.jl:498-517)
+// the winding counter (step.jl:362-398)
 template <class T> struct Winding { T zprev; int n; bool inside, crossed; };
 template <class T> Winding<T> winding_zero() { return {T(NAN), 0, false, false}; }
 template <class T> void wind(Winding<T>& w, T z, T h) {

@@ -6,7 +6,7 @@
 //     rotativities) and kappa (Pandya+ 2016, Marszewski+ 2021) fits (coefficients.jl, kappa.jl);
 //     the fitting formulae are published and are written out as such;
 //   * the polarization cap, the nu-hat invariants and the rotation into the screen basis
-//     (coefficients.jl:342-398, step.jl:526-532);
+//     (coefficients.jl:264-314, step.jl:407-413);
 //   * the exact constant-coefficient 4x4 step O = exp(-K D), Ej = int_0^D exp(-K u) du
 //     (step.jl:27-243), here from the spectral projectors of K'^2 (eigenvalues L1^2 and -L2^2);
 //   * the fluid frame, pitch angle and Walker-Penrose screen angle (frames.jl:31-128), with the
@@ -51,7 +51,7 @@ template <class T> void besselk012(T x, T& k0, T& k1, T& k2) {
     k2 = k0 + 2.0 * k1 / x;
 }
 
-// ---- Planck function in the nu-hat scaling (coefficients.jl:134-145) -------------------------------
+// ---- Planck function in the nu-hat scaling (coefficients.jl:45-67) -------------------------------
 template <class T> T planck(T nu, T Th) {
     T x = HPL * nu / (ME * CL * CL * Th);
     T nh = nu / NU0;
@@ -60,14 +60,14 @@ template <class T> T planck(T nu, T Th) {
     return pref / (kexp(x) - 1.0);
 }
 
-// ---- thermal synchrotron (coefficients.jl:171-229) ---------------------------------------------------
+// ---- thermal synchrotron (coefficients.jl:86-151) ---------------------------------------------------
 template <class T> T dexter_II(T x) { return 2.5651 * (1.0 + 1.92 * kpow(x, -1.0 / 3.0) + 0.9977 * kpow(x, -2.0 / 3.0)) * kexp(-1.8899 * kpow(x, 1.0 / 3.0)); }
 template <class T> T dexter_IQ(T x) { return 2.5651 * (1.0 + 0.93193 * kpow(x, -1.0 / 3.0) + 0.499873 * kpow(x, -2.0 / 3.0)) * kexp(-1.8899 * kpow(x, 1.0 / 3.0)); }
 template <class T> T dexter_IV(T x) {
     return (1.81384 / x + 3.42319 * kpow(x, -2.0 / 3.0) + 0.0292545 * kpow(x, -0.5) + 2.03773 * kpow(x, -1.0 / 3.0)) * kexp(-1.8899 * kpow(x, 1.0 / 3.0));
 }
 
-// Pandya+ (2016) thermal emissivities (coefficients.jl:237-253)
+// Pandya+ (2016) thermal emissivities (coefficients.jl:153-175)
 template <class T> void thermal_pandya(T ne, T Th, T B, T nu, T th, T& jI, T& jQ, T& jV) {
     T sth = ksin(th);
     T nuc = EE * B / (2.0 * KS_PI * ME * CL);
@@ -87,7 +87,7 @@ template <class T> void thermal_pandya(T ne, T Th, T B, T nu, T th, T& jI, T& jQ
     jV = pref * tv1 * tv2 * e13;
 }
 
-// Leung+ (2011) unpolarized emissivity (coefficients.jl:261-273)
+// Leung+ (2011) unpolarized emissivity (coefficients.jl:177-189)
 template <class T> T thermal_leung(T ne, T Th, T B, T nu, T th) {
     T k0, k1, k2;
     besselk012(1.0 / Th, k0, k1, k2);
@@ -147,7 +147,7 @@ template <class T> Coef<T> thermal_synchrotron(T ne, T Th, T B, T nu, T th, bool
     return thermal_synchrotron_k(ne, Th, B, nu, th, k0, k1, k2, dexter_rhoV, pandya);
 }
 
-// ---- power law (coefficients.jl:292-340) -------------------------------------------------------------------
+// ---- power law (coefficients.jl:208-256) -------------------------------------------------------------------
 template <class T> void powerlaw_rotativities(T ne, T p, T gmin, T gmax, T B, T nu, T th, T& rQ, T& rV) {
     T sth = ksin(th), cth = kcos(th);
     T nuB = EE * B * sth / (2.0 * KS_PI * ME * CL);

@@ -438,3 +438,66 @@ def test_mixed_gradient_error(dens):
     ierr = np.abs(img32 - img64).max() / np.abs(img64[..., 0]).max()
     assert 0 < gerr <= 1e-3, gerr
     assert ierr <= 1e-4, ierr
+
+
+@pytest.mark.parametrize("a,thdeg,fov", [(0.94, 17.0, 12.0), (0.5, 60.0, 15.0), (0.9, 163.0, 10.0)])
+def test_pixel_constants_and_residuals_vs_oracle(oracle, a, thdeg, fov):
+    """host(cache.consts) and the per-ray anchor residuals (cache.jl:98-99, quadrature.jl:283-291) against the
+    oracle's K1 and Recurrence(64) march: the root count and case bit-exact, the per-pixel constants to 1e-10
+    (scaled by max(|x|, 1)); the residuals are the largest differences between the recurrence and the closed
+    forms at the anchors, rounding-level in both, and agree to 1e-9."""
+    th = math.radians(thdeg)
+    cam = camera(fov, 16)
+    N = 256
+    cache = K.GeodesicCache(cam, N, store_samples=True)
+    K.regenerate(cache, a, th, marcher=K.Recurrence(64))
+    c = K.consts(cache)
+    idx = cache.perm_host - 1
+    assert np.array_equal(c["alpha"], cam.alphas[idx]) and np.array_equal(c["beta"], cam.betas[idx])
+    for j in range(cache.npix):
+        ref = oracle.pixel_constants(a, th, cam.alphas[idx[j]], cam.betas[idx[j]])
+        assert int(c["nreal"][j]) == ref["nreal"] and int(c["case"][j]) == ref["case"], j
+        for f in ("tau_total", "fo", "k", "lam", "eta", "up", "um"):
+            if np.isfinite(ref[f]) or np.isfinite(c[f][j]):
+                assert scaled_err(c[f][j], ref[f]) < 1e-10, (j, f, c[f][j], ref[f])
+    rt, rp = K.residuals(cache)
+    smp = oracle.samples(a, th, cam.alphas[idx], cam.betas[idx], N, marcher=1)
+    assert np.all(np.isfinite(rt)) and np.all(np.isfinite(rp))
+    assert np.abs(rt - smp["resid_t"]).max() < 1e-9 and np.abs(rp - smp["resid_phi"]).max() < 1e-9
+    assert np.abs(rt).max() < 2e-8 and np.abs(rp).max() < 2e-9
+
+
+def test_reflection_parity_on_the_device(oracle):
+    """The equatorial and azimuthal reflections of the reference's parity gate (test_polarized_splats.jl:197-241)
+    on the device: theta_o = 163 deg is the beta-mirror of the reflected source at 17 deg, spin -a the
+    alpha-mirror at +a, with U and V reversed (1e-10); and the 163 deg image against the oracle's (1e-10 of
+    each pixel's I)."""
+    import sys, os
+    sys.path.insert(0, os.path.dirname(__file__))
+    from test_oracle_endtoend import reflection_params, rel, grav
+    res, N, fov = 24, 60, 20.0
+    d = fov / res
+    cam = K.Camera((-fov / 2 + d / 2, fov / 2 - d / 2), (-fov / 2 + d / 2, fov / 2 - d / 2), res)
+    L = grav(6.5e9)
+
+    def image(a, th, P):
+        cache = K.GeodesicCache(cam, N, store_samples=False)
+        K.regenerate(cache, a, th, marcher=K.Fused(64))
+        return K.polarized_image(cache, P, 0.0, 230e9, L)
+
+    p = reflection_params()
+    A = image(0.94, math.radians(163.0), p)
+    q = p.copy(); q[2] = -q[2]; q[16] += math.pi; q[19] = -q[19]
+    D = image(0.94, math.radians(17.0), q)
+    sgn = np.array([1, 1, -1, -1])
+    for k in range(4):
+        assert rel(D[:, ::-1, k] * sgn[k], A[:, :, k]) < 1e-10, k
+    B = image(0.94, math.radians(17.0), p)
+    r = p.copy(); r[1] = -r[1]; r[15] = math.pi - r[15]; r[16] = math.pi - r[16]; r[18] = -r[18]
+    E = image(-0.94, math.radians(17.0), r)
+    for k in range(4):
+        assert rel(E[::-1, :, k] * sgn[k], B[:, :, k]) < 1e-10, k
+    ref, _, _ = oracle.polarized_image(0.94, math.radians(163.0), cam.alphas, cam.betas, N, p, 0.0, 230e9, L)
+    ref = ref.reshape(res, res, 4, order="F")
+    Iref = np.maximum(np.abs(ref[..., 0]), 1e-300)
+    assert (np.abs(A - ref).max(axis=2) / Iref)[ref[..., 0] > 1e-6 * ref[..., 0].max()].max() < 1e-10

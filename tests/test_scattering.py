@@ -1,4 +1,4 @@
-"""The diffractive scattering kernel of Sgr A* (ScatteringKernel, taper: scattering.jl:1-40) against ehtim's
+"""The diffractive scattering kernel of Sgr A* (ScatteringKernel, taper: scattering.jl:1-38) against ehtim's
 sgra_kernel_uv, the reference's own gate (test_scattering.jl:12-28): tests/golden/ehtim/sgra_kernel_ehtim.csv
 holds ehtim's kernel at 45 (u, v) points at 230 GHz for ehtim's constants (Bower et al. 2006) and for
 Johnson et al. 2018's. A kernel with the position angle entering un-negated (the convention the kernel
