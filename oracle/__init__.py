@@ -147,12 +147,10 @@ def spacetime_tangent(a, thetao, alphas, betas, N, params, t_obs, nu, L, kind="t
     return out, dout
 
 
-def polarized_composite(a, thetao, alphas, betas, N, parts, t_obs, nu, L, nmax=-1, slab=0.0, M=64, marcher=1):
-    """Stokes image of CompositeModel(parts...) (transport.jl:298-316). parts: a list of
-    (kind, params) or (kind, params, hyp, gamma_max) with kind thermal, powerlaw or kappa, or
-    ("knot", params, knots, knot_times) with knots 3 x nknots x ns."""
-    al, be = _f64(alphas), _f64(betas)
-    n, k = len(al), len(parts)
+def _parts(parts):
+    """ctypes arrays of composite parts: (kind, params[, hyp[, gamma_max[, knots, knot_times]]]) for thermal,
+    powerlaw or kappa parts, ("knot", params, knots, knot_times) for thermal knot parts; knots 3 x nknots x ns."""
+    k = len(parts)
     kinds, rows, ns, nk = (ctypes.c_int * k)(), (ctypes.c_int * k)(), (ctypes.c_int * k)(), (ctypes.c_int * k)()
     Ps, Hs, gm = (ctypes.c_void_p * k)(), (ctypes.c_void_p * k)(), (ctypes.c_double * k)()
     Ks, Ts = (ctypes.c_void_p * k)(), (ctypes.c_void_p * k)()
@@ -165,23 +163,58 @@ def polarized_composite(a, thetao, alphas, betas, N, parts, t_obs, nu, L, nmax=-
         gm[q] = 1e5
         keep.append(P)
         if kind == "knot":
-            kn = np.asfortranarray(part[2], dtype=np.float64)
-            kt = _f64(part[3])
+            kn_, kt_ = part[2], part[3]
+        else:
+            hyp = part[2] if len(part) > 2 else None
+            hy = _f64(hyp) if hyp is not None else None
+            keep.append(hy)
+            Hs[q] = hy.ctypes.data if hy is not None else None
+            gm[q] = part[3] if len(part) > 3 and part[3] is not None else 1e5
+            kn_, kt_ = (part[4], part[5]) if len(part) > 5 else (None, None)
+        if kn_ is not None:
+            kn = np.asfortranarray(kn_, dtype=np.float64)
+            kt = _f64(kt_)
             keep += [kn, kt]
             Ks[q], Ts[q], nk[q] = kn.ctypes.data, kt.ctypes.data, len(kt)
-            continue
-        hyp = part[2] if len(part) > 2 else None
-        hy = _f64(hyp) if hyp is not None else None
-        keep.append(hy)
-        Hs[q] = hy.ctypes.data if hy is not None else None
-        gm[q] = part[3] if len(part) > 3 else 1e5
+    return (kinds, rows, ns, Ps, Hs, gm, Ks, Ts, nk), keep
+
+
+def polarized_composite(a, thetao, alphas, betas, N, parts, t_obs, nu, L, nmax=-1, slab=0.0, M=64, marcher=1):
+    """Stokes image of CompositeModel(parts...) (transport.jl:298-316). parts: a list of
+    (kind, params[, hyp[, gamma_max[, knots, knot_times]]]) with kind thermal, powerlaw or kappa, or
+    ("knot", params, knots, knot_times) with knots 3 x nknots x ns."""
+    al, be = _f64(alphas), _f64(betas)
+    n = len(al)
+    arrs, keep = _parts(parts)
     out = np.zeros((n, 4))
     rt, rp = np.zeros(n), np.zeros(n)
-    rc = lib().kso_polarized_composite(_d(a), _d(thetao), _i(n), _p(al), _p(be), _i(N), _i(M), _i(marcher), _i(k), kinds, rows, ns,
-                                       Ps, Hs, gm, Ks, Ts, nk, _d(t_obs), _d(nu), _d(L), _i(nmax), _d(slab), _p(out), _p(rt), _p(rp))
+    rc = lib().kso_polarized_composite(_d(a), _d(thetao), _i(n), _p(al), _p(be), _i(N), _i(M), _i(marcher), _i(len(parts)), *arrs,
+                                       _d(t_obs), _d(nu), _d(L), _i(nmax), _d(slab), _p(out), _p(rt), _p(rp))
     if rc:
         raise ValueError(f"kso_polarized_composite: bad parts (code {rc})")
     return out, rt, rp
+
+
+def model_gradient(a, thetao, alphas, betas, N, parts, t_obs, nu, L, dstokes, nmax=-1, slab=0.0, M=64):
+    """The dual-sweep gradient of sum dstokes . image for any model (parts as in polarized_composite; a single
+    population is one part), every row of every splat by the generic element adjoint: returns (a list of
+    nrows x ns gradient blocks, one per part, and the image npix x 4)."""
+    al, be = _f64(alphas), _f64(betas)
+    n = len(al)
+    arrs, keep = _parts(parts)
+    ds = _f64(dstokes)
+    sizes = [np.asarray(p[1]).shape for p in parts]
+    g = np.zeros(sum(r * c for r, c in sizes))
+    img = np.zeros((n, 4))
+    rc = lib().kso_model_gradient(_d(a), _d(thetao), _i(n), _p(al), _p(be), _i(N), _i(M), _i(len(parts)), *arrs,
+                                  _d(t_obs), _d(nu), _d(L), _i(nmax), _d(slab), _p(ds), _p(g), _p(img))
+    if rc:
+        raise ValueError(f"kso_model_gradient: bad parts (code {rc})")
+    out, off = [], 0
+    for r, c in sizes:
+        out.append(g[off:off + r * c].reshape((r, c), order="F").copy())
+        off += r * c
+    return out, img
 
 
 def count_polarized(a, thetao, alphas, betas, N, params, t_obs, nu, L, kind="thermal", hyp=None, gamma_max=1e5, M=64):

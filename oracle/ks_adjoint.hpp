@@ -360,11 +360,32 @@ inline void element_adjoint(const Model& m, int i, const RayCtx& rc, const Sampl
     }
 }
 
-// Gradient of sum_pixels dstokes . observed_stokes over one ray (thermal splats); adds into grad
-// (21 x ns, column-major) and returns the ray's observed Stokes vector. With S = Counted, phases (if
+// One element's share of one sample for any fluid population, with or without knots: every row of the splat
+// seeded as a dual number (up to 22 partials) through the same element_rows and screen coefficients that the
+// transport evaluates, q = j_bar . j + alpha_bar . alpha + rho_bar . rho; adds dq/d(row) into grad. It restates
+// nothing by hand (no chain rule written out), so it checks the device's hand-split element adjoints. The 2F1
+// factor of kappa splats is differentiated with the splat's kappa and w (recomputed in duals), not held.
+inline void element_adjoint_generic(const Model& m, int i, const RayCtx& rc, const Sample<double>& s, double t_obs, double nu,
+                                    const double jbar[4], const double abar[4], const double rbar[3], double* grad) {
+    using DG = Dual<22, double>;
+    DG pr[32];
+    for (int r = 0; r < m.nrows; ++r) pr[r] = DG::seed(m.P(r, i), r);
+    const Sample<DG> sd{DG(s.t), DG(s.r), DG(s.th), DG(s.ph), s.nur, s.nuth, s.ok};
+    Coef<DG> cf; Frame<DG> fr;
+    if (!element_rows(m, i, rc, sd, DG(t_obs), DG(nu), pr, cf, fr)) return;
+    Screen<DG> sc;
+    if (!screen_coefficients<DG>(cf, fr, DG(nu), sc)) return;
+    DG q(0.0);
+    for (int k = 0; k < 4; ++k) q = q + DG(jbar[k]) * sc.j[k] + DG(abar[k]) * sc.a[k];
+    for (int k = 0; k < 3; ++k) q = q + DG(rbar[k]) * sc.r[k];
+    for (int r = 0; r < m.nrows; ++r) grad[r] = grad[r] + q.d[r];
+}
+
+// Gradient of sum_pixels dstokes . observed_stokes over one ray; adds into grad (21 x ns, column-major, thermal
+// splats; with GENERIC any population, knots and composites, nrows x ns per part in order) and returns the ray's observed Stokes vector. With S = Counted, phases (if
 // given) receives the operation counts at the ends of the march, the forward steps, the tails and the
 // adjoint sweep, and phases[4] the part of the sweep spent in sample_adjoint.
-template <class S, bool ANALYTIC = false>
+template <class S, bool ANALYTIC = false, bool GENERIC = false>
 inline void polarized_gradient_ray(const Model& m, double a, double thetao, double alpha, double beta, int N, int M, double t_obs, double nu,
                                    double L, int nmax, double slab, const double dstokes[4], S* grad, S image[4], OpCounts* phases = nullptr) {
     Pixel<S> p = pixel_constants<S>(S(a), S(alpha), S(beta), S(thetao));
@@ -428,7 +449,17 @@ inline void polarized_gradient_ray(const Model& m, double a, double thetao, doub
         sample_adjoint<S, ANALYTIC>(q.j4, q.a4, q.r3, q.D, w, &tails[4 * k], Otw, E, jb, ab, rb);
         in_sample.flops += op_counts().flops - before.flops;
         in_sample.special += op_counts().special - before.special;
-        for (int i = 0; i < m.ns; ++i) element_adjoint<S, ANALYTIC>(m, i, rc, Sm[k - 1], tS, nuS, jb, ab, rb, grad + (size_t)m.nrows * i);
+        if constexpr (GENERIC) {
+            // every part of a composite, its gradient block after the previous parts' (nrows x ns each)
+            size_t off = 0;
+            for (const Model* pm = &m; pm; pm = pm->next) {
+                for (int i = 0; i < pm->ns; ++i)
+                    element_adjoint_generic(*pm, i, rc, Sm[k - 1], t_obs, nu, jb, ab, rb, grad + off + (size_t)pm->nrows * i);
+                off += (size_t)pm->nrows * pm->ns;
+            }
+        } else {
+            for (int i = 0; i < m.ns; ++i) element_adjoint<S, ANALYTIC>(m, i, rc, Sm[k - 1], tS, nuS, jb, ab, rb, grad + (size_t)m.nrows * i);
+        }
         for (int c = 0; c < 4; ++c) w[c] = Otw[c];
     }
     if (phases) { phases[3] = op_counts(); phases[4] = in_sample; }

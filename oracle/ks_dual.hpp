@@ -78,6 +78,7 @@ KSD_T Dual<N, S> ktanh(const Dual<N, S>& x) { S t = ktanh(x.v); return chain(t, 
 KSD_T Dual<N, S> ksinh(const Dual<N, S>& x) { return chain(ksinh(x.v), kcosh(x.v), x); }
 KSD_T Dual<N, S> kcosh(const Dual<N, S>& x) { return chain(kcosh(x.v), ksinh(x.v), x); }
 KSD_T Dual<N, S> kcbrt(const Dual<N, S>& x) { S c = kcbrt(x.v); return chain(c, S(c / (3.0 * x.v)), x); }
+KSD_T Dual<N, S> kgamma(const Dual<N, S>& x) { const double g = std::tgamma(val(x.v)); return chain(S(g), S(g * kdigamma(val(x.v))), x); }
 KSD_T Dual<N, S> kfloor(const Dual<N, S>& x) { return Dual<N, S>(kfloor(x.v)); }
 KSD_T Dual<N, S> kabs(const Dual<N, S>& x) { return val(x.v) < 0 ? -x : x; }
 KSD_T Dual<N, S> kcopysign(const Dual<N, S>& a, const Dual<N, S>& b) { return (std::signbit(val(a.v)) != std::signbit(val(b.v))) ? -a : a; }

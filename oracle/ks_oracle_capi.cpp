@@ -271,7 +271,7 @@ int kso_polarized_composite(double a, double thetao, int npix, const double* alp
     for (int p = 0; p < nparts; ++p) {
         if (kinds[p] == MODEL_THIN) return 2;
         if (kinds[p] == MODEL_KNOT && (!knots || !knots[p] || !knot_times[p] || nknots[p] < 1)) return 3;
-        const bool kn = kinds[p] == MODEL_KNOT;
+        const bool kn = knots && knots[p] && knot_times[p] && nknots[p] >= 1;
         parts.push_back(make_model<double>(kinds[p], nrows[p], ns[p], params[p], hyp ? hyp[p] : nullptr, gamma_max[p], kn ? knots[p] : nullptr,
                                            kn ? knot_times[p] : nullptr, kn ? nknots[p] : 0));
     }
@@ -329,6 +329,44 @@ int kso_polarized_gradient(double a, double thetao, int npix, const double* alph
     }
     for (int t = 0; t < nth; ++t)
         for (size_t k = 0; k < 21 * (size_t)ns; ++k) dparams[k] += part[t][k];
+    return 0;
+}
+
+// The dual-sweep gradient for any model: a composite of parts (as kso_polarized_composite; a single population
+// is one part; any fluid part may carry knots), every row of every splat differentiated by the generic element
+// adjoint. dparams: the parts' nrows x ns blocks in order (accumulates); image (npix x 4) is written.
+int kso_model_gradient(double a, double thetao, int npix, const double* alphas, const double* betas, int N, int M,
+                       int nparts, const int* kinds, const int* nrows, const int* ns, const double* const* params,
+                       const double* const* hyp, const double* gamma_max, const double* const* knots, const double* const* knot_times,
+                       const int* nknots, double t_obs, double nu, double L, int nmax, double slab,
+                       const double* dstokes, double* dparams, double* image) {
+    if (nparts < 1) return 1;
+    std::vector<ModelHolder> parts;
+    parts.reserve(nparts);
+    size_t total = 0;
+    for (int p = 0; p < nparts; ++p) {
+        if (kinds[p] == MODEL_THIN) return 2;
+        if (kinds[p] == MODEL_KNOT && (!knots || !knots[p] || !knot_times[p] || nknots[p] < 1)) return 3;
+        const bool kn = knots && knots[p] && knot_times[p] && nknots[p] >= 1;
+        parts.push_back(make_model<double>(kinds[p], nrows[p], ns[p], params[p], hyp ? hyp[p] : nullptr, gamma_max[p], kn ? knots[p] : nullptr,
+                                           kn ? knot_times[p] : nullptr, kn ? nknots[p] : 0));
+        total += (size_t)nrows[p] * ns[p];
+    }
+    for (int p = 0; p < nparts; ++p) {
+        parts[p].m.bessel = parts[p].bessel.data();
+        parts[p].m.next = p + 1 < nparts ? &parts[p + 1].m : nullptr;
+    }
+    const Model& m = parts[0].m;
+    int nth = omp_get_max_threads();
+    std::vector<std::vector<double>> part(nth, std::vector<double>(std::max(total, (size_t)1), 0.0));
+#pragma omp parallel for schedule(dynamic, 2)
+    for (int i = 0; i < npix; ++i) {
+        std::vector<double>& g = part[omp_get_thread_num()];
+        polarized_gradient_ray<double, false, true>(m, a, thetao, alphas[i], betas[i], N, M, t_obs, nu, L, nmax, slab, dstokes + 4 * (size_t)i,
+                                                    g.data(), image + 4 * (size_t)i);
+    }
+    for (int t = 0; t < nth; ++t)
+        for (size_t k = 0; k < total; ++k) dparams[k] += part[t][k];
     return 0;
 }
 

@@ -139,4 +139,16 @@ template <class T> Cx<T> ccbrt(Cx<T> z) {
     return {m * kcos(th), m * ksin(th)};
 }
 
+// digamma psi(x) for x > 0: the recurrence psi(x) = psi(x + 1) - 1/x up to x >= 10, then the asymptotic series
+// ln x - 1/(2x) - sum B_2k / (2k x^2k) (DLMF 5.5.2, 5.11.2), accurate to rounding
+inline double kdigamma(double x) {
+    double r = 0.0;
+    while (x < 10.0) { r -= 1.0 / x; x += 1.0; }
+    const double i2 = 1.0 / (x * x);
+    const double ser = i2 * (1.0 / 12.0 - i2 * (1.0 / 120.0 - i2 * (1.0 / 252.0 - i2 * (1.0 / 240.0 - i2 * (1.0 / 132.0 - i2 * (691.0 / 32760.0 - i2 / 12.0))))));
+    return r + std::log(x) - 0.5 / x - ser;
+}
+// the gamma function of a scalar (no operation count); the dual overload in ks_dual.hpp carries Gamma psi
+template <class T> T kgamma(const T& x) { return T(std::tgamma(val(x))); }
+
 }  // namespace kso

@@ -13,6 +13,7 @@
 #pragma once
 #include "ks_geodesic.hpp"
 #include "ks_transfer.hpp"
+#include "ks_dual.hpp"
 
 namespace kso {
 
@@ -34,6 +35,8 @@ struct Model {
     // `next` (and its successors); coefficients of overlapping elements add
     const Model* next = nullptr;
     double P(int row, int i) const { return params[row + nrows * i]; }
+    // centres that follow knots (KnotSplats, trajectories.jl:80-128; here for any fluid population)
+    bool has_knots() const { return knots != nullptr && knot_times != nullptr && nknots > 0 && kind != MODEL_THIN; }
 };
 
 // pattern-frame offset of (x, y, z) from the centre of splat i at time t (Splats.jl:58-69)
@@ -93,45 +96,103 @@ template <class T> void knot_centre(const Model& m, int i, T t, T c[3]) {
 template <class R> struct RayCtxT { R a, alpha, beta, thetao, eta, lam; };
 using RayCtx = RayCtxT<double>;
 
-template <class T, class RS> bool element(const Model& m, int i, const RayCtxT<RS>& rc, const Sample<T>& s, T t_obs, T nu_obs, Coef<T>& cf, Frame<T>& fr) {
+// K0, K1, K2 at x_b = 1 / Theta_e (or 1 / w) from the splat's table kb at the primal value: the table itself
+// for plain scalars, and for dual numbers the table with the derivatives K0' = -K1, K1' = -K0 - K1/x,
+// K2' = K0' + 2 K1'/x - 2 K1/x^2 carried through x_b
+template <class T> struct is_dual : std::false_type {};
+template <int N, class S> struct is_dual<Dual<N, S>> : std::true_type {};
+template <class T> void bessel_at(const double* kb, const T& xb, T& k0, T& k1, T& k2) {
+    if constexpr (is_dual<T>::value) {
+        const double x = val(xb);
+        const double d0 = -kb[1], d1 = -kb[0] - kb[1] / x, d2 = d0 + 2.0 * d1 / x - 2.0 * kb[1] / (x * x);
+        k0 = chain(kb[0], d0, xb); k1 = chain(kb[1], d1, xb); k2 = chain(kb[2], d2, xb);
+    } else {
+        k0 = T(kb[0]); k1 = T(kb[1]); k2 = T(kb[2]);
+    }
+}
+
+// The Gaussian exponent of splat i at (t, x, y, z) from its rows pr (given in the scalar T, so that dual
+// rows carry their derivatives): the pattern-frame offset rotated by omega (t - t0) about the spin axis
+// (Splats.jl:58-93), or for a model with knots the offset from the knot-interpolated centre (trajectories.jl:
+// 93-128; no pattern rotation), where the centre rows translate the trajectory rigidly (their derivative is
+// the derivative with respect to a shift of every knot).
+template <class T> T gexp_rows(const Model& m, int i, const T* pr, T t, T x, T y, T z) {
+    T dr[3];
+    if (m.has_knots()) {
+        T c[3]; knot_centre(m, i, t, c);
+        for (int a = 0; a < 3; ++a) c[a] = c[a] + (pr[a] - m.P(a, i));
+        dr[0] = x - c[0]; dr[1] = y - c[1]; dr[2] = z - c[2];
+    } else {
+        T ph = pr[m.nrows - 1] * (t - pr[10]);
+        T sp = ksin(ph), cp = kcos(ph);
+        T cx = pr[0] * cp - pr[1] * sp;
+        T cy = pr[0] * sp + pr[1] * cp;
+        T d[3] = {x - cx, y - cy, z - pr[2]};
+        dr[0] = d[0] * cp + d[1] * sp; dr[1] = -d[0] * sp + d[1] * cp; dr[2] = d[2];
+    }
+    T R[3][3]; quat_rot(pr[6], pr[7], pr[8], pr[9], R);
+    T u[3];
+    for (int a = 0; a < 3; ++a) u[a] = R[0][a] * dr[0] + R[1][a] * dr[1] + R[2][a] * dr[2];
+    T q2 = ksq(u[0] * kexp(-pr[3])) + ksq(u[1] * kexp(-pr[4])) + ksq(u[2] * kexp(-pr[5]));
+    T tau2 = ksq((t - pr[10]) * kexp(-pr[11]));
+    return q2 + tau2;
+}
+
+// One element's fluid coefficients and frame at a sample (element(::PolarizedSplats / PowerLawSplats /
+// KappaSplats / KnotSplats), polarized.jl:96-150, populations.jl:13-104, trajectories.jl:113-128), from the
+// splat's rows pr in the scalar T. Plain scalars take the per-splat tables (Bessel values, the kappa 2F1
+// factor) as they are; dual rows recompute what depends on them (Gamma functions, 2F1) or carry the tables'
+// derivatives, so that the result is differentiable in every row.
+template <class T, class RS> bool element_rows(const Model& m, int i, const RayCtxT<RS>& rc, const Sample<T>& s, T t_obs, T nu_obs,
+                                               const T* pr, Coef<T>& cf, Frame<T>& fr) {
     T x, y, z;
     quasi_cartesian_ks(T(rc.a), s.r, s.th, s.ph, x, y, z);
     T t = t_obs - s.t;
-    T G;
-    int rB, rthB, rphB, ru;
-    if (m.kind == MODEL_KNOT) {
-        T c[3]; knot_centre(m, i, t, c);
-        T d[3] = {x - c[0], y - c[1], z - c[2]};
-        T R[3][3]; quat_rot(T(m.P(6, i)), T(m.P(7, i)), T(m.P(8, i)), T(m.P(9, i)), R);
-        T u[3];
-        for (int a = 0; a < 3; ++a) u[a] = R[0][a] * d[0] + R[1][a] * d[1] + R[2][a] * d[2];
-        G = kexp(-gauss_exponent(m, i, t, u) / 2.0);
-        if (!(G > WEIGHT_CUTOFF)) return false;
-    } else {
-        if (outside_support(m, i, t, x, y, z)) return false;
-        G = splat_weight(m, i, t, x, y, z);
-        if (!(G > WEIGHT_CUTOFF)) return false;
+    if (!m.has_knots()) {
+        if constexpr (is_dual<T>::value) { if (outside_support(m, i, val(t), val(x), val(y), val(z))) return false; }
+        else if (outside_support(m, i, t, x, y, z)) return false;
     }
-    if (m.kind == MODEL_THERMAL || m.kind == MODEL_KNOT) { rB = 14; rthB = 15; rphB = 16; ru = 17; }
-    else { rB = 15; rthB = 16; rphB = 17; ru = 18; }
-    T Bmag = kexp(T(m.P(rB, i)));
-    T st = ksin(T(m.P(rthB, i))), ct = kcos(T(m.P(rthB, i))), sp = ksin(T(m.P(rphB, i))), cp = kcos(T(m.P(rphB, i)));
+    T G = kexp(-gexp_rows(m, i, pr, t, x, y, z) / 2.0);
+    if (!(G > WEIGHT_CUTOFF)) return false;
+    const bool th_rows = m.kind == MODEL_THERMAL || m.kind == MODEL_KNOT;
+    const int rB = th_rows ? 14 : 15, rthB = rB + 1, rphB = rB + 2, ru = rB + 3;
+    T Bmag = kexp(pr[rB]);
+    T st = ksin(pr[rthB]), ct = kcos(pr[rthB]), sp = ksin(pr[rphB]), cp = kcos(pr[rphB]);
     T B[3] = {Bmag * st * cp, Bmag * st * sp, Bmag * ct};
-    T u[3] = {m.P(ru, i), m.P(ru + 1, i), m.P(ru + 2, i)};
+    T u[3] = {pr[ru], pr[ru + 1], pr[ru + 2]};
     fr = local_frame(T(rc.a), s.r, s.th, T(rc.eta), T(rc.lam), s.nur, s.nuth, T(rc.alpha), T(rc.beta), T(rc.thetao), u, B);
     T nuf = nu_obs / fr.g;
     T thB = kacos(kmin(kmax(fr.cosB, T(-1.0)), T(1.0)));
-    T ne = kexp(T(m.P(12, i))) * G;
-    if (m.kind == MODEL_THERMAL || m.kind == MODEL_KNOT) {
-        const double* kb = m.bessel + 3 * i;
-        cf = thermal_synchrotron_k(ne, kexp(T(m.P(13, i))), Bmag, nuf, thB, T(kb[0]), T(kb[1]), T(kb[2]));
+    T ne = kexp(pr[12]) * G;
+    const double* kb = m.bessel + 3 * i;
+    if (th_rows) {
+        T Th = kexp(pr[13]);
+        T k0, k1, k2;
+        bessel_at(kb, T(1.0) / Th, k0, k1, k2);
+        cf = thermal_synchrotron_k(ne, Th, Bmag, nuf, thB, k0, k1, k2);
     } else if (m.kind == MODEL_POWERLAW) {
-        cf = powerlaw_synchrotron(ne, T(m.P(13, i)), kexp(T(m.P(14, i))), T(m.gamma_max), Bmag, nuf, thB);
+        cf = powerlaw_synchrotron(ne, pr[13], kexp(pr[14]), T(m.gamma_max), Bmag, nuf, thB);
     } else {
-        const double* kb = m.bessel + 3 * i;
-        cf = kappa_synchrotron(ne, T(m.P(13, i)), kexp(T(m.P(14, i))), Bmag, nuf, thB, T(m.hyp[i]), T(kb[0] / (kb[2] + 1e-40)));
+        T w = kexp(pr[14]);
+        T hyp, k0k2;
+        if constexpr (is_dual<T>::value) {
+            hyp = kappa_hypergeometric_t(pr[13], w);
+            T k0, k1, k2;
+            bessel_at(kb, T(1.0) / w, k0, k1, k2);
+            k0k2 = k0 / (k2 + 1e-40);
+        } else {
+            hyp = T(m.hyp[i]);
+            k0k2 = T(kb[0] / (kb[2] + 1e-40));
+        }
+        cf = kappa_synchrotron(ne, pr[13], w, Bmag, nuf, thB, hyp, k0k2);
     }
     return true;
+}
+
+template <class T, class RS> bool element(const Model& m, int i, const RayCtxT<RS>& rc, const Sample<T>& s, T t_obs, T nu_obs, Coef<T>& cf, Frame<T>& fr) {
+    T pr[32];
+    for (int r = 0; r < m.nrows; ++r) pr[r] = T(m.P(r, i));
+    return element_rows(m, i, rc, s, t_obs, nu_obs, pr, cf, fr);
 }
 
 // screen-basis invariant coefficients of one element (transport.jl:223-228)

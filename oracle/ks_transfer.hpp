@@ -162,12 +162,12 @@ template <class T> Coef<T> powerlaw_synchrotron(T ne, T p, T gmin, T gmax, T B, 
     T gspan = kpow(gmin, 1.0 - p) - kpow(gmax, 1.0 - p);
     T pref = ne * EE * EE * nuc / CL;
     T jI = pref * (kpow(T(3.0), p / 2.0) * (p - 1.0) * sth) / (2.0 * (p + 1.0) * gspan) *
-           T(std::tgamma(val((3.0 * p - 1.0) / 12.0))) * T(std::tgamma(val((3.0 * p + 19.0) / 12.0))) * kpow(nu / (nuc * sth), -(p - 1.0) / 2.0);
+           kgamma((3.0 * p - 1.0) / 12.0) * kgamma((3.0 * p + 19.0) / 12.0) * kpow(nu / (nuc * sth), -(p - 1.0) / 2.0);
     T jQ = (p + 1.0) / (p + 7.0 / 3.0) * jI;
     T jV = (171.0 / 250.0) * kpow(p, T(49.0 / 100.0)) * cth / sth * kpow(nu / (3.0 * nuc * sth), T(-0.5)) * jI;
     T apref = ne * EE * EE / (nu * ME * CL);
     T aI = apref * kpow(T(3.0), (p + 1.0) / 2.0) * (p - 1.0) / (4.0 * gspan) *
-           T(std::tgamma(val((3.0 * p + 2.0) / 12.0))) * T(std::tgamma(val((3.0 * p + 22.0) / 12.0))) * kpow(nu / (nuc * sth), -(p + 2.0) / 2.0);
+           kgamma((3.0 * p + 2.0) / 12.0) * kgamma((3.0 * p + 22.0) / 12.0) * kpow(nu / (nuc * sth), -(p + 2.0) / 2.0);
     T aQ = aI * kpow((17.0 / 500.0) * p - 43.0 / 1250.0, T(43.0 / 500.0));
     T term6 = kpow((31.0 / 10.0) * kpow(sth, T(-48.0 / 25.0)) - 31.0 / 10.0, T(64.0 / 125.0));
     T sgn = cth > 0.0 ? T(1.0) : (cth < 0.0 ? T(-1.0) : T(0.0));
@@ -191,6 +191,18 @@ inline double hyp2f1_neg(double a, double b, double c, double z) {
     return std::pow(1.0 - z, -a) * sum;
 }
 inline double kappa_hypergeometric(double kappa, double w) { return hyp2f1_neg(kappa - 1.0 / 3.0, kappa + 1.0, kappa + 2.0 / 3.0, -kappa * w); }
+// the same series in any scalar (dual numbers carry the parameters' and the argument's derivatives)
+template <class T> T hyp2f1_neg_t(T a, T b, T c, T z) {
+    T w = z / (z - 1.0);
+    T bb = c - b, term = T(1.0), sum = T(1.0);
+    for (int n = 0; n < 200000; ++n) {
+        term = term * (a + double(n)) * (bb + double(n)) / ((c + double(n)) * (n + 1.0)) * w;
+        sum = sum + term;
+        if (std::fabs(val(term)) < 1e-17 * std::fabs(val(sum))) break;
+    }
+    return kpow(1.0 - z, -a) * sum;
+}
+template <class T> T kappa_hypergeometric_t(T kappa, T w) { return hyp2f1_neg_t(kappa - 1.0 / 3.0, kappa + 1.0, kappa + 2.0 / 3.0, -kappa * w); }
 
 template <class T> T kappa_rhoQ_fit(int i, T w, T X) {
     T sw = ksqrt(w), e5 = kexp(-5.0 * w);
@@ -230,8 +242,8 @@ template <class T> Coef<T> kappa_synchrotron(T ne, T kappa, T w, T B, T nu, T th
     T X = nu / nuw;
     const double small = 1e-40;
     T pref = ne * EE * EE * nuc * sth / CL;
-    T g43 = T(std::tgamma(val(kappa - 4.0 / 3.0))), g2 = T(std::tgamma(val(kappa - 2.0)));
-    T gq = T(std::tgamma(val(kappa / 4.0 - 1.0 / 3.0))), gp = T(std::tgamma(val(kappa / 4.0 + 4.0 / 3.0)));
+    T g43 = kgamma(kappa - 4.0 / 3.0), g2 = kgamma(kappa - 2.0);
+    T gq = kgamma(kappa / 4.0 - 1.0 / 3.0), gp = kgamma(kappa / 4.0 + 4.0 / 3.0);
     T c37 = std::pow(3.0, 7.0 / 3.0);
     T NlowI = 4.0 * KS_PI * g43 / (c37 * g2);
     T NhighI = 0.25 * kpow(T(3.0), (kappa - 1.0) / 2.0) * (kappa - 2.0) * (kappa - 1.0) * gq * gp + small;
@@ -250,7 +262,7 @@ template <class T> Coef<T> kappa_synchrotron(T ne, T kappa, T w, T B, T nu, T th
     T jV = val(kabs(NhighV)) < small * small ? T(0.0)
          : -(pref * NlowV * kpow(X, T(1.0 / 3.0)) * kpow(1.0 + kpow(X, xV * (3.0 * kappa - 4.0) / 6.0) * kpow(NlowV / NhighV, xV), -1.0 / xV)) * sgn;
     T apref = ne * EE / (B * sth);
-    T g53 = T(std::tgamma(5.0 / 3.0)), gk2 = T(std::tgamma(val(2.0 + kappa / 2.0)));
+    T g53 = T(std::tgamma(5.0 / 3.0)), gk2 = kgamma(2.0 + kappa / 2.0);
     T ck = (kappa - 2.0) * (kappa - 1.0) * kappa;
     T c16 = std::pow(3.0, 1.0 / 6.0);
     T tpi2 = (2.0 * KS_PI) * (2.0 * KS_PI);
