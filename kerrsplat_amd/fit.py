@@ -358,3 +358,201 @@ def taper(kernel, model, u, v):
     if kernel is None:
         return model
     return np.asarray(model) * kernel(u, v)[:, None]
+
+
+# ---- the fitting schedule: stages, Adam, annealing, partition hygiene (Fit.jl:94-275, 308-438, 620-659) -------
+class Stage:
+    """Stage(; free, iterations, eta, eta_end, batch, freqs, steps, label) (Fit.jl:318-338): the rows to free (0-based
+    indices; None frees all), Adam iterations, the learning rate annealed from eta to eta_end on a cosine, the
+    minibatch (nframes, nfreqs) or None, the frequency indices that enter (None = all) and per-row step
+    multipliers {row: factor}."""
+
+    def __init__(self, free=None, iterations=100, eta=0.02, eta_end=None, batch=None, freqs=None, steps=None, label=""):
+        self.free, self.iterations, self.eta = free, int(iterations), float(eta)
+        self.eta_end = self.eta if eta_end is None else float(eta_end)
+        self.batch, self.freqs, self.steps, self.label = batch, freqs, steps, label
+
+
+class Hygiene:
+    """Hygiene(; every, prune_fraction, densify_threshold, merge_position, merge_shape, max_splats) (Fit.jl:340-356):
+    prune, merge and densify every `every` iterations of a stage (0 disables); each pass resets the optimizer."""
+
+    def __init__(self, every=0, prune_fraction=1e-4, densify_threshold=np.inf, merge_position=0.2, merge_shape=0.1, max_splats=2 ** 62):
+        self.every, self.prune_fraction, self.densify_threshold = int(every), float(prune_fraction), float(densify_threshold)
+        self.merge_position, self.merge_shape, self.max_splats = float(merge_position), float(merge_shape), int(max_splats)
+
+
+def freeze(params, rows):
+    """freeze(params, rows) (Fit.jl:96-103): a boolean mask that frees the given rows (0-based) of every splat."""
+    free = np.zeros(np.shape(params), dtype=bool)
+    for r in rows:
+        if not 0 <= int(r) < free.shape[0]:
+            raise ValueError(f"unknown parameter row {r}")
+        free[int(r), :] = True
+    return free
+
+
+def step_scale(params, steps):
+    """step_scale(params, steps) (Fit.jl:117-126): per-entry multipliers of the Adam step, 1 except for the rows
+    given in steps ({row: factor}); applied as old + scale (new - old)."""
+    scale = np.ones(np.shape(params))
+    if steps is None:
+        return scale
+    for r, f in dict(steps).items():
+        if not 0 <= int(r) < scale.shape[0]:
+            raise ValueError(f"unknown parameter row {r}")
+        scale[int(r), :] = f
+    return scale
+
+
+class Adam:
+    """Adam (Kingma & Ba 2015) with Optimisers.jl's defaults, beta = (0.9, 0.999), epsilon = 1e-8: per entry,
+    m = b1 m + (1 - b1) g, v = b2 v + (1 - b2) g^2, x -= eta (m / (1 - b1^t)) / (sqrt(v / (1 - b2^t)) + eps)."""
+
+    def __init__(self, shape, eta, beta=(0.9, 0.999), eps=1e-8):
+        self.eta, self.b1, self.b2, self.eps = float(eta), beta[0], beta[1], eps
+        self.m, self.v = np.zeros(shape), np.zeros(shape)
+        self.p1, self.p2 = 1.0, 1.0
+
+    def step(self, x, g):
+        self.p1 *= self.b1
+        self.p2 *= self.b2
+        self.m = self.b1 * self.m + (1.0 - self.b1) * g
+        self.v = self.b2 * self.v + (1.0 - self.b2) * g * g
+        return x - self.eta * (self.m / (1.0 - self.p1)) / (np.sqrt(self.v / (1.0 - self.p2)) + self.eps)
+
+
+def _rot(q):
+    w, x, y, z = np.asarray(q, dtype=np.float64) / np.linalg.norm(q)
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def prune(params, fraction=1e-4):
+    """prune(params; fraction) (Fit.jl:172-176): drop splats whose density x volume e^(ln n_e + s1 + s2 + s3) is
+    below fraction of the largest; returns (params, kept indices)."""
+    P = np.asarray(params)
+    w = np.exp(P[12] + P[3] + P[4] + P[5])
+    kept = np.nonzero(w >= fraction * w.max())[0]
+    return np.asfortranarray(P[:, kept]), kept
+
+
+def densify(params, grad, threshold, scale_factor=1.6):
+    """densify(params, grad; threshold, scale_factor) (Fit.jl:187-209): split every splat whose centre-gradient
+    norm exceeds threshold into two at +-1/2 of its largest principal scale, every scale divided by scale_factor
+    and the density set so that the pair's density x volume equals the parent's."""
+    P, G = np.asarray(params), np.asarray(grad)
+    cols = []
+    for i in range(P.shape[1]):
+        col = P[:, i]
+        if np.sqrt(G[0, i] ** 2 + G[1, i] ** 2 + G[2, i] ** 2) > threshold:
+            k = int(np.argmax(col[3:6]))
+            axis = _rot(col[6:10])[:, k] * np.exp(col[3 + k]) / 2
+            for sgn in (-1.0, 1.0):
+                c = col.copy()
+                c[0:3] += sgn * axis
+                c[3:6] -= np.log(scale_factor)
+                c[12] += 3 * np.log(scale_factor) - np.log(2.0)
+                cols.append(c)
+        else:
+            cols.append(col)
+    return np.asfortranarray(np.stack(cols, axis=1))
+
+
+def merge(params, position_tol=0.2, shape_tol=0.1):
+    """merge(params; position_tol, shape_tol) (Fit.jl:217-240): splats whose centres lie within position_tol and
+    whose log scales agree within shape_tol become one with the summed density and the density-weighted mean of
+    the other rows; returns (params, groups), greedily in index order."""
+    P = np.asarray(params)
+    n = P.shape[1]
+    assigned = np.zeros(n, dtype=int)
+    groups = []
+    for i in range(n):
+        if assigned[i]:
+            continue
+        g = [i]
+        assigned[i] = len(groups) + 1
+        for j in range(i + 1, n):
+            if assigned[j]:
+                continue
+            if np.sqrt(np.sum((P[0:3, i] - P[0:3, j]) ** 2)) <= position_tol and np.max(np.abs(P[3:6, i] - P[3:6, j])) <= shape_tol:
+                g.append(j)
+                assigned[j] = assigned[i]
+        groups.append(g)
+    out = np.empty((P.shape[0], len(groups)))
+    for k, g in enumerate(groups):
+        w = np.exp(P[12, g])
+        out[:, k] = (P[:, g] * (w / w.sum())).sum(axis=1)
+        out[12, k] = np.log(w.sum())
+    return np.asfortranarray(out), groups
+
+
+def fit_loop(params, make_valgrad, stages, hygiene=None, callback=None):
+    """_fit_loop!(params, make_valgrad, stages; hygiene, callback) (Fit.jl:620-659): for each stage, Adam on the
+    free rows with the cosine-annealed rate; make_valgrad(stage, it) returns a function q -> (chi2, gradient);
+    every hygiene.every iterations prune, merge and densify (the splat count changes; the optimizer restarts).
+    Returns (params, history, events)."""
+    hygiene = hygiene or Hygiene()
+    params = np.asfortranarray(np.array(params, dtype=np.float64))
+    history, events = [], []
+    for si, st in enumerate(stages, start=1):
+        mask = np.ones(params.shape) if st.free is None else freeze(params, st.free).astype(np.float64)
+        scale = None if st.steps is None else step_scale(params, st.steps)
+        opt = Adam(params.shape, st.eta)
+        for it in range(1, st.iterations + 1):
+            eta = st.eta_end + (st.eta - st.eta_end) * (1 + np.cos(np.pi * (it - 1) / max(st.iterations - 1, 1))) / 2
+            opt.eta = eta
+            value, g = make_valgrad(st, it)(params)
+            g = np.asarray(g)
+            if hygiene.every > 0 and it % hygiene.every == 0:
+                before = params.shape[1]
+                q, kept = prune(params, hygiene.prune_fraction)
+                gk = g[:, kept]
+                q, groups = merge(q, hygiene.merge_position, hygiene.merge_shape)
+                gm = np.stack([gk[:, grp].sum(axis=1) for grp in groups], axis=1)
+                if q.shape[1] < hygiene.max_splats:
+                    q = densify(q, gm, hygiene.densify_threshold)
+                if q.shape != params.shape or not np.array_equal(q, params):
+                    params = q
+                    events.append((si, it, before, params.shape[1]))
+                    mask = np.ones(params.shape) if st.free is None else freeze(params, st.free).astype(np.float64)
+                    scale = None if st.steps is None else step_scale(params, st.steps)
+                    opt = Adam(params.shape, eta)
+                    continue
+            old = params
+            new = opt.step(params, g * mask)
+            params = np.asfortranarray(new if scale is None else old + scale * (new - old))
+            history.append(value)
+            if callback is not None:
+                callback(si, it, params, value)
+    return params, history, events
+
+
+def fit(params, movie, cache, L, stages, hygiene=None, rng=None, callback=None, nmax=-1, slab=0.0, binning=None, kind="thermal"):
+    """fit!(params, movie, cache, L, stages; hygiene, rng, callback, nmax, slab, gradient = :dual, binning)
+    (Fit.jl:366-404): the staged Adam schedule on the movie chi2 with the gradient of the device dual sweep
+    (chi2_gradient); minibatches of frames and frequencies drawn per iteration when a stage asks for them. The
+    parameters live on the host (hygiene changes their count) and a device model is built from them every
+    iteration. Returns (params, history, events)."""
+    rng = rng or np.random.default_rng()
+
+    def batches(st):
+        freqs_all = list(range(len(movie.nus))) if st.freqs is None else list(st.freqs)
+        if st.batch is None:
+            return list(range(len(movie.times))), freqs_all
+        frames = sorted(rng.permutation(len(movie.times))[:min(st.batch[0], len(movie.times))])
+        freqs = sorted(np.asarray(freqs_all)[rng.permutation(len(freqs_all))[:min(st.batch[1], len(freqs_all))]])
+        return frames, freqs
+
+    def make_valgrad(st, it):
+        frames, freqs = batches(st)
+
+        def valgrad(q):
+            m = Model(q, kind=kind)
+            g = np.zeros(q.shape, order="F")
+            chi = chi2_gradient(g, m, movie, cache, L, frames=frames, freqs=freqs, nmax=nmax, slab=slab, binning=binning)
+            return chi, g
+        return valgrad
+
+    return fit_loop(params, make_valgrad, stages, hygiene=hygiene, callback=callback)
