@@ -102,14 +102,15 @@ int ks_copy_samples(const ks_cache* cache, double* t, double* r, double* theta, 
 /* A splat parameter set on the device: the model argument of ThinRenderer / RadiativeTransport
  * (Splats.jl:105-111, polarized.jl:41-51, populations.jl:34-60, trajectories.jl:80-88).
  * params: nrows x ns column-major (14 thin, 21 thermal/knot, 22 power law/kappa);
- * gamma_max: power law; knots: 3 x nknots x ns, knot_times: nknots (knot splats). */
+ * gamma_max: power law; knots: 3 x nknots x ns, knot_times: nknots, increasing (knot splats, whose
+ * centres follow the knots as KnotSplats; thermal, power-law and kappa splats given knots follow them too). */
 int ks_model_create(ks_ctx* ctx, int kind, int nrows, int ns, const double* params, double gamma_max,
                     const double* knots, const double* knot_times, int nknots, ks_model** model);
 /* CompositeModel(a, b) - transport.jl:298-316: the elements of parts[0], then parts[1], ...; the
  * coefficients of overlapping elements add. Parts are thermal, power-law, kappa, knot or composite
  * models of the same context (thin splats are not composed; knot parts keep their own knots and knot
- * times); the splat constants and knots are copied, so the parts may be destroyed afterwards. Renders through ks_render like any population; the dual
- * sweep takes thermal splats only. */
+ * times); the splat constants and knots are copied, so the parts may be destroyed afterwards. Renders through ks_render like any
+ * population and differentiates through the dual sweep. */
 int ks_model_composite(ks_ctx* ctx, int nparts, const ks_model* const* parts, ks_model** model);
 int ks_model_destroy(ks_model* model);
 
@@ -144,13 +145,17 @@ int ks_ray_lists(ks_cache* cache, const ks_model* model, const double* times, in
 
 /* polarized_gradient!(dparams, dstokes, cache, params, t_obs, ν_obs, L; method = :dual, nmax, slab)
  * - polarized.jl:473-480 with polarized_tails! and polarized_dual_sweep! (dual_gradient.jl:180-249):
- * the gradient of l = Σ dstokes · I_obs over every (pixel, time, frequency) with respect to the
- * thermal splat parameters, ADDED into dparams (21 x ns, column-major, as the reference's
- * dparams .+= ...). dstokes and the optional image (the observed Stokes vectors of the same
- * forward pass) are 4 x npix x nt x nν, in screen order when screen_order, else in the cache's
- * sorted order. Needs stored samples (store_samples and a storing marcher): the forward transfer
- * records each sample's coefficients and the reverse pass re-uses them. Thermal splats only
- * (KS_ERR_ARG otherwise). Equal to ks_gradient_forward followed by ks_gradient_reverse. */
+ * the gradient of l = Σ dstokes · I_obs over every (pixel, time, frequency) with respect to every
+ * parameter row of every splat, ADDED into dparams (as the reference's dparams .+= ...): nrows x ns
+ * column-major for one population (21 thermal and knot, 22 power law and kappa), and for a composite
+ * its parts' blocks one after another. The reference's dual sweep covers thermal splats
+ * (dual_gradient.jl:46); here every population is covered with these semantics: the kappa 2F1 factor is
+ * differentiated with kappa and w (the reference holds it), and for splats whose centres follow knots
+ * the centre rows shift the whole trajectory and the pattern rate has no effect (zero). dstokes and the
+ * optional image (the observed Stokes vectors of the same forward pass) are 4 x npix x nt x nν, in
+ * screen order when screen_order, else in the cache's sorted order. Needs stored samples (store_samples
+ * and a storing marcher): the forward transfer records each sample's coefficients and the reverse pass
+ * re-uses them. Thin splats: KS_ERR_ARG. Equal to ks_gradient_forward followed by ks_gradient_reverse. */
 int ks_polarized_gradient(ks_cache* cache, const ks_model* model, const double* times, int nt, const double* nus, int nnu,
                           double L, int nmax, double slab, const double* dstokes, int screen_order,
                           double* dparams, double* image);

@@ -436,6 +436,7 @@ class Model:
                                          _d(gamma_max), _ptr(kn), _ptr(kt), _i(0 if kt is None else len(kt)), ctypes.byref(h)))
         self._h = h
         self.ns = P.shape[1]
+        self.grad_shape = P.shape
 
     def __del__(self):
         try:
@@ -465,6 +466,18 @@ class CompositeModel(Model):
         _check(self._lib.ks_model_composite(context(device), _i(len(parts)), arr, ctypes.byref(h)))
         self._h = h
         self.ns = sum(p.ns for p in parts)
+        # the gradient of a composite: its parts' nrows x ns blocks (column-major) one after another
+        self.grad_shape = (sum(int(np.prod(p.grad_shape)) for p in parts),)
+
+    def split_gradient(self, g):
+        """The parts' gradient blocks (each shaped like its part's parameters) of a composite gradient vector."""
+        out, off = [], 0
+        for p in self.parts:
+            n = int(np.prod(p.grad_shape))
+            blk = np.asarray(g)[off:off + n]
+            out.extend(p.split_gradient(blk) if isinstance(p, CompositeModel) else [blk.reshape(p.grad_shape, order="F")])
+            off += n
+        return out
 
 
 def _as_model(params, kind):
@@ -634,13 +647,14 @@ def march(cache):
 def polarized_gradient(cache, params, times, nus, L, dstokes, dparams=None, nmax=-1, slab=0.0, screen=True):
     """polarized_gradient!(dparams, dstokes, cache, params, t_obs, ν, L; method = :dual, nmax, slab)
     (polarized.jl:473-480, dual_gradient.jl:180-249): the gradient of sum(dstokes * I_obs) over every
-    pixel, time and frequency with respect to the thermal splat parameters, added into dparams
-    (21 x ns, allocated as zeros when None). dstokes has the shape polarized_cube returns
+    pixel, time and frequency with respect to the splat parameters (thermal, power-law, kappa, knot or composite
+    models), added into dparams (nrows x ns; for a composite the parts' blocks one after another as a vector,
+    split by CompositeModel.split_gradient; allocated as zeros when None). dstokes has the shape polarized_cube returns
     ((n_alpha, n_beta, nt, nnu, 4) with screen=True, (nnu, nt, npix, 4) in sorted order otherwise);
     returns (dparams, cube) with the cube of the same forward pass. Needs stored samples."""
     m = _as_model(params, "thermal")
-    if m.kind != "thermal":
-        raise ValueError("the dual sweep takes thermal splats (PolarizedSplats)")
+    if m.kind == "thin":
+        raise ValueError("the dual sweep takes polarized splats (thermal, power-law, kappa, knot or composite)")
     t, n = _f64(np.atleast_1d(times)), _f64(np.atleast_1d(nus))
     na, nb = cache.screen_size
     ds = np.asarray(dstokes, dtype=np.float64)
@@ -654,9 +668,9 @@ def polarized_gradient(cache, params, times, nus, L, dstokes, dparams=None, nmax
             raise DimensionMismatch(f"dstokes must be {(len(n), len(t), cache.npix, 4)}, got {ds.shape}")
         ds = np.ascontiguousarray(ds)
     if dparams is None:
-        dparams = np.zeros(m.params.shape, order="F")
-    elif dparams.shape != m.params.shape or not dparams.flags.f_contiguous or dparams.dtype != np.float64:
-        raise DimensionMismatch("dparams must be a Fortran-ordered float64 array shaped like the parameters")
+        dparams = np.zeros(m.grad_shape, order="F")
+    elif dparams.shape != m.grad_shape or not dparams.flags.f_contiguous or dparams.dtype != np.float64:
+        raise DimensionMismatch("dparams must be a Fortran-ordered float64 array shaped like the parameters (a vector for composites)")
     out = np.zeros((len(n), len(t), cache.npix, 4))
     _check(cache._lib.ks_polarized_gradient(cache._h, m._h, _ptr(t), _i(len(t)), _ptr(n), _i(len(n)), _d(L), _i(nmax), _d(slab),
                                             _ptr(ds), _i(int(screen)), _ptr(dparams), _ptr(out)))
@@ -682,8 +696,8 @@ def gradient_forward(cache, model, times, nus, L, nmax=-1, slab=0.0, copy=True):
     """The forward pass of sweep_passes(:dual) (Fit.jl:451-474): the transfer with the records the
     reverse pass needs; the observed images stay on the device. Returns them in sorted order,
     (nnu, nt, npix, 4), when copy, else None."""
-    if model.kind != "thermal":
-        raise ValueError("the dual sweep takes thermal splats (PolarizedSplats)")
+    if model.kind == "thin":
+        raise ValueError("the dual sweep takes polarized splats (thermal, power-law, kappa, knot or composite)")
     t, n = _f64(np.atleast_1d(times)), _f64(np.atleast_1d(nus))
     out = np.zeros((len(n), len(t), cache.npix, 4)) if copy else None
     _check(cache._lib.ks_gradient_forward(cache._h, model._h, _ptr(t), _i(len(t)), _ptr(n), _i(len(n)), _d(L), _i(nmax), _d(slab),
@@ -696,8 +710,8 @@ def gradient_reverse(cache, model, dparams, dstokes=None):
     """The reverse pass of sweep_passes(:dual): adds d(sum dstokes . I_obs)/dparams of the last
     forward pass into dparams (Fortran-ordered, shaped like the parameters). dstokes in sorted order,
     (nnu, nt, npix, 4); None takes the device seeds that frame_chi2_seed / image_chi2_seed wrote."""
-    if dparams.shape != model.params.shape or not dparams.flags.f_contiguous or dparams.dtype != np.float64:
-        raise DimensionMismatch("dparams must be a Fortran-ordered float64 array shaped like the parameters")
+    if dparams.shape != model.grad_shape or not dparams.flags.f_contiguous or dparams.dtype != np.float64:
+        raise DimensionMismatch("dparams must be a Fortran-ordered float64 array shaped like the parameters (a vector for composites)")
     ds = None
     if dstokes is not None:
         nt, nnu = getattr(cache, "_fw", (0, 0))

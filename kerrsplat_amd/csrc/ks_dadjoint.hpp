@@ -22,8 +22,11 @@ template <class S> struct SplatAdjT {
     S qn[4];          // normalised quaternion (w, x, y, z) of rows 7-10
     S iqn;            // 1 / |q|
     S dk0k2, dk1k2;   // d(K0/K2)/d ln Theta_e, d(K1/K2)/d ln Theta_e at 1/Theta_e
-    S sthB, cthB, sphB, cphB;   // field direction angles (rows 16, 17)
-    S pad[5];
+    S sthB, cthB, sphB, cphB;   // field direction angles
+    // derivatives of the per-splat tables with respect to the population rows (x1, x2) = (p, ln gamma_min) for
+    // power law, (kappa, ln w) for kappa: power law {dgPL0/dp, dgPL1/dp}; kappa {d2F1/dkappa, d2F1/d ln w,
+    // dGamma(k-4/3)/dk, dGamma(k-2)/dk, dGamma(k/4-1/3)/dk, dGamma(k/4+4/3)/dk, dGamma(2+k/2)/dk, d(K0/K2)/d ln w}
+    S dpop[8];
 };
 using SplatAdj = SplatAdjT<double>;
 using SplatAdjf = SplatAdjT<f32>;
@@ -103,6 +106,59 @@ KS_DEV bool weight_active(const SplatCT<S>& c, S t, S x, S y, S z) {
     return exp(-0.5 * gauss_exponent(c, t, x, y, z, sp, cp)) > WEIGHT_CUTOFF;
 }
 
+// rows 7-10 of d ln G: dlnG/dR[a][b] = e_a gu_b (R the rotation of the pattern offset e, gu = dlnG/du) through the
+// normalised quaternion (w, x, y, z)
+template <class S>
+KS_DEV void quat_dlog(const SplatAdjT<S>& sa, const S e[3], const S gu[3], S dl[13]) {
+    S GR[9];
+    for (int a_ = 0; a_ < 3; ++a_) for (int b = 0; b < 3; ++b) GR[3 * a_ + b] = e[a_] * gu[b];
+    const S qw = sa.qn[0], qx = sa.qn[1], qy = sa.qn[2], qz = sa.qn[3];
+    S gn[4];
+    gn[0] = 2.0 * (-qz * GR[1] + qy * GR[2] + qz * GR[3] - qx * GR[5] - qy * GR[6] + qx * GR[7]);
+    gn[1] = 2.0 * (qy * GR[1] + qz * GR[2] + qy * GR[3] - 2.0 * qx * GR[4] - qw * GR[5] + qz * GR[6] + qw * GR[7] - 2.0 * qx * GR[8]);
+    gn[2] = 2.0 * (-2.0 * qy * GR[0] + qx * GR[1] + qw * GR[2] + qx * GR[3] + qz * GR[5] - qw * GR[6] + qz * GR[7] - 2.0 * qy * GR[8]);
+    gn[3] = 2.0 * (-2.0 * qz * GR[0] - qw * GR[1] + qx * GR[2] + qw * GR[3] - 2.0 * qz * GR[4] + qy * GR[5] + qx * GR[6] + qy * GR[7]);
+    const S ng = qw * gn[0] + qx * gn[1] + qy * gn[2] + qz * gn[3];
+    for (int i = 0; i < 4; ++i) dl[6 + i] = (gn[i] - sa.qn[i] * ng) * sa.iqn;
+}
+
+// whether a splat whose centre follows knots passes the weight cutoff at the sample, by the arithmetic of
+// weight_dlog_knot
+template <class S>
+KS_DEV bool weight_active_knot(const SplatCT<S>& c, const S* knots, const S* ktimes, S t, S x, S y, S z) {
+    S cx, cy, cz;
+    knot_centre(c, knots, ktimes, t, cx, cy, cz);
+    const S e0 = x - cx, e1 = y - cy, e2 = z - cz;
+    S q2(0.0);
+    for (int b = 0; b < 3; ++b) { const S q = (c.R[b] * e0 + c.R[3 + b] * e1 + c.R[6 + b] * e2) * c.is[b]; q2 += q * q; }
+    const S tt = (t - c.t0) * c.iw;
+    return exp(-0.5 * (q2 + tt * tt)) > WEIGHT_CUTOFF;
+}
+
+// d ln G of a splat whose centre follows knots (trajectories.jl:93-128) with respect to rows 1-12: the centre rows
+// shift the whole trajectory (dlnG/dc at the interpolated centre), widths, orientation, t0 and temporal width as for
+// pattern splats; the pattern rate does not enter (dl[12] = 0). Returns the Gaussian (0 when culled).
+template <class S>
+KS_DEV S weight_dlog_knot(const SplatCT<S>& c, const SplatAdjT<S>& sa, const S* knots, const S* ktimes, S t, S x, S y, S z, S dl[13]) {
+    S cx, cy, cz;
+    knot_centre(c, knots, ktimes, t, cx, cy, cz);
+    const S e[3] = {x - cx, y - cy, z - cz};
+    S u[3], q[3];
+    for (int b = 0; b < 3; ++b) { u[b] = c.R[b] * e[0] + c.R[3 + b] * e[1] + c.R[6 + b] * e[2]; q[b] = u[b] * c.is[b]; }
+    const S tt = (t - c.t0) * c.iw;
+    const S G = exp(-0.5 * (q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + tt * tt));
+    if (!(G > WEIGHT_CUTOFF)) return S(0.0);
+    S gu[3], ge[3];
+    for (int b = 0; b < 3; ++b) { gu[b] = -q[b] * c.is[b]; dl[3 + b] = q[b] * q[b]; }
+    for (int a_ = 0; a_ < 3; ++a_) ge[a_] = c.R[3 * a_] * gu[0] + c.R[3 * a_ + 1] * gu[1] + c.R[3 * a_ + 2] * gu[2];
+    quat_dlog(sa, e, gu, dl);
+    for (int a_ = 0; a_ < 3; ++a_) dl[a_] = -ge[a_];
+    dl[10] = tt * c.iw;
+    dl[11] = tt * tt;
+    dl[12] = S(0.0);
+    return G;
+}
+
 // d ln G of the splat weight (Splats.jl:58-93) with respect to rows 1-12 and the pattern rate; the
 // Gaussian itself is returned (0 when culled). dl[0..11] = rows 1..12, dl[12] = omega.
 template <class S>
@@ -123,17 +179,7 @@ KS_DEV S weight_dlog(const SplatCT<S>& c, const SplatAdjT<S>& sa, S t, S x, S y,
     S gu[3], ge[3];
     for (int b = 0; b < 3; ++b) { gu[b] = -q[b] * c.is[b]; dl[3 + b] = q[b] * q[b]; }
     for (int a_ = 0; a_ < 3; ++a_) ge[a_] = c.R[3 * a_] * gu[0] + c.R[3 * a_ + 1] * gu[1] + c.R[3 * a_ + 2] * gu[2];
-    // rotation: dlnG/dR[a][b] = e_a gu_b, through the normalised quaternion (w, x, y, z)
-    S GR[9];
-    for (int a_ = 0; a_ < 3; ++a_) for (int b = 0; b < 3; ++b) GR[3 * a_ + b] = e[a_] * gu[b];
-    const S qw = sa.qn[0], qx = sa.qn[1], qy = sa.qn[2], qz = sa.qn[3];
-    S gn[4];
-    gn[0] = 2.0 * (-qz * GR[1] + qy * GR[2] + qz * GR[3] - qx * GR[5] - qy * GR[6] + qx * GR[7]);
-    gn[1] = 2.0 * (qy * GR[1] + qz * GR[2] + qy * GR[3] - 2.0 * qx * GR[4] - qw * GR[5] + qz * GR[6] + qw * GR[7] - 2.0 * qx * GR[8]);
-    gn[2] = 2.0 * (-2.0 * qy * GR[0] + qx * GR[1] + qw * GR[2] + qx * GR[3] + qz * GR[5] - qw * GR[6] + qz * GR[7] - 2.0 * qy * GR[8]);
-    gn[3] = 2.0 * (-2.0 * qz * GR[0] - qw * GR[1] + qx * GR[2] + qw * GR[3] - 2.0 * qz * GR[4] + qy * GR[5] + qx * GR[6] + qy * GR[7]);
-    const S ng = qw * gn[0] + qx * gn[1] + qy * gn[2] + qz * gn[3];
-    for (int i = 0; i < 4; ++i) dl[6 + i] = (gn[i] - sa.qn[i] * ng) * sa.iqn;
+    quat_dlog(sa, e, gu, dl);
     // centre: dlnG/dc = -Rz(ph) ge, c = Rz(ph) (c0, c1) + c2 e_z
     const S gd0 = ge[0] * cp - ge[1] * sp, gd1 = ge[0] * sp + ge[1] * cp;
     dl[0] = -(gd0 * cp + gd1 * sp);
@@ -269,6 +315,64 @@ KS_DEV void element_adjoint_thermal(const SplatCT<S>& c, const SplatAdjT<S>& sa,
     grad[14] += q.d[1];
     for (int k = 0; k < 5; ++k) grad[15 + k] += q.d[2] * fr.dg[k] + q.d[3] * fr.dcosB[k] + qchi * fr.dchi[k];
     grad[20] += sv * dl[12];
+}
+
+// One power-law or kappa splat's share of one sample (the element of PowerLawSplats / KappaSplats,
+// populations.jl:13-104, differentiated as element_adjoint_thermal): the 22 rows are the 12 geometric rows, ln n_e,
+// the two population rows (p, ln gamma_min or kappa, ln w), ln B, the field angles, u and the pattern rate. The
+// contribution q is taken in five-partial duals over (x1, x2, ln B, g, cos B); the per-splat tables (Gamma
+// products, the kappa 2F1 factor and Bessel ratio) enter with their derivatives in x1, x2 (dpop), so that the
+// 2F1 factor is differentiated with kappa and w, not held; the frame rows join by the chain rule through
+// frame_jacobian.
+template <class S>
+KS_DEV void element_adjoint_pop(const SplatCT<S>& c, const SplatAdjT<S>& sa, const GeoT<S>& G, S a, S alpha, S beta, S thetao_sin,
+                                S nu, const S bars[11], S Gw, const S dl[13], S grad[22]) {
+    const S uv[3] = {c.u[0], c.u[1], c.u[2]};
+    const FrameJ<S> fr = frame_jacobian(G, a, alpha, beta, thetao_sin, uv, sa.sthB, sa.cthB, sa.sphB, sa.cphB);
+    using D5 = DN<5, S>;
+    D5 Bm(c.Bmag); Bm.d[2] = c.Bmag;
+    const D5 g = D5::seed(fr.g, 3), cosB = D5::seed(fr.cosB, 4);
+    D5 nuf = nu / g;
+    D5 cb = dmin(dmax(cosB, -1.0), 1.0);
+    D5 sth, cth;
+    pitch_sincos(cb, sth, cth);
+    const D5 ne(exp(c.lnA) * Gw);
+    CoefT<D5> cf;
+    if (c.kind == MODEL_POWERLAW) {
+        const D5 p = D5::seed(c.p1, 0);
+        D5 gmin(c.p2); gmin.d[1] = c.p2;
+        D5 gPL[2];
+        for (int k = 0; k < 2; ++k) { gPL[k] = D5(c.g[k]); gPL[k].d[0] = sa.dpop[k]; }
+        cf = powerlaw(ne, p, gmin, D5(c.p3), gPL, Bm, nuf, sth, cth);
+    } else {
+        const D5 kap = D5::seed(c.p1, 0);
+        D5 w(c.p2); w.d[1] = c.p2;
+        D5 hyp(c.p3); hyp.d[0] = sa.dpop[0]; hyp.d[1] = sa.dpop[1];
+        D5 gK[6];
+        for (int k = 0; k < 5; ++k) { gK[k] = D5(c.g[k]); gK[k].d[0] = sa.dpop[2 + k]; }
+        gK[5] = D5(c.g[5]); gK[5].d[1] = sa.dpop[7];
+        cf = kappa(ne, kap, w, hyp, gK, Bm, nuf, sth, cth);
+    }
+    if (!(cf.jI > 0.0 || cf.aI > 0.0 || cf.rQ.v != 0.0 || cf.rV.v != 0.0)) return;
+    D5 fe = cap_factor(cf.jI, cf.jQ, cf.jV), fa = cap_factor(cf.aI, cf.aQ, cf.aV);
+    D5 nh = nuf * (1.0 / NU0);
+    D5 i2 = 1.0 / (nh * nh);
+    S s2c, c2c;
+    sincos(2.0 * fr.chi, &s2c, &c2c);
+    D5 jQ = fe * cf.jQ * i2, aQ = fa * cf.aQ * nh, rQ = cf.rQ * nh;
+    D5 A = bars[0] * (cf.jI * i2) + bars[3] * (fe * cf.jV * i2) + bars[4] * (cf.aI * nh) + bars[7] * (fa * cf.aV * nh) + bars[10] * (cf.rV * nh);
+    D5 Bc = bars[1] * jQ + bars[5] * aQ + bars[8] * rQ;
+    D5 Bs = bars[2] * jQ + bars[6] * aQ + bars[9] * rQ;
+    D5 q = A + Bc * c2c + Bs * s2c;
+    const S qchi = 2.0 * (Bs.v * c2c - Bc.v * s2c);
+    const S sv = q.v;
+    for (int r = 0; r < 12; ++r) grad[r] += sv * dl[r];
+    grad[12] += sv;
+    grad[13] += q.d[0];
+    grad[14] += q.d[1];
+    grad[15] += q.d[2];
+    for (int k = 0; k < 5; ++k) grad[16 + k] += q.d[3] * fr.dg[k] + q.d[4] * fr.dcosB[k] + qchi * fr.dchi[k];
+    grad[21] += sv * dl[12];
 }
 
 }  // namespace ksd

@@ -186,15 +186,15 @@ KSDD_T KS_DEV DN<N, S> pow(const DN<N, S>& x, double y) {
     return chain(p, y == 0.0 ? S(0.0) : S(y * pow(x.v, y - 1.0)), x);
 }
 KSDD_T KS_DEV DN<N, S> pow(const DN<N, S>& x, const DN<N, S>& y) {
-    const S p = ::pow(x.v, y.v);
-    const S dx = y.v == 0.0 ? S(0.0) : S(y.v * ::pow(x.v, y.v - 1.0)), dy = x.v > 0.0 ? S(p * ::log(x.v)) : S(0.0);
+    const S p = pow(x.v, y.v);
+    const S dx = y.v == 0.0 ? S(0.0) : S(y.v * pow(x.v, y.v - 1.0)), dy = x.v > 0.0 ? S(p * log(x.v)) : S(0.0);
     DN<N, S> r; r.v = p;
     KSDD_LOOP r.d[i] = dx * x.d[i] + dy * y.d[i];
     return r;
 }
 KSDD_T KS_DEV DN<N, S> pow(double x, const DN<N, S>& y) {
-    const S p = ::pow(x, y.v);
-    return chain(p, x > 0.0 ? S(p * ::log(x)) : S(0.0), y);
+    const S p = pow(x, y.v);
+    return chain(p, x > 0.0 ? S(p * log(x)) : S(0.0), y);
 }
 KSDD_T KS_DEV DN<N, S> dmax(const DN<N, S>& a, const DN<N, S>& b) { return a.v > b.v ? a : b; }
 KSDD_T KS_DEV DN<N, S> dmin(const DN<N, S>& a, const DN<N, S>& b) { return a.v < b.v ? a : b; }

@@ -42,7 +42,9 @@ template <class S> struct SplatCT {
                           // kappa: kappa, w, hyp
     S g[6];               // power law: Gamma((3p-1)/12) Gamma((3p+19)/12), Gamma((3p+2)/12) Gamma((3p+22)/12);
                           // kappa: Gamma(k-4/3), Gamma(k-2), Gamma(k/4-1/3), Gamma(k/4+4/3), Gamma(2+k/2), K0/(K2+1e-40) at 1/w
-    int knot0, nk, kt0;   // knot splats: the first of its nk knots (xyz triples) and of their times in the model's arrays
+    S kc[3], kr;          // splats whose centre follows knots: a sphere holding every knot (centre, radius; kr = 0 without)
+    int knot0, nk, kt0;   // splats whose centre follows knots (nk > 0): the first of its nk knots (xyz triples) and of their
+                          // times in the model's arrays
     int kind;             // the splat's population (MODEL_THERMAL, MODEL_POWERLAW, MODEL_KAPPA, ...)
 };
 using SplatC = SplatCT<double>;
@@ -75,6 +77,14 @@ KS_DEV bool outside_support(const SplatCT<S>& s, S t, S x, S y, S z, S& sp, S& c
 // can be inside the support sphere (outside_support), with a margin for rounding.
 template <class S>
 KS_DEV bool splat_may_reach(const SplatCT<S>& c, double px, double py, double pz, double R, double tmin, double tmax) {
+    if (c.nk > 0) {
+        // a centre that follows knots stays inside the sphere holding the knots (linear interpolation between
+        // them, held at the ends): no arc term
+        const double dx = px - val(c.kc[0]), dy = py - val(c.kc[1]), dz = pz - val(c.kc[2]);
+        const double d = sqrt(dx * dx + dy * dy + dz * dz);
+        const double reach = R + sqrt(val(c.rsup2)) + val(c.kr);
+        return d <= reach + 1e-4 * (reach + d) + 1e-6;
+    }
     const double x = val(c.x), y = val(c.y), z = val(c.z), w = val(c.omega);
     const double tm = 0.5 * (tmin + tmax), h = 0.5 * fabs(w) * (tmax - tmin);
     double sp, cp;
@@ -103,6 +113,25 @@ KS_DEV S gauss_exponent(const SplatCT<S>& s, S t, S x, S y, S z, S sp, S cp) {
     S q0 = u0 * s.is[0], q1 = u1 * s.is[1], q2 = u2 * s.is[2];
     S tt = (t - s.t0) * s.iw;
     return q0 * q0 + q1 * q1 + q2 * q2 + tt * tt;
+}
+
+// The centre of a splat that follows knots at emission time t (trajectories.jl:93-111): linear interpolation
+// between the knots around t, held at the first and last knot outside their times.
+template <class S>
+KS_DEV void knot_centre(const SplatCT<S>& sc, const S* __restrict__ knots, const S* __restrict__ ktimes, S t, S& cx, S& cy, S& cz) {
+    const S* K = knots + 3 * (int64_t)sc.knot0;
+    const S* kt = ktimes + sc.kt0;
+    const int nk = sc.nk;
+    if (t <= kt[0]) { cx = K[0]; cy = K[1]; cz = K[2]; }
+    else if (t >= kt[nk - 1]) { cx = K[3 * (nk - 1)]; cy = K[3 * (nk - 1) + 1]; cz = K[3 * (nk - 1) + 2]; }
+    else {
+        int kk = 0;
+        while (kk < nk - 2 && kt[kk + 1] <= t) ++kk;
+        S w = (t - kt[kk]) / (kt[kk + 1] - kt[kk]);
+        cx = (1.0 - w) * K[3 * kk] + w * K[3 * (kk + 1)];
+        cy = (1.0 - w) * K[3 * kk + 1] + w * K[3 * (kk + 1) + 1];
+        cz = (1.0 - w) * K[3 * kk + 2] + w * K[3 * (kk + 1) + 2];
+    }
 }
 
 // ---- metric, ZAMO tetrad, photon momentum ----------------------------------------------------------------------

@@ -171,11 +171,15 @@ __global__ void __launch_bounds__(64) ks_unit_bounds(int64_t npix, int N, Sample
 // through it. Every pair of a wavefront belongs to
 // splat i, so the sums stay per lane in registers; the queue order is fixed, so the result is
 // deterministic.
-template <class S>
+// POPS: the populations the launch meets, POP_THERMAL (thermal and knot splats), POP_NONTHERMAL (power law and
+// kappa), POP_KNOTS (centres that follow knots); each instantiation carries only those branches, whose registers
+// would otherwise cost the common single-population launches occupancy.
+template <class S, int POPS>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(Waves<S>::adjelem))) ks_adj_splats(double a, double thetao, const double* __restrict__ soa, int64_t npix, int N, Samples smp,
                                                     const SplatCT<S>* __restrict__ sp, const SplatAdjT<S>* __restrict__ sadj, int ns,
+                                                    const S* __restrict__ knots, const S* __restrict__ ktimes,
                                                     const double* __restrict__ times, int nt, const double* __restrict__ nus, int64_t nitems,
-                                                    const S* __restrict__ bar, const uint8_t* __restrict__ aflg, int64_t nunits,
+                                                    const S* __restrict__ bar, const uint8_t* __restrict__ aflg, int64_t unit0, int64_t nb,
                                                     const double* __restrict__ ub, double* __restrict__ part) {
     constexpr int WALK = 8;                 // samples tested per group, their loads issued together
     constexpr int QCAP = 1024;              // ring capacity: < 64 pending + 64 per sample of a group
@@ -185,19 +189,22 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(Waves<S
     __shared__ int s_q[QCAP];   // entry = lane | k << 6
     // XCD-aware order: the hardware deals consecutive blocks round-robin over the 8 XCDs, so block b runs on
     // XCD b % 8; all ns splats of a unit go to one XCD (and its L2, which then reads the unit's samples once)
+    // this launch covers the units unit0 .. unit0 + nb - 1 (a batch: the partial sums of a batch are bounded by
+    // nb x ns x ADJ_ROWS doubles); `local` numbers the unit within the batch
     int i;
-    int64_t unit;
+    int64_t unit, local;
     {
         const int64_t b = blockIdx.x, q = b / 8, x = b % 8;
         i = (int)(q % ns);
-        unit = (q / ns) * 8 + x;
-        if (unit >= nunits) return;       // the padding of the last group of 8 units
+        local = (q / ns) * 8 + x;
+        if (local >= nb) return;          // the padding of the last group of 8 units
+        unit = unit0 + local;
     }
     // the per-unit cull: a splat that cannot reach any step-taking sample of the unit adds zeros
     {
         const double* u = ub + UB_FIELDS * unit;
         if (!(u[6] > 0.0) || !splat_may_reach(sp[i], u[0], u[1], u[2], u[3], u[4], u[5])) {
-            if (threadIdx.x < ADJ_ROWS) part[(unit * ns + i) * ADJ_ROWS + threadIdx.x] = 0.0;
+            if (threadIdx.x < ADJ_ROWS) part[(local * ns + i) * ADJ_ROWS + threadIdx.x] = 0.0;
             return;
         }
     }
@@ -243,7 +250,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(Waves<S
             const int64_t fo = (int64_t)(k - 1) * nitems + it2;
             const int64_t so = (int64_t)(k - 1) * npix + it2 % npix;
             S dl[13];
-            const S Gw = weight_dlog(c, sa, S(s_tobs[l] - smp.t[so]), S(smp.x[so]), S(smp.y[so]), S(smp.z[so]), dl);
+            const S tt = S(s_tobs[l] - smp.t[so]), xx = S(smp.x[so]), yy = S(smp.y[so]), zz = S(smp.z[so]);
+            S Gw;
+            if constexpr ((POPS & POP_KNOTS) != 0) Gw = c.nk > 0 ? weight_dlog_knot(c, sa, knots, ktimes, tt, xx, yy, zz, dl) : weight_dlog(c, sa, tt, xx, yy, zz, dl);
+            else Gw = weight_dlog(c, sa, tt, xx, yy, zz, dl);
             if (Gw > 0.0) {
                 const S* o = bar + fo;
                 S bars[BAR_FIELDS];
@@ -252,7 +262,15 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(Waves<S
                 Geo Gd;
                 make_geo(a, smp.r[so], smp.th[so], s_eta[l], s_lam[l], (fl & 2) != 0, (fl & 4) != 0, Gd);
                 const GeoT<S> G = geo_as<S>(Gd);
-                element_adjoint_thermal(c, sa, G, aS, S(s_alpha[l]), S(s_beta[l]), tos, S(s_nu[l]), bars, Gw, dl, grad);
+                constexpr int both = POP_THERMAL | POP_NONTHERMAL;
+                if constexpr ((POPS & both) == POP_THERMAL)
+                    element_adjoint_thermal(c, sa, G, aS, S(s_alpha[l]), S(s_beta[l]), tos, S(s_nu[l]), bars, Gw, dl, grad);
+                else if constexpr ((POPS & both) == POP_NONTHERMAL)
+                    element_adjoint_pop(c, sa, G, aS, S(s_alpha[l]), S(s_beta[l]), tos, S(s_nu[l]), bars, Gw, dl, grad);
+                else if (c.kind == MODEL_POWERLAW || c.kind == MODEL_KAPPA)
+                    element_adjoint_pop(c, sa, G, aS, S(s_alpha[l]), S(s_beta[l]), tos, S(s_nu[l]), bars, Gw, dl, grad);
+                else
+                    element_adjoint_thermal(c, sa, G, aS, S(s_alpha[l]), S(s_beta[l]), tos, S(s_nu[l]), bars, Gw, dl, grad);
             }
         }
         qhead = (qhead + cnt) & (QCAP - 1);
@@ -274,7 +292,12 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(Waves<S
         }
 #pragma unroll
         for (int u = 0; u < WALK; ++u) {
-            const bool act = fl[u] && weight_active(c, S(t_obs - tk[u]), S(xk[u]), S(yk[u]), S(zk[u]));
+            bool act;
+            if constexpr ((POPS & POP_KNOTS) != 0)
+                act = fl[u] && (c.nk > 0 ? weight_active_knot(c, knots, ktimes, S(t_obs - tk[u]), S(xk[u]), S(yk[u]), S(zk[u]))
+                                         : weight_active(c, S(t_obs - tk[u]), S(xk[u]), S(yk[u]), S(zk[u])));
+            else
+                act = fl[u] && weight_active(c, S(t_obs - tk[u]), S(xk[u]), S(yk[u]), S(zk[u]));
             const uint64_t m = __ballot(act);
             if (act) {
                 const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
@@ -300,21 +323,23 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(Waves<S
         gsum[r] = v;
     }
     if (threadIdx.x == 0) {
-        double* o = part + (unit * ns + i) * ADJ_ROWS;
+        double* o = part + (local * ns + i) * ADJ_ROWS;
         for (int r = 0; r < ADJ_ROWS; ++r) o[r] = gsum[r];
     }
 }
 
-// dparams[r + 21 i] = the sum over the work units in a fixed order (deterministic): one wavefront per
-// (splat, row), lane l summing units l, l + 64, ..., then a fixed butterfly over the lanes
-__global__ void __launch_bounds__(64) ks_adj_reduce(const double* __restrict__ part, int64_t nunits, int ns, double* __restrict__ dparams) {
+// dparams[r + ADJ_ROWS i] = the sum over a batch's work units in a fixed order (deterministic): one wavefront per
+// (splat, row), lane l summing units l, l + 64, ..., then a fixed butterfly over the lanes; added to the sums of
+// the earlier batches when accumulate (the batches in order)
+__global__ void __launch_bounds__(64) ks_adj_reduce(const double* __restrict__ part, int64_t nunits, int ns, int accumulate,
+                                                    double* __restrict__ dparams) {
     const int64_t t = blockIdx.x;
     if (t >= (int64_t)ns * ADJ_ROWS) return;
     const int i = (int)(t / ADJ_ROWS), r = (int)(t % ADJ_ROWS);
     double s = 0.0;
     for (int64_t b = threadIdx.x; b < nunits; b += 64) s += part[(b * ns + i) * ADJ_ROWS + r];
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if (threadIdx.x == 0) dparams[t] = s;
+    if (threadIdx.x == 0) dparams[t] = accumulate ? dparams[t] + s : s;
 }
 
 // ---- launchers ----
@@ -337,29 +362,46 @@ hipError_t launch_adj_segment_f32(hipStream_t s, int N, int64_t nitems, const do
     hipLaunchKernelGGL(ks_adj_segment<f32>, dim3(grid_for(nitems * nch, 128)), dim3(128), 0, s, N, nitems, nch, wT, rec, bar, aflg, tails, smp, npix);
     return hipGetLastError();
 }
-hipError_t launch_adj_splats(hipStream_t s, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
-                             const SplatC* sp, const SplatAdj* sadj, int ns, const double* times, int nt, const double* nus,
-                             int64_t nitems, const double* bar, const uint8_t* aflg, const double* ub, double* part) {
-    const int64_t nunits = adj_units(nitems, N);
-    hipLaunchKernelGGL(ks_adj_splats<double>, dim3((unsigned)((nunits + 7) / 8 * 8 * ns)), dim3(64), 0, s, a, thetao, soa, npix, N, smp, sp, sadj, ns, times,
-                       nt, nus, nitems, bar, aflg, nunits, ub, part);
+template <class S>
+static hipError_t adj_splats_k(hipStream_t s, int pops, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
+                               const SplatCT<S>* sp, const SplatAdjT<S>* sadj, int ns, const S* knots, const S* ktimes,
+                               const double* times, int nt, const double* nus, int64_t nitems, const S* bar, const uint8_t* aflg,
+                               int64_t unit0, int64_t nb, const double* ub, double* part) {
+    const dim3 grid((unsigned)((nb + 7) / 8 * 8 * ns)), block(64);
+#define KS_ADJ_LAUNCH(P) hipLaunchKernelGGL((ks_adj_splats<S, P>), grid, block, 0, s, a, thetao, soa, npix, N, smp, sp, sadj, ns, knots, ktimes, \
+                                            times, nt, nus, nitems, bar, aflg, unit0, nb, ub, part)
+    switch (pops) {
+        case POP_THERMAL: KS_ADJ_LAUNCH(POP_THERMAL); break;
+        case POP_NONTHERMAL: KS_ADJ_LAUNCH(POP_NONTHERMAL); break;
+        case POP_THERMAL | POP_NONTHERMAL: KS_ADJ_LAUNCH(POP_THERMAL | POP_NONTHERMAL); break;
+        case POP_THERMAL | POP_KNOTS: KS_ADJ_LAUNCH(POP_THERMAL | POP_KNOTS); break;
+        case POP_NONTHERMAL | POP_KNOTS: KS_ADJ_LAUNCH(POP_NONTHERMAL | POP_KNOTS); break;
+        default: KS_ADJ_LAUNCH(POP_THERMAL | POP_NONTHERMAL | POP_KNOTS); break;
+    }
+#undef KS_ADJ_LAUNCH
     return hipGetLastError();
 }
-hipError_t launch_adj_splats_f32(hipStream_t s, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
-                                 const SplatCf* sp, const SplatAdjf* sadj, int ns, const double* times, int nt, const double* nus,
-                                 int64_t nitems, const f32* bar, const uint8_t* aflg, const double* ub, double* part) {
-    const int64_t nunits = adj_units(nitems, N);
-    hipLaunchKernelGGL(ks_adj_splats<f32>, dim3((unsigned)((nunits + 7) / 8 * 8 * ns)), dim3(64), 0, s, a, thetao, soa, npix, N, smp, sp, sadj, ns, times,
-                       nt, nus, nitems, bar, aflg, nunits, ub, part);
-    return hipGetLastError();
+hipError_t launch_adj_splats(hipStream_t s, int pops, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
+                             const SplatC* sp, const SplatAdj* sadj, int ns, const double* knots, const double* ktimes,
+                             const double* times, int nt, const double* nus, int64_t nitems, const double* bar, const uint8_t* aflg,
+                             int64_t unit0, int64_t nb, const double* ub, double* part) {
+    return adj_splats_k<double>(s, pops, a, thetao, soa, npix, N, smp, sp, sadj, ns, knots, ktimes, times, nt, nus, nitems, bar, aflg, unit0, nb,
+                                ub, part);
+}
+hipError_t launch_adj_splats_f32(hipStream_t s, int pops, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
+                                 const SplatCf* sp, const SplatAdjf* sadj, int ns, const f32* knots, const f32* ktimes,
+                                 const double* times, int nt, const double* nus, int64_t nitems, const f32* bar, const uint8_t* aflg,
+                                 int64_t unit0, int64_t nb, const double* ub, double* part) {
+    return adj_splats_k<f32>(s, pops, a, thetao, soa, npix, N, smp, sp, sadj, ns, knots, ktimes, times, nt, nus, nitems, bar, aflg, unit0, nb, ub,
+                             part);
 }
 hipError_t launch_unit_bounds(hipStream_t s, int64_t npix, int N, Samples smp, const double* times, int nt, int64_t nitems,
                               const uint8_t* aflg, double* ub) {
     hipLaunchKernelGGL(ks_unit_bounds, dim3((unsigned)adj_units(nitems, N)), dim3(64), 0, s, npix, N, smp, times, nt, nitems, aflg, ub);
     return hipGetLastError();
 }
-hipError_t launch_adj_reduce(hipStream_t s, const double* part, int64_t nunits, int ns, double* dparams) {
-    hipLaunchKernelGGL(ks_adj_reduce, dim3((unsigned)(ns * ADJ_ROWS)), dim3(64), 0, s, part, nunits, ns, dparams);
+hipError_t launch_adj_reduce(hipStream_t s, const double* part, int64_t nunits, int ns, int accumulate, double* dparams) {
+    hipLaunchKernelGGL(ks_adj_reduce, dim3((unsigned)(ns * ADJ_ROWS)), dim3(64), 0, s, part, nunits, ns, accumulate, dparams);
     return hipGetLastError();
 }
 

@@ -28,24 +28,12 @@ KS_DEV TransCtxT<S> make_ctx(double a, double rp, double rm, double thetao_sin, 
     return c;
 }
 
-// A knot splat's Gaussian exponent at (t, x, y, z): its centre interpolated linearly between its knots
-// (trajectories.jl:93-128), held at the first and last knot outside their times; no pattern rotation.
+// The Gaussian exponent at (t, x, y, z) of a splat whose centre follows knots (trajectories.jl:93-128): the offset
+// from the knot-interpolated centre (knot_centre), no pattern rotation.
 template <class S>
 KS_DEV S knot_gexp(const TransCtxT<S>& c, const SplatCT<S>& sc, S t, S x, S y, S z) {
-    const S* K = c.knots + 3 * (int64_t)sc.knot0;
-    const S* kt = c.ktimes + sc.kt0;
-    const int nk = sc.nk;
     S cx, cy, cz;
-    if (t <= kt[0]) { cx = K[0]; cy = K[1]; cz = K[2]; }
-    else if (t >= kt[nk - 1]) { cx = K[3 * (nk - 1)]; cy = K[3 * (nk - 1) + 1]; cz = K[3 * (nk - 1) + 2]; }
-    else {
-        int kk = 0;
-        while (kk < nk - 2 && kt[kk + 1] <= t) ++kk;
-        S w = (t - kt[kk]) / (kt[kk + 1] - kt[kk]);
-        cx = (1.0 - w) * K[3 * kk] + w * K[3 * (kk + 1)];
-        cy = (1.0 - w) * K[3 * kk + 1] + w * K[3 * (kk + 1) + 1];
-        cz = (1.0 - w) * K[3 * kk + 2] + w * K[3 * (kk + 1) + 2];
-    }
+    knot_centre(sc, c.knots, c.ktimes, t, cx, cy, cz);
     S d0 = x - cx, d1 = y - cy, d2 = z - cz;
     S u0 = sc.R[0] * d0 + sc.R[3] * d1 + sc.R[6] * d2;
     S u1 = sc.R[1] * d0 + sc.R[4] * d1 + sc.R[7] * d2;
@@ -72,7 +60,7 @@ KS_DEV bool sample_coefficients(const TransCtxT<S>& c, double st, double r, doub
         const int i = cand ? (int)cand[ci] : ci;
         const SplatCT<S>& sc = c.sp[i];
         S gexp;
-        if (KIND == MODEL_KNOT || (KIND == MODEL_COMPOSITE && sc.kind == MODEL_KNOT)) {
+        if (KIND == MODEL_KNOT || sc.nk > 0) {
             gexp = knot_gexp(c, sc, t, x, y, z);
         } else {
             S sph, cph;
@@ -201,7 +189,7 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(Waves<
     const int64_t st = (int64_t)N * nitems;
     // the per-tile cull: the splats that can reach the wavefront's samples of this chunk, in ascending
     // order, as a candidate list in LDS (the rest add exact zeros to every sample of the tile)
-    constexpr bool cull = CULL && KIND != MODEL_KNOT;     // the launcher picks CULL for CULL_MIN_NS <= ns <= CAND_MAX
+    constexpr bool cull = CULL;     // the launcher picks CULL for CULL_MIN_NS <= ns <= CAND_MAX
     __shared__ uint16_t s_cand[2][cull ? CAND_MAX : 1];
     const int wv = (int)(threadIdx.x / 64), lane = (int)(threadIdx.x % 64);
     int ncand = 0;
@@ -399,8 +387,8 @@ static void chunk_transfer_k(hipStream_t s, double a, double thetao, const doubl
                              const double* nus, int nnu, double L, const int* kstop, double* seg, S* rec, uint8_t* aflg) {
     const int nch = (N + CHUNK - 1) / CHUNK;
     const int64_t items = npix * nt * nnu;
-    // the per-tile cull bounds pattern-rotating splats: not for knot splats (alone, or in a composite: nknots > 0)
-    if (ns >= CULL_MIN_NS && ns <= CAND_MAX && !(KIND == MODEL_COMPOSITE && nknots > 0))
+    // the per-tile cull bounds pattern-rotating splats by their arc and knot splats by the sphere holding their knots
+    if (ns >= CULL_MIN_NS && ns <= CAND_MAX)
         hipLaunchKernelGGL((ks_chunk_transfer<KIND, S, true>), dim3(grid_for(items * nch, 128)), dim3(128), 0, s, a, thetao, soa, npix, N, smp, sp,
                            ns, knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, nch, seg, rec, aflg);
     else

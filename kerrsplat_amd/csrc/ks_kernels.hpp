@@ -20,7 +20,9 @@ namespace ksd {
 constexpr int PART_BLOCK = 256;    // block of the case partition
 constexpr int CHUNK = 32;          // samples per work item of the transfer from stored samples
 constexpr int SEG_FIELDS = 20;     // a segment map: P (16, row-major), then Sv (4)
-constexpr int ADJ_ROWS = 21;       // parameter rows of a thermal splat
+// the populations an element-adjoint launch meets (a bit mask)
+constexpr int POP_THERMAL = 1, POP_NONTHERMAL = 2, POP_KNOTS = 4;
+constexpr int ADJ_ROWS = 22;       // gradient rows per splat (21 thermal and knot, 22 power law and kappa)
 constexpr int ADJ_SLICE = 32;      // samples per work unit of the element adjoints
 constexpr int CAND_MAX = 1024;     // splats a tile's candidate list holds (more splats: no per-tile cull)
 constexpr int CULL_MIN_NS = 64;    // the transfer culls per tile from this many splats on (below, the per-sample test is cheaper)
@@ -216,13 +218,15 @@ hipError_t launch_adj_segment(hipStream_t s, int N, int64_t nitems, const double
                               double* tails, Samples smp, int64_t npix);
 hipError_t launch_adj_segment_f32(hipStream_t s, int N, int64_t nitems, const double* wT, const f32* rec, f32* bar, const uint8_t* aflg,
                                   f32* tails, Samples smp, int64_t npix);
-hipError_t launch_adj_splats_f32(hipStream_t s, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
-                                 const SplatCf* sp, const SplatAdjf* sadj, int ns, const double* times, int nt, const double* nus,
-                                 int64_t nitems, const f32* bar, const uint8_t* aflg, const double* ub, double* part);
-hipError_t launch_adj_splats(hipStream_t s, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
-                             const SplatC* sp, const SplatAdj* sadj, int ns, const double* times, int nt, const double* nus,
-                             int64_t nitems, const double* bar, const uint8_t* aflg, const double* ub, double* part);
-hipError_t launch_adj_reduce(hipStream_t s, const double* part, int64_t nunits, int ns, double* dparams);
+hipError_t launch_adj_splats_f32(hipStream_t s, int pops, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
+                                 const SplatCf* sp, const SplatAdjf* sadj, int ns, const f32* knots, const f32* ktimes,
+                                 const double* times, int nt, const double* nus, int64_t nitems, const f32* bar, const uint8_t* aflg,
+                                 int64_t unit0, int64_t nb, const double* ub, double* part);
+hipError_t launch_adj_splats(hipStream_t s, int pops, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
+                             const SplatC* sp, const SplatAdj* sadj, int ns, const double* knots, const double* ktimes,
+                             const double* times, int nt, const double* nus, int64_t nitems, const double* bar, const uint8_t* aflg,
+                             int64_t unit0, int64_t nb, const double* ub, double* part);
+hipError_t launch_adj_reduce(hipStream_t s, const double* part, int64_t nunits, int ns, int accumulate, double* dparams);
 hipError_t launch_unit_bounds(hipStream_t s, int64_t npix, int N, Samples smp, const double* times, int nt, int64_t nitems,
                               const uint8_t* aflg, double* ub);
 

@@ -160,6 +160,8 @@ struct ks_model {
     uint64_t id = 0;                                   // unique per created model: a model freed and re-created at the same
                                                        // address is not the model of an earlier forward pass
     int kind = KS_MODEL_THERMAL, nrows = 0, ns = 0;
+    std::vector<int> hrows;                            // parameter rows of every splat (21 thermal/knot, 22 power law/kappa)
+    int pops = 0;                                      // the populations it holds (POP_THERMAL | POP_NONTHERMAL | POP_KNOTS)
     SplatC* d_splats = nullptr;
     SplatAdj* d_adj = nullptr;                         // thermal splats: constants of the adjoint
     double* d_knots = nullptr; double* d_ktimes = nullptr; int nknots = 0;
@@ -433,6 +435,25 @@ static double host_hyp2f1_neg(double a, double b, double c, double z) {
     return std::pow(1.0 - z, -a) * sum;
 }
 
+// digamma for x > 0: the recurrence psi(x) = psi(x + 1) - 1/x up to x >= 10, then the asymptotic series
+// ln x - 1/(2x) - sum B_2k / (2k x^2k) (DLMF 5.5.2, 5.11.2)
+static double host_digamma(double x) {
+    double r = 0.0;
+    while (x < 10.0) { r -= 1.0 / x; x += 1.0; }
+    const double i2 = 1.0 / (x * x);
+    const double ser = i2 * (1.0 / 12.0 - i2 * (1.0 / 120.0 - i2 * (1.0 / 252.0 - i2 * (1.0 / 240.0 - i2 * (1.0 / 132.0 - i2 * (691.0 / 32760.0 - i2 / 12.0))))));
+    return r + std::log(x) - 0.5 / x - ser;
+}
+// kappa's 2F1 factor 2F1(k - 1/3, k + 1; k + 2/3; -k w) (kappa.jl:13-16)
+static double host_kappa_hyp(double k, double w) { return host_hyp2f1_neg(k - 1.0 / 3.0, k + 1.0, k + 2.0 / 3.0, -k * w); }
+// d/d ln Theta (or d/d ln w) of K0/K2 and K1/K2 at x = 1/Theta: -x d/dx with K0' = -K1, K1' = -K0 - K1/x,
+// K2 = K0 + 2 K1/x
+static void host_bessel_ratio_dlog(double xb, double k0, double k1, double k2, double& dk0k2, double& dk1k2) {
+    const double dk1 = -k0 - k1 / xb, dk2 = -k1 - 2.0 * k0 / xb - 4.0 * k1 / (xb * xb);
+    dk0k2 = -xb * (-k1 * k2 - k0 * dk2) / (k2 * k2);
+    dk1k2 = -xb * (dk1 * k2 - k1 * dk2) / (k2 * k2);
+}
+
 // FP32 copies of the per-splat constants (every field converted)
 static void to_f32(const SplatC& a, SplatCf& b) {
     std::memset(&b, 0, sizeof b);
@@ -442,6 +463,8 @@ static void to_f32(const SplatC& a, SplatCf& b) {
     b.t0 = a.t0; b.iw = a.iw; b.omega = a.omega; b.rsup2 = a.rsup2; b.lnA = a.lnA; b.Bmag = a.Bmag;
     b.p1 = a.p1; b.p2 = a.p2; b.p3 = a.p3;
     for (int i = 0; i < 6; ++i) b.g[i] = a.g[i];
+    for (int i = 0; i < 3; ++i) b.kc[i] = a.kc[i];
+    b.kr = a.kr;
     b.knot0 = a.knot0; b.nk = a.nk; b.kt0 = a.kt0; b.kind = a.kind;
 }
 static void to_f32(const SplatAdj& a, SplatAdjf& b) {
@@ -449,6 +472,7 @@ static void to_f32(const SplatAdj& a, SplatAdjf& b) {
     for (int i = 0; i < 4; ++i) b.qn[i] = a.qn[i];
     b.iqn = a.iqn; b.dk0k2 = a.dk0k2; b.dk1k2 = a.dk1k2;
     b.sthB = a.sthB; b.cthB = a.cthB; b.sphB = a.sphB; b.cphB = a.cphB;
+    for (int i = 0; i < 8; ++i) b.dpop[i] = a.dpop[i];
 }
 
 int ks_model_create(ks_ctx* ctx, int kind, int nrows, int ns, const double* params, double gamma_max,
@@ -459,6 +483,11 @@ int ks_model_create(ks_ctx* ctx, int kind, int nrows, int ns, const double* para
     if (nrows != want) return fail(KS_ERR_DIM, "model kind %d takes %d parameter rows, got %d", kind, want, nrows);
     if (ns < 0 || (ns > 0 && !params)) return fail(KS_ERR_ARG, "bad splat count or parameters");
     if (kind == KS_MODEL_KNOT && (nknots < 1 || !knots || !knot_times)) return fail(KS_ERR_ARG, "knot splats need knots and knot times");
+    // centres that follow knots: knot splats, or thermal, power-law and kappa splats given knots
+    const bool with_knots = kind != KS_MODEL_THIN && knots && knot_times && nknots >= 1;
+    if (with_knots)
+        for (int k = 0; k + 1 < nknots; ++k)
+            if (!(knot_times[k + 1] > knot_times[k])) return fail(KS_ERR_ARG, "knot times must increase");
     HIPCHK(hipSetDevice(ctx->device));
     std::vector<SplatC> h(ns > 0 ? ns : 1);
     std::vector<SplatAdj> ha(ns > 0 ? ns : 1);
@@ -482,7 +511,22 @@ int ks_model_create(ks_ctx* ctx, int kind, int nrows, int ns, const double* para
         c.rsup2 = SR2 * std::exp(2.0 * smax);
         c.lnA = p[12];
         c.kind = kind;
-        if (kind == KS_MODEL_KNOT) { c.knot0 = nknots * i; c.nk = nknots; c.kt0 = 0; }
+        if (with_knots) {
+            c.knot0 = nknots * i; c.nk = nknots; c.kt0 = 0;
+            // the sphere holding the knots (the centre stays in their convex hull)
+            const double* K = knots + (size_t)3 * nknots * i;
+            double lo[3] = {K[0], K[1], K[2]}, hi[3] = {K[0], K[1], K[2]};
+            for (int k = 1; k < nknots; ++k)
+                for (int a_ = 0; a_ < 3; ++a_) { lo[a_] = std::min(lo[a_], K[3 * k + a_]); hi[a_] = std::max(hi[a_], K[3 * k + a_]); }
+            double r2 = 0.0;
+            for (int a_ = 0; a_ < 3; ++a_) c.kc[a_] = 0.5 * (lo[a_] + hi[a_]);
+            for (int k = 0; k < nknots; ++k) {
+                double d2 = 0.0;
+                for (int a_ = 0; a_ < 3; ++a_) d2 += (K[3 * k + a_] - c.kc[a_]) * (K[3 * k + a_] - c.kc[a_]);
+                r2 = std::max(r2, d2);
+            }
+            c.kr = std::sqrt(r2) * (1.0 + 1e-12);
+        }
         if (kind == KS_MODEL_THIN) continue;
         int rB = (kind == KS_MODEL_THERMAL || kind == KS_MODEL_KNOT) ? 14 : 15;
         c.Bmag = std::exp(p[rB]);
@@ -502,30 +546,50 @@ int ks_model_create(ks_ctx* ctx, int kind, int nrows, int ns, const double* para
             c.p2 = k2 > 0.0 ? k0 / k2 : 1.0;
             c.p3 = k2 > 0.0 ? k1 / k2 : 1.0;
             // d/d ln Theta = -x d/dx at x = 1/Theta, with K0' = -K1, K1' = -K0 - K1/x, K2 = K0 + 2 K1/x
-            if (k2 > 0.0) {
-                const double dk1 = -k0 - k1 / xb, dk2 = -k1 - 2.0 * k0 / xb - 4.0 * k1 / (xb * xb);
-                sa.dk0k2 = -xb * (-k1 * k2 - k0 * dk2) / (k2 * k2);
-                sa.dk1k2 = -xb * (dk1 * k2 - k1 * dk2) / (k2 * k2);
-            }
+            if (k2 > 0.0) host_bessel_ratio_dlog(xb, k0, k1, k2, sa.dk0k2, sa.dk1k2);
         } else if (kind == KS_MODEL_POWERLAW) {
             double pp = p[13];
             c.p1 = pp; c.p2 = std::exp(p[14]); c.p3 = gamma_max;
             c.g[0] = std::tgamma((3 * pp - 1) / 12.0) * std::tgamma((3 * pp + 19) / 12.0);
             c.g[1] = std::tgamma((3 * pp + 2) / 12.0) * std::tgamma((3 * pp + 22) / 12.0);
+            // d/dp of the Gamma products: Gamma'(x) = Gamma(x) psi(x), dx/dp = 1/4
+            sa.dpop[0] = c.g[0] * 0.25 * (host_digamma((3 * pp - 1) / 12.0) + host_digamma((3 * pp + 19) / 12.0));
+            sa.dpop[1] = c.g[1] * 0.25 * (host_digamma((3 * pp + 2) / 12.0) + host_digamma((3 * pp + 22) / 12.0));
         } else {
             double kap = p[13], wk = std::exp(p[14]);
-            c.p1 = kap; c.p2 = wk; c.p3 = host_hyp2f1_neg(kap - 1.0 / 3.0, kap + 1.0, kap + 2.0 / 3.0, -kap * wk);
+            c.p1 = kap; c.p2 = wk; c.p3 = host_kappa_hyp(kap, wk);
             c.g[0] = std::tgamma(kap - 4.0 / 3.0); c.g[1] = std::tgamma(kap - 2.0);
             c.g[2] = std::tgamma(kap / 4.0 - 1.0 / 3.0); c.g[3] = std::tgamma(kap / 4.0 + 4.0 / 3.0);
             c.g[4] = std::tgamma(2.0 + kap / 2.0);
             double k0, k1;
-            double k2 = host_besselk(1.0 / wk, k0, k1);
+            const double xw = 1.0 / wk;
+            double k2 = host_besselk(xw, k0, k1);
             c.g[5] = k0 / (k2 + 1e-40);
+            // the 2F1 factor's derivatives: in kappa by a four-point central difference (Richardson, error
+            // ~h^4), in ln w in closed form, w dF/dw = w (-kappa) (a b / c) 2F1(a + 1, b + 1; c + 1; z)
+            const double h = 1e-3 * kap;
+            sa.dpop[0] = (8.0 * (host_kappa_hyp(kap + h, wk) - host_kappa_hyp(kap - h, wk)) -
+                          (host_kappa_hyp(kap + 2 * h, wk) - host_kappa_hyp(kap - 2 * h, wk))) / (12.0 * h);
+            {
+                const double a2 = kap - 1.0 / 3.0, b2 = kap + 1.0, c2 = kap + 2.0 / 3.0;
+                sa.dpop[1] = wk * (-kap) * a2 * b2 / c2 * host_hyp2f1_neg(a2 + 1.0, b2 + 1.0, c2 + 1.0, -kap * wk);
+            }
+            sa.dpop[2] = c.g[0] * host_digamma(kap - 4.0 / 3.0);
+            sa.dpop[3] = c.g[1] * host_digamma(kap - 2.0);
+            sa.dpop[4] = c.g[2] * 0.25 * host_digamma(kap / 4.0 - 1.0 / 3.0);
+            sa.dpop[5] = c.g[3] * 0.25 * host_digamma(kap / 4.0 + 4.0 / 3.0);
+            sa.dpop[6] = c.g[4] * 0.5 * host_digamma(2.0 + kap / 2.0);
+            // d(K0 / (K2 + 1e-40))/d ln w at 1/w (the 1e-40 guard is far below K2 for every w of the fits)
+            double dk0k2 = 0.0, dk1k2 = 0.0;
+            if (k2 > 0.0) host_bessel_ratio_dlog(xw, k0, k1, k2, dk0k2, dk1k2);
+            sa.dpop[7] = dk0k2;
         }
     }
     ks_model* m = new ks_model;
     m->id = next_model_id();
     m->ctx = ctx; m->kind = kind; m->nrows = nrows; m->ns = ns;
+    m->hrows.assign((size_t)ns, nrows);
+    m->pops = (kind == KS_MODEL_POWERLAW || kind == KS_MODEL_KAPPA ? POP_NONTHERMAL : POP_THERMAL) | (with_knots ? POP_KNOTS : 0);
     if (dalloc(&m->d_splats, h.size()) || dalloc(&m->d_adj, ha.size()) || dalloc(&m->d_splats_f, h.size()) || dalloc(&m->d_adj_f, ha.size())) {
         ks_model_destroy(m); return KS_ERR_HIP;
     }
@@ -538,7 +602,7 @@ int ks_model_create(ks_ctx* ctx, int kind, int nrows, int ns, const double* para
         HIPCHK(hipMemcpy(m->d_splats_f, hf.data(), hf.size() * sizeof(SplatCf), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(m->d_adj_f, haf.data(), haf.size() * sizeof(SplatAdjf), hipMemcpyHostToDevice));
     }
-    if (kind == KS_MODEL_KNOT) {
+    if (with_knots) {
         m->nknots = nknots;
         m->nkn = (int64_t)nknots * ns; m->nkt = nknots;
         if (dalloc(&m->d_knots, (size_t)3 * nknots * ns) || dalloc(&m->d_ktimes, (size_t)nknots)) { ks_model_destroy(m); return KS_ERR_HIP; }
@@ -578,6 +642,10 @@ int ks_model_composite(ks_ctx* ctx, int nparts, const ks_model* const* parts, ks
     ks_model* m = new ks_model;
     m->id = next_model_id();
     m->ctx = ctx; m->kind = KS_MODEL_COMPOSITE; m->nrows = 0; m->ns = (int)ns;
+    for (int p = 0; p < nparts; ++p) {
+        m->hrows.insert(m->hrows.end(), parts[p]->hrows.begin(), parts[p]->hrows.end());
+        m->pops |= parts[p]->pops;
+    }
     const size_t n = ns > 0 ? (size_t)ns : 1;
     if (dalloc(&m->d_splats, n) || dalloc(&m->d_adj, n) || dalloc(&m->d_splats_f, n) || dalloc(&m->d_adj_f, n)) {
         ks_model_destroy(m); return KS_ERR_HIP;
@@ -595,7 +663,7 @@ int ks_model_composite(ks_ctx* ctx, int nparts, const ks_model* const* parts, ks
             HIPCHK_M(hipMemcpy(m->d_adj + off, q->d_adj, q->ns * sizeof(SplatAdj), hipMemcpyDeviceToDevice));
             HIPCHK_M(hipMemcpy(m->d_adj_f + off, q->d_adj_f, q->ns * sizeof(SplatAdjf), hipMemcpyDeviceToDevice));
             for (int i = 0; i < q->ns; ++i)
-                if (h[off + i].kind == KS_MODEL_KNOT) { h[off + i].knot0 += (int)okn; h[off + i].kt0 += (int)okt; }
+                if (h[off + i].nk > 0) { h[off + i].knot0 += (int)okn; h[off + i].kt0 += (int)okt; }
         }
         if (q->nkn > 0) {
             HIPCHK_M(hipMemcpy(kn.data() + 3 * okn, q->d_knots, (size_t)3 * q->nkn * sizeof(double), hipMemcpyDeviceToHost));
@@ -740,6 +808,15 @@ int ks_render(ks_cache* c, const ks_model* m, const double* times, int nt, const
 }
 
 // ---- the dual-sweep gradient (polarized_gradient!, polarized.jl:473-480; dual_gradient.jl:180-249) ----
+// Units per launch of the element adjoints: their partial sums (units x splats x ADJ_ROWS doubles) stay within
+// ADJ_PART_MAX doubles (4 GiB), in whole groups of 8 units (the XCD order of the launch).
+constexpr int64_t ADJ_PART_MAX = (int64_t)1 << 29;
+static int64_t adj_unit_batch(int64_t nunits, int ns) {
+    const int64_t per = (int64_t)std::max(ns, 1) * ADJ_ROWS;
+    int64_t nb = std::max<int64_t>(8, ADJ_PART_MAX / per / 8 * 8);
+    return std::min(nb, (nunits + 7) / 8 * 8);
+}
+
 static int ensure_adj(ks_cache* c, int64_t items, int ns) {
     const int nseg = (c->N + CHUNK - 1) / CHUNK;
     if (c->a_items < items) {
@@ -752,10 +829,11 @@ static int ensure_adj(ks_cache* c, int64_t items, int ns) {
         c->a_items = items;
     }
     const int64_t nunits = adj_units(items, c->N);
-    if (c->a_part < nunits * ns * ADJ_ROWS) {
+    const int64_t nb = adj_unit_batch(nunits, ns);
+    if (c->a_part < nb * ns * ADJ_ROWS) {
         dfree(c->apart);
-        if (dalloc(&c->apart, (size_t)nunits * ns * ADJ_ROWS)) return KS_ERR_HIP;
-        c->a_part = nunits * ns * ADJ_ROWS;
+        if (dalloc(&c->apart, (size_t)nb * ns * ADJ_ROWS)) return KS_ERR_HIP;
+        c->a_part = nb * ns * ADJ_ROWS;
     }
     if (c->a_ub < nunits) {
         dfree(c->aub);
@@ -775,7 +853,7 @@ int ks_gradient_forward(ks_cache* c, const ks_model* m, const double* times, int
                         double* image, int screen_order) {
     if (!c || !m || !times || !nus) return fail(KS_ERR_ARG, "ks_gradient_forward: null argument");
     if (c->generation < 1) return fail(KS_ERR_STATE, "regenerate! the cache first");
-    if (m->kind != KS_MODEL_THERMAL) return fail(KS_ERR_ARG, "the dual sweep takes thermal splats (PolarizedSplats)");
+    if (m->kind == KS_MODEL_THIN) return fail(KS_ERR_ARG, "the dual sweep takes polarized splats (thermal, power-law, kappa, knot or composite)");
     if (!has_samples(c)) return fail(KS_ERR_ARG, "the cache holds no stored samples: build it with store_samples = true and a storing marcher");
     if (nt < 1 || nnu < 1) return fail(KS_ERR_ARG, "need at least one time and one frequency");
     if (!(L > 0.0)) return fail(KS_ERR_ARG, "length unit L must be positive");
@@ -837,19 +915,32 @@ int ks_gradient_reverse(ks_cache* c, const ks_model* m, const double* dstokes, i
     if (ns > 0) {
         HIPCHK(x->stage_begin(KS_STAGE_ADJ_ELEMENTS));
         HIPCHK(launch_unit_bounds(x->stream, c->npix, c->N, samples_of(c), d_tn, nt, items, c->aflg, c->aub));
-        if (mixed)
-            HIPCHK(launch_adj_splats_f32(x->stream, c->spin, c->thetao, c->soa, c->npix, c->N, samples_of(c), m->d_splats_f, m->d_adj_f, ns,
-                                         d_tn, nt, d_tn + nt, items, (const f32*)c->abar, c->aflg, c->aub, c->apart));
-        else
-            HIPCHK(launch_adj_splats(x->stream, c->spin, c->thetao, c->soa, c->npix, c->N, samples_of(c), m->d_splats, m->d_adj, ns, d_tn, nt,
-                                     d_tn + nt, items, c->abar, c->aflg, c->aub, c->apart));
+        // the units in batches whose partial sums fit the workspace, each reduced into the running sums in order
+        const int64_t nunits = adj_units(items, c->N), nbmax = adj_unit_batch(nunits, ns);
+        for (int64_t u0 = 0; u0 < nunits; u0 += nbmax) {
+            const int64_t nb = std::min(nbmax, nunits - u0);
+            if (mixed)
+                HIPCHK(launch_adj_splats_f32(x->stream, m->pops, c->spin, c->thetao, c->soa, c->npix, c->N, samples_of(c), m->d_splats_f, m->d_adj_f, ns,
+                                             m->d_knots_f, m->d_ktimes_f, d_tn, nt, d_tn + nt, items, (const f32*)c->abar, c->aflg, u0, nb, c->aub,
+                                             c->apart));
+            else
+                HIPCHK(launch_adj_splats(x->stream, m->pops, c->spin, c->thetao, c->soa, c->npix, c->N, samples_of(c), m->d_splats, m->d_adj, ns, m->d_knots,
+                                         m->d_ktimes, d_tn, nt, d_tn + nt, items, c->abar, c->aflg, u0, nb, c->aub, c->apart));
+            HIPCHK(launch_adj_reduce(x->stream, c->apart, nb, ns, u0 > 0, c->adpar));
+        }
         HIPCHK(x->stage_end(KS_STAGE_ADJ_ELEMENTS));
-        HIPCHK(launch_adj_reduce(x->stream, c->apart, adj_units(items, c->N), ns, c->adpar));
     }
     std::vector<double> g((size_t)ns * ADJ_ROWS);
     if (ns > 0) HIPCHK(hipMemcpyAsync(g.data(), c->adpar, g.size() * sizeof(double), hipMemcpyDeviceToHost, x->stream));
     HIPCHK(hipStreamSynchronize(x->stream));
-    for (size_t i = 0; i < g.size(); ++i) dparams[i] += g[i];
+    // every splat's rows into its column of dparams (21 rows thermal and knot, 22 power law and kappa; a composite's
+    // parts in order, each part nrows x ns column-major)
+    size_t off = 0;
+    for (int i = 0; i < ns; ++i) {
+        const int nr = m->hrows[i];
+        for (int r = 0; r < nr; ++r) dparams[off + r] += g[(size_t)i * ADJ_ROWS + r];
+        off += nr;
+    }
     return KS_OK;
 }
 

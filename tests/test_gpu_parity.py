@@ -248,8 +248,13 @@ def test_composite_errors():
         K.CompositeModel()
     cam = camera(10.0, 4)
     c = make_cache(cam, 16, 0.9, 0.5, "stored")
-    with pytest.raises(ValueError, match="thermal"):
-        K.polarized_gradient(c, K.CompositeModel(th, th), [0.0], [230e9], 1e14, np.zeros((4, 4, 1, 1, 4)))
+    thin = K.Model(syn.thin_splats(0, 0.9, 2), kind="thin")
+    with pytest.raises(ValueError, match="polarized splats"):
+        K.polarized_gradient(c, thin, [0.0], [230e9], 1e14, np.zeros((4, 4, 1, 1, 4)))
+    # a composite's gradient is one vector of its parts' blocks
+    with pytest.raises(K.api.DimensionMismatch):
+        K.polarized_gradient(c, K.CompositeModel(th, th), [0.0], [230e9], 1e14, np.zeros((4, 4, 1, 1, 4)),
+                             dparams=np.zeros((21, 4), order="F"))
 
 
 @pytest.mark.parametrize("path", PATHS)
