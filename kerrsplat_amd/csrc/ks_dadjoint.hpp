@@ -27,9 +27,23 @@ template <class S> struct SplatAdjT {
     // power law, (kappa, ln w) for kappa: power law {dgPL0/dp, dgPL1/dp}; kappa {d2F1/dkappa, d2F1/d ln w,
     // dGamma(k-4/3)/dk, dGamma(k-2)/dk, dGamma(k/4-1/3)/dk, dGamma(k/4+4/3)/dk, dGamma(2+k/2)/dk, d(K0/K2)/d ln w}
     S dpop[8];
+    // kappa: the partials of the fits' per-splat factors (kappa_consts) in kappa, ln w and ln B
+    S dkk[3][KK_N];
 };
 using SplatAdj = SplatAdjT<double>;
 using SplatAdjf = SplatAdjT<f32>;
+
+// A kappa splat's per-splat factors as five-partial duals over (kappa, ln w, ln B, g, cos B): the values and their
+// partials in the three population rows, read as the fits use them
+template <class S> struct KappaDualGet {
+    const KappaK<S>& k;
+    const SplatAdjT<S>& sa;
+    KS_DEV DN<5, S> operator()(int f) const {
+        DN<5, S> r(k.v[f]);
+        r.d[0] = sa.dkk[0][f]; r.d[1] = sa.dkk[1][f]; r.d[2] = sa.dkk[2][f];
+        return r;
+    }
+};
 
 // One sample: given its coefficients, the tail behind it (R = T_{k+1}) and the adjoint w in front of
 // it, returns O^T w, j_bar, alpha_bar, rho_bar.
@@ -345,13 +359,7 @@ KS_DEV void element_adjoint_pop(const SplatCT<S>& c, const SplatAdjT<S>& sa, con
         for (int k = 0; k < 2; ++k) { gPL[k] = D5(c.g[k]); gPL[k].d[0] = sa.dpop[k]; }
         cf = powerlaw(ne, p, gmin, D5(c.p3), gPL, Bm, nuf, sth, cth);
     } else {
-        const D5 kap = D5::seed(c.p1, 0);
-        D5 w(c.p2); w.d[1] = c.p2;
-        D5 hyp(c.p3); hyp.d[0] = sa.dpop[0]; hyp.d[1] = sa.dpop[1];
-        D5 gK[6];
-        for (int k = 0; k < 5; ++k) { gK[k] = D5(c.g[k]); gK[k].d[0] = sa.dpop[2 + k]; }
-        gK[5] = D5(c.g[5]); gK[5].d[1] = sa.dpop[7];
-        cf = kappa(ne, kap, w, hyp, gK, Bm, nuf, sth, cth);
+        cf = kappa_coefs(KappaDualGet<S>{c.kk, sa}, ne, nuf, sth, cth);
     }
     if (!(cf.jI > 0.0 || cf.aI > 0.0 || cf.rQ.v != 0.0 || cf.rV.v != 0.0)) return;
     D5 fe = cap_factor(cf.jI, cf.jQ, cf.jV), fa = cap_factor(cf.aI, cf.aQ, cf.aV);

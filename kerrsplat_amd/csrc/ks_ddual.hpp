@@ -52,6 +52,17 @@ KS_DEV f32& operator+=(f32& a, f32 b) { a.v += b.v; return a; }
 KS_DEV f32& operator-=(f32& a, f32 b) { a.v -= b.v; return a; }
 KS_DEV f32& operator*=(f32& a, f32 b) { a.v *= b.v; return a; }
 KS_DEV f32 sqrt(f32 x) { return f32(sqrtf(x.v)); }
+// pow of the FP32 path on the hardware base-2 transcendentals (v_log_f32, v_exp_f32, about one ulp each):
+// x^y = 2^(y log2 x) for x > 0, with a relative error that grows with the exponent's magnitude, |y log2 x| 2^-24
+// (4e-6 at |y log2 x| = 100), far inside the FP32 path's error bars; a base that is zero, negative or not finite
+// takes the library's powf. exp, log and cbrt stay the library's: exp through 2^(x log2 e) loses |x| 2^-24 of
+// relative accuracy, which optically thick, Faraday-rotating parcels amplify past the FP32 path's bar.
+KS_DEV float hw_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+KS_DEV float hw_log2(float x) { return __builtin_amdgcn_logf(x); }
+KS_DEV float hw_pow(float x, float y) {
+    if (x > 0.0f && x < INFINITY) return hw_exp2(y * hw_log2(x));
+    return powf(x, y);
+}
 KS_DEV f32 exp(f32 x) { return f32(expf(x.v)); }
 KS_DEV f32 expm1(f32 x) { return f32(expm1f(x.v)); }
 KS_DEV f32 log(f32 x) { return f32(logf(x.v)); }
@@ -66,9 +77,9 @@ KS_DEV f32 cbrt(f32 x) { return f32(cbrtf(x.v)); }
 KS_DEV f32 acos(f32 x) { return f32(acosf(x.v)); }
 KS_DEV f32 fabs(f32 x) { return f32(fabsf(x.v)); }
 KS_DEV f32 atan2(f32 y, f32 x) { return f32(atan2f(y.v, x.v)); }
-KS_DEV f32 pow(f32 x, double y) { return f32(powf(x.v, (float)y)); }
-KS_DEV f32 pow(f32 x, f32 y) { return f32(powf(x.v, y.v)); }
-KS_DEV f32 pow(double x, f32 y) { return f32(powf((float)x, y.v)); }
+KS_DEV f32 pow(f32 x, double y) { return f32(hw_pow(x.v, (float)y)); }
+KS_DEV f32 pow(f32 x, f32 y) { return f32(hw_pow(x.v, y.v)); }
+KS_DEV f32 pow(double x, f32 y) { return f32(hw_pow((float)x, y.v)); }
 KS_DEV f32 dmax(f32 a, f32 b) { return a.v > b.v ? a : b; }
 KS_DEV f32 dmin(f32 a, f32 b) { return a.v < b.v ? a : b; }
 KS_DEV f32 dmax(f32 a, double b) { return a.v > (float)b ? a : f32(b); }
@@ -181,13 +192,15 @@ KSDD_T KS_DEV DN<N, S> atan2(const DN<N, S>& y, const DN<N, S>& x) {
     KSDD_LOOP r.d[i] = (x.v * y.d[i] - y.v * x.d[i]) / r2;
     return r;
 }
+// d x^y / dx = y x^y / x: one power per dual power (x^(y - 1) is taken only at x = 0)
 KSDD_T KS_DEV DN<N, S> pow(const DN<N, S>& x, double y) {
     const S p = pow(x.v, y);
-    return chain(p, y == 0.0 ? S(0.0) : S(y * pow(x.v, y - 1.0)), x);
+    return chain(p, y == 0.0 ? S(0.0) : (x.v != 0.0 ? S(y * p / x.v) : S(y * pow(x.v, y - 1.0))), x);
 }
 KSDD_T KS_DEV DN<N, S> pow(const DN<N, S>& x, const DN<N, S>& y) {
     const S p = pow(x.v, y.v);
-    const S dx = y.v == 0.0 ? S(0.0) : S(y.v * pow(x.v, y.v - 1.0)), dy = x.v > 0.0 ? S(p * log(x.v)) : S(0.0);
+    const S dx = y.v == 0.0 ? S(0.0) : (x.v != 0.0 ? S(y.v * p / x.v) : S(y.v * pow(x.v, y.v - 1.0)));
+    const S dy = x.v > 0.0 ? S(p * log(x.v)) : S(0.0);
     DN<N, S> r; r.v = p;
     KSDD_LOOP r.d[i] = dx * x.d[i] + dy * y.d[i];
     return r;

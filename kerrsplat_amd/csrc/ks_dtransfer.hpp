@@ -26,6 +26,24 @@ constexpr double WEIGHT_CUTOFF = 1e-6;
 // each splat's population in SplatCT::kind
 enum ModelKind : int { MODEL_THIN = 0, MODEL_THERMAL = 1, MODEL_POWERLAW = 2, MODEL_KAPPA = 3, MODEL_KNOT = 4, MODEL_COMPOSITE = 5 };
 
+// The kappa fits' factors that depend on the splat alone (kappa, w, the 2F1 factor, the Gamma values, |B|), built
+// once per splat by kappa_consts; kappa_coefs evaluates the fits at one sample from them. Indices into KappaK::v:
+enum KappaField : int {
+    KK_NUW1,                       // (w kappa)^2 nu_c: nu_w = KK_NUW1 sin(theta), X = nu / nu_w
+    KK_JPRE, KK_APRE,              // e^2 nu_c / c (emissivity prefactor / (n_e sin)), e / B (absorptivity's, x sin / n_e)
+    KK_NLOWI, KK_RI, KK_EI, KK_MI, KK_XI,       // N_low; (N_low / N_high)^x; x (3k - 4) / 6; -1 / x; x   (j_I; x_V = x_I)
+    KK_NLOWQ, KK_RQ, KK_EQ, KK_MQ,              // j_Q
+    KK_CV1, KK_CV2,                // N_low,V = CV1 (sin^-2.4 - 1)^0.48 X^-0.35, N_high,V = CV2 (sin^-2.5 - 1)^0.44 / sqrt(X)
+    KK_LOWA, KK_RA, KK_EA, KK_MA,               // alpha_I (x (3k - 1) / 6)
+    KK_NLOWQA, KK_RQA, KK_EQA, KK_MQA,          // alpha_Q
+    KK_CVA1, KK_CVA2, KK_XVA, KK_EVA, KK_MVA,   // alpha_V: N_low = CVA1 (sin^-2.28 - 1)^0.446 X^-0.35, N_high = CVA2 sqrt(sin^-2.05 - 1) / sqrt(X)
+    KK_FQ0, KK_FQ1, KK_FV0, KK_FV1,             // rotativity fits' w-dependent weights at the fit below kappa and the next
+    KK_RHOQ, KK_RHOV,              // e^2 nu_c^2 / (m c); 2 e^2 nu_c K0/K2 / (m c)
+    KK_IFIT,                       // the rotativity fit below kappa (1..3)
+    KK_N
+};
+template <class T> struct KappaK { T v[KK_N]; };
+
 // Per-splat constants, built on the host by ks_model_create in both precisions (SplatC, SplatCf);
 // a kernel reads them with wave-uniform addresses.
 template <class S> struct SplatCT {
@@ -43,6 +61,7 @@ template <class S> struct SplatCT {
     S g[6];               // power law: Gamma((3p-1)/12) Gamma((3p+19)/12), Gamma((3p+2)/12) Gamma((3p+22)/12);
                           // kappa: Gamma(k-4/3), Gamma(k-2), Gamma(k/4-1/3), Gamma(k/4+4/3), Gamma(2+k/2), K0/(K2+1e-40) at 1/w
     S kc[3], kr;          // splats whose centre follows knots: a sphere holding every knot (centre, radius; kr = 0 without)
+    KappaK<S> kk;         // kappa: the fits' per-splat factors (kappa_consts)
     int knot0, nk, kt0;   // splats whose centre follows knots (nk > 0): the first of its nk knots (xyz triples) and of their
                           // times in the model's arrays
     int kind;             // the splat's population (MODEL_THERMAL, MODEL_POWERLAW, MODEL_KAPPA, ...)
@@ -75,15 +94,47 @@ KS_DEV bool outside_support(const SplatCT<S>& s, S t, S x, S y, S z, S& sp, S& c
 // splat's centre sweeps an arc of the circle about the spin axis; every point of an arc of half-angle h
 // is within 2 rho sin(min(h, pi) / 2) of the arc's midpoint. False only when no sample of the region
 // can be inside the support sphere (outside_support), with a margin for rounding.
+// A centre that follows knots (knots, ktimes: the model's arrays, nullptr to use the sphere holding every knot)
+// moves over [tmin, tmax] along the polyline through its positions at tmin and tmax and the knots between (linear
+// between knots, held outside them): it stays inside the bounding box of those points, bounded by the sphere
+// around the box's centre through the farthest of them.
 template <class S>
-KS_DEV bool splat_may_reach(const SplatCT<S>& c, double px, double py, double pz, double R, double tmin, double tmax) {
+KS_DEV bool splat_may_reach(const SplatCT<S>& c, double px, double py, double pz, double R, double tmin, double tmax,
+                            const S* knots = nullptr, const S* ktimes = nullptr) {
     if (c.nk > 0) {
-        // a centre that follows knots stays inside the sphere holding the knots (linear interpolation between
-        // them, held at the ends): no arc term
-        const double dx = px - val(c.kc[0]), dy = py - val(c.kc[1]), dz = pz - val(c.kc[2]);
+        double cx = val(c.kc[0]), cy = val(c.kc[1]), cz = val(c.kc[2]), cr = val(c.kr);
+        if (knots && ktimes) {
+            const S* K = knots + 3 * (int64_t)c.knot0;
+            const S* kt = ktimes + c.kt0;
+            const int nk = c.nk;
+            double pt[2][3];
+            for (int e = 0; e < 2; ++e) {
+                const double t = e == 0 ? tmin : tmax;
+                if (!(t > val(kt[0]))) { for (int d = 0; d < 3; ++d) pt[e][d] = val(K[d]); }
+                else if (t >= val(kt[nk - 1])) { for (int d = 0; d < 3; ++d) pt[e][d] = val(K[3 * (nk - 1) + d]); }
+                else {
+                    int kk = 0;
+                    while (kk < nk - 2 && val(kt[kk + 1]) <= t) ++kk;
+                    const double w = (t - val(kt[kk])) / (val(kt[kk + 1]) - val(kt[kk]));
+                    for (int d = 0; d < 3; ++d) pt[e][d] = (1.0 - w) * val(K[3 * kk + d]) + w * val(K[3 * (kk + 1) + d]);
+                }
+            }
+            double lo[3], hi[3];
+            for (int d = 0; d < 3; ++d) { lo[d] = fmin(pt[0][d], pt[1][d]); hi[d] = fmax(pt[0][d], pt[1][d]); }
+            for (int k = 0; k < nk; ++k) {
+                const double tk = val(kt[k]);
+                if (tk > tmin && tk < tmax)
+                    for (int d = 0; d < 3; ++d) { lo[d] = fmin(lo[d], val(K[3 * k + d])); hi[d] = fmax(hi[d], val(K[3 * k + d])); }
+            }
+            cx = 0.5 * (lo[0] + hi[0]); cy = 0.5 * (lo[1] + hi[1]); cz = 0.5 * (lo[2] + hi[2]);
+            cr = 0.5 * sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]) + (hi[2] - lo[2]) * (hi[2] - lo[2]));
+        }
+        const double dx = px - cx, dy = py - cy, dz = pz - cz;
         const double d = sqrt(dx * dx + dy * dy + dz * dz);
-        const double reach = R + sqrt(val(c.rsup2)) + val(c.kr);
-        return d <= reach + 1e-4 * (reach + d) + 1e-6;
+        const double reach = R + sqrt(val(c.rsup2)) + cr;
+        // margin: the per-sample test in FP32 rounds the emission time (which moves the interpolated centre by
+        // |dc/dt| 6e-8 |t|) and the positions; 1e-4 relative covers them
+        return d <= reach + 1e-4 * (reach + d + cr) + 1e-6;
     }
     const double x = val(c.x), y = val(c.y), z = val(c.z), w = val(c.omega);
     const double tm = 0.5 * (tmin + tmax), h = 0.5 * fabs(w) * (tmax - tmin);
@@ -231,6 +282,23 @@ template <class T> KS_DEV void pitch_sincos(const T& cb, T& sth, T& cth) {
 }
 
 // ---- coefficients ------------------------------------------------------------------------------------------------
+// The power-law and kappa fits diverge as the pitch-angle sine goes to zero (a photon along the field). A sine that
+// rounds to zero (FP32 does so below about 3.5e-4 rad, FP64 below 1.5e-8 rad) is taken as half the smallest nonzero
+// sine the scalar resolves next to |cos| = 1, sqrt(eps) / 2, as a constant (zero partials).
+KS_DEV double sin_floor(double) { return 7.45e-9; }
+KS_DEV double sin_floor(f32) { return 1.73e-4; }
+template <int N, class S> KS_DEV double sin_floor(const DN<N, S>&) { return sin_floor(S(0.0)); }
+// sin^-p - 1 of the pitch angle: near sin = 1 from the cosine, expm1(-(p / 2) log1p(-cos^2)), without the
+// cancellation of the difference (which rounds to zero, and leaves the partials of x^e at zero unbounded, when the
+// sine rounds to one); directly elsewhere
+template <class T> KS_DEV T sin_pow_m1(T sth, T cth, double p) {
+    if (cth * cth < 0.5) return expm1((-0.5 * p) * log1p(-(cth * cth)));
+    return pow(sth, -p) - 1.0;
+}
+// x^e (0 < e < 1) of x >= 0, zero with zero partials at x = 0, where the derivative is unbounded (the reference's
+// safe_sqrt makes the same choice for the square root, frames.jl:83-97)
+template <class T> KS_DEV T pow_nonneg(T x, double e) { return x > 0.0 ? pow(x, e) : T(0.0); }
+
 template <class T> struct CoefT { T jI, jQ, jV, aI, aQ, aV, rQ, rV; };
 using Coef = CoefT<double>;
 
@@ -283,6 +351,7 @@ KS_DEV CoefT<T> thermal(NE ne, T Th, T k0k2, T k1k2, T B, T nu, T sth, T cth) {
 template <class T>
 KS_DEV CoefT<T> powerlaw(T ne, T p, T gmin, T gmax, const T* gPL, T B, T nu, T sth, T cth) {
     CoefT<T> c;
+    if (!(sth > 0.0)) sth = T(sin_floor(sth));
     T nuc = EE * B / (2.0 * PI * ME * CL);
     T gspan = pow(gmin, 1.0 - p) - pow(gmax, 1.0 - p);
     T pref = ne * EE * EE * nuc / CL;
@@ -293,7 +362,7 @@ KS_DEV CoefT<T> powerlaw(T ne, T p, T gmin, T gmax, const T* gPL, T B, T nu, T s
     T apref = ne * EE * EE / (nu * ME * CL);
     c.aI = apref * pow(3.0, (p + 1.0) * 0.5) * (p - 1.0) / (4.0 * gspan) * gPL[1] * pow(X, -(p + 2.0) * 0.5);
     c.aQ = c.aI * pow((17.0 / 500.0) * p - 43.0 / 1250.0, 43.0 / 500.0);
-    T term6 = pow(3.1 * pow(sth, -48.0 / 25.0) - 3.1, 64.0 / 125.0);
+    T term6 = pow_nonneg(3.1 * sin_pow_m1(sth, cth, 48.0 / 25.0), 64.0 / 125.0);
     const double sgn = cth > 0.0 ? 1.0 : (cth < 0.0 ? -1.0 : 0.0);
     c.aV = c.aI * pow(0.71 * p + 22.0 / 625.0, 197.0 / 500.0) * term6 * pow(X, -0.5) * sgn;
     T nuB = nuc * sth;
@@ -305,50 +374,34 @@ KS_DEV CoefT<T> powerlaw(T ne, T p, T gmin, T gmax, const T* gPL, T B, T nu, T s
     return c;
 }
 
+// kappa (kappa.jl:25-119), split into its per-splat factors and its per-sample evaluation; gK = {G(k-4/3),
+// G(k-2), G(k/4-1/3), G(k/4+4/3), G(2+k/2), K0/(K2+1e-40) at 1/w}; B the field magnitude
 template <class T>
-KS_DEV T kappa_rhoQ_fit(int i, T w, T X) {
-    T sw = sqrt(w), e5 = exp(-5.0 * w);
-    if (i == 1) return (17.0 * w - 3.0 * sw + 7.0 * sw * e5) * (1.0 - exp(-pow(X, 0.84) / 30.0) - sin(X / 10.0) * exp(-1.5 * pow(X, 0.471)));
-    if (i == 2) return (46.0 * w / 3.0 - 5.0 * sw / 3.0 + 17.0 * sw / 3.0 * e5) * (1.0 - exp(-pow(X, 0.84) / 18.0) - sin(X / 6.0) * exp(-1.75 * sqrt(X)));
-    if (i == 3) return (14.0 * w - 13.0 * sw / 8.0 + 4.5 * sw * e5) * (1.0 - exp(-pow(X, 0.84) / 12.0) - sin(X / 4.0) * exp(-2.0 * pow(X, 0.525)));
-    return (12.5 * w - sw + 5.0 * sw * e5) * (1.0 - exp(-pow(X, 0.84) / 8.0) - sin(0.375 * X) * exp(-2.25 * pow(X, 0.541)));
-}
-template <class T>
-KS_DEV T kappa_rhoV_fit(int i, T w, T X) {
-    T xm = 1.0 / sqrt(X);
-    if (i == 1) return (w * w + 2.0 * w + 1.0) / (3.125 * w * w + 4.0 * w + 1.0) * (1.0 - 0.17 * log1p(0.447 * xm));
-    if (i == 2) return (w * w + 54.0 * w + 50.0) / (30.0 * w * w / 11.0 + 134.0 * w + 50.0) * (1.0 - 0.17 * log1p(0.391 * xm));
-    if (i == 3) return (w * w + 43.0 * w + 38.0) / (7.0 * w * w / 3.0 + 92.5 * w + 38.0) * (1.0 - 0.17 * log1p(0.348 * xm));
-    return (w + 13.0 / 14.0) / (2.0 * w + 13.0 / 14.0) * (1.0 - 0.17 * log1p(0.313 * xm));
-}
-// kappa (kappa.jl:25-119); gK = {G(k-4/3), G(k-2), G(k/4-1/3), G(k/4+4/3), G(2+k/2), K0/(K2+1e-40) at 1/w}
-template <class T>
-KS_DEV CoefT<T> kappa(T ne, T kap, T w, T hyp, const T* gK, T B, T nu, T sth, T cth) {
-    CoefT<T> c;
+KS_DEV void kappa_consts(T kap, T w, T hyp, const T* gK, T B, KappaK<T>& K) {
     const double small = 1e-40;
     T nuc = EE * B / (2.0 * PI * ME * CL);
-    T nuw = (w * kap) * (w * kap) * nuc * sth;
-    T X = nu / nuw;
-    T pref = ne * EE * EE * nuc * sth / CL;
+    K.v[KK_NUW1] = (w * kap) * (w * kap) * nuc;
+    K.v[KK_JPRE] = EE * EE * nuc / CL;
+    K.v[KK_APRE] = EE / B;
     T g43 = gK[0], g2 = gK[1], gq = gK[2], gp = gK[3], gk2 = gK[4];
     const double c37 = 12.980246132766677;      // 3^(7/3)
     T NlowI = 4.0 * PI * g43 / (c37 * g2);
     T hi = 0.25 * pow(3.0, (kap - 1.0) * 0.5) * (kap - 2.0) * (kap - 1.0) * gq * gp;
-    T NhighI = hi + small;
-    T X13 = cbrt(X);
     T xI = 3.0 * pow(kap, -1.5);
-    c.jI = pref * NlowI * X13 * pow(1.0 + pow(X, xI * (3.0 * kap - 4.0) / 6.0) * pow(NlowI / NhighI, xI), -1.0 / xI);
-    T NlowQ = -0.5 * 4.0 * PI * g43 / (c37 * g2);
+    K.v[KK_NLOWI] = NlowI;
+    K.v[KK_RI] = pow(NlowI / (hi + small), xI);
+    K.v[KK_EI] = xI * (3.0 * kap - 4.0) / 6.0;
+    K.v[KK_MI] = -1.0 / xI;
+    K.v[KK_XI] = xI;
+    T NlowQ = -0.5 * NlowI;
     T NhighQ = -(0.64 + kap / 50.0) * hi + small;
     T xQ = 3.7 * pow(kap, -1.6);
-    c.jQ = -(pref * NlowQ * X13 * pow(1.0 + pow(X, xQ * (3.0 * kap - 4.0) / 6.0) * pow(NlowQ / NhighQ, xQ), -1.0 / xQ));
-    T NlowV = -0.5625 * pow(pow(sth, -2.4) - 1.0, 0.48) * (pow(kap, -0.528) / w) * pow(X, -0.35) * 4.0 * PI * g43 / (c37 * g2);
-    T NhighV = -0.765625 * pow(pow(sth, -2.5) - 1.0, 0.44) * (pow(kap, -0.44) / w) / sqrt(X) * hi - small;
-    T xV = xI;
-    const double sgn = cth > 0.0 ? 1.0 : (cth < 0.0 ? -1.0 : 0.0);
-    c.jV = fabs(NhighV) < small * small ? T(0.0)
-         : -(pref * NlowV * X13 * pow(1.0 + pow(X, xV * (3.0 * kap - 4.0) / 6.0) * pow(NlowV / NhighV, xV), -1.0 / xV)) * sgn;
-    T apref = ne * EE / (B * sth);
+    K.v[KK_NLOWQ] = NlowQ;
+    K.v[KK_RQ] = pow(NlowQ / NhighQ, xQ);
+    K.v[KK_EQ] = xQ * (3.0 * kap - 4.0) / 6.0;
+    K.v[KK_MQ] = -1.0 / xQ;
+    K.v[KK_CV1] = -0.5625 * (pow(kap, -0.528) / w) * NlowI;
+    K.v[KK_CV2] = -0.765625 * (pow(kap, -0.44) / w) * hi;
     const double g53 = 0.9027452929509338;     // Gamma(5/3)
     T ck = (kap - 2.0) * (kap - 1.0) * kap;
     const double c16 = 1.2009369551760027;     // 3^(1/6)
@@ -356,31 +409,112 @@ KS_DEV CoefT<T> kappa(T ne, T kap, T w, T hyp, const T* gK, T B, T nu, T sth, T 
     T wk = w * kap;
     T lowA = c16 * (10.0 / 41.0) * tpi2 / pow(wk, 16.0 / 3.0 - kap) * ck / (3.0 * kap - 1.0) * g53 * hyp;
     T highA = 2.0 * 17.493418327624862 / 3.0 * ck / pow(wk, 5.0) * (2.0 * gk2 / (2.0 + kap) - 1.0);   // 2 pi^(5/2)/3
-    T NlowA = lowA;
     T NhighA = highA * (pow(3.0 / kap, 4.75) + 0.6) + small;
     T xA = pow(-1.75 + 1.6 * kap, -0.86);
-    T Xm53 = pow(X, -5.0 / 3.0);
-    c.aI = apref * NlowA * Xm53 * pow(1.0 + pow(X, xA * (3.0 * kap - 1.0) / 6.0) * pow(NlowA / NhighA, xA), -1.0 / xA);
+    K.v[KK_LOWA] = lowA;
+    K.v[KK_RA] = pow(lowA / NhighA, xA);
+    K.v[KK_EA] = xA * (3.0 * kap - 1.0) / 6.0;
+    K.v[KK_MA] = -1.0 / xA;
     T NlowQA = -(25.0 / 48.0) * lowA;
     T NhighQA = -(441.0 * pow(kap, -5.76) + 0.55) * highA + small;
     T xQA = 1.4 * pow(kap, -1.15);
-    c.aQ = -(apref * NlowQA * Xm53 * pow(1.0 + pow(X, xQA * (3.0 * kap - 1.0) / 6.0) * pow(NlowQA / NhighQA, xQA), -1.0 / xQA));
-    T NlowVA = -(0.77 / w) * pow(pow(sth, -2.28) - 1.0, 0.446) * pow(X, -0.35) * pow(kap, -0.7) * lowA;
-    T NhighVA = -(14.3 * pow(w, -0.928)) * sqrt(pow(sth, -2.05) - 1.0) *
-                     (169.0 * pow(kap, -8.0) + 13.0 / 2500.0 * kap - 263.0 / 5000.0 + 47.0 / (200.0 * kap)) / sqrt(X) * highA + small;
+    K.v[KK_NLOWQA] = NlowQA;
+    K.v[KK_RQA] = pow(NlowQA / NhighQA, xQA);
+    K.v[KK_EQA] = xQA * (3.0 * kap - 1.0) / 6.0;
+    K.v[KK_MQA] = -1.0 / xQA;
+    K.v[KK_CVA1] = -(0.77 / w) * pow(kap, -0.7) * lowA;
+    K.v[KK_CVA2] = -(14.3 * pow(w, -0.928)) * (169.0 * pow(kap, -8.0) + 13.0 / 2500.0 * kap - 263.0 / 5000.0 + 47.0 / (200.0 * kap)) * highA;
     T xVA = 1.22 * pow(kap, -1.136) + 0.007;
-    c.aV = -(apref * NlowVA * Xm53 * pow(1.0 + pow(X, xVA * (3.0 * kap - 1.0) / 6.0) * pow(NlowVA / NhighVA, xVA), -1.0 / xVA)) * sgn;
-    // rotativities (Marszewski+ 2021), linear in kappa between the fits
+    K.v[KK_XVA] = xVA;
+    K.v[KK_EVA] = xVA * (3.0 * kap - 1.0) / 6.0;
+    K.v[KK_MVA] = -1.0 / xVA;
+    // rotativities (Marszewski+ 2021), linear in kappa between the fits at kappa = 3.5, 4, 4.5, 5: the fits'
+    // w-dependent factors times the interpolation weights
     T kc = dmin(dmax(kap, 3.5), 5.0);
-    int i = kc < 4.0 ? 1 : (kc < 4.5 ? 2 : 3);
+    const int i = val(kc) < 4.0 ? 1 : (val(kc) < 4.5 ? 2 : 3);
     const double klo = 3.5 + (i - 1) * 0.5;
     T f = (kc - klo) * 2.0;
-    T Xr = nu / ((w * kap) * (w * kap) * nuc * sth);
-    T fQ = (1.0 - f) * kappa_rhoQ_fit<T>(i, w, Xr) + f * kappa_rhoQ_fit<T>(i + 1, w, Xr);
-    T fV = (1.0 - f) * kappa_rhoV_fit<T>(i, w, Xr) + f * kappa_rhoV_fit<T>(i + 1, w, Xr);
-    c.rQ = -(ne * EE * EE * nuc * nuc * sth * sth) / (ME * CL * nu * nu * nu) * fQ;
-    c.rV = 2.0 * (ne * EE * EE * nuc * cth) / (ME * CL * nu * nu) * gK[5] * fV;
+    T sw = sqrt(w), e5 = exp(-5.0 * w);
+    auto aq = [&](int n) -> T {
+        if (n == 1) return 17.0 * w - 3.0 * sw + 7.0 * sw * e5;
+        if (n == 2) return 46.0 * w / 3.0 - 5.0 * sw / 3.0 + 17.0 * sw / 3.0 * e5;
+        if (n == 3) return 14.0 * w - 13.0 * sw / 8.0 + 4.5 * sw * e5;
+        return 12.5 * w - sw + 5.0 * sw * e5;
+    };
+    auto rv = [&](int n) -> T {
+        if (n == 1) return (w * w + 2.0 * w + 1.0) / (3.125 * w * w + 4.0 * w + 1.0);
+        if (n == 2) return (w * w + 54.0 * w + 50.0) / (30.0 * w * w / 11.0 + 134.0 * w + 50.0);
+        if (n == 3) return (w * w + 43.0 * w + 38.0) / (7.0 * w * w / 3.0 + 92.5 * w + 38.0);
+        return (w + 13.0 / 14.0) / (2.0 * w + 13.0 / 14.0);
+    };
+    K.v[KK_FQ0] = (1.0 - f) * aq(i);
+    K.v[KK_FQ1] = f * aq(i + 1);
+    K.v[KK_FV0] = (1.0 - f) * rv(i);
+    K.v[KK_FV1] = f * rv(i + 1);
+    K.v[KK_RHOQ] = EE * EE * nuc * nuc / (ME * CL);
+    K.v[KK_RHOV] = 2.0 * EE * EE * nuc / (ME * CL) * gK[5];
+    K.v[KK_IFIT] = T((double)i);
+}
+// the X-dependent factors of the rotativity fits n = 1..4 (kappa.jl:72-97); p084 = X^0.84, xm = X^-1/2
+template <class T>
+KS_DEV T kappa_rhoQ_x(int n, T X, T p084) {
+    if (n == 1) return 1.0 - exp(-p084 / 30.0) - sin(X / 10.0) * exp(-1.5 * pow(X, 0.471));
+    if (n == 2) return 1.0 - exp(-p084 / 18.0) - sin(X / 6.0) * exp(-1.75 * sqrt(X));
+    if (n == 3) return 1.0 - exp(-p084 / 12.0) - sin(X / 4.0) * exp(-2.0 * pow(X, 0.525));
+    return 1.0 - exp(-p084 / 8.0) - sin(0.375 * X) * exp(-2.25 * pow(X, 0.541));
+}
+template <class T>
+KS_DEV T kappa_rhoV_x(int n, T xm) {
+    const double b = n == 1 ? 0.447 : (n == 2 ? 0.391 : (n == 3 ? 0.348 : 0.313));
+    return 1.0 - 0.17 * log1p(b * xm);
+}
+// One sample's kappa coefficients from the splat's factors: K(f) gives factor f (a value, or a dual carrying its
+// partials in the splat's rows); ne the density, nu the fluid-frame frequency, (sth, cth) the pitch angle.
+template <class T, class KF>
+KS_DEV CoefT<T> kappa_coefs(const KF& K, T ne, T nu, T sth, T cth) {
+    CoefT<T> c;
+    if (!(sth > 0.0)) sth = T(sin_floor(sth));
+    const double small = 1e-40;
+    T X = nu / (K(KK_NUW1) * sth);
+    T pref = ne * K(KK_JPRE) * sth;
+    T X13 = cbrt(X);
+    T pXI = pow(X, K(KK_EI));
+    c.jI = pref * K(KK_NLOWI) * X13 * pow(1.0 + pXI * K(KK_RI), K(KK_MI));
+    c.jQ = -(pref * K(KK_NLOWQ) * X13 * pow(1.0 + pow(X, K(KK_EQ)) * K(KK_RQ), K(KK_MQ)));
+    // the sin^-p - 1 factors from one logarithm of sin^2 (sin_pow_m1)
+    T ls2 = cth * cth < 0.5 ? log1p(-(cth * cth)) : 2.0 * log(sth);
+    T Xm035 = pow(X, -0.35), sX = sqrt(X);
+    T NlowV = K(KK_CV1) * pow_nonneg(expm1(-1.2 * ls2), 0.48) * Xm035;
+    T NhighV = K(KK_CV2) * pow_nonneg(expm1(-1.25 * ls2), 0.44) / sX - small;
+    const double sgn = cth > 0.0 ? 1.0 : (cth < 0.0 ? -1.0 : 0.0);
+    c.jV = fabs(NhighV) < small * small ? T(0.0)
+         : -(pref * NlowV * X13 * pow(1.0 + pXI * pow(NlowV / NhighV, K(KK_XI)), K(KK_MI))) * sgn;
+    T apref = ne * K(KK_APRE) / sth;
+    T Xm53 = pow(X, -5.0 / 3.0);
+    c.aI = apref * K(KK_LOWA) * Xm53 * pow(1.0 + pow(X, K(KK_EA)) * K(KK_RA), K(KK_MA));
+    c.aQ = -(apref * K(KK_NLOWQA) * Xm53 * pow(1.0 + pow(X, K(KK_EQA)) * K(KK_RQA), K(KK_MQA)));
+    T NlowVA = K(KK_CVA1) * pow_nonneg(expm1(-1.14 * ls2), 0.446) * Xm035;
+    T NhighVA = K(KK_CVA2) * sqrt(expm1(-1.025 * ls2)) / sX + small;
+    c.aV = -(apref * NlowVA * Xm53 * pow(1.0 + pow(X, K(KK_EVA)) * pow(NlowVA / NhighVA, K(KK_XVA)), K(KK_MVA))) * sgn;
+    const int i = (int)val(K(KK_IFIT));
+    T p084 = pow(X, 0.84), xm = 1.0 / sX;
+    T fQ = K(KK_FQ0) * kappa_rhoQ_x<T>(i, X, p084) + K(KK_FQ1) * kappa_rhoQ_x<T>(i + 1, X, p084);
+    T fV = K(KK_FV0) * kappa_rhoV_x<T>(i, xm) + K(KK_FV1) * kappa_rhoV_x<T>(i + 1, xm);
+    T inu = 1.0 / nu;
+    c.rQ = -(ne * K(KK_RHOQ) * sth * sth) * inu * inu * inu * fQ;
+    c.rV = ne * K(KK_RHOV) * cth * inu * inu * fV;
     return c;
+}
+template <class T> struct KappaGet {
+    const KappaK<T>& k;
+    KS_DEV T operator()(int f) const { return k.v[f]; }
+};
+// the whole evaluation from the splat's population values (the spacetime tangent's path)
+template <class T>
+KS_DEV CoefT<T> kappa(T ne, T kap, T w, T hyp, const T* gK, T B, T nu, T sth, T cth) {
+    KappaK<T> K;
+    kappa_consts(kap, w, hyp, gK, B, K);
+    return kappa_coefs(KappaGet<T>{K}, ne, nu, sth, cth);
 }
 
 // ---- cap, invariants, rotation -----------------------------------------------------------------------------------

@@ -203,7 +203,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(Waves<S
     // the per-unit cull: a splat that cannot reach any step-taking sample of the unit adds zeros
     {
         const double* u = ub + UB_FIELDS * unit;
-        if (!(u[6] > 0.0) || !splat_may_reach(sp[i], u[0], u[1], u[2], u[3], u[4], u[5])) {
+        if (!(u[6] > 0.0) || !splat_may_reach(sp[i], u[0], u[1], u[2], u[3], u[4], u[5], knots, ktimes)) {
             if (threadIdx.x < ADJ_ROWS) part[(local * ns + i) * ADJ_ROWS + threadIdx.x] = 0.0;
             return;
         }

@@ -16,6 +16,7 @@
 #include <mutex>
 
 #include "ks_kernels.hpp"
+#include "ks_hostsplat.hpp"
 #include "../../include/kerrsplat_hip.h"
 
 using namespace ksd;
@@ -327,7 +328,9 @@ int ks_regenerate(ks_cache* c, const double* alpha, const double* beta, int nalp
     if (marcher != KS_MARCHER_FUSED && !c->store)
         return fail(KS_ERR_ARG, "this cache was built without sample storage; use marcher = Fused(M) and fused_march!");
     if (M < 1) return fail(KS_ERR_ARG, "re-anchoring interval must be positive");
-    if (!(a > 0.0 && a < 1.0)) return fail(KS_ERR_ARG, "spin a = %g outside (0, 1)", a);
+    // negative spins are the y -> -y mirror of positive ones (the reference's reflection gate runs at a = -0.94);
+    // a = 0 is excluded: the closed-form roots and the polar integrals divide by a
+    if (!(fabs(a) > 0.0 && fabs(a) < 1.0)) return fail(KS_ERR_ARG, "spin a = %g outside 0 < |a| < 1", a);
     ks_ctx* x = c->ctx;
     HIPCHK(hipSetDevice(x->device));
     hipStream_t s = x->stream;
@@ -412,69 +415,6 @@ int ks_copy_samples(const ks_cache* c, double* t, double* r, double* th, double*
 }
 
 // ---- models ----
-static double host_besselk(double x, double& k0, double& k1) {
-    // K_nu(x) = int_0^inf exp(-x cosh s) cosh(nu s) ds, trapezoidal rule (exponentially convergent)
-    const double h = 0.125;
-    double s0 = 0.5 * std::exp(-x), s1 = s0;
-    for (int i = 1; i < 4000; ++i) {
-        double s = h * i;
-        double e = std::exp(-x * std::cosh(s));
-        s0 += e; s1 += e * std::cosh(s);
-        if (e < 1e-40 * s0) break;
-    }
-    k0 = h * s0; k1 = h * s1;
-    return k0 + 2.0 * k1 / x;
-}
-static double host_hyp2f1_neg(double a, double b, double c, double z) {
-    double w = z / (z - 1.0), bb = c - b, term = 1.0, sum = 1.0;
-    for (int n = 0; n < 200000; ++n) {
-        term *= (a + n) * (bb + n) / ((c + n) * (n + 1.0)) * w;
-        sum += term;
-        if (std::fabs(term) < 1e-17 * std::fabs(sum)) break;
-    }
-    return std::pow(1.0 - z, -a) * sum;
-}
-
-// digamma for x > 0: the recurrence psi(x) = psi(x + 1) - 1/x up to x >= 10, then the asymptotic series
-// ln x - 1/(2x) - sum B_2k / (2k x^2k) (DLMF 5.5.2, 5.11.2)
-static double host_digamma(double x) {
-    double r = 0.0;
-    while (x < 10.0) { r -= 1.0 / x; x += 1.0; }
-    const double i2 = 1.0 / (x * x);
-    const double ser = i2 * (1.0 / 12.0 - i2 * (1.0 / 120.0 - i2 * (1.0 / 252.0 - i2 * (1.0 / 240.0 - i2 * (1.0 / 132.0 - i2 * (691.0 / 32760.0 - i2 / 12.0))))));
-    return r + std::log(x) - 0.5 / x - ser;
-}
-// kappa's 2F1 factor 2F1(k - 1/3, k + 1; k + 2/3; -k w) (kappa.jl:13-16)
-static double host_kappa_hyp(double k, double w) { return host_hyp2f1_neg(k - 1.0 / 3.0, k + 1.0, k + 2.0 / 3.0, -k * w); }
-// d/d ln Theta (or d/d ln w) of K0/K2 and K1/K2 at x = 1/Theta: -x d/dx with K0' = -K1, K1' = -K0 - K1/x,
-// K2 = K0 + 2 K1/x
-static void host_bessel_ratio_dlog(double xb, double k0, double k1, double k2, double& dk0k2, double& dk1k2) {
-    const double dk1 = -k0 - k1 / xb, dk2 = -k1 - 2.0 * k0 / xb - 4.0 * k1 / (xb * xb);
-    dk0k2 = -xb * (-k1 * k2 - k0 * dk2) / (k2 * k2);
-    dk1k2 = -xb * (dk1 * k2 - k1 * dk2) / (k2 * k2);
-}
-
-// FP32 copies of the per-splat constants (every field converted)
-static void to_f32(const SplatC& a, SplatCf& b) {
-    std::memset(&b, 0, sizeof b);
-    b.x = a.x; b.y = a.y; b.z = a.z;
-    for (int i = 0; i < 9; ++i) b.R[i] = a.R[i];
-    for (int i = 0; i < 3; ++i) { b.is[i] = a.is[i]; b.bh[i] = a.bh[i]; b.u[i] = a.u[i]; }
-    b.t0 = a.t0; b.iw = a.iw; b.omega = a.omega; b.rsup2 = a.rsup2; b.lnA = a.lnA; b.Bmag = a.Bmag;
-    b.p1 = a.p1; b.p2 = a.p2; b.p3 = a.p3;
-    for (int i = 0; i < 6; ++i) b.g[i] = a.g[i];
-    for (int i = 0; i < 3; ++i) b.kc[i] = a.kc[i];
-    b.kr = a.kr;
-    b.knot0 = a.knot0; b.nk = a.nk; b.kt0 = a.kt0; b.kind = a.kind;
-}
-static void to_f32(const SplatAdj& a, SplatAdjf& b) {
-    std::memset(&b, 0, sizeof b);
-    for (int i = 0; i < 4; ++i) b.qn[i] = a.qn[i];
-    b.iqn = a.iqn; b.dk0k2 = a.dk0k2; b.dk1k2 = a.dk1k2;
-    b.sthB = a.sthB; b.cthB = a.cthB; b.sphB = a.sphB; b.cphB = a.cphB;
-    for (int i = 0; i < 8; ++i) b.dpop[i] = a.dpop[i];
-}
-
 int ks_model_create(ks_ctx* ctx, int kind, int nrows, int ns, const double* params, double gamma_max,
                     const double* knots, const double* knot_times, int nknots, ks_model** out) {
     if (!ctx || !out) return fail(KS_ERR_ARG, "ks_model_create: null argument");
@@ -491,100 +431,8 @@ int ks_model_create(ks_ctx* ctx, int kind, int nrows, int ns, const double* para
     HIPCHK(hipSetDevice(ctx->device));
     std::vector<SplatC> h(ns > 0 ? ns : 1);
     std::vector<SplatAdj> ha(ns > 0 ? ns : 1);
-    const double SR2 = -2.0 * std::log(1e-6);
-    for (int i = 0; i < ns; ++i) {
-        const double* p = params + (size_t)nrows * i;
-        SplatC& c = h[i];
-        std::memset(&c, 0, sizeof c);
-        c.x = p[0]; c.y = p[1]; c.z = p[2];
-        double q1 = p[6], q2 = p[7], q3 = p[8], q4 = p[9];
-        double nq = std::sqrt(q1 * q1 + q2 * q2 + q3 * q3 + q4 * q4);
-        double w = q1 / nq, x = q2 / nq, y = q3 / nq, z = q4 / nq;
-        double R[9] = {1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y),
-                       2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x),
-                       2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)};
-        std::memcpy(c.R, R, sizeof R);
-        for (int a_ = 0; a_ < 3; ++a_) c.is[a_] = std::exp(-p[3 + a_]);
-        c.t0 = p[10]; c.iw = std::exp(-p[11]);
-        c.omega = p[nrows - 1];
-        double smax = std::max(std::max(p[3], p[4]), p[5]);
-        c.rsup2 = SR2 * std::exp(2.0 * smax);
-        c.lnA = p[12];
-        c.kind = kind;
-        if (with_knots) {
-            c.knot0 = nknots * i; c.nk = nknots; c.kt0 = 0;
-            // the sphere holding the knots (the centre stays in their convex hull)
-            const double* K = knots + (size_t)3 * nknots * i;
-            double lo[3] = {K[0], K[1], K[2]}, hi[3] = {K[0], K[1], K[2]};
-            for (int k = 1; k < nknots; ++k)
-                for (int a_ = 0; a_ < 3; ++a_) { lo[a_] = std::min(lo[a_], K[3 * k + a_]); hi[a_] = std::max(hi[a_], K[3 * k + a_]); }
-            double r2 = 0.0;
-            for (int a_ = 0; a_ < 3; ++a_) c.kc[a_] = 0.5 * (lo[a_] + hi[a_]);
-            for (int k = 0; k < nknots; ++k) {
-                double d2 = 0.0;
-                for (int a_ = 0; a_ < 3; ++a_) d2 += (K[3 * k + a_] - c.kc[a_]) * (K[3 * k + a_] - c.kc[a_]);
-                r2 = std::max(r2, d2);
-            }
-            c.kr = std::sqrt(r2) * (1.0 + 1e-12);
-        }
-        if (kind == KS_MODEL_THIN) continue;
-        int rB = (kind == KS_MODEL_THERMAL || kind == KS_MODEL_KNOT) ? 14 : 15;
-        c.Bmag = std::exp(p[rB]);
-        double st = std::sin(p[rB + 1]), ct = std::cos(p[rB + 1]), sp = std::sin(p[rB + 2]), cp = std::cos(p[rB + 2]);
-        c.bh[0] = st * cp; c.bh[1] = st * sp; c.bh[2] = ct;
-        SplatAdj& sa = ha[i];
-        std::memset(&sa, 0, sizeof sa);
-        sa.qn[0] = w; sa.qn[1] = x; sa.qn[2] = y; sa.qn[3] = z; sa.iqn = 1.0 / nq;
-        sa.sthB = st; sa.cthB = ct; sa.sphB = sp; sa.cphB = cp;
-        c.u[0] = p[rB + 3]; c.u[1] = p[rB + 4]; c.u[2] = p[rB + 5];
-        if (kind == KS_MODEL_THERMAL || kind == KS_MODEL_KNOT) {
-            double Th = std::exp(p[13]);
-            double k0, k1;
-            const double xb = 1.0 / Th;
-            double k2 = host_besselk(xb, k0, k1);
-            c.p1 = Th;
-            c.p2 = k2 > 0.0 ? k0 / k2 : 1.0;
-            c.p3 = k2 > 0.0 ? k1 / k2 : 1.0;
-            // d/d ln Theta = -x d/dx at x = 1/Theta, with K0' = -K1, K1' = -K0 - K1/x, K2 = K0 + 2 K1/x
-            if (k2 > 0.0) host_bessel_ratio_dlog(xb, k0, k1, k2, sa.dk0k2, sa.dk1k2);
-        } else if (kind == KS_MODEL_POWERLAW) {
-            double pp = p[13];
-            c.p1 = pp; c.p2 = std::exp(p[14]); c.p3 = gamma_max;
-            c.g[0] = std::tgamma((3 * pp - 1) / 12.0) * std::tgamma((3 * pp + 19) / 12.0);
-            c.g[1] = std::tgamma((3 * pp + 2) / 12.0) * std::tgamma((3 * pp + 22) / 12.0);
-            // d/dp of the Gamma products: Gamma'(x) = Gamma(x) psi(x), dx/dp = 1/4
-            sa.dpop[0] = c.g[0] * 0.25 * (host_digamma((3 * pp - 1) / 12.0) + host_digamma((3 * pp + 19) / 12.0));
-            sa.dpop[1] = c.g[1] * 0.25 * (host_digamma((3 * pp + 2) / 12.0) + host_digamma((3 * pp + 22) / 12.0));
-        } else {
-            double kap = p[13], wk = std::exp(p[14]);
-            c.p1 = kap; c.p2 = wk; c.p3 = host_kappa_hyp(kap, wk);
-            c.g[0] = std::tgamma(kap - 4.0 / 3.0); c.g[1] = std::tgamma(kap - 2.0);
-            c.g[2] = std::tgamma(kap / 4.0 - 1.0 / 3.0); c.g[3] = std::tgamma(kap / 4.0 + 4.0 / 3.0);
-            c.g[4] = std::tgamma(2.0 + kap / 2.0);
-            double k0, k1;
-            const double xw = 1.0 / wk;
-            double k2 = host_besselk(xw, k0, k1);
-            c.g[5] = k0 / (k2 + 1e-40);
-            // the 2F1 factor's derivatives: in kappa by a four-point central difference (Richardson, error
-            // ~h^4), in ln w in closed form, w dF/dw = w (-kappa) (a b / c) 2F1(a + 1, b + 1; c + 1; z)
-            const double h = 1e-3 * kap;
-            sa.dpop[0] = (8.0 * (host_kappa_hyp(kap + h, wk) - host_kappa_hyp(kap - h, wk)) -
-                          (host_kappa_hyp(kap + 2 * h, wk) - host_kappa_hyp(kap - 2 * h, wk))) / (12.0 * h);
-            {
-                const double a2 = kap - 1.0 / 3.0, b2 = kap + 1.0, c2 = kap + 2.0 / 3.0;
-                sa.dpop[1] = wk * (-kap) * a2 * b2 / c2 * host_hyp2f1_neg(a2 + 1.0, b2 + 1.0, c2 + 1.0, -kap * wk);
-            }
-            sa.dpop[2] = c.g[0] * host_digamma(kap - 4.0 / 3.0);
-            sa.dpop[3] = c.g[1] * host_digamma(kap - 2.0);
-            sa.dpop[4] = c.g[2] * 0.25 * host_digamma(kap / 4.0 - 1.0 / 3.0);
-            sa.dpop[5] = c.g[3] * 0.25 * host_digamma(kap / 4.0 + 4.0 / 3.0);
-            sa.dpop[6] = c.g[4] * 0.5 * host_digamma(2.0 + kap / 2.0);
-            // d(K0 / (K2 + 1e-40))/d ln w at 1/w (the 1e-40 guard is far below K2 for every w of the fits)
-            double dk0k2 = 0.0, dk1k2 = 0.0;
-            if (k2 > 0.0) host_bessel_ratio_dlog(xw, k0, k1, k2, dk0k2, dk1k2);
-            sa.dpop[7] = dk0k2;
-        }
-    }
+    for (int i = 0; i < ns; ++i)
+        host_splat_constants(kind, nrows, i, params + (size_t)nrows * i, gamma_max, with_knots, knots, nknots, h[i], ha[i]);
     ks_model* m = new ks_model;
     m->id = next_model_id();
     m->ctx = ctx; m->kind = kind; m->nrows = nrows; m->ns = ns;
@@ -598,7 +446,7 @@ int ks_model_create(ks_ctx* ctx, int kind, int nrows, int ns, const double* para
     {
         std::vector<SplatCf> hf(h.size());
         std::vector<SplatAdjf> haf(ha.size());
-        for (size_t i = 0; i < h.size(); ++i) { to_f32(h[i], hf[i]); to_f32(ha[i], haf[i]); }
+        for (size_t i = 0; i < h.size(); ++i) { splat_as(h[i], hf[i]); adj_as(ha[i], haf[i]); }
         HIPCHK(hipMemcpy(m->d_splats_f, hf.data(), hf.size() * sizeof(SplatCf), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(m->d_adj_f, haf.data(), haf.size() * sizeof(SplatAdjf), hipMemcpyHostToDevice));
     }
@@ -671,7 +519,7 @@ int ks_model_composite(ks_ctx* ctx, int nparts, const ks_model* const* parts, ks
         }
         off += q->ns; okn += q->nkn; okt += q->nkt;
     }
-    for (size_t i = 0; i < n; ++i) to_f32(h[i], hf[i]);
+    for (size_t i = 0; i < n; ++i) splat_as(h[i], hf[i]);
     HIPCHK_M(hipMemcpy(m->d_splats, h.data(), n * sizeof(SplatC), hipMemcpyHostToDevice));
     HIPCHK_M(hipMemcpy(m->d_splats_f, hf.data(), n * sizeof(SplatCf), hipMemcpyHostToDevice));
     if (nkn > 0) {

@@ -290,6 +290,23 @@ int kso_polarized_composite(double a, double thetao, int npix, const double* alp
     return 0;
 }
 
+// algorithmic operation count of the Recurrence(M) march alone over a set of rays (Counted<double>): the samples
+// (t, r, theta, phi, momentum signs) of every ray, consumed by nothing
+int kso_count_march(double a, double thetao, int npix, const double* alphas, const double* betas, int N, int M, double* flops,
+                    double* special) {
+    double f = 0, s = 0;
+#pragma omp parallel for reduction(+ : f, s) schedule(dynamic, 4)
+    for (int i = 0; i < npix; ++i) {
+        op_counts().reset();
+        Pixel<Counted> p = pixel_constants<Counted>(Counted(a), Counted(alphas[i]), Counted(betas[i]), Counted(thetao));
+        Counted rt, rp;
+        march_ray(p, N, M, [](int, const Sample<Counted>&, Counted) {}, rt, rp);
+        f += op_counts().flops; s += op_counts().special;
+    }
+    *flops = f; *special = s;
+    return 0;
+}
+
 // algorithmic operation count of the polarized forward transport over a set of rays (Counted<double>)
 int kso_count_polarized(double a, double thetao, int npix, const double* alphas, const double* betas, int N, int M,
                         int kind, int nrows, int ns, const double* params, const double* hyp, double gamma_max,
