@@ -1,18 +1,25 @@
-"""Benchmark: ray-samples/s of the geodesic march + polarized transfer + dual-sweep adjoint on one MI355X.
+"""Benchmark: ray-samples/s of the fit step (geodesic march + polarized transfer + visibility chi2 + dual-sweep
+adjoint) on one MI355X, the mixed FP32 path against the plain FP64 path.
 
-Workload (the configuration the headline metric is quoted on that fits one GPU): a = 0.5,
-theta_o = 60 deg, 128 x 128 screen over +-15 M, 1024 Mino-time samples per ray, 32 thermal splats,
-full IQUV at 230 GHz, visibilities on 512 uv points (seeded synthetic draws). The per-pixel
-constants (K0/K1) are built once beforehand and the splat parameters are resident on the device.
-One fwd+adjoint step is the device pipeline of a visibility fit iteration (frame_chi2_seed! between
-the two passes of the dual sweep): the Recurrence(64) march of every ray into sample storage, the
-polarized transfer with the adjoint records, the visibilities of the image with their χ² against
-fixed synthetic data and the adjoint seed of the weighted residuals (left on the device), and the
-reverse pass (the gradient with respect to all 21 x 32 splat parameters). The forward-only step
-(march + transfer + visibilities) is timed beside it.
+Workload (default: the full fit step the headline metric is quoted on): a = 0.9, theta_o = 163 deg, 512 x 512
+screen over +-15 M, 2048 Mino-time samples per ray, 1024 kappa-electron splats (22 rows each) whose centres follow
+4 trajectory knots (RK4 in Boyer-Lindquist coordinates, t = -100 .. 200 M), 3 frequencies (86, 230, 345 GHz) x 32
+slow-light frames over 100 M, 8192 uv points; seeded synthetic draws (kerrsplat_amd/synthetic.py).
 
-Prints one JSON line; value = ray-samples per second (pixels x samples x frames x frequencies / s)
-of the fwd+adjoint step.
+One fit step: the model build (the per-splat constants on the host and their upload), the Recurrence(64) march
+of every ray into sample storage, then for each of the 96 (frame, frequency) slices the polarized transfer with
+the adjoint records, the slice's visibilities with their chi2 against synthetic data and the adjoint seed of the
+weighted residuals (on the device), and the reverse pass that adds the slice's share of the gradient with
+respect to all 22 x 1024 splat parameters. A bench step is one slice (the slices cycle through the steps); the
+model build and the march happen once per fit step, so each bench step is charged 1/96 of their measured
+duration:
+    ms_per_step = (slice wall time, averaged over the timed steps) + (model build + march) / 96,
+    value       = pixels x samples / ms_per_step  (= the fit step's ray-samples / its duration).
+The FP64 path runs the same steps (fewer of them by default: a slice is several times longer); the mixed
+path's images and gradient of one slice are compared with the FP64 path's (Stokes error over the peak I,
+gradient error over the largest entry), and the mixed path is the headline only within 1e-4 and 1e-3.
+
+Prints one JSON line.
 """
 import argparse
 import json
@@ -34,11 +41,15 @@ FP32_PEAK_TFLOPS = 157.3
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=5)
-    p.add_argument("--warmup", type=int, default=2)
-    p.add_argument("--config", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5, help="timed slices of the mixed path")
+    p.add_argument("--warmup", type=int, default=2, help="untimed slices before them")
+    p.add_argument("--fp64-steps", type=int, default=1, help="timed slices of the FP64 path (at most --steps)")
+    p.add_argument("--fp64-warmup", type=int, default=1)
+    p.add_argument("--config", type=int, default=4)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
+    p.add_argument("--paths", default="mixed,fp64",
+                   help="the paths to run (a profiling run may take one; the headline needs both)")
     p.add_argument("--copy-probe", action="store_true",
                    help="also run device-to-device copies of 256 MiB (the reference for scaling HBM byte counters)")
     return p.parse_args()
@@ -62,34 +73,68 @@ def copy_probe(nbytes=1 << 28, reps=3):
     hip.hipFree(b)
 
 
-def cpu_baseline(cfg, P, L, target_s, ds):
-    """The oracle (FP64 C++ restatement of the reference's per-ray CPU path: Recurrence(64) march,
-    transfer and dual sweep, OpenMP over all host cores) on a bounded random subset of the same rays
-    with the same seeds; fwd+adjoint ray-samples/s."""
+def model_parts(syn, cfg_id):
+    """The configuration's model as (kind, params[, knots, knot_times]) parts."""
+    m = syn.config_model(cfg_id)
+    if isinstance(m, list):
+        return m
+    return [m]
+
+
+def build_model(K, parts):
+    ms = []
+    for p in parts:
+        kind, P = p[0], p[1]
+        if len(p) > 2:
+            ms.append(K.Model(P, kind=kind, knots=p[2], knot_times=p[3]))
+        else:
+            ms.append(K.Model(P, kind=kind))
+    return ms[0] if len(ms) == 1 else K.CompositeModel(*ms)
+
+
+def cpu_baseline(cfg, parts, L, target_s, t_obs, nu, res):
+    """The oracle (FP64 C++ restatement of the reference's per-ray CPU path: Recurrence(64) march, transfer and
+    dual sweep with every row of every splat, OpenMP over all host cores) on a bounded random subset of the
+    same rays at the first slice; fwd+adjoint ray-samples/s."""
     import oracle
-    a, th, res, N = cfg["a"], math.radians(cfg["theta_o_deg"]), cfg["res"], cfg["N"]
+    a, th, N = cfg["a"], math.radians(cfg["theta_o_deg"]), cfg["N"]
     from kerrsplat_amd import Camera
     cam = Camera((-cfg["fov"], cfg["fov"]), (-cfg["fov"], cfg["fov"]), res)
-    dso = ds[:, :, 0, 0, :].reshape(cam.npix, 4, order="F")
+    oparts = []
+    for p in parts:
+        kind, P = p[0], p[1]
+        if kind == "kappa":
+            hyp = [oracle.kappa_hyp(P[13, i], math.exp(P[14, i])) for i in range(P.shape[1])]
+            oparts.append(("kappa", P, hyp, 1e5) + ((p[2], p[3]) if len(p) > 2 else ()))
+        elif kind == "powerlaw":
+            oparts.append(("powerlaw", P, None, 1e5))
+        else:
+            oparts.append(tuple(p))
     threads = oracle.num_threads()
     rng = np.random.default_rng(7)
     order = rng.permutation(cam.npix)
-    n = 64
-    t0 = time.perf_counter()
-    oracle.polarized_gradient(a, th, cam.alphas[order[:n]], cam.betas[order[:n]], N, P, 0.0, cfg["nus"][0], L, dso[order[:n]])
-    dt = time.perf_counter() - t0
-    n2 = int(min(cam.npix, max(n, n * target_s / max(dt, 1e-3))))
-    t0 = time.perf_counter()
-    oracle.polarized_gradient(a, th, cam.alphas[order[:n2]], cam.betas[order[:n2]], N, P, 0.0, cfg["nus"][0], L, dso[order[:n2]])
-    dt2 = time.perf_counter() - t0
-    return {"value": n2 * N / dt2, "unit": "ray-samples/s", "cores": threads, "kind": "port",
-            "sample": f"{n2} of {cam.npix} rays (random subset) x {N} samples, fwd+adjoint (dual sweep), FP64 oracle, {dt2:.1f} s"}
+    ds = rng.normal(size=(cam.npix, 4))
+
+    def run(n):
+        idx = order[:n]
+        t0 = time.perf_counter()
+        oracle.model_gradient(a, th, cam.alphas[idx], cam.betas[idx], N, oparts, t_obs, nu, L, ds[idx])
+        return time.perf_counter() - t0
+
+    n = 1
+    dt = run(n)
+    while dt < 0.2 * target_s and n < cam.npix:
+        n = min(cam.npix, max(n + 1, int(n * min(8.0, target_s / max(dt, 1e-3)))))
+        dt = run(n)
+    return {"value": n * N / dt, "unit": "ray-samples/s", "cores": threads, "kind": "port",
+            "sample": f"{n} of {cam.npix} rays (random subset) x {N} samples at one (frame, frequency), fwd+adjoint "
+                      f"(dual sweep over every splat), FP64 oracle, {dt:.1f} s"}
 
 
 def visibility_stage(K, syn, cfg, model, L, res=512, nuv=8192, reps=3):
-    """The visibility transform's matrix-core product at the largest configuration's size (512 x 512
-    screen, 8192 uv points, one frame; the image rendered from the same splats over a fused 64-sample
-    march): HIP-event duration of the FP64 GEMM and its rate against the FP64 matrix peak."""
+    """The visibility transform's matrix-core product at the largest configuration's size (512 x 512 screen,
+    8192 uv points, one frame; the image rendered from the same splats over a fused 64-sample march): HIP-event
+    duration of the FP64 GEMM and its rate against the FP64 matrix peak."""
     a, th = cfg["a"], math.radians(cfg["theta_o_deg"])
     cam = K.Camera((-cfg["fov"], cfg["fov"]), (-cfg["fov"], cfg["fov"]), res)
     cache = K.GeodesicCache(cam, 64, store_samples=False)
@@ -119,149 +164,202 @@ def main():
     from kerrsplat_amd import api, synthetic as syn
 
     cfg = syn.CONFIGS[args.config]
+    if cfg["kind"] == "thin":
+        raise SystemExit("the fit-step bench needs a polarized configuration (1, 2, 3 or 4)")
     a, th, res, N = cfg["a"], math.radians(cfg["theta_o_deg"]), cfg["res"], cfg["N"]
     cam = K.Camera((-cfg["fov"], cfg["fov"]), (-cfg["fov"], cfg["fov"]), res)
     L = syn.length_unit()
-    P = syn.thermal_splats(args.config, a, cfg["ns"])
+    parts = model_parts(syn, args.config)
+    times, nus = list(cfg["times"]), list(cfg["nus"])
+    slices = [(t, nu) for nu in nus for t in times]            # frame fastest, as the device numbers them
+    nsl = len(slices)
+    lib, ctx = api._load(), api.context()
+    sync = lambda: lib.ks_ctx_sync(ctx)
+    print(f"bench: {cfg['name']}: {res}x{res} x {N}, {sum(p[1].shape[1] for p in parts)} splats, {nsl} slices", file=sys.stderr, flush=True)
+
     cache = K.GeodesicCache(cam, N, store_samples=True)
     K.regenerate(cache, a, th, marcher=K.Recurrence(64))
-    model = K.Model(P, kind="thermal")
-    ds = np.random.default_rng(20261202).normal(size=(res, res, len(cfg["times"]), len(cfg["nus"]), 4))
-    u, v = syn.uv_disc(args.config, cfg["nuv"])
+    model = build_model(K, parts)
+    nuv = cfg["nuv"]
+    u, v = syn.uv_disc(args.config, max(nuv, 1))
     psize = syn.pixel_size_rad(-cfg["fov"], cfg["fov"], res)
-    lib, ctx = api._load(), api.context()
-    times, nus = cfg["times"], cfg["nus"]
-    nframes = len(times) * len(nus)
-    # fixed synthetic visibility data: the FP64 model's visibilities, perturbed by 5 % and thermal noise
-    rng = np.random.default_rng(20261203)
+
+    # the once-per-fit-step parts, timed on their own: the model build (host constants + upload) and the march
+    t0 = time.perf_counter()
+    for _ in range(3):
+        model = build_model(K, parts)
+    sync()
+    model_ms = 1e3 * (time.perf_counter() - t0) / 3
+    march_ms = []
+    for _ in range(3):
+        K.march(cache)
+        march_ms.append(K.stage_ms("march"))
+    sync()
+    march_ms = float(np.mean(march_ms))
+    once_ms = model_ms + march_ms
+
+    # fixed synthetic visibility data: the FP64 model's visibilities of the first slice, perturbed by 5 % and
+    # thermal noise at 1 % of the peak, with a different draw for each slice
     K.precision(cache, "fp64")
-    K.render_device(cache, model, times, nus, L)
-    V0 = K.visibilities(cache, nframes, psize, u, v) if cfg["nuv"] else None
-    if cfg["nuv"]:
-        sig = 0.01 * np.abs(V0[..., 0]).max() * np.ones(V0.shape)
-        vdata = V0 * (1 + 0.05 * rng.normal(size=V0.shape)) + sig * (rng.normal(size=V0.shape) + 1j * rng.normal(size=V0.shape))
-    dparams = np.zeros(P.shape, order="F")
-    chis = []
+    rng = np.random.default_rng(20261203)
+    vdata, sig = None, None
+    if nuv:
+        K.gradient_forward(cache, model, [slices[0][0]], [slices[0][1]], L, copy=False)
+        V0 = K.visibilities(cache, 1, psize, u, v)[0]
+        sig = 0.01 * np.abs(V0[:, 0]).max() * np.ones(V0.shape)
+        vdata = [V0 * (1 + 0.05 * rng.normal(size=V0.shape)) + sig * (rng.normal(size=V0.shape) + 1j * rng.normal(size=V0.shape))
+                 for _ in range(nsl)]
+    ds = np.random.default_rng(20261202).normal(size=(1, 1, cam.npix, 4))     # the image seed when there is no uv set
+    dparams = np.zeros(model.grad_shape, order="F")
+    STG = ["transfer", "adjoint_steps", "adjoint_elements"] + (["vis"] if nuv else [])
 
-    def forward_step():
-        K.march(cache)
-        K.render_device(cache, model, times, nus, L)
-        if cfg["nuv"]:
-            K.visibilities(cache, nframes, psize, u, v)
+    def slice_step(s, dp, copy=False):
+        """One slice: forward pass with the records, the visibility chi2 and seed, the reverse pass."""
+        t, nu = slices[s % nsl]
+        cube = K.gradient_forward(cache, model, [t], [nu], L, copy=copy)
+        chi = None
+        if nuv:
+            chi = K.vis_chi2_seed(cache, 0, psize, u, v, vdata[s % nsl], sig)
+            K.gradient_reverse(cache, model, dp)
+        else:
+            K.gradient_reverse(cache, model, dp, ds)
+        return chi, cube
 
-    def adjoint_step():
-        K.march(cache)
-        if not cfg["nuv"]:
-            K.polarized_gradient(cache, model, times, nus, L, ds, dparams=dparams)
-            return
-        K.gradient_forward(cache, model, times, nus, L, copy=False)
-        chis.append(sum(K.vis_chi2_seed(cache, f, psize, u, v, vdata[f], sig[f]) for f in range(nframes)))
-        K.gradient_reverse(cache, model, dparams)
-
-    def timed(step, stages):
-        """W untimed warmups, then K steps bracketed by stream synchronizations; per step, each listed
-        stage's kernel duration from the HIP events the library records around its launch."""
-        for _ in range(args.warmup):
-            step()
-        lib.ks_ctx_sync(ctx)
-        ms = {k: [] for k in stages}
+    def timed(steps, warmup):
+        """warmup untimed slices (the first, slice 0, also gives the images and gradient of the error bars), then
+        `steps` slices bracketed by stream synchronizations; per step, each stage's kernel time from the HIP events
+        the library records around its launches."""
+        g0 = np.zeros(model.grad_shape, order="F")
+        _, cube0 = slice_step(0, g0, copy=True)
+        for s in range(1, warmup):
+            slice_step(s, dparams)
+        sync()
+        ms = {k: [] for k in STG}
+        chis = []
+        first = max(warmup, 1)
         t0 = time.perf_counter()
-        for _ in range(args.steps):
-            step()
-            for k in stages:
+        for s in range(first, first + steps):
+            chis.append(slice_step(s, dparams)[0])
+            for k in STG:
                 ms[k].append(api.stage_ms(k))
-        lib.ks_ctx_sync(ctx)
-        return (time.perf_counter() - t0) / args.steps, {k: float(np.mean(v)) for k, v in ms.items()}
+        sync()
+        return (time.perf_counter() - t0) / steps, {k: float(np.mean(x)) for k, x in ms.items()}, chis, cube0, g0
 
-    STG_FWD, STG_ADJ = ["march", "transfer"], ["march", "transfer", "adjoint_steps", "adjoint_elements"]
-    work = cam.npix * N * len(times) * len(nus)
     runs = {}
-    for prec in ("fp64", "mixed"):
+    for prec in [p_ for p_ in ("mixed", "fp64") if p_ in args.paths.split(",")]:
         K.precision(cache, prec)
-        fwd_s, fwd_ms = timed(forward_step, STG_FWD)
-        adj_s, adj_ms = timed(adjoint_step, STG_ADJ)
-        dparams[:] = 0.0
-        adjoint_step()
-        cube = K.gradient_forward(cache, model, times, nus, L)
-        runs[prec] = {"fwd_s": fwd_s, "fwd_ms": fwd_ms, "adj_s": adj_s, "adj_ms": adj_ms, "grad": dparams.copy(), "cube": cube,
+        steps = args.steps if prec == "mixed" else max(1, min(args.steps, args.fp64_steps))
+        warm = args.warmup if prec == "mixed" else min(args.warmup, args.fp64_warmup)
+        print(f"bench: {prec}: {max(warm, 1)} warmup + {steps} timed slices", file=sys.stderr, flush=True)
+        wall, stage, chis, cube, g = timed(steps, warm)
+        runs[prec] = {"wall_s": wall, "stage_ms": stage, "steps": steps, "warmup": max(warm, 1), "cube": cube, "grad": g,
                       "chi2": chis[-1] if chis else None}
-    img_err = float(np.abs(runs["mixed"]["cube"] - runs["fp64"]["cube"]).max() / np.abs(runs["fp64"]["cube"][..., 0]).max())
-    grad_err = float(np.abs(runs["mixed"]["grad"] - runs["fp64"]["grad"]).max() / np.abs(runs["fp64"]["grad"]).max())
-    head = "mixed" if img_err <= 1e-4 else "fp64"        # the mixed path counts only within its error bar
-    adj_s, adj_ms = runs[head]["adj_s"], runs[head]["adj_ms"]
-    ms_per_step = 1e3 * adj_s
-    value = work / adj_s
+        print(f"bench: {prec}: {1e3 * wall:.0f} ms per slice {stage}", file=sys.stderr, flush=True)
+    K.precision(cache, "fp64")
 
-    # per-stage rooflines from the algorithmic work model (oracle on the Counted scalar): VALU bound,
-    # against the FP64 vector peak on the FP64 path and the FP32 vector peak on the mixed path
+    both = "mixed" in runs and "fp64" in runs
+    img_err = grad_err = None
+    finite = all(bool(np.isfinite(r["grad"]).all() and np.isfinite(r["cube"]).all()) for r in runs.values())
+    if both:
+        peak = np.abs(runs["fp64"]["cube"][..., 0]).max()
+        img_err = float(np.abs(runs["mixed"]["cube"] - runs["fp64"]["cube"]).max() / peak)
+        grad_err = float(np.abs(runs["mixed"]["grad"] - runs["fp64"]["grad"]).max() / np.abs(runs["fp64"]["grad"]).max())
+        head = "mixed" if (img_err <= 1e-4 and grad_err <= 1e-3 and finite) else "fp64"
+    else:
+        head = next(iter(runs))         # a profiling run of one path: no error bars, no headline comparison
+    per_slice = cam.npix * N
+    ms_step = {p_: 1e3 * r["wall_s"] + once_ms / nsl for p_, r in runs.items()}
+    ms_per_step = ms_step[head]
+    value = per_slice / (ms_per_step * 1e-3)
+
+    # the dominant kernel's roofline: algorithmic flops per ray-sample from the work model (profiles/work_model.json,
+    # the per-ray-sample operation counts of this workload), against the FP32 vector peak on the mixed path and
+    # the FP64 vector peak on the FP64 path
     wm_path = os.path.join(ROOT, "profiles", "work_model.json")
     wm = json.load(open(wm_path)).get(cfg["name"]) if os.path.exists(wm_path) else None
     tpl = {"fp64": "double", "mixed": "f32"}
-    cull = "true" if 64 <= cfg["ns"] <= 1024 else "false"      # the transfer's per-tile cull (from 64 to 1024 splats)
-    kernel_of = {"march": lambda p_: "ks_march_store<1>", "transfer": lambda p_: f"ks_chunk_transfer<1, {tpl[p_]}, {cull}>",
-                 "adjoint_steps": lambda p_: f"ks_adj_segment<{tpl[p_]}>", "adjoint_elements": lambda p_: f"ks_adj_splats<{tpl[p_]}>"}
-    # HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE summary (rocprofv3 --pmc, scaled by a copy probe)
-    tr_path = os.path.join(ROOT, "profiles", "traffic_128x128_N1024_32thermal.json")
-    traffic = json.load(open(tr_path))["kernels"] if (os.path.exists(tr_path) and args.config == 1) else {}
+    nsp = sum(p[1].shape[1] for p in parts)
+    cull = "true" if 64 <= nsp <= 1024 else "false"          # the transfer's per-tile cull (from 64 to 1024 splats)
+    kernel_of = {"transfer": lambda p_: f"ks_chunk_transfer<{model_kind_id(parts)}, {tpl[p_]}, {cull}>",
+                 "adjoint_steps": lambda p_: f"ks_adj_segment<{tpl[p_]}>",
+                 "adjoint_elements": lambda p_: f"ks_adj_splats<{tpl[p_]}, {adj_pops(parts)}>",
+                 "vis": lambda p_: "ks_dgemm<false>"}
+    tr_path = os.path.join(ROOT, "profiles", f"traffic_{cfg['name']}.json")
+    traffic = json.load(open(tr_path))["kernels"] if os.path.exists(tr_path) else {}
 
     def stage_table(prec):
-        peak = FP64_PEAK_TFLOPS if prec == "fp64" else FP32_PEAK_TFLOPS
+        peak_ = FP64_PEAK_TFLOPS if prec == "fp64" else FP32_PEAK_TFLOPS
         out = {}
-        for k, t_ms in runs[prec]["adj_ms"].items():
+        for k, t_ms in runs[prec]["stage_ms"].items():
             st = {"kernel": kernel_of[k](prec), "ms": t_ms}
             if st["kernel"] in traffic:
                 st["hbm_bytes_per_launch"] = traffic[st["kernel"]]["bytes"]
-            if wm and "phases" in wm:
-                ph = wm["phases"]
-                per = {"march": ph["march"]["flops_per_ray_sample"], "transfer": ph["transfer"]["flops_per_ray_sample"],
-                       "adjoint_steps": ph["adjoint_steps"]["flops_per_ray_sample"] + ph["tails"]["flops_per_ray_sample"],
-                       "adjoint_elements": ph["adjoint_elements"]["flops_per_ray_sample"]}[k]
-                units = cam.npix * N if k == "march" else work
-                achieved = per * units / (t_ms * 1e-3) / 1e12
-                pk = FP64_PEAK_TFLOPS if k == "march" else peak     # the geodesics are FP64 on both paths
-                st.update({"flops_per_ray_sample": per, "achieved_tflops": achieved, "peak_tflops": pk, "frac": achieved / pk})
+            ph = (wm or {}).get("phases_fp64_path" if prec == "fp64" and "phases_fp64_path" in (wm or {}) else "phases", {}).get(k)
+            if ph and k != "vis":
+                achieved = ph["flops_per_ray_sample"] * per_slice / (t_ms * 1e-3) / 1e12
+                st.update({"flops_per_ray_sample": ph["flops_per_ray_sample"], "achieved_tflops": achieved, "peak_tflops": peak_,
+                           "frac": achieved / peak_})
             out[k] = st
         return out
 
     stages = {p_: stage_table(p_) for p_ in runs}
-    roofline = None
-    dom = max(adj_ms, key=adj_ms.get)
+    sm = runs[head]["stage_ms"]
+    dom = max((k for k in sm if k != "vis"), key=sm.get)
     d = stages[head][dom]
+    roofline = None
     if "achieved_tflops" in d:
-        roofline = {"bound": "valu-fp64" if (head == "fp64" or dom == "march") else "valu-fp32", "achieved": d["achieved_tflops"],
-                    "peak": d["peak_tflops"], "unit": "TFLOP/s", "frac": d["frac"], "traffic": d.get("hbm_bytes_per_launch"), "kernel": d["kernel"],
-                    "kernel_ms": d["ms"], "flops_per_ray_sample": d["flops_per_ray_sample"]}
-    paths = {p_: {"fwd_adjoint": {"value": work / r["adj_s"], "unit": "ray-samples/s", "ms_per_step": 1e3 * r["adj_s"],
-                                  "gradient_evals_per_s": 1.0 / r["adj_s"], "stage_ms": r["adj_ms"]},
-                  "forward": {"value": work / r["fwd_s"], "unit": "ray-samples/s", "ms_per_step": 1e3 * r["fwd_s"], "stage_ms": r["fwd_ms"]}}
+        roofline = {"bound": "mfma" if dom == "vis" else "valu", "achieved": d["achieved_tflops"], "peak": d["peak_tflops"],
+                    "unit": "TFLOP/s", "frac": d["frac"], "traffic": d.get("hbm_bytes_per_launch"), "kernel": d["kernel"],
+                    "kernel_ms_per_slice": d["ms"], "flops_per_ray_sample": d["flops_per_ray_sample"],
+                    "precision": "fp32" if head == "mixed" else "fp64"}
+    paths = {p_: {"value": per_slice / (ms_step[p_] * 1e-3), "unit": "ray-samples/s", "ms_per_step": ms_step[p_],
+                  "slice_wall_ms": 1e3 * r["wall_s"], "gradient_evals_per_s": 1.0 / (nsl * ms_step[p_] * 1e-3),
+                  "steps": r["steps"], "warmup": r["warmup"], "stage_ms": r["stage_ms"], "vis_chi2_last": r["chi2"]}
              for p_, r in runs.items()}
 
-    vis_stage = visibility_stage(K, syn, cfg, model, L) if cfg["nuv"] else None
+    vis_stage = visibility_stage(K, syn, cfg, model, L) if nuv else None
     if args.copy_probe:
         copy_probe()
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(cfg, P, L, args.cpu_seconds, ds)
-    out = {"metric": METRIC, "value": value, "unit": "ray-samples/s", "n_gpus": 1, "steps": args.steps,
-           "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        cpu = cpu_baseline(cfg, parts, L, args.cpu_seconds, slices[0][0], slices[0][1], res)
+    out = {"metric": METRIC, "value": value, "unit": "ray-samples/s", "n_gpus": 1, "steps": runs[head]["steps"],
+           "warmup": runs[head]["warmup"], "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "mixed f32/f64" if head == "mixed" else "f64",
-           "data": "synthetic (seeded splat draws and uv disc)",
+           "data": "synthetic (seeded splat draws, RK4 trajectory knots, uv disc, visibility data from the FP64 model)",
            "config": {"workload": cfg["name"], "a": a, "theta_o_deg": cfg["theta_o_deg"], "screen": [res, res],
-                      "samples_per_ray": N, "splats": cfg["ns"], "kind": cfg["kind"], "frames": len(times),
-                      "frequencies": len(nus), "uv_points": cfg["nuv"],
-                      "path": ("mixed: FP64 geodesics (Recurrence(64) into stored samples, anchor segments in parallel), "
-                               "FP32 transfer and dual-sweep adjoint in 32-sample chunks (FP64 step above Faraday depth 10), "
-                               "visibility chi2 and seed on the device (FP64 GEMMs)")
+                      "samples_per_ray": N, "splats": sum(p[1].shape[1] for p in parts), "kind": cfg["kind"],
+                      "knots": len(cfg.get("knot_times", [])), "frames": len(times), "frequencies": len(nus), "uv_points": nuv,
+                      "step": "one (frame, frequency) slice of the fit step: transfer with records, visibility chi2 and seed, "
+                              "reverse pass; plus 1/%d of the model build and the march" % nsl,
+                      "path": ("mixed: FP64 geodesics (Recurrence(64) into stored samples), FP32 transfer and dual-sweep adjoint "
+                               "(FP64 step above Faraday depth 10), visibility chi2 and seed on the device (FP64 GEMMs)")
                               if head == "mixed" else
-                              "plain FP64: Recurrence(64) march into stored samples (anchor segments in parallel), "
-                              "transfer in 32-sample chunks, dual-sweep adjoint, visibility chi2 and seed on the device"},
-           "mixed_over_fp64": paths["fp64"]["fwd_adjoint"]["ms_per_step"] / paths["mixed"]["fwd_adjoint"]["ms_per_step"],
-           "mixed_error_vs_fp64": {"stokes_over_peak_I": img_err, "gradient_over_max": grad_err},
-           "vis_chi2": {p_: runs[p_]["chi2"] for p_ in runs},
-           "paths": paths, "gradient_evals_per_s": 1.0 / adj_s,
-           "stages": stages,
-           "roofline": roofline, "cpu_baseline": cpu, "vis_stage": vis_stage}
+                              "plain FP64: Recurrence(64) march into stored samples, transfer, dual-sweep adjoint, visibility "
+                              "chi2 and seed on the device"},
+           "mixed_over_fp64": ms_step["fp64"] / ms_step["mixed"] if both else None,
+           "mixed_error_vs_fp64": {"stokes_over_peak_I": img_err, "gradient_over_max": grad_err, "finite": finite},
+           "fit_step_s": {p_: nsl * ms_step[p_] * 1e-3 for p_ in runs},
+           "once_per_fit_step_ms": {"model_build": model_ms, "march": march_ms},
+           "paths": paths, "gradient_evals_per_s": 1.0 / (nsl * ms_per_step * 1e-3),
+           "stages": stages, "roofline": roofline, "cpu_baseline": cpu, "vis_stage": vis_stage}
     print(json.dumps(out), flush=True)
+
+
+def model_kind_id(parts):
+    kinds = {"thermal": 1, "powerlaw": 2, "kappa": 3, "knot": 4}
+    return 5 if len(parts) > 1 else kinds[parts[0][0]]
+
+
+def adj_pops(parts):
+    """The element-adjoint instantiation of the model's populations (1 thermal, 2 power law, 8 kappa, 4 knots; more
+    than one population takes the instantiation of all, 15)."""
+    pops, knots = 0, 0
+    for p in parts:
+        pops |= {"thermal": 1, "knot": 1, "powerlaw": 2, "kappa": 8}[p[0]]
+        if p[0] == "knot" or len(p) > 2:
+            knots = 4
+    return pops | knots if pops in (1, 2, 8) else 15
 
 
 if __name__ == "__main__":

@@ -331,13 +331,305 @@ KS_DEV void element_adjoint_thermal(const SplatCT<S>& c, const SplatAdjT<S>& sa,
     grad[20] += sv * dl[12];
 }
 
-// One power-law or kappa splat's share of one sample (the element of PowerLawSplats / KappaSplats,
-// populations.jl:13-104, differentiated as element_adjoint_thermal): the 22 rows are the 12 geometric rows, ln n_e,
-// the two population rows (p, ln gamma_min or kappa, ln w), ln B, the field angles, u and the pattern rate. The
-// contribution q is taken in five-partial duals over (x1, x2, ln B, g, cos B); the per-splat tables (Gamma
-// products, the kappa 2F1 factor and Bessel ratio) enter with their derivatives in x1, x2 (dpop), so that the
-// 2F1 factor is differentiated with kappa and w, not held; the frame rows join by the chain rule through
-// frame_jacobian.
+// The X-dependent rotativity factors of kappa_coefs with their derivatives in X (kappa_rhoQ_x, kappa_rhoV_x).
+template <class S>
+KS_DEV void kappa_rhoQ_x_d(int n, S X, S p084, S& h, S& dh) {
+    const double d = n == 1 ? 30.0 : (n == 2 ? 18.0 : (n == 3 ? 12.0 : 8.0));
+    const double sd = n == 1 ? 10.0 : (n == 2 ? 6.0 : 4.0);         // sin(X / sd); the fourth fit takes sin(0.375 X)
+    const double is_ = n == 4 ? 0.375 : 1.0 / sd;
+    const double cc = n == 1 ? 1.5 : (n == 2 ? 1.75 : (n == 3 ? 2.0 : 2.25));
+    const double b = n == 1 ? 0.471 : (n == 2 ? 0.5 : (n == 3 ? 0.525 : 0.541));
+    const S xb = n == 2 ? sqrt(X) : pow(X, b);
+    const S e1 = exp(-p084 / d), E = exp(-cc * xb);
+    S sn, cs;
+    sincos(n == 4 ? 0.375 * X : X / sd, &sn, &cs);
+    h = 1.0 - e1 - sn * E;
+    dh = e1 * (0.84 / d) * p084 / X - E * (cs * is_ - sn * (cc * b) * xb / X);
+}
+template <class S>
+KS_DEV void kappa_rhoV_x_d(int n, S xm, S& h, S& dh) {
+    const double b = n == 1 ? 0.447 : (n == 2 ? 0.391 : (n == 3 ? 0.348 : 0.313));
+    h = 1.0 - 0.17 * log1p(b * xm);
+    dh = -0.17 * b / (1.0 + b * xm);
+}
+
+// The kappa coefficients of one sample (kappa_coefs) with the reverse sweep of a scalar q of them: the forward
+// pass, then qbar(c, cbar), the caller's dq/dc of the coefficient values, then the partials of q in the splat's
+// factors (bK, added), the fluid-frame frequency (bnu) and the pitch angle's sine and cosine (bsth, bcth; zero when
+// the sine was floored). Where a base of a power is zero its partials are zero (pow_nonneg's choice).
+template <class S, class KF, class QBar>
+KS_DEV CoefT<S> kappa_coefs_rev(const KF& K, S ne, S nu, S sth, S cth, QBar&& qbar, S bK[KK_N], S& bnu, S& bsth, S& bcth) {
+    bool floored = false;
+    if (!(sth > 0.0)) { sth = S(sin_floor(sth)); floored = true; }
+    const double small = 1e-40;
+    // ---- forward (the operations of kappa_coefs)
+    const S NUW1 = K(KK_NUW1), JPRE = K(KK_JPRE), APRE = K(KK_APRE);
+    const S X = nu / (NUW1 * sth);
+    const S pref = ne * JPRE * sth;
+    const S X13 = cbrt(X);
+    const S lX = log(X);
+    const S EI = K(KK_EI), RI = K(KK_RI), MI = K(KK_MI), NLI = K(KK_NLOWI), XI = K(KK_XI);
+    const S pXI = pow(X, EI), tI = 1.0 + pXI * RI, PI_ = pow(tI, MI);
+    CoefT<S> c;
+    c.jI = pref * NLI * X13 * PI_;
+    const S EQ = K(KK_EQ), RQ = K(KK_RQ), MQ = K(KK_MQ), NLQ = K(KK_NLOWQ);
+    const S pXQ = pow(X, EQ), tQ = 1.0 + pXQ * RQ, PQ = pow(tQ, MQ);
+    c.jQ = -(pref * NLQ * X13 * PQ);
+    const S c2 = cth * cth;
+    const bool near1 = c2 < 0.5;
+    const S ls2 = near1 ? log1p(-c2) : 2.0 * log(sth);
+    const S u24 = expm1(-1.2 * ls2), u25 = expm1(-1.25 * ls2), u228 = expm1(-1.14 * ls2), u205 = expm1(-1.025 * ls2);
+    const S Xm035 = pow(X, -0.35), sX = sqrt(X);
+    const S CV1 = K(KK_CV1), CV2 = K(KK_CV2);
+    const S v48 = pow_nonneg(u24, 0.48), v44 = pow_nonneg(u25, 0.44);
+    const S NlowV = CV1 * v48 * Xm035;
+    const S NhighV = CV2 * v44 / sX - small;
+    const double sgn = cth > 0.0 ? 1.0 : (cth < 0.0 ? -1.0 : 0.0);
+    const bool jv = !(fabs(NhighV) < small * small);
+    S rVr(0.0), pV(0.0), tV(1.0), PV(1.0);
+    if (jv) {
+        rVr = NlowV / NhighV; pV = pow(rVr, XI); tV = 1.0 + pXI * pV; PV = pow(tV, MI);
+        c.jV = -(pref * NlowV * X13 * PV) * sgn;
+    } else {
+        c.jV = S(0.0);
+    }
+    const S apref = ne * APRE / sth;
+    const S Xm53 = pow(X, -5.0 / 3.0);
+    const S EA = K(KK_EA), RA = K(KK_RA), MA = K(KK_MA), LA = K(KK_LOWA);
+    const S pXA = pow(X, EA), tA = 1.0 + pXA * RA, PA = pow(tA, MA);
+    c.aI = apref * LA * Xm53 * PA;
+    const S EQA = K(KK_EQA), RQA = K(KK_RQA), MQA = K(KK_MQA), NLQA = K(KK_NLOWQA);
+    const S pXQA = pow(X, EQA), tQA = 1.0 + pXQA * RQA, PQA = pow(tQA, MQA);
+    c.aQ = -(apref * NLQA * Xm53 * PQA);
+    const S CVA1 = K(KK_CVA1), CVA2 = K(KK_CVA2), XVA = K(KK_XVA), EVA = K(KK_EVA), MVA = K(KK_MVA);
+    const S v446 = pow_nonneg(u228, 0.446), s205 = sqrt(u205);
+    const S NlowVA = CVA1 * v446 * Xm035;
+    const S NhighVA = CVA2 * s205 / sX + small;
+    const S rVA = NlowVA / NhighVA, pVA = pow(rVA, XVA), pXVA = pow(X, EVA), tVA = 1.0 + pXVA * pVA, PVA = pow(tVA, MVA);
+    c.aV = -(apref * NlowVA * Xm53 * PVA) * sgn;
+    const int i = (int)val(K(KK_IFIT));
+    const S p084 = pow(X, 0.84), xm = 1.0 / sX;
+    S hq0, dhq0, hq1, dhq1, hv0, dhv0, hv1, dhv1;
+    kappa_rhoQ_x_d<S>(i, X, p084, hq0, dhq0);
+    kappa_rhoQ_x_d<S>(i + 1, X, p084, hq1, dhq1);
+    kappa_rhoV_x_d<S>(i, xm, hv0, dhv0);
+    kappa_rhoV_x_d<S>(i + 1, xm, hv1, dhv1);
+    const S FQ0 = K(KK_FQ0), FQ1 = K(KK_FQ1), FV0 = K(KK_FV0), FV1 = K(KK_FV1), RHOQ = K(KK_RHOQ), RHOV = K(KK_RHOV);
+    const S fQ = FQ0 * hq0 + FQ1 * hq1, fV = FV0 * hv0 + FV1 * hv1;
+    const S inu = 1.0 / nu;
+    c.rQ = -(ne * RHOQ * sth * sth) * inu * inu * inu * fQ;
+    c.rV = ne * RHOV * cth * inu * inu * fV;
+    // ---- the caller's adjoints of the coefficients
+    S cb[8];
+    qbar(c, cb);
+    const S bjI = cb[0], bjQ = cb[1], bjV = cb[2], baI = cb[3], baQ = cb[4], baV = cb[5], brQ = cb[6], brV = cb[7];
+    // ---- reverse
+    S bX(0.0), bs(0.0), bc(0.0), binu(0.0), bpref(0.0), bapref(0.0), bX13(0.0), bXm53(0.0), bXm035(0.0), bsX(0.0), bls2(0.0), bpXI(0.0);
+    {   // rV = ne RHOV cth inu^2 fV, rQ = -ne RHOQ sth^2 inu^3 fQ
+        const S t = brV * ne, i2 = inu * inu;
+        bK[KK_RHOV] += t * cth * i2 * fV;
+        bc += t * RHOV * i2 * fV;
+        binu += t * RHOV * cth * 2.0 * inu * fV;
+        const S bfV = t * RHOV * cth * i2;
+        const S u = -(brQ * ne), i3 = i2 * inu;
+        bK[KK_RHOQ] += u * sth * sth * i3 * fQ;
+        bs += u * RHOQ * 2.0 * sth * i3 * fQ;
+        binu += u * RHOQ * sth * sth * 3.0 * i2 * fQ;
+        const S bfQ = u * RHOQ * sth * sth * i3;
+        bK[KK_FV0] += bfV * hv0; bK[KK_FV1] += bfV * hv1;
+        const S bxm = bfV * (FV0 * dhv0 + FV1 * dhv1);
+        bK[KK_FQ0] += bfQ * hq0; bK[KK_FQ1] += bfQ * hq1;
+        bX += bfQ * (FQ0 * dhq0 + FQ1 * dhq1);
+        bsX += bxm * (-(xm * xm));
+    }
+    {   // aV = -sgn apref NlowVA Xm53 PVA
+        const S g0 = -(baV * sgn);
+        bapref += g0 * NlowVA * Xm53 * PVA;
+        S bNlowVA = g0 * apref * Xm53 * PVA;
+        bXm53 += g0 * apref * NlowVA * PVA;
+        const S bPVA = g0 * apref * NlowVA * Xm53;
+        const S btVA = bPVA * MVA * PVA / tVA;
+        bK[KK_MVA] += bPVA * PVA * log(tVA);
+        const S bpXVA = btVA * pVA, bpVA = btVA * pXVA;
+        bX += bpXVA * EVA * pXVA / X;
+        bK[KK_EVA] += bpXVA * pXVA * lX;
+        S brVA(0.0);
+        if (rVA > 0.0) { brVA = bpVA * XVA * pVA / rVA; bK[KK_XVA] += bpVA * pVA * log(rVA); }
+        bNlowVA += brVA / NhighVA;
+        const S bNhighVA = -(brVA * rVA / NhighVA);
+        bK[KK_CVA2] += bNhighVA * s205 / sX;
+        const S bs205 = bNhighVA * CVA2 / sX;
+        bsX += -(bNhighVA * CVA2 * s205 / (sX * sX));
+        const S bu205 = s205 > 0.0 ? bs205 * 0.5 / s205 : S(0.0);
+        bK[KK_CVA1] += bNlowVA * v446 * Xm035;
+        const S bv446 = bNlowVA * CVA1 * Xm035;
+        bXm035 += bNlowVA * CVA1 * v446;
+        const S bu228 = u228 > 0.0 ? bv446 * 0.446 * v446 / u228 : S(0.0);
+        bls2 += bu205 * (-1.025) * (u205 + 1.0) + bu228 * (-1.14) * (u228 + 1.0);
+    }
+    {   // aQ = -apref NLQA Xm53 PQA, aI = apref LA Xm53 PA
+        const S g0 = -baQ;
+        bapref += g0 * NLQA * Xm53 * PQA;
+        bK[KK_NLOWQA] += g0 * apref * Xm53 * PQA;
+        bXm53 += g0 * apref * NLQA * PQA;
+        const S bPQA = g0 * apref * NLQA * Xm53;
+        const S btQA = bPQA * MQA * PQA / tQA;
+        bK[KK_MQA] += bPQA * PQA * log(tQA);
+        const S bpXQA = btQA * RQA;
+        bK[KK_RQA] += btQA * pXQA;
+        bX += bpXQA * EQA * pXQA / X;
+        bK[KK_EQA] += bpXQA * pXQA * lX;
+        bapref += baI * LA * Xm53 * PA;
+        bK[KK_LOWA] += baI * apref * Xm53 * PA;
+        bXm53 += baI * apref * LA * PA;
+        const S bPA = baI * apref * LA * Xm53;
+        const S btA = bPA * MA * PA / tA;
+        bK[KK_MA] += bPA * PA * log(tA);
+        const S bpXA = btA * RA;
+        bK[KK_RA] += btA * pXA;
+        bX += bpXA * EA * pXA / X;
+        bK[KK_EA] += bpXA * pXA * lX;
+    }
+    bX += bXm53 * (-5.0 / 3.0) * Xm53 / X;
+    bK[KK_APRE] += bapref * ne / sth;
+    bs += -(bapref * apref / sth);
+    if (jv) {   // jV = -sgn pref NlowV X13 PV
+        const S g0 = -(bjV * sgn);
+        bpref += g0 * NlowV * X13 * PV;
+        S bNlowV = g0 * pref * X13 * PV;
+        bX13 += g0 * pref * NlowV * PV;
+        const S bPV = g0 * pref * NlowV * X13;
+        const S btV = bPV * MI * PV / tV;
+        bK[KK_MI] += bPV * PV * log(tV);
+        bpXI += btV * pV;
+        const S bpV = btV * pXI;
+        S brV(0.0);
+        if (rVr > 0.0) { brV = bpV * XI * pV / rVr; bK[KK_XI] += bpV * pV * log(rVr); }
+        bNlowV += brV / NhighV;
+        const S bNhighV = -(brV * rVr / NhighV);
+        bK[KK_CV2] += bNhighV * v44 / sX;
+        const S bv44 = bNhighV * CV2 / sX;
+        bsX += -(bNhighV * CV2 * v44 / (sX * sX));
+        const S bu25 = u25 > 0.0 ? bv44 * 0.44 * v44 / u25 : S(0.0);
+        bK[KK_CV1] += bNlowV * v48 * Xm035;
+        const S bv48 = bNlowV * CV1 * Xm035;
+        bXm035 += bNlowV * CV1 * v48;
+        const S bu24 = u24 > 0.0 ? bv48 * 0.48 * v48 / u24 : S(0.0);
+        bls2 += bu25 * (-1.25) * (u25 + 1.0) + bu24 * (-1.2) * (u24 + 1.0);
+    }
+    bX += bsX * 0.5 / sX + bXm035 * (-0.35) * Xm035 / X;
+    if (near1) bc += bls2 * (-2.0) * cth / (1.0 - c2);
+    else bs += bls2 * 2.0 / sth;
+    {   // jQ = -pref NLQ X13 PQ, jI = pref NLI X13 PI
+        const S g0 = -bjQ;
+        bpref += g0 * NLQ * X13 * PQ;
+        bK[KK_NLOWQ] += g0 * pref * X13 * PQ;
+        bX13 += g0 * pref * NLQ * PQ;
+        const S bPQ = g0 * pref * NLQ * X13;
+        const S btQ = bPQ * MQ * PQ / tQ;
+        bK[KK_MQ] += bPQ * PQ * log(tQ);
+        const S bpXQ = btQ * RQ;
+        bK[KK_RQ] += btQ * pXQ;
+        bX += bpXQ * EQ * pXQ / X;
+        bK[KK_EQ] += bpXQ * pXQ * lX;
+        bpref += bjI * NLI * X13 * PI_;
+        bK[KK_NLOWI] += bjI * pref * X13 * PI_;
+        bX13 += bjI * pref * NLI * PI_;
+        const S bPI = bjI * pref * NLI * X13;
+        const S btI = bPI * MI * PI_ / tI;
+        bK[KK_MI] += bPI * PI_ * log(tI);
+        bpXI += btI * RI;
+        bK[KK_RI] += btI * pXI;
+    }
+    bX += bpXI * EI * pXI / X;
+    bK[KK_EI] += bpXI * pXI * lX;
+    bX += bX13 * X13 / (3.0 * X);
+    bK[KK_JPRE] += bpref * ne * sth;
+    bs += bpref * ne * JPRE;
+    bnu = bX * X / nu + binu * (-(inu * inu));
+    bK[KK_NUW1] += -(bX * X / NUW1);
+    bs += -(bX * X / sth);
+    bsth = floored ? S(0.0) : bs;
+    bcth = bc;
+    return c;
+}
+
+// One kappa splat's share of one sample (the element of KappaSplats, populations.jl:34-60) by the reverse sweep of
+// its coefficients (kappa_coefs_rev), with the same 22 rows (kappa, ln w the population rows): the screen sums
+// and caps give the coefficients' adjoints, the sweep gives q's partials in the splat's factors (joined to kappa,
+// ln w and ln B through their partials dkk), in the fluid-frame frequency (to g) and in the pitch angle (to cos B);
+// the frame rows join through frame_jacobian. kappa's 2F1 factor is differentiated with kappa and w (its partials
+// are among the factors'), not held. The same partials as five-partial forward duals, in fewer operations and
+// registers.
+template <class S>
+KS_DEV void element_adjoint_kappa(const SplatCT<S>& c, const SplatAdjT<S>& sa, const GeoT<S>& G, S a, S alpha, S beta, S thetao_sin,
+                                  S nu, const S bars[11], S Gw, const S dl[13], S grad[22]) {
+    const S uv[3] = {c.u[0], c.u[1], c.u[2]};
+    const FrameJ<S> fr = frame_jacobian(G, a, alpha, beta, thetao_sin, uv, sa.sthB, sa.cthB, sa.sphB, sa.cphB);
+    const S nuf = nu / fr.g;
+    const bool clipped = !(fr.cosB > -1.0 && fr.cosB < 1.0);
+    const S cb = dmin(dmax(fr.cosB, -1.0), 1.0);
+    S sth, cth;
+    pitch_sincos(cb, sth, cth);
+    const S ne = exp(c.lnA) * Gw;
+    const S nh = nuf * (1.0 / NU0);
+    const S i2 = 1.0 / (nh * nh);
+    S s2c, c2c;
+    sincos(2.0 * fr.chi, &s2c, &c2c);
+    S q(0.0), qchi(0.0), bnh(0.0);
+    bool on = true;
+    // q = i2 (b0 jI + fe (b3 jV + Bj jQ)) + nh (b4 aI + fa (b7 aV + Ba aQ) + b10 rV + Br rQ), with the caps fe, fa
+    auto qbar = [&](const CoefT<S>& cf, S cbar[8]) {
+        for (int k = 0; k < 8; ++k) cbar[k] = S(0.0);
+        on = cf.jI > 0.0 || cf.aI > 0.0 || cf.rQ != 0.0 || cf.rV != 0.0;
+        if (!on) return;
+        const S Bj = bars[1] * c2c + bars[2] * s2c, Ba = bars[5] * c2c + bars[6] * s2c, Br = bars[8] * c2c + bars[9] * s2c;
+        S fe, fa, dfe[3], dfa[3];
+        cap_factor_d(cf.jI, cf.jQ, cf.jV, fe, dfe);
+        cap_factor_d(cf.aI, cf.aQ, cf.aV, fa, dfa);
+        const S Wj = bars[3] * cf.jV + Bj * cf.jQ, Wa = bars[7] * cf.aV + Ba * cf.aQ;
+        const S qj = bars[0] * cf.jI + fe * Wj, qa = bars[4] * cf.aI + fa * Wa + bars[10] * cf.rV + Br * cf.rQ;
+        q = i2 * qj + nh * qa;
+        cbar[0] = i2 * (bars[0] + Wj * dfe[0]);
+        cbar[1] = i2 * (fe * Bj + Wj * dfe[1]);
+        cbar[2] = i2 * (fe * bars[3] + Wj * dfe[2]);
+        cbar[3] = nh * (bars[4] + Wa * dfa[0]);
+        cbar[4] = nh * (fa * Ba + Wa * dfa[1]);
+        cbar[5] = nh * (fa * bars[7] + Wa * dfa[2]);
+        cbar[6] = nh * Br;
+        cbar[7] = nh * bars[10];
+        bnh = -2.0 * i2 / nh * qj + qa;
+        // d q / d chi: the cos 2chi, sin 2chi parts
+        const S Bc = bars[1] * fe * cf.jQ * i2 + bars[5] * fa * cf.aQ * nh + bars[8] * cf.rQ * nh;
+        const S Bs = bars[2] * fe * cf.jQ * i2 + bars[6] * fa * cf.aQ * nh + bars[9] * cf.rQ * nh;
+        qchi = 2.0 * (Bs * c2c - Bc * s2c);
+    };
+    S bK[KK_N];
+    for (int f = 0; f < KK_N; ++f) bK[f] = S(0.0);
+    S bnu, bsth, bcth;
+    kappa_coefs_rev(KappaGet<S>{c.kk}, ne, nuf, sth, cth, qbar, bK, bnu, bsth, bcth);
+    if (!on) return;
+    S dk(0.0), dw(0.0), dB(0.0);
+    for (int f = 0; f < KK_N; ++f) { dk += bK[f] * sa.dkk[0][f]; dw += bK[f] * sa.dkk[1][f]; dB += bK[f] * sa.dkk[2][f]; }
+    // nu_f = nu / g: d/dg = -(nu_f / g) d/dnu_f; cos B: cth = cb, sth = sqrt((1 - cb)(1 + cb))
+    const S dnuf = bnu + bnh * (1.0 / NU0);
+    const S dg = -(dnuf * nuf / fr.g);
+    const S dcb = clipped ? S(0.0) : bcth + (sth > 0.0 ? bsth * (-(cb / sth)) : S(0.0));
+    const S sv = q;
+    for (int r = 0; r < 12; ++r) grad[r] += sv * dl[r];
+    grad[12] += sv;
+    grad[13] += dk;
+    grad[14] += dw;
+    grad[15] += dB;
+    for (int k = 0; k < 5; ++k) grad[16 + k] += dg * fr.dg[k] + dcb * fr.dcosB[k] + qchi * fr.dchi[k];
+    grad[21] += sv * dl[12];
+}
+
+// One power-law splat's share of one sample (the element of PowerLawSplats, populations.jl:13-33, differentiated as
+// element_adjoint_thermal): the 22 rows are the 12 geometric rows, ln n_e, the two population rows (p, ln gamma_min),
+// ln B, the field angles, u and the pattern rate. The contribution q is taken in five-partial duals over (p,
+// ln gamma_min, ln B, g, cos B); the Gamma products enter with their derivatives in p (dpop); the frame rows join by
+// the chain rule through frame_jacobian.
 template <class S>
 KS_DEV void element_adjoint_pop(const SplatCT<S>& c, const SplatAdjT<S>& sa, const GeoT<S>& G, S a, S alpha, S beta, S thetao_sin,
                                 S nu, const S bars[11], S Gw, const S dl[13], S grad[22]) {
@@ -352,15 +644,11 @@ KS_DEV void element_adjoint_pop(const SplatCT<S>& c, const SplatAdjT<S>& sa, con
     pitch_sincos(cb, sth, cth);
     const D5 ne(exp(c.lnA) * Gw);
     CoefT<D5> cf;
-    if (c.kind == MODEL_POWERLAW) {
-        const D5 p = D5::seed(c.p1, 0);
-        D5 gmin(c.p2); gmin.d[1] = c.p2;
-        D5 gPL[2];
-        for (int k = 0; k < 2; ++k) { gPL[k] = D5(c.g[k]); gPL[k].d[0] = sa.dpop[k]; }
-        cf = powerlaw(ne, p, gmin, D5(c.p3), gPL, Bm, nuf, sth, cth);
-    } else {
-        cf = kappa_coefs(KappaDualGet<S>{c.kk, sa}, ne, nuf, sth, cth);
-    }
+    const D5 p = D5::seed(c.p1, 0);
+    D5 gmin(c.p2); gmin.d[1] = c.p2;
+    D5 gPL[2];
+    for (int k = 0; k < 2; ++k) { gPL[k] = D5(c.g[k]); gPL[k].d[0] = sa.dpop[k]; }
+    cf = powerlaw(ne, p, gmin, D5(c.p3), gPL, Bm, nuf, sth, cth);
     if (!(cf.jI > 0.0 || cf.aI > 0.0 || cf.rQ.v != 0.0 || cf.rV.v != 0.0)) return;
     D5 fe = cap_factor(cf.jI, cf.jQ, cf.jV), fa = cap_factor(cf.aI, cf.aQ, cf.aV);
     D5 nh = nuf * (1.0 / NU0);

@@ -57,8 +57,21 @@ KS_DEV f32 sqrt(f32 x) { return f32(sqrtf(x.v)); }
 // (4e-6 at |y log2 x| = 100), far inside the FP32 path's error bars; a base that is zero, negative or not finite
 // takes the library's powf. exp, log and cbrt stay the library's: exp through 2^(x log2 e) loses |x| 2^-24 of
 // relative accuracy, which optically thick, Faraday-rotating parcels amplify past the FP32 path's bar.
-KS_DEV float hw_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
-KS_DEV float hw_log2(float x) { return __builtin_amdgcn_logf(x); }
+// (host instantiations, the work model and host checks, take the C library's base-2 functions)
+KS_DEV float hw_exp2(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_exp2f(x);
+#else
+    return exp2f(x);
+#endif
+}
+KS_DEV float hw_log2(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_logf(x);
+#else
+    return log2f(x);
+#endif
+}
 KS_DEV float hw_pow(float x, float y) {
     if (x > 0.0f && x < INFINITY) return hw_exp2(y * hw_log2(x));
     return powf(x, y);

@@ -46,10 +46,11 @@ KS_DEV S knot_gexp(const TransCtxT<S>& c, const SplatCT<S>& sc, S t, S x, S y, S
 // The splat sum of one sample (accumulate_elements, transport.jl:254-276): invariant j, alpha, rho in
 // screen axes; false when no splat contributes. One instantiation per population and scalar.
 // With a candidate list (ascending splat indices, the per-tile cull) only those splats are visited; the
-// others lie outside their support at every sample of the tile and would add exact zeros.
+// others lie outside their support at every sample of the tile and would add exact zeros. With gout the sample's
+// geometry (in S) is returned when the sample contributes.
 template <int KIND, class S>
 KS_DEV bool sample_coefficients(const TransCtxT<S>& c, double st, double r, double th, bool nur, bool nuth, S x, S y, S z,
-                                S j4[4], S a4[4], S r3[3], const uint16_t* cand = nullptr, int ncand = 0) {
+                                S j4[4], S a4[4], S r3[3], const uint16_t* cand = nullptr, int ncand = 0, GeoT<S>* gout = nullptr) {
     const S t = S(c.t_obs - st);
     for (int i = 0; i < 4; ++i) { j4[i] = S(0.0); a4[i] = S(0.0); }
     r3[0] = r3[1] = r3[2] = S(0.0);
@@ -104,6 +105,7 @@ KS_DEV bool sample_coefficients(const TransCtxT<S>& c, double st, double r, doub
         r3[0] += rQ * c2c; r3[1] += rQ * s2c; r3[2] += cf.rV * nh;
         active = true;
     }
+    if (gout && active) *gout = G;
     return active;
 }
 

@@ -222,6 +222,19 @@ KS_DEV void make_geo(Q a, Q r, Q th, Q eta, Q lam, bool nur, bool nuth, GeoT<Q>&
     g.pz[2] = g.sgpp * (g.pu[3] - g.omega * g.pu[0]);
     g.pz[3] = -g.sgthth * g.pu[2];
 }
+// The fields of a sample's geometry that the frames use (local_frame, frame_jacobian), stored field-major at
+// w[f * st] (f < 16): pz, pu, lapse, sgrr, sgthth, sgpp, omega, r, st, ct. geo_load leaves th, S, Dl and Xi unset.
+template <class T> KS_DEV void geo_store(const GeoT<T>& g, T* w, int64_t st) {
+    for (int i = 0; i < 4; ++i) { w[i * st] = g.pz[i]; w[(4 + i) * st] = g.pu[i]; }
+    w[8 * st] = g.lapse; w[9 * st] = g.sgrr; w[10 * st] = g.sgthth; w[11 * st] = g.sgpp; w[12 * st] = g.omega;
+    w[13 * st] = g.r; w[14 * st] = g.st; w[15 * st] = g.ct;
+}
+template <class T> KS_DEV void geo_load(const T* o, int64_t st, GeoT<T>& g) {
+    for (int i = 0; i < 4; ++i) { g.pz[i] = o[i * st]; g.pu[i] = o[(4 + i) * st]; }
+    g.lapse = o[8 * st]; g.sgrr = o[9 * st]; g.sgthth = o[10 * st]; g.sgpp = o[11 * st]; g.omega = o[12 * st];
+    g.r = o[13 * st]; g.st = o[14 * st]; g.ct = o[15 * st];
+    g.th = T(0.0); g.S = T(0.0); g.Dl = T(0.0); g.Xi = T(0.0);
+}
 // the FP64 geometry of a sample in another scalar (built once per sample, shared by the splats)
 template <class T> KS_DEV GeoT<T> geo_as(const Geo& g) {
     GeoT<T> o;
@@ -539,6 +552,22 @@ template <class T> KS_DEV T cap_factor(T I, T Q, T V) {
     T pp = q * q + v * v;
     T p = pp > 0.0 ? sqrt(pp) : T(0.0);
     return p > fmax * Ih ? fmax * Ih / p : T(1.0);
+}
+// cap_factor and its partials in (I, Q, V): f = fmax I / sqrt(Q^2 + V^2) where the cap acts, else a constant
+template <class T> KS_DEV void cap_factor_d(T I, T Q, T V, T& f, T df[3]) {
+    const double fmax = 0.99;
+    df[0] = df[1] = df[2] = T(0.0);
+    if (!(I > 0.0)) { f = (val(Q) == 0.0 && val(V) == 0.0) ? T(1.0) : T(0.0); return; }
+    const double s = pow2_below(dmax(dmax(fabs(val(I)), fabs(val(Q))), fabs(val(V))));
+    const double is = 1.0 / s;
+    T Ih = I * is, q = Q * is, v = V * is;
+    T pp = q * q + v * v;
+    T p = pp > 0.0 ? sqrt(pp) : T(0.0);
+    if (!(p > fmax * Ih)) { f = T(1.0); return; }
+    f = fmax * Ih / p;
+    df[0] = fmax * is / p;
+    df[1] = -(f * q / pp) * is;
+    df[2] = -(f * v / pp) * is;
 }
 
 // ---- the exact 4x4 step -------------------------------------------------------------------------------------------

@@ -171,20 +171,21 @@ __global__ void __launch_bounds__(64) ks_unit_bounds(int64_t npix, int N, Sample
 // through it. Every pair of a wavefront belongs to
 // splat i, so the sums stay per lane in registers; the queue order is fixed, so the result is
 // deterministic.
-// POPS: the populations the launch meets, POP_THERMAL (thermal and knot splats), POP_NONTHERMAL (power law and
-// kappa), POP_KNOTS (centres that follow knots); each instantiation carries only those branches, whose registers
-// would otherwise cost the common single-population launches occupancy.
+// POPS: the populations the launch meets, POP_THERMAL (thermal and knot splats), POP_POWERLAW, POP_KAPPA, and
+// POP_KNOTS (centres that follow knots); a launch of one population carries only its branch, whose registers would
+// otherwise cost it occupancy; mixed populations take a runtime branch on the splat's kind.
 template <class S, int POPS>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(Waves<S>::adjelem))) ks_adj_splats(double a, double thetao, const double* __restrict__ soa, int64_t npix, int N, Samples smp,
                                                     const SplatCT<S>* __restrict__ sp, const SplatAdjT<S>* __restrict__ sadj, int ns,
                                                     const S* __restrict__ knots, const S* __restrict__ ktimes,
                                                     const double* __restrict__ times, int nt, const double* __restrict__ nus, int64_t nitems,
-                                                    const S* __restrict__ bar, const uint8_t* __restrict__ aflg, int64_t unit0, int64_t nb,
+                                                    const S* __restrict__ rec, const S* __restrict__ bar, const uint8_t* __restrict__ aflg,
+                                                    int64_t unit0, int64_t nb,
                                                     const double* __restrict__ ub, double* __restrict__ part) {
     constexpr int WALK = 8;                 // samples tested per group, their loads issued together
     constexpr int QCAP = 1024;              // ring capacity: < 64 pending + 64 per sample of a group
     static_assert(QCAP >= 64 * (WALK + 1) && (QCAP & (QCAP - 1)) == 0, "the ring must hold a group's pushes");
-    __shared__ double s_alpha[64], s_beta[64], s_eta[64], s_lam[64], s_tobs[64], s_nu[64];
+    __shared__ double s_alpha[64], s_beta[64], s_tobs[64], s_nu[64];
     __shared__ int64_t s_item[64];
     __shared__ int s_q[QCAP];   // entry = lane | k << 6
     // XCD-aware order: the hardware deals consecutive blocks round-robin over the 8 XCDs, so block b runs on
@@ -230,8 +231,6 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(Waves<S
         t_obs = times[fr % nt];
         s_alpha[lane] = soa[(int64_t)PF_ALPHA * npix + j];
         s_beta[lane] = soa[(int64_t)PF_BETA * npix + j];
-        s_eta[lane] = soa[(int64_t)PF_ETA * npix + j];
-        s_lam[lane] = soa[(int64_t)PF_LAM * npix + j];
         s_tobs[lane] = t_obs;
         s_nu[lane] = nus[fr / nt];
         s_item[lane] = item;
@@ -258,16 +257,18 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(Waves<S
                 const S* o = bar + fo;
                 S bars[BAR_FIELDS];
                 for (int q = 0; q < BAR_FIELDS; ++q) bars[q] = o[q * st];
-                const uint8_t fl = smp.fl[so];
-                Geo Gd;
-                make_geo(a, smp.r[so], smp.th[so], s_eta[l], s_lam[l], (fl & 2) != 0, (fl & 4) != 0, Gd);
-                const GeoT<S> G = geo_as<S>(Gd);
-                constexpr int both = POP_THERMAL | POP_NONTHERMAL;
-                if constexpr ((POPS & both) == POP_THERMAL)
+                GeoT<S> G;
+                geo_load(rec + fo + REC_GEO * st, st, G);        // written by the forward transfer
+                constexpr int fl = POPS & POP_FLUIDS;
+                if constexpr (fl == POP_THERMAL)
                     element_adjoint_thermal(c, sa, G, aS, S(s_alpha[l]), S(s_beta[l]), tos, S(s_nu[l]), bars, Gw, dl, grad);
-                else if constexpr ((POPS & both) == POP_NONTHERMAL)
+                else if constexpr (fl == POP_POWERLAW)
                     element_adjoint_pop(c, sa, G, aS, S(s_alpha[l]), S(s_beta[l]), tos, S(s_nu[l]), bars, Gw, dl, grad);
-                else if (c.kind == MODEL_POWERLAW || c.kind == MODEL_KAPPA)
+                else if constexpr (fl == POP_KAPPA)
+                    element_adjoint_kappa(c, sa, G, aS, S(s_alpha[l]), S(s_beta[l]), tos, S(s_nu[l]), bars, Gw, dl, grad);
+                else if (c.kind == MODEL_KAPPA)
+                    element_adjoint_kappa(c, sa, G, aS, S(s_alpha[l]), S(s_beta[l]), tos, S(s_nu[l]), bars, Gw, dl, grad);
+                else if (c.kind == MODEL_POWERLAW)
                     element_adjoint_pop(c, sa, G, aS, S(s_alpha[l]), S(s_beta[l]), tos, S(s_nu[l]), bars, Gw, dl, grad);
                 else
                     element_adjoint_thermal(c, sa, G, aS, S(s_alpha[l]), S(s_beta[l]), tos, S(s_nu[l]), bars, Gw, dl, grad);
@@ -365,35 +366,36 @@ hipError_t launch_adj_segment_f32(hipStream_t s, int N, int64_t nitems, const do
 template <class S>
 static hipError_t adj_splats_k(hipStream_t s, int pops, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
                                const SplatCT<S>* sp, const SplatAdjT<S>* sadj, int ns, const S* knots, const S* ktimes,
-                               const double* times, int nt, const double* nus, int64_t nitems, const S* bar, const uint8_t* aflg,
-                               int64_t unit0, int64_t nb, const double* ub, double* part) {
+                               const double* times, int nt, const double* nus, int64_t nitems, const S* rec, const S* bar,
+                               const uint8_t* aflg, int64_t unit0, int64_t nb, const double* ub, double* part) {
     const dim3 grid((unsigned)((nb + 7) / 8 * 8 * ns)), block(64);
 #define KS_ADJ_LAUNCH(P) hipLaunchKernelGGL((ks_adj_splats<S, P>), grid, block, 0, s, a, thetao, soa, npix, N, smp, sp, sadj, ns, knots, ktimes, \
-                                            times, nt, nus, nitems, bar, aflg, unit0, nb, ub, part)
+                                            times, nt, nus, nitems, rec, bar, aflg, unit0, nb, ub, part)
     switch (pops) {
         case POP_THERMAL: KS_ADJ_LAUNCH(POP_THERMAL); break;
-        case POP_NONTHERMAL: KS_ADJ_LAUNCH(POP_NONTHERMAL); break;
-        case POP_THERMAL | POP_NONTHERMAL: KS_ADJ_LAUNCH(POP_THERMAL | POP_NONTHERMAL); break;
         case POP_THERMAL | POP_KNOTS: KS_ADJ_LAUNCH(POP_THERMAL | POP_KNOTS); break;
-        case POP_NONTHERMAL | POP_KNOTS: KS_ADJ_LAUNCH(POP_NONTHERMAL | POP_KNOTS); break;
-        default: KS_ADJ_LAUNCH(POP_THERMAL | POP_NONTHERMAL | POP_KNOTS); break;
+        case POP_POWERLAW: KS_ADJ_LAUNCH(POP_POWERLAW); break;
+        case POP_POWERLAW | POP_KNOTS: KS_ADJ_LAUNCH(POP_POWERLAW | POP_KNOTS); break;
+        case POP_KAPPA: KS_ADJ_LAUNCH(POP_KAPPA); break;
+        case POP_KAPPA | POP_KNOTS: KS_ADJ_LAUNCH(POP_KAPPA | POP_KNOTS); break;
+        default: KS_ADJ_LAUNCH(POP_FLUIDS | POP_KNOTS); break;        // more than one population
     }
 #undef KS_ADJ_LAUNCH
     return hipGetLastError();
 }
 hipError_t launch_adj_splats(hipStream_t s, int pops, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
                              const SplatC* sp, const SplatAdj* sadj, int ns, const double* knots, const double* ktimes,
-                             const double* times, int nt, const double* nus, int64_t nitems, const double* bar, const uint8_t* aflg,
-                             int64_t unit0, int64_t nb, const double* ub, double* part) {
-    return adj_splats_k<double>(s, pops, a, thetao, soa, npix, N, smp, sp, sadj, ns, knots, ktimes, times, nt, nus, nitems, bar, aflg, unit0, nb,
-                                ub, part);
+                             const double* times, int nt, const double* nus, int64_t nitems, const double* rec, const double* bar,
+                             const uint8_t* aflg, int64_t unit0, int64_t nb, const double* ub, double* part) {
+    return adj_splats_k<double>(s, pops, a, thetao, soa, npix, N, smp, sp, sadj, ns, knots, ktimes, times, nt, nus, nitems, rec, bar, aflg, unit0,
+                                nb, ub, part);
 }
 hipError_t launch_adj_splats_f32(hipStream_t s, int pops, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
                                  const SplatCf* sp, const SplatAdjf* sadj, int ns, const f32* knots, const f32* ktimes,
-                                 const double* times, int nt, const double* nus, int64_t nitems, const f32* bar, const uint8_t* aflg,
-                                 int64_t unit0, int64_t nb, const double* ub, double* part) {
-    return adj_splats_k<f32>(s, pops, a, thetao, soa, npix, N, smp, sp, sadj, ns, knots, ktimes, times, nt, nus, nitems, bar, aflg, unit0, nb, ub,
-                             part);
+                                 const double* times, int nt, const double* nus, int64_t nitems, const f32* rec, const f32* bar,
+                                 const uint8_t* aflg, int64_t unit0, int64_t nb, const double* ub, double* part) {
+    return adj_splats_k<f32>(s, pops, a, thetao, soa, npix, N, smp, sp, sadj, ns, knots, ktimes, times, nt, nus, nitems, rec, bar, aflg, unit0, nb,
+                             ub, part);
 }
 hipError_t launch_unit_bounds(hipStream_t s, int64_t npix, int N, Samples smp, const double* times, int nt, int64_t nitems,
                               const uint8_t* aflg, double* ub) {

@@ -77,8 +77,9 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(Waves<
         if ((fl & 1) && r > hor && k < ks) {
             const double th = smp.th[o];
             S j4[4], a4[4], r3[3];
+            GeoT<S> G;
             if (sample_coefficients<KIND, S>(c, smp.t[o], r, th, (fl & 2) != 0, (fl & 4) != 0, S(smp.x[o]), S(smp.y[o]), S(smp.z[o]),
-                                             j4, a4, r3, cull ? s_cand[wv] : nullptr, ncand)) {
+                                             j4, a4, r3, cull ? s_cand[wv] : nullptr, ncand, rec ? &G : nullptr)) {
                 const S D = S(path_weight(L, nus[inu], a, r, th, dtau));
                 map_step(sm, j4, a4, r3, D);
                 step = true;
@@ -87,6 +88,7 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(Waves<
                     for (int i = 0; i < 4; ++i) { w[(REC_J + i) * st] = j4[i]; w[(REC_A + i) * st] = a4[i]; }
                     for (int i = 0; i < 3; ++i) w[(REC_RHO + i) * st] = r3[i];
                     w[REC_D * st] = D;
+                    geo_store(G, w + REC_GEO * st, st);
                 }
             }
         }

@@ -21,7 +21,8 @@ constexpr int PART_BLOCK = 256;    // block of the case partition
 constexpr int CHUNK = 32;          // samples per work item of the transfer from stored samples
 constexpr int SEG_FIELDS = 20;     // a segment map: P (16, row-major), then Sv (4)
 // the populations an element-adjoint launch meets (a bit mask)
-constexpr int POP_THERMAL = 1, POP_NONTHERMAL = 2, POP_KNOTS = 4;
+constexpr int POP_THERMAL = 1, POP_POWERLAW = 2, POP_KNOTS = 4, POP_KAPPA = 8;
+constexpr int POP_FLUIDS = POP_THERMAL | POP_POWERLAW | POP_KAPPA;
 constexpr int ADJ_ROWS = 22;       // gradient rows per splat (21 thermal and knot, 22 power law and kappa)
 constexpr int ADJ_SLICE = 32;      // samples per work unit of the element adjoints
 constexpr int CAND_MAX = 1024;     // splats a tile's candidate list holds (more splats: no per-tile cull)
@@ -64,7 +65,7 @@ inline int64_t adj_units(int64_t nitems, int N) { return ((nitems + 63) / 64) * 
 #define KS_WAVES_ADJELEM64 1
 #endif
 #ifndef KS_WAVES_ADJELEM32
-#define KS_WAVES_ADJELEM32 1
+#define KS_WAVES_ADJELEM32 2
 #endif
 template <class S> struct Waves;
 template <> struct Waves<double> { static constexpr int transfer = KS_WAVES_TRANSFER64, adjstep = KS_WAVES_ADJSTEP64, adjelem = KS_WAVES_ADJELEM64; };
@@ -89,7 +90,10 @@ KS_DEV int64_t slot_item(const Samples& smp, int64_t t, int64_t npix) {
 
 // Adjoint records of a step-taking sample: rec[(f N + k - 1) nitems + item], f = the invariant j (4),
 // alpha (4), rho (3), Delta; the adjoint sweep writes j_bar, alpha_bar, rho_bar over j, alpha, rho.
-enum RecField { REC_J, REC_A = REC_J + 4, REC_RHO = REC_A + 4, REC_D = REC_RHO + 3, REC_FIELDS };
+// the sample's geometry the frames of its splats need (geo_store / geo_load): written once per step-taking sample
+// by the transfer, read by every element adjoint of the sample instead of rebuilding it in FP64 per (sample, splat)
+constexpr int GEO_FIELDS = 16;
+enum RecField { REC_J, REC_A = REC_J + 4, REC_RHO = REC_A + 4, REC_D = REC_RHO + 3, REC_GEO, REC_FIELDS = REC_GEO + GEO_FIELDS };
 constexpr int BAR_FIELDS = 11;    // the adjoints of j (4), alpha (4), rho (3) of every step-taking sample
 
 // A segment's composite transfer map: for its samples, front to back, P = O_k0 ... O_k1 and
@@ -220,12 +224,12 @@ hipError_t launch_adj_segment_f32(hipStream_t s, int N, int64_t nitems, const do
                                   f32* tails, Samples smp, int64_t npix);
 hipError_t launch_adj_splats_f32(hipStream_t s, int pops, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
                                  const SplatCf* sp, const SplatAdjf* sadj, int ns, const f32* knots, const f32* ktimes,
-                                 const double* times, int nt, const double* nus, int64_t nitems, const f32* bar, const uint8_t* aflg,
-                                 int64_t unit0, int64_t nb, const double* ub, double* part);
+                                 const double* times, int nt, const double* nus, int64_t nitems, const f32* rec, const f32* bar,
+                                 const uint8_t* aflg, int64_t unit0, int64_t nb, const double* ub, double* part);
 hipError_t launch_adj_splats(hipStream_t s, int pops, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
                              const SplatC* sp, const SplatAdj* sadj, int ns, const double* knots, const double* ktimes,
-                             const double* times, int nt, const double* nus, int64_t nitems, const double* bar, const uint8_t* aflg,
-                             int64_t unit0, int64_t nb, const double* ub, double* part);
+                             const double* times, int nt, const double* nus, int64_t nitems, const double* rec, const double* bar,
+                             const uint8_t* aflg, int64_t unit0, int64_t nb, const double* ub, double* part);
 hipError_t launch_adj_reduce(hipStream_t s, const double* part, int64_t nunits, int ns, int accumulate, double* dparams);
 hipError_t launch_unit_bounds(hipStream_t s, int64_t npix, int N, Samples smp, const double* times, int nt, int64_t nitems,
                               const uint8_t* aflg, double* ub);

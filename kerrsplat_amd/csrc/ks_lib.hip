@@ -162,7 +162,7 @@ struct ks_model {
                                                        // address is not the model of an earlier forward pass
     int kind = KS_MODEL_THERMAL, nrows = 0, ns = 0;
     std::vector<int> hrows;                            // parameter rows of every splat (21 thermal/knot, 22 power law/kappa)
-    int pops = 0;                                      // the populations it holds (POP_THERMAL | POP_NONTHERMAL | POP_KNOTS)
+    int pops = 0;                                      // the populations it holds (POP_THERMAL, POP_POWERLAW, POP_KAPPA, POP_KNOTS)
     SplatC* d_splats = nullptr;
     SplatAdj* d_adj = nullptr;                         // thermal splats: constants of the adjoint
     double* d_knots = nullptr; double* d_ktimes = nullptr; int nknots = 0;
@@ -437,7 +437,7 @@ int ks_model_create(ks_ctx* ctx, int kind, int nrows, int ns, const double* para
     m->id = next_model_id();
     m->ctx = ctx; m->kind = kind; m->nrows = nrows; m->ns = ns;
     m->hrows.assign((size_t)ns, nrows);
-    m->pops = (kind == KS_MODEL_POWERLAW || kind == KS_MODEL_KAPPA ? POP_NONTHERMAL : POP_THERMAL) | (with_knots ? POP_KNOTS : 0);
+    m->pops = (kind == KS_MODEL_POWERLAW ? POP_POWERLAW : kind == KS_MODEL_KAPPA ? POP_KAPPA : POP_THERMAL) | (with_knots ? POP_KNOTS : 0);
     if (dalloc(&m->d_splats, h.size()) || dalloc(&m->d_adj, ha.size()) || dalloc(&m->d_splats_f, h.size()) || dalloc(&m->d_adj_f, ha.size())) {
         ks_model_destroy(m); return KS_ERR_HIP;
     }
@@ -769,11 +769,11 @@ int ks_gradient_reverse(ks_cache* c, const ks_model* m, const double* dstokes, i
             const int64_t nb = std::min(nbmax, nunits - u0);
             if (mixed)
                 HIPCHK(launch_adj_splats_f32(x->stream, m->pops, c->spin, c->thetao, c->soa, c->npix, c->N, samples_of(c), m->d_splats_f, m->d_adj_f, ns,
-                                             m->d_knots_f, m->d_ktimes_f, d_tn, nt, d_tn + nt, items, (const f32*)c->abar, c->aflg, u0, nb, c->aub,
-                                             c->apart));
+                                             m->d_knots_f, m->d_ktimes_f, d_tn, nt, d_tn + nt, items, (const f32*)c->arec, (const f32*)c->abar, c->aflg,
+                                             u0, nb, c->aub, c->apart));
             else
                 HIPCHK(launch_adj_splats(x->stream, m->pops, c->spin, c->thetao, c->soa, c->npix, c->N, samples_of(c), m->d_splats, m->d_adj, ns, m->d_knots,
-                                         m->d_ktimes, d_tn, nt, d_tn + nt, items, c->abar, c->aflg, u0, nb, c->aub, c->apart));
+                                         m->d_ktimes, d_tn, nt, d_tn + nt, items, c->arec, c->abar, c->aflg, u0, nb, c->aub, c->apart));
             HIPCHK(launch_adj_reduce(x->stream, c->apart, nb, ns, u0 > 0, c->adpar));
         }
         HIPCHK(x->stage_end(KS_STAGE_ADJ_ELEMENTS));

@@ -5,6 +5,9 @@
     python -m kerrsplat_amd.profiling valu <counter_collection.csv with SQ_INSTS_VALU, GRBM_GUI_ACTIVE> > summary.md
     python -m kerrsplat_amd.profiling traffic <fetch counter_collection.csv> <write counter_collection.csv> \
         [--copy-bytes N] [--json out.json]                                           > summary.md
+    python -m kerrsplat_amd.profiling mix <counter_collection.csv with SQ_INSTS_VALU and the FP32/FP64 counters> > summary.md
+    python -m kerrsplat_amd.profiling mfma <counter_collection.csv with SQ_INSTS_VALU_MFMA_MOPS_F64,
+        SQ_VALU_MFMA_BUSY_CYCLES, GRBM_GUI_ACTIVE>                                 > summary.md
 
 `trace` tabulates the kernel-trace statistics (calls, average and total duration per kernel).
 `launches` does the same from the per-dispatch trace, one row per (kernel, grid size), with the
@@ -17,6 +20,11 @@ SQ_INSTS_VALU / (32 GRBM_GUI_ACTIVE) (quarter-rate transcendentals make the true
 scaled by what it reads for the runtime's device-to-device copy of a known size (bench.py
 --copy-probe copies 256 MiB, so FETCH and WRITE should each read 256 MiB for that dispatch); the
 largest copy dispatch is taken as the probe.
+`mix` splits each kernel's vector instructions by type (FP32 and FP64 FMA, add, multiply and transcendental
+counters, as collected). `mfma` turns SQ_INSTS_VALU_MFMA_MOPS_F64 (FP64 matrix operations / 512) into FLOPs per
+dispatch and, with the dispatch's duration, a rate against the FP64 matrix peak, and SQ_VALU_MFMA_BUSY_CYCLES into
+the matrix-core busy fraction: busy cycles / (GRBM_GUI_ACTIVE of one XCD x 1024 SIMDs), GRBM_GUI_ACTIVE being
+summed over the 8 XCDs.
 """
 import argparse
 import collections
@@ -107,6 +115,52 @@ def valu_summary(path, top=12):
     return rows[:top]
 
 
+def mix_summary(path, top=8):
+    """Per (kernel, grid), heaviest first: SQ_INSTS_VALU and every FP32 / FP64 instruction counter present, per
+    dispatch (wave instructions)."""
+    per = collections.defaultdict(lambda: collections.defaultdict(list))
+    for r in csv.DictReader(open(path)):
+        per[(short_name(r["Kernel_Name"]), int(r["Grid_Size"]))][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    names = sorted({c for d in per.values() for c in d if c.startswith("SQ_INSTS_VALU_")})
+    rows = []
+    for key, d in per.items():
+        if "SQ_INSTS_VALU" not in d:
+            continue
+        mean = {c: sum(v) / len(v) for c, v in d.items()}
+        rows.append((key, len(d["SQ_INSTS_VALU"]), mean))
+    rows.sort(key=lambda r: -r[2]["SQ_INSTS_VALU"])
+    return names, rows[:top]
+
+
+MFMA_F64_PEAK = 78.6e12
+
+
+def mfma_summary(path):
+    """Per (kernel, grid) with FP64 matrix operations: FLOPs per dispatch (MOPS x 512), duration, rate, fraction of
+    the FP64 matrix peak, and the matrix-core busy fraction."""
+    per = collections.defaultdict(lambda: collections.defaultdict(list))
+    dur = collections.defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        key = (short_name(r["Kernel_Name"]), int(r["Grid_Size"]))
+        per[key][r["Counter_Name"]].append(float(r["Counter_Value"]))
+        if r["Counter_Name"] == "GRBM_GUI_ACTIVE":
+            dur[key].append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    rows = []
+    for key, d in per.items():
+        mops = d.get("SQ_INSTS_VALU_MFMA_MOPS_F64")
+        if not mops or sum(mops) == 0 or not dur[key]:
+            continue
+        flops = 512.0 * sum(mops) / len(mops)
+        ns = sum(dur[key]) / len(dur[key])
+        busy = sum(d["SQ_VALU_MFMA_BUSY_CYCLES"]) / len(d["SQ_VALU_MFMA_BUSY_CYCLES"]) if "SQ_VALU_MFMA_BUSY_CYCLES" in d else None
+        gui = sum(d["GRBM_GUI_ACTIVE"]) / len(d["GRBM_GUI_ACTIVE"]) / 8.0 if "GRBM_GUI_ACTIVE" in d else None
+        rows.append({"kernel": key[0], "grid": key[1], "dispatches": len(mops), "flops": flops, "us": ns / 1e3,
+                     "tflops": flops / ns / 1e3, "frac": flops / (ns * 1e-9) / MFMA_F64_PEAK,
+                     "busy": busy / (gui * 1024.0) if busy is not None and gui else None})
+    rows.sort(key=lambda r: -r["flops"])
+    return rows
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -121,7 +175,26 @@ def main(argv=None):
     c.add_argument("write")
     c.add_argument("--copy-bytes", type=int, default=1 << 28)
     c.add_argument("--json")
+    x = sub.add_parser("mix")
+    x.add_argument("counters")
+    m = sub.add_parser("mfma")
+    m.add_argument("counters")
     a = ap.parse_args(argv)
+    if a.cmd == "mix":
+        names, rows = mix_summary(a.counters)
+        short = [n.replace("SQ_INSTS_VALU_", "") for n in names]
+        print("| kernel | grid (threads) | dispatches | VALU (wave instr.) | " + " | ".join(short) + " |\n|" + "---|" * (4 + len(names)))
+        for key, nd, mean in rows:
+            print(f"| `{key[0]}` | {key[1]} | {nd} | {mean['SQ_INSTS_VALU']:.3g} | " + " | ".join(f"{mean.get(n, 0.0):.3g}" for n in names) + " |")
+        return
+    if a.cmd == "mfma":
+        print("| kernel | grid (threads) | dispatches | FP64 matrix FLOPs per dispatch | avg (us) | TFLOP/s | of the FP64 matrix peak | MFMA busy |\n"
+              "|---|---|---|---|---|---|---|---|")
+        for r in mfma_summary(a.counters):
+            busy = f"{100 * r['busy']:.0f}%" if r["busy"] is not None else "-"
+            print(f"| `{r['kernel']}` | {r['grid']} | {r['dispatches']} | {r['flops']:.4g} | {r['us']:.1f} | {r['tflops']:.1f} | "
+                  f"{100 * r['frac']:.0f}% | {busy} |")
+        return
     if a.cmd == "valu":
         print("| kernel | grid (threads) | VALU instr. (wave) | SALU instr. (wave) | busy cycles (8 XCDs) | VALU issue utilisation |\n"
               "|---|---|---|---|---|---|")

@@ -98,11 +98,11 @@ template <class F> Tally count(F&& f) {
 // splats kept[koff[s] .. koff[s + 1]) (ascending splat indices with weight above the cutoff). Output (flops,
 // special) pairs, summed over the samples (fp32: the branch gates of the FP32 path, else those of the FP64 path):
 //   out[0..1]   the transfer's splat sums (sample_coefficients over the kept splats, the FP64 geometry excluded)
-//   out[2..3]   the FP64 geometry (make_geo), once per sample in the transfer and once per pair in the adjoint
+//   out[2..3]   the FP64 geometry (make_geo), once per sample in the transfer (the adjoint reads it back)
 //   out[4..5]   the transfer's step (SegMapT::step)
 //   out[6..7]   the adjoint's per-sample work (the tail step_apply and sample_adjoint)
-//   out[8..9]   the element adjoints (weight_dlog_knot / weight_dlog + element_adjoint_pop), over every kept pair
-//   out[10..11] one FP64 geometry (make_geo), per pair (the adjoint recomputes it for each pair)
+//   out[8..9]   the element adjoints (weight_dlog_knot / weight_dlog + element_adjoint_kappa), over every kept pair
+//   out[10..11] the loads of the stored geometry: no arithmetic (kept for the layout of the output)
 extern "C" int kswm_count_slice(int kind, int nrows, int ns, const double* params, double gamma_max, const double* knots,
                                 const double* knot_times, int nknots, double a, double thetao, double t_obs, double nu, double L,
                                 int nsmp, const double* pix4, const double* smp7, const int* koff, const int* kept, int fp32,
@@ -161,10 +161,9 @@ extern "C" int kswm_count_slice(int kind, int nrows, int ns, const double* param
                 const Cnt Gw = sc[i].nk > 0 ? weight_dlog_knot(sc[i], sa[i], kn.data(), kt.data(), tt, xx, yy, zz, dl)
                                             : weight_dlog(sc[i], sa[i], tt, xx, yy, zz, dl);
                 if (Gw > 0.0)
-                    element_adjoint_pop(sc[i], sa[i], G, Cnt(a), Cnt(px[0]), Cnt(px[1]), Cnt(std::sin(thetao)), Cnt(nu), bars, Gw, dl, grad);
+                    element_adjoint_kappa(sc[i], sa[i], G, Cnt(a), Cnt(px[0]), Cnt(px[1]), Cnt(std::sin(thetao)), Cnt(nu), bars, Gw, dl, grad);
             });
             out[8] += e.flops; out[9] += e.special;
-            out[10] += g.flops; out[11] += g.special;
         }
     }
     return 0;

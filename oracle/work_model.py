@@ -98,8 +98,8 @@ def kept_pairs(cfg, P, kn, kt, al, be, N, t_obs):
 def kappa_knot_work(cfg_id=4, nrays=96, count_samples=1500, seed=11):
     """Per ray-sample work of the full fit step's slice (kappa splats on knots): march, transfer (splat sums over
     the kept splats, the FP64 sample geometry, the step), the adjoint's per-sample steps and its element adjoints
-    (each kept pair: the weight's log-derivatives, the FP64 geometry again, the element adjoint), split into FP64
-    and scalar-of-the-path (FP32 on the mixed path) operations."""
+    (each kept pair: the weight's log-derivatives and the element adjoint over the sample geometry the transfer
+    stored), split into FP64 and scalar-of-the-path (FP32 on the mixed path) operations."""
     import ctypes
     import oracle
     cfg = syn.CONFIGS[cfg_id]
@@ -166,7 +166,7 @@ def kappa_knot_work(cfg_id=4, nrays=96, count_samples=1500, seed=11):
                                  "special_per_ray_sample": sp(9, f_pairs) + sp(11, f_pairs)},
         }
     return {"phases": phases(res_["mixed"]), "phases_fp64_path": phases(res_["fp64"]),
-            "flops_per_pair": {"element_adjoint": res_["mixed"][8] / p_cnt, "geometry_fp64": res_["mixed"][10] / p_cnt,
+            "flops_per_pair": {"element_adjoint": res_["mixed"][8] / p_cnt, "geometry_fp64_per_sample": res_["mixed"][2] / n_cnt,
                                "transfer_splat_sum": res_["mixed"][0] / p_cnt},
             "pairs_per_ray_sample": f_pairs, "active_samples_per_ray_sample": f_active, "valid_samples_per_ray_sample": nvalid / rs,
             "rays_counted": int(nrays), "samples_counted": n_cnt, "pairs_counted": p_cnt, "samples_per_ray": N, "splats": int(P.shape[1]),
