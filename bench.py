@@ -73,6 +73,11 @@ def copy_probe(nbytes=1 << 28, reps=3):
     hip.hipFree(b)
 
 
+def progress(msg):
+    """A line on stderr as each step of the run ends (the JSON result is the only line on stdout)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def model_parts(syn, cfg_id):
     """The configuration's model as (kind, params[, knots, knot_times]) parts."""
     m = syn.config_model(cfg_id)
@@ -121,11 +126,13 @@ def cpu_baseline(cfg, parts, L, target_s, t_obs, nu, res):
         oracle.model_gradient(a, th, cam.alphas[idx], cam.betas[idx], N, oparts, t_obs, nu, L, ds[idx])
         return time.perf_counter() - t0
 
-    n = 1
+    n = threads                   # one ray per thread to start: the oracle runs the rays in parallel
     dt = run(n)
+    progress(f"CPU baseline: {n} rays in {dt:.1f} s")
     while dt < 0.2 * target_s and n < cam.npix:
         n = min(cam.npix, max(n + 1, int(n * min(8.0, target_s / max(dt, 1e-3)))))
         dt = run(n)
+        progress(f"CPU baseline: {n} rays in {dt:.1f} s")
     return {"value": n * N / dt, "unit": "ray-samples/s", "cores": threads, "kind": "port",
             "sample": f"{n} of {cam.npix} rays (random subset) x {N} samples at one (frame, frequency), fwd+adjoint "
                       f"(dual sweep over every splat), FP64 oracle, {dt:.1f} s"}
@@ -230,9 +237,12 @@ def main():
         `steps` slices bracketed by stream synchronizations; per step, each stage's kernel time from the HIP events
         the library records around its launches."""
         g0 = np.zeros(model.grad_shape, order="F")
+        tw = time.perf_counter()
         _, cube0 = slice_step(0, g0, copy=True)
+        progress(f"warmup slice 0 done ({time.perf_counter() - tw:.1f} s)")
         for s in range(1, warmup):
             slice_step(s, dparams)
+            progress(f"warmup slice {s} done ({time.perf_counter() - tw:.1f} s)")
         sync()
         ms = {k: [] for k in STG}
         chis = []
@@ -242,6 +252,7 @@ def main():
             chis.append(slice_step(s, dparams)[0])
             for k in STG:
                 ms[k].append(api.stage_ms(k))
+            progress(f"timed slice {s - first + 1} of {steps} done ({time.perf_counter() - t0:.1f} s)")
         sync()
         return (time.perf_counter() - t0) / steps, {k: float(np.mean(x)) for k, x in ms.items()}, chis, cube0, g0
 

@@ -117,7 +117,8 @@ template <class S>
 KS_DEV bool weight_active(const SplatCT<S>& c, S t, S x, S y, S z) {
     S sp, cp;
     if (outside_support(c, t, x, y, z, sp, cp)) return false;
-    return exp(-0.5 * gauss_exponent(c, t, x, y, z, sp, cp)) > WEIGHT_CUTOFF;
+    const S g = gauss_exponent(c, t, x, y, z, sp, cp);
+    return !(g > GEXP_SKIP) && exp(-0.5 * g) > WEIGHT_CUTOFF;
 }
 
 // rows 7-10 of d ln G: dlnG/dR[a][b] = e_a gu_b (R the rotation of the pattern offset e, gu = dlnG/du) through the
@@ -146,7 +147,8 @@ KS_DEV bool weight_active_knot(const SplatCT<S>& c, const S* knots, const S* kti
     S q2(0.0);
     for (int b = 0; b < 3; ++b) { const S q = (c.R[b] * e0 + c.R[3 + b] * e1 + c.R[6 + b] * e2) * c.is[b]; q2 += q * q; }
     const S tt = (t - c.t0) * c.iw;
-    return exp(-0.5 * (q2 + tt * tt)) > WEIGHT_CUTOFF;
+    const S g = q2 + tt * tt;
+    return !(g > GEXP_SKIP) && exp(-0.5 * g) > WEIGHT_CUTOFF;
 }
 
 // d ln G of a splat whose centre follows knots (trajectories.jl:93-128) with respect to rows 1-12: the centre rows

@@ -68,6 +68,7 @@ KS_DEV bool sample_coefficients(const TransCtxT<S>& c, double st, double r, doub
             if (outside_support(sc, t, x, y, z, sph, cph)) continue;
             gexp = gauss_exponent(sc, t, x, y, z, sph, cph);
         }
+        if (gexp > GEXP_SKIP) continue;
         S G_ = exp(-0.5 * gexp);
         if (!(G_ > WEIGHT_CUTOFF)) continue;
         if (!geo_ready) {

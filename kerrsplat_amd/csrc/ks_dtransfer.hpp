@@ -21,6 +21,9 @@ constexpr double NU0 = 1e11;
 constexpr double JY = 1e-23;                  // erg s^-1 cm^-2 Hz^-1
 constexpr double PLANCK0 = 2.0 * HPL * NU0 * NU0 * NU0 / (CL * CL);
 constexpr double WEIGHT_CUTOFF = 1e-6;
+// an exponent above GEXP_SKIP leaves the Gaussian weight exp(-gexp / 2) below the cutoff in any precision
+// (exp(-13.82) = 9.96e-7: a margin of 0.4%, far above FP32 rounding), so the exponential is not taken there
+constexpr double GEXP_SKIP = 27.64;
 
 // MODEL_COMPOSITE: concatenated thermal, power-law and kappa splats (CompositeModel, transport.jl:298-316),
 // each splat's population in SplatCT::kind
