@@ -43,7 +43,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=5, help="timed slices of the mixed path")
     p.add_argument("--warmup", type=int, default=2, help="untimed slices before them")
-    p.add_argument("--fp64-steps", type=int, default=1, help="timed slices of the FP64 path (at most --steps)")
+    p.add_argument("--fp64-steps", type=int, default=2, help="timed slices of the FP64 path, taken among the mixed path's (at most --steps)")
     p.add_argument("--fp64-warmup", type=int, default=1)
     p.add_argument("--config", type=int, default=4)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -99,8 +99,10 @@ def build_model(K, parts):
 
 def cpu_baseline(cfg, parts, L, target_s, t_obs, nu, res):
     """The oracle (FP64 C++ restatement of the reference's per-ray CPU path: Recurrence(64) march, transfer and
-    dual sweep with every row of every splat, OpenMP over all host cores) on a bounded random subset of the
-    same rays at the first slice; fwd+adjoint ray-samples/s."""
+    dual sweep with every row of every splat, OpenMP over the rays on all host cores) on a bounded sample of the
+    same workload at the first slice: one random ray per thread, with as many samples per ray (at most the
+    workload's) as fit the target time (the oracle's work per ray-sample does not depend on the sampling: each
+    sample visits every splat); fwd+adjoint ray-samples/s."""
     import oracle
     a, th, N = cfg["a"], math.radians(cfg["theta_o_deg"]), cfg["N"]
     from kerrsplat_amd import Camera
@@ -120,22 +122,22 @@ def cpu_baseline(cfg, parts, L, target_s, t_obs, nu, res):
     order = rng.permutation(cam.npix)
     ds = rng.normal(size=(cam.npix, 4))
 
-    def run(n):
+    def run(n, ns_):
         idx = order[:n]
         t0 = time.perf_counter()
-        oracle.model_gradient(a, th, cam.alphas[idx], cam.betas[idx], N, oparts, t_obs, nu, L, ds[idx])
+        oracle.model_gradient(a, th, cam.alphas[idx], cam.betas[idx], ns_, oparts, t_obs, nu, L, ds[idx])
         return time.perf_counter() - t0
 
-    n = threads                   # one ray per thread to start: the oracle runs the rays in parallel
-    dt = run(n)
-    progress(f"CPU baseline: {n} rays in {dt:.1f} s")
-    while dt < 0.2 * target_s and n < cam.npix:
-        n = min(cam.npix, max(n + 1, int(n * min(8.0, target_s / max(dt, 1e-3)))))
-        dt = run(n)
-        progress(f"CPU baseline: {n} rays in {dt:.1f} s")
-    return {"value": n * N / dt, "unit": "ray-samples/s", "cores": threads, "kind": "port",
-            "sample": f"{n} of {cam.npix} rays (random subset) x {N} samples at one (frame, frequency), fwd+adjoint "
-                      f"(dual sweep over every splat), FP64 oracle, {dt:.1f} s"}
+    n, ns_ = threads, min(N, 32)  # one ray per thread: the oracle runs the rays in parallel
+    dt = run(n, ns_)
+    progress(f"CPU baseline: {n} rays x {ns_} samples in {dt:.1f} s")
+    if dt < 0.5 * target_s and ns_ < N:
+        ns_ = int(min(N, max(ns_ + 1, ns_ * target_s / max(dt, 1e-3))))
+        dt = run(n, ns_)
+        progress(f"CPU baseline: {n} rays x {ns_} samples in {dt:.1f} s")
+    return {"value": n * ns_ / dt, "unit": "ray-samples/s", "cores": threads, "kind": "port",
+            "sample": f"{n} random rays of the {cam.npix} x {ns_} samples per ray (of {N}) at one (frame, frequency), "
+                      f"fwd+adjoint (dual sweep over every splat), FP64 oracle, {dt:.1f} s"}
 
 
 def visibility_stage(K, syn, cfg, model, L, res=512, nuv=8192, reps=3):
@@ -232,40 +234,49 @@ def main():
             K.gradient_reverse(cache, model, dp, ds)
         return chi, cube
 
-    def timed(steps, warmup):
-        """warmup untimed slices (the first, slice 0, also gives the images and gradient of the error bars), then
-        `steps` slices bracketed by stream synchronizations; per step, each stage's kernel time from the HIP events
-        the library records around its launches."""
+    def timed(timed_slices, warm_slices):
+        """The warmup slices untimed (the first, slice 0, also gives the images and gradient of the error bars), then
+        the timed slices, each ended by a stream synchronization; per slice its wall time and each stage's kernel
+        time from the HIP events the library records around its launches."""
         g0 = np.zeros(model.grad_shape, order="F")
         tw = time.perf_counter()
-        _, cube0 = slice_step(0, g0, copy=True)
-        progress(f"warmup slice 0 done ({time.perf_counter() - tw:.1f} s)")
-        for s in range(1, warmup):
+        _, cube0 = slice_step(warm_slices[0], g0, copy=True)
+        progress(f"warmup slice {warm_slices[0]} done ({time.perf_counter() - tw:.1f} s)")
+        for s in warm_slices[1:]:
             slice_step(s, dparams)
             progress(f"warmup slice {s} done ({time.perf_counter() - tw:.1f} s)")
         sync()
         ms = {k: [] for k in STG}
-        chis = []
-        first = max(warmup, 1)
+        chis, wall = [], {}
         t0 = time.perf_counter()
-        for s in range(first, first + steps):
+        for n, s in enumerate(timed_slices):
+            ts = time.perf_counter()
             chis.append(slice_step(s, dparams)[0])
+            sync()
+            wall[s] = time.perf_counter() - ts
             for k in STG:
                 ms[k].append(api.stage_ms(k))
-            progress(f"timed slice {s - first + 1} of {steps} done ({time.perf_counter() - t0:.1f} s)")
-        sync()
-        return (time.perf_counter() - t0) / steps, {k: float(np.mean(x)) for k, x in ms.items()}, chis, cube0, g0
+            progress(f"timed slice {n + 1} of {len(timed_slices)} (slice {s}) done ({time.perf_counter() - t0:.1f} s)")
+        return (time.perf_counter() - t0) / len(timed_slices), wall, {k: float(np.mean(x)) for k, x in ms.items()}, chis, cube0, g0
 
+    # the mixed path times --steps slices after --warmup; the FP64 path, several times slower, times --fp64-steps of
+    # those same slices (spread over them), so that the two are compared on the same work
+    n_w = max(args.warmup, 1)
+    mixed_timed = list(range(n_w, n_w + args.steps))
+    k64 = max(1, min(args.steps, args.fp64_steps))
+    fp64_timed = [mixed_timed[int((q + 0.5) * len(mixed_timed) / k64)] for q in range(k64)]
+    plan = {"mixed": (mixed_timed, list(range(n_w))), "fp64": (fp64_timed, list(range(max(min(args.warmup, args.fp64_warmup), 1))))}
     runs = {}
     for prec in [p_ for p_ in ("mixed", "fp64") if p_ in args.paths.split(",")]:
         K.precision(cache, prec)
-        steps = args.steps if prec == "mixed" else max(1, min(args.steps, args.fp64_steps))
-        warm = args.warmup if prec == "mixed" else min(args.warmup, args.fp64_warmup)
-        print(f"bench: {prec}: {max(warm, 1)} warmup + {steps} timed slices", file=sys.stderr, flush=True)
-        wall, stage, chis, cube, g = timed(steps, warm)
-        runs[prec] = {"wall_s": wall, "stage_ms": stage, "steps": steps, "warmup": max(warm, 1), "cube": cube, "grad": g,
-                      "chi2": chis[-1] if chis else None}
-        print(f"bench: {prec}: {1e3 * wall:.0f} ms per slice {stage}", file=sys.stderr, flush=True)
+        tsl, wsl = plan[prec]
+        if prec == "fp64" and "mixed" not in args.paths.split(","):
+            tsl = mixed_timed[:k64]
+        progress(f"{prec}: {len(wsl)} warmup + {len(tsl)} timed slices")
+        wall, per, stage, chis, cube, g = timed(tsl, wsl)
+        runs[prec] = {"wall_s": wall, "per_slice_s": per, "stage_ms": stage, "steps": len(tsl), "warmup": len(wsl), "cube": cube,
+                      "grad": g, "chi2": chis[-1] if chis else None}
+        progress(f"{prec}: {1e3 * wall:.0f} ms per slice {stage}")
     K.precision(cache, "fp64")
 
     both = "mixed" in runs and "fp64" in runs
@@ -280,6 +291,13 @@ def main():
         head = next(iter(runs))         # a profiling run of one path: no error bars, no headline comparison
     per_slice = cam.npix * N
     ms_step = {p_: 1e3 * r["wall_s"] + once_ms / nsl for p_, r in runs.items()}
+    # the speed-up on the slices both paths timed (each with its share of the once-per-fit-step work)
+    ratio, matched = None, []
+    if both:
+        matched = sorted(runs["fp64"]["per_slice_s"])
+        t64 = sum(runs["fp64"]["per_slice_s"][s] for s in matched) + len(matched) * once_ms / nsl * 1e-3
+        t32 = sum(runs["mixed"]["per_slice_s"][s] for s in matched) + len(matched) * once_ms / nsl * 1e-3
+        ratio = t64 / t32
     ms_per_step = ms_step[head]
     value = per_slice / (ms_per_step * 1e-3)
 
@@ -295,8 +313,16 @@ def main():
                  "adjoint_steps": lambda p_: f"ks_adj_segment<{tpl[p_]}>",
                  "adjoint_elements": lambda p_: f"ks_adj_splats<{tpl[p_]}, {adj_pops(parts)}>",
                  "vis": lambda p_: "ks_dgemm<false>"}
-    tr_path = os.path.join(ROOT, "profiles", f"traffic_{cfg['name']}.json")
-    traffic = json.load(open(tr_path))["kernels"] if os.path.exists(tr_path) else {}
+    # HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE summary of the workload (rocprofv3 --pmc, scaled by
+    # a copy probe); a kernel launched at several grid sizes is keyed name@grid: its largest launch is taken
+    tr_file = {1: "traffic_128x128_N1024_32thermal.json", 4: "traffic_512x512_N2048_1024kappa_knots.json"}.get(args.config)
+    tr_path = os.path.join(ROOT, "profiles", tr_file) if tr_file else ""
+    traffic = {}
+    if tr_path and os.path.exists(tr_path):
+        for key, v in json.load(open(tr_path))["kernels"].items():
+            name = key.split("@")[0]
+            if name not in traffic or v["grid"] > traffic[name]["grid"]:
+                traffic[name] = v
 
     def stage_table(prec):
         peak_ = FP64_PEAK_TFLOPS if prec == "fp64" else FP32_PEAK_TFLOPS
@@ -324,7 +350,8 @@ def main():
                     "kernel_ms_per_slice": d["ms"], "flops_per_ray_sample": d["flops_per_ray_sample"],
                     "precision": "fp32" if head == "mixed" else "fp64"}
     paths = {p_: {"value": per_slice / (ms_step[p_] * 1e-3), "unit": "ray-samples/s", "ms_per_step": ms_step[p_],
-                  "slice_wall_ms": 1e3 * r["wall_s"], "gradient_evals_per_s": 1.0 / (nsl * ms_step[p_] * 1e-3),
+                  "slice_wall_ms": 1e3 * r["wall_s"], "slice_wall_ms_by_slice": {str(s): 1e3 * t for s, t in r["per_slice_s"].items()},
+                  "gradient_evals_per_s": 1.0 / (nsl * ms_step[p_] * 1e-3),
                   "steps": r["steps"], "warmup": r["warmup"], "stage_ms": r["stage_ms"], "vis_chi2_last": r["chi2"]}
              for p_, r in runs.items()}
 
@@ -348,7 +375,7 @@ def main():
                               if head == "mixed" else
                               "plain FP64: Recurrence(64) march into stored samples, transfer, dual-sweep adjoint, visibility "
                               "chi2 and seed on the device"},
-           "mixed_over_fp64": ms_step["fp64"] / ms_step["mixed"] if both else None,
+           "mixed_over_fp64": ratio, "mixed_over_fp64_slices": matched,
            "mixed_error_vs_fp64": {"stokes_over_peak_I": img_err, "gradient_over_max": grad_err, "finite": finite},
            "fit_step_s": {p_: nsl * ms_step[p_] * 1e-3 for p_ in runs},
            "once_per_fit_step_ms": {"model_build": model_ms, "march": march_ms},
