@@ -234,6 +234,18 @@ int ks_binning_create(ks_ctx* ctx, int64_t npoints, const int64_t* pixel, int n_
 int ks_binning_destroy(ks_binning* binning);
 
 /* _frame_seed! through a Binning (Fit.jl:555-574): the χ² of the binned frame `frame` of the last
+/* chi2_closures (visibilities.jl:141-199) of frame `frame` of the last ks_gradient_forward, on the device: the
+ * (tapered) model visibilities at the nuv (u, v) points, the closure phases of Stokes I on ntri triangles (3
+ * 1-based baseline indices each, negative = conjugated, the phase difference taken to the nearest multiple of
+ * 2 pi) and the log closure amplitudes ln(|V1||V2| / (|V3||V4|)) on nquad quadrangles (4 indices each, signs
+ * ignored) against the data and their sigmas; the χ² in chi2 and its adjoint seed (d χ² / d V through the
+ * adjoint transform) written into the frame's device seed, or added to it when add, for
+ * ks_gradient_reverse(dstokes = NULL). The reference differentiates this term with host Enzyme
+ * (timeresolved.jl:157-168); here no visibility leaves the device. */
+int ks_closure_chi2_seed(ks_cache* cache, int64_t frame, double psize, const double* u, const double* v, const double* taper,
+                         int nuv, int ntri, const int* triangles, const double* phases, const double* sigma_phase, int nquad,
+                         const int* quadrangles, const double* logamps, const double* sigma_logamp, int add, double* chi2);
+
  * ks_gradient_forward, Σ ((bin(I) - d) / σ)² over the masked pixels of the binned screen, returned in
  * chi2, and its seed, 2 r / σ / count shared by the points of each pixel (0 outside the mask),
  * written into the device seed of that frame. The binning's points must be the cache's pixels.

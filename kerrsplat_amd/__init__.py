@@ -18,7 +18,7 @@ is missing or no GPU is present.
 from .api import (Camera, Direct, Recurrence, Fused, GeodesicCache, Model, CompositeModel, regenerate, thin_image, polarized_image,
                   polarized_cube, polarized_gradient, march, precision, stage_ms, render_device, visibilities, unsort, to_screen, samples, residuals, consts, KernelError,
                   NSPLATPARAMS, NPOLARIZEDPARAMS, NPOWERLAWPARAMS, NKAPPAPARAMS, gravitational_radius, grid_axis,
-                  library_path, gradient_forward, gradient_reverse, vis_chi2_seed, image_chi2_seed, vis_seed, tiles, Binning,
+                  library_path, gradient_forward, gradient_reverse, vis_chi2_seed, image_chi2_seed, vis_seed, closure_chi2_seed, tiles, Binning,
                   bin, binned_grid, binned_polar, concatenate, spacetime_tangent, RayLists, ray_lists, framed, flattened, frame_lists)
 from . import fit
 from .fit import (ScatteringKernel, taper, spacetime_valgrad, spacetime_jacobian, spacetime_movie_loss, StokesMovie, FrameScans, chi2, chi2_gradient, sweep_passes, frame_chi2_seed, image_loss_gradient, closure_phases,

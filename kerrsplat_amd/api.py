@@ -88,6 +88,7 @@ def _load():
     lib.ks_binning_destroy.argtypes = [_P]
     lib.ks_image_chi2_seed_binned.argtypes = [_P, _i64, _P, _P, _P, _i, _P, _P]
     lib.ks_vis_seed.argtypes = [_P, _i64, _d, _P, _P, _P, _P, _i, _i]
+    lib.ks_closure_chi2_seed.argtypes = [_P, _i64, _d, _P, _P, _P, _i, _i, _P, _P, _P, _i, _P, _P, _P, _i, _P]
     lib.ks_march.argtypes = [_P]
     lib.ks_cache_precision.argtypes = [_P, _i]
     _lib = lib
@@ -745,6 +746,30 @@ def vis_chi2_seed(cache, frame, psize, u, v, vis, sigma, taper=None, gains=None,
                                        _i(nuv), _ptr(chi), _ptr(V)))
     if return_model:
         return float(chi[0]), V[..., 0] + 1j * V[..., 1]
+    return float(chi[0])
+
+
+def closure_chi2_seed(cache, frame, psize, data, taper=None, add=False):
+    """chi2_closures (visibilities.jl:141-199) of frame `frame` of the last forward pass on the device: the closure
+    phases and log closure amplitudes of the model's Stokes I at data.u, data.v against `data` (a ClosureData:
+    1-based signed baseline indices, phases and sigmas, log amplitudes and sigmas), the chi2 returned and its
+    adjoint seed written on the device (added to the frame's seed with add=True)."""
+    uu, vv = _f64(data.u), _f64(data.v)
+    nuv = len(uu)
+    tri = np.ascontiguousarray(np.asarray(data.triangles, dtype=np.int32).reshape(-1, 3))
+    quad = np.ascontiguousarray(np.asarray(data.quadrangles, dtype=np.int32).reshape(-1, 4))
+    ntri, nquad = len(tri), len(quad)
+    cp, sp = _f64(data.phases), _f64(data.sigma_phase)
+    la, sl = _f64(data.logamps), _f64(data.sigma_logamp)
+    if len(cp) != ntri or len(sp) != ntri or len(la) != nquad or len(sl) != nquad:
+        raise DimensionMismatch("closure data: one phase and sigma per triangle, one log amplitude and sigma per quadrangle")
+    tp = _f64(taper) if taper is not None else None
+    chi = np.zeros(1)
+    ip = ctypes.POINTER(ctypes.c_int)
+    _check(cache._lib.ks_closure_chi2_seed(cache._h, _i64(frame), _d(psize), _ptr(uu), _ptr(vv), _ptr(tp), _i(nuv), _i(ntri),
+                                           tri.ctypes.data_as(ip) if ntri else None, _ptr(cp) if ntri else None,
+                                           _ptr(sp) if ntri else None, _i(nquad), quad.ctypes.data_as(ip) if nquad else None,
+                                           _ptr(la) if nquad else None, _ptr(sl) if nquad else None, _i(int(add)), _ptr(chi)))
     return float(chi[0])
 
 

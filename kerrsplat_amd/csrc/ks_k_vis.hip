@@ -190,6 +190,71 @@ __global__ void ks_vis_scale_weights(int nuv, const double* __restrict__ w, cons
     for (int q = 0; q < 8; ++q) wq[8 * (int64_t)k + q] = w[8 * (int64_t)k + q] * t;
 }
 
+// ---- closure phases and log closure amplitudes of one frame (chi2_closures, visibilities.jl:141-199) ------------------
+// The Stokes I visibility of 1-based signed baseline index k (negative: the conjugate), as (re, im).
+KS_DEV void signed_vis(const double* __restrict__ V, int k, double& re, double& im) {
+    const int64_t o = 8 * (int64_t)(k < 0 ? -k - 1 : k - 1);
+    re = V[o]; im = k < 0 ? -V[o + 1] : V[o + 1];
+}
+// One thread per closure term: triangles first (the bispectrum's phase against the data, the difference taken to
+// the nearest multiple of 2 pi, rem(., 2 pi, RoundNearest)), then quadrangles (ln |V1||V2| / (|V3||V4|) against the
+// data). chi_t: the term's chi2; coef: d chi2 / d (phase or log amplitude).
+__global__ void ks_closure_terms(int ntri, const int* __restrict__ tri, const double* __restrict__ cp_data, const double* __restrict__ cp_sig,
+                                 int nquad, const int* __restrict__ quad, const double* __restrict__ la_data, const double* __restrict__ la_sig,
+                                 const double* __restrict__ V, double* __restrict__ chi_t, double* __restrict__ coef) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntri + nquad) return;
+    if (t < ntri) {
+        double br = 1.0, bi = 0.0;
+        for (int l = 0; l < 3; ++l) {
+            double zr, zi;
+            signed_vis(V, tri[3 * t + l], zr, zi);
+            const double nr = br * zr - bi * zi, ni = br * zi + bi * zr;
+            br = nr; bi = ni;
+        }
+        const double d = remainder(atan2(bi, br) - cp_data[t], 2.0 * PI);
+        const double is = 1.0 / cp_sig[t];
+        chi_t[t] = d * d * is * is;
+        coef[t] = 2.0 * d * is * is;
+    } else {
+        const int q = t - ntri;
+        double la = 0.0;
+        for (int l = 0; l < 4; ++l) {
+            double zr, zi;
+            signed_vis(V, quad[4 * q + l], zr, zi);
+            const double lz = 0.5 * log(zr * zr + zi * zi);
+            la += l < 2 ? lz : -lz;
+        }
+        const double r = (la - la_data[q]) / la_sig[q];
+        chi_t[t] = r * r;
+        coef[t] = 2.0 * r / la_sig[q];
+    }
+}
+// One thread per baseline k: w = d chi2 / d Re V + i d chi2 / d Im V of its Stokes I visibility (Q, U, V zero), summed
+// over the terms that use it in a fixed order (the lists legs[off[k] .. off[k + 1]): entry = term index, with the leg's
+// role in bit 30 (set: a conjugated triangle leg or a denominator amplitude)). d arg V = Im(dV / V), d ln |V| = Re(dV / V):
+// w += c i V / |V|^2 for a triangle leg (negated when conjugated), c V / |V|^2 for a numerator amplitude (negated in
+// the denominator).
+__global__ void ks_closure_weights(int nuv, int ntri, const int* __restrict__ off, const int* __restrict__ legs, const double* __restrict__ coef,
+                                   const double* __restrict__ V, double* __restrict__ w) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nuv) return;
+    const double zr = V[8 * (int64_t)k], zi = V[8 * (int64_t)k + 1];
+    const double n2 = zr * zr + zi * zi;
+    double wr = 0.0, wi = 0.0;
+    if (n2 > 0.0)
+        for (int e = off[k]; e < off[k + 1]; ++e) {
+            const int t = legs[e] & 0x3fffffff;
+            const double s = (legs[e] & 0x40000000) ? -1.0 : 1.0;
+            const double c = s * coef[t] / n2;
+            if (t < ntri) { wr += -c * zi; wi += c * zr; }     // c i z
+            else { wr += c * zr; wi += c * zi; }               // c z
+        }
+    double* o = w + 8 * (int64_t)k;
+    o[0] = wr; o[1] = wi;
+    for (int q = 2; q < 8; ++q) o[q] = 0.0;
+}
+
 // ---- the image χ² of one frame (_frame_seed!, unbinned): r = (I - d) / sigma over masked pixels, seed 2 r / sigma ---
 // data, sigma (4 or 4 per pixel) and mask (per pixel, or null) in screen order; image and seed in sorted order.
 __global__ void ks_image_residual(const double* __restrict__ out, const int* __restrict__ perm, int64_t npix, int64_t f,
@@ -321,6 +386,16 @@ hipError_t launch_vis_scale_weights(hipStream_t s, int nuv, const double* w, con
 hipError_t launch_image_residual(hipStream_t s, const double* out, const int* perm, int64_t npix, int64_t f, const double* data,
                                  const double* sigma, int sigma_per_pixel, const uint8_t* mask, double* ads, double* chi_p) {
     hipLaunchKernelGGL(ks_image_residual, dim3(grid_for(npix, 256)), dim3(256), 0, s, out, perm, npix, f, data, sigma, sigma_per_pixel, mask, ads, chi_p);
+    return hipGetLastError();
+}
+hipError_t launch_closure_terms(hipStream_t s, int ntri, const int* tri, const double* cp_data, const double* cp_sig, int nquad, const int* quad,
+                                const double* la_data, const double* la_sig, const double* V, double* chi_t, double* coef) {
+    hipLaunchKernelGGL(ks_closure_terms, dim3(grid_for(ntri + nquad, 256)), dim3(256), 0, s, ntri, tri, cp_data, cp_sig, nquad, quad, la_data,
+                       la_sig, V, chi_t, coef);
+    return hipGetLastError();
+}
+hipError_t launch_closure_weights(hipStream_t s, int nuv, int ntri, const int* off, const int* legs, const double* coef, const double* V, double* w) {
+    hipLaunchKernelGGL(ks_closure_weights, dim3(grid_for(nuv, 256)), dim3(256), 0, s, nuv, ntri, off, legs, coef, V, w);
     return hipGetLastError();
 }
 hipError_t launch_sum_fixed(hipStream_t s, const double* x, int64_t n, double* out) {

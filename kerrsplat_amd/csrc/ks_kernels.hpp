@@ -264,3 +264,6 @@ hipError_t launch_tangent(hipStream_t s, int kind, double a, double thetao, doub
 inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
 }  // namespace ksd
+hipError_t launch_closure_terms(hipStream_t s, int ntri, const int* tri, const double* cp_data, const double* cp_sig, int nquad, const int* quad,
+                                const double* la_data, const double* la_sig, const double* V, double* chi_t, double* coef);
+hipError_t launch_closure_weights(hipStream_t s, int nuv, int ntri, const int* off, const int* legs, const double* coef, const double* V, double* w);

@@ -131,6 +131,8 @@ struct ks_cache {
     Buf<double> vu, vv, vt, vg, vd, vs, vA2, vB2, vIm, vC2, vV, vwq, vE2, vSd, vSp, vchi, vsum, idata, isig;
     Buf<uint8_t> imask;
     Buf<double> bseed;                                 // binned χ²: the seed of every binned pixel
+    Buf<int> ctri, cquad, coff, clegs;                 // closures: triangles, quadrangles, per-baseline term lists
+    Buf<double> cdat, cterm;                           // closures: data and sigmas; per-term χ² and coefficients
     double* seg = nullptr; int64_t seg_cap = 0;        // per-segment transfer maps
     double* dtn = nullptr; int64_t dtn_cap = 0;        // observation times then frequencies (device)
     std::vector<double> htn;                           // their host staging copy (outlives the async copy)
@@ -293,6 +295,7 @@ int ks_cache_info(const ks_cache* c, int64_t* npix, int* N, int* generation, int
     if (counts) for (int i = 0; i < 3; ++i) counts[i] = c->counts[i];
     return KS_OK;
 }
+    c->ctri.release(); c->cquad.release(); c->coff.release(); c->clegs.release(); c->cdat.release(); c->cterm.release();
 
 static Samples samples_of(const ks_cache* c) {
     Samples S;
@@ -1187,3 +1190,72 @@ int ks_image_chi2_seed_binned(ks_cache* c, int64_t frame, const ks_binning* b, c
 }
 
 }  // extern "C"
+int ks_closure_chi2_seed(ks_cache* c, int64_t frame, double psize, const double* u, const double* v, const double* taper, int nuv,
+                         int ntri, const int* triangles, const double* phases, const double* sigma_phase, int nquad, const int* quadrangles,
+                         const double* logamps, const double* sigma_logamp, int add, double* chi2) {
+    if (!c || !u || !v || !chi2) return fail(KS_ERR_ARG, "ks_closure_chi2_seed: null argument");
+    if (nuv < 1) return fail(KS_ERR_ARG, "need at least one (u, v) point");
+    if (ntri < 0 || nquad < 0 || ntri + nquad < 1) return fail(KS_ERR_ARG, "need at least one closure triangle or quadrangle");
+    if ((ntri > 0 && (!triangles || !phases || !sigma_phase)) || (nquad > 0 && (!quadrangles || !logamps || !sigma_logamp)))
+        return fail(KS_ERR_ARG, "ks_closure_chi2_seed: null closure data");
+    if (int rc = seed_frame_ok(c, frame)) return rc;
+    if (int rc = screen_ready(c)) return rc;
+    if (add && !c->fw_seeded[frame]) return fail(KS_ERR_STATE, "frame %lld has no seed to add to", (long long)frame);
+    // the per-baseline lists of the terms that use each baseline, in term order (the legs' order within a term)
+    std::vector<int> off(nuv + 1, 0), legs;
+    std::vector<std::pair<int, int>> uses;               // (baseline, entry)
+    for (int t = 0; t < ntri; ++t) {
+        if (!(sigma_phase[t] > 0.0)) return fail(KS_ERR_ARG, "closure phase sigma must be positive (triangle %d)", t);
+        for (int l = 0; l < 3; ++l) {
+            const int k = triangles[3 * t + l];
+            if (k == 0 || k > nuv || k < -nuv) return fail(KS_ERR_ARG, "triangle %d: baseline index %d outside 1:%d", t, k, nuv);
+            uses.push_back({std::abs(k) - 1, t | (k < 0 ? 0x40000000 : 0)});
+        }
+    }
+    for (int q = 0; q < nquad; ++q) {
+        if (!(sigma_logamp[q] > 0.0)) return fail(KS_ERR_ARG, "log closure amplitude sigma must be positive (quadrangle %d)", q);
+        for (int l = 0; l < 4; ++l) {
+            const int k = quadrangles[4 * q + l];
+            if (k == 0 || k > nuv || k < -nuv) return fail(KS_ERR_ARG, "quadrangle %d: baseline index %d outside 1:%d", q, k, nuv);
+            uses.push_back({std::abs(k) - 1, (ntri + q) | (l >= 2 ? 0x40000000 : 0)});
+        }
+    }
+    std::stable_sort(uses.begin(), uses.end(), [](const std::pair<int, int>& x, const std::pair<int, int>& y) { return x.first < y.first; });
+    legs.reserve(uses.size());
+    for (const auto& e : uses) { off[e.first + 1] += 1; legs.push_back(e.second); }
+    for (int k = 0; k < nuv; ++k) off[k + 1] += off[k];
+    std::vector<double> dat;
+    dat.insert(dat.end(), phases, phases + ntri); dat.insert(dat.end(), sigma_phase, sigma_phase + ntri);
+    dat.insert(dat.end(), logamps, logamps + nquad); dat.insert(dat.end(), sigma_logamp, sigma_logamp + nquad);
+    ks_ctx* x = c->ctx;
+    HIPCHK(hipSetDevice(x->device));
+    const int na = c->nalpha, nb = c->nbeta, nt = ntri + nquad;
+    if (int rc = vis_setup(c, psize, u, v, taper, nuv)) return rc;
+    if (c->ctri.grow((size_t)std::max(3 * ntri, 1)) || c->cquad.grow((size_t)std::max(4 * nquad, 1)) || c->coff.grow(off.size()) ||
+        c->clegs.grow(std::max(legs.size(), (size_t)1)) || c->cdat.grow(std::max(dat.size(), (size_t)1)) || c->cterm.grow((size_t)2 * nt) ||
+        c->vd.grow((size_t)8 * nuv) || c->vwq.grow((size_t)8 * nuv) || c->vE2.grow((size_t)2 * nuv * 4 * nb) || c->vSd.grow((size_t)na * 4 * nb) ||
+        c->vSp.grow((size_t)VIS_SPLITK * na * 4 * nb) || c->vsum.grow(1))
+        return KS_ERR_HIP;
+    if (ntri) HIPCHK(hipMemcpyAsync(c->ctri.p, triangles, (size_t)3 * ntri * sizeof(int), hipMemcpyHostToDevice, x->stream));
+    if (nquad) HIPCHK(hipMemcpyAsync(c->cquad.p, quadrangles, (size_t)4 * nquad * sizeof(int), hipMemcpyHostToDevice, x->stream));
+    HIPCHK(hipMemcpyAsync(c->coff.p, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, x->stream));
+    HIPCHK(hipMemcpyAsync(c->clegs.p, legs.data(), legs.size() * sizeof(int), hipMemcpyHostToDevice, x->stream));
+    HIPCHK(hipMemcpyAsync(c->cdat.p, dat.data(), dat.size() * sizeof(double), hipMemcpyHostToDevice, x->stream));
+    const double scale = psize * psize / JY;
+    const double* d_tap = taper ? c->vt.p : nullptr;
+    if (int rc = vis_forward(c, frame, 1, nuv, d_tap, scale)) return rc;
+    const double* d_cp = c->cdat.p;
+    HIPCHK(launch_closure_terms(x->stream, ntri, c->ctri.p, d_cp, d_cp + ntri, nquad, c->cquad.p, d_cp + 2 * ntri, d_cp + 2 * ntri + nquad, c->vV.p,
+                                c->cterm.p, c->cterm.p + nt));
+    HIPCHK(launch_closure_weights(x->stream, nuv, ntri, c->coff.p, c->clegs.p, c->cterm.p + nt, c->vV.p, c->vd.p));
+    HIPCHK(launch_vis_scale_weights(x->stream, nuv, c->vd.p, d_tap, scale, c->vwq.p));
+    HIPCHK(launch_vis_rotate(x->stream, nuv, nb, c->vwq.p, c->vB2.p, c->vE2.p));
+    HIPCHK(launch_dgemm_tn_splitk(x->stream, na, 4 * nb, 2 * nuv, c->vA2.p, na, c->vE2.p, 4 * nb, c->vSd.p, VIS_SPLITK, c->vSp.p));
+    HIPCHK(launch_vis_scatter_seed(x->stream, c->vSd.p, c->perm, c->npix, na, nb, frame, add, c->ads));
+    HIPCHK(launch_sum_fixed(x->stream, c->cterm.p, nt, c->vsum.p));
+    HIPCHK(hipMemcpyAsync(chi2, c->vsum.p, sizeof(double), hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    c->fw_seeded[frame] = 1;
+    return KS_OK;
+}
+

@@ -220,11 +220,14 @@ def closure_chi2(V, data, grad=False):
     return (total, w) if grad else total
 
 
-def frame_chi2_closures(cache, c, data, dalpha, L, D, taper=None, add=False):
-    """chi2_closures of frame c of the last forward pass (visibilities on the device, closures on the host,
-    as the reference computes them) with its adjoint seed written on the device through vis_seed
-    (added to the frame's seed with add=True: a visibility and a closure term in one likelihood)."""
+def frame_chi2_closures(cache, c, data, dalpha, L, D, taper=None, add=False, on_host=False):
+    """chi2_closures of frame c of the last forward pass with its adjoint seed written on the device (added to the
+    frame's seed with add=True: a visibility and a closure term in one likelihood): on the device
+    (api.closure_chi2_seed), or with on_host=True the visibilities copied to the host, the closures and their
+    gradient there (closure_chi2) and the seed through vis_seed, as the reference computes them."""
     psize = dalpha * L / D
+    if not on_host:
+        return api.closure_chi2_seed(cache, c, psize, data, taper=taper, add=add)
     V = api.visibilities(cache, c + 1, psize, data.u, data.v, taper=taper)[c]
     chi, w = closure_chi2(V, data, grad=True)
     api.vis_seed(cache, c, psize, data.u, data.v, w, taper=taper, add=add)

@@ -257,10 +257,13 @@ def test_reverse_rejects_a_model_recreated_at_the_same_address():
         K.gradient_reverse(cache, m2, dp, np.zeros((1, 1, cache.npix, 4)))
 
 
-def test_closure_seed_vs_oracle(oracle):
-    """frame_chi2_closures: closure χ² of a frame's device visibilities and its seed through vis_seed (the
-    adjoint transform of dχ²/dV on the device), added to a visibility χ² seed of the same frame; the
-    gradient it drives equals the host-seeded dual sweep with the oracle's adjoint transform."""
+@pytest.mark.parametrize("on_host", [False, True])
+def test_closure_seed_vs_oracle(oracle, on_host):
+    """frame_chi2_closures: closure χ² of a frame's visibilities and its seed, added to a visibility χ² seed of the
+    same frame, on the device (ks_closure_chi2_seed: closure phases and log closure amplitudes, their χ² and
+    dχ²/dV, the adjoint transform) or with the closures on the host and the seed through vis_seed; the χ² equals
+    the host formula on the oracle's visibilities and the gradient it drives equals the host-seeded dual sweep with
+    the oracle's adjoint transform."""
     import itertools
     from kerrsplat_amd import fit
     na = 14
@@ -287,7 +290,7 @@ def test_closure_seed_vs_oracle(oracle):
     vdata = V0 * 1.1
     chi_v = K.vis_chi2_seed(cache, 0, psize, u, v, vdata, sig)
     dalpha = 30.0 / (na - 1)
-    chi_c = fit.frame_chi2_closures(cache, 0, data, dalpha, L, syn.DISTANCE_CM, add=True)
+    chi_c = fit.frame_chi2_closures(cache, 0, data, dalpha, L, syn.DISTANCE_CM, add=True, on_host=on_host)
     chi_c_ref, w = fit.closure_chi2(V0, data, grad=True)
     assert abs(chi_c - chi_c_ref) <= 1e-9 * chi_c_ref
     dp_dev = np.zeros(P.shape, order="F")
