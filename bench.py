@@ -50,6 +50,7 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
     p.add_argument("--paths", default="mixed,fp64",
                    help="the paths to run (a profiling run may take one; the headline needs both)")
+    p.add_argument("--no-tune", action="store_true", help="keep the default launch choices (no autotuner)")
     p.add_argument("--copy-probe", action="store_true",
                    help="also run device-to-device copies of 256 MiB (the reference for scaling HBM byte counters)")
     return p.parse_args()
@@ -166,9 +167,11 @@ def visibility_stage(K, syn, cfg, model, L, res=512, nuv=8192, reps=3):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on exactly one GPU; --gpus N > 1 is not supported "
-                                   "(the sharding design is described in DESIGN.md)"}))
+        print(json.dumps({"error": "this implementation runs on exactly one GPU; --gpus N > 1 is not supported "
+                                   "(the fit step would shard by frame and frequency slices, one GPU per process)"}))
         sys.exit(2)
+    if args.config == 2:
+        return batched_main(args)
     import kerrsplat_amd as K
     from kerrsplat_amd import api, synthetic as syn
 
@@ -206,6 +209,18 @@ def main():
     sync()
     march_ms = float(np.mean(march_ms))
     once_ms = model_ms + march_ms
+
+    # the transfer's launch choices by the autotuner (on a hashed 1/16 sample of the first slice's tasks, the mixed
+    # path with the adjoint records); both paths then run with them
+    # the element adjoints' launch form per path (one reverse pass in each form, on the first slice)
+    tune = None
+    if not args.no_tune:
+        K.precision(cache, "mixed")
+        t0 = time.perf_counter()
+        choice, table = K.autotune(cache, model, [slices[0][0]], [slices[0][1]], L, records=True, sample_every=16, adjoint=True)
+        tune = {"mixed": {"choice": choice, "seconds": time.perf_counter() - t0,
+                          "table": [{"chunk": c_, "patch": list(p_), "queue": q_, "ms": m_} for c_, p_, q_, m_ in table]}}
+        progress(f"autotune mixed ({tune['mixed']['seconds']:.1f} s): {choice}")
 
     # fixed synthetic visibility data: the FP64 model's visibilities of the first slice, perturbed by 5 % and
     # thermal noise at 1 % of the peak, with a different draw for each slice
@@ -269,6 +284,17 @@ def main():
     runs = {}
     for prec in [p_ for p_ in ("mixed", "fp64") if p_ in args.paths.split(",")]:
         K.precision(cache, prec)
+        if tune is not None:
+            mixed_choice = tune["mixed"]["choice"]
+            K.tuning(cache, chunk=mixed_choice["chunk"], queue=mixed_choice["queue"], patch=mixed_choice["patch"],
+                     adjoint_queue=mixed_choice["adjoint_queue"])
+            if prec == "fp64":
+                # the FP64 path keeps the transfer's choices and takes its own element-adjoint form
+                t0 = time.perf_counter()
+                choice, table = K.autotune(cache, model, [slices[0][0]], [slices[0][1]], L, adjoint=True, transfer=False)
+                tune["fp64"] = {"choice": choice, "seconds": time.perf_counter() - t0,
+                                "table": [{"chunk": c_, "patch": list(p_), "queue": q_, "ms": m_} for c_, p_, q_, m_ in table]}
+                progress(f"autotune fp64 element adjoints ({tune['fp64']['seconds']:.1f} s): {choice}")
         tsl, wsl = plan[prec]
         if prec == "fp64" and "mixed" not in args.paths.split(","):
             tsl = mixed_timed[:k64]
@@ -380,7 +406,94 @@ def main():
            "fit_step_s": {p_: nsl * ms_step[p_] * 1e-3 for p_ in runs},
            "once_per_fit_step_ms": {"model_build": model_ms, "march": march_ms},
            "paths": paths, "gradient_evals_per_s": 1.0 / (nsl * ms_per_step * 1e-3),
-           "stages": stages, "roofline": roofline, "cpu_baseline": cpu, "vis_stage": vis_stage}
+           "stages": stages, "roofline": roofline, "cpu_baseline": cpu, "vis_stage": vis_stage, "launch_tuning": tune}
+    print(json.dumps(out), flush=True)
+
+
+def batched_main(args):
+    """The batched configuration: one forward cube (the polarized transfer of every point of a 256 x 256 screen in
+    uneven 32^2-128^2 tiles plus 32 extra screens of U[48, 160]^2, 1024 samples per ray, 128 splats of the three
+    populations, 3 frequencies x 16 frames) in one launch of the mixed-precision transfer. The launch is timed as
+    one workgroup per task, as the persistent work queue (index order, then the cost order of the previous launch)
+    and with the autotuner's choice (timed on a sample of one slice's tasks); a step is one cube with the chosen
+    launch. The FP64 path renders the first slice for the error bar of the mixed cube. No adjoint: this
+    configuration is a forward render."""
+    import kerrsplat_amd as K
+    from kerrsplat_amd import api, synthetic as syn
+    cfg = syn.CONFIGS[2]
+    a, th, N = cfg["a"], math.radians(cfg["theta_o_deg"]), cfg["N"]
+    main_cam, splits, extras = syn.config_screens(2)
+    tiles, _ = K.screen_tiles(main_cam, splits)
+    cam = K.concatenate_screens(*tiles, *extras)
+    L = syn.length_unit()
+    parts = model_parts(syn, 2)
+    model = build_model(K, parts)
+    times, nus = list(cfg["times"]), list(cfg["nus"])
+    nsl = len(times) * len(nus)
+    lib, ctx = api._load(), api.context()
+    sync = lambda: lib.ks_ctx_sync(ctx)
+    progress(f"{cfg['name']}: {len(tiles)} tiles + {len(extras)} screens = {cam.npix} points x {N}, {model.ns} splats, {nsl} slices")
+    cache = K.GeodesicCache(cam, N, store_samples=True)
+    K.regenerate(cache, a, th, marcher=K.Recurrence(64))
+    sync()
+    progress("marched")
+    K.precision(cache, "mixed")
+    work = cam.npix * N * nsl
+
+    def cube_ms(tag):
+        t0 = time.perf_counter()
+        K.render_device(cache, model, times, nus, L)
+        sync()
+        wall = 1e3 * (time.perf_counter() - t0)
+        tr = K.stage_ms("transfer")
+        progress(f"{tag}: {K.tuning(cache)}: cube {wall:.0f} ms (transfer {tr:.0f} ms)")
+        return wall, tr
+
+    runs = {}
+    K.tuning(cache, chunk=32, queue=False, patch=(8, 8))
+    runs["grid"] = cube_ms("one workgroup per task (warm-up)")
+    runs["grid"] = cube_ms("one workgroup per task")
+    K.tuning(cache, queue=True)
+    runs["queue_index_order"] = cube_ms("work queue, index order")
+    runs["queue_cost_order"] = cube_ms("work queue, cost order")
+    t0 = time.perf_counter()
+    choice, table = K.autotune(cache, model, [times[0]], [nus[1]], L, records=False, sample_every=4)
+    tune_s = time.perf_counter() - t0
+    progress(f"autotune ({tune_s:.1f} s): {choice}")
+    steps = []
+    try:
+        w, tr = cube_ms("tuned cube (first)")
+    except api.KernelError as e:
+        # the segment maps of the whole cube at the tuned chunk length did not fit: the next longer chunk
+        progress(f"tuned launch failed ({e}); chunk {2 * choice['chunk']}")
+        K.tuning(cache, chunk=2 * choice["chunk"])
+        choice = K.tuning(cache)
+        w, tr = cube_ms("tuned cube (first)")
+    for k in range(max(args.warmup, 0) + max(args.steps, 1)):
+        w, tr = cube_ms(f"tuned cube {k + 1}")
+        if k >= args.warmup:
+            steps.append((w, tr))
+    if choice["queue"]:
+        w, tr = cube_ms("tuned cube (cost order)")
+    ms_per_step = float(np.mean([w for w, _ in steps]))
+    # the error bar: the first slice on both paths
+    img32 = np.array(K.polarized_cube(cache, model, [times[0]], [nus[0]], L))
+    K.precision(cache, "fp64")
+    img64 = np.array(K.polarized_cube(cache, model, [times[0]], [nus[0]], L))
+    sync()
+    peak = float(np.abs(img64[..., 0]).max())
+    err = float(np.abs(img32 - img64).max() / peak)
+    progress(f"mixed vs FP64 on the first slice: {err:.2e} of the peak")
+    out = {"metric": "ray-samples/s, forward cube over batched screens (mixed FP32 transfer, FP64 geodesics)", "value": work / (ms_per_step * 1e-3),
+           "unit": "ray-samples/s", "n_gpus": 1, "steps": len(steps), "warmup": args.warmup, "ms_per_step": ms_per_step,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "mixed f32/f64", "data": "synthetic (seeded draws)",
+           "config": {"workload": cfg["name"], "points": cam.npix, "tiles": [[a1 - a0, b1 - b0] for a0, a1, b0, b1 in splits],
+                      "extra_screens": [list(c.size) for c in extras], "samples_per_ray": N, "splats": model.ns,
+                      "frames": len(times), "frequencies": len(nus), "step": "one forward cube (transfer + composition) of every point, frame and frequency"},
+           "launches_ms": {k: {"cube_wall_ms": w, "transfer_ms": tr} for k, (w, tr) in runs.items()},
+           "tuned": {"choice": choice, "autotune_s": tune_s, "transfer_ms": float(np.mean([tr for _, tr in steps])),
+                     "table": [{"chunk": c_, "patch": list(p_), "queue": q_, "sample_ms": m_} for c_, p_, q_, m_ in table]},
+           "mixed_error_vs_fp64_first_slice": err, "roofline": None, "cpu_baseline": None}
     print(json.dumps(out), flush=True)
 
 

@@ -34,6 +34,7 @@ typedef struct ks_ctx ks_ctx;
 typedef struct ks_cache ks_cache;
 typedef struct ks_model ks_model;
 typedef struct ks_binning ks_binning;
+typedef struct ks_graph ks_graph;
 
 /* Description of the last error on this thread (empty string if none). */
 const char* ks_last_error(void);
@@ -84,6 +85,49 @@ int ks_regenerate(ks_cache* cache, const double* alpha, const double* beta, int 
 #define KS_PRECISION_FP64 0
 #define KS_PRECISION_MIXED 1
 int ks_cache_precision(ks_cache* cache, int precision);
+/* The scalar of the Recurrence(M) march into stored samples (cache.jl:182-190, quadrature.jl:231-296), for the next
+ * ks_regenerate / ks_march. KS_PRECISION_FP64 (default): the plain FP64 march. KS_PRECISION_MIXED: its bulk in FP32
+ * (the Jacobi addition-theorem steps, the radius and polar angle, the integrand rates and Simpson increments) between
+ * FP64 closed-form anchors (Jacobi functions, t~ and phi every M samples), FP64 root cases, pixel constants, Mino
+ * times and running t~ / phi sums; the stored samples stay FP64. The reference marches in FP64 only and converts
+ * afterwards (cache.jl:221-232). Direct and Fused marchers always run FP64. */
+int ks_cache_march_precision(ks_cache* cache, int precision);
+
+/* Launch choices of the transfer over stored samples and of the element adjoints. The reference fixes
+ * them: one work item per pixel over all N samples, a workgroup of consecutive pixels (cache.jl:234-246
+ * splits a screen into even tiles of consecutive pixels). Here: chunk = samples per work item of the
+ * transfer (8:1024, default 32); queue = bits: KS_QUEUE_TRANSFER, the transfer as a persistent grid of
+ * the resident workgroups taking tasks of 128 work items from a work queue (the tasks of a repeated
+ * launch in descending order of the ticks each took the last time), else one workgroup per task;
+ * KS_QUEUE_ADJOINT, the element adjoints as a persistent grid taking (unit, splat) blocks from per-XCD
+ * queues, else one workgroup per block; patch_a x patch_b = the screen patch a wavefront's 64 work
+ * items cover (default 8 x 8) within each screen part. The results do not depend on the patch or the
+ * launch forms (bit for bit); the chunk length changes the order of the Stokes sums only. */
+#define KS_QUEUE_TRANSFER 1
+#define KS_QUEUE_ADJOINT 2
+int ks_cache_tuning(ks_cache* cache, int chunk, int queue, int patch_a, int patch_b);
+int ks_cache_get_tuning(const ks_cache* cache, int* chunk, int* queue, int* patch_a, int* patch_b);
+/* Screen parts of a concatenated camera (concatenate, camera.jl:151-164; tiles, cache.jl:234-246):
+ * the cache's points are nparts consecutive screens, part q an n_alpha[q] x n_beta[q] screen in screen
+ * order; the wavefront patches then stay inside one part. nparts = 0: the camera of ks_regenerate is
+ * one screen. The sum of the part sizes must be the cache's point count. */
+int ks_cache_set_parts(ks_cache* cache, int nparts, const int* n_alpha, const int* n_beta);
+/* The autotuner (no counterpart in the reference). Unless flags has KS_TUNE_KEEP_TRANSFER: times every launch
+ * choice of the transfer (chunk 16, 32, 64, 128 x patch 8x8, 16x4, 4x16 x the two transfer launch forms) on
+ * a hashed sample of 1 / sample_every of its tasks for (times, nus), with the dual sweep's records written
+ * when flags has KS_TUNE_RECORDS, and keeps the fastest. With KS_TUNE_ADJOINT: one forward pass with the
+ * records, then the reverse pass in each launch form of the element adjoints, and keeps the faster.
+ * table[5 q + {0..4}] = (chunk, patch_a, patch_b, queue bits, milliseconds) of candidate q (the element
+ * adjoints' rows with chunk 0; at most cap rows); *ncand = the candidates timed. Leaves no forward pass
+ * to reverse. */
+#define KS_TUNE_RECORDS 1
+#define KS_TUNE_ADJOINT 2
+#define KS_TUNE_KEEP_TRANSFER 4
+int ks_autotune(ks_cache* cache, const ks_model* model, const double* times, int nt, const double* nus, int nnu, double L,
+                int flags, int sample_every, double* table, int cap, int* ncand);
+/* The per-task wall-clock ticks (100 MHz) of the cache's last work-queue launch: *ntasks tasks, the
+ * first cap of them into cost (cost may be null). */
+int ks_queue_costs(const ks_cache* cache, int64_t* ntasks, uint32_t* cost, int cap);
 
 /* march!(marcher, cache, ...) - cache.jl:182-190: re-run the cache's Direct or Recurrence(M) marcher
  * into its sample storage on the current pixel constants (KS_ERR_STATE without stored samples). */
@@ -92,7 +136,7 @@ int ks_get_perm(const ks_cache* cache, int64_t* perm, int64_t counts[3]);
 /* cache.residual_t, cache.residual_ϕ (sorted order) - cache.jl:98-99, quadrature.jl:283-291. */
 int ks_get_residuals(const ks_cache* cache, double* resid_t, double* resid_phi);
 /* host(cache.consts) - Geodesics.jl:66-73 over PixelConstants (pixel_constants.jl:29-62):
- * ks_num_const_fields() doubles per pixel, field-major, sorted order (layout in DESIGN.md). */
+ * ks_num_const_fields() doubles per pixel, field-major, sorted order. */
 int ks_num_const_fields(void);
 int ks_copy_consts(const ks_cache* cache, double* soa);
 /* host(cache.samples) - GeodesicSamples (direct_march.jl:71-115): N x npix per field
@@ -208,6 +252,18 @@ int ks_vis_chi2_seed(ks_cache* cache, int64_t frame, double psize, const double*
 int ks_vis_seed(ks_cache* cache, int64_t frame, double psize, const double* u, const double* v, const double* taper,
                 const double* w, int nuv, int add);
 
+/* chi2_closures (visibilities.jl:141-199) of frame `frame` of the last ks_gradient_forward, on the device: the
+ * (tapered) model visibilities at the nuv (u, v) points, the closure phases of Stokes I on ntri triangles (3
+ * 1-based baseline indices each, negative = conjugated, the phase difference taken to the nearest multiple of
+ * 2 pi) and the log closure amplitudes ln(|V1||V2| / (|V3||V4|)) on nquad quadrangles (4 indices each, signs
+ * ignored) against the data and their sigmas; the χ² in chi2 and its adjoint seed (d χ² / d V through the
+ * adjoint transform) written into the frame's device seed, or added to it when add, for
+ * ks_gradient_reverse(dstokes = NULL). The reference differentiates this term with host Enzyme
+ * (timeresolved.jl:157-168); here no visibility leaves the device. */
+int ks_closure_chi2_seed(ks_cache* cache, int64_t frame, double psize, const double* u, const double* v, const double* taper,
+                         int nuv, int ntri, const int* triangles, const double* phases, const double* sigma_phase, int nquad,
+                         const int* quadrangles, const double* logamps, const double* sigma_logamp, int add, double* chi2);
+
 /* _frame_seed! without a Binning (Fit.jl:537-577; the terms of chi2/_chi2_frames, Fit.jl:66-93):
  * the χ² of frame `frame` of the last ks_gradient_forward against an observed Stokes image,
  * Σ ((I - d) / σ)² over the masked pixels, returned in chi2, and its seed 2 r / σ (0 outside the
@@ -234,18 +290,6 @@ int ks_binning_create(ks_ctx* ctx, int64_t npoints, const int64_t* pixel, int n_
 int ks_binning_destroy(ks_binning* binning);
 
 /* _frame_seed! through a Binning (Fit.jl:555-574): the χ² of the binned frame `frame` of the last
-/* chi2_closures (visibilities.jl:141-199) of frame `frame` of the last ks_gradient_forward, on the device: the
- * (tapered) model visibilities at the nuv (u, v) points, the closure phases of Stokes I on ntri triangles (3
- * 1-based baseline indices each, negative = conjugated, the phase difference taken to the nearest multiple of
- * 2 pi) and the log closure amplitudes ln(|V1||V2| / (|V3||V4|)) on nquad quadrangles (4 indices each, signs
- * ignored) against the data and their sigmas; the χ² in chi2 and its adjoint seed (d χ² / d V through the
- * adjoint transform) written into the frame's device seed, or added to it when add, for
- * ks_gradient_reverse(dstokes = NULL). The reference differentiates this term with host Enzyme
- * (timeresolved.jl:157-168); here no visibility leaves the device. */
-int ks_closure_chi2_seed(ks_cache* cache, int64_t frame, double psize, const double* u, const double* v, const double* taper,
-                         int nuv, int ntri, const int* triangles, const double* phases, const double* sigma_phase, int nquad,
-                         const int* quadrangles, const double* logamps, const double* sigma_logamp, int add, double* chi2);
-
  * ks_gradient_forward, Σ ((bin(I) - d) / σ)² over the masked pixels of the binned screen, returned in
  * chi2, and its seed, 2 r / σ / count shared by the points of each pixel (0 outside the mask),
  * written into the device seed of that frame. The binning's points must be the cache's pixels.
@@ -253,6 +297,31 @@ int ks_closure_chi2_seed(ks_cache* cache, int64_t frame, double psize, const dou
  * pixel when sigma_per_pixel; mask: one byte per binned pixel, NULL = all. */
 int ks_image_chi2_seed_binned(ks_cache* cache, int64_t frame, const ks_binning* binning, const double* data, const double* sigma,
                               int sigma_per_pixel, const uint8_t* mask, double* chi2);
+
+/* A fit iteration captured as one hipGraph (no counterpart in the reference, whose fit! loop
+ * (Fit.jl:620-659) launches every kernel from the host each iteration). Created on a regenerated cache
+ * with the loss's data resident on the device: loss = KS_LOSS_IMAGE, the image χ² of every (frame,
+ * frequency) slice f = it + nt inu (data 4 npix per slice in screen order, sigma 4 per slice or 4 npix
+ * per slice with sigma_per_pixel, mask npix per slice or NULL; Fit.jl:537-577), or KS_LOSS_VIS, the
+ * visibility χ² of every slice (psize, u, v, taper, nuv as ks_vis_chi2_seed; data 8 nuv, sigma 4 nuv
+ * per slice; Fit.jl:540-551, device_visibilities.jl:53-111).
+ * ks_fit_graph_step: the per-splat constants of params (nrows x ns of the model's kind, column-major;
+ * NULL: the model's current constants) made on the host into pinned staging, then one replay of the
+ * graph: their upload, the transfer with the records, the images, every slice's χ² and seed, the reverse
+ * pass, the copies back. chi2 = the sum over the slices; the gradient is added to dparams. The model is
+ * one thermal, power-law, kappa or knot population; its device constants are overwritten. The graph
+ * is captured again (after one eager iteration that sizes every buffer) on the first step, for another
+ * model (a changed splat count), and when the cache's buffers, generation, precision or launch choices
+ * changed. Afterwards the cache holds the iteration's forward pass (images, seeds). */
+#define KS_LOSS_IMAGE 0
+#define KS_LOSS_VIS 1
+int ks_fit_graph_create(ks_cache* cache, int nt, const double* times, int nnu, const double* nus, double L, int loss,
+                        const double* data, const double* sigma, int sigma_per_pixel, const uint8_t* mask, double psize,
+                        const double* u, const double* v, const double* taper, int nuv, ks_graph** graph);
+int ks_fit_graph_step(ks_graph* graph, const ks_model* model, const double* params, double* chi2, double* dparams);
+/* captures made and replays run */
+int ks_fit_graph_info(const ks_graph* graph, int64_t* captures, int64_t* replays);
+int ks_fit_graph_destroy(ks_graph* graph);
 
 #ifdef __cplusplus
 }

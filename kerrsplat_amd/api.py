@@ -91,8 +91,22 @@ def _load():
     lib.ks_closure_chi2_seed.argtypes = [_P, _i64, _d, _P, _P, _P, _i, _i, _P, _P, _P, _i, _P, _P, _P, _i, _P]
     lib.ks_march.argtypes = [_P]
     lib.ks_cache_precision.argtypes = [_P, _i]
+    lib.ks_cache_march_precision.argtypes = [_P, _i]
+    lib.ks_cache_tuning.argtypes = [_P, _i, _i, _i, _i]
+    lib.ks_cache_get_tuning.argtypes = [_P, _P, _P, _P, _P]
+    lib.ks_cache_set_parts.argtypes = [_P, _i, _P, _P]
+    lib.ks_autotune.argtypes = [_P, _P, _P, _i, _P, _i, _d, _i, _i, _P, _i, ctypes.POINTER(_i)]
+    lib.ks_queue_costs.argtypes = [_P, ctypes.POINTER(_i64), _P, _i]
+    lib.ks_fit_graph_create.argtypes = [_P, _i, _P, _i, _P, _d, _i, _P, _P, _i, _P, _d, _P, _P, _P, _i, ctypes.POINTER(_P)]
+    lib.ks_fit_graph_step.argtypes = [_P, _P, _P, _P, _P]
+    lib.ks_fit_graph_info.argtypes = [_P, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]
+    lib.ks_fit_graph_destroy.argtypes = [_P]
     _lib = lib
     return lib
+
+
+KS_QUEUE_TRANSFER, KS_QUEUE_ADJOINT = 1, 2
+KS_TUNE_RECORDS, KS_TUNE_ADJOINT, KS_TUNE_KEEP_TRANSFER = 1, 2, 4
 
 
 def _check(rc):
@@ -285,6 +299,35 @@ def concatenate(*parts):
     return Camera.points(al, be), Binning(np.concatenate(pixel), (offset, 1))
 
 
+def concatenate_screens(*cameras):
+    """One point camera from several screens (concatenate, camera.jl:151-164, without a binning): the points of
+    each camera in order, each camera kept as a screen part (its size) so that the device groups the work of
+    one screen patch by patch (set_parts). Returns the camera; camera.parts lists the part sizes."""
+    cam = Camera.points(np.concatenate([c.alphas for c in cameras]), np.concatenate([c.betas for c in cameras]))
+    cam.parts = [tuple(int(x) for x in c.size) for c in cameras]
+    return cam
+
+
+def screen_tiles(camera, splits):
+    """The screen of a grid camera cut into rectangular tiles: splits = [(a0, a1, b0, b1), ...] pixel index
+    ranges (half-open) that cover the screen once. Returns (cameras, index lists): each tile as a grid camera
+    over its own pixels (Krang's grid positions of the full screen) and the screen indices of its pixels in
+    the tile's screen order (the reference's tiles, cache.jl:234-246, cut runs of consecutive pixels instead)."""
+    na, nb = camera.size
+    A = camera.alphas.reshape(na, nb, order="F")
+    B = camera.betas.reshape(na, nb, order="F")
+    cams, idx, seen = [], [], np.zeros((na, nb), dtype=np.int64)
+    for a0, a1, b0, b1 in splits:
+        c = Camera.points(A[a0:a1, b0:b1].ravel(order="F"), B[a0:a1, b0:b1].ravel(order="F"), size=(a1 - a0, b1 - b0))
+        cams.append(c)
+        ia, ib = np.meshgrid(np.arange(a0, a1), np.arange(b0, b1), indexing="ij")
+        idx.append((ia + na * ib).ravel(order="F"))
+        seen[a0:a1, b0:b1] += 1
+    if not (seen == 1).all():
+        raise ValueError("the tiles must cover the screen exactly once")
+    return cams, idx
+
+
 class Direct:
     code, M = KS_MARCHER_DIRECT, 64
 
@@ -324,6 +367,7 @@ class GeodesicCache:
         self.perm_host = np.arange(1, camera.npix + 1)
         self.ranges = (0, 0, 0)
         self.precision = "fp64"
+        _set_parts(self, camera)
 
     def __del__(self):
         try:
@@ -355,6 +399,7 @@ def regenerate(cache, a, theta_o, camera=None, marcher=None):
         if camera.npix != cache.npix:
             raise DimensionMismatch(f"camera has {camera.npix} pixels, cache was built for {cache.npix}")
         cache.camera = camera
+        _set_parts(cache, camera)
     cam = cache.camera
     al, be = _f64(cam.alphas), _f64(cam.betas)
     lib = cache._lib
@@ -369,6 +414,139 @@ def regenerate(cache, a, theta_o, camera=None, marcher=None):
     cache.generation += 1
     cache.marcher = marcher
     return cache
+
+
+def _set_parts(cache, camera):
+    parts = getattr(camera, "parts", None) or []
+    na = np.array([p[0] for p in parts], dtype=np.int32)
+    nb = np.array([p[1] for p in parts], dtype=np.int32)
+    _check(cache._lib.ks_cache_set_parts(cache._h, _i(len(parts)), _ptr(na) if len(parts) else None, _ptr(nb) if len(parts) else None))
+
+
+def tuning(cache, chunk=None, queue=None, patch=None, adjoint_queue=None):
+    """The launch choices of the cache's transfer over stored samples and of its element adjoints: samples per work
+    item (chunk), the transfer's persistent work-queue form (queue), the element adjoints' persistent form
+    (adjoint_queue), the wavefront's screen patch (patch = (n_alpha, n_beta) points); arguments left at None keep
+    their value. Returns the current choices as a dict."""
+    lib = cache._lib
+    c, q, pa, pb = _i(), _i(), _i(), _i()
+    _check(lib.ks_cache_get_tuning(cache._h, ctypes.byref(c), ctypes.byref(q), ctypes.byref(pa), ctypes.byref(pb)))
+    if chunk is not None or queue is not None or patch is not None or adjoint_queue is not None:
+        nc = int(chunk) if chunk is not None else c.value
+        tq = bool(queue) if queue is not None else bool(q.value & KS_QUEUE_TRANSFER)
+        aq = bool(adjoint_queue) if adjoint_queue is not None else bool(q.value & KS_QUEUE_ADJOINT)
+        npa, npb = (int(patch[0]), int(patch[1])) if patch is not None else (pa.value, pb.value)
+        _check(lib.ks_cache_tuning(cache._h, _i(nc), _i((KS_QUEUE_TRANSFER if tq else 0) | (KS_QUEUE_ADJOINT if aq else 0)), _i(npa), _i(npb)))
+        _check(lib.ks_cache_get_tuning(cache._h, ctypes.byref(c), ctypes.byref(q), ctypes.byref(pa), ctypes.byref(pb)))
+    return {"chunk": c.value, "queue": bool(q.value & KS_QUEUE_TRANSFER), "patch": (pa.value, pb.value),
+            "adjoint_queue": bool(q.value & KS_QUEUE_ADJOINT)}
+
+
+def autotune(cache, model, times, nus, L, records=True, sample_every=16, adjoint=False, transfer=True):
+    """The autotuner: times every launch choice of the transfer (chunk 16/32/64/128 x patch 8x8/16x4/4x16 x queue
+    off/on) on a hashed sample of 1 / sample_every of the transfer's tasks for (times, nus) (records = the dual
+    sweep's records written, as in gradient_forward) and keeps the fastest on the cache (transfer=False keeps the
+    cache's transfer choices); with adjoint, also times one reverse pass in each launch form of the element adjoints
+    and keeps the faster. Returns (choice, table), table = [(chunk, (patch_a, patch_b), queue bits, ms), ...] (the
+    element adjoints' rows with chunk 0)."""
+    m = _as_model(model, "thermal")
+    tv, nv = _f64(np.atleast_1d(times)), _f64(np.atleast_1d(nus))
+    cap = 64
+    table = np.zeros(5 * cap)
+    n = _i()
+    flags = (KS_TUNE_RECORDS if records else 0) | (KS_TUNE_ADJOINT if adjoint else 0) | (0 if transfer else KS_TUNE_KEEP_TRANSFER)
+    _check(cache._lib.ks_autotune(cache._h, m._h, _ptr(tv), _i(len(tv)), _ptr(nv), _i(len(nv)), _d(L), _i(flags),
+                                  _i(int(sample_every)), _ptr(table), _i(cap), ctypes.byref(n)))
+    rows = [(int(r[0]), (int(r[1]), int(r[2])), int(r[3]), float(r[4])) for r in table[:5 * min(n.value, cap)].reshape(-1, 5)]
+    return tuning(cache), rows
+
+
+def queue_costs(cache):
+    """The per-task wall-clock ticks (100 MHz) of the cache's last work-queue launch, indexed by task."""
+    n = _i64()
+    _check(cache._lib.ks_queue_costs(cache._h, ctypes.byref(n), None, _i(0)))
+    out = np.zeros(n.value, dtype=np.uint32)
+    _check(cache._lib.ks_queue_costs(cache._h, ctypes.byref(n), _ptr(out), _i(len(out))))
+    return out
+
+
+class FitGraph:
+    """A fit iteration captured as one hipGraph (ks_fit_graph_*): the upload of the splat constants, the transfer with
+    the records over every (frame, frequency) slice, the images, every slice's chi2 and seed, the reverse pass.
+    FitGraph(cache, times, nus, L, image=(data, sigma[, mask])) with data (n_alpha, n_beta, nt, nnu, 4) in screen
+    order (or (npix, nt, nnu, 4)) and sigma 4 numbers or an array of data's shape, or vis=(psize, u, v, data, sigma[,
+    taper]) with data (nt, nnu, nuv, 4) complex and sigma (nt, nnu, nuv, 4). step(model, params=None) -> (chi2,
+    gradient): the model's constants remade from params (its shape; None: the model's current constants), one replay
+    (captured again for another model, e.g. after a hygiene pass changed the splat count)."""
+
+    def __init__(self, cache, times, nus, L, image=None, vis=None):
+        if (image is None) == (vis is None):
+            raise ValueError("FitGraph takes one loss: image=(data, sigma[, mask]) or vis=(psize, u, v, data, sigma[, taper])")
+        self._lib, self.cache = cache._lib, cache
+        tv, nv = _f64(np.atleast_1d(times)), _f64(np.atleast_1d(nus))
+        nt, nnu, n = len(tv), len(nv), cache.npix
+        self.frames = nt * nnu
+        h = _P()
+        if image is not None:
+            data, sigma = image[0], image[1]
+            mask = image[2] if len(image) > 2 else None
+            d = np.asarray(data, dtype=np.float64).reshape(n, nt, nnu, 4, order="F")
+            dd = np.ascontiguousarray(np.stack([d[:, it, inu, :] for inu in range(nnu) for it in range(nt)]))
+            sg = np.asarray(sigma, dtype=np.float64)
+            per_pixel = sg.size > 4
+            if per_pixel:
+                sg = sg.reshape(n, nt, nnu, 4, order="F")
+                sd = np.ascontiguousarray(np.stack([sg[:, it, inu, :] for inu in range(nnu) for it in range(nt)]))
+            else:
+                sd = np.ascontiguousarray(np.tile(sg.ravel(), self.frames))
+            md = None
+            if mask is not None:
+                mk = np.asarray(mask, dtype=bool).reshape(n, nt, nnu, order="F")
+                md = np.ascontiguousarray(np.stack([mk[:, it, inu] for inu in range(nnu) for it in range(nt)]).astype(np.uint8))
+            self._keep = (tv, nv, dd, sd, md)
+            _check(self._lib.ks_fit_graph_create(cache._h, _i(nt), _ptr(tv), _i(nnu), _ptr(nv), _d(L), _i(0), _ptr(dd), _ptr(sd),
+                                                 _i(int(per_pixel)), _ptr(md), _d(0.0), None, None, None, _i(0), ctypes.byref(h)))
+        else:
+            psize, u, v, data, sigma = vis[:5]
+            taper = vis[5] if len(vis) > 5 else None
+            uu, vv = _f64(u), _f64(v)
+            nuv = len(uu)
+            D = np.asarray(data).reshape(nt, nnu, nuv, 4)
+            S = np.asarray(sigma, dtype=np.float64).reshape(nt, nnu, nuv, 4)
+            dd = np.ascontiguousarray(np.stack([np.stack([D[it, inu].real, D[it, inu].imag], axis=-1) for inu in range(nnu) for it in range(nt)]))
+            sd = np.ascontiguousarray(np.stack([S[it, inu] for inu in range(nnu) for it in range(nt)]))
+            tp = _f64(taper) if taper is not None else None
+            self._keep = (tv, nv, uu, vv, dd, sd, tp)
+            _check(self._lib.ks_fit_graph_create(cache._h, _i(nt), _ptr(tv), _i(nnu), _ptr(nv), _d(L), _i(1), _ptr(dd), _ptr(sd), _i(0),
+                                                 None, _d(psize), _ptr(uu), _ptr(vv), _ptr(tp), _i(nuv), ctypes.byref(h)))
+        self._h = h
+
+    def step(self, model, params=None):
+        if model.kind not in ("thermal", "powerlaw", "kappa", "knot"):
+            raise ValueError("a fit graph takes one thermal, power-law, kappa or knot population")
+        P = None
+        if params is not None:
+            P = np.asfortranarray(params, dtype=np.float64)
+            if P.shape != model.grad_shape:
+                raise DimensionMismatch(f"params of shape {P.shape}, the model holds {model.grad_shape}")
+            model.params = P
+        chi = _d()
+        g = np.zeros(model.grad_shape, order="F")
+        _check(self._lib.ks_fit_graph_step(self._h, model._h, _ptr(P), ctypes.byref(chi), _ptr(g)))
+        return chi.value, g
+
+    def info(self):
+        c, r = _i64(), _i64()
+        _check(self._lib.ks_fit_graph_info(self._h, ctypes.byref(c), ctypes.byref(r)))
+        return {"captures": c.value, "replays": r.value}
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None) is not None:
+                self._lib.ks_fit_graph_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
 
 
 def unsort(cache, x):
@@ -636,6 +814,17 @@ def precision(cache, policy):
         raise ValueError(f"precision policy must be one of {sorted(PRECISIONS)}, got {policy!r}")
     _check(cache._lib.ks_cache_precision(cache._h, _i(PRECISIONS[policy])))
     cache.precision = policy
+    return cache
+
+
+def march_precision(cache, policy):
+    """The scalar of the Recurrence(M) march into stored samples for the next regenerate / march: "fp64" (default) or
+    "mixed" (the bulk of the per-sample work in FP32 between FP64 closed-form anchors; FP64 root cases, pixel
+    constants, Mino times, running t~ / phi sums and stored samples). Set in place; returns the cache."""
+    if policy not in PRECISIONS:
+        raise ValueError(f"precision policy must be one of {sorted(PRECISIONS)}, got {policy!r}")
+    _check(cache._lib.ks_cache_march_precision(cache._h, _i(PRECISIONS[policy])))
+    cache.march_precision = policy
     return cache
 
 

@@ -15,6 +15,7 @@ namespace ksd {
 // the double overloads stay visible beside the ones declared in this namespace
 using ::sqrt; using ::exp; using ::expm1; using ::log; using ::log1p; using ::sin; using ::cos; using ::sincos;
 using ::sinh; using ::cosh; using ::tanh; using ::cbrt; using ::acos; using ::fabs; using ::atan2; using ::pow; using ::frexp;
+using ::atan; using ::floor;
 
 // ---- f32 ------------------------------------------------------------------------------------------------
 struct f32 {
@@ -90,6 +91,8 @@ KS_DEV f32 cbrt(f32 x) { return f32(cbrtf(x.v)); }
 KS_DEV f32 acos(f32 x) { return f32(acosf(x.v)); }
 KS_DEV f32 fabs(f32 x) { return f32(fabsf(x.v)); }
 KS_DEV f32 atan2(f32 y, f32 x) { return f32(atan2f(y.v, x.v)); }
+KS_DEV f32 atan(f32 x) { return f32(atanf(x.v)); }
+KS_DEV f32 floor(f32 x) { return f32(floorf(x.v)); }
 KS_DEV f32 pow(f32 x, double y) { return f32(hw_pow(x.v, (float)y)); }
 KS_DEV f32 pow(f32 x, f32 y) { return f32(hw_pow(x.v, y.v)); }
 KS_DEV f32 pow(double x, f32 y) { return f32(hw_pow((float)x, y.v)); }
@@ -189,7 +192,7 @@ KSDD_T KS_DEV DN<N, S> acos(const DN<N, S>& x) {
     return chain(acos(x.v), s > 0.0 ? S(-1.0 / sqrt(s)) : S(0.0), x);
 }
 KSDD_T KS_DEV DN<N, S> fabs(const DN<N, S>& x) { return x.v < 0.0 ? -x : x; }
-// (the base-scalar calls are qualified: these names have no f32 overload here to keep ::atan etc. visible)
+// (the base-scalar calls are qualified)
 KSDD_T KS_DEV DN<N, S> atan(const DN<N, S>& x) { return chain(::atan(x.v), S(1.0 / (1.0 + x.v * x.v)), x); }
 KSDD_T KS_DEV DN<N, S> asin(const DN<N, S>& x) { return chain(::asin(x.v), S(1.0 / sqrt(1.0 - x.v * x.v)), x); }
 KSDD_T KS_DEV DN<N, S> tan(const DN<N, S>& x) { const S t = ::tan(x.v); return chain(t, S(1.0 + t * t), x); }

@@ -11,13 +11,13 @@ namespace ksd {
 // (T_out[last] = 0), the image nu-hat^3 T_in[0], and the adjoint in front of each chunk
 // w_in[m] = (P_0 ... P_{m-1})^T w_1 with w_1 = nu-hat^3 dstokes (dual_gradient.jl:162).
 // wT[(8 m + q) nitems + item] = w_in, wT[(8 m + 4 + q) nitems + item] = T_out; the image only when out is given.
-__global__ void ks_adj_combine(int64_t npix, int N, int nt, const double* __restrict__ nus, int nnu, const double* __restrict__ seg,
+__global__ void ks_adj_combine(int64_t npix, int N, int chunk, int nt, const double* __restrict__ nus, int nnu, const double* __restrict__ seg,
                                const double* __restrict__ dstokes, double* __restrict__ wT, double* __restrict__ out) {
     const int64_t nitems = npix * nt * nnu;
     const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (item >= nitems) return;
     const int inu = (int)((item / npix) / nt);
-    const int nseg = (N + CHUNK - 1) / CHUNK;
+    const int nseg = (N + chunk - 1) / chunk;
     const double nh = nus[inu] * (1.0 / NU0);
     const double s3 = nh * nh * nh;
     double T[4] = {0, 0, 0, 0};
@@ -92,14 +92,14 @@ KS_DEV void step_apply_s(const S j4[4], const S a4[4], const S r3[3], S D, const
 // records, which stay intact so that the reverse pass can run again on the same forward pass) and
 // w <- O^T w (polarized_dual_kernel!'s ray loop, dual_gradient.jl:151-178).
 template <class S>
-__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(Waves<S>::adjstep))) ks_adj_segment(int N, int64_t nitems, int nch, const double* __restrict__ wT, const S* __restrict__ rec,
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(Waves<S>::adjstep))) ks_adj_segment(int N, int chunk, int64_t nitems, int nch, const double* __restrict__ wT, const S* __restrict__ rec,
                                                       S* __restrict__ bar, const uint8_t* __restrict__ aflg, S* __restrict__ tails, Samples smp,
                                                       int64_t npix) {
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= nitems * nch) return;
     const int64_t item = slot_item(smp, tid % nitems, npix);     // wavefronts over compact patches of sky
     const int m = (int)(tid / nitems);
-    const int kbeg = m * CHUNK + 1, kend = kbeg + CHUNK - 1 < N ? kbeg + CHUNK - 1 : N;
+    const int kbeg = m * chunk + 1, kend = kbeg + chunk - 1 < N ? kbeg + chunk - 1 : N;
     const int64_t st = (int64_t)N * nitems;
     S T[4], w[4];
     for (int q = 0; q < 4; ++q) { w[q] = S(wT[(8 * (int64_t)m + q) * nitems + item]); T[q] = S(wT[(8 * (int64_t)m + 4 + q) * nitems + item]); }
@@ -175,13 +175,12 @@ __global__ void __launch_bounds__(64) ks_unit_bounds(int64_t npix, int N, Sample
 // POP_KNOTS (centres that follow knots); a launch of one population carries only its branch, whose registers would
 // otherwise cost it occupancy; mixed populations take a runtime branch on the splat's kind.
 template <class S, int POPS>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(Waves<S>::adjelem))) ks_adj_splats(double a, double thetao, const double* __restrict__ soa, int64_t npix, int N, Samples smp,
-                                                    const SplatCT<S>* __restrict__ sp, const SplatAdjT<S>* __restrict__ sadj, int ns,
-                                                    const S* __restrict__ knots, const S* __restrict__ ktimes,
-                                                    const double* __restrict__ times, int nt, const double* __restrict__ nus, int64_t nitems,
-                                                    const S* __restrict__ rec, const S* __restrict__ bar, const uint8_t* __restrict__ aflg,
-                                                    int64_t unit0, int64_t nb,
-                                                    const double* __restrict__ ub, double* __restrict__ part) {
+__device__ __forceinline__ void adj_splats_block(int64_t blk, double a, double thetao, const double* __restrict__ soa, int64_t npix, int N,
+                                                 Samples smp, const SplatCT<S>* __restrict__ sp, const SplatAdjT<S>* __restrict__ sadj,
+                                                 int ns, const S* __restrict__ knots, const S* __restrict__ ktimes,
+                                                 const double* __restrict__ times, int nt, const double* __restrict__ nus, int64_t nitems,
+                                                 const S* __restrict__ rec, const S* __restrict__ bar, const uint8_t* __restrict__ aflg,
+                                                 int64_t unit0, int64_t nb, const double* __restrict__ ub, double* __restrict__ part) {
     constexpr int WALK = 8;                 // samples tested per group, their loads issued together
     constexpr int QCAP = 1024;              // ring capacity: < 64 pending + 64 per sample of a group
     static_assert(QCAP >= 64 * (WALK + 1) && (QCAP & (QCAP - 1)) == 0, "the ring must hold a group's pushes");
@@ -195,7 +194,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(Waves<S
     int i;
     int64_t unit, local;
     {
-        const int64_t b = blockIdx.x, q = b / 8, x = b % 8;
+        const int64_t b = blk, q = b / 8, x = b % 8;
         i = (int)(q % ns);
         local = (q / ns) * 8 + x;
         if (local >= nb) return;          // the padding of the last group of 8 units
@@ -205,7 +204,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(Waves<S
     {
         const double* u = ub + UB_FIELDS * unit;
         if (!(u[6] > 0.0) || !splat_may_reach(sp[i], u[0], u[1], u[2], u[3], u[4], u[5], knots, ktimes)) {
-            if (threadIdx.x < ADJ_ROWS) part[(local * ns + i) * ADJ_ROWS + threadIdx.x] = 0.0;
+            // every lane stores (lanes past the rows repeat the last row's zero): no lane-dependent branch
+            part[(local * ns + i) * ADJ_ROWS + min((int)threadIdx.x, ADJ_ROWS - 1)] = 0.0;
             return;
         }
     }
@@ -323,9 +323,39 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(Waves<S
         for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
         gsum[r] = v;
     }
-    if (threadIdx.x == 0) {
-        double* o = part + (local * ns + i) * ADJ_ROWS;
-        for (int r = 0; r < ADJ_ROWS; ++r) o[r] = gsum[r];
+    // lane r < ADJ_ROWS stores row r (the lanes past the rows repeat the last row): selects, no lane-dependent branch
+    double mine = gsum[ADJ_ROWS - 1];
+    for (int r = 0; r < ADJ_ROWS - 1; ++r) mine = lane == r ? gsum[r] : mine;
+    part[(local * ns + i) * ADJ_ROWS + min(lane, ADJ_ROWS - 1)] = mine;
+}
+
+// The launch forms of the element adjoints: one workgroup (one wavefront) per block b of the XCD-aware numbering
+// (heads == nullptr), or a persistent grid of the resident workgroups (a multiple of 8, so workgroup w runs on XCD
+// w % 8) in which each workgroup takes tickets t from its XCD's queue heads[w % 8] and runs block b = 8 t + w % 8
+// until the blocks (nblocks) are taken: the blocks of a unit stay on one XCD as in the one-per-block form, and the
+// units whose splat is culled cost a ticket instead of a dispatch. The ticket is taken by all 64 lanes (lane 0 adds
+// one) and made scalar, so the loop's exit and back edge are uniform branches.
+template <class S, int POPS>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(Waves<S>::adjelem)))
+ks_adj_splats(double a, double thetao, const double* __restrict__ soa, int64_t npix, int N, Samples smp, const SplatCT<S>* __restrict__ sp,
+              const SplatAdjT<S>* __restrict__ sadj, int ns, const S* __restrict__ knots, const S* __restrict__ ktimes,
+              const double* __restrict__ times, int nt, const double* __restrict__ nus, int64_t nitems, const S* __restrict__ rec,
+              const S* __restrict__ bar, const uint8_t* __restrict__ aflg, int64_t unit0, int64_t nb, const double* __restrict__ ub,
+              double* __restrict__ part, int64_t nblocks, unsigned* __restrict__ heads) {
+    const int x = (int)(blockIdx.x % 8);
+    int64_t next = blockIdx.x;
+    for (;;) {
+        int64_t blk;
+        if (heads) {
+            const unsigned t = __builtin_amdgcn_readfirstlane(atomicAdd(heads + x, threadIdx.x == 0 ? 1u : 0u));
+            blk = 8 * (int64_t)t + x;
+        } else {
+            blk = next;
+            next = nblocks;
+        }
+        if (blk >= nblocks) break;
+        adj_splats_block<S, POPS>(blk, a, thetao, soa, npix, N, smp, sp, sadj, ns, knots, ktimes, times, nt, nus, nitems, rec, bar, aflg,
+                                  unit0, nb, ub, part);
     }
 }
 
@@ -344,33 +374,55 @@ __global__ void __launch_bounds__(64) ks_adj_reduce(const double* __restrict__ p
 }
 
 // ---- launchers ----
-hipError_t launch_adj_combine(hipStream_t s, int64_t npix, int N, int nt, const double* nus, int nnu, const double* seg,
+hipError_t launch_adj_combine(hipStream_t s, int64_t npix, int N, int chunk, int nt, const double* nus, int nnu, const double* seg,
                               const double* dstokes, double* wT, double* out) {
     const int64_t items = npix * nt * nnu;
-    hipLaunchKernelGGL(ks_adj_combine, dim3(grid_for(items, 128)), dim3(128), 0, s, npix, N, nt, nus, nnu, seg, dstokes, wT, out);
+    hipLaunchKernelGGL(ks_adj_combine, dim3(grid_for(items, 128)), dim3(128), 0, s, npix, N, chunk, nt, nus, nnu, seg, dstokes, wT, out);
     return hipGetLastError();
 }
-hipError_t launch_adj_segment(hipStream_t s, int N, int64_t nitems, const double* wT, const double* rec, double* bar, const uint8_t* aflg,
-                              double* tails, Samples smp, int64_t npix) {
-    const int nch = (N + CHUNK - 1) / CHUNK;
-    hipLaunchKernelGGL(ks_adj_segment<double>, dim3(grid_for(nitems * nch, 128)), dim3(128), 0, s, N, nitems, nch, wT, rec, bar, aflg, tails, smp,
-                       npix);
+hipError_t launch_adj_segment(hipStream_t s, int N, int chunk, int64_t nitems, const double* wT, const double* rec, double* bar,
+                              const uint8_t* aflg, double* tails, Samples smp, int64_t npix) {
+    const int nch = (N + chunk - 1) / chunk;
+    hipLaunchKernelGGL(ks_adj_segment<double>, dim3(grid_for(nitems * nch, 128)), dim3(128), 0, s, N, chunk, nitems, nch, wT, rec, bar, aflg,
+                       tails, smp, npix);
     return hipGetLastError();
 }
-hipError_t launch_adj_segment_f32(hipStream_t s, int N, int64_t nitems, const double* wT, const f32* rec, f32* bar, const uint8_t* aflg,
-                                  f32* tails, Samples smp, int64_t npix) {
-    const int nch = (N + CHUNK - 1) / CHUNK;
-    hipLaunchKernelGGL(ks_adj_segment<f32>, dim3(grid_for(nitems * nch, 128)), dim3(128), 0, s, N, nitems, nch, wT, rec, bar, aflg, tails, smp, npix);
+hipError_t launch_adj_segment_f32(hipStream_t s, int N, int chunk, int64_t nitems, const double* wT, const f32* rec, f32* bar,
+                                  const uint8_t* aflg, f32* tails, Samples smp, int64_t npix) {
+    const int nch = (N + chunk - 1) / chunk;
+    hipLaunchKernelGGL(ks_adj_segment<f32>, dim3(grid_for(nitems * nch, 128)), dim3(128), 0, s, N, chunk, nitems, nch, wT, rec, bar, aflg,
+                       tails, smp, npix);
     return hipGetLastError();
 }
 template <class S>
 static hipError_t adj_splats_k(hipStream_t s, int pops, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
                                const SplatCT<S>* sp, const SplatAdjT<S>* sadj, int ns, const S* knots, const S* ktimes,
                                const double* times, int nt, const double* nus, int64_t nitems, const S* rec, const S* bar,
-                               const uint8_t* aflg, int64_t unit0, int64_t nb, const double* ub, double* part) {
-    const dim3 grid((unsigned)((nb + 7) / 8 * 8 * ns)), block(64);
+                               const uint8_t* aflg, int64_t unit0, int64_t nb, const double* ub, double* part, int* heads) {
+    const int64_t nblocks = (nb + 7) / 8 * 8 * ns;
+    dim3 grid((unsigned)nblocks), block(64);
+    unsigned* hq = nullptr;
+    if (heads) {
+        // the persistent grid: the resident workgroups of the launch's instantiation, a multiple of 8
+        int groups = 0;
+#define KS_ADJ_GROUPS(P) groups = resident_groups(ks_adj_splats<S, P>, 64)
+        switch (pops) {
+            case POP_THERMAL: KS_ADJ_GROUPS(POP_THERMAL); break;
+            case POP_THERMAL | POP_KNOTS: KS_ADJ_GROUPS(POP_THERMAL | POP_KNOTS); break;
+            case POP_POWERLAW: KS_ADJ_GROUPS(POP_POWERLAW); break;
+            case POP_POWERLAW | POP_KNOTS: KS_ADJ_GROUPS(POP_POWERLAW | POP_KNOTS); break;
+            case POP_KAPPA: KS_ADJ_GROUPS(POP_KAPPA); break;
+            case POP_KAPPA | POP_KNOTS: KS_ADJ_GROUPS(POP_KAPPA | POP_KNOTS); break;
+            default: KS_ADJ_GROUPS(POP_FLUIDS | POP_KNOTS); break;
+        }
+#undef KS_ADJ_GROUPS
+        if (groups < 8) return hipErrorInvalidValue;
+        grid = dim3((unsigned)std::min<int64_t>(groups / 8 * 8, nblocks));
+        if (hipError_t e = launch_fill_i32(s, heads, 8, 0)) return e;
+        hq = (unsigned*)heads;
+    }
 #define KS_ADJ_LAUNCH(P) hipLaunchKernelGGL((ks_adj_splats<S, P>), grid, block, 0, s, a, thetao, soa, npix, N, smp, sp, sadj, ns, knots, ktimes, \
-                                            times, nt, nus, nitems, rec, bar, aflg, unit0, nb, ub, part)
+                                            times, nt, nus, nitems, rec, bar, aflg, unit0, nb, ub, part, nblocks, hq)
     switch (pops) {
         case POP_THERMAL: KS_ADJ_LAUNCH(POP_THERMAL); break;
         case POP_THERMAL | POP_KNOTS: KS_ADJ_LAUNCH(POP_THERMAL | POP_KNOTS); break;
@@ -386,16 +438,16 @@ static hipError_t adj_splats_k(hipStream_t s, int pops, double a, double thetao,
 hipError_t launch_adj_splats(hipStream_t s, int pops, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
                              const SplatC* sp, const SplatAdj* sadj, int ns, const double* knots, const double* ktimes,
                              const double* times, int nt, const double* nus, int64_t nitems, const double* rec, const double* bar,
-                             const uint8_t* aflg, int64_t unit0, int64_t nb, const double* ub, double* part) {
+                             const uint8_t* aflg, int64_t unit0, int64_t nb, const double* ub, double* part, int* heads) {
     return adj_splats_k<double>(s, pops, a, thetao, soa, npix, N, smp, sp, sadj, ns, knots, ktimes, times, nt, nus, nitems, rec, bar, aflg, unit0,
-                                nb, ub, part);
+                                nb, ub, part, heads);
 }
 hipError_t launch_adj_splats_f32(hipStream_t s, int pops, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
                                  const SplatCf* sp, const SplatAdjf* sadj, int ns, const f32* knots, const f32* ktimes,
                                  const double* times, int nt, const double* nus, int64_t nitems, const f32* rec, const f32* bar,
-                                 const uint8_t* aflg, int64_t unit0, int64_t nb, const double* ub, double* part) {
+                                 const uint8_t* aflg, int64_t unit0, int64_t nb, const double* ub, double* part, int* heads) {
     return adj_splats_k<f32>(s, pops, a, thetao, soa, npix, N, smp, sp, sadj, ns, knots, ktimes, times, nt, nus, nitems, rec, bar, aflg, unit0, nb,
-                             ub, part);
+                             ub, part, heads);
 }
 hipError_t launch_unit_bounds(hipStream_t s, int64_t npix, int N, Samples smp, const double* times, int nt, int64_t nitems,
                               const uint8_t* aflg, double* ub) {

@@ -65,10 +65,14 @@ struct StoreF {
         S.x[o] = x; S.y[o] = y; S.z[o] = z;
     }
 };
-// Work items (segment m, pixel j), pixel fastest. Recurrence: the pixel's anchor segments (march_range);
-// Direct: the closed forms at every sample of chunk m (SEG_MIN samples). Anchor residuals per segment
-// res[2 (m npix + j) + {0, 1}] (zero for Direct and past the pixel's last segment).
-template <int MARCHER>
+// Work items (segment m, pixel j), pixel fastest. Recurrence: the pixel's anchor segments (march_range, its bulk
+// in the scalar B: double, or f32 between FP64 anchors); Direct: the closed forms at every sample of chunk m (SEG_MIN
+// samples). Anchor residuals per segment res[2 (m npix + j) + {0, 1}] (zero for Direct and past the pixel's last
+// segment).
+constexpr double FP32_MARCH_RMIN = 1.2;      // x r+
+constexpr double FP32_MARCH_RMAX = 40.0;     // M
+constexpr double FP32_MARCH_KGAP = 1e-3;     // 1 - k, the radial modulus' distance from the critical curve
+template <int MARCHER, class B>
 __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(KS_WAVES_MARCH))) ks_march_store(double a, double thetao, const double* __restrict__ soa, int64_t npix, int N, int M,
                                                       int nseg_max, Samples smp, double* __restrict__ res) {
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -87,8 +91,22 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(KS_WAV
             const VisC vis = vis_constants(p, a, thetao);
             const double dtau = p.tau_total / (double)(N + 1);
             for (int k = kbeg; k <= kend; ++k) f(k, direct_sample(p, S, vis, k * dtau), dtau);
+        } else if constexpr (std::is_same<B, double>::value) {
+            march_range<double>(p, S, thetao, N, M, kbeg, kend, f, r1, r2);
         } else {
-            march_range(p, S, thetao, N, M, kbeg, kend, f, r1, r2);
+            // the FP32 bulk where it keeps its digits: the segment's radii (closed forms at its ends, and the radial
+            // turning point when it lies inside) within FP32_MARCH_RMIN r+ .. FP32_MARCH_RMAX, off the critical curve;
+            // FP64 elsewhere (the Moebius form of r and the desingularised rates cancel at large r, the logarithmic
+            // pieces near the horizon)
+            const double dtau = p.tau_total / (double)(N + 1);
+            const double ta = kbeg * dtau, tb = kend * dtau;
+            const JS ja = radial_state(p, ta), jb = radial_state(p, tb);
+            double rlo = dmin(radius_jac(p, ja.sn, ja.cn), radius_jac(p, jb.sn, jb.cn));
+            const double rhi = dmax(radius_jac(p, ja.sn, ja.cn), radius_jac(p, jb.sn, jb.cn));
+            if (p.rcase == 2 && ta < p.fo && p.fo < tb) rlo = dmin(rlo, p.r4);
+            const bool fp32 = rlo > FP32_MARCH_RMIN * S.rp && rhi < FP32_MARCH_RMAX && 1.0 - p.k > FP32_MARCH_KGAP && tb <= p.tau_h;
+            if (fp32) march_range<B>(p, S, thetao, N, M, kbeg, kend, f, r1, r2);
+            else march_range<double>(p, S, thetao, N, M, kbeg, kend, f, r1, r2);
         }
     }
     res[2 * (m * npix + j)] = r1;
@@ -158,9 +176,11 @@ hipError_t launch_pixel_constants(hipStream_t s, double a, double thetao, const 
     return hipGetLastError();
 }
 hipError_t launch_march_store(hipStream_t s, double a, double thetao, const double* soa, int64_t npix, int N, int M, int marcher, int nseg_max,
-                              Samples smp, double* res) {
-    if (marcher == 0) hipLaunchKernelGGL(ks_march_store<0>, dim3(grid_for(npix * nseg_max, 128)), dim3(128), 0, s, a, thetao, soa, npix, N, M, nseg_max, smp, res);
-    else hipLaunchKernelGGL(ks_march_store<1>, dim3(grid_for(npix * nseg_max, 128)), dim3(128), 0, s, a, thetao, soa, npix, N, M, nseg_max, smp, res);
+                              Samples smp, double* res, int fp32) {
+    const dim3 grid(grid_for(npix * nseg_max, 128)), block(128);
+    if (marcher == 0) hipLaunchKernelGGL((ks_march_store<0, double>), grid, block, 0, s, a, thetao, soa, npix, N, M, nseg_max, smp, res);
+    else if (fp32) hipLaunchKernelGGL((ks_march_store<1, f32>), grid, block, 0, s, a, thetao, soa, npix, N, M, nseg_max, smp, res);
+    else hipLaunchKernelGGL((ks_march_store<1, double>), grid, block, 0, s, a, thetao, soa, npix, N, M, nseg_max, smp, res);
     return hipGetLastError();
 }
 hipError_t launch_res_reduce(hipStream_t s, int64_t npix, int nseg_max, const double* res, double* rt, double* rp) {

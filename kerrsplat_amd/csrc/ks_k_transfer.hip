@@ -1,48 +1,63 @@
 // Transfer kernels: the polarized transport of every (pixel, time, frequency) item as per-segment
-// 4x4 transfer maps, from stored samples (CHUNK samples per work item) or fused with the march (one
+// 4x4 transfer maps, from stored samples (a chunk of samples per work item) or fused with the march (one
 // anchor segment per work item), their composition along each ray, and thin (Stokes I) images.
 // Reference: transport.jl:99-316, polarized.jl:41-198, Splats.jl:58-147, populations.jl:13-104,
 // trajectories.jl:80-128, step.jl:342-398.
 #include "ks_kernels.hpp"
 #include "ks_dsample.hpp"
+#include <rocprim/device/device_radix_sort.hpp>
 
 namespace ksd {
 
 // ---- polarized transport from stored samples (transport.jl:263-290) ----------------------------------------
-// Work items (chunk m, item), item = j + npix (it + nt inu) fastest: samples m CHUNK + 1 .. of the
+// Work items (chunk m, item), item = j + npix (it + nt inu) fastest: samples m chunk + 1 .. of the
 // item's ray, composed into the chunk's map seg[(20 m + f) nitems + item]. With rec != nullptr the
 // adjoint records (RecField) of every step-taking sample are written, and aflg[(k - 1) nitems + item]
-// = 1 when the sample takes a step.
-
+// = 1 when the sample takes a step. A task is one workgroup's 128 consecutive slots (slot = item slot +
+// nitems m); the item slots run over the cache's wavefront patches (slot_item).
+struct TransferArgs {
+    double a, thetao;
+    const double* soa;
+    int64_t npix;
+    int N, chunk, nch;
+    Samples smp;
+    int ns, nknots;
+    const double* times; int nt;
+    const double* nus; int nnu;
+    double L;
+    const int* kstop;
+    double* seg;
+    uint8_t* aflg;
+};
+KS_DEV int64_t npix_items(const TransferArgs& A) { return A.npix * A.nt * A.nnu; }
 template <int KIND, class S, bool CULL>
-__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(Waves<S>::transfer))) ks_chunk_transfer(double a, double thetao, const double* __restrict__ soa, int64_t npix, int N,
-                                                         Samples smp, const SplatCT<S>* __restrict__ sp, int ns,
-                                                         const S* __restrict__ knots, const S* __restrict__ ktimes, int nknots,
-                                                         const double* __restrict__ times, int nt, const double* __restrict__ nus, int nnu,
-                                                         double L, const int* __restrict__ kstop, int nch,
-                                                         double* __restrict__ seg, S* __restrict__ rec, uint8_t* __restrict__ aflg) {
-    const int64_t nitems = npix * nt * nnu;
-    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = tid < nitems * nch;       // the tail's lanes take slot 0 and write nothing
+__device__ __forceinline__ void transfer_task(const TransferArgs& A, const SplatCT<S>* __restrict__ sp, const S* __restrict__ knots,
+                                    const S* __restrict__ ktimes, S* __restrict__ rec, int64_t task, uint16_t* s_cand) {
+    const int64_t npix = A.npix;
+    const int N = A.N;
+    const int64_t nitems = npix * A.nt * A.nnu;
+    const int64_t tid = task * 128 + threadIdx.x;
+    const bool live = tid < nitems * A.nch;     // the tail's lanes take slot 0 and write nothing
     const int64_t slot = live ? tid : 0;
+    const Samples& smp = A.smp;
     const int64_t item = slot_item(smp, slot % nitems, npix);     // wavefronts over compact patches of sky
     const int m = (int)(slot / nitems);
     const int64_t j = item % npix;
     const int64_t fr = item / npix;
-    const int it = (int)(fr % nt), inu = (int)(fr / nt);
+    const int it = (int)(fr % A.nt), inu = (int)(fr / A.nt);
+    const double a = A.a;
     const Spin Sp = make_spin(a);
-    const double dtau = soa[(int64_t)PF_TAUTOT * npix + j] / (double)(N + 1);
-    const TransCtxT<S> c = make_ctx<S>(a, Sp.rp, Sp.rm, sin(thetao), soa[(int64_t)PF_ALPHA * npix + j], soa[(int64_t)PF_BETA * npix + j],
-                                       soa[(int64_t)PF_ETA * npix + j], soa[(int64_t)PF_LAM * npix + j], sp, ns, knots, ktimes, nknots,
-                                       times[it], nus[inu], L);
-    const int ks = kstop ? kstop[j] : N + 1;
+    const double dtau = A.soa[(int64_t)PF_TAUTOT * npix + j] / (double)(N + 1);
+    const TransCtxT<S> c = make_ctx<S>(a, Sp.rp, Sp.rm, sin(A.thetao), A.soa[(int64_t)PF_ALPHA * npix + j], A.soa[(int64_t)PF_BETA * npix + j],
+                                       A.soa[(int64_t)PF_ETA * npix + j], A.soa[(int64_t)PF_LAM * npix + j], sp, A.ns, knots, ktimes, A.nknots,
+                                       A.times[it], A.nus[inu], A.L);
+    const int ks = A.kstop ? A.kstop[j] : N + 1;
     const double hor = Sp.rp * (1.0 + 1e-3);
-    const int kbeg = m * CHUNK + 1, kend = kbeg + CHUNK - 1 < N ? kbeg + CHUNK - 1 : N;
+    const int kbeg = m * A.chunk + 1, kend = kbeg + A.chunk - 1 < N ? kbeg + A.chunk - 1 : N;
     const int64_t st = (int64_t)N * nitems;
     // the per-tile cull: the splats that can reach the wavefront's samples of this chunk, in ascending
     // order, as a candidate list in LDS (the rest add exact zeros to every sample of the tile)
     constexpr bool cull = CULL;     // the launcher picks CULL for CULL_MIN_NS <= ns <= CAND_MAX
-    __shared__ uint16_t s_cand[2][cull ? CAND_MAX : 1];
     const int wv = (int)(threadIdx.x / 64), lane = (int)(threadIdx.x % 64);
     int ncand = 0;
     if constexpr (cull) {
@@ -57,11 +72,11 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(Waves<
             }
         double px = 0.0, py = 0.0, pz = 0.0, R = 0.0, tmin = 0.0, tmax = 0.0;
         const bool any = wave_region(lo, hi, px, py, pz, R, tmin, tmax);
-        for (int i0 = 0; i0 < ns; i0 += 64) {
+        for (int i0 = 0; i0 < A.ns; i0 += 64) {
             const int i = i0 + lane;
-            const bool hit = any && i < ns && splat_may_reach(sp[i], px, py, pz, R, tmin, tmax, knots, ktimes);
+            const bool hit = any && i < A.ns && splat_may_reach(sp[i], px, py, pz, R, tmin, tmax, knots, ktimes);
             const uint64_t mk = __ballot(hit);
-            if (hit) s_cand[wv][ncand + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk, 0u))] = (uint16_t)i;
+            if (hit) s_cand[wv * CAND_MAX + ncand + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk, 0u))] = (uint16_t)i;
             ncand += __popcll(mk);
         }
     }
@@ -79,8 +94,8 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(Waves<
             S j4[4], a4[4], r3[3];
             GeoT<S> G;
             if (sample_coefficients<KIND, S>(c, smp.t[o], r, th, (fl & 2) != 0, (fl & 4) != 0, S(smp.x[o]), S(smp.y[o]), S(smp.z[o]),
-                                             j4, a4, r3, cull ? s_cand[wv] : nullptr, ncand, rec ? &G : nullptr)) {
-                const S D = S(path_weight(L, nus[inu], a, r, th, dtau));
+                                             j4, a4, r3, cull ? s_cand + wv * CAND_MAX : nullptr, ncand, rec ? &G : nullptr)) {
+                const S D = S(path_weight(A.L, A.nus[inu], a, r, th, dtau));
                 map_step(sm, j4, a4, r3, D);
                 step = true;
                 if (rec) {
@@ -92,9 +107,54 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(Waves<
                 }
             }
         }
-        if (rec) aflg[(int64_t)(k - 1) * nitems + item] = step ? 1 : 0;
+        if (rec) A.aflg[(int64_t)(k - 1) * nitems + item] = step ? 1 : 0;
     }
-    sm.store(seg, m, nitems, item);
+    sm.store(A.seg, m, nitems, item);
+}
+
+// The launch forms share one kernel and one copy of the task body (so both give the same numbers bit for bit):
+//  * one workgroup per task (head == nullptr): workgroup b runs task q = b, or tasks[b] when a task list is given
+//    (the autotuner's sample of the work), ntasks = the grid;
+//  * the persistent work queue (head != nullptr): a grid sized to the resident workgroups, each taking the next
+//    ticket q from the queue head until ntasks are taken; task q is tasks[q] (a cost-descending order from an
+//    earlier launch's per-task times, or a sample of the work) or q itself (ntasks < 2^31).
+// With cost != nullptr the wall-clock ticks each task took go to cost[task] (zeroed by the launcher). The loop holds
+// no lane-dependent branch outside the task: all 64 lanes of wavefront 0 execute the pop (lane 0 adds one, the others
+// zero), the first lane's ticket is shared through LDS, and every exit and back edge of the loop is a
+// wavefront-uniform (scalar) branch, so all lanes of the workgroup iterate together and each pass through the
+// barriers is a pass through the pop.
+template <int KIND, class S, bool CULL>
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(Waves<S>::transfer)))
+ks_chunk_transfer(TransferArgs A, const SplatCT<S>* __restrict__ sp, const S* __restrict__ knots, const S* __restrict__ ktimes,
+                  S* __restrict__ rec, const int* __restrict__ tasks, int64_t ntasks, unsigned* __restrict__ head,
+                  uint32_t* __restrict__ cost) {
+    __shared__ uint16_t s_cand[2][CULL ? CAND_MAX : 1];
+    __shared__ unsigned s_q;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
+    int64_t next = blockIdx.x;
+    for (;;) {
+        int64_t q;
+        if (head) {
+            if (wave == 0) {
+                const unsigned ticket = atomicAdd(head, threadIdx.x == 0 ? 1u : 0u);
+                s_q = __builtin_amdgcn_readfirstlane(ticket);
+            }
+            __syncthreads();
+            q = (int64_t)__builtin_amdgcn_readfirstlane(s_q);
+            __syncthreads();             // every wavefront has read s_q before the next pop overwrites it
+        } else {
+            q = next;                    // one task, then the loop ends
+            next = ntasks;
+        }
+        if (q >= ntasks) break;
+        const int64_t task = tasks ? (int64_t)tasks[q] : q;
+        const uint64_t t0 = cost ? wall_clock64() : 0;
+        transfer_task<KIND, S, CULL>(A, sp, knots, ktimes, rec, task, &s_cand[0][0]);
+        if (cost) {
+            __syncthreads();
+            atomicMax(cost + task, (uint32_t)min<uint64_t>(wall_clock64() - t0, 0xffffffffull));
+        }
+    }
 }
 
 // ---- the fused march + transport (caches without stored samples) --------------------------------------------
@@ -235,45 +295,86 @@ __global__ void ks_seg_combine(const double* __restrict__ soa, int64_t npix, int
 }
 
 // ---- launchers ----
+__global__ void ks_fill_i32(int* __restrict__ p, int64_t n, int v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+hipError_t launch_fill_i32(hipStream_t s, int* p, int64_t n, int v) {
+    hipLaunchKernelGGL(ks_fill_i32, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+template <int KIND, class S, bool CULL>
+static hipError_t chunk_transfer_c(hipStream_t s, const TransferArgs& A, const SplatCT<S>* sp, const S* knots, const S* ktimes, S* rec,
+                                   const TransferLaunch& tl) {
+    const int64_t all = (npix_items(A) * A.nch + 127) / 128;
+    const int64_t ntasks = tl.tasks ? tl.ntasks : all;
+    if (ntasks <= 0) return hipSuccess;
+    if (ntasks >= ((int64_t)1 << 31)) return hipErrorInvalidValue;
+    if (!tl.queue) {
+        hipLaunchKernelGGL((ks_chunk_transfer<KIND, S, CULL>), dim3((unsigned)ntasks), dim3(128), 0, s, A, sp, knots, ktimes, rec, tl.tasks,
+                           ntasks, (unsigned*)nullptr, (uint32_t*)nullptr);
+        return hipGetLastError();
+    }
+    if (!tl.head) return hipErrorInvalidValue;
+    static int groups = 0;       // per instantiation
+    if (!groups) groups = resident_groups(ks_chunk_transfer<KIND, S, CULL>, 128);
+    if (!groups) return hipErrorInvalidValue;
+    // the head and the per-task costs start at zero (fill kernels: no memset nodes in a captured graph)
+    hipLaunchKernelGGL(ks_fill_i32, dim3(1), dim3(64), 0, s, tl.head, (int64_t)1, 0);
+    if (tl.cost) hipLaunchKernelGGL(ks_fill_i32, dim3(grid_for(ntasks, 256)), dim3(256), 0, s, (int*)tl.cost, ntasks, 0);
+    const int64_t grid = std::min<int64_t>(ntasks, (int64_t)groups * (tl.group_scale > 0 ? tl.group_scale : 1));
+    hipLaunchKernelGGL((ks_chunk_transfer<KIND, S, CULL>), dim3((unsigned)grid), dim3(128), 0, s, A, sp, knots, ktimes, rec, tl.tasks,
+                       ntasks, (unsigned*)tl.head, tl.cost);
+    return hipGetLastError();
+}
 template <int KIND, class S>
-static void chunk_transfer_k(hipStream_t s, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
-                             const SplatCT<S>* sp, int ns, const S* knots, const S* ktimes, int nknots, const double* times, int nt,
-                             const double* nus, int nnu, double L, const int* kstop, double* seg, S* rec, uint8_t* aflg) {
-    const int nch = (N + CHUNK - 1) / CHUNK;
-    const int64_t items = npix * nt * nnu;
+static hipError_t chunk_transfer_k(hipStream_t s, const TransferArgs& A, const SplatCT<S>* sp, const S* knots, const S* ktimes, S* rec,
+                                   const TransferLaunch& tl) {
     // the per-tile cull bounds pattern-rotating splats by their arc and knot splats by the sphere holding their knots
-    if (ns >= CULL_MIN_NS && ns <= CAND_MAX)
-        hipLaunchKernelGGL((ks_chunk_transfer<KIND, S, true>), dim3(grid_for(items * nch, 128)), dim3(128), 0, s, a, thetao, soa, npix, N, smp, sp,
-                           ns, knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, nch, seg, rec, aflg);
-    else
-        hipLaunchKernelGGL((ks_chunk_transfer<KIND, S, false>), dim3(grid_for(items * nch, 128)), dim3(128), 0, s, a, thetao, soa, npix, N, smp, sp,
-                           ns, knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, nch, seg, rec, aflg);
+    if (A.ns >= CULL_MIN_NS && A.ns <= CAND_MAX) return chunk_transfer_c<KIND, S, true>(s, A, sp, knots, ktimes, rec, tl);
+    return chunk_transfer_c<KIND, S, false>(s, A, sp, knots, ktimes, rec, tl);
 }
 template <class S>
-static void chunk_transfer_s(hipStream_t s, int kind, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
-                             const SplatCT<S>* sp, int ns, const S* knots, const S* ktimes, int nknots, const double* times, int nt,
-                             const double* nus, int nnu, double L, const int* kstop, double* seg, S* rec, uint8_t* aflg) {
+static hipError_t chunk_transfer_s(hipStream_t s, int kind, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
+                                   const SplatCT<S>* sp, int ns, const S* knots, const S* ktimes, int nknots, const double* times, int nt,
+                                   const double* nus, int nnu, double L, const int* kstop, double* seg, S* rec, uint8_t* aflg,
+                                   const TransferLaunch& tl) {
+    if (tl.chunk < 1) return hipErrorInvalidValue;
+    const TransferArgs A{a, thetao, soa, npix, N, tl.chunk, (N + tl.chunk - 1) / tl.chunk, smp, ns, nknots, times, nt, nus, nnu, L, kstop, seg, aflg};
     switch (kind) {
-        case MODEL_THERMAL: chunk_transfer_k<MODEL_THERMAL, S>(s, a, thetao, soa, npix, N, smp, sp, ns, knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, seg, rec, aflg); break;
-        case MODEL_POWERLAW: chunk_transfer_k<MODEL_POWERLAW, S>(s, a, thetao, soa, npix, N, smp, sp, ns, knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, seg, rec, aflg); break;
-        case MODEL_KAPPA: chunk_transfer_k<MODEL_KAPPA, S>(s, a, thetao, soa, npix, N, smp, sp, ns, knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, seg, rec, aflg); break;
-        case MODEL_COMPOSITE: chunk_transfer_k<MODEL_COMPOSITE, S>(s, a, thetao, soa, npix, N, smp, sp, ns, knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, seg, rec, aflg); break;
-        default: chunk_transfer_k<MODEL_KNOT, S>(s, a, thetao, soa, npix, N, smp, sp, ns, knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, seg, rec, aflg); break;
+        case MODEL_THERMAL: return chunk_transfer_k<MODEL_THERMAL, S>(s, A, sp, knots, ktimes, rec, tl);
+        case MODEL_POWERLAW: return chunk_transfer_k<MODEL_POWERLAW, S>(s, A, sp, knots, ktimes, rec, tl);
+        case MODEL_KAPPA: return chunk_transfer_k<MODEL_KAPPA, S>(s, A, sp, knots, ktimes, rec, tl);
+        case MODEL_COMPOSITE: return chunk_transfer_k<MODEL_COMPOSITE, S>(s, A, sp, knots, ktimes, rec, tl);
+        default: return chunk_transfer_k<MODEL_KNOT, S>(s, A, sp, knots, ktimes, rec, tl);
     }
 }
 hipError_t launch_chunk_transfer(hipStream_t s, int kind, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
                                  const SplatC* sp, int ns, const double* knots, const double* ktimes, int nknots,
                                  const double* times, int nt, const double* nus, int nnu, double L, const int* kstop,
-                                 double* seg, double* rec, uint8_t* aflg) {
-    chunk_transfer_s<double>(s, kind, a, thetao, soa, npix, N, smp, sp, ns, knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, seg, rec, aflg);
-    return hipGetLastError();
+                                 double* seg, double* rec, uint8_t* aflg, const TransferLaunch& tl) {
+    return chunk_transfer_s<double>(s, kind, a, thetao, soa, npix, N, smp, sp, ns, knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, seg, rec,
+                                    aflg, tl);
 }
 hipError_t launch_chunk_transfer_f32(hipStream_t s, int kind, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
                                      const SplatCf* sp, int ns, const f32* knots, const f32* ktimes, int nknots,
                                      const double* times, int nt, const double* nus, int nnu, double L, const int* kstop,
-                                     double* seg, f32* rec, uint8_t* aflg) {
-    chunk_transfer_s<f32>(s, kind, a, thetao, soa, npix, N, smp, sp, ns, knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, seg, rec, aflg);
-    return hipGetLastError();
+                                     double* seg, f32* rec, uint8_t* aflg, const TransferLaunch& tl) {
+    return chunk_transfer_s<f32>(s, kind, a, thetao, soa, npix, N, smp, sp, ns, knots, ktimes, nknots, times, nt, nus, nnu, L, kstop, seg, rec,
+                                 aflg, tl);
+}
+__global__ void ks_iota_i32(int* __restrict__ p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (int)i;
+}
+// The tasks in descending order of their recorded costs (ticks): order[0 ..n) from cost[0 ..n); keys and ids are
+// n-long work arrays. Called with tmp = nullptr it sets tmp_bytes to the workspace the sort needs.
+hipError_t launch_order_tasks(hipStream_t s, const uint32_t* cost, int64_t n, uint32_t* keys, int* ids, int* order, void* tmp, size_t& tmp_bytes) {
+    if (tmp) hipLaunchKernelGGL(ks_iota_i32, dim3(grid_for(n, 256)), dim3(256), 0, s, ids, n);
+    return rocprim::radix_sort_pairs_desc(tmp, tmp_bytes, cost, keys, ids, order, (unsigned)n, 0, 32, s);
+}
+int64_t transfer_tasks(int64_t npix, int nt, int nnu, int N, int chunk) {
+    return (npix * nt * nnu * ((N + chunk - 1) / chunk) + 127) / 128;
 }
 template <int KIND>
 static void seg_polarized_k(hipStream_t s, double a, double thetao, const double* soa, int64_t npix, int N, int M, const SplatC* sp, int ns,

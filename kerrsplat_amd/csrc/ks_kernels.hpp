@@ -11,6 +11,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <algorithm>
 #include "ks_march.hpp"
 #include "ks_dtransfer.hpp"
 #include "ks_dadjoint.hpp"
@@ -53,7 +54,7 @@ inline int64_t adj_units(int64_t nitems, int N) { return ((nitems + 63) / 64) * 
 #define KS_WAVES_TRANSFER64 2
 #endif
 #ifndef KS_WAVES_TRANSFER32
-#define KS_WAVES_TRANSFER32 3
+#define KS_WAVES_TRANSFER32 4
 #endif
 #ifndef KS_WAVES_ADJSTEP64
 #define KS_WAVES_ADJSTEP64 1
@@ -186,7 +187,7 @@ hipError_t launch_case_scatter(hipStream_t s, int nblocks, const int8_t* numreal
 hipError_t launch_pixel_constants(hipStream_t s, double a, double thetao, const double* al, const double* be, const int* perm, int64_t npix,
                                   double* soa);
 hipError_t launch_march_store(hipStream_t s, double a, double thetao, const double* soa, int64_t npix, int N, int M, int marcher, int nseg_max,
-                              Samples smp, double* res);
+                              Samples smp, double* res, int fp32);
 hipError_t launch_res_reduce(hipStream_t s, int64_t npix, int nseg_max, const double* res, double* rt, double* rp);
 hipError_t launch_wind_cutoff(hipStream_t s, double a, double thetao, const double* soa, int64_t npix, int N, int M, int nmax, double slab,
                               Samples smp, int* kstop);
@@ -194,14 +195,28 @@ hipError_t launch_unsort4(hipStream_t s, const double* src, const int* perm, int
 hipError_t launch_unsort1(hipStream_t s, const double* src, const int* perm, int64_t npix, double* dst);
 
 // ---- ks_k_transfer.hip ----
+// How the transfer from stored samples is launched: samples per work item (chunk), one workgroup per task or a
+// persistent grid over a work queue, and optionally a task list (the queue's order, or a sample of the work).
+struct TransferLaunch {
+    int chunk = CHUNK;
+    int queue = 0;                 // 1: persistent workgroups taking tasks from a queue
+    const int* tasks = nullptr;    // tasks to run, in this order (null: every task, in index order)
+    int64_t ntasks = 0;            // the list's length
+    int* head = nullptr;           // the queue head (one int of device memory; zeroed by the launch)
+    uint32_t* cost = nullptr;      // queue form: per-task wall-clock ticks, indexed by task (null: not recorded)
+    int group_scale = 1;           // queue form: persistent workgroups = resident workgroups x group_scale
+};
+int64_t transfer_tasks(int64_t npix, int nt, int nnu, int N, int chunk);
+hipError_t launch_order_tasks(hipStream_t s, const uint32_t* cost, int64_t n, uint32_t* keys, int* ids, int* order, void* tmp, size_t& tmp_bytes);
+hipError_t launch_fill_i32(hipStream_t s, int* p, int64_t n, int v);
 hipError_t launch_chunk_transfer(hipStream_t s, int kind, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
                                  const SplatC* sp, int ns, const double* knots, const double* ktimes, int nknots,
                                  const double* times, int nt, const double* nus, int nnu, double L, const int* kstop,
-                                 double* seg, double* rec, uint8_t* aflg);
+                                 double* seg, double* rec, uint8_t* aflg, const TransferLaunch& tl);
 hipError_t launch_chunk_transfer_f32(hipStream_t s, int kind, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
                                      const SplatCf* sp, int ns, const f32* knots, const f32* ktimes, int nknots,
                                      const double* times, int nt, const double* nus, int nnu, double L, const int* kstop,
-                                     double* seg, f32* rec, uint8_t* aflg);
+                                     double* seg, f32* rec, uint8_t* aflg, const TransferLaunch& tl);
 hipError_t launch_seg_polarized(hipStream_t s, int kind, double a, double thetao, const double* soa, int64_t npix, int N, int M,
                                 const SplatC* sp, int ns, const double* knots, const double* ktimes, int nknots,
                                 const double* times, int nt, const double* nus, int nnu, double L, const int* kstop,
@@ -216,20 +231,20 @@ hipError_t launch_seg_combine(hipStream_t s, const double* soa, int64_t npix, in
                               const double* seg, double* out);
 
 // ---- ks_k_adjoint.hip ----
-hipError_t launch_adj_combine(hipStream_t s, int64_t npix, int N, int nt, const double* nus, int nnu, const double* seg,
+hipError_t launch_adj_combine(hipStream_t s, int64_t npix, int N, int chunk, int nt, const double* nus, int nnu, const double* seg,
                               const double* dstokes, double* wT, double* out);
-hipError_t launch_adj_segment(hipStream_t s, int N, int64_t nitems, const double* wT, const double* rec, double* bar, const uint8_t* aflg,
-                              double* tails, Samples smp, int64_t npix);
-hipError_t launch_adj_segment_f32(hipStream_t s, int N, int64_t nitems, const double* wT, const f32* rec, f32* bar, const uint8_t* aflg,
-                                  f32* tails, Samples smp, int64_t npix);
+hipError_t launch_adj_segment(hipStream_t s, int N, int chunk, int64_t nitems, const double* wT, const double* rec, double* bar,
+                              const uint8_t* aflg, double* tails, Samples smp, int64_t npix);
+hipError_t launch_adj_segment_f32(hipStream_t s, int N, int chunk, int64_t nitems, const double* wT, const f32* rec, f32* bar,
+                                  const uint8_t* aflg, f32* tails, Samples smp, int64_t npix);
 hipError_t launch_adj_splats_f32(hipStream_t s, int pops, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
                                  const SplatCf* sp, const SplatAdjf* sadj, int ns, const f32* knots, const f32* ktimes,
                                  const double* times, int nt, const double* nus, int64_t nitems, const f32* rec, const f32* bar,
-                                 const uint8_t* aflg, int64_t unit0, int64_t nb, const double* ub, double* part);
+                                 const uint8_t* aflg, int64_t unit0, int64_t nb, const double* ub, double* part, int* heads);
 hipError_t launch_adj_splats(hipStream_t s, int pops, double a, double thetao, const double* soa, int64_t npix, int N, Samples smp,
                              const SplatC* sp, const SplatAdj* sadj, int ns, const double* knots, const double* ktimes,
                              const double* times, int nt, const double* nus, int64_t nitems, const double* rec, const double* bar,
-                             const uint8_t* aflg, int64_t unit0, int64_t nb, const double* ub, double* part);
+                             const uint8_t* aflg, int64_t unit0, int64_t nb, const double* ub, double* part, int* heads);
 hipError_t launch_adj_reduce(hipStream_t s, const double* part, int64_t nunits, int ns, int accumulate, double* dparams);
 hipError_t launch_unit_bounds(hipStream_t s, int64_t npix, int N, Samples smp, const double* times, int nt, int64_t nitems,
                               const uint8_t* aflg, double* ub);
@@ -251,6 +266,9 @@ hipError_t launch_vis_scale_weights(hipStream_t s, int nuv, const double* w, con
 hipError_t launch_image_residual(hipStream_t s, const double* out, const int* perm, int64_t npix, int64_t f, const double* data,
                                  const double* sigma, int sigma_per_pixel, const uint8_t* mask, double* ads, double* chi_p);
 hipError_t launch_sum_fixed(hipStream_t s, const double* x, int64_t n, double* out);
+hipError_t launch_closure_terms(hipStream_t s, int ntri, const int* tri, const double* cp_data, const double* cp_sig, int nquad, const int* quad,
+                                const double* la_data, const double* la_sig, const double* V, double* chi_t, double* coef);
+hipError_t launch_closure_weights(hipStream_t s, int nuv, int ntri, const int* off, const int* legs, const double* coef, const double* V, double* w);
 hipError_t launch_bin_residual(hipStream_t s, const double* out, const int* iperm, int64_t npix, int64_t f, const int* off, const int* pts,
                                int64_t nbins, const double* data, const double* sigma, int sigma_per_pixel, const uint8_t* mask,
                                double* pseed, double* chi_p);
@@ -262,8 +280,14 @@ hipError_t launch_tangent(hipStream_t s, int kind, double a, double thetao, doub
                           double* out, double* dout);
 
 inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+// resident workgroups of `block` threads of a kernel on the current device (a persistent grid); 0 when unknown
+template <class K>
+inline int resident_groups(K kernel, int block) {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, block, 0) != hipSuccess)
+        return 0;
+    return cus * (per > 0 ? per : 1);
+}
 
 }  // namespace ksd
-hipError_t launch_closure_terms(hipStream_t s, int ntri, const int* tri, const double* cp_data, const double* cp_sig, int nquad, const int* quad,
-                                const double* la_data, const double* la_sig, const double* V, double* chi_t, double* coef);
-hipError_t launch_closure_weights(hipStream_t s, int nuv, int ntri, const int* off, const int* legs, const double* coef, const double* V, double* w);

@@ -95,8 +95,13 @@ struct ks_ctx {
     static constexpr int NSTAGE = 5;
     hipEvent_t evs[NSTAGE][2] = {};                // around the last launch of each stage
     bool staged[NSTAGE] = {};
-    hipError_t stage_begin(int k) { return hipEventRecord(evs[k][0], stream); }
-    hipError_t stage_end(int k) { staged[k] = true; return hipEventRecord(evs[k][1], stream); }
+    bool capturing = false;                        // a fit graph is being captured on the stream: no events, no host reads
+    hipError_t stage_begin(int k) { return capturing ? hipSuccess : hipEventRecord(evs[k][0], stream); }
+    hipError_t stage_end(int k) {
+        if (capturing) return hipSuccess;
+        staged[k] = true;
+        return hipEventRecord(evs[k][1], stream);
+    }
 };
 
 struct ks_cache {
@@ -108,12 +113,13 @@ struct ks_cache {
     double spin = NAN, thetao = NAN;
     int marcher = KS_MARCHER_DIRECT, M = 64;
     int precision = KS_PRECISION_FP64;                 // of the transfer and the adjoint over stored samples
+    int march_precision = KS_PRECISION_FP64;           // of the Recurrence march into stored samples (mixed: FP32 bulk)
     int generation = 0;
     int64_t counts[3] = {0, 0, 0};
     double *al = nullptr, *be = nullptr;
     int8_t* numreals = nullptr;
     int* perm = nullptr;
-    int* tmap = nullptr;                               // wavefront grouping of the sorted pixels (8 x 8 screen tiles)
+    int* tmap = nullptr;                               // wavefront grouping of the sorted pixels (patches of each screen part)
     int* iperm = nullptr;                              // screen index -> sorted slot
     int* bcount = nullptr; int nblocks = 0;
     double* soa = nullptr;
@@ -141,12 +147,26 @@ struct ks_cache {
     // adjoint workspace (grow-only): records, flags, tails, per-segment (w_in, T_out), seeds, partial sums
     double* arec = nullptr; double* abar = nullptr; uint8_t* aflg = nullptr; double* atail = nullptr; double* awT = nullptr; double* ads = nullptr;
     double* apart = nullptr; double* adpar = nullptr; double* aub = nullptr;
-    int64_t a_items = 0, a_part = 0, a_dpar = 0, a_ub = 0;
+    int64_t a_items = 0, a_part = 0, a_dpar = 0, a_ub = 0, a_wT = 0;
     // the last forward pass of the dual sweep (ks_gradient_forward): what ks_gradient_reverse needs
     uint64_t fw_model = 0;                             // id of its model (0: no forward pass to reverse)
     int fw_nt = 0, fw_nnu = 0, fw_generation = -1, fw_precision = -1;
     std::vector<uint8_t> fw_seeded;                    // frames whose device seed a *_chi2_seed call wrote
+    int fw_chunk = CHUNK;                              // samples per chunk of its segment maps
     std::vector<int> perm_host;
+    // launch choices of the transfer over stored samples (ks_cache_tuning, ks_autotune): samples per work item,
+    // the persistent work-queue form, the wavefront patch (patch_a x patch_b points of a screen part)
+    int chunk = CHUNK, queue = 0, patch_a = 8, patch_b = 8;
+    // screen parts of the point list (ks_cache_set_parts): consecutive runs of points, each an n_alpha x n_beta
+    // screen in screen order; empty: the whole camera is one nalpha x nbeta screen
+    std::vector<int64_t> part_off; std::vector<int> part_na, part_nb;
+    // the work queue: its head, the per-task ticks of the last queue launch, the cost-descending order made from
+    // them and the launch shape that order belongs to, the sort's work arrays
+    Buf<int> qhead, qorder, qids, aheads;              // aheads: the element adjoints' per-XCD queue heads
+    Buf<uint32_t> qcost, qkeys;
+    Buf<uint8_t> qtmp;
+    int64_t q_shape[6] = {-1, -1, -1, -1, -1, -1};
+    bool q_ordered = false;
 };
 
 // Binning (camera.jl:62-80) on the device: the points of every pixel as a list in ascending screen order
@@ -171,6 +191,8 @@ struct ks_model {
     int64_t nkn = 0, nkt = 0;                          // knot triples and knot times held (knot models and composites with knot parts)
     // the same constants in FP32 (the mixed-precision transfer and adjoint)
     SplatCf* d_splats_f = nullptr; SplatAdjf* d_adj_f = nullptr; f32* d_knots_f = nullptr; f32* d_ktimes_f = nullptr;
+    double gamma_max = 0.0;                            // of a single-population model: what its constants were made with
+    std::vector<double> h_knots;                       // and its knots (the per-splat constants bound the knot paths)
 };
 
 extern "C" {
@@ -281,8 +303,10 @@ int ks_cache_destroy(ks_cache* c) {
                            &c->vSd, &c->vSp, &c->vchi, &c->vsum, &c->idata, &c->isig, &c->bseed})
         b->release();
     c->imask.release();
+    c->ctri.release(); c->cquad.release(); c->coff.release(); c->clegs.release(); c->cdat.release(); c->cterm.release();
     dfree(c->seg); dfree(c->segres); dfree(c->kstop); dfree(c->dtn);
     dfree(c->arec); dfree(c->abar); dfree(c->aflg); dfree(c->atail); dfree(c->awT); dfree(c->ads); dfree(c->apart); dfree(c->adpar); dfree(c->aub);
+    c->qhead.release(); c->qorder.release(); c->qids.release(); c->aheads.release(); c->qcost.release(); c->qkeys.release(); c->qtmp.release();
     delete c;
     return KS_OK;
 }
@@ -295,7 +319,6 @@ int ks_cache_info(const ks_cache* c, int64_t* npix, int* N, int* generation, int
     if (counts) for (int i = 0; i < 3; ++i) counts[i] = c->counts[i];
     return KS_OK;
 }
-    c->ctri.release(); c->cquad.release(); c->coff.release(); c->clegs.release(); c->cdat.release(); c->cterm.release();
 
 static Samples samples_of(const ks_cache* c) {
     Samples S;
@@ -316,12 +339,40 @@ static int launch_store_march(ks_cache* c) {
     const int M = c->marcher == KS_MARCHER_DIRECT ? 64 : c->M;
     HIPCHK(x->stage_begin(KS_STAGE_MARCH));
     HIPCHK(launch_march_store(x->stream, c->spin, c->thetao, c->soa, c->npix, c->N, M, c->marcher == KS_MARCHER_DIRECT ? 0 : 1, nseg,
-                              samples_of(c), c->segres));
+                              samples_of(c), c->segres, c->march_precision == KS_PRECISION_MIXED ? 1 : 0));
     HIPCHK(x->stage_end(KS_STAGE_MARCH));
     HIPCHK(launch_res_reduce(x->stream, c->npix, nseg, c->segres, c->rt, c->rp));
     return KS_OK;
 }
 static bool has_samples(const ks_cache* c) { return c->store && c->marcher != KS_MARCHER_FUSED && c->generation >= 1; }
+
+// screen index -> sorted slot
+static std::vector<int> iperm_of(const ks_cache* c) {
+    std::vector<int> inv(c->npix);
+    for (int64_t j = 0; j < c->npix; ++j) inv[c->perm_host[j]] = (int)j;
+    return inv;
+}
+// Wavefront grouping for the kernels over stored samples: the sorted pixels in patch_a x patch_b patches of each
+// screen part, so that the 64 lanes of a wavefront look at one compact patch of sky (a splat then covers most or none
+// of them); a patch row stays contiguous in the case-sorted order.
+static int build_tmap(ks_cache* c) {
+    const std::vector<int> inv = iperm_of(c);
+    std::vector<int64_t> off = c->part_off;
+    std::vector<int> na = c->part_na, nb = c->part_nb;
+    if (off.empty()) { off = {0}; na = {c->nalpha}; nb = {c->nbeta}; }
+    std::vector<int> tm;
+    tm.reserve(c->npix);
+    const int pa = c->patch_a, pb = c->patch_b;
+    for (size_t q = 0; q < off.size(); ++q)
+        for (int tb = 0; tb < nb[q]; tb += pb)
+            for (int ta = 0; ta < na[q]; ta += pa)
+                for (int ib = tb; ib < std::min(tb + pb, nb[q]); ++ib)
+                    for (int ia = ta; ia < std::min(ta + pa, na[q]); ++ia) tm.push_back(inv[off[q] + ia + (int64_t)na[q] * ib]);
+    if ((int64_t)tm.size() != c->npix) return fail(KS_ERR_STATE, "screen parts cover %zu of %lld points", tm.size(), (long long)c->npix);
+    HIPCHK(hipMemcpy(c->tmap, tm.data(), c->npix * sizeof(int), hipMemcpyHostToDevice));
+    c->q_ordered = false;          // the tasks hold other items now
+    return KS_OK;
+}
 
 int ks_regenerate(ks_cache* c, const double* alpha, const double* beta, int nalpha, int nbeta, double a, double thetao, int marcher, int M) {
     if (!c || !alpha || !beta) return fail(KS_ERR_ARG, "ks_regenerate: null argument");
@@ -364,20 +415,8 @@ int ks_regenerate(ks_cache* c, const double* alpha, const double* beta, int nalp
     }
     HIPCHK(hipMemcpyAsync(c->perm_host.data(), c->perm, n * sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    // Wavefront grouping for the kernels over stored samples: the sorted pixels in 8 x 8 screen tiles, so
-    // that the 64 lanes of a wavefront look at one compact patch of sky (a splat then covers most or none
-    // of them); a tile row of 8 pixels stays contiguous in the case-sorted order.
-    {
-        std::vector<int> inv(n), tm;
-        tm.reserve(n);
-        for (int64_t j = 0; j < n; ++j) inv[c->perm_host[j]] = (int)j;
-        for (int tb = 0; tb < nbeta; tb += 8)
-            for (int ta = 0; ta < nalpha; ta += 8)
-                for (int ib = tb; ib < std::min(tb + 8, nbeta); ++ib)
-                    for (int ia = ta; ia < std::min(ta + 8, nalpha); ++ia) tm.push_back(inv[ia + (int64_t)nalpha * ib]);
-        HIPCHK(hipMemcpy(c->tmap, tm.data(), n * sizeof(int), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(c->iperm, inv.data(), n * sizeof(int), hipMemcpyHostToDevice));
-    }
+    HIPCHK(hipMemcpy(c->iperm, iperm_of(c).data(), c->npix * sizeof(int), hipMemcpyHostToDevice));
+    if (int rc = build_tmap(c)) return rc;
     c->generation += 1;
     return KS_OK;
 }
@@ -441,6 +480,8 @@ int ks_model_create(ks_ctx* ctx, int kind, int nrows, int ns, const double* para
     m->ctx = ctx; m->kind = kind; m->nrows = nrows; m->ns = ns;
     m->hrows.assign((size_t)ns, nrows);
     m->pops = (kind == KS_MODEL_POWERLAW ? POP_POWERLAW : kind == KS_MODEL_KAPPA ? POP_KAPPA : POP_THERMAL) | (with_knots ? POP_KNOTS : 0);
+    m->gamma_max = gamma_max;
+    if (with_knots) m->h_knots.assign(knots, knots + (size_t)3 * nknots * ns);
     if (dalloc(&m->d_splats, h.size()) || dalloc(&m->d_adj, ha.size()) || dalloc(&m->d_splats_f, h.size()) || dalloc(&m->d_adj_f, ha.size())) {
         ks_model_destroy(m); return KS_ERR_HIP;
     }
@@ -577,14 +618,66 @@ static const double* stage_times(ks_cache* c, const double* times, int nt, const
     return c->dtn;
 }
 
+// Segment maps per item of the transfer: chunks of the stored samples or the pixel's anchor segments
+static int seg_count(const ks_cache* c, int chunk) { return std::max(max_segments(c->N), (c->N + chunk - 1) / chunk); }
+
+// The launch of the transfer over stored samples as the cache's tuning asks: one workgroup per task, or the
+// persistent work queue. The queue takes the tasks in the cost-descending order of an earlier launch of the same
+// shape when there is one, records every task's ticks, and orders the tasks by them for the next launch.
+static int transfer_launch(ks_cache* c, const ks_model* m, const double* d_tn, int nt, int nnu, double L, const int* kst, void* rec,
+                           uint8_t* aflg, const TransferLaunch* sample) {
+    ks_ctx* x = c->ctx;
+    TransferLaunch tl;
+    tl.chunk = c->chunk;
+    tl.queue = (c->queue & KS_QUEUE_TRANSFER) ? 1 : 0;
+    const int64_t ntasks = transfer_tasks(c->npix, nt, nnu, c->N, c->chunk);
+    bool order_after = false;
+    if (sample) {
+        tl.tasks = sample->tasks; tl.ntasks = sample->ntasks;
+    }
+    if (tl.queue) {
+        if (c->qhead.grow(1)) return KS_ERR_HIP;
+        tl.head = c->qhead.p;
+        if (!sample && x->capturing) {
+            // a captured launch takes the order the last eager launch of this shape left, and records nothing
+            const int64_t shape[6] = {ntasks, c->chunk, (int64_t)m->id, c->precision, rec ? 1 : 0, (int64_t)nt * 65536 + nnu};
+            if (c->q_ordered && std::equal(shape, shape + 6, c->q_shape)) { tl.tasks = c->qorder.p; tl.ntasks = ntasks; }
+        } else if (!sample) {
+            const int64_t shape[6] = {ntasks, c->chunk, (int64_t)m->id, c->precision, rec ? 1 : 0, (int64_t)nt * 65536 + nnu};
+            if (!std::equal(shape, shape + 6, c->q_shape)) { std::copy(shape, shape + 6, c->q_shape); c->q_ordered = false; }
+            if (c->qcost.grow((size_t)ntasks)) return KS_ERR_HIP;
+            tl.cost = c->qcost.p;
+            if (c->q_ordered) { tl.tasks = c->qorder.p; tl.ntasks = ntasks; }
+            order_after = true;
+        }
+    }
+    const int kind = m->kind;
+    if (c->precision == KS_PRECISION_MIXED)
+        HIPCHK(launch_chunk_transfer_f32(x->stream, kind, c->spin, c->thetao, c->soa, c->npix, c->N, samples_of(c), m->d_splats_f, m->ns,
+                                         m->d_knots_f, m->d_ktimes_f, m->nknots, d_tn, nt, d_tn + nt, nnu, L, kst, c->seg, (f32*)rec, aflg, tl));
+    else
+        HIPCHK(launch_chunk_transfer(x->stream, kind, c->spin, c->thetao, c->soa, c->npix, c->N, samples_of(c), m->d_splats, m->ns,
+                                     m->d_knots, m->d_ktimes, m->nknots, d_tn, nt, d_tn + nt, nnu, L, kst, c->seg, (double*)rec, aflg, tl));
+    if (order_after) {
+        size_t bytes = 0;
+        HIPCHK(launch_order_tasks(x->stream, c->qcost.p, ntasks, nullptr, nullptr, nullptr, nullptr, bytes));
+        if (c->qorder.grow((size_t)ntasks) || c->qids.grow((size_t)ntasks) || c->qkeys.grow((size_t)ntasks) || c->qtmp.grow(std::max<size_t>(bytes, 1)))
+            return KS_ERR_HIP;
+        // the order for the next launch (stream order: the sort starts after the launch that reads qorder has ended)
+        HIPCHK(launch_order_tasks(x->stream, c->qcost.p, ntasks, c->qkeys.p, c->qids.p, c->qorder.p, c->qtmp.p, bytes));
+        c->q_ordered = true;
+    }
+    return KS_OK;
+}
+
 // The transfer of every (pixel, time, frequency) item into the cache's segment maps: from the stored
-// samples when the cache holds them (one work item per CHUNK samples), else fused with the march (one
+// samples when the cache holds them (one work item per chunk of samples), else fused with the march (one
 // work item per anchor segment). Records for the adjoint when rec != nullptr (stored samples only).
 static int launch_transfer(ks_cache* c, const ks_model* m, const double* d_tn, int nt, int nnu, double L, int nmax, double slab,
                            double* rec, uint8_t* aflg) {
     ks_ctx* x = c->ctx;
     const int64_t items = c->npix * nt * nnu;
-    const int nseg = max_segments(c->N);
+    const int nseg = seg_count(c, c->chunk);
     if (c->seg_cap < (int64_t)SEG_FIELDS * nseg * items) {
         dfree(c->seg);
         if (dalloc(&c->seg, (size_t)SEG_FIELDS * nseg * items)) return KS_ERR_HIP;
@@ -598,19 +691,14 @@ static int launch_transfer(ks_cache* c, const ks_model* m, const double* d_tn, i
         kst = c->kstop;
     }
     HIPCHK(x->stage_begin(KS_STAGE_TRANSFER));
-    const int kind = m->kind;
-    if (has_samples(c) && c->precision == KS_PRECISION_MIXED)
-        HIPCHK(launch_chunk_transfer_f32(x->stream, kind, c->spin, c->thetao, c->soa, c->npix, c->N, samples_of(c), m->d_splats_f, m->ns,
-                                         m->d_knots_f, m->d_ktimes_f, m->nknots, d_tn, nt, d_tn + nt, nnu, L, kst, c->seg, (f32*)rec, aflg));
-    else if (has_samples(c))
-        HIPCHK(launch_chunk_transfer(x->stream, kind, c->spin, c->thetao, c->soa, c->npix, c->N, samples_of(c), m->d_splats, m->ns,
-                                     m->d_knots, m->d_ktimes, m->nknots, d_tn, nt, d_tn + nt, nnu, L, kst, c->seg, rec, aflg));
-    else
-        HIPCHK(launch_seg_polarized(x->stream, kind, c->spin, c->thetao, c->soa, c->npix, c->N, c->M, m->d_splats, m->ns,
+    if (has_samples(c)) {
+        if (int rc = transfer_launch(c, m, d_tn, nt, nnu, L, kst, rec, aflg, nullptr)) return rc;
+    } else
+        HIPCHK(launch_seg_polarized(x->stream, m->kind, c->spin, c->thetao, c->soa, c->npix, c->N, c->M, m->d_splats, m->ns,
                                     m->d_knots, m->d_ktimes, m->nknots, d_tn, nt, d_tn + nt, nnu, L, kst, c->seg, c->segres));
     HIPCHK(x->stage_end(KS_STAGE_TRANSFER));
     if (!has_samples(c)) {
-        HIPCHK(launch_res_reduce(x->stream, c->npix, nseg, c->segres, c->rt, c->rp));
+        HIPCHK(launch_res_reduce(x->stream, c->npix, max_segments(c->N), c->segres, c->rt, c->rp));
     }
     return KS_OK;
 }
@@ -632,7 +720,14 @@ int ks_render_device(ks_cache* c, const ks_model* m, const double* times, int nt
     int rc = launch_transfer(c, m, d_tn, nt, nnu, L, nmax, slab, nullptr, nullptr);
     if (rc) return rc;
     const int64_t items = c->npix * frames;
-    HIPCHK(launch_seg_combine(x->stream, c->soa, c->npix, c->N, c->M, has_samples(c) ? CHUNK : 0, nt, d_tn + nt, nnu, c->seg, c->out));
+    HIPCHK(launch_seg_combine(x->stream, c->soa, c->npix, c->N, c->M, has_samples(c) ? c->chunk : 0, nt, d_tn + nt, nnu, c->seg, c->out));
+    return KS_OK;
+}
+
+int ks_cache_march_precision(ks_cache* c, int precision) {
+    if (!c) return fail(KS_ERR_ARG, "null cache");
+    if (precision != KS_PRECISION_FP64 && precision != KS_PRECISION_MIXED) return fail(KS_ERR_ARG, "unknown precision policy %d", precision);
+    c->march_precision = precision;
     return KS_OK;
 }
 
@@ -640,6 +735,178 @@ int ks_cache_precision(ks_cache* c, int precision) {
     if (!c) return fail(KS_ERR_ARG, "null cache");
     if (precision != KS_PRECISION_FP64 && precision != KS_PRECISION_MIXED) return fail(KS_ERR_ARG, "unknown precision policy %d", precision);
     c->precision = precision;
+    return KS_OK;
+}
+
+static int ensure_adj(ks_cache* c, int64_t items, int ns);
+static int reverse_launches(ks_cache* c, const ks_model* m, const double* d_tn, int nt, int nnu);
+
+int ks_cache_tuning(ks_cache* c, int chunk, int queue, int patch_a, int patch_b) {
+    if (!c) return fail(KS_ERR_ARG, "null cache");
+    if (chunk < 8 || chunk > 1024) return fail(KS_ERR_ARG, "samples per work item %d outside 8:1024", chunk);
+    if (queue < 0 || queue > (KS_QUEUE_TRANSFER | KS_QUEUE_ADJOINT))
+        return fail(KS_ERR_ARG, "queue takes the bits KS_QUEUE_TRANSFER and KS_QUEUE_ADJOINT (got %d)", queue);
+    if (patch_a < 1 || patch_b < 1 || patch_a * patch_b > 4096) return fail(KS_ERR_ARG, "wavefront patch %d x %d outside 1:4096 points", patch_a, patch_b);
+    HIPCHK(hipSetDevice(c->ctx->device));
+    c->fw_model = 0;                   // segment maps of another chunk length would not match a later reverse pass
+    c->chunk = chunk; c->queue = queue;
+    if (patch_a != c->patch_a || patch_b != c->patch_b) {
+        c->patch_a = patch_a; c->patch_b = patch_b;
+        if (c->generation > 0) return build_tmap(c);
+    }
+    return KS_OK;
+}
+
+int ks_cache_get_tuning(const ks_cache* c, int* chunk, int* queue, int* patch_a, int* patch_b) {
+    if (!c) return fail(KS_ERR_ARG, "null cache");
+    if (chunk) *chunk = c->chunk;
+    if (queue) *queue = c->queue;
+    if (patch_a) *patch_a = c->patch_a;
+    if (patch_b) *patch_b = c->patch_b;
+    return KS_OK;
+}
+
+int ks_cache_set_parts(ks_cache* c, int nparts, const int* n_alpha, const int* n_beta) {
+    if (!c) return fail(KS_ERR_ARG, "null cache");
+    if (nparts < 0 || (nparts > 0 && (!n_alpha || !n_beta))) return fail(KS_ERR_ARG, "ks_cache_set_parts: null part sizes");
+    std::vector<int64_t> off;
+    int64_t tot = 0;
+    for (int q = 0; q < nparts; ++q) {
+        if (n_alpha[q] < 1 || n_beta[q] < 1) return fail(KS_ERR_ARG, "part %d has size %d x %d", q, n_alpha[q], n_beta[q]);
+        off.push_back(tot);
+        tot += (int64_t)n_alpha[q] * n_beta[q];
+    }
+    if (nparts > 0 && tot != c->npix) return fail(KS_ERR_ARG, "the parts hold %lld points, the cache %lld", (long long)tot, (long long)c->npix);
+    HIPCHK(hipSetDevice(c->ctx->device));
+    c->part_off = off;
+    c->part_na.assign(n_alpha, n_alpha + nparts);
+    c->part_nb.assign(n_beta, n_beta + nparts);
+    return c->generation > 0 ? build_tmap(c) : KS_OK;
+}
+
+int ks_queue_costs(const ks_cache* c, int64_t* ntasks, uint32_t* cost, int cap) {
+    if (!c || !ntasks) return fail(KS_ERR_ARG, "null argument");
+    if (c->q_shape[0] < 0 || !c->qcost.p) return fail(KS_ERR_STATE, "no work-queue launch has run on this cache");
+    *ntasks = c->q_shape[0];
+    if (cost) {
+        HIPCHK(hipStreamSynchronize(c->ctx->stream));
+        HIPCHK(hipMemcpy(cost, c->qcost.p, (size_t)std::min<int64_t>(cap, c->q_shape[0]) * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    }
+    return KS_OK;
+}
+
+// The autotuner. Transfer (unless KS_TUNE_KEEP_TRANSFER): every candidate launch (samples per work item x wavefront
+// patch x launch form) timed with HIP events on a sample of the transfer's work (each task of the candidate's own
+// task list with probability 1 / sample_every, by a hash of its index: the same fraction of the (item, sample)
+// pairs for every candidate, spread over the whole screen), twice, the faster run counting. Element adjoints (with
+// KS_TUNE_ADJOINT): one forward pass with the records at the chosen transfer, then the reverse pass once in each
+// launch form of the element adjoints, timed by their stage events (zero seeds: the work does not depend on them).
+// The cache keeps the fastest of each; rows of the table: (chunk, patch_a, patch_b, queue bits, ms), the element
+// adjoints' rows with chunk = 0.
+int ks_autotune(ks_cache* c, const ks_model* m, const double* times, int nt, const double* nus, int nnu, double L, int flags, int sample_every,
+                double* table, int cap, int* ncand) {
+    if (!c || !m || !times || !nus || !ncand) return fail(KS_ERR_ARG, "ks_autotune: null argument");
+    if (c->generation < 1) return fail(KS_ERR_STATE, "regenerate! the cache first");
+    if (!has_samples(c)) return fail(KS_ERR_ARG, "the autotuner tunes the transfer over stored samples: build the cache with store_samples = true");
+    if (m->kind == KS_MODEL_THIN) return fail(KS_ERR_ARG, "thin splats render through ks_thin_image");
+    if (nt < 1 || nnu < 1 || !(L > 0.0)) return fail(KS_ERR_ARG, "need times, frequencies and a positive length unit");
+    if (sample_every < 1) return fail(KS_ERR_ARG, "sample_every must be positive");
+    if (flags & ~(KS_TUNE_RECORDS | KS_TUNE_ADJOINT | KS_TUNE_KEEP_TRANSFER)) return fail(KS_ERR_ARG, "unknown autotuner flags %d", flags);
+    const bool records = (flags & (KS_TUNE_RECORDS | KS_TUNE_ADJOINT)) != 0;
+    ks_ctx* x = c->ctx;
+    HIPCHK(hipSetDevice(x->device));
+    c->fw_model = 0;
+    const int64_t frames = (int64_t)nt * nnu, items = c->npix * frames;
+    const int chunks[4] = {16, 32, 64, 128};
+    const int patches[3][2] = {{8, 8}, {16, 4}, {4, 16}};
+    const int adj_bit = c->queue & KS_QUEUE_ADJOINT;
+    const double* d_tn = stage_times(c, times, nt, nus, nnu);
+    if (!d_tn) return KS_ERR_HIP;
+    int n = 0;
+    std::vector<double> rows;
+    if (!(flags & KS_TUNE_KEEP_TRANSFER)) {
+        const int old[4] = {c->chunk, c->queue, c->patch_a, c->patch_b};
+        int best = -1;
+        double best_ms = INFINITY;
+        std::vector<int> tasks;
+        Buf<int> dtasks;
+        for (int pc = 0; pc < 3; ++pc) {
+            c->patch_a = patches[pc][0]; c->patch_b = patches[pc][1];
+            if (int rc = build_tmap(c)) return rc;
+            for (int ci = 0; ci < 4; ++ci) {
+                if (chunks[ci] > std::max(c->N, 16)) continue;
+                c->chunk = chunks[ci];
+                if (records && ensure_adj(c, items, std::max(m->ns, 1))) return KS_ERR_HIP;
+                const int nseg = seg_count(c, c->chunk);
+                if (c->seg_cap < (int64_t)SEG_FIELDS * nseg * items) {
+                    dfree(c->seg);
+                    if (dalloc(&c->seg, (size_t)SEG_FIELDS * nseg * items)) return KS_ERR_HIP;
+                    c->seg_cap = (int64_t)SEG_FIELDS * nseg * items;
+                }
+                const int64_t all = transfer_tasks(c->npix, nt, nnu, c->N, c->chunk);
+                // a hashed Bernoulli sample of rate 1 / sample_every (a regular stride aliases with the screen: every k-th
+                // task can fall on one column of patches)
+                tasks.clear();
+                for (int64_t q = 0; q < all; ++q) {
+                    uint64_t h = (uint64_t)q * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+                    h ^= h >> 31; h *= 0xBF58476D1CE4E5B9ull; h ^= h >> 29;
+                    if (h % (uint64_t)sample_every == 0) tasks.push_back((int)q);
+                }
+                if (tasks.empty()) tasks.push_back(0);
+                if (dtasks.grow(tasks.size())) return KS_ERR_HIP;
+                HIPCHK(hipMemcpy(dtasks.p, tasks.data(), tasks.size() * sizeof(int), hipMemcpyHostToDevice));
+                TransferLaunch smp;
+                smp.tasks = dtasks.p; smp.ntasks = (int64_t)tasks.size();
+                for (int qu = 0; qu < 2; ++qu) {
+                    c->queue = qu | adj_bit;
+                    float ms = INFINITY;
+                    for (int rep = 0; rep < 2; ++rep) {
+                        HIPCHK(hipEventRecord(x->ev0, x->stream));
+                        int rc = transfer_launch(c, m, d_tn, nt, nnu, L, nullptr, records ? (void*)c->arec : nullptr, records ? c->aflg : nullptr, &smp);
+                        if (rc) { dtasks.release(); return rc; }
+                        HIPCHK(hipEventRecord(x->ev1, x->stream));
+                        HIPCHK(hipEventSynchronize(x->ev1));
+                        float t = 0.0f;
+                        HIPCHK(hipEventElapsedTime(&t, x->ev0, x->ev1));
+                        ms = std::min(ms, t);
+                    }
+                    rows.insert(rows.end(), {(double)c->chunk, (double)c->patch_a, (double)c->patch_b, (double)c->queue, (double)ms});
+                    if (ms < best_ms) { best_ms = ms; best = n; }
+                    ++n;
+                }
+            }
+        }
+        dtasks.release();
+        if (best < 0) { c->chunk = old[0]; c->queue = old[1]; c->patch_a = old[2]; c->patch_b = old[3]; }
+        else {
+            c->chunk = (int)rows[5 * best]; c->patch_a = (int)rows[5 * best + 1]; c->patch_b = (int)rows[5 * best + 2];
+            c->queue = (int)rows[5 * best + 3];
+        }
+        if (int rc = build_tmap(c)) return rc;
+    }
+    if (flags & KS_TUNE_ADJOINT) {
+        if (m->ns < 1) return fail(KS_ERR_ARG, "the element adjoints need splats");
+        if (ensure_out(c, frames) || ensure_adj(c, items, m->ns)) return KS_ERR_HIP;
+        if (int rc = launch_transfer(c, m, d_tn, nt, nnu, L, -1, 0.0, c->arec, c->aflg)) return rc;
+        c->fw_chunk = c->chunk;
+        HIPCHK(hipMemsetAsync(c->ads, 0, (size_t)4 * items * sizeof(double), x->stream));
+        int best_q = c->queue;
+        double best_ms = INFINITY;
+        for (int qa = 0; qa < 2; ++qa) {
+            c->queue = (c->queue & KS_QUEUE_TRANSFER) | (qa ? KS_QUEUE_ADJOINT : 0);
+            if (int rc = reverse_launches(c, m, d_tn, nt, nnu)) return rc;
+            HIPCHK(hipStreamSynchronize(x->stream));
+            float ms = 0.0f;
+            HIPCHK(hipEventElapsedTime(&ms, x->evs[KS_STAGE_ADJ_ELEMENTS][0], x->evs[KS_STAGE_ADJ_ELEMENTS][1]));
+            rows.insert(rows.end(), {0.0, 0.0, 0.0, (double)c->queue, (double)ms});
+            ++n;
+            if (ms < best_ms) { best_ms = ms; best_q = c->queue; }
+        }
+        c->queue = best_q;
+    }
+    c->fw_model = 0;
+    *ncand = n;
+    if (table) std::copy(rows.begin(), rows.begin() + 5 * std::min(n, std::max(cap, 0)), table);
     return KS_OK;
 }
 
@@ -669,15 +936,21 @@ static int64_t adj_unit_batch(int64_t nunits, int ns) {
 }
 
 static int ensure_adj(ks_cache* c, int64_t items, int ns) {
-    const int nseg = (c->N + CHUNK - 1) / CHUNK;
+    const int nseg = (c->N + c->chunk - 1) / c->chunk;
     if (c->a_items < items) {
-        dfree(c->arec); dfree(c->abar); dfree(c->aflg); dfree(c->atail); dfree(c->awT); dfree(c->ads);
+        dfree(c->arec); dfree(c->abar); dfree(c->aflg); dfree(c->atail); dfree(c->ads);
         c->a_items = 0;
         const size_t per = (size_t)c->N * items;
         if (dalloc(&c->arec, (size_t)REC_FIELDS * per) || dalloc(&c->abar, (size_t)BAR_FIELDS * per) || dalloc(&c->aflg, per) || dalloc(&c->atail, 4 * per) ||
-            dalloc(&c->awT, (size_t)8 * nseg * items) || dalloc(&c->ads, (size_t)4 * items))
+            dalloc(&c->ads, (size_t)4 * items))
             return KS_ERR_HIP;
         c->a_items = items;
+    }
+    if (c->a_wT < (int64_t)8 * nseg * items) {      // per chunk (w_in, T_out): grows with the items and the chunks per ray
+        dfree(c->awT);
+        c->a_wT = 0;
+        if (dalloc(&c->awT, (size_t)8 * nseg * items)) return KS_ERR_HIP;
+        c->a_wT = (int64_t)8 * nseg * items;
     }
     const int64_t nunits = adj_units(items, c->N);
     const int64_t nb = adj_unit_batch(nunits, ns);
@@ -695,6 +968,46 @@ static int ensure_adj(ks_cache* c, int64_t items, int ns) {
         dfree(c->adpar);
         if (dalloc(&c->adpar, (size_t)ns * ADJ_ROWS)) return KS_ERR_HIP;
         c->a_dpar = (int64_t)ns * ADJ_ROWS;
+    }
+    return KS_OK;
+}
+
+// the reverse pass's launches after the seeds: tails and adjoints per chunk, the element adjoints in batches, the
+// reduction into c->adpar (ns x ADJ_ROWS)
+static int reverse_launches(ks_cache* c, const ks_model* m, const double* d_tn, int nt, int nnu) {
+    ks_ctx* x = c->ctx;
+    const int64_t items = c->npix * nt * nnu;
+    const int ns = m->ns;
+    HIPCHK(launch_adj_combine(x->stream, c->npix, c->N, c->fw_chunk, nt, d_tn + nt, nnu, c->seg, c->ads, c->awT, nullptr));
+    HIPCHK(x->stage_begin(KS_STAGE_ADJ_STEPS));
+    const bool mixed = c->precision == KS_PRECISION_MIXED;
+    if (mixed) HIPCHK(launch_adj_segment_f32(x->stream, c->N, c->fw_chunk, items, c->awT, (const f32*)c->arec, (f32*)c->abar, c->aflg, (f32*)c->atail,
+                                             samples_of(c), c->npix));
+    else HIPCHK(launch_adj_segment(x->stream, c->N, c->fw_chunk, items, c->awT, c->arec, c->abar, c->aflg, c->atail, samples_of(c), c->npix));
+    HIPCHK(x->stage_end(KS_STAGE_ADJ_STEPS));
+    if (ns > 0) {
+        HIPCHK(x->stage_begin(KS_STAGE_ADJ_ELEMENTS));
+        HIPCHK(launch_unit_bounds(x->stream, c->npix, c->N, samples_of(c), d_tn, nt, items, c->aflg, c->aub));
+        // the persistent form of the element adjoints with the cache's work-queue choice: per-XCD queue heads
+        int* heads = nullptr;
+        if (c->queue & KS_QUEUE_ADJOINT) {
+            if (c->aheads.grow(8)) return KS_ERR_HIP;
+            heads = c->aheads.p;
+        }
+        // the units in batches whose partial sums fit the workspace, each reduced into the running sums in order
+        const int64_t nunits = adj_units(items, c->N), nbmax = adj_unit_batch(nunits, ns);
+        for (int64_t u0 = 0; u0 < nunits; u0 += nbmax) {
+            const int64_t nb = std::min(nbmax, nunits - u0);
+            if (mixed)
+                HIPCHK(launch_adj_splats_f32(x->stream, m->pops, c->spin, c->thetao, c->soa, c->npix, c->N, samples_of(c), m->d_splats_f, m->d_adj_f, ns,
+                                             m->d_knots_f, m->d_ktimes_f, d_tn, nt, d_tn + nt, items, (const f32*)c->arec, (const f32*)c->abar, c->aflg,
+                                             u0, nb, c->aub, c->apart, heads));
+            else
+                HIPCHK(launch_adj_splats(x->stream, m->pops, c->spin, c->thetao, c->soa, c->npix, c->N, samples_of(c), m->d_splats, m->d_adj, ns, m->d_knots,
+                                         m->d_ktimes, d_tn, nt, d_tn + nt, items, c->arec, c->abar, c->aflg, u0, nb, c->aub, c->apart, heads));
+            HIPCHK(launch_adj_reduce(x->stream, c->apart, nb, ns, u0 > 0, c->adpar));
+        }
+        HIPCHK(x->stage_end(KS_STAGE_ADJ_ELEMENTS));
     }
     return KS_OK;
 }
@@ -720,8 +1033,9 @@ int ks_gradient_forward(ks_cache* c, const ks_model* m, const double* times, int
     if (!d_tn) return KS_ERR_HIP;
     int rc = launch_transfer(c, m, d_tn, nt, nnu, L, nmax, slab, c->arec, c->aflg);
     if (rc) return rc;
-    HIPCHK(launch_seg_combine(x->stream, c->soa, c->npix, c->N, c->M, CHUNK, nt, d_tn + nt, nnu, c->seg, c->out));
+    HIPCHK(launch_seg_combine(x->stream, c->soa, c->npix, c->N, c->M, c->chunk, nt, d_tn + nt, nnu, c->seg, c->out));
     c->fw_model = m->id; c->fw_nt = nt; c->fw_nnu = nnu; c->fw_generation = c->generation; c->fw_precision = c->precision;
+    c->fw_chunk = c->chunk;
     c->fw_seeded.assign((size_t)frames, 0);
     if (image) return ks_copy_stokes(c, frames, image, screen_order);
     return KS_OK;
@@ -755,32 +1069,7 @@ int ks_gradient_reverse(ks_cache* c, const ks_model* m, const double* dstokes, i
         for (int64_t f = 0; f < frames; ++f)
             if (!c->fw_seeded[f]) return fail(KS_ERR_STATE, "frame %lld has no device seed: give dstokes or call a *_chi2_seed for every frame", (long long)f);
     }
-    const double* d_tn = c->dtn;
-    HIPCHK(launch_adj_combine(x->stream, c->npix, c->N, nt, d_tn + nt, nnu, c->seg, c->ads, c->awT, nullptr));
-    HIPCHK(x->stage_begin(KS_STAGE_ADJ_STEPS));
-    const bool mixed = c->precision == KS_PRECISION_MIXED;
-    if (mixed) HIPCHK(launch_adj_segment_f32(x->stream, c->N, items, c->awT, (const f32*)c->arec, (f32*)c->abar, c->aflg, (f32*)c->atail, samples_of(c),
-                                             c->npix));
-    else HIPCHK(launch_adj_segment(x->stream, c->N, items, c->awT, c->arec, c->abar, c->aflg, c->atail, samples_of(c), c->npix));
-    HIPCHK(x->stage_end(KS_STAGE_ADJ_STEPS));
-    if (ns > 0) {
-        HIPCHK(x->stage_begin(KS_STAGE_ADJ_ELEMENTS));
-        HIPCHK(launch_unit_bounds(x->stream, c->npix, c->N, samples_of(c), d_tn, nt, items, c->aflg, c->aub));
-        // the units in batches whose partial sums fit the workspace, each reduced into the running sums in order
-        const int64_t nunits = adj_units(items, c->N), nbmax = adj_unit_batch(nunits, ns);
-        for (int64_t u0 = 0; u0 < nunits; u0 += nbmax) {
-            const int64_t nb = std::min(nbmax, nunits - u0);
-            if (mixed)
-                HIPCHK(launch_adj_splats_f32(x->stream, m->pops, c->spin, c->thetao, c->soa, c->npix, c->N, samples_of(c), m->d_splats_f, m->d_adj_f, ns,
-                                             m->d_knots_f, m->d_ktimes_f, d_tn, nt, d_tn + nt, items, (const f32*)c->arec, (const f32*)c->abar, c->aflg,
-                                             u0, nb, c->aub, c->apart));
-            else
-                HIPCHK(launch_adj_splats(x->stream, m->pops, c->spin, c->thetao, c->soa, c->npix, c->N, samples_of(c), m->d_splats, m->d_adj, ns, m->d_knots,
-                                         m->d_ktimes, d_tn, nt, d_tn + nt, items, c->arec, c->abar, c->aflg, u0, nb, c->aub, c->apart));
-            HIPCHK(launch_adj_reduce(x->stream, c->apart, nb, ns, u0 > 0, c->adpar));
-        }
-        HIPCHK(x->stage_end(KS_STAGE_ADJ_ELEMENTS));
-    }
+    if (int rc = reverse_launches(c, m, c->dtn, nt, nnu)) return rc;
     std::vector<double> g((size_t)ns * ADJ_ROWS);
     if (ns > 0) HIPCHK(hipMemcpyAsync(g.data(), c->adpar, g.size() * sizeof(double), hipMemcpyDeviceToHost, x->stream));
     HIPCHK(hipStreamSynchronize(x->stream));
@@ -1067,6 +1356,75 @@ int ks_vis_seed(ks_cache* c, int64_t frame, double psize, const double* u, const
     return KS_OK;
 }
 
+int ks_closure_chi2_seed(ks_cache* c, int64_t frame, double psize, const double* u, const double* v, const double* taper, int nuv,
+                         int ntri, const int* triangles, const double* phases, const double* sigma_phase, int nquad, const int* quadrangles,
+                         const double* logamps, const double* sigma_logamp, int add, double* chi2) {
+    if (!c || !u || !v || !chi2) return fail(KS_ERR_ARG, "ks_closure_chi2_seed: null argument");
+    if (nuv < 1) return fail(KS_ERR_ARG, "need at least one (u, v) point");
+    if (ntri < 0 || nquad < 0 || ntri + nquad < 1) return fail(KS_ERR_ARG, "need at least one closure triangle or quadrangle");
+    if ((ntri > 0 && (!triangles || !phases || !sigma_phase)) || (nquad > 0 && (!quadrangles || !logamps || !sigma_logamp)))
+        return fail(KS_ERR_ARG, "ks_closure_chi2_seed: null closure data");
+    if (int rc = seed_frame_ok(c, frame)) return rc;
+    if (int rc = screen_ready(c)) return rc;
+    if (add && !c->fw_seeded[frame]) return fail(KS_ERR_STATE, "frame %lld has no seed to add to", (long long)frame);
+    // the per-baseline lists of the terms that use each baseline, in term order (the legs' order within a term)
+    std::vector<int> off(nuv + 1, 0), legs;
+    std::vector<std::pair<int, int>> uses;               // (baseline, entry)
+    for (int t = 0; t < ntri; ++t) {
+        if (!(sigma_phase[t] > 0.0)) return fail(KS_ERR_ARG, "closure phase sigma must be positive (triangle %d)", t);
+        for (int l = 0; l < 3; ++l) {
+            const int k = triangles[3 * t + l];
+            if (k == 0 || k > nuv || k < -nuv) return fail(KS_ERR_ARG, "triangle %d: baseline index %d outside 1:%d", t, k, nuv);
+            uses.push_back({std::abs(k) - 1, t | (k < 0 ? 0x40000000 : 0)});
+        }
+    }
+    for (int q = 0; q < nquad; ++q) {
+        if (!(sigma_logamp[q] > 0.0)) return fail(KS_ERR_ARG, "log closure amplitude sigma must be positive (quadrangle %d)", q);
+        for (int l = 0; l < 4; ++l) {
+            const int k = quadrangles[4 * q + l];
+            if (k == 0 || k > nuv || k < -nuv) return fail(KS_ERR_ARG, "quadrangle %d: baseline index %d outside 1:%d", q, k, nuv);
+            uses.push_back({std::abs(k) - 1, (ntri + q) | (l >= 2 ? 0x40000000 : 0)});
+        }
+    }
+    std::stable_sort(uses.begin(), uses.end(), [](const std::pair<int, int>& x, const std::pair<int, int>& y) { return x.first < y.first; });
+    legs.reserve(uses.size());
+    for (const auto& e : uses) { off[e.first + 1] += 1; legs.push_back(e.second); }
+    for (int k = 0; k < nuv; ++k) off[k + 1] += off[k];
+    std::vector<double> dat;
+    dat.insert(dat.end(), phases, phases + ntri); dat.insert(dat.end(), sigma_phase, sigma_phase + ntri);
+    dat.insert(dat.end(), logamps, logamps + nquad); dat.insert(dat.end(), sigma_logamp, sigma_logamp + nquad);
+    ks_ctx* x = c->ctx;
+    HIPCHK(hipSetDevice(x->device));
+    const int na = c->nalpha, nb = c->nbeta, nt = ntri + nquad;
+    if (int rc = vis_setup(c, psize, u, v, taper, nuv)) return rc;
+    if (c->ctri.grow((size_t)std::max(3 * ntri, 1)) || c->cquad.grow((size_t)std::max(4 * nquad, 1)) || c->coff.grow(off.size()) ||
+        c->clegs.grow(std::max(legs.size(), (size_t)1)) || c->cdat.grow(std::max(dat.size(), (size_t)1)) || c->cterm.grow((size_t)2 * nt) ||
+        c->vd.grow((size_t)8 * nuv) || c->vwq.grow((size_t)8 * nuv) || c->vE2.grow((size_t)2 * nuv * 4 * nb) || c->vSd.grow((size_t)na * 4 * nb) ||
+        c->vSp.grow((size_t)VIS_SPLITK * na * 4 * nb) || c->vsum.grow(1))
+        return KS_ERR_HIP;
+    if (ntri) HIPCHK(hipMemcpyAsync(c->ctri.p, triangles, (size_t)3 * ntri * sizeof(int), hipMemcpyHostToDevice, x->stream));
+    if (nquad) HIPCHK(hipMemcpyAsync(c->cquad.p, quadrangles, (size_t)4 * nquad * sizeof(int), hipMemcpyHostToDevice, x->stream));
+    HIPCHK(hipMemcpyAsync(c->coff.p, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, x->stream));
+    HIPCHK(hipMemcpyAsync(c->clegs.p, legs.data(), legs.size() * sizeof(int), hipMemcpyHostToDevice, x->stream));
+    HIPCHK(hipMemcpyAsync(c->cdat.p, dat.data(), dat.size() * sizeof(double), hipMemcpyHostToDevice, x->stream));
+    const double scale = psize * psize / JY;
+    const double* d_tap = taper ? c->vt.p : nullptr;
+    if (int rc = vis_forward(c, frame, 1, nuv, d_tap, scale)) return rc;
+    const double* d_cp = c->cdat.p;
+    HIPCHK(launch_closure_terms(x->stream, ntri, c->ctri.p, d_cp, d_cp + ntri, nquad, c->cquad.p, d_cp + 2 * ntri, d_cp + 2 * ntri + nquad, c->vV.p,
+                                c->cterm.p, c->cterm.p + nt));
+    HIPCHK(launch_closure_weights(x->stream, nuv, ntri, c->coff.p, c->clegs.p, c->cterm.p + nt, c->vV.p, c->vd.p));
+    HIPCHK(launch_vis_scale_weights(x->stream, nuv, c->vd.p, d_tap, scale, c->vwq.p));
+    HIPCHK(launch_vis_rotate(x->stream, nuv, nb, c->vwq.p, c->vB2.p, c->vE2.p));
+    HIPCHK(launch_dgemm_tn_splitk(x->stream, na, 4 * nb, 2 * nuv, c->vA2.p, na, c->vE2.p, 4 * nb, c->vSd.p, VIS_SPLITK, c->vSp.p));
+    HIPCHK(launch_vis_scatter_seed(x->stream, c->vSd.p, c->perm, c->npix, na, nb, frame, add, c->ads));
+    HIPCHK(launch_sum_fixed(x->stream, c->cterm.p, nt, c->vsum.p));
+    HIPCHK(hipMemcpyAsync(chi2, c->vsum.p, sizeof(double), hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    c->fw_seeded[frame] = 1;
+    return KS_OK;
+}
+
 // _frame_seed! (unbinned): χ² of one frame's image and the dual sweep's seed of that frame on the device
 int ks_image_chi2_seed(ks_cache* c, int64_t frame, const double* data, const double* sigma, int sigma_per_pixel, const uint8_t* mask,
                        double* chi2) {
@@ -1189,73 +1547,250 @@ int ks_image_chi2_seed_binned(ks_cache* c, int64_t frame, const ks_binning* b, c
     return KS_OK;
 }
 
-}  // extern "C"
-int ks_closure_chi2_seed(ks_cache* c, int64_t frame, double psize, const double* u, const double* v, const double* taper, int nuv,
-                         int ntri, const int* triangles, const double* phases, const double* sigma_phase, int nquad, const int* quadrangles,
-                         const double* logamps, const double* sigma_logamp, int add, double* chi2) {
-    if (!c || !u || !v || !chi2) return fail(KS_ERR_ARG, "ks_closure_chi2_seed: null argument");
-    if (nuv < 1) return fail(KS_ERR_ARG, "need at least one (u, v) point");
-    if (ntri < 0 || nquad < 0 || ntri + nquad < 1) return fail(KS_ERR_ARG, "need at least one closure triangle or quadrangle");
-    if ((ntri > 0 && (!triangles || !phases || !sigma_phase)) || (nquad > 0 && (!quadrangles || !logamps || !sigma_logamp)))
-        return fail(KS_ERR_ARG, "ks_closure_chi2_seed: null closure data");
-    if (int rc = seed_frame_ok(c, frame)) return rc;
-    if (int rc = screen_ready(c)) return rc;
-    if (add && !c->fw_seeded[frame]) return fail(KS_ERR_STATE, "frame %lld has no seed to add to", (long long)frame);
-    // the per-baseline lists of the terms that use each baseline, in term order (the legs' order within a term)
-    std::vector<int> off(nuv + 1, 0), legs;
-    std::vector<std::pair<int, int>> uses;               // (baseline, entry)
-    for (int t = 0; t < ntri; ++t) {
-        if (!(sigma_phase[t] > 0.0)) return fail(KS_ERR_ARG, "closure phase sigma must be positive (triangle %d)", t);
-        for (int l = 0; l < 3; ++l) {
-            const int k = triangles[3 * t + l];
-            if (k == 0 || k > nuv || k < -nuv) return fail(KS_ERR_ARG, "triangle %d: baseline index %d outside 1:%d", t, k, nuv);
-            uses.push_back({std::abs(k) - 1, t | (k < 0 ? 0x40000000 : 0)});
-        }
-    }
-    for (int q = 0; q < nquad; ++q) {
-        if (!(sigma_logamp[q] > 0.0)) return fail(KS_ERR_ARG, "log closure amplitude sigma must be positive (quadrangle %d)", q);
-        for (int l = 0; l < 4; ++l) {
-            const int k = quadrangles[4 * q + l];
-            if (k == 0 || k > nuv || k < -nuv) return fail(KS_ERR_ARG, "quadrangle %d: baseline index %d outside 1:%d", q, k, nuv);
-            uses.push_back({std::abs(k) - 1, (ntri + q) | (l >= 2 ? 0x40000000 : 0)});
-        }
-    }
-    std::stable_sort(uses.begin(), uses.end(), [](const std::pair<int, int>& x, const std::pair<int, int>& y) { return x.first < y.first; });
-    legs.reserve(uses.size());
-    for (const auto& e : uses) { off[e.first + 1] += 1; legs.push_back(e.second); }
-    for (int k = 0; k < nuv; ++k) off[k + 1] += off[k];
-    std::vector<double> dat;
-    dat.insert(dat.end(), phases, phases + ntri); dat.insert(dat.end(), sigma_phase, sigma_phase + ntri);
-    dat.insert(dat.end(), logamps, logamps + nquad); dat.insert(dat.end(), sigma_logamp, sigma_logamp + nquad);
+// ---- captured fit iterations (hipGraph) ----------------------------------------------------------------------
+// One fit iteration of a single-population model as one graph: the upload of the per-splat constants (made on the
+// host from the parameters into pinned staging), the transfer with the records over every (frame, frequency)
+// slice, the images, each slice's χ² and seed (image or visibility loss, data resident in graph-owned buffers), the
+// reverse pass, and the copies of the gradient rows and per-slice χ² into pinned host memory. A step replays it;
+// it is captured again when the model (its splat count), the cache's buffers or its launch choices have changed.
+struct ks_graph {
+    ks_cache* c = nullptr;
+    int nt = 0, nnu = 0, frames = 0;
+    double L = 0.0;
+    int loss = 0;                                      // KS_LOSS_IMAGE or KS_LOSS_VIS
+    Buf<double> dtn, data, sig, chif;
+    Buf<uint8_t> mask;
+    int sigma_per_pixel = 0;
+    // visibility loss: (u, v, taper), phase tables and per-slice workspace
+    int nuv = 0; double psize = 0.0; bool tap = false;
+    Buf<double> u, v, t, A2, B2, Im, C2, V, wq, E2, Sd, Sp, chi;
+    // pinned staging
+    SplatC* h_sp = nullptr; SplatAdj* h_ad = nullptr; SplatCf* h_spf = nullptr; SplatAdjf* h_adf = nullptr;
+    double* h_grad = nullptr; double* h_chi = nullptr;
+    int cap_ns = 0;
+    // what the captured graph refers to
+    const ks_model* m = nullptr; uint64_t m_id = 0;
+    std::vector<const void*> ptrs;
+    int gen = -1, prec = -1, chunk = -1, queue = -1, pa = -1, pb = -1;
+    hipGraph_t graph = nullptr; hipGraphExec_t exec = nullptr;
+    int64_t replays = 0, captures = 0;
+};
+
+static void graph_release_capture(ks_graph* g) {
+    if (g->exec) { (void)hipGraphExecDestroy(g->exec); g->exec = nullptr; }
+    if (g->graph) { (void)hipGraphDestroy(g->graph); g->graph = nullptr; }
+}
+static void graph_release_host(ks_graph* g) {
+    if (g->h_sp) (void)hipHostFree(g->h_sp);
+    if (g->h_ad) (void)hipHostFree(g->h_ad);
+    if (g->h_spf) (void)hipHostFree(g->h_spf);
+    if (g->h_adf) (void)hipHostFree(g->h_adf);
+    if (g->h_grad) (void)hipHostFree(g->h_grad);
+    g->h_sp = nullptr; g->h_ad = nullptr; g->h_spf = nullptr; g->h_adf = nullptr; g->h_grad = nullptr;
+    g->cap_ns = 0;
+}
+
+// the cache buffers a captured iteration reads or writes
+static std::vector<const void*> graph_ptrs(const ks_cache* c) {
+    return {c->seg, c->out, c->arec, c->abar, c->aflg, c->atail, c->awT, c->ads, c->apart, c->adpar, c->aub, c->tmap, c->soa,
+            c->qhead.p, c->qorder.p, c->kstop, c->aheads.p};
+}
+
+// the iteration's launches on the (capturing) stream
+static int graph_record(ks_graph* g, const ks_model* m) {
+    ks_cache* c = g->c;
     ks_ctx* x = c->ctx;
-    HIPCHK(hipSetDevice(x->device));
-    const int na = c->nalpha, nb = c->nbeta, nt = ntri + nquad;
-    if (int rc = vis_setup(c, psize, u, v, taper, nuv)) return rc;
-    if (c->ctri.grow((size_t)std::max(3 * ntri, 1)) || c->cquad.grow((size_t)std::max(4 * nquad, 1)) || c->coff.grow(off.size()) ||
-        c->clegs.grow(std::max(legs.size(), (size_t)1)) || c->cdat.grow(std::max(dat.size(), (size_t)1)) || c->cterm.grow((size_t)2 * nt) ||
-        c->vd.grow((size_t)8 * nuv) || c->vwq.grow((size_t)8 * nuv) || c->vE2.grow((size_t)2 * nuv * 4 * nb) || c->vSd.grow((size_t)na * 4 * nb) ||
-        c->vSp.grow((size_t)VIS_SPLITK * na * 4 * nb) || c->vsum.grow(1))
-        return KS_ERR_HIP;
-    if (ntri) HIPCHK(hipMemcpyAsync(c->ctri.p, triangles, (size_t)3 * ntri * sizeof(int), hipMemcpyHostToDevice, x->stream));
-    if (nquad) HIPCHK(hipMemcpyAsync(c->cquad.p, quadrangles, (size_t)4 * nquad * sizeof(int), hipMemcpyHostToDevice, x->stream));
-    HIPCHK(hipMemcpyAsync(c->coff.p, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, x->stream));
-    HIPCHK(hipMemcpyAsync(c->clegs.p, legs.data(), legs.size() * sizeof(int), hipMemcpyHostToDevice, x->stream));
-    HIPCHK(hipMemcpyAsync(c->cdat.p, dat.data(), dat.size() * sizeof(double), hipMemcpyHostToDevice, x->stream));
-    const double scale = psize * psize / JY;
-    const double* d_tap = taper ? c->vt.p : nullptr;
-    if (int rc = vis_forward(c, frame, 1, nuv, d_tap, scale)) return rc;
-    const double* d_cp = c->cdat.p;
-    HIPCHK(launch_closure_terms(x->stream, ntri, c->ctri.p, d_cp, d_cp + ntri, nquad, c->cquad.p, d_cp + 2 * ntri, d_cp + 2 * ntri + nquad, c->vV.p,
-                                c->cterm.p, c->cterm.p + nt));
-    HIPCHK(launch_closure_weights(x->stream, nuv, ntri, c->coff.p, c->clegs.p, c->cterm.p + nt, c->vV.p, c->vd.p));
-    HIPCHK(launch_vis_scale_weights(x->stream, nuv, c->vd.p, d_tap, scale, c->vwq.p));
-    HIPCHK(launch_vis_rotate(x->stream, nuv, nb, c->vwq.p, c->vB2.p, c->vE2.p));
-    HIPCHK(launch_dgemm_tn_splitk(x->stream, na, 4 * nb, 2 * nuv, c->vA2.p, na, c->vE2.p, 4 * nb, c->vSd.p, VIS_SPLITK, c->vSp.p));
-    HIPCHK(launch_vis_scatter_seed(x->stream, c->vSd.p, c->perm, c->npix, na, nb, frame, add, c->ads));
-    HIPCHK(launch_sum_fixed(x->stream, c->cterm.p, nt, c->vsum.p));
-    HIPCHK(hipMemcpyAsync(chi2, c->vsum.p, sizeof(double), hipMemcpyDeviceToHost, x->stream));
-    HIPCHK(hipStreamSynchronize(x->stream));
-    c->fw_seeded[frame] = 1;
+    hipStream_t s = x->stream;
+    const int ns = m->ns;
+    HIPCHK(hipMemcpyAsync(m->d_splats, g->h_sp, ns * sizeof(SplatC), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(m->d_adj, g->h_ad, ns * sizeof(SplatAdj), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(m->d_splats_f, g->h_spf, ns * sizeof(SplatCf), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(m->d_adj_f, g->h_adf, ns * sizeof(SplatAdjf), hipMemcpyHostToDevice, s));
+    const int nt = g->nt, nnu = g->nnu;
+    if (int rc = launch_transfer(c, m, g->dtn.p, nt, nnu, g->L, -1, 0.0, c->arec, c->aflg)) return rc;
+    HIPCHK(launch_seg_combine(s, c->soa, c->npix, c->N, c->M, c->chunk, nt, g->dtn.p + nt, nnu, c->seg, c->out));
+    const int na = c->nalpha, nb = c->nbeta;
+    for (int f = 0; f < g->frames; ++f) {
+        if (g->loss == KS_LOSS_IMAGE) {
+            const double* sg = g->sig.p + (g->sigma_per_pixel ? (int64_t)4 * c->npix * f : (int64_t)4 * f);
+            HIPCHK(launch_image_residual(s, c->out, c->perm, c->npix, f, g->data.p + (int64_t)4 * c->npix * f, sg, g->sigma_per_pixel,
+                                         g->mask.p ? g->mask.p + (int64_t)c->npix * f : nullptr, c->ads, g->chi.p));
+            HIPCHK(launch_sum_fixed(s, g->chi.p, c->npix, g->chif.p + f));
+        } else {
+            const VisWs w{g->u.p, g->v.p, g->t.p, g->A2.p, g->B2.p, g->Im.p, g->C2.p, g->V.p};
+            const double scale = g->psize * g->psize / JY;
+            const double* d_tap = g->tap ? g->t.p : nullptr;
+            if (int rc = vis_forward_ws(c, w, f, 1, g->nuv, d_tap, scale)) return rc;
+            HIPCHK(launch_vis_residual(s, g->nuv, g->V.p, g->data.p + (int64_t)8 * g->nuv * f, g->sig.p + (int64_t)4 * g->nuv * f, nullptr, d_tap,
+                                       scale, g->wq.p, g->chi.p));
+            HIPCHK(launch_vis_rotate(s, g->nuv, nb, g->wq.p, g->B2.p, g->E2.p));
+            HIPCHK(launch_dgemm_tn_splitk(s, na, 4 * nb, 2 * g->nuv, g->A2.p, na, g->E2.p, 4 * nb, g->Sd.p, VIS_SPLITK, g->Sp.p));
+            HIPCHK(launch_vis_scatter_seed(s, g->Sd.p, c->perm, c->npix, na, nb, f, 0, c->ads));
+            HIPCHK(launch_sum_fixed(s, g->chi.p, g->nuv, g->chif.p + f));
+        }
+    }
+    if (int rc = reverse_launches(c, m, g->dtn.p, nt, nnu)) return rc;
+    if (ns > 0) HIPCHK(hipMemcpyAsync(g->h_grad, c->adpar, (size_t)ns * ADJ_ROWS * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(g->h_chi, g->chif.p, (size_t)g->frames * sizeof(double), hipMemcpyDeviceToHost, s));
     return KS_OK;
 }
+
+// an eager iteration (every buffer allocated at its size), then the capture
+static int graph_capture(ks_graph* g, const ks_model* m) {
+    ks_cache* c = g->c;
+    ks_ctx* x = c->ctx;
+    graph_release_capture(g);
+    if (g->cap_ns < m->ns || !g->h_sp) {
+        graph_release_host(g);
+        const size_t n = (size_t)std::max(m->ns, 1);
+        HIPCHK(hipHostMalloc((void**)&g->h_sp, n * sizeof(SplatC), hipHostMallocDefault));
+        HIPCHK(hipHostMalloc((void**)&g->h_ad, n * sizeof(SplatAdj), hipHostMallocDefault));
+        HIPCHK(hipHostMalloc((void**)&g->h_spf, n * sizeof(SplatCf), hipHostMallocDefault));
+        HIPCHK(hipHostMalloc((void**)&g->h_adf, n * sizeof(SplatAdjf), hipHostMallocDefault));
+        HIPCHK(hipHostMalloc((void**)&g->h_grad, n * ADJ_ROWS * sizeof(double), hipHostMallocDefault));
+        g->cap_ns = (int)n;
+    }
+    const int64_t frames = g->frames, items = c->npix * frames;
+    if (ensure_out(c, frames) || ensure_adj(c, items, std::max(m->ns, 1))) return KS_ERR_HIP;
+    // the eager iteration: the constants already on the device, the same launches, a synchronization
+    HIPCHK(hipMemcpy(g->h_sp, m->d_splats, m->ns * sizeof(SplatC), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(g->h_ad, m->d_adj, m->ns * sizeof(SplatAdj), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(g->h_spf, m->d_splats_f, m->ns * sizeof(SplatCf), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(g->h_adf, m->d_adj_f, m->ns * sizeof(SplatAdjf), hipMemcpyDeviceToHost));
+    c->fw_chunk = c->chunk;
+    if (int rc = graph_record(g, m)) return rc;
+    HIPCHK(hipStreamSynchronize(x->stream));
+    // the capture
+    HIPCHK(hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal));
+    x->capturing = true;
+    const int rc = graph_record(g, m);
+    x->capturing = false;
+    hipGraph_t gr = nullptr;
+    const hipError_t e = hipStreamEndCapture(x->stream, &gr);
+    if (rc) { if (gr) (void)hipGraphDestroy(gr); return rc; }
+    if (e != hipSuccess) return fail(KS_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    g->graph = gr;
+    HIPCHK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    g->m = m; g->m_id = m->id;
+    g->ptrs = graph_ptrs(c);
+    g->gen = c->generation; g->prec = c->precision; g->chunk = c->chunk; g->queue = c->queue; g->pa = c->patch_a; g->pb = c->patch_b;
+    g->captures += 1;
+    return KS_OK;
+}
+
+int ks_fit_graph_create(ks_cache* c, int nt, const double* times, int nnu, const double* nus, double L, int loss, const double* data,
+                        const double* sigma, int sigma_per_pixel, const uint8_t* mask, double psize, const double* u, const double* v,
+                        const double* taper, int nuv, ks_graph** out) {
+    if (!c || !times || !nus || !data || !sigma || !out) return fail(KS_ERR_ARG, "ks_fit_graph_create: null argument");
+    if (c->generation < 1) return fail(KS_ERR_STATE, "regenerate! the cache first");
+    if (!has_samples(c)) return fail(KS_ERR_ARG, "the cache holds no stored samples: build it with store_samples = true and a storing marcher");
+    if (nt < 1 || nnu < 1 || !(L > 0.0)) return fail(KS_ERR_ARG, "need times, frequencies and a positive length unit");
+    for (int i = 0; i < nnu; ++i) if (!(nus[i] > 0.0)) return fail(KS_ERR_ARG, "frequencies must be positive");
+    if (loss != KS_LOSS_IMAGE && loss != KS_LOSS_VIS) return fail(KS_ERR_ARG, "unknown loss %d", loss);
+    if (loss == KS_LOSS_VIS && (!u || !v || nuv < 1)) return fail(KS_ERR_ARG, "the visibility loss needs (u, v) points");
+    if (int rc = screen_ready(c)) return rc;
+    const int frames = nt * nnu;
+    const int64_t nsig = loss == KS_LOSS_VIS ? (int64_t)4 * nuv * frames : sigma_per_pixel ? (int64_t)4 * c->npix * frames : (int64_t)4 * frames;
+    for (int64_t k = 0; k < nsig; ++k) {
+        const bool masked = loss == KS_LOSS_IMAGE && mask && sigma_per_pixel && !mask[k / 4];
+        if (!(sigma[k] > 0.0) && !masked) return fail(KS_ERR_ARG, "sigma must be positive (entry %lld)", (long long)k);
+    }
+    ks_ctx* x = c->ctx;
+    HIPCHK(hipSetDevice(x->device));
+    ks_graph* g = new ks_graph;
+    g->c = c; g->nt = nt; g->nnu = nnu; g->frames = frames; g->L = L; g->loss = loss; g->sigma_per_pixel = sigma_per_pixel;
+    std::vector<double> tn(times, times + nt);
+    tn.insert(tn.end(), nus, nus + nnu);
+    const int64_t ndata = loss == KS_LOSS_VIS ? (int64_t)8 * nuv * frames : (int64_t)4 * c->npix * frames;
+    int rc = 0;
+    rc |= g->dtn.grow(tn.size()) | g->data.grow(ndata) | g->sig.grow(nsig) | g->chif.grow(frames);
+    if (hipHostMalloc((void**)&g->h_chi, frames * sizeof(double), hipHostMallocDefault) != hipSuccess) rc = KS_ERR_HIP;
+    if (loss == KS_LOSS_IMAGE) {
+        rc |= g->chi.grow(c->npix);
+        if (mask) rc |= g->mask.grow((size_t)c->npix * frames);
+    } else {
+        const int na = c->nalpha, nb = c->nbeta;
+        g->nuv = nuv; g->psize = psize; g->tap = taper != nullptr;
+        rc |= g->u.grow(nuv) | g->v.grow(nuv) | g->t.grow(nuv) | g->A2.grow((size_t)2 * nuv * na) | g->B2.grow((size_t)2 * nuv * nb) |
+              g->Im.grow((size_t)na * 4 * nb) | g->C2.grow((size_t)2 * nuv * 4 * nb) | g->V.grow((size_t)8 * nuv) | g->wq.grow((size_t)8 * nuv) |
+              g->E2.grow((size_t)2 * nuv * 4 * nb) | g->Sd.grow((size_t)na * 4 * nb) | g->Sp.grow((size_t)VIS_SPLITK * na * 4 * nb) | g->chi.grow(nuv);
+    }
+    if (rc) { ks_fit_graph_destroy(g); return fail(KS_ERR_HIP, "ks_fit_graph_create: device allocation failed"); }
+#define HIPCHK_G(e) do { hipError_t e_ = (e); if (e_ != hipSuccess) { ks_fit_graph_destroy(g); return fail(KS_ERR_HIP, "%s: %s", #e, hipGetErrorString(e_)); } } while (0)
+    HIPCHK_G(hipMemcpy(g->dtn.p, tn.data(), tn.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK_G(hipMemcpy(g->data.p, data, ndata * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK_G(hipMemcpy(g->sig.p, sigma, nsig * sizeof(double), hipMemcpyHostToDevice));
+    if (loss == KS_LOSS_IMAGE && mask) HIPCHK_G(hipMemcpy(g->mask.p, mask, (size_t)c->npix * frames, hipMemcpyHostToDevice));
+    if (loss == KS_LOSS_VIS) {
+        const VisWs w{g->u.p, g->v.p, g->t.p, g->A2.p, g->B2.p, g->Im.p, g->C2.p, g->V.p};
+        if (int r2 = vis_tables_ws(c, w, psize, u, v, taper, nuv)) { ks_fit_graph_destroy(g); return r2; }
+        HIPCHK_G(hipStreamSynchronize(x->stream));
+    }
+#undef HIPCHK_G
+    *out = g;
+    return KS_OK;
+}
+
+int ks_fit_graph_step(ks_graph* g, const ks_model* m, const double* params, double* chi2, double* dparams) {
+    if (!g || !m || !chi2 || !dparams) return fail(KS_ERR_ARG, "ks_fit_graph_step: null argument");
+    ks_cache* c = g->c;
+    if (m->ctx != c->ctx) return fail(KS_ERR_ARG, "the model belongs to another context");
+    if (m->kind != KS_MODEL_THERMAL && m->kind != KS_MODEL_POWERLAW && m->kind != KS_MODEL_KAPPA && m->kind != KS_MODEL_KNOT)
+        return fail(KS_ERR_ARG, "a fit graph takes one thermal, power-law, kappa or knot population (model kind %d)", m->kind);
+    ks_ctx* x = c->ctx;
+    HIPCHK(hipSetDevice(x->device));
+    const bool stale = !g->exec || m != g->m || m->id != g->m_id || graph_ptrs(c) != g->ptrs || c->generation != g->gen ||
+                       c->precision != g->prec || c->chunk != g->chunk || c->queue != g->queue || c->patch_a != g->pa || c->patch_b != g->pb;
+    if (stale)
+        if (int rc = graph_capture(g, m)) return rc;
+    // the constants of the new parameters into the pinned staging the graph uploads from
+    if (params) {
+        const bool with_knots = m->nknots >= 1 && !m->h_knots.empty();
+        for (int i = 0; i < m->ns; ++i) {
+            host_splat_constants(m->kind, m->nrows, i, params + (size_t)m->nrows * i, m->gamma_max, with_knots, with_knots ? m->h_knots.data() : nullptr,
+                                 m->nknots, g->h_sp[i], g->h_ad[i]);
+            splat_as(g->h_sp[i], g->h_spf[i]);
+            adj_as(g->h_ad[i], g->h_adf[i]);
+        }
+    }
+    HIPCHK(hipGraphLaunch(g->exec, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    g->replays += 1;
+    // the forward pass the graph ran is the cache's: images and seeds as after ks_gradient_forward + the seeds
+    c->fw_model = m->id; c->fw_nt = g->nt; c->fw_nnu = g->nnu; c->fw_generation = c->generation; c->fw_precision = c->precision;
+    c->fw_chunk = c->chunk;
+    c->fw_seeded.assign((size_t)g->frames, 1);
+    double s = 0.0;
+    for (int f = 0; f < g->frames; ++f) s += g->h_chi[f];
+    *chi2 = s;
+    size_t off = 0;
+    for (int i = 0; i < m->ns; ++i) {
+        for (int r = 0; r < m->nrows; ++r) dparams[off + r] += g->h_grad[(size_t)i * ADJ_ROWS + r];
+        off += m->nrows;
+    }
+    return KS_OK;
+}
+
+int ks_fit_graph_info(const ks_graph* g, int64_t* captures, int64_t* replays) {
+    if (!g) return fail(KS_ERR_ARG, "null graph");
+    if (captures) *captures = g->captures;
+    if (replays) *replays = g->replays;
+    return KS_OK;
+}
+
+int ks_fit_graph_destroy(ks_graph* g) {
+    if (!g) return KS_OK;
+    (void)hipSetDevice(g->c->ctx->device);
+    graph_release_capture(g);
+    graph_release_host(g);
+    if (g->h_chi) (void)hipHostFree(g->h_chi);
+    for (Buf<double>* b : {&g->dtn, &g->data, &g->sig, &g->chif, &g->u, &g->v, &g->t, &g->A2, &g->B2, &g->Im, &g->C2, &g->V, &g->wq, &g->E2,
+                           &g->Sd, &g->Sp, &g->chi})
+        b->release();
+    g->mask.release();
+    delete g;
+    return KS_OK;
+}
+
+}  // extern "C"
 

@@ -9,26 +9,36 @@
 // the critical curve) a re-anchoring of the states and of t~, phi to the closed forms.
 #pragma once
 #include "ks_dgeo.hpp"
+#include "ks_ddual.hpp"
 
 namespace ksd {
 
-struct JS { double sn, cn, dn; };
+// The march is written once as templates on its bulk scalar B: double (the plain FP64 march) or f32 (the bulk of
+// the per-sample work in FP32: the Jacobi addition-theorem steps, the radius and polar angle, the integrand rates
+// and the Simpson increments). The closed-form anchors (Jacobi functions, t~ and phi), the quadrature constants, the
+// visibility constants, the Mino times and the running t~ / phi sums stay FP64 for both.
+template <class B> struct JST { B sn, cn, dn; };
+using JS = JST<double>;
 KS_DEV JS js_at(double u, double m) { Jac j = jac_pos(u, m); return {j.sn, j.cn, j.dn}; }
-KS_DEV JS js_step(const JS& x, const JS& d, double m) {
-    double den = 1.0 / (1.0 - m * (x.sn * x.sn) * (d.sn * d.sn));
+template <class B>
+KS_DEV JST<B> js_step(const JST<B>& x, const JST<B>& d, B m) {
+    B den = 1.0 / (1.0 - m * (x.sn * x.sn) * (d.sn * d.sn));
     return {(x.sn * d.cn * d.dn + d.sn * x.cn * x.dn) * den,
             (x.cn * d.cn - x.sn * x.dn * d.sn * d.dn) * den,
             (x.dn * d.dn - m * x.sn * x.cn * d.sn * d.cn) * den};
 }
 KS_DEV JS js_step_constants(double delta, double m) {
     JS x = js_at(fabs(delta), m);
-    return {copysign(x.sn, delta), x.cn, x.dn};
+    return {::copysign(x.sn, delta), x.cn, x.dn};
 }
+template <class B> KS_DEV JST<B> js_as(const JS& x) { return {B(x.sn), B(x.cn), B(x.dn)}; }
 
-struct QC {   // quadrature constants (quadrature.jl:42-97)
-    double sp, sm, Qt, Qtm, Qph, Qphm, C, nq, sqrt_eps, gc, mpp;
+template <class R>
+struct QCT {   // quadrature constants (quadrature.jl:42-97)
+    R sp, sm, Qt, Qtm, Qph, Qphm, C, nq, sqrt_eps, gc, mpp;
     bool scattering;
 };
+using QC = QCT<double>;
 KS_DEV QC make_qc(const PixC& p, const Spin& S) {
     QC q;
     const double a = S.a, lam = p.lam;
@@ -47,37 +57,56 @@ KS_DEV QC make_qc(const PixC& p, const Spin& S) {
     q.mpp = p.mth / (1.0 - p.mth);
     return q;
 }
+template <class B> KS_DEV QCT<B> qc_as(const QC& q) {
+    return {B(q.sp), B(q.sm), B(q.Qt), B(q.Qtm), B(q.Qph), B(q.Qphm), B(q.C), B(q.nq), B(q.sqrt_eps), B(q.gc), B(q.mpp), q.scattering};
+}
+template <class B> KS_DEV SpinT<B> spin_as(const Spin& S) { return {B(S.a), B(S.rp), B(S.rm)}; }
+// the pixel constants in the bulk scalar (the flags and root case unchanged)
+template <class B> KS_DEV PixCT<B> pix_as(const PixC& p) {
+    PixCT<B> b;
+    b.alpha = B(p.alpha); b.beta = B(p.beta); b.lam = B(p.lam); b.eta = B(p.eta); b.rcase = p.rcase; b.nreal = p.nreal;
+    b.k = B(p.k); b.c = B(p.c); b.fo = B(p.fo); b.tau_total = B(p.tau_total); b.tau_h = B(p.tau_h);
+    b.r1 = B(p.r1); b.r3 = B(p.r3); b.r4 = B(p.r4); b.n = B(p.n); b.n0 = B(p.n0); b.n1 = B(p.n1); b.d0 = B(p.d0); b.d1 = B(p.d1);
+    b.a2 = B(p.a2); b.b1 = B(p.b1); b.g0 = B(p.g0); b.psi0 = B(p.psi0);
+    b.Pp = B(p.Pp); b.Pm = B(p.Pm); b.Qp = B(p.Qp); b.Qm = B(p.Qm); b.It0 = B(p.It0); b.Iph0 = B(p.Iph0);
+    b.up = B(p.up); b.um = B(p.um); b.f = B(p.f); b.mth = B(p.mth); b.off = B(p.off); b.sgn = B(p.sgn); b.npol = B(p.npol);
+    b.Xth0 = B(p.Xth0); b.Gt0 = B(p.Gt0); b.Gphi0 = B(p.Gphi0); b.vortical = p.vortical; b.mu = B(p.mu); b.scale = B(p.scale); b.K2 = B(p.K2);
+    return b;
+}
 
-struct RP { double r, th, cth, s, sn, cn, gt, gph; bool nur, nuth; };
+template <class B> struct RPT { B r, th, cth, s, sn, cn, gt, gph; bool nur, nuth; };
+using RP = RPT<double>;
 
-KS_DEV void smooth_integrands(const PixC& p, const Spin& S, const QC& q, double r, double cth, double s, double sn, double cn, double dn,
-                              double& gt, double& gph) {
-    const double a = S.a, lam = p.lam;
-    double D = r * r - 2.0 * r + a * a;
-    double s2 = 1.0 - cth * cth;
-    double iD = 1.0 / D;
-    double ft = (r * r + a * a) * (r * r + a * a - a * lam) * iD + a * (lam - a * s2);
-    double fph = a * (2.0 * r - a * lam) * iD;
-    double wp = s * S.rp / (r * (r - S.rp)) / q.sp;
-    double wm = s * S.rm / (r * (r - S.rm)) / q.sm;
+template <class B>
+KS_DEV void smooth_integrands(const PixCT<B>& p, const SpinT<B>& S, const QCT<B>& q, B r, B cth, B s, B sn, B cn, B dn, B& gt, B& gph) {
+    const B a = S.a, lam = p.lam;
+    B D = r * r - 2.0 * r + a * a;
+    B s2 = 1.0 - cth * cth;
+    B iD = 1.0 / D;
+    B ft = (r * r + a * a) * (r * r + a * a - a * lam) * iD + a * (lam - a * s2);
+    B fph = a * (2.0 * r - a * lam) * iD;
+    B wp = s * S.rp / (r * (r - S.rp)) / q.sp;
+    B wm = s * S.rm / (r * (r - S.rm)) / q.sm;
     gt = ft - fabs(s) * (1.0 + 2.0 / r) - q.Qt * wp - q.Qtm * wm;
-    double gpol = lam * q.C * (1.0 - q.gc * dn * (1.0 - q.mpp * cn * cn * 0.5)) / (1.0 - q.nq * sn * sn);
+    B gpol = lam * q.C * (1.0 - q.gc * dn * (1.0 - q.mpp * cn * cn * 0.5)) / (1.0 - q.nq * sn * sn);
     gph = fph - q.Qph * wp - q.Qphm * wm + gpol;
 }
 
-KS_DEV RP ray_point(const PixC& p, const Spin& S, const QC& q, const JS& xr, const JS& xt, double tau) {
-    RP o;
-    o.r = radius_jac(p, xr.sn, xr.cn);
-    polar_from_jac(p, xt.sn, xt.cn, xt.dn, o.th, o.cth, o.nuth);
-    o.nur = before_turning<double>(tau, p.fo);
-    double R = r_potential(S.a, p.lam, p.eta, o.r);
-    const double w = o.r * o.r + S.a * S.a - S.a * p.lam;
+// tau, fo: the Mino time of the point and of the radial turning point (FP64: they decide nu_r)
+template <class B>
+KS_DEV RPT<B> ray_point(const PixCT<B>& p, const SpinT<B>& S, const QCT<B>& q, const JST<B>& xr, const JST<B>& xt, double tau, double fo) {
+    RPT<B> o;
+    o.r = radius_jac<B>(p, xr.sn, xr.cn);
+    polar_from_jac<B>(p, xt.sn, xt.cn, xt.dn, o.th, o.cth, o.nuth);
+    o.nur = before_turning<double>(tau, fo);
+    B R = r_potential<B>(S.a, p.lam, p.eta, o.r);
+    const B w = o.r * o.r + S.a * S.a - S.a * p.lam;
     // R below 1e-14 w^2 is rounding (a sample on the radial turning point, where R = 0): s = 0 there
-    o.s = (o.nur ? -1.0 : 1.0) * (R > 1e-14 * (w * w) ? sqrt(R) : 0.0);
-    double idn = 1.0 / xt.dn;
+    o.s = (o.nur ? -1.0 : 1.0) * (R > 1e-14 * (w * w) ? sqrt(R) : B(0.0));
+    B idn = 1.0 / xt.dn;
     o.sn = xt.sn * idn / p.scale;
     o.cn = xt.cn * idn;
-    smooth_integrands(p, S, q, o.r, o.cth, o.s, o.sn, o.cn, idn, o.gt, o.gph);
+    smooth_integrands<B>(p, S, q, o.r, o.cth, o.s, o.sn, o.cn, idn, o.gt, o.gph);
     return o;
 }
 KS_COLD JS radial_state(const PixC& p, double tau) { return js_at(p.c * (p.fo - tau), p.k); }
@@ -93,44 +122,51 @@ KS_COLD RP ray_point_turning(const PixC& p, const Spin& S, const QC& q) {
     double idn = 1.0 / xt.dn;
     o.sn = xt.sn * idn / p.scale;
     o.cn = xt.cn * idn;
-    smooth_integrands(p, S, q, o.r, o.cth, o.s, o.sn, o.cn, idn, o.gt, o.gph);
+    smooth_integrands<double>(p, S, q, o.r, o.cth, o.s, o.sn, o.cn, idn, o.gt, o.gph);
     return o;
 }
+template <class B> KS_DEV RPT<B> rp_as(const RP& o) {
+    return {B(o.r), B(o.th), B(o.cth), B(o.s), B(o.sn), B(o.cn), B(o.gt), B(o.gph), o.nur, o.nuth};
+}
 
-KS_DEV double polar_closed_form(const PixC& p, const QC& q, double tau, double sn, double cn) {
-    double X = (tau + p.off) / p.f;
-    double nw = floor((X + p.K2 * 0.5) / p.K2);
-    double A = (atan(q.sqrt_eps * sn / cn) + nw * PI) / q.sqrt_eps;
-    double ph = atan(sn / cn) + nw * PI;
-    double eps = q.sqrt_eps * q.sqrt_eps;
+template <class B>
+KS_DEV B polar_closed_form(const PixCT<B>& p, const QCT<B>& q, double tau, B sn, B cn) {
+    B X = (B(tau) + p.off) / p.f;
+    B nw = floor((X + p.K2 * 0.5) / p.K2);
+    B A = (atan(q.sqrt_eps * sn / cn) + nw * PI) / q.sqrt_eps;
+    B ph = atan(sn / cn) + nw * PI;
+    B eps = q.sqrt_eps * q.sqrt_eps;
     return p.lam * q.C * p.f * q.gc * ((1.0 + q.mpp * eps / (2.0 * q.nq)) * A - q.mpp / (2.0 * q.nq) * ph);
 }
 
 // Pa, Pb: polar_closed_form at the interval's ends (each end's value serves both intervals that share it)
-KS_DEV void interval_increment(const PixC& p, const Spin& S, const QC& q, const RP& pa, const RP& pm, const RP& pb, double ta, double tb,
-                               double Pa, double Pb, double& It, double& Iph) {
-    double h = tb - ta;
-    double ra = pa.r, rb = pb.r;
-    double dFr = (rb - ra) + 2.0 * log(rb / ra);
-    double dFp = log((rb - S.rp) * ra / ((ra - S.rp) * rb)) / q.sp;
-    double dFm = log((rb - S.rm) * ra / ((ra - S.rm) * rb)) / q.sm;
+template <class B>
+KS_DEV void interval_increment(const PixCT<B>& p, const SpinT<B>& S, const QCT<B>& q, const RPT<B>& pa, const RPT<B>& pm, const RPT<B>& pb,
+                               double ta, double tb, B Pa, B Pb, B& It, B& Iph) {
+    B h = B(tb - ta);
+    B ra = pa.r, rb = pb.r;
+    B dFr = (rb - ra) + 2.0 * log(rb / ra);
+    B dFp = log((rb - S.rp) * ra / ((ra - S.rp) * rb)) / q.sp;
+    B dFm = log((rb - S.rm) * ra / ((ra - S.rm) * rb)) / q.sm;
     It = h * (1.0 / 6.0) * (pa.gt + 4.0 * pm.gt + pb.gt) + fabs(dFr) + q.Qt * dFp + q.Qtm * dFm;
     Iph = h * (1.0 / 6.0) * (pa.gph + 4.0 * pm.gph + pb.gph) + q.Qph * dFp + q.Qphm * dFm + Pb - Pa;
 }
 
-// the interval holding the radial turning point of a scattering ray, split there (quadrature.jl:255-270)
-KS_COLD void turning_increment(const PixC& p, const Spin& S, const QC& q, const RP& pa, const RP& pb, double ta, double tb, double Pa, double Pb,
-                               double& t, double& ph) {
-    RP pt = ray_point_turning(p, S, q);
-    const double Pt = polar_closed_form(p, q, p.fo, pt.sn, pt.cn);
+// the interval holding the radial turning point of a scattering ray, split there (quadrature.jl:255-270); the
+// turning point and the two midpoints come from the closed forms (FP64), the increments in the bulk scalar
+template <class B>
+KS_COLD void turning_increment(const PixC& p, const Spin& S, const QC& q, const PixCT<B>& pb_, const SpinT<B>& Sb, const QCT<B>& qb,
+                               const RPT<B>& pa, const RPT<B>& pb, double ta, double tb, B Pa, B Pb, double& t, double& ph) {
+    const RPT<B> pt = rp_as<B>(ray_point_turning(p, S, q));
+    const B Pt = polar_closed_form<B>(pb_, qb, p.fo, pt.sn, pt.cn);
     double t1 = 0.5 * (ta + p.fo), t2 = 0.5 * (p.fo + tb);
-    RP p1 = ray_point(p, S, q, radial_state(p, t1), polar_state(p, t1), t1);
-    RP p2 = ray_point(p, S, q, radial_state(p, t2), polar_state(p, t2), t2);
-    double It1, Ip1, It2, Ip2;
-    interval_increment(p, S, q, pa, p1, pt, ta, p.fo, Pa, Pt, It1, Ip1);
-    interval_increment(p, S, q, pt, p2, pb, p.fo, tb, Pt, Pb, It2, Ip2);
-    t += It1 + It2;
-    ph += Ip1 + Ip2;
+    RPT<B> p1 = ray_point<B>(pb_, Sb, qb, js_as<B>(radial_state(p, t1)), js_as<B>(polar_state(p, t1)), t1, p.fo);
+    RPT<B> p2 = ray_point<B>(pb_, Sb, qb, js_as<B>(radial_state(p, t2)), js_as<B>(polar_state(p, t2)), t2, p.fo);
+    B It1, Ip1, It2, Ip2;
+    interval_increment<B>(pb_, Sb, qb, pa, p1, pt, ta, p.fo, Pa, Pt, It1, Ip1);
+    interval_increment<B>(pb_, Sb, qb, pt, p2, pb, p.fo, tb, Pt, Pb, It2, Ip2);
+    t += val(It1) + val(It2);
+    ph += val(Ip1) + val(Ip2);
 }
 
 KS_DEV int anchor_interval(int M, double one_minus_k) {
@@ -151,10 +187,14 @@ KS_DEV int anchor_interval(int M, double one_minus_k) {
 // anchor kbeg is not a valid sample, t~ and phi carry the quadrature from the last valid anchor
 // before it, and the march starts from that anchor (or from sample 1, always a closed-form start)
 // and runs without calling f up to kbeg.
-template <class F>
-KS_DEV void march_range(const PixC& p, const Spin& S, double thetao, int N, int M, int kbeg, int kend, F& f, double& resid_t, double& resid_ph) {
-    const QC q = make_qc(p, S);
-    const VisC vis = vis_constants(p, S.a, thetao);
+// B: the bulk scalar (above). The running t~ and phi are FP64 sums of the B-scalar increments between the FP64
+// anchors; the samples handed to f are FP64 (converted from B).
+// (pb_, Sb, qb: the pixel, spin and quadrature constants in B; for B = double the FP64 ones themselves. Krang's
+// visibility rule takes the polar angle in FP64: its cos^2 theta_o - cos^2 theta cancels near the threshold.)
+template <class B, class F>
+KS_DEV void march_range_b(const PixC& p, const Spin& S, const QC& q, const VisC& vis, const PixCT<B>& pb_, const SpinT<B>& Sb, const QCT<B>& qb,
+                          int N, int M, int kbeg, int kend, F& f, double& resid_t, double& resid_ph) {
+    const B kb = B(p.k), mub = B(p.mu);
     const double dtau = p.tau_total / (double)(N + 1);
     const double h = 0.5 * dtau;
     // the radial turning point splits the interval that holds it (ta < fo < tb). With N + 1 even, fo = tau_total / 2
@@ -162,27 +202,27 @@ KS_DEV void march_range(const PixC& p, const Spin& S, double thetao, int N, int 
     // k dtau; a turning point within 1e-9 dtau of a sample is taken as on it (no split, the exact-arithmetic case)
     const double tol = 1e-9 * dtau;
     const int Man = anchor_interval(M, 1.0 - p.k);
-    const JS Dr = js_step_constants(-p.c * h, p.k);
-    const JS Dt = js_step_constants(p.scale * h / p.f, p.mu);
+    const JST<B> Dr = js_as<B>(js_step_constants(-p.c * h, p.k));
+    const JST<B> Dt = js_as<B>(js_step_constants(p.scale * h / p.f, p.mu));
     resid_t = 0.0; resid_ph = 0.0;
     // start: the last anchor A <= kbeg whose sample is valid, or sample 1
     int A = kbeg;
-    JS xr, xt;
-    RP pa;
+    JST<B> xr, xt;
+    RPT<B> pa;
     for (;;) {
         const double ta = (double)A * dtau;
-        xr = radial_state(p, ta);
-        xt = polar_state(p, ta);
-        pa = ray_point(p, S, q, xr, xt, ta);
-        if (A <= 1 || ((ta <= p.tau_h) && krang_visible(vis, pa.cth))) break;
+        xr = js_as<B>(radial_state(p, ta));
+        xt = js_as<B>(polar_state(p, ta));
+        pa = ray_point<B>(pb_, Sb, qb, xr, xt, ta, p.fo);
+        if (A <= 1 || ((ta <= p.tau_h) && krang_visible(vis, val(pa.cth)))) break;
         A = A - Man > 1 ? A - Man : 1;
     }
     double t, ph;
     t_phi_direct(p, S, (double)A * dtau, t, ph);
-    double Pa = polar_closed_form(p, q, (double)A * dtau, pa.sn, pa.cn);
+    B Pa = polar_closed_form<B>(pb_, qb, (double)A * dtau, pa.sn, pa.cn);
     if (A == kbeg) {
         const double ta = (double)A * dtau;
-        Samp s{t, pa.r, pa.th, ph, pa.nur, pa.nuth, (ta <= p.tau_h) && krang_visible(vis, pa.cth)};
+        Samp s{t, val(pa.r), val(pa.th), ph, pa.nur, pa.nuth, (ta <= p.tau_h) && krang_visible(vis, val(pa.cth))};
         f(A, s, dtau);
     }
     const int klast = kend < N ? kend + 1 : N;
@@ -190,22 +230,22 @@ KS_DEV void march_range(const PixC& p, const Spin& S, double thetao, int N, int 
         const double ta = (double)(k - 1) * dtau, tb = (double)k * dtau;
         const bool anchor = ((k - 1) % Man) == 0;
         if (k > kend && !anchor) break;            // the step past kend only serves the next anchor's residual
-        xr = js_step(xr, Dr, p.k);
-        xt = js_step(xt, Dt, p.mu);
-        RP pmid = ray_point(p, S, q, xr, xt, ta + h);
-        if (anchor) { xr = radial_state(p, tb); xt = polar_state(p, tb); }
-        else { xr = js_step(xr, Dr, p.k); xt = js_step(xt, Dt, p.mu); }
-        RP pb = ray_point(p, S, q, xr, xt, tb);
-        const double Pb = polar_closed_form(p, q, tb, pb.sn, pb.cn);
+        xr = js_step<B>(xr, Dr, kb);
+        xt = js_step<B>(xt, Dt, mub);
+        RPT<B> pmid = ray_point<B>(pb_, Sb, qb, xr, xt, ta + h, p.fo);
+        if (anchor) { xr = js_as<B>(radial_state(p, tb)); xt = js_as<B>(polar_state(p, tb)); }
+        else { xr = js_step<B>(xr, Dr, kb); xt = js_step<B>(xt, Dt, mub); }
+        RPT<B> pb = ray_point<B>(pb_, Sb, qb, xr, xt, tb, p.fo);
+        const B Pb = polar_closed_form<B>(pb_, qb, tb, pb.sn, pb.cn);
         if (q.scattering && p.fo - ta > tol && tb - p.fo > tol) {
-            turning_increment(p, S, q, pa, pb, ta, tb, Pa, Pb, t, ph);
+            turning_increment<B>(p, S, q, pb_, Sb, qb, pa, pb, ta, tb, Pa, Pb, t, ph);
         } else {
-            double It, Ip;
-            interval_increment(p, S, q, pa, pmid, pb, ta, tb, Pa, Pb, It, Ip);
-            t += It;
-            ph += Ip;
+            B It, Ip;
+            interval_increment<B>(pb_, Sb, qb, pa, pmid, pb, ta, tb, Pa, Pb, It, Ip);
+            t += val(It);
+            ph += val(Ip);
         }
-        const bool okb = (tb <= p.tau_h) && krang_visible(vis, pb.cth);
+        const bool okb = (tb <= p.tau_h) && krang_visible(vis, val(pb.cth));
         if (anchor && okb) {
             // closed-form anchor; Krang's ok for the sample decides whether it resets (quadrature.jl:283-291)
             double td, pd;
@@ -217,12 +257,27 @@ KS_DEV void march_range(const PixC& p, const Spin& S, double thetao, int N, int 
             t = td; ph = pd;
         }
         if (k >= kbeg && k <= kend) {
-            Samp s{t, pb.r, pb.th, ph, pb.nur, pb.nuth, okb};
+            Samp s{t, val(pb.r), val(pb.th), ph, pb.nur, pb.nuth, okb};
             f(k, s, dtau);
         }
         pa = pb;
         Pa = Pb;
     }
+}
+template <class B, class F>
+KS_DEV void march_range(const PixC& p, const Spin& S, double thetao, int N, int M, int kbeg, int kend, F& f, double& resid_t, double& resid_ph) {
+    const QC q = make_qc(p, S);
+    const VisC vis = vis_constants(p, S.a, thetao);
+    if constexpr (std::is_same<B, double>::value) {
+        march_range_b<double, F>(p, S, q, vis, p, S, q, N, M, kbeg, kend, f, resid_t, resid_ph);
+    } else {
+        const PixCT<B> pb_ = pix_as<B>(p);
+        march_range_b<B, F>(p, S, q, vis, pb_, spin_as<B>(S), qc_as<B>(q), N, M, kbeg, kend, f, resid_t, resid_ph);
+    }
+}
+template <class F>
+KS_DEV void march_range(const PixC& p, const Spin& S, double thetao, int N, int M, int kbeg, int kend, F& f, double& resid_t, double& resid_ph) {
+    march_range<double, F>(p, S, thetao, N, M, kbeg, kend, f, resid_t, resid_ph);
 }
 
 // March one whole ray (samples 1..N). Returns the largest anchor residuals.

@@ -532,13 +532,21 @@ def fit_loop(params, make_valgrad, stages, hygiene=None, callback=None):
     return params, history, events
 
 
-def fit(params, movie, cache, L, stages, hygiene=None, rng=None, callback=None, nmax=-1, slab=0.0, binning=None, kind="thermal"):
+def fit(params, movie, cache, L, stages, hygiene=None, rng=None, callback=None, nmax=-1, slab=0.0, binning=None, kind="thermal",
+        graph=False):
     """fit!(params, movie, cache, L, stages; hygiene, rng, callback, nmax, slab, gradient = :dual, binning)
     (Fit.jl:366-404): the staged Adam schedule on the movie chi2 with the gradient of the device dual sweep
     (chi2_gradient); minibatches of frames and frequencies drawn per iteration when a stage asks for them. The
     parameters live on the host (hygiene changes their count) and a device model is built from them every
-    iteration. Returns (params, history, events)."""
+    iteration. With graph, iterations over the whole movie (no minibatch, no binning, no half-orbit cut) replay
+    one captured graph (api.FitGraph) with the new parameters, captured again when the splat count changes.
+    Returns (params, history, events)."""
     rng = rng or np.random.default_rng()
+    fg = {}
+    if graph:
+        if binning is not None or nmax >= 0:
+            raise ValueError("the captured fit iteration takes the plain image chi2 (no binning, no half-orbit cut)")
+        fg["graph"] = api.FitGraph(cache, movie.times, movie.nus, L, image=(movie.data, movie.sigma, movie.mask))
 
     def batches(st):
         freqs_all = list(range(len(movie.nus))) if st.freqs is None else list(st.freqs)
@@ -549,9 +557,16 @@ def fit(params, movie, cache, L, stages, hygiene=None, rng=None, callback=None, 
         return frames, freqs
 
     def make_valgrad(st, it):
+        if graph and (st.batch is not None or st.freqs is not None):
+            raise ValueError("the captured fit iteration covers the whole movie: stages without minibatches")
         frames, freqs = batches(st)
 
         def valgrad(q):
+            if graph:
+                m = fg.get("model")
+                if m is None or m.ns != q.shape[1]:
+                    m = fg["model"] = Model(q, kind=kind)
+                return fg["graph"].step(m, q)
             m = Model(q, kind=kind)
             g = np.zeros(q.shape, order="F")
             chi = chi2_gradient(g, m, movie, cache, L, frames=frames, freqs=freqs, nmax=nmax, slab=slab, binning=binning)

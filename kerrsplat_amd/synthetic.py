@@ -243,3 +243,43 @@ def config_model(cfg):
     P = kappa_splats(cfg, a, ns, single_frame=single)
     kt = np.asarray(c["knot_times"], dtype=float)
     return "kappa", P, trajectory_knots(a, P, kt), kt
+
+
+def tile_splits(rng, res, tile_sizes):
+    """Uneven tiles of a res x res screen: blocks of the largest tile size, each cut into four halves with
+    probability 1/2 until the smallest size, as (a0, a1, b0, b1) half-open pixel ranges in screen order."""
+    tmin, tmax = tile_sizes
+    out = []
+
+    def split(a0, b0, s):
+        if s > tmin and rng.uniform() < 0.5:
+            h = s // 2
+            for db in (0, h):
+                for da in (0, h):
+                    split(a0 + da, b0 + db, h)
+        else:
+            out.append((a0, min(a0 + s, res), b0, min(b0 + s, res)))
+
+    for b0 in range(0, res, tmax):
+        for a0 in range(0, res, tmax):
+            split(a0, b0, tmax)
+    return out
+
+
+def config_screens(cfg, res=None, extra=None, extra_size=None, tile_sizes=None):
+    """The screens of a batched configuration: the main res x res screen over +-fov cut into uneven tiles
+    (tile_splits) and `extra` square screens of sizes drawn from U[extra_size] over the same field of view.
+    Returns (main camera, tile splits, extra cameras); the defaults are the configuration's sizes."""
+    from .api import Camera
+    c = CONFIGS[cfg]
+    res = c["res"] if res is None else res
+    extra = c["extra_screens"] if extra is None else extra
+    lo, hi = c["extra_size"] if extra_size is None else extra_size
+    ts = c["tile_sizes"] if tile_sizes is None else tile_sizes
+    rng = np.random.Generator(np.random.PCG64(20261202 + 1000 * cfg + 7))
+    fov = c["fov"]
+    main = Camera((-fov, fov), (-fov, fov), res)
+    splits = tile_splits(rng, res, ts)
+    sizes = rng.integers(lo, hi + 1, size=extra)
+    extras = [Camera((-fov, fov), (-fov, fov), int(n)) for n in sizes]
+    return main, splits, extras

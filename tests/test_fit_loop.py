@@ -113,3 +113,32 @@ def test_fit_on_the_device_gradient():
     print(f"device fit: chi2 {hist[0]:.3e} -> {hist[-1]:.3e}", flush=True)
     assert hist[-1] < 0.5 * hist[0]
     assert np.all(q[:12] == P0[:12])
+
+
+@pytest.mark.gpu
+def test_fit_on_the_captured_graph():
+    """The same fit with every iteration one replay of a captured graph: the same chi2 history and parameters up to
+    the order of the gradient sums (the graph reverses all slices in one pass, the eager loop per frequency)."""
+    import kerrsplat_amd as K
+    from kerrsplat_amd import synthetic as syn
+    a, th, res, N = 0.5, math.radians(60.0), 10, 96
+    cam = K.Camera((-15, 15), (-15, 15), res)
+    cache = K.GeodesicCache(cam, N, store_samples=True)
+    K.regenerate(cache, a, th, marcher=K.Recurrence(64))
+    L = syn.length_unit()
+    P = syn.thermal_splats(1, a, 2)
+    P[12] += math.log(30.0)
+    times, nus = [0.0, 20.0], [230e9, 345e9]
+    truth = K.polarized_cube(cache, P, times, nus, L)
+    sig = np.full(4, 0.02 * np.abs(truth[..., 0]).max())
+    movie = fit.StokesMovie(truth, times, nus, sig)
+    P0 = P.copy()
+    P0[12] += 0.3
+    P0[13] -= 0.2
+    stages = [fit.Stage(free=[12, 13], iterations=6, eta=0.05)]
+    q0, h0, _ = fit.fit(P0, movie, cache, L, stages)
+    q1, h1, _ = fit.fit(P0, movie, cache, L, stages, graph=True)
+    print(f"graph fit: chi2 {h1[0]:.3e} -> {h1[-1]:.3e} (eager {h0[-1]:.3e})", flush=True)
+    assert np.allclose(h1, h0, rtol=1e-9, atol=0)
+    assert np.allclose(q1, q0, rtol=1e-9, atol=1e-12)
+    assert h1[-1] < h1[0]
