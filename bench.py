@@ -51,6 +51,9 @@ def parse():
     p.add_argument("--paths", default="mixed,fp64",
                    help="the paths to run (a profiling run may take one; the headline needs both)")
     p.add_argument("--no-tune", action="store_true", help="keep the default launch choices (no autotuner)")
+    p.add_argument("--launch", default=None,
+                   help="fixed launch choices instead of the autotuner (profiling runs): chunk,queue,patch_a,patch_b,"
+                        "adjoint_queue_mixed,adjoint_queue_fp64, e.g. 16,1,8,8,0,1")
     p.add_argument("--copy-probe", action="store_true",
                    help="also run device-to-device copies of 256 MiB (the reference for scaling HBM byte counters)")
     return p.parse_args()
@@ -214,7 +217,13 @@ def main():
     # path with the adjoint records); both paths then run with them
     # the element adjoints' launch form per path (one reverse pass in each form, on the first slice)
     tune = None
-    if not args.no_tune:
+    if args.launch:
+        ch, qu, pa, pb, aqm, aq64 = (int(x) for x in args.launch.split(","))
+        choice = {"chunk": ch, "queue": bool(qu), "patch": (pa, pb), "adjoint_queue": bool(aqm)}
+        tune = {"mixed": {"choice": choice, "seconds": 0.0, "table": [], "fixed": True},
+                "fp64_adjoint_queue": bool(aq64)}
+        progress(f"fixed launch choices: {choice}, FP64 element adjoints queue {bool(aq64)}")
+    elif not args.no_tune:
         K.precision(cache, "mixed")
         t0 = time.perf_counter()
         choice, table = K.autotune(cache, model, [slices[0][0]], [slices[0][1]], L, records=True, sample_every=16, adjoint=True)
@@ -288,7 +297,9 @@ def main():
             mixed_choice = tune["mixed"]["choice"]
             K.tuning(cache, chunk=mixed_choice["chunk"], queue=mixed_choice["queue"], patch=mixed_choice["patch"],
                      adjoint_queue=mixed_choice["adjoint_queue"])
-            if prec == "fp64":
+            if prec == "fp64" and "fp64_adjoint_queue" in tune:
+                K.tuning(cache, adjoint_queue=tune["fp64_adjoint_queue"])
+            elif prec == "fp64":
                 # the FP64 path keeps the transfer's choices and takes its own element-adjoint form
                 t0 = time.perf_counter()
                 choice, table = K.autotune(cache, model, [slices[0][0]], [slices[0][1]], L, adjoint=True, transfer=False)

@@ -86,10 +86,11 @@ int ks_regenerate(ks_cache* cache, const double* alpha, const double* beta, int 
 #define KS_PRECISION_MIXED 1
 int ks_cache_precision(ks_cache* cache, int precision);
 /* The scalar of the Recurrence(M) march into stored samples (cache.jl:182-190, quadrature.jl:231-296), for the next
- * ks_regenerate / ks_march. KS_PRECISION_FP64 (default): the plain FP64 march. KS_PRECISION_MIXED: its bulk in FP32
- * (the Jacobi addition-theorem steps, the radius and polar angle, the integrand rates and Simpson increments) between
- * FP64 closed-form anchors (Jacobi functions, t~ and phi every M samples), FP64 root cases, pixel constants, Mino
- * times and running t~ / phi sums; the stored samples stay FP64. The reference marches in FP64 only and converts
+ * ks_regenerate / ks_march. KS_PRECISION_FP64 (default): the plain FP64 march. KS_PRECISION_MIXED: the integrand
+ * rates of t~ and phi, their Simpson sums and closed-form logarithmic pieces and the polar closed-form increments in
+ * FP32, on the segments whose radii lie within 1.2 r+ .. 20 M off the critical curve (FP64 elsewhere); the Jacobi
+ * states and steps, the positions, the anchors, root cases, pixel constants, Mino times, the visibility rule and the
+ * running t~ / phi sums stay FP64, and the stored samples are FP64. The reference marches in FP64 only and converts
  * afterwards (cache.jl:221-232). Direct and Fused marchers always run FP64. */
 int ks_cache_march_precision(ks_cache* cache, int precision);
 

@@ -819,8 +819,9 @@ def precision(cache, policy):
 
 def march_precision(cache, policy):
     """The scalar of the Recurrence(M) march into stored samples for the next regenerate / march: "fp64" (default) or
-    "mixed" (the bulk of the per-sample work in FP32 between FP64 closed-form anchors; FP64 root cases, pixel
-    constants, Mino times, running t~ / phi sums and stored samples). Set in place; returns the cache."""
+    "mixed" (the integrand rates and Simpson increments of t~ and phi in FP32 on the well-conditioned segments; FP64
+    Jacobi states, positions, anchors, root cases, pixel constants, Mino times, running sums and stored samples).
+    Set in place; returns the cache."""
     if policy not in PRECISIONS:
         raise ValueError(f"precision policy must be one of {sorted(PRECISIONS)}, got {policy!r}")
     _check(cache._lib.ks_cache_march_precision(cache._h, _i(PRECISIONS[policy])))

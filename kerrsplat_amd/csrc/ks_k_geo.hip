@@ -70,7 +70,7 @@ struct StoreF {
 // samples). Anchor residuals per segment res[2 (m npix + j) + {0, 1}] (zero for Direct and past the pixel's last
 // segment).
 constexpr double FP32_MARCH_RMIN = 1.2;      // x r+
-constexpr double FP32_MARCH_RMAX = 40.0;     // M
+constexpr double FP32_MARCH_RMAX = 20.0;     // M
 constexpr double FP32_MARCH_KGAP = 1e-3;     // 1 - k, the radial modulus' distance from the critical curve
 template <int MARCHER, class B>
 __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(KS_WAVES_MARCH))) ks_march_store(double a, double thetao, const double* __restrict__ soa, int64_t npix, int N, int M,

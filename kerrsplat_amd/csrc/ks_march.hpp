@@ -74,7 +74,8 @@ template <class B> KS_DEV PixCT<B> pix_as(const PixC& p) {
     return b;
 }
 
-template <class B> struct RPT { B r, th, cth, s, sn, cn, gt, gph; bool nur, nuth; };
+// (rd, thd, cthd: the FP64 position the samples store and the visibility rule takes; r, th, cth the same in B)
+template <class B> struct RPT { B r, th, cth, s, sn, cn, gt, gph; bool nur, nuth; double rd, thd, cthd; };
 using RP = RPT<double>;
 
 template <class B>
@@ -93,20 +94,27 @@ KS_DEV void smooth_integrands(const PixCT<B>& p, const SpinT<B>& S, const QCT<B>
 }
 
 // tau, fo: the Mino time of the point and of the radial turning point (FP64: they decide nu_r)
+// (p64: the FP64 constants of the same pixel, for the polar angle of the FP32 bulk)
+// The Jacobi states are FP64 for every B: FP32 addition-theorem steps drift by ~1e-7 per step, which over an anchor
+// interval moved the emission radius by ~5e-5 of itself and the polar angle (near the axis, where acos amplifies the
+// rounding of cos theta to ~sqrt(eps)) by more; the positions decide the splat weights. Their two steps per sample
+// are a small part of the march; B carries the integrand rates and the Simpson increments.
 template <class B>
-KS_DEV RPT<B> ray_point(const PixCT<B>& p, const SpinT<B>& S, const QCT<B>& q, const JST<B>& xr, const JST<B>& xt, double tau, double fo) {
+KS_DEV RPT<B> ray_point(const PixC& p64, const PixCT<B>& p, const SpinT<B>& S, const QCT<B>& q, const JS& xr, const JS& xt, double tau,
+                        double fo) {
     RPT<B> o;
-    o.r = radius_jac<B>(p, xr.sn, xr.cn);
-    polar_from_jac<B>(p, xt.sn, xt.cn, xt.dn, o.th, o.cth, o.nuth);
+    o.rd = radius_jac<double>(p64, xr.sn, xr.cn);
+    polar_from_jac<double>(p64, xt.sn, xt.cn, xt.dn, o.thd, o.cthd, o.nuth);
+    o.r = B(o.rd); o.th = B(o.thd); o.cth = B(o.cthd);
     o.nur = before_turning<double>(tau, fo);
     B R = r_potential<B>(S.a, p.lam, p.eta, o.r);
     const B w = o.r * o.r + S.a * S.a - S.a * p.lam;
     // R below 1e-14 w^2 is rounding (a sample on the radial turning point, where R = 0): s = 0 there
     o.s = (o.nur ? -1.0 : 1.0) * (R > 1e-14 * (w * w) ? sqrt(R) : B(0.0));
-    B idn = 1.0 / xt.dn;
-    o.sn = xt.sn * idn / p.scale;
-    o.cn = xt.cn * idn;
-    smooth_integrands<B>(p, S, q, o.r, o.cth, o.s, o.sn, o.cn, idn, o.gt, o.gph);
+    const double idn = 1.0 / xt.dn;
+    o.sn = B(xt.sn * idn / p64.scale);
+    o.cn = B(xt.cn * idn);
+    smooth_integrands<B>(p, S, q, o.r, o.cth, o.s, o.sn, o.cn, B(idn), o.gt, o.gph);
     return o;
 }
 KS_COLD JS radial_state(const PixC& p, double tau) { return js_at(p.c * (p.fo - tau), p.k); }
@@ -117,6 +125,7 @@ KS_COLD RP ray_point_turning(const PixC& p, const Spin& S, const QC& q) {
     RP o;
     o.r = p.r4;
     polar_from_jac(p, xt.sn, xt.cn, xt.dn, o.th, o.cth, o.nuth);
+    o.rd = o.r; o.thd = o.th; o.cthd = o.cth;
     o.nur = true;
     o.s = 0.0;
     double idn = 1.0 / xt.dn;
@@ -126,7 +135,7 @@ KS_COLD RP ray_point_turning(const PixC& p, const Spin& S, const QC& q) {
     return o;
 }
 template <class B> KS_DEV RPT<B> rp_as(const RP& o) {
-    return {B(o.r), B(o.th), B(o.cth), B(o.s), B(o.sn), B(o.cn), B(o.gt), B(o.gph), o.nur, o.nuth};
+    return {B(o.r), B(o.th), B(o.cth), B(o.s), B(o.sn), B(o.cn), B(o.gt), B(o.gph), o.nur, o.nuth, o.rd, o.thd, o.cthd};
 }
 
 template <class B>
@@ -160,8 +169,8 @@ KS_COLD void turning_increment(const PixC& p, const Spin& S, const QC& q, const 
     const RPT<B> pt = rp_as<B>(ray_point_turning(p, S, q));
     const B Pt = polar_closed_form<B>(pb_, qb, p.fo, pt.sn, pt.cn);
     double t1 = 0.5 * (ta + p.fo), t2 = 0.5 * (p.fo + tb);
-    RPT<B> p1 = ray_point<B>(pb_, Sb, qb, js_as<B>(radial_state(p, t1)), js_as<B>(polar_state(p, t1)), t1, p.fo);
-    RPT<B> p2 = ray_point<B>(pb_, Sb, qb, js_as<B>(radial_state(p, t2)), js_as<B>(polar_state(p, t2)), t2, p.fo);
+    RPT<B> p1 = ray_point<B>(p, pb_, Sb, qb, radial_state(p, t1), polar_state(p, t1), t1, p.fo);
+    RPT<B> p2 = ray_point<B>(p, pb_, Sb, qb, radial_state(p, t2), polar_state(p, t2), t2, p.fo);
     B It1, Ip1, It2, Ip2;
     interval_increment<B>(pb_, Sb, qb, pa, p1, pt, ta, p.fo, Pa, Pt, It1, Ip1);
     interval_increment<B>(pb_, Sb, qb, pt, p2, pb, p.fo, tb, Pt, Pb, It2, Ip2);
@@ -194,7 +203,6 @@ KS_DEV int anchor_interval(int M, double one_minus_k) {
 template <class B, class F>
 KS_DEV void march_range_b(const PixC& p, const Spin& S, const QC& q, const VisC& vis, const PixCT<B>& pb_, const SpinT<B>& Sb, const QCT<B>& qb,
                           int N, int M, int kbeg, int kend, F& f, double& resid_t, double& resid_ph) {
-    const B kb = B(p.k), mub = B(p.mu);
     const double dtau = p.tau_total / (double)(N + 1);
     const double h = 0.5 * dtau;
     // the radial turning point splits the interval that holds it (ta < fo < tb). With N + 1 even, fo = tau_total / 2
@@ -202,19 +210,19 @@ KS_DEV void march_range_b(const PixC& p, const Spin& S, const QC& q, const VisC&
     // k dtau; a turning point within 1e-9 dtau of a sample is taken as on it (no split, the exact-arithmetic case)
     const double tol = 1e-9 * dtau;
     const int Man = anchor_interval(M, 1.0 - p.k);
-    const JST<B> Dr = js_as<B>(js_step_constants(-p.c * h, p.k));
-    const JST<B> Dt = js_as<B>(js_step_constants(p.scale * h / p.f, p.mu));
+    const JS Dr = js_step_constants(-p.c * h, p.k);
+    const JS Dt = js_step_constants(p.scale * h / p.f, p.mu);
     resid_t = 0.0; resid_ph = 0.0;
     // start: the last anchor A <= kbeg whose sample is valid, or sample 1
     int A = kbeg;
-    JST<B> xr, xt;
+    JS xr, xt;
     RPT<B> pa;
     for (;;) {
         const double ta = (double)A * dtau;
-        xr = js_as<B>(radial_state(p, ta));
-        xt = js_as<B>(polar_state(p, ta));
-        pa = ray_point<B>(pb_, Sb, qb, xr, xt, ta, p.fo);
-        if (A <= 1 || ((ta <= p.tau_h) && krang_visible(vis, val(pa.cth)))) break;
+        xr = radial_state(p, ta);
+        xt = polar_state(p, ta);
+        pa = ray_point<B>(p, pb_, Sb, qb, xr, xt, ta, p.fo);
+        if (A <= 1 || ((ta <= p.tau_h) && krang_visible(vis, pa.cthd))) break;
         A = A - Man > 1 ? A - Man : 1;
     }
     double t, ph;
@@ -222,7 +230,7 @@ KS_DEV void march_range_b(const PixC& p, const Spin& S, const QC& q, const VisC&
     B Pa = polar_closed_form<B>(pb_, qb, (double)A * dtau, pa.sn, pa.cn);
     if (A == kbeg) {
         const double ta = (double)A * dtau;
-        Samp s{t, val(pa.r), val(pa.th), ph, pa.nur, pa.nuth, (ta <= p.tau_h) && krang_visible(vis, val(pa.cth))};
+        Samp s{t, pa.rd, pa.thd, ph, pa.nur, pa.nuth, (ta <= p.tau_h) && krang_visible(vis, pa.cthd)};
         f(A, s, dtau);
     }
     const int klast = kend < N ? kend + 1 : N;
@@ -230,12 +238,12 @@ KS_DEV void march_range_b(const PixC& p, const Spin& S, const QC& q, const VisC&
         const double ta = (double)(k - 1) * dtau, tb = (double)k * dtau;
         const bool anchor = ((k - 1) % Man) == 0;
         if (k > kend && !anchor) break;            // the step past kend only serves the next anchor's residual
-        xr = js_step<B>(xr, Dr, kb);
-        xt = js_step<B>(xt, Dt, mub);
-        RPT<B> pmid = ray_point<B>(pb_, Sb, qb, xr, xt, ta + h, p.fo);
-        if (anchor) { xr = js_as<B>(radial_state(p, tb)); xt = js_as<B>(polar_state(p, tb)); }
-        else { xr = js_step<B>(xr, Dr, kb); xt = js_step<B>(xt, Dt, mub); }
-        RPT<B> pb = ray_point<B>(pb_, Sb, qb, xr, xt, tb, p.fo);
+        xr = js_step<double>(xr, Dr, p.k);
+        xt = js_step<double>(xt, Dt, p.mu);
+        RPT<B> pmid = ray_point<B>(p, pb_, Sb, qb, xr, xt, ta + h, p.fo);
+        if (anchor) { xr = radial_state(p, tb); xt = polar_state(p, tb); }
+        else { xr = js_step<double>(xr, Dr, p.k); xt = js_step<double>(xt, Dt, p.mu); }
+        RPT<B> pb = ray_point<B>(p, pb_, Sb, qb, xr, xt, tb, p.fo);
         const B Pb = polar_closed_form<B>(pb_, qb, tb, pb.sn, pb.cn);
         if (q.scattering && p.fo - ta > tol && tb - p.fo > tol) {
             turning_increment<B>(p, S, q, pb_, Sb, qb, pa, pb, ta, tb, Pa, Pb, t, ph);
@@ -245,7 +253,7 @@ KS_DEV void march_range_b(const PixC& p, const Spin& S, const QC& q, const VisC&
             t += val(It);
             ph += val(Ip);
         }
-        const bool okb = (tb <= p.tau_h) && krang_visible(vis, val(pb.cth));
+        const bool okb = (tb <= p.tau_h) && krang_visible(vis, pb.cthd);
         if (anchor && okb) {
             // closed-form anchor; Krang's ok for the sample decides whether it resets (quadrature.jl:283-291)
             double td, pd;
@@ -257,7 +265,7 @@ KS_DEV void march_range_b(const PixC& p, const Spin& S, const QC& q, const VisC&
             t = td; ph = pd;
         }
         if (k >= kbeg && k <= kend) {
-            Samp s{t, val(pb.r), val(pb.th), ph, pb.nur, pb.nuth, okb};
+            Samp s{t, pb.rd, pb.thd, ph, pb.nur, pb.nuth, okb};
             f(k, s, dtau);
         }
         pa = pb;
