@@ -238,6 +238,9 @@ __device__ __forceinline__ void adj_splats_block(int64_t blk, double a, double t
     __syncthreads();
     const S tos = S(sin(thetao)), aS = S(a);
     const int64_t st = (int64_t)N * nitems;
+    // the FP32 path reads the samples' FP32 copy (one 16-byte load per sample instead of four 8-byte loads; its
+    // launcher requires it)
+    constexpr bool use_p4 = std::is_same<S, f32>::value;
     S grad[ADJ_ROWS];
     for (int r = 0; r < ADJ_ROWS; ++r) grad[r] = S(0.0);
     int qhead = 0, qn = 0;   // wavefront-uniform ring state
@@ -249,7 +252,13 @@ __device__ __forceinline__ void adj_splats_block(int64_t blk, double a, double t
             const int64_t fo = (int64_t)(k - 1) * nitems + it2;
             const int64_t so = (int64_t)(k - 1) * npix + it2 % npix;
             S dl[13];
-            const S tt = S(s_tobs[l] - smp.t[so]), xx = S(smp.x[so]), yy = S(smp.y[so]), zz = S(smp.z[so]);
+            S tt, xx, yy, zz;
+            if constexpr (use_p4) {
+                const float4 q = smp.p4[so];
+                tt = S((float)s_tobs[l] - q.w); xx = S(q.x); yy = S(q.y); zz = S(q.z);
+            } else {
+                tt = S(s_tobs[l] - smp.t[so]); xx = S(smp.x[so]); yy = S(smp.y[so]); zz = S(smp.z[so]);
+            }
             S Gw;
             if constexpr ((POPS & POP_KNOTS) != 0) Gw = c.nk > 0 ? weight_dlog_knot(c, sa, knots, ktimes, tt, xx, yy, zz, dl) : weight_dlog(c, sa, tt, xx, yy, zz, dl);
             else Gw = weight_dlog(c, sa, tt, xx, yy, zz, dl);
@@ -279,26 +288,31 @@ __device__ __forceinline__ void adj_splats_block(int64_t blk, double a, double t
     };
     for (int k0 = kfirst; k0 <= klast; k0 += WALK) {
         uint8_t fl[WALK];
-        double tk[WALK], xk[WALK], yk[WALK], zk[WALK];
+        S tk[WALK], xk[WALK], yk[WALK], zk[WALK];     // t_obs - t~ and the position of each sample of the group
 #pragma unroll
         for (int u = 0; u < WALK; ++u) {
             const int k = k0 + u;
             fl[u] = 0;
-            tk[u] = xk[u] = yk[u] = zk[u] = 0.0;
+            tk[u] = xk[u] = yk[u] = zk[u] = S(0.0);
             if (valid && k <= klast) {
                 const int64_t so = (int64_t)(k - 1) * npix + j;
                 fl[u] = aflg[(int64_t)(k - 1) * nitems + item];
-                tk[u] = smp.t[so]; xk[u] = smp.x[so]; yk[u] = smp.y[so]; zk[u] = smp.z[so];
+                if constexpr (use_p4) {
+                    const float4 q = smp.p4[so];
+                    tk[u] = S((float)t_obs - q.w); xk[u] = S(q.x); yk[u] = S(q.y); zk[u] = S(q.z);
+                } else {
+                    tk[u] = S(t_obs - smp.t[so]); xk[u] = S(smp.x[so]); yk[u] = S(smp.y[so]); zk[u] = S(smp.z[so]);
+                }
             }
         }
 #pragma unroll
         for (int u = 0; u < WALK; ++u) {
             bool act;
             if constexpr ((POPS & POP_KNOTS) != 0)
-                act = fl[u] && (c.nk > 0 ? weight_active_knot(c, knots, ktimes, S(t_obs - tk[u]), S(xk[u]), S(yk[u]), S(zk[u]))
-                                         : weight_active(c, S(t_obs - tk[u]), S(xk[u]), S(yk[u]), S(zk[u])));
+                act = fl[u] && (c.nk > 0 ? weight_active_knot(c, knots, ktimes, tk[u], xk[u], yk[u], zk[u])
+                                         : weight_active(c, tk[u], xk[u], yk[u], zk[u]));
             else
-                act = fl[u] && weight_active(c, S(t_obs - tk[u]), S(xk[u]), S(yk[u]), S(zk[u]));
+                act = fl[u] && weight_active(c, tk[u], xk[u], yk[u], zk[u]);
             const uint64_t m = __ballot(act);
             if (act) {
                 const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
@@ -446,6 +460,7 @@ hipError_t launch_adj_splats_f32(hipStream_t s, int pops, double a, double theta
                                  const SplatCf* sp, const SplatAdjf* sadj, int ns, const f32* knots, const f32* ktimes,
                                  const double* times, int nt, const double* nus, int64_t nitems, const f32* rec, const f32* bar,
                                  const uint8_t* aflg, int64_t unit0, int64_t nb, const double* ub, double* part, int* heads) {
+    if (!smp.p4) return hipErrorInvalidValue;          // the FP32 element adjoints read the samples' FP32 copy
     return adj_splats_k<f32>(s, pops, a, thetao, soa, npix, N, smp, sp, sadj, ns, knots, ktimes, times, nt, nus, nitems, rec, bar, aflg, unit0, nb,
                              ub, part, heads);
 }

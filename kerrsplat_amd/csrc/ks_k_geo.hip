@@ -112,6 +112,17 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(KS_WAV
     res[2 * (m * npix + j)] = r1;
     res[2 * (m * npix + j) + 1] = r2;
 }
+// the FP32 copy of the sample positions and times the FP32 element adjoints read: one 16-byte load per sample
+__global__ void ks_samples_f32(int64_t n, const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+                               const double* __restrict__ t, float4* __restrict__ p4) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p4[i] = make_float4((float)x[i], (float)y[i], (float)z[i], (float)t[i]);
+}
+hipError_t launch_samples_f32(hipStream_t s, int64_t n, const double* x, const double* y, const double* z, const double* t, float4* p4) {
+    hipLaunchKernelGGL(ks_samples_f32, dim3(grid_for(n, 256)), dim3(256), 0, s, n, x, y, z, t, p4);
+    return hipGetLastError();
+}
+
 // per pixel: the largest residual over its segments (frame 0 of a render, or a stored march)
 __global__ void ks_res_reduce(int64_t npix, int nseg_max, const double* __restrict__ res, double* __restrict__ rt, double* __restrict__ rp) {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -83,6 +83,7 @@ struct Samples {
     double *t, *r, *th, *ph, *x, *y, *z;
     uint8_t* fl;
     const int* tmap;      // the sorted pixels in wavefront order (8 x 8 screen tiles), or null for sorted order
+    const float4* p4 = nullptr;   // (x, y, z, t~) of every sample in FP32 for the FP32 element adjoints, or null
 };
 // the item of work slot t (t = j + npix f): the pixel at slot t % npix of the wavefront order, same frame
 KS_DEV int64_t slot_item(const Samples& smp, int64_t t, int64_t npix) {
@@ -191,6 +192,7 @@ hipError_t launch_march_store(hipStream_t s, double a, double thetao, const doub
 hipError_t launch_res_reduce(hipStream_t s, int64_t npix, int nseg_max, const double* res, double* rt, double* rp);
 hipError_t launch_wind_cutoff(hipStream_t s, double a, double thetao, const double* soa, int64_t npix, int N, int M, int nmax, double slab,
                               Samples smp, int* kstop);
+hipError_t launch_samples_f32(hipStream_t s, int64_t n, const double* x, const double* y, const double* z, const double* t, float4* p4);
 hipError_t launch_unsort4(hipStream_t s, const double* src, const int* perm, int64_t npix, int64_t nframes, double* dst);
 hipError_t launch_unsort1(hipStream_t s, const double* src, const int* perm, int64_t npix, double* dst);
 

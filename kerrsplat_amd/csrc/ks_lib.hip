@@ -125,6 +125,8 @@ struct ks_cache {
     double* soa = nullptr;
     double *st = nullptr, *sr = nullptr, *sth = nullptr, *sph = nullptr; uint8_t* sfl = nullptr;
     double *sx = nullptr, *sy = nullptr, *sz = nullptr;        // quasi-Cartesian position of each stored sample
+    float4* sp4 = nullptr;                                     // their FP32 copy with t~ (x, y, z, t) for the FP32 element adjoints
+    int64_t march_version = 0, p4_version = -1;                // marches into the samples; the march the copy was made from
     double* thin_part = nullptr;                               // per-segment thin sums (max_segments x npix)
     double *rt = nullptr, *rp = nullptr;
     double* out = nullptr; int64_t out_frames = 0;    // sorted-order Stokes images
@@ -297,7 +299,7 @@ int ks_cache_destroy(ks_cache* c) {
     (void)hipSetDevice(c->ctx->device);
     dfree(c->al); dfree(c->be); dfree(c->numreals); dfree(c->perm); dfree(c->tmap); dfree(c->iperm); dfree(c->bcount); dfree(c->soa);
     dfree(c->st); dfree(c->sr); dfree(c->sth); dfree(c->sph); dfree(c->sfl); dfree(c->rt); dfree(c->rp);
-    dfree(c->sx); dfree(c->sy); dfree(c->sz); dfree(c->thin_part);
+    dfree(c->sx); dfree(c->sy); dfree(c->sz); dfree(c->thin_part); dfree(c->sp4);
     dfree(c->out); dfree(c->out_screen); dfree(c->img1);
     for (Buf<double>* b : {&c->vu, &c->vv, &c->vt, &c->vg, &c->vd, &c->vs, &c->vA2, &c->vB2, &c->vIm, &c->vC2, &c->vV, &c->vwq, &c->vE2,
                            &c->vSd, &c->vSp, &c->vchi, &c->vsum, &c->idata, &c->isig, &c->bseed})
@@ -324,7 +326,17 @@ static Samples samples_of(const ks_cache* c) {
     Samples S;
     S.t = c->st; S.r = c->sr; S.th = c->sth; S.ph = c->sph; S.x = c->sx; S.y = c->sy; S.z = c->sz; S.fl = c->sfl;
     S.tmap = c->tmap;
+    S.p4 = c->p4_version == c->march_version ? c->sp4 : nullptr;
     return S;
+}
+// the FP32 copy of the stored positions and times (for the FP32 element adjoints), made once per march
+static int ensure_p4(ks_cache* c) {
+    if (c->sp4 && c->p4_version == c->march_version) return KS_OK;
+    const int64_t n = c->npix * (int64_t)c->N;
+    if (!c->sp4 && dalloc(&c->sp4, (size_t)n)) return KS_ERR_HIP;
+    HIPCHK(launch_samples_f32(c->ctx->stream, n, c->sx, c->sy, c->sz, c->st, c->sp4));
+    c->p4_version = c->march_version;
+    return KS_OK;
 }
 static Samples no_samples() {
     Samples S;
@@ -341,6 +353,7 @@ static int launch_store_march(ks_cache* c) {
     HIPCHK(launch_march_store(x->stream, c->spin, c->thetao, c->soa, c->npix, c->N, M, c->marcher == KS_MARCHER_DIRECT ? 0 : 1, nseg,
                               samples_of(c), c->segres, c->march_precision == KS_PRECISION_MIXED ? 1 : 0));
     HIPCHK(x->stage_end(KS_STAGE_MARCH));
+    c->march_version += 1;
     HIPCHK(launch_res_reduce(x->stream, c->npix, nseg, c->segres, c->rt, c->rp));
     return KS_OK;
 }
@@ -986,6 +999,8 @@ static int reverse_launches(ks_cache* c, const ks_model* m, const double* d_tn, 
     else HIPCHK(launch_adj_segment(x->stream, c->N, c->fw_chunk, items, c->awT, c->arec, c->abar, c->aflg, c->atail, samples_of(c), c->npix));
     HIPCHK(x->stage_end(KS_STAGE_ADJ_STEPS));
     if (ns > 0) {
+        if (mixed)
+            if (int rc = ensure_p4(c)) return rc;
         HIPCHK(x->stage_begin(KS_STAGE_ADJ_ELEMENTS));
         HIPCHK(launch_unit_bounds(x->stream, c->npix, c->N, samples_of(c), d_tn, nt, items, c->aflg, c->aub));
         // the persistent form of the element adjoints with the cache's work-queue choice: per-XCD queue heads
@@ -1593,7 +1608,7 @@ static void graph_release_host(ks_graph* g) {
 // the cache buffers a captured iteration reads or writes
 static std::vector<const void*> graph_ptrs(const ks_cache* c) {
     return {c->seg, c->out, c->arec, c->abar, c->aflg, c->atail, c->awT, c->ads, c->apart, c->adpar, c->aub, c->tmap, c->soa,
-            c->qhead.p, c->qorder.p, c->kstop, c->aheads.p};
+            c->qhead.p, c->qorder.p, c->kstop, c->aheads.p, c->sp4};
 }
 
 // the iteration's launches on the (capturing) stream
