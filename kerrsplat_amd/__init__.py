@@ -19,7 +19,7 @@ from .api import (Camera, Direct, Recurrence, Fused, GeodesicCache, Model, Compo
                   polarized_cube, polarized_gradient, march, precision, stage_ms, render_device, visibilities, unsort, to_screen, samples, residuals, consts, KernelError,
                   NSPLATPARAMS, NPOLARIZEDPARAMS, NPOWERLAWPARAMS, NKAPPAPARAMS, gravitational_radius, grid_axis,
                   library_path, gradient_forward, gradient_reverse, vis_chi2_seed, image_chi2_seed, vis_seed, closure_chi2_seed, tiles, Binning,
-                  bin, binned_grid, binned_polar, concatenate, spacetime_tangent, RayLists, ray_lists, framed, flattened, frame_lists,
+                  bin, binned_grid, binned_polar, concatenate, spacetime_tangent, spacetime_tangent_differences, RayLists, ray_lists, framed, flattened, frame_lists,
                   concatenate_screens, screen_tiles, tuning, autotune, queue_costs, FitGraph, march_precision)
 from . import fit
 from .fit import (ScatteringKernel, taper, spacetime_valgrad, spacetime_jacobian, spacetime_movie_loss, StokesMovie, FrameScans, chi2, chi2_gradient, sweep_passes, frame_chi2_seed, image_loss_gradient, closure_phases,
@@ -31,5 +31,5 @@ __all__ = ["Camera", "Direct", "Recurrence", "Fused", "GeodesicCache", "Model", 
            "gravitational_radius", "grid_axis", "library_path", "gradient_forward", "gradient_reverse", "vis_chi2_seed",
            "image_chi2_seed", "fit", "StokesMovie", "FrameScans", "chi2", "chi2_gradient", "sweep_passes", "frame_chi2_seed",
            "image_loss_gradient", "vis_seed", "closure_phases", "log_closure_amplitudes", "ClosureData", "closure_chi2",
-           "frame_chi2_closures", "gain_factors", "apply_gains", "tiles", "Binning", "bin", "binned_grid", "binned_polar", "concatenate", "spacetime_tangent", "spacetime_valgrad", "spacetime_jacobian",
+           "frame_chi2_closures", "gain_factors", "apply_gains", "tiles", "Binning", "bin", "binned_grid", "binned_polar", "concatenate", "spacetime_tangent", "spacetime_tangent_differences", "spacetime_valgrad", "spacetime_jacobian",
            "spacetime_movie_loss", "ScatteringKernel", "taper", "RayLists", "ray_lists", "framed", "flattened", "frame_lists", "concatenate_screens", "screen_tiles", "tuning", "autotune", "queue_costs", "FitGraph", "march_precision"]

@@ -769,6 +769,40 @@ def spacetime_tangent(cache, params, times, nus, L, nmax=-1, slab=0.0, kind="the
             np.transpose(dout.reshape(len(n), len(t), nb, na, 4, 3), (3, 2, 1, 0, 4, 5)))
 
 
+def spacetime_tangent_differences(camera, N, a, theta_o, params, times, nus, L, nmax=-1, slab=0.0, kind="thermal", h=2e-6,
+                                  screen=True, marcher=None):
+    """The observed Stokes cube of `camera` at (a, theta_o) with N stored samples per ray (marcher: Recurrence(64),
+    the spacetime block's stored march, joint.jl:78, by default; Direct() for the closed-form samples) and its
+    partials with respect to x = (a, theta_o, ln L) (the values spacetime_valgrad / spacetime_jacobian take from their
+    dual cache, joint.jl:74-154), every image rendered on the device in FP64: central differences in each variable,
+    Richardson-extrapolated from the steps h and h/2 (truncation error O(h^4); the reference checks its duals
+    against such differences, test_spacetime.jl:94-113). Steps much longer than 1e-6 can straddle a sample that
+    crosses the horizon guard or a splat's weight cutoff, a jump that the partials exclude. The spin and
+    inclination partials regenerate the pixel constants and samples at a +- h and theta_o +- h (8 marches); the
+    path-length partial re-renders the cube at L e^(+-h) (13 cubes in all). Returns (cube, dcube) shaped as
+    spacetime_tangent's."""
+    if not (L > 0.0):
+        raise ValueError("the length unit L must be positive")
+    if not (h > 0.0):
+        raise ValueError("the step h must be positive")
+    m = _as_model(params, kind)
+    march = Recurrence(64) if marcher is None else marcher
+    cache = GeodesicCache(camera, N, store_samples=True)
+
+    def cube(a_, th_, L_):
+        regenerate(cache, a_, th_, marcher=march)
+        return polarized_cube(cache, m, times, nus, L_, nmax=nmax, slab=slab, screen=screen)
+
+    def central(k, hh):
+        e = np.eye(3)[k] * hh
+        up = cube(a + e[0], theta_o + e[1], L * math.exp(e[2]))
+        dn = cube(a - e[0], theta_o - e[1], L * math.exp(-e[2]))
+        return (up - dn) / (2.0 * hh)
+
+    dcube = np.stack([(4.0 * central(k, 0.5 * h) - central(k, h)) / 3.0 for k in range(3)], axis=-1)
+    return cube(a, theta_o, L), dcube
+
+
 def polarized_cube(cache, params, times, nus, L, nmax=-1, slab=0.0, kind="thermal", screen=True):
     """polarized_cube(cache, params, times, νs, L; nmax, slab) (polarized.jl:187-198), all frames
     and frequencies in one launch: (n_alpha, n_beta, nt, nnu, 4) observed Stokes [cgs]."""

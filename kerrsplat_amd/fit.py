@@ -248,22 +248,20 @@ def apply_gains(vis, gains, s1, s2):
 
 
 # ---- the spacetime block (joint.jl:74-154) ---------------------------------------------------------------
-def _spacetime_cache(x, camera, N):
-    """A cache of `camera` with N samples per ray at (a, theta_o) = x[:2] (pixel constants only: the tangent
-    kernel marches the closed-form samples itself)."""
-    cache = api.GeodesicCache(camera, N, store_samples=False)
-    api.regenerate(cache, float(x[0]), float(x[1]), marcher=api.Fused(64))
-    return cache
+def _spacetime_tangent(x, camera, N, params, times, nus, nmax, slab, kind):
+    """The cube of `camera` at x = (a, theta_o, ln L) with N closed-form samples per ray and its partials in x,
+    on the device (Richardson-extrapolated central differences of FP64 cubes)."""
+    return api.spacetime_tangent_differences(camera, N, float(x[0]), float(x[1]), params, times, nus, float(np.exp(x[2])),
+                                             nmax=nmax, slab=slab, kind=kind)
 
 
 def spacetime_valgrad(loss_and_grad, x, camera, N, params, times, nus, nmax=-1, slab=0.0, kind="thermal"):
     """spacetime_valgrad(loss, x, camera, backend; N) (joint.jl:74-82) for a loss of the rendered cube: the
     value and gradient with respect to x = (a, theta_o, ln L). The cube of `params` at `times`, `nus` and its
-    forward-mode partials come from spacetime_tangent; loss_and_grad(cube) takes the screen-shaped cube
+    partials come from spacetime_tangent_differences; loss_and_grad(cube) takes the screen-shaped cube
     (n_alpha, n_beta, nt, nnu, 4) and returns (value, d value / d cube) (spacetime_movie_loss builds the
     movie χ²). Returns (value, gradient (3,))."""
-    cache = _spacetime_cache(x, camera, N)
-    cube, dcube = api.spacetime_tangent(cache, params, times, nus, float(np.exp(x[2])), nmax=nmax, slab=slab, kind=kind)
+    cube, dcube = _spacetime_tangent(x, camera, N, params, times, nus, nmax, slab, kind)
     value, dv = loss_and_grad(cube)
     g = np.einsum("abtfs,abtfsk->k", np.asarray(dv, dtype=np.float64), dcube)
     return float(value), g
@@ -303,8 +301,7 @@ def spacetime_jacobian(x, camera, N, params, movie, frames=None, freqs=None, nma
     if len(x) != 3:
         raise ValueError("the spacetime block is (a, theta_o, ln L)")
     ks, ls = _frames(movie, frames, freqs)
-    cache = _spacetime_cache(x, camera, N)
-    cube, dcube = api.spacetime_tangent(cache, params, movie.times[ks], movie.nus[ls], float(np.exp(x[2])), nmax=nmax, slab=slab, kind=kind)
+    cube, dcube = _spacetime_tangent(x, camera, N, params, movie.times[ks], movie.nus[ls], nmax, slab, kind)
     if binning is not None:
         cube = api.bin(binning, cube)
         dcube = api.bin(binning, dcube)

@@ -1,12 +1,17 @@
 """GPU parity of the spacetime tangent (spacetime_valgrad / spacetime_jacobian, joint.jl:74-154).
 
-The device tangent (three-partial duals through the pixel constants, the closed-form samples and the
-transport) against the oracle's forward-mode tangent on the same Direct samples (pinned by central
-differences and the thin-limit linearity in tests/test_oracle_spacetime.py): values at the FP64 image bar
-(1e-10 of the peak), partials at 1e-8 of each partial's largest entry. Then the reference's own checks of
-the spacetime block (test_spacetime.jl:94-113): the valgrad value equals the χ² of a stored Recurrence(64)
-cache, its gradient matches central differences of the device χ², and the Jacobian's residuals square to
-the same χ² with 2 Jᵀr equal to the gradient.
+The device tangent by differences (spacetime_tangent_differences: FP64 cubes rendered on the device at the
+spacetime and at steps h, h/2 either side in a, theta_o and ln L, Richardson-extrapolated) against the oracle's
+forward-mode tangent on the same samples, Direct and Recurrence(64) (pinned by central differences and the thin-limit linearity in
+tests/test_oracle_spacetime.py): values at the FP64 image bar (1e-10 of the peak), partials at 2e-5 of each
+partial's largest entry (the oracle's own differences meet its duals at 5e-6). A composite's partials in the thin
+limit are the sums of its parts'. Then the reference's own checks of the spacetime block (test_spacetime.jl:94-113):
+the valgrad value equals the chi2 of a stored Recurrence(64) cache, its gradient matches central differences of the
+device chi2, and the Jacobian's residuals square to the same chi2 with 2 J^T r equal to the gradient.
+
+The device kernel of the dual tangent (three-partial duals through the pixel constants, the closed-form samples
+and the transport) stays disabled: it took a GPU memory fault on its first launch, the cause is not found, and
+ks_spacetime_tangent refuses to launch it; its test is kept, skipped.
 """
 import math
 
@@ -61,27 +66,52 @@ def test_tangent_vs_oracle(oracle, kind, nmax):
                 assert err <= 1e-8, (kind, it, inu, k, err)
 
 
-@DISABLED
+@pytest.mark.parametrize("kind,nmax,marcher", [("thermal", -1, 0), ("thermal", 0, 0), ("powerlaw", -1, 0), ("kappa", -1, 0),
+                                                ("thermal", -1, 1), ("kappa", 0, 1)])
+def test_tangent_differences_vs_oracle(oracle, kind, nmax, marcher):
+    cam = K.Camera((-9.0, 9.0), (-9.0, 9.0), 10)
+    N = 64
+    L = syn.length_unit()
+    P, hyp = parts(oracle, kind, 3)
+    times, nus = [0.0, 15.0], [230e9, 345e9]
+    print(f"tangent by differences: {kind}, nmax {nmax}, marcher {marcher}", flush=True)
+    march = K.Direct() if marcher == 0 else K.Recurrence(64)
+    cube, dcube = K.spacetime_tangent_differences(cam, N, A, TH, P, times, nus, L, nmax=nmax, slab=0.5, kind=kind, marcher=march)
+    assert cube.shape == (10, 10, 2, 2, 4) and dcube.shape == (10, 10, 2, 2, 4, 3)
+    for it, t in enumerate(times):
+        for inu, nu in enumerate(nus):
+            I, dI = oracle.spacetime_tangent(A, TH, cam.alphas, cam.betas, N, P, t, nu, L, kind=kind, hyp=hyp, nmax=nmax, slab=0.5,
+                                             marcher=marcher)
+            I = I.reshape(10, 10, 4, order="F")
+            dI = dI.reshape(10, 10, 4, 3, order="F")
+            peak = np.abs(I[..., 0]).max()
+            assert peak > 0
+            assert np.abs(cube[:, :, it, inu] - I).max() <= 1e-10 * peak
+            for k in range(3):
+                scale = np.abs(dI[..., k]).max()
+                assert scale > 0
+                err = np.abs(dcube[:, :, it, inu, :, k] - dI[..., k]).max() / scale
+                print(f"  t {t}, nu {nu:.3g}, partial {k}: {err:.2e} of its largest entry", flush=True)
+                assert err <= 2e-5, (kind, it, inu, k, err)
+
+
 def test_composite_tangent_is_the_sum_of_its_thin_parts(oracle):
     """A composite's partials in the optically thin limit are the sums of its parts' partials."""
     cam = K.Camera((-9.0, 9.0), (-9.0, 9.0), 8)
     L = syn.length_unit()
-    cache = K.GeodesicCache(cam, 48, store_samples=False)
-    K.regenerate(cache, A, TH, marcher=K.Fused(64))
     T, Pl = syn.thermal_splats(1, A, 2), syn.powerlaw_splats(2, A, 2)
     T[12] += math.log(1e-9)
     Pl[12] += math.log(1e-6)
     mt, mp = K.Model(T), K.Model(Pl, kind="powerlaw")
-    c, dc = K.spacetime_tangent(cache, K.CompositeModel(mt, mp), [0.0], [230e9], L)
-    ct, dct = K.spacetime_tangent(cache, mt, [0.0], [230e9], L)
-    cp, dcp = K.spacetime_tangent(cache, mp, [0.0], [230e9], L)
+    c, dc = K.spacetime_tangent_differences(cam, 48, A, TH, K.CompositeModel(mt, mp), [0.0], [230e9], L)
+    ct, dct = K.spacetime_tangent_differences(cam, 48, A, TH, mt, [0.0], [230e9], L)
+    cp, dcp = K.spacetime_tangent_differences(cam, 48, A, TH, mp, [0.0], [230e9], L)
     for k in range(3):
         scale = np.abs(dc[..., k]).max()
         assert np.abs(dc[..., k] - dct[..., k] - dcp[..., k]).max() <= 1e-4 * scale
     assert np.abs(dct[..., 0]).max() > 1e-2 * np.abs(dc[..., 0]).max() and np.abs(dcp[..., 0]).max() > 1e-2 * np.abs(dc[..., 0]).max()
 
 
-@DISABLED
 def test_spacetime_valgrad_and_jacobian():
     """test_spacetime.jl:94-113 on this build: value, gradient against central differences, Jacobian."""
     rng = np.random.default_rng(21)
@@ -136,3 +166,7 @@ def test_spacetime_errors():
         K.spacetime_tangent(cache, syn.thermal_splats(1, A, 1), [0.0], [230e9], -1.0)
     with pytest.raises(K.api.StateError, match="disabled"):                # refuses to launch the kernel
         K.spacetime_tangent(cache, syn.thermal_splats(1, A, 1), [0.0], [230e9], 1e14)
+    with pytest.raises(ValueError, match="positive"):
+        K.spacetime_tangent_differences(cam, 16, A, TH, syn.thermal_splats(1, A, 1), [0.0], [230e9], -1.0)
+    with pytest.raises(ValueError, match="positive"):
+        K.spacetime_tangent_differences(cam, 16, A, TH, syn.thermal_splats(1, A, 1), [0.0], [230e9], 1e14, h=0.0)
