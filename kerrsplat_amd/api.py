@@ -772,7 +772,8 @@ def spacetime_tangent(cache, params, times, nus, L, nmax=-1, slab=0.0, kind="the
 def spacetime_tangent_differences(camera, N, a, theta_o, params, times, nus, L, nmax=-1, slab=0.0, kind="thermal", h=2e-6,
                                   screen=True, marcher=None):
     """The observed Stokes cube of `camera` at (a, theta_o) with N stored samples per ray (marcher: Recurrence(64),
-    the spacetime block's stored march, joint.jl:78, by default; Direct() for the closed-form samples) and its
+    the spacetime block's stored march, joint.jl:78, by default; Direct() for the closed-form samples; Fused(64), the
+unstored form, marches inside each render) and its
     partials with respect to x = (a, theta_o, ln L) (the values spacetime_valgrad / spacetime_jacobian take from their
     dual cache, joint.jl:74-154), every image rendered on the device in FP64: central differences in each variable,
     Richardson-extrapolated from the steps h and h/2 (truncation error O(h^4); the reference checks its duals
@@ -787,7 +788,7 @@ def spacetime_tangent_differences(camera, N, a, theta_o, params, times, nus, L, 
         raise ValueError("the step h must be positive")
     m = _as_model(params, kind)
     march = Recurrence(64) if marcher is None else marcher
-    cache = GeodesicCache(camera, N, store_samples=True)
+    cache = GeodesicCache(camera, N, store_samples=not isinstance(march, Fused))
 
     def cube(a_, th_, L_):
         regenerate(cache, a_, th_, marcher=march)

@@ -248,20 +248,21 @@ def apply_gains(vis, gains, s1, s2):
 
 
 # ---- the spacetime block (joint.jl:74-154) ---------------------------------------------------------------
-def _spacetime_tangent(x, camera, N, params, times, nus, nmax, slab, kind):
-    """The cube of `camera` at x = (a, theta_o, ln L) with N closed-form samples per ray and its partials in x,
-    on the device (Richardson-extrapolated central differences of FP64 cubes)."""
+def _spacetime_tangent(x, camera, N, params, times, nus, nmax, slab, kind, stored):
+    """The cube of `camera` at x = (a, theta_o, ln L) with N samples per ray and its partials in x, on the device
+    (Richardson-extrapolated central differences of FP64 cubes): stored Recurrence(64) samples, or the fused march
+    (joint.jl:78)."""
     return api.spacetime_tangent_differences(camera, N, float(x[0]), float(x[1]), params, times, nus, float(np.exp(x[2])),
-                                             nmax=nmax, slab=slab, kind=kind)
+                                             nmax=nmax, slab=slab, kind=kind, marcher=None if stored else api.Fused(64))
 
 
-def spacetime_valgrad(loss_and_grad, x, camera, N, params, times, nus, nmax=-1, slab=0.0, kind="thermal"):
+def spacetime_valgrad(loss_and_grad, x, camera, N, params, times, nus, nmax=-1, slab=0.0, kind="thermal", stored=True):
     """spacetime_valgrad(loss, x, camera, backend; N) (joint.jl:74-82) for a loss of the rendered cube: the
     value and gradient with respect to x = (a, theta_o, ln L). The cube of `params` at `times`, `nus` and its
     partials come from spacetime_tangent_differences; loss_and_grad(cube) takes the screen-shaped cube
     (n_alpha, n_beta, nt, nnu, 4) and returns (value, d value / d cube) (spacetime_movie_loss builds the
     movie χ²). Returns (value, gradient (3,))."""
-    cube, dcube = _spacetime_tangent(x, camera, N, params, times, nus, nmax, slab, kind)
+    cube, dcube = _spacetime_tangent(x, camera, N, params, times, nus, nmax, slab, kind, stored)
     value, dv = loss_and_grad(cube)
     g = np.einsum("abtfs,abtfsk->k", np.asarray(dv, dtype=np.float64), dcube)
     return float(value), g
@@ -293,7 +294,8 @@ def spacetime_movie_loss(movie, frames=None, freqs=None, binning=None):
     return loss_and_grad
 
 
-def spacetime_jacobian(x, camera, N, params, movie, frames=None, freqs=None, nmax=-1, slab=0.0, binning=None, kind="thermal"):
+def spacetime_jacobian(x, camera, N, params, movie, frames=None, freqs=None, nmax=-1, slab=0.0, binning=None, kind="thermal",
+                       stored=True):
     """spacetime_jacobian with the movie residuals (joint.jl:98-154, spacetime_movie_residuals): the scaled
     residuals (model - data)/σ of the masked pixels, ordered frequency, frame, β, α, Stokes as the reference
     appends them, and their Jacobian with respect to x = (a, theta_o, ln L). Rows with a finite residual and
@@ -301,7 +303,7 @@ def spacetime_jacobian(x, camera, N, params, movie, frames=None, freqs=None, nma
     if len(x) != 3:
         raise ValueError("the spacetime block is (a, theta_o, ln L)")
     ks, ls = _frames(movie, frames, freqs)
-    cube, dcube = _spacetime_tangent(x, camera, N, params, movie.times[ks], movie.nus[ls], nmax, slab, kind)
+    cube, dcube = _spacetime_tangent(x, camera, N, params, movie.times[ks], movie.nus[ls], nmax, slab, kind, stored)
     if binning is not None:
         cube = api.bin(binning, cube)
         dcube = api.bin(binning, dcube)

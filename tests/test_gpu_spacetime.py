@@ -5,9 +5,10 @@ spacetime and at steps h, h/2 either side in a, theta_o and ln L, Richardson-ext
 forward-mode tangent on the same samples, Direct and Recurrence(64) (pinned by central differences and the thin-limit linearity in
 tests/test_oracle_spacetime.py): values at the FP64 image bar (1e-10 of the peak), partials at 2e-5 of each
 partial's largest entry (the oracle's own differences meet its duals at 5e-6). A composite's partials in the thin
-limit are the sums of its parts'. Then the reference's own checks of the spacetime block (test_spacetime.jl:94-113):
+limit are the sums of its parts'. Then the reference's own checks of the spacetime block (test_spacetime.jl:94-127):
 the valgrad value equals the chi2 of a stored Recurrence(64) cache, its gradient matches central differences of the
-device chi2, and the Jacobian's residuals square to the same chi2 with 2 J^T r equal to the gradient.
+device chi2, the Jacobian's residuals square to the same chi2 with 2 J^T r equal to the gradient, and the stored and
+fused forms agree with and without the half-orbit cutoff (1e-10 of the residuals, 1e-8 of the Jacobian).
 
 The device kernel of the dual tangent (three-partial duals through the pixel constants, the closed-form samples
 and the transport) stays disabled: it took a GPU memory fault on its first launch, the cause is not found, and
@@ -152,6 +153,14 @@ def test_spacetime_valgrad_and_jacobian():
     r, J = K.spacetime_jacobian(x3, cam, N, q, movie)
     assert abs(np.sum(r * r) - chi_stored) <= 1e-8 * chi_stored and J.shape == (len(r), 3)
     assert np.abs(2 * J.T @ r - g).max() <= 1e-8 * np.abs(g).max()
+    # the stored and the fused forms agree, with and without the half-orbit truncation, and stay finite
+    for nmax in (-1, 1):
+        print(f"stored vs fused, nmax {nmax}", flush=True)
+        rs, Js = K.spacetime_jacobian(x3, cam, N, q, movie, nmax=nmax, slab=0.5, stored=True)
+        rf, Jf = K.spacetime_jacobian(x3, cam, N, q, movie, nmax=nmax, slab=0.5, stored=False)
+        assert np.all(np.isfinite(Js)) and np.all(np.isfinite(Jf))
+        print(f"  residuals {np.abs(rs - rf).max() / np.abs(rf).max():.2e}, Jacobian {np.abs(Js - Jf).max() / np.abs(Jf).max():.2e}", flush=True)
+        assert np.abs(rs - rf).max() <= 1e-10 * np.abs(rf).max() and np.abs(Js - Jf).max() <= 1e-8 * np.abs(Jf).max()
 
 
 def test_spacetime_errors():
