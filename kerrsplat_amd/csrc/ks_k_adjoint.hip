@@ -51,21 +51,28 @@ __global__ void ks_adj_combine(int64_t npix, int N, int chunk, int nt, const dou
 
 // The FP64 step and step adjoint of an FP32 sample, out of line (the rare samples with a Faraday depth
 // |rho| Delta above FARADAY_FP64, where the forward pass takes the FP64 step too).
-KS_COLD void step_adjoint_f64_for_f32(const f32 j4[4], const f32 a4[4], const f32 r3[3], f32 D, const f32 w[4], const f32 R[4], f32 Otw[4],
-                                      f32 jb[4], f32 ab[4], f32 rb[3]) {
+// (inputs and outputs by value, as step_f64_for_f32's: the callers' arrays stay in registers)
+struct Vec4_32 { f32 v[4]; };
+struct Seeds32 { f32 w[4], R[4]; };
+struct StepAdj32 { f32 Otw[4], jb[4], ab[4], rb[3]; };
+KS_COLD StepAdj32 step_adjoint_f64_for_f32(const Step32 s, const Seeds32 z) {
     double jd[4], ad[4], rd[3], wd[4], Rd[4], Od[4], jbd[4], abd[4], rbd[3];
-    for (int i = 0; i < 4; ++i) { jd[i] = val(j4[i]); ad[i] = val(a4[i]); wd[i] = val(w[i]); Rd[i] = val(R[i]); }
-    for (int i = 0; i < 3; ++i) rd[i] = val(r3[i]);
-    sample_adjoint(jd, ad, rd, val(D), wd, Rd, Od, jbd, abd, rbd);
-    for (int i = 0; i < 4; ++i) { Otw[i] = f32(Od[i]); jb[i] = f32(jbd[i]); ab[i] = f32(abd[i]); }
-    for (int i = 0; i < 3; ++i) rb[i] = f32(rbd[i]);
+    for (int i = 0; i < 4; ++i) { jd[i] = val(s.j[i]); ad[i] = val(s.a[i]); wd[i] = val(z.w[i]); Rd[i] = val(z.R[i]); }
+    for (int i = 0; i < 3; ++i) rd[i] = val(s.r[i]);
+    sample_adjoint(jd, ad, rd, val(s.D), wd, Rd, Od, jbd, abd, rbd);
+    StepAdj32 o;
+    for (int i = 0; i < 4; ++i) { o.Otw[i] = f32(Od[i]); o.jb[i] = f32(jbd[i]); o.ab[i] = f32(abd[i]); }
+    for (int i = 0; i < 3; ++i) o.rb[i] = f32(rbd[i]);
+    return o;
 }
-KS_COLD void step_apply_f64_for_f32(const f32 j4[4], const f32 a4[4], const f32 r3[3], f32 D, const f32 Tin[4], f32 Tout[4]) {
+KS_COLD Vec4_32 step_apply_f64_for_f32(const Step32 s, const Vec4_32 Tin) {
     double jd[4], ad[4], rd[3], Ti[4], To[4];
-    for (int i = 0; i < 4; ++i) { jd[i] = val(j4[i]); ad[i] = val(a4[i]); Ti[i] = val(Tin[i]); }
-    for (int i = 0; i < 3; ++i) rd[i] = val(r3[i]);
-    step_apply(jd, ad, rd, val(D), Ti, To);
-    for (int i = 0; i < 4; ++i) Tout[i] = f32(To[i]);
+    for (int i = 0; i < 4; ++i) { jd[i] = val(s.j[i]); ad[i] = val(s.a[i]); Ti[i] = val(Tin.v[i]); }
+    for (int i = 0; i < 3; ++i) rd[i] = val(s.r[i]);
+    step_apply(jd, ad, rd, val(s.D), Ti, To);
+    Vec4_32 o;
+    for (int i = 0; i < 4; ++i) o.v[i] = f32(To[i]);
+    return o;
 }
 
 // The adjoint of one sample's step, and the step applied to a tail, in the transfer's scalar; in FP32
@@ -74,7 +81,14 @@ template <class S>
 KS_DEV void step_adjoint_s(const S j4[4], const S a4[4], const S r3[3], S D, const S w[4], const S R[4], S Otw[4], S jb[4], S ab[4], S rb[3]) {
     if constexpr (!std::is_same<S, double>::value) {
         const S fd = sqrt(r3[0] * r3[0] + r3[1] * r3[1] + r3[2] * r3[2]) * D;
-        if (fd > FARADAY_FP64) { step_adjoint_f64_for_f32(j4, a4, r3, D, w, R, Otw, jb, ab, rb); return; }
+        if (fd > FARADAY_FP64) {
+            Seeds32 z;
+            for (int i = 0; i < 4; ++i) { z.w[i] = w[i]; z.R[i] = R[i]; }
+            const StepAdj32 o = step_adjoint_f64_for_f32(step32_of(j4, a4, r3, D), z);
+            for (int i = 0; i < 4; ++i) { Otw[i] = o.Otw[i]; jb[i] = o.jb[i]; ab[i] = o.ab[i]; }
+            for (int i = 0; i < 3; ++i) rb[i] = o.rb[i];
+            return;
+        }
     }
     sample_adjoint(j4, a4, r3, D, w, R, Otw, jb, ab, rb);
 }
@@ -82,7 +96,13 @@ template <class S>
 KS_DEV void step_apply_s(const S j4[4], const S a4[4], const S r3[3], S D, const S Tin[4], S Tout[4]) {
     if constexpr (!std::is_same<S, double>::value) {
         const S fd = sqrt(r3[0] * r3[0] + r3[1] * r3[1] + r3[2] * r3[2]) * D;
-        if (fd > FARADAY_FP64) { step_apply_f64_for_f32(j4, a4, r3, D, Tin, Tout); return; }
+        if (fd > FARADAY_FP64) {
+            Vec4_32 ti;
+            for (int i = 0; i < 4; ++i) ti.v[i] = Tin[i];
+            const Vec4_32 o = step_apply_f64_for_f32(step32_of(j4, a4, r3, D), ti);
+            for (int i = 0; i < 4; ++i) Tout[i] = o.v[i];
+            return;
+        }
     }
     step_apply(j4, a4, r3, D, Tin, Tout);
 }
