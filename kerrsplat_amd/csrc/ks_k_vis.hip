@@ -51,7 +51,8 @@ __global__ void ks_vis_gather(const double* __restrict__ out, const int* __restr
 }
 
 // ---- FP64 GEMM on the matrix cores: C[M x N] = op(A) B, row-major; A is M x K, or K x M with TA; batched over z ----
-// 64 x 64 tile per 256-thread block, K in steps of 16 through LDS. Wavefront w owns rows 16 w .. 16 w + 15
+// 64 x 64 tile per 256-thread block, K in steps of 16 through LDS (the next step's operands loaded into registers
+// during this step's products). Wavefront w owns rows 16 w .. 16 w + 15
 // and the four 16 x 16 output tiles along N, each a v_mfma_f64_16x16x4f64 accumulator; per step of 4 in K
 // lane l supplies A[16 w + l % 16][k + l / 16] and B[k + l / 16][16 t + l % 16], and holds
 // D[4 r + l / 16][l % 16] in its register r (the instruction's lane layout, checked on the device).
@@ -68,20 +69,32 @@ __global__ void __launch_bounds__(256) ks_dgemm(int M, int N, int K, const doubl
     const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
     dbl4 acc[4];
     for (int t = 0; t < 4; ++t) acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < K; k0 += GK) {
-        for (int e = tid; e < GT * GK; e += 256) {
-            int mm, kk;
-            if (TA) { kk = e / GT; mm = e % GT; }
-            else { mm = e / GK; kk = e % GK; }
+    // each thread moves 4 elements of A and 4 of B per step of K (GT x GK = 4 x 256); the next step's elements are
+    // loaded into registers while the matrix cores work on this step's tiles
+    static_assert(GT * GK == 4 * 256, "four elements of each operand per thread and step");
+    double ra[4], rb[4];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = tid + 256 * u;
+            const int mm = TA ? e % GT : e / GK, kk = TA ? e / GT : e % GK;
             const int gm = m0 + mm, gk = k0 + kk;
-            double a = 0.0;
-            if (gm < M && gk < K) a = TA ? A[(int64_t)gk * lda + gm] : A[(int64_t)gm * lda + gk];
-            As[kk][mm] = a;
-            const int bk = e / GT, bn = e % GT;
-            const int gk2 = k0 + bk, gn = n0 + bn;
-            Bs[bk][bn] = (gk2 < K && gn < N) ? B[(int64_t)gk2 * ldb + gn] : 0.0;
+            ra[u] = (gm < M && gk < K) ? (TA ? A[(int64_t)gk * lda + gm] : A[(int64_t)gm * lda + gk]) : 0.0;
+            const int gk2 = k0 + e / GT, gn = n0 + e % GT;
+            rb[u] = (gk2 < K && gn < N) ? B[(int64_t)gk2 * ldb + gn] : 0.0;
+        }
+    };
+    load(0);
+    for (int k0 = 0; k0 < K; k0 += GK) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = tid + 256 * u;
+            const int mm = TA ? e % GT : e / GK, kk = TA ? e / GT : e % GK;
+            As[kk][mm] = ra[u];
+            Bs[e / GT][e % GT] = rb[u];
         }
         __syncthreads();
+        if (k0 + GK < K) load(k0 + GK);
 #pragma unroll
         for (int k4 = 0; k4 < GK; k4 += 4) {
             const double a = As[k4 + l / 16][16 * w + l % 16];
