@@ -69,13 +69,14 @@ __global__ void __launch_bounds__(256) ks_dgemm(int M, int N, int K, const doubl
     const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
     dbl4 acc[4];
     for (int t = 0; t < 4; ++t) acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
-    // each thread moves 4 elements of A and 4 of B per step of K (GT x GK = 4 x 256); the next step's elements are
-    // loaded into registers while the matrix cores work on this step's tiles
-    static_assert(GT * GK == 4 * 256, "four elements of each operand per thread and step");
-    double ra[4], rb[4];
+    // each thread moves EPT elements of A and EPT of B per step of K (GT x GK = EPT x 256); the next step's elements
+    // are loaded into registers while the matrix cores work on this step's tiles
+    constexpr int EPT = GT * GK / 256;
+    static_assert(EPT * 256 == GT * GK, "whole elements of each operand per thread and step");
+    double ra[EPT], rb[EPT];
     auto load = [&](int k0) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < EPT; ++u) {
             const int e = tid + 256 * u;
             const int mm = TA ? e % GT : e / GK, kk = TA ? e / GT : e % GK;
             const int gm = m0 + mm, gk = k0 + kk;
@@ -87,7 +88,7 @@ __global__ void __launch_bounds__(256) ks_dgemm(int M, int N, int K, const doubl
     load(0);
     for (int k0 = 0; k0 < K; k0 += GK) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < EPT; ++u) {
             const int e = tid + 256 * u;
             const int mm = TA ? e % GT : e / GK, kk = TA ? e / GT : e % GK;
             As[kk][mm] = ra[u];
